@@ -1,0 +1,195 @@
+#!/usr/bin/env python3
+"""Benchmark of the DD hot path on one MI355X.
+
+Default (the headline): float64x2 GEMM C = A*B at m = n = k = 4096, entries
+s*2^U(-20,20)*u with lo per the config-0 rule, renorm interval 8, inputs and
+output resident in HBM. A step is one GEMM launch over the whole matrix.
+Prints ONE JSON line with value = DD-GFLOP/s (2*m*n*k per step).
+
+  python bench.py [--steps K] [--warmup W] [--workload gemm|elementwise|reduce]
+                  [--no-cpu] [--variant V]
+
+Timing: torch.cuda events recorded on the stream the kernels are launched on
+(torch's current stream, whose handle is passed to the C ABI), bracketed by
+device synchronisation. The CPU baseline is the reference's own compiled
+matmul (oracle/_ref) when present, else the oracle's C restatement, timed on
+one host core over a bounded slice (all of A times the first columns of B)
+of the same workload; its outputs are also compared bit-for-bit with the GPU's.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 vector (and matrix) peak
+HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
+FP64_FLOPS_PER_DDMAC = 15.75  # 2 MUL + 2 FMA(x2) + 9 ADD + 6/8 renorm at ri = 8
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--workload", default="gemm", choices=["gemm", "elementwise", "reduce"])
+    p.add_argument("--size", type=int, default=4096)
+    p.add_argument("--variant", type=int, default=-1)
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--cpu-cols", type=int, default=48)
+    return p.parse_args()
+
+
+def timed(torch, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(steps):
+        fn()
+    e1.record(s)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps  # ms per step
+
+
+def cpu_gemm_baseline(A, B, C_gpu, cols, ri):
+    """Time the reference matmul on one core over all of A times B[:, :cols]."""
+    from tests import _native as nt  # test infrastructure, timed as the baseline
+    from multifloats_amd import datagen
+    m, k = A.shape[:2]
+    Bs = datagen.as_colmajor(B[:, :cols])
+    A = datagen.as_colmajor(A)
+    C = datagen.as_colmajor(np.zeros((m, cols, 2)))
+    kind = "reference"
+    if os.path.exists(nt.REF):
+        L = ctypes.CDLL(nt.REF)
+        fn = L.matmuldd_mm
+    else:
+        kind = "port"
+        fn = nt.oracle().or_matmul_mm
+    fn.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int64] * 4
+    t0 = time.perf_counter()
+    fn(A.ctypes.data, Bs.ctypes.data, C.ctypes.data, m, k, cols, ri)
+    dt = time.perf_counter() - t0
+    same = bool(nt.bits_equal(C, C_gpu[:, :cols]).all())
+    return dict(value=round(2.0 * m * k * cols / dt / 1e9, 4), unit="DD-GFLOP/s", cores=1, kind=kind,
+                sample="C[:, :%d] = A(%dx%d) * B[:, :%d] on one host core, %.1f s; bit-identical to the GPU: %s"
+                % (cols, m, k, cols, dt, same)), same
+
+
+def bench_gemm(args, torch, mf):
+    from multifloats_amd import datagen
+    m = n = k = args.size
+    ri = 8
+    rng = np.random.default_rng(2026)
+    A = datagen.matrix(rng, m, k, 20)
+    B = datagen.matrix(rng, k, n, 20)
+    dA = torch.from_numpy(datagen.flat(A)).cuda()
+    dB = torch.from_numpy(datagen.flat(B)).cuda()
+    dC = torch.empty((m * n, 2), dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def step():
+        mf.dev_matmul(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), m, k, n, ri, stream, args.variant)
+
+    ms = timed(torch, step, args.steps, args.warmup)
+    flops = 2.0 * m * n * k
+    value = flops / (ms * 1e-3) / 1e9
+    achieved = FP64_FLOPS_PER_DDMAC * m * n * k / (ms * 1e-3) / 1e12
+    out = {
+        "metric": "DD-GEMM DD-GFLOP/s at 4096^3 (plain reference-order float64x2 GEMM)",
+        "value": round(value, 2), "unit": "DD-GFLOP/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "float64x2 GEMM m=n=k=%d, exponent spread 2^+-20, renorm interval 8, "
+                               "plain form (bit-identical to the reference)" % m,
+                   "m": m, "n": n, "k": k, "renorm_interval": ri, "variant": args.variant},
+        "roofline": {"bound": "fp64", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                     "note": "FP64 VALU bound: %.2f FP64 flops per DD-MAC (FMA = 2)" % FP64_FLOPS_PER_DDMAC},
+    }
+    if not args.no_cpu:
+        C = datagen.unflat(dC.cpu().numpy(), m, n)
+        out["cpu_baseline"], same = cpu_gemm_baseline(A, B, C, args.cpu_cols, ri)
+    return out
+
+
+def bench_elementwise(args, torch, mf):
+    from multifloats_amd import datagen
+    n = 1 << 26
+    rng = np.random.default_rng(7)
+    stream = torch.cuda.current_stream().cuda_stream
+    res = {}
+    a = torch.from_numpy(datagen.config0(rng, n)).cuda()
+    b = torch.from_numpy(datagen.config0(rng, n)).cuda()
+    c = torch.empty_like(a)
+    for nm, op in (("add", mf.MFHIP_ADD), ("mul", mf.MFHIP_MUL), ("div", mf.MFHIP_DIV)):
+        ms = timed(torch, lambda: mf.dev_binary(op, a.data_ptr(), b.data_ptr(), c.data_ptr(), n, stream),
+                   args.steps, args.warmup)
+        res[nm] = dict(ms=round(ms, 4), GBs=round(48.0 * n / ms / 1e6, 1))
+    a.abs_()
+    ms = timed(torch, lambda: mf.dev_unary(mf.MFHIP_SQRT, a.data_ptr(), c.data_ptr(), n, stream), args.steps, args.warmup)
+    res["sqrt"] = dict(ms=round(ms, 4), GBs=round(32.0 * n / ms / 1e6, 1))
+    gens = {"exp": datagen.exp_inputs, "log": datagen.log_inputs, "sin": datagen.sin_inputs, "erf": datagen.erf_inputs}
+    ops = {"exp": mf.MFHIP_EXP, "log": mf.MFHIP_LOG, "sin": mf.MFHIP_SIN, "erf": mf.MFHIP_ERF}
+    for nm, gen in gens.items():
+        x = torch.from_numpy(gen(rng, n)).cuda()
+        ms = timed(torch, lambda: mf.dev_unary(ops[nm], x.data_ptr(), c.data_ptr(), n, stream), args.steps, args.warmup)
+        res[nm] = dict(ms=round(ms, 4), GBs=round(32.0 * n / ms / 1e6, 1), Gelem_s=round(n / ms / 1e6, 2))
+        del x
+    best = max(v["GBs"] for k_, v in res.items() if k_ in ("add", "mul", "div", "sqrt"))
+    return {"metric": "elementwise DD arrays, N=2^26", "value": best, "unit": "GB/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "dtype": "f64",
+            "data": "synthetic", "config": {"workload": "elementwise float64x2 N=2^26"},
+            "roofline": {"bound": "hbm", "achieved": best, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(best / HBM_PEAK_GBS, 4), "traffic": None}, "ops": res}
+
+
+def bench_reduce(args, torch, mf):
+    from multifloats_amd import datagen
+    n = 1 << 26
+    rng = np.random.default_rng(8)
+    a_np, b_np = datagen.cancel_pairs(rng, n)
+    a = torch.from_numpy(a_np).cuda()
+    b = torch.from_numpy(b_np).cuda()
+    out = torch.empty(2, dtype=torch.float64, device="cuda")
+    work = torch.empty(mf.reduce_workspace_bytes(n) // 8 + 2, dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    res = {}
+    for nm, kind, bytes_per in (("dot", mf.MFHIP_DOT, 32), ("sum", mf.MFHIP_SUM, 16), ("norm2", mf.MFHIP_NORM2, 16)):
+        ms = timed(torch, lambda: mf.dev_reduce(kind, a.data_ptr(), b.data_ptr(), n, out.data_ptr(),
+                                                work.data_ptr(), 8, stream), args.steps, args.warmup)
+        res[nm] = dict(ms=round(ms, 4), GBs=round(bytes_per * n / ms / 1e6, 1))
+    best = res["dot"]["GBs"]
+    return {"metric": "DD dot_product N=2^26", "value": best, "unit": "GB/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "dtype": "f64",
+            "data": "synthetic", "config": {"workload": "dot/sum/norm2 float64x2 N=2^26"},
+            "roofline": {"bound": "hbm", "achieved": best, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(best / HBM_PEAK_GBS, 4), "traffic": None}, "ops": res}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the DD kernels run on a single MI355X here; "
+              "multi-GPU sharding is a design note, not a measured path.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import multifloats_amd as mf
+    if not torch.cuda.is_available():
+        sys.exit("bench.py: no GPU visible")
+    fn = {"gemm": bench_gemm, "elementwise": bench_elementwise, "reduce": bench_reduce}[args.workload]
+    print(json.dumps(fn(args, torch, mf)), flush=True)
+
+
+if __name__ == "__main__":
+    main()

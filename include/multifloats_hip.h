@@ -1,0 +1,169 @@
+/* multifloats_hip -- MI355X (gfx950) kernels behind the multifloats C ABI.
+ *
+ * Drop-in boundary for the reference's data-parallel hot path. Paths in the
+ * citations are relative to the reference (kyungminlee/multifloats 0.10.3).
+ *
+ *  1. Reference-compatible entry points (identical names, signatures and
+ *     semantics): matmuldd_mm / _mv / _vm and their matmulf64x2_* spellings.
+ *     Host pointers, synchronous, caller owns every buffer, C overwritten,
+ *     column-major with leading dimension = first extent.
+ *  2. Array entry points that replace the reference's per-element and
+ *     Fortran-native loops (the reference has no array-level C ABI for these;
+ *     its Fortran generics are the interface they stand in for). Host
+ *     pointers, synchronous.
+ *  3. A device-pointer API (mfhip_*) taking a HIP stream, returning a status,
+ *     never allocating and never synchronising: for callers that keep data
+ *     resident in HBM, chain operations, or capture them in a HIP graph.
+ *
+ * Error model. The reference is NaN-in/NaN-out with no status channel
+ * (README.md:274-296). The numerical results here keep that model exactly.
+ * A device-side failure (no GPU, launch or copy error) cannot be reported
+ * through a void reference signature, so entry points of groups 1 and 2
+ * print the HIP error to stderr, fill their outputs with NaN and record the
+ * error for mfhip_last_error(); there is no silent CPU fallback.
+ *
+ * Thread safety: every host thread gets its own stream and staging buffers,
+ * so the group-1/2 entry points may be called concurrently (the reference's
+ * Fortran interfaces are `pure`, fsrc/multifloats.fypp:135-152).
+ */
+#ifndef MULTIFLOATS_HIP_H
+#define MULTIFLOATS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#define MULTIFLOATS_HIP_VERSION 1
+
+#ifndef MULTIFLOATS_ABI_VERSION
+/* Same ABI version and POD layout as include/multifloats/float64x2.h:33,
+ * 125-139, 348-351: two back-to-back IEEE binary64 limbs (hi, lo); complex
+ * is re then im. When the reference header is included first, its types are
+ * used unchanged. */
+#define MULTIFLOATS_ABI_VERSION 2
+#if defined(__GNUC__) || defined(__clang__)
+#define MULTIFLOATS_API __attribute__((visibility("default")))
+#else
+#define MULTIFLOATS_API
+#endif
+#ifdef __cplusplus
+namespace multifloats {
+#endif
+struct float64x2 {
+  double limbs[2];
+};
+struct complex64x2 {
+  struct float64x2 re;
+  struct float64x2 im;
+};
+#ifndef __cplusplus
+typedef struct float64x2 float64x2;
+typedef struct complex64x2 complex64x2;
+#endif
+#ifdef __cplusplus
+} /* namespace multifloats */
+#endif
+#endif /* MULTIFLOATS_ABI_VERSION */
+
+#ifdef __cplusplus
+namespace multifloats {
+extern "C" {
+#endif
+
+/* ---- 1. reference-compatible GEMM entry points ------------------------- */
+
+/* C(m,n) = A(m,k) * B(k,n). Replaces matmuldd_mm
+ * (include/multifloats/float64x2.h:682-685, src/float64x2/abi.inc:133-137).
+ * Bit-identical to the reference for every input: each output is the same
+ * sequential compensated chain (src/float64x2/matmul.inc:6-39, 132-205).
+ * renorm_interval: > 0 renormalises every that many MACs (Fortran default 8,
+ * fsrc/multifloats.fypp:664); <= 0 only at the end. k = 0 writes zeros. */
+MULTIFLOATS_API void matmuldd_mm(const float64x2 *a, const float64x2 *b, float64x2 *c,
+                                 int64_t m, int64_t k, int64_t n, int64_t renorm_interval);
+/* y(m) = A(m,k) * x(k). Replaces matmuldd_mv (float64x2.h:687-690). */
+MULTIFLOATS_API void matmuldd_mv(const float64x2 *a, const float64x2 *x, float64x2 *y,
+                                 int64_t m, int64_t k, int64_t renorm_interval);
+/* y(n) = x(k) * B(k,n). Replaces matmuldd_vm (float64x2.h:692-695). */
+MULTIFLOATS_API void matmuldd_vm(const float64x2 *x, const float64x2 *b, float64x2 *y,
+                                 int64_t k, int64_t n, int64_t renorm_interval);
+/* f64x2-tagged spellings (float64x2.h:1042-1046, src/float64x2/f64x2_aliases.inc:114-116). */
+MULTIFLOATS_API void matmulf64x2_mm(const float64x2 *a, const float64x2 *b, float64x2 *c,
+                                    int64_t m, int64_t k, int64_t n, int64_t renorm_interval);
+MULTIFLOATS_API void matmulf64x2_mv(const float64x2 *a, const float64x2 *x, float64x2 *y,
+                                    int64_t m, int64_t k, int64_t renorm_interval);
+MULTIFLOATS_API void matmulf64x2_vm(const float64x2 *x, const float64x2 *b, float64x2 *y,
+                                    int64_t k, int64_t n, int64_t renorm_interval);
+
+/* ---- 2. array entry points -------------------------------------------- */
+
+/* c[i] = a[i] op b[i], bit-identical to the reference operators
+ * (float64x2.h:203-311; Fortran native fsrc/multifloats.fypp:2746-2837). */
+MULTIFLOATS_API void adddd_v(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n);
+MULTIFLOATS_API void subdd_v(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n);
+MULTIFLOATS_API void muldd_v(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n);
+MULTIFLOATS_API void divdd_v(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n);
+/* c[i] = sqrt(a[i]), bit-identical to sqrtdd (float64x2.h:2190-2213). */
+MULTIFLOATS_API void sqrtdd_v(const float64x2 *a, float64x2 *c, int64_t n);
+/* Array forms of expdd / logdd / sindd (float64x2.h:453-483; src/float64x2/
+ * exp_log.inc:237-341, trig.inc:241-253), bit-identical to them; they replace
+ * the Fortran elemental loop over c_expdd/... (fypp:709-736, 1407-1412). */
+MULTIFLOATS_API void expdd_v(const float64x2 *a, float64x2 *c, int64_t n);
+MULTIFLOATS_API void logdd_v(const float64x2 *a, float64x2 *c, int64_t n);
+MULTIFLOATS_API void sindd_v(const float64x2 *a, float64x2 *c, int64_t n);
+/* Array form of erfdd (float64x2.h:525; src/float64x2/special.inc:18-51): own
+ * piecewise-Taylor algorithm held to the reference's max relative error. */
+MULTIFLOATS_API void erfdd_v(const float64x2 *a, float64x2 *c, int64_t n);
+/* Fortran sum(a) on a rank-1 real64x2 array (fypp:1980-2020). */
+MULTIFLOATS_API float64x2 sumdd(const float64x2 *a, int64_t n);
+/* Fortran dot_product(a, b) (fypp:2254-2308); renorm_interval plays the role
+ * of DD_FMA_RENORM_INTERVAL (fypp:664, 2268). */
+MULTIFLOATS_API float64x2 dotdd(const float64x2 *a, const float64x2 *b, int64_t n,
+                                int64_t renorm_interval);
+/* Fortran norm2(a) (fypp:2338-2352); like the reference, no overflow scaling. */
+MULTIFLOATS_API float64x2 norm2dd(const float64x2 *a, int64_t n);
+
+/* ---- 3. device-pointer API ------------------------------------------- */
+/* All pointers below are device pointers (16-byte aligned) and `stream` is a
+ * hipStream_t (NULL = the default stream). Returns 0 on success, otherwise a
+ * hipError_t value (see mfhip_error_string). Work is only enqueued. */
+
+enum {
+  MFHIP_SQRT = 0, MFHIP_EXP = 1, MFHIP_LOG = 2, MFHIP_SIN = 3, MFHIP_ERF = 4, MFHIP_NEG = 5
+};
+enum { MFHIP_ADD = 0, MFHIP_SUB = 1, MFHIP_MUL = 2, MFHIP_DIV = 3 };
+enum { MFHIP_DOT = 0, MFHIP_SUM = 1, MFHIP_NORM2 = 2 };
+
+MULTIFLOATS_API int mfhip_matmul_mm(const float64x2 *a, const float64x2 *b, float64x2 *c,
+                                    int64_t m, int64_t k, int64_t n, int64_t renorm_interval,
+                                    void *stream);
+/* Same, with an explicit tile-shape variant in [0, mfhip_matmul_variants()) --
+ * the hook the autotuner drives; variant -1 is the built-in size rule. */
+MULTIFLOATS_API int mfhip_matmul_variants(void);
+MULTIFLOATS_API int mfhip_matmul_mm_variant(const float64x2 *a, const float64x2 *b, float64x2 *c,
+                                            int64_t m, int64_t k, int64_t n,
+                                            int64_t renorm_interval, int variant, void *stream);
+MULTIFLOATS_API int mfhip_matmul_mv(const float64x2 *a, const float64x2 *x, float64x2 *y,
+                                    int64_t m, int64_t k, int64_t renorm_interval, void *stream);
+MULTIFLOATS_API int mfhip_matmul_vm(const float64x2 *x, const float64x2 *b, float64x2 *y,
+                                    int64_t k, int64_t n, int64_t renorm_interval, void *stream);
+MULTIFLOATS_API int mfhip_unary(int op, const float64x2 *a, float64x2 *c, int64_t n,
+                                void *stream);
+MULTIFLOATS_API int mfhip_binary(int op, const float64x2 *a, const float64x2 *b, float64x2 *c,
+                                 int64_t n, void *stream);
+/* Reductions write one float64x2 to *out (device). `work` is caller-owned
+ * device scratch of at least mfhip_reduce_workspace_bytes(n) bytes. b is
+ * only read for MFHIP_DOT; renorm_interval only matters for MFHIP_DOT. */
+MULTIFLOATS_API size_t mfhip_reduce_workspace_bytes(int64_t n);
+MULTIFLOATS_API int mfhip_reduce(int kind, const float64x2 *a, const float64x2 *b, int64_t n,
+                                 int64_t renorm_interval, float64x2 *out, void *work,
+                                 void *stream);
+
+/* Last HIP error recorded on the calling thread by any entry point (0 = none). */
+MULTIFLOATS_API int mfhip_last_error(void);
+MULTIFLOATS_API const char *mfhip_error_string(int err);
+
+#ifdef __cplusplus
+} /* extern "C" */
+} /* namespace multifloats */
+#endif
+
+#endif /* MULTIFLOATS_HIP_H */

@@ -1,0 +1,232 @@
+"""multifloats_amd -- MI355X (gfx950) double-double hot path.
+
+Python mirror of the reference's operator surface for the hot path, over the
+C ABI of libmultifloats_hip.so (include/multifloats_hip.h):
+
+  matmul(a, b, renorm_interval=8)   Fortran matmul on real64x2 -> matmuldd_mm/mv/vm
+  dot_product(a, b, renorm_interval=8), sum(a), norm2(a)
+  exp, log, sin, erf, sqrt, add, sub, mul, div   (elementwise)
+
+DD arrays are numpy float64 arrays whose last axis has length 2 (hi, lo) --
+the float64x2 memory layout. Matrices are (m, n, 2) arrays in the column-major
+float64x2 layout, element (i, j) at byte offset 16*(i + m*j) (datagen.matrix,
+datagen.as_colmajor); other layouts are copied into it. The default renorm
+interval 8 is the Fortran module's DD_FMA_RENORM_INTERVAL
+(fsrc/multifloats.fypp:664); the C++ matmul_* default is 0 (float64x2.h:2500).
+
+The device-pointer functions (dev_*) take integer device addresses (e.g.
+torch tensor .data_ptr()) and a HIP stream handle; they enqueue only.
+
+There is no CPU fallback: if the library or a GPU is missing, calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import datagen
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmultifloats_hip.so")
+
+DD_FMA_RENORM_INTERVAL = 8
+
+MFHIP_SQRT, MFHIP_EXP, MFHIP_LOG, MFHIP_SIN, MFHIP_ERF, MFHIP_NEG = range(6)
+MFHIP_ADD, MFHIP_SUB, MFHIP_MUL, MFHIP_DIV = range(4)
+MFHIP_DOT, MFHIP_SUM, MFHIP_NORM2 = range(3)
+
+
+class float64x2(ctypes.Structure):
+    _fields_ = [("limbs", ctypes.c_double * 2)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libmultifloats_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("multifloats_amd: %s is not built (run __graft_entry__.build())" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        P, I64, VP, I = ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int
+        for nm in ("matmuldd_mm", "matmulf64x2_mm"):
+            getattr(L, nm).argtypes = [P, P, P, I64, I64, I64, I64]
+            getattr(L, nm).restype = None
+        for nm in ("matmuldd_mv", "matmulf64x2_mv", "matmuldd_vm", "matmulf64x2_vm"):
+            getattr(L, nm).argtypes = [P, P, P, I64, I64, I64]
+            getattr(L, nm).restype = None
+        for nm in ("adddd_v", "subdd_v", "muldd_v", "divdd_v"):
+            getattr(L, nm).argtypes = [P, P, P, I64]
+            getattr(L, nm).restype = None
+        for nm in ("sqrtdd_v", "expdd_v", "logdd_v", "sindd_v", "erfdd_v"):
+            getattr(L, nm).argtypes = [P, P, I64]
+            getattr(L, nm).restype = None
+        L.sumdd.argtypes = [P, I64]
+        L.sumdd.restype = float64x2
+        L.norm2dd.argtypes = [P, I64]
+        L.norm2dd.restype = float64x2
+        L.dotdd.argtypes = [P, P, I64, I64]
+        L.dotdd.restype = float64x2
+        L.mfhip_matmul_mm.argtypes = [P, P, P, I64, I64, I64, I64, VP]
+        L.mfhip_matmul_mm_variant.argtypes = [P, P, P, I64, I64, I64, I64, I, VP]
+        L.mfhip_matmul_variants.argtypes = []
+        L.mfhip_matmul_mv.argtypes = [P, P, P, I64, I64, I64, VP]
+        L.mfhip_matmul_vm.argtypes = [P, P, P, I64, I64, I64, VP]
+        L.mfhip_unary.argtypes = [I, P, P, I64, VP]
+        L.mfhip_binary.argtypes = [I, P, P, P, I64, VP]
+        L.mfhip_reduce_workspace_bytes.argtypes = [I64]
+        L.mfhip_reduce_workspace_bytes.restype = ctypes.c_size_t
+        L.mfhip_reduce.argtypes = [I, P, P, I64, I64, P, P, VP]
+        L.mfhip_last_error.argtypes = []
+        L.mfhip_error_string.argtypes = [I]
+        L.mfhip_error_string.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _check_status():
+    e = lib().mfhip_last_error()
+    if e:
+        raise RuntimeError("multifloats_hip: %s" % lib().mfhip_error_string(e).decode())
+
+
+def _dd(x):
+    a = np.require(np.asarray(x, dtype=np.float64), requirements=["C", "A"])
+    if a.shape[-1:] != (2,):
+        raise ValueError("DD arrays need a trailing axis of length 2 (hi, lo)")
+    return a
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _mat(a):
+    a = datagen.as_colmajor(a)
+    if a.shape[-1:] != (2,):
+        raise ValueError("DD matrices are (m, n, 2) arrays")
+    return a
+
+
+def matmul(a, b, renorm_interval=DD_FMA_RENORM_INTERVAL):
+    """Fortran matmul on real64x2 (fsrc/multifloats.fypp:2568-2593).
+
+    a: (m, k, 2) column-major or (k, 2) vector; b: (k, n, 2) or (k, 2).
+    """
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    if a.ndim == 3 and b.ndim == 3:
+        m, k = a.shape[:2]
+        if b.shape[0] != k:
+            raise ValueError("matmul: inner dimensions differ")
+        n = b.shape[1]
+        a, b = _mat(a), _mat(b)
+        c = _mat(np.zeros((m, n, 2)))
+        lib().matmuldd_mm(_ptr(a), _ptr(b), _ptr(c), m, k, n, renorm_interval)
+    elif a.ndim == 3 and b.ndim == 2:
+        m, k = a.shape[:2]
+        if b.shape[0] != k:
+            raise ValueError("matmul: inner dimensions differ")
+        a, b = _mat(a), _dd(b)
+        c = np.zeros((m, 2))
+        lib().matmuldd_mv(_ptr(a), _ptr(b), _ptr(c), m, k, renorm_interval)
+    elif a.ndim == 2 and b.ndim == 3:
+        k, n = b.shape[:2]
+        if a.shape[0] != k:
+            raise ValueError("matmul: inner dimensions differ")
+        a, b = _dd(a), _mat(b)
+        c = np.zeros((n, 2))
+        lib().matmuldd_vm(_ptr(a), _ptr(b), _ptr(c), k, n, renorm_interval)
+    else:
+        raise ValueError("matmul: need (matrix, matrix), (matrix, vector) or (vector, matrix)")
+    _check_status()
+    return c
+
+
+def _unary(name, a):
+    a = _dd(a)
+    c = np.empty_like(a)
+    getattr(lib(), name)(_ptr(a), _ptr(c), a.size // 2)
+    _check_status()
+    return c
+
+
+def _binary(name, a, b):
+    a, b = np.broadcast_arrays(_dd(a), _dd(b))
+    a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
+    c = np.empty_like(a)
+    getattr(lib(), name)(_ptr(a), _ptr(b), _ptr(c), a.size // 2)
+    _check_status()
+    return c
+
+
+def exp(a): return _unary("expdd_v", a)
+def log(a): return _unary("logdd_v", a)
+def sin(a): return _unary("sindd_v", a)
+def erf(a): return _unary("erfdd_v", a)
+def sqrt(a): return _unary("sqrtdd_v", a)
+def add(a, b): return _binary("adddd_v", a, b)
+def sub(a, b): return _binary("subdd_v", a, b)
+def mul(a, b): return _binary("muldd_v", a, b)
+def div(a, b): return _binary("divdd_v", a, b)
+
+
+def _res(r):
+    return np.array([r.limbs[0], r.limbs[1]])
+
+
+def sum(a):  # noqa: A001 - mirrors the Fortran intrinsic name
+    """Fortran sum over every element of a real64x2 array (storage order)."""
+    a = np.ascontiguousarray(_dd(a)).reshape(-1, 2)
+    r = lib().sumdd(_ptr(a), a.shape[0])
+    _check_status()
+    return _res(r)
+
+
+def dot_product(a, b, renorm_interval=DD_FMA_RENORM_INTERVAL):
+    a, b = _dd(a), _dd(b)
+    if a.shape != b.shape or a.ndim != 2:
+        raise ValueError("dot_product: rank-1 arrays of equal size")
+    r = lib().dotdd(_ptr(a), _ptr(b), a.shape[0], renorm_interval)
+    _check_status()
+    return _res(r)
+
+
+def norm2(a):
+    a = np.ascontiguousarray(_dd(a)).reshape(-1, 2)
+    r = lib().norm2dd(_ptr(a), a.shape[0])
+    _check_status()
+    return _res(r)
+
+
+# ---- device-pointer API (ints are device addresses) ----
+
+def _dev(rc):
+    if rc:
+        raise RuntimeError("multifloats_hip: %s" % lib().mfhip_error_string(rc).decode())
+
+
+def dev_matmul(a, b, c, m, k, n, renorm_interval=DD_FMA_RENORM_INTERVAL, stream=0, variant=-1):
+    _dev(lib().mfhip_matmul_mm_variant(a, b, c, m, k, n, renorm_interval, variant, stream))
+
+
+def dev_unary(op, a, c, n, stream=0):
+    _dev(lib().mfhip_unary(op, a, c, n, stream))
+
+
+def dev_binary(op, a, b, c, n, stream=0):
+    _dev(lib().mfhip_binary(op, a, b, c, n, stream))
+
+
+def dev_reduce(kind, a, b, n, out, work, renorm_interval=DD_FMA_RENORM_INTERVAL, stream=0):
+    _dev(lib().mfhip_reduce(kind, a, b, n, renorm_interval, out, work, stream))
+
+
+def reduce_workspace_bytes(n):
+    return lib().mfhip_reduce_workspace_bytes(n)
+
+
+def matmul_variants():
+    return lib().mfhip_matmul_variants()

@@ -1,0 +1,344 @@
+// C-ABI layer: the reference-compatible host-pointer entry points, the array
+// entry points and the device-pointer API declared in include/multifloats_hip.h.
+//
+// Host-pointer calls stage through per-thread device buffers and a per-thread
+// stream (created once, default priority), copy in, launch, copy out and
+// synchronise -- so they are synchronous and re-entrant like the reference's.
+// Nothing here computes on the CPU: on any HIP failure the outputs are set to
+// NaN, the error is printed and kept for mfhip_last_error().
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+
+#include "../../include/multifloats_hip.h"
+#include "mf_internal.h"
+
+using multifloats::float64x2;
+
+namespace {
+
+struct Staging {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  void *buf[4] = {nullptr, nullptr, nullptr, nullptr};
+  size_t cap[4] = {0, 0, 0, 0};
+};
+
+thread_local Staging tl_stage;
+thread_local int tl_err = 0;
+
+hipError_t stage_init() {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (tl_stage.stream && tl_stage.device == dev) return hipSuccess;
+  // A device switch: the old buffers and stream belong to the other device and
+  // are left to it (freeing would need that device current again).
+  tl_stage = Staging();
+  tl_stage.device = dev;
+  return hipStreamCreateWithFlags(&tl_stage.stream, hipStreamNonBlocking);
+}
+
+hipError_t stage_buf(int slot, size_t bytes, void **out) {
+  if (bytes == 0) bytes = 16;
+  if (tl_stage.cap[slot] < bytes) {
+    if (tl_stage.buf[slot]) {
+      hipError_t e = hipFree(tl_stage.buf[slot]);
+      tl_stage.buf[slot] = nullptr;
+      tl_stage.cap[slot] = 0;
+      if (e != hipSuccess) return e;
+    }
+    size_t want = bytes + bytes / 4;  // grow with headroom
+    hipError_t e = hipMalloc(&tl_stage.buf[slot], want);
+    if (e != hipSuccess) return e;
+    tl_stage.cap[slot] = want;
+  }
+  *out = tl_stage.buf[slot];
+  return hipSuccess;
+}
+
+void nan_fill(float64x2 *out, int64_t n) {
+  if (!out) return;
+  for (int64_t i = 0; i < n; ++i) out[i].limbs[0] = out[i].limbs[1] = NAN;
+}
+
+void report(const char *fn, hipError_t e) {
+  tl_err = (int)e;
+  fprintf(stderr, "multifloats_hip: %s failed: %s\n", fn, hipGetErrorString(e));
+}
+
+#define TRY(x)                     \
+  do {                             \
+    hipError_t e_ = (x);           \
+    if (e_ != hipSuccess) return e_; \
+  } while (0)
+
+hipError_t h2d(void *dst, const void *src, size_t bytes) {
+  if (!bytes) return hipSuccess;
+  return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, tl_stage.stream);
+}
+hipError_t d2h(void *dst, const void *src, size_t bytes) {
+  if (!bytes) return hipSuccess;
+  return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, tl_stage.stream);
+}
+
+const double2 *D(const void *p) { return static_cast<const double2 *>(p); }
+double2 *W(void *p) { return static_cast<double2 *>(p); }
+
+hipError_t host_mm(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t m, int64_t k,
+                   int64_t n, int64_t ri) {
+  tl_err = 0;
+  if (m <= 0 || n <= 0) return hipSuccess;
+  TRY(stage_init());
+  const size_t kk = k > 0 ? (size_t)k : 0;
+  void *da, *db, *dc;
+  TRY(stage_buf(0, 16 * (size_t)m * kk, &da));
+  TRY(stage_buf(1, 16 * kk * (size_t)n, &db));
+  TRY(stage_buf(2, 16 * (size_t)m * n, &dc));
+  TRY(h2d(da, a, 16 * (size_t)m * kk));
+  TRY(h2d(db, b, 16 * kk * (size_t)n));
+  TRY(mfhip::gemm_plain(D(da), D(db), W(dc), m, k, n, ri, -1, tl_stage.stream));
+  TRY(d2h(c, dc, 16 * (size_t)m * n));
+  return hipStreamSynchronize(tl_stage.stream);
+}
+
+hipError_t host_mv(const float64x2 *a, const float64x2 *x, float64x2 *y, int64_t m, int64_t k,
+                   int64_t ri) {
+  tl_err = 0;
+  if (m <= 0) return hipSuccess;
+  TRY(stage_init());
+  const size_t kk = k > 0 ? (size_t)k : 0;
+  void *da, *dx, *dy;
+  TRY(stage_buf(0, 16 * (size_t)m * kk, &da));
+  TRY(stage_buf(1, 16 * kk, &dx));
+  TRY(stage_buf(2, 16 * (size_t)m, &dy));
+  TRY(h2d(da, a, 16 * (size_t)m * kk));
+  TRY(h2d(dx, x, 16 * kk));
+  TRY(mfhip::gemv(D(da), D(dx), W(dy), m, k, ri, tl_stage.stream));
+  TRY(d2h(y, dy, 16 * (size_t)m));
+  return hipStreamSynchronize(tl_stage.stream);
+}
+
+hipError_t host_vm(const float64x2 *x, const float64x2 *b, float64x2 *y, int64_t k, int64_t n,
+                   int64_t ri) {
+  tl_err = 0;
+  if (n <= 0) return hipSuccess;
+  TRY(stage_init());
+  const size_t kk = k > 0 ? (size_t)k : 0;
+  void *dx, *db, *dy;
+  TRY(stage_buf(0, 16 * kk, &dx));
+  TRY(stage_buf(1, 16 * kk * (size_t)n, &db));
+  TRY(stage_buf(2, 16 * (size_t)n, &dy));
+  TRY(h2d(dx, x, 16 * kk));
+  TRY(h2d(db, b, 16 * kk * (size_t)n));
+  TRY(mfhip::gevm(D(dx), D(db), W(dy), k, n, ri, tl_stage.stream));
+  TRY(d2h(y, dy, 16 * (size_t)n));
+  return hipStreamSynchronize(tl_stage.stream);
+}
+
+hipError_t host_unary(int op, const float64x2 *a, float64x2 *c, int64_t n) {
+  tl_err = 0;
+  if (n <= 0) return hipSuccess;
+  TRY(stage_init());
+  void *da, *dc;
+  TRY(stage_buf(0, 16 * (size_t)n, &da));
+  TRY(stage_buf(2, 16 * (size_t)n, &dc));
+  TRY(h2d(da, a, 16 * (size_t)n));
+  TRY(mfhip::unary(op, D(da), W(dc), n, tl_stage.stream));
+  TRY(d2h(c, dc, 16 * (size_t)n));
+  return hipStreamSynchronize(tl_stage.stream);
+}
+
+hipError_t host_binary(int op, const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n) {
+  tl_err = 0;
+  if (n <= 0) return hipSuccess;
+  TRY(stage_init());
+  void *da, *db, *dc;
+  TRY(stage_buf(0, 16 * (size_t)n, &da));
+  TRY(stage_buf(1, 16 * (size_t)n, &db));
+  TRY(stage_buf(2, 16 * (size_t)n, &dc));
+  TRY(h2d(da, a, 16 * (size_t)n));
+  TRY(h2d(db, b, 16 * (size_t)n));
+  TRY(mfhip::binary(op, D(da), D(db), W(dc), n, tl_stage.stream));
+  TRY(d2h(c, dc, 16 * (size_t)n));
+  return hipStreamSynchronize(tl_stage.stream);
+}
+
+hipError_t host_reduce(int kind, const float64x2 *a, const float64x2 *b, int64_t n, int64_t ri,
+                       float64x2 *out) {
+  tl_err = 0;
+  TRY(stage_init());
+  const size_t nn = n > 0 ? (size_t)n : 0;
+  void *da, *db = nullptr, *dw;
+  TRY(stage_buf(0, 16 * nn + 16, &da));
+  if (kind == mfhip::RED_DOT) TRY(stage_buf(1, 16 * nn, &db));
+  TRY(stage_buf(3, mfhip::reduce_workspace_bytes(n) + 16, &dw));
+  TRY(h2d(da, a, 16 * nn));
+  if (kind == mfhip::RED_DOT) TRY(h2d(db, b, 16 * nn));
+  double2 *dout = W(dw);  // result slot first, partials after it
+  TRY(mfhip::reduce(kind, D(da), D(db), n, ri, dout, dout + 1, tl_stage.stream));
+  TRY(d2h(out, dout, 16));
+  return hipStreamSynchronize(tl_stage.stream);
+}
+
+float64x2 nan_dd() {
+  float64x2 r;
+  r.limbs[0] = r.limbs[1] = NAN;
+  return r;
+}
+
+}  // namespace
+
+namespace multifloats {
+extern "C" {
+
+void matmuldd_mm(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t m, int64_t k,
+                 int64_t n, int64_t ri) {
+  hipError_t e = host_mm(a, b, c, m, k, n, ri);
+  if (e != hipSuccess) {
+    report("matmuldd_mm", e);
+    nan_fill(c, (m > 0 && n > 0) ? m * n : 0);
+  }
+}
+void matmuldd_mv(const float64x2 *a, const float64x2 *x, float64x2 *y, int64_t m, int64_t k,
+                 int64_t ri) {
+  hipError_t e = host_mv(a, x, y, m, k, ri);
+  if (e != hipSuccess) {
+    report("matmuldd_mv", e);
+    nan_fill(y, m > 0 ? m : 0);
+  }
+}
+void matmuldd_vm(const float64x2 *x, const float64x2 *b, float64x2 *y, int64_t k, int64_t n,
+                 int64_t ri) {
+  hipError_t e = host_vm(x, b, y, k, n, ri);
+  if (e != hipSuccess) {
+    report("matmuldd_vm", e);
+    nan_fill(y, n > 0 ? n : 0);
+  }
+}
+void matmulf64x2_mm(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t m, int64_t k,
+                    int64_t n, int64_t ri) {
+  matmuldd_mm(a, b, c, m, k, n, ri);
+}
+void matmulf64x2_mv(const float64x2 *a, const float64x2 *x, float64x2 *y, int64_t m, int64_t k,
+                    int64_t ri) {
+  matmuldd_mv(a, x, y, m, k, ri);
+}
+void matmulf64x2_vm(const float64x2 *x, const float64x2 *b, float64x2 *y, int64_t k, int64_t n,
+                    int64_t ri) {
+  matmuldd_vm(x, b, y, k, n, ri);
+}
+
+#define MF_BINARY(name, op)                                                        \
+  void name(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n) {      \
+    hipError_t e = host_binary(op, a, b, c, n);                                    \
+    if (e != hipSuccess) {                                                          \
+      report(#name, e);                                                             \
+      nan_fill(c, n > 0 ? n : 0);                                                   \
+    }                                                                               \
+  }
+#define MF_UNARY(name, op)                                                         \
+  void name(const float64x2 *a, float64x2 *c, int64_t n) {                          \
+    hipError_t e = host_unary(op, a, c, n);                                        \
+    if (e != hipSuccess) {                                                          \
+      report(#name, e);                                                             \
+      nan_fill(c, n > 0 ? n : 0);                                                   \
+    }                                                                               \
+  }
+MF_BINARY(adddd_v, mfhip::OP_ADD)
+MF_BINARY(subdd_v, mfhip::OP_SUB)
+MF_BINARY(muldd_v, mfhip::OP_MUL)
+MF_BINARY(divdd_v, mfhip::OP_DIV)
+MF_UNARY(sqrtdd_v, mfhip::OP_SQRT)
+MF_UNARY(expdd_v, mfhip::OP_EXP)
+MF_UNARY(logdd_v, mfhip::OP_LOG)
+MF_UNARY(sindd_v, mfhip::OP_SIN)
+MF_UNARY(erfdd_v, mfhip::OP_ERF)
+
+float64x2 sumdd(const float64x2 *a, int64_t n) {
+  float64x2 r;
+  hipError_t e = host_reduce(mfhip::RED_SUM, a, nullptr, n, 0, &r);
+  if (e != hipSuccess) {
+    report("sumdd", e);
+    return nan_dd();
+  }
+  return r;
+}
+float64x2 dotdd(const float64x2 *a, const float64x2 *b, int64_t n, int64_t ri) {
+  float64x2 r;
+  hipError_t e = host_reduce(mfhip::RED_DOT, a, b, n, ri, &r);
+  if (e != hipSuccess) {
+    report("dotdd", e);
+    return nan_dd();
+  }
+  return r;
+}
+float64x2 norm2dd(const float64x2 *a, int64_t n) {
+  float64x2 r;
+  hipError_t e = host_reduce(mfhip::RED_NORM2, a, nullptr, n, 0, &r);
+  if (e != hipSuccess) {
+    report("norm2dd", e);
+    return nan_dd();
+  }
+  return r;
+}
+
+// ---- device-pointer API ----
+
+static int status(hipError_t e) {
+  if (e != hipSuccess) tl_err = (int)e;
+  return (int)e;
+}
+static bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
+#define CHECK_ALIGN(...)                                                   \
+  do {                                                                     \
+    const void *ps_[] = {__VA_ARGS__};                                     \
+    for (const void *p_ : ps_)                                             \
+      if (p_ && !aligned16(p_)) return status(hipErrorInvalidValue);       \
+  } while (0)
+
+int mfhip_matmul_variants(void) { return mfhip::gemm_plain_variants(); }
+
+int mfhip_matmul_mm_variant(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t m,
+                            int64_t k, int64_t n, int64_t ri, int variant, void *stream) {
+  CHECK_ALIGN(a, b, c);
+  if (variant >= mfhip::gemm_plain_variants()) return status(hipErrorInvalidValue);
+  return status(mfhip::gemm_plain(D(a), D(b), W(c), m, k, n, ri, variant, (hipStream_t)stream));
+}
+int mfhip_matmul_mm(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t m, int64_t k,
+                    int64_t n, int64_t ri, void *stream) {
+  return mfhip_matmul_mm_variant(a, b, c, m, k, n, ri, -1, stream);
+}
+int mfhip_matmul_mv(const float64x2 *a, const float64x2 *x, float64x2 *y, int64_t m, int64_t k,
+                    int64_t ri, void *stream) {
+  CHECK_ALIGN(a, x, y);
+  return status(mfhip::gemv(D(a), D(x), W(y), m, k, ri, (hipStream_t)stream));
+}
+int mfhip_matmul_vm(const float64x2 *x, const float64x2 *b, float64x2 *y, int64_t k, int64_t n,
+                    int64_t ri, void *stream) {
+  CHECK_ALIGN(x, b, y);
+  return status(mfhip::gevm(D(x), D(b), W(y), k, n, ri, (hipStream_t)stream));
+}
+int mfhip_unary(int op, const float64x2 *a, float64x2 *c, int64_t n, void *stream) {
+  CHECK_ALIGN(a, c);
+  return status(mfhip::unary(op, D(a), W(c), n, (hipStream_t)stream));
+}
+int mfhip_binary(int op, const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n,
+                 void *stream) {
+  CHECK_ALIGN(a, b, c);
+  return status(mfhip::binary(op, D(a), D(b), W(c), n, (hipStream_t)stream));
+}
+size_t mfhip_reduce_workspace_bytes(int64_t n) { return mfhip::reduce_workspace_bytes(n); }
+int mfhip_reduce(int kind, const float64x2 *a, const float64x2 *b, int64_t n, int64_t ri,
+                 float64x2 *out, void *work, void *stream) {
+  CHECK_ALIGN(a, b, out, work);
+  if (!work || !out) return status(hipErrorInvalidValue);
+  return status(mfhip::reduce(kind, D(a), D(b), n, ri, W(out), W(work), (hipStream_t)stream));
+}
+
+int mfhip_last_error(void) { return tl_err; }
+const char *mfhip_error_string(int err) { return hipGetErrorString((hipError_t)err); }
+
+}  // extern "C"
+}  // namespace multifloats

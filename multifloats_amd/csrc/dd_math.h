@@ -1,0 +1,250 @@
+// DD transcendentals for the elementwise kernels: exp, log, sin, erf.
+//
+// exp, log and sin keep the reference's algorithms step for step (the same
+// range reductions, table sizes, polynomial degrees and evaluation trees), so
+// their results are bit-identical to the reference CPU functions:
+//   dd_exp  src/float64x2/exp_log.inc:237-277 with exp2_from_reduced 26-72
+//   dd_log  exp_log.inc:329-341, log2 311-327, log2_kernel 280-308
+//   dd_sin  src/float64x2/trig.inc:241-253, reduce_pi_half 114-147,
+//           sin_eval/cos_eval 165-187, kernels 151-160
+//   estrin10 / estrin12 / horner7   src/float64x2/poly.inc:9-16, 69-104
+// The constant tables are closed forms produced by gen/gen_tables.py.
+//
+// dd_erf is this project's own design, not the reference's rational fits
+// (src/float64x2/special.inc:18-139): one branch-free piecewise Taylor
+// evaluation whose coefficient row is picked per lane, so a wavefront never
+// diverges between regimes. Its accuracy target is the reference's own
+// measured max relative error against a 200-bit oracle.
+#pragma once
+#include "dd_core.h"
+#include "dd_tables.h"
+
+MF_HD ddv cdd(const double *hi, const double *lo, int i) { return mk(hi[i], lo[i]); }
+
+// c0 + c1 x + ... + c10 x^10, Estrin tree of the reference (degree-10 case).
+MF_HD ddv estrin10(ddv x, const double *hi, const double *lo) {
+  ddv x2 = dd_mul(x, x);
+  ddv x4 = dd_mul(x2, x2);
+  ddv x8 = dd_mul(x4, x4);
+  ddv p01 = dd_add(cdd(hi, lo, 0), dd_mul(cdd(hi, lo, 1), x));
+  ddv p23 = dd_add(cdd(hi, lo, 2), dd_mul(cdd(hi, lo, 3), x));
+  ddv p45 = dd_add(cdd(hi, lo, 4), dd_mul(cdd(hi, lo, 5), x));
+  ddv p67 = dd_add(cdd(hi, lo, 6), dd_mul(cdd(hi, lo, 7), x));
+  ddv p89 = dd_add(cdd(hi, lo, 8), dd_mul(cdd(hi, lo, 9), x));
+  ddv p03 = dd_add(p01, dd_mul(p23, x2));
+  ddv p47 = dd_add(p45, dd_mul(p67, x2));
+  ddv p810 = dd_add(p89, dd_mul(cdd(hi, lo, 10), x2));
+  return dd_add(dd_add(p03, dd_mul(p47, x4)), dd_mul(p810, x8));
+}
+
+// Degree-12 case of the same tree.
+MF_HD ddv estrin12(ddv x, const double *hi, const double *lo) {
+  ddv x2 = dd_mul(x, x);
+  ddv x4 = dd_mul(x2, x2);
+  ddv x8 = dd_mul(x4, x4);
+  ddv p01 = dd_add(cdd(hi, lo, 0), dd_mul(cdd(hi, lo, 1), x));
+  ddv p23 = dd_add(cdd(hi, lo, 2), dd_mul(cdd(hi, lo, 3), x));
+  ddv p45 = dd_add(cdd(hi, lo, 4), dd_mul(cdd(hi, lo, 5), x));
+  ddv p67 = dd_add(cdd(hi, lo, 6), dd_mul(cdd(hi, lo, 7), x));
+  ddv p89 = dd_add(cdd(hi, lo, 8), dd_mul(cdd(hi, lo, 9), x));
+  ddv p1011 = dd_add(cdd(hi, lo, 10), dd_mul(cdd(hi, lo, 11), x));
+  ddv p03 = dd_add(p01, dd_mul(p23, x2));
+  ddv p47 = dd_add(p45, dd_mul(p67, x2));
+  ddv p811 = dd_add(p89, dd_mul(p1011, x2));
+  ddv top = dd_add(p811, dd_mul(cdd(hi, lo, 12), x4));
+  return dd_add(dd_add(p03, dd_mul(p47, x4)), dd_mul(top, x8));
+}
+
+// ---------------------------------------------------------------- exp
+MF_HD ddv exp2_reduced(double n_d, double rh, double rl) {
+  if (n_d > 1100.0) return mk(__builtin_inf(), 0.0);
+  if (n_d < -1100.0) return mk(0.0, 0.0);
+  double k_d = __builtin_rint(rh * 256.0);
+  int k = (int)k_d;
+  k = k > 128 ? 128 : (k < -128 ? -128 : k);
+  double sh = rh - (double)k * 0.00390625;
+  double sl = rl;
+  fast_two_sum(sh, sl, sh, sl);
+  ddv s = mk(sh, sl);
+  ddv q = cdd(mf_exp2c_hi, mf_exp2c_lo, 9);
+  for (int j = 8; j >= 1; --j) {
+    q = dd_mul(q, s);
+    q = dd_add(q, cdd(mf_exp2c_hi, mf_exp2c_lo, j));
+  }
+  ddv pow2s = dd_add(mk(1.0, 0.0), dd_mul(s, q));
+  ddv mant = dd_mul(cdd(mf_exp2t_hi, mf_exp2t_lo, k + 128), pow2s);
+  int n = (int)n_d;
+  return mk(__builtin_ldexp(mant.hi, n), __builtin_ldexp(mant.lo, n));
+}
+
+MF_HD ddv dd_exp(ddv x) {
+  if (!mf_isfinite(x.hi)) {
+    if (x.hi > 0.0) return mk(x.hi, 0.0);
+    if (x.hi < 0.0) return mk(0.0, 0.0);
+    return mk(x.hi, 0.0);
+  }
+  if (x.hi > mf_exp_max) return mk(__builtin_inf(), 0.0);
+  if (x.hi < mf_exp_min) return mk(0.0, 0.0);
+  double n_d = __builtin_rint(x.hi * mf_log2e_hi);
+  double rh, rl, t;
+  two_sum(x.hi, -(n_d * mf_ln2_cw1), rh, t);
+  rl = t + x.lo;
+  fast_two_sum(rh, rl, rh, rl);
+  double p_hi = n_d * mf_ln2_cw2;
+  double p_lo = __builtin_fma(n_d, mf_ln2_cw2, -p_hi);
+  two_sum(rh, -p_hi, rh, t);
+  rl = rl + (t - p_lo);
+  fast_two_sum(rh, rl, rh, rl);
+  two_sum(rh, -(n_d * mf_ln2_cw3), rh, t);
+  rl = rl + t;
+  ddv rr;
+  fast_two_sum(rh, rl, rr.hi, rr.lo);
+  ddv y = dd_mul(rr, mk(mf_log2e_hi, mf_log2e_lo));
+  return exp2_reduced(n_d, y.hi, y.lo);
+}
+
+// ---------------------------------------------------------------- log
+MF_HD ddv log2_kernel(ddv x) {
+  double hi = x.hi;
+  if (hi > 15.0 / 16.0 && hi < 17.0 / 16.0) {
+    ddv t = dd_div(dd_sub(x, mk(1.0, 0.0)), dd_add(x, mk(1.0, 0.0)));
+    ddv p = estrin10(dd_mul(t, t), mf_log2c_hi, mf_log2c_lo);
+    return dd_mul(t, p);
+  }
+  int fe;
+  (void)__builtin_frexp(hi, &fe);
+  int e = fe - 1;  // ilogb for finite nonzero input, subnormals included
+  ddv m = mk(__builtin_ldexp(x.hi, -e), __builtin_ldexp(x.lo, -e));
+  int idx = (int)((m.hi - 1.0) * 32.0);
+  idx = idx < 0 ? 0 : (idx > 31 ? 31 : idx);
+  ddv c = mk(mf_log2_centre[idx], 0.0);
+  ddv v = cdd(mf_log2v_hi, mf_log2v_lo, idx);
+  ddv t = dd_div(dd_sub(m, c), dd_add(m, c));
+  ddv ts = dd_mul(t, t);
+  ddv p = cdd(mf_log2c_hi, mf_log2c_lo, 6);
+  for (int i = 5; i >= 0; --i) p = dd_add(dd_mul(p, ts), cdd(mf_log2c_hi, mf_log2c_lo, i));
+  return dd_add(dd_add(mk((double)e, 0.0), v), dd_mul(t, p));
+}
+
+MF_HD ddv dd_log(ddv x) {
+  if (!mf_isfinite(x.hi)) {
+    // log(+inf) = +inf; log(-inf) and log(NaN) are NaN.
+    return mk(x.hi > 0.0 ? x.hi : __builtin_nan(""), 0.0);
+  }
+  ddv l2;
+  if (x.hi < 0.0) return mk(__builtin_nan(""), 0.0);
+  if (x.hi == 0.0) return mk(-__builtin_inf(), 0.0);
+  l2 = log2_kernel(x);
+  if (!mf_isfinite(l2.hi)) return l2;
+  return dd_mul(l2, mk(mf_ln2_hi, mf_ln2_lo));
+}
+
+// ---------------------------------------------------------------- sin
+MF_HD void reduce_pio2(ddv x, ddv &r, int &q) {
+  double n = __builtin_rint(x.hi * 0.6366197723675814);
+  double rh = x.hi, rl = x.lo;
+  const double cw[3] = {mf_pio2_cw1, mf_pio2_cw2, mf_pio2_cw3};
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+    double ph = n * cw[s];
+    double pl = s < 2 ? __builtin_fma(n, cw[s], -ph) : 0.0;
+    double sh, sl, tl, tll;
+    two_sum(rh, -ph, sh, sl);
+    two_sum(rl, -pl, tl, tll);
+    sl = sl + tl;
+    fast_two_sum(sh, sl, rh, rl);
+    rl = rl + tll;
+    fast_two_sum(rh, rl, rh, rl);
+  }
+  r = mk(rh, rl);
+  long long nn = (long long)__builtin_fabs(n) & 3LL;
+  if (n < 0.0) nn = (4 - nn) & 3;
+  q = (int)nn;
+}
+
+MF_HD ddv sin_kernel(ddv x) { return dd_mul(estrin12(dd_mul(x, x), mf_sinc_hi, mf_sinc_lo), x); }
+MF_HD ddv cos_kernel(ddv x) { return estrin12(dd_mul(x, x), mf_cosc_hi, mf_cosc_lo); }
+
+// want_cos = 0: sin(r); 1: cos(r), for |r| <= pi/4.
+MF_HD ddv sincos_eval(ddv r, int want_cos) {
+  const double pi8 = 0.392699081698724;
+  if (__builtin_fabs(r.hi) <= pi8) return want_cos ? cos_kernel(r) : sin_kernel(r);
+  ddv pi4 = mk(mf_pi_quarter_hi, mf_pi_quarter_lo);
+  ddv is2 = mk(mf_inv_sqrt2_hi, mf_inv_sqrt2_lo);
+  bool pos = r.hi > 0.0;
+  ddv rp = pos ? dd_sub(r, pi4) : dd_add(r, pi4);
+  ddv sk = sin_kernel(rp);
+  ddv ck = cos_kernel(rp);
+  ddv comb;
+  if (!want_cos) comb = pos ? dd_add(sk, ck) : dd_sub(sk, ck);
+  else comb = pos ? dd_sub(ck, sk) : dd_add(ck, sk);
+  return dd_mul(comb, is2);
+}
+
+MF_HD ddv dd_sin(ddv x) {
+  if (x.hi == 0.0 && x.lo == 0.0) return x;
+  if (!mf_isfinite(x.hi) || !(__builtin_fabs(x.hi) < 0x1p55)) return mk(__builtin_nan(""), 0.0);
+  ddv r;
+  int q;
+  reduce_pio2(x, r, q);
+  ddv v = sincos_eval(r, q & 1);
+  return (q & 2) ? dd_neg(v) : v;
+}
+
+// ---------------------------------------------------------------- erf
+// Row 0 (|x| < 1/4):   erf(x) = x + x * Q(x^2)
+// Row i (1/4 <= |x| < 9): erf(c_i + t) = a0 + t * P(t), |t| <= 1/32
+// P and Q share one evaluator: the first MF_ERF_ND coefficients are
+// double-doubles, the tail (contributing < 2^-56 of the result) is plain
+// double Horner in u.hi. a0 is a three-limb constant and the final sum
+// base + mult*P is rounded once, so the result error stays near one half-ulp
+// of a double-double.
+MF_HD ddv dd_erf(ddv x) {
+  if (x.hi == 0.0 && x.lo == 0.0) return x;
+  if (!mf_isfinite(x.hi)) {
+    if (x.hi != x.hi) return mk(x.hi, 0.0);
+    return mk(x.hi > 0.0 ? 1.0 : -1.0, 0.0);
+  }
+  bool neg = x.hi < 0.0;
+  ddv ax = neg ? dd_neg(x) : x;
+  if (ax.hi >= mf_erf_xmax) return mk(neg ? -1.0 : 1.0, 0.0);
+  int row = 0;
+  if (ax.hi >= mf_erf_x0) {
+    row = 1 + (int)((ax.hi - mf_erf_x0) * mf_erf_inv_h);
+    row = row > MF_ERF_ROWS - 1 ? MF_ERF_ROWS - 1 : row;
+  }
+  ddv u, mult;
+  double b0, b1, b2;
+  if (row == 0) {
+    u = dd_mul(ax, ax);
+    mult = ax;
+    b0 = ax.hi;
+    b1 = ax.lo;
+    b2 = 0.0;
+  } else {
+    double th = ax.hi - mf_erf_centre[row];  // exact (Sterbenz)
+    double tl = ax.lo;
+    fast_two_sum(th, tl, th, tl);
+    u = mk(th, tl);
+    mult = u;
+    b0 = mf_erf_base[3 * row];
+    b1 = mf_erf_base[3 * row + 1];
+    b2 = mf_erf_base[3 * row + 2];
+  }
+  const double *dtail = mf_erf_d + row * (MF_ERF_NT - MF_ERF_ND);
+  double d = dtail[MF_ERF_NT - MF_ERF_ND - 1];
+  for (int j = MF_ERF_NT - MF_ERF_ND - 2; j >= 0; --j) d = __builtin_fma(d, u.hi, dtail[j]);
+  const double *ph = mf_erf_ddhi + row * MF_ERF_ND;
+  const double *pl = mf_erf_ddlo + row * MF_ERF_ND;
+  ddv p = dd_add(mk(ph[MF_ERF_ND - 1], pl[MF_ERF_ND - 1]), dd_mul(u, mk(d, 0.0)));
+  for (int j = MF_ERF_ND - 2; j >= 0; --j) p = dd_add(mk(ph[j], pl[j]), dd_mul(u, p));
+  ddv corr = dd_mul(mult, p);
+  double s, e1, u1, v1, w, z;
+  two_sum(b0, corr.hi, s, e1);
+  two_sum(b1, corr.lo, u1, v1);
+  two_sum(e1, u1, w, z);
+  double lo = w + (z + (v1 + b2));
+  ddv r;
+  fast_two_sum(s, lo, r.hi, r.lo);
+  return neg ? dd_neg(r) : r;
+}

@@ -1,0 +1,136 @@
+// Elementwise DD array kernels: + - * / sqrt (HBM-bound, bit-exact with the
+// reference operators) and exp / log / sin / erf (FP64-issue-bound).
+//
+// The reference evaluates these one scalar C-ABI call per element from a
+// compiler-generated Fortran elemental loop (fsrc/multifloats.fypp:1407-1412,
+// 793-799); here one launch covers the whole array. Each lane handles kUnroll
+// elements spaced one grid-stride apart so that kUnroll independent 16-byte
+// loads are in flight before any arithmetic, and loads/stores stay fully
+// coalesced on the AoS {hi,lo} layout (one global_load_dwordx4 per element).
+#include "dd_math.h"
+#include "mf_internal.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+
+template <int OP>
+__device__ __forceinline__ ddv unary_op(ddv x) {
+  if constexpr (OP == mfhip::OP_SQRT) return dd_sqrt(x);
+  else if constexpr (OP == mfhip::OP_EXP) return dd_exp(x);
+  else if constexpr (OP == mfhip::OP_LOG) return dd_log(x);
+  else if constexpr (OP == mfhip::OP_SIN) return dd_sin(x);
+  else if constexpr (OP == mfhip::OP_ERF) return dd_erf(x);
+  else if constexpr (OP == mfhip::OP_NEG) return dd_neg(x);
+  else return x;
+}
+
+template <int OP>
+__device__ __forceinline__ ddv binary_op(ddv x, ddv y) {
+  if constexpr (OP == mfhip::OP_ADD) return dd_add(x, y);
+  else if constexpr (OP == mfhip::OP_SUB) return dd_sub(x, y);
+  else if constexpr (OP == mfhip::OP_MUL) return dd_mul(x, y);
+  else return dd_div(x, y);
+}
+
+template <int OP, int U>
+__global__ __launch_bounds__(kThreads) void k_unary(const double2 *__restrict__ a,
+                                                    double2 *__restrict__ c, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * kThreads;
+  for (int64_t base = (int64_t)blockIdx.x * kThreads + threadIdx.x; base < n; base += stride * U) {
+    v2d v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      int64_t i = base + u * stride;
+      v[u] = i < n ? __builtin_nontemporal_load((const v2d *)a + i) : v2d{0.0, 0.0};
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      int64_t i = base + u * stride;
+      if (i < n) {
+        ddv r = unary_op<OP>(mk(v[u].x, v[u].y));
+        __builtin_nontemporal_store(v2d{r.hi, r.lo}, (v2d *)c + i);
+      }
+    }
+  }
+}
+
+template <int OP, int U>
+__global__ __launch_bounds__(kThreads) void k_binary(const double2 *__restrict__ a,
+                                                     const double2 *__restrict__ b,
+                                                     double2 *__restrict__ c, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * kThreads;
+  for (int64_t base = (int64_t)blockIdx.x * kThreads + threadIdx.x; base < n; base += stride * U) {
+    v2d va[U], vb[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      int64_t i = base + u * stride;
+      va[u] = i < n ? __builtin_nontemporal_load((const v2d *)a + i) : v2d{0.0, 0.0};
+      vb[u] = i < n ? __builtin_nontemporal_load((const v2d *)b + i) : v2d{0.0, 0.0};
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      int64_t i = base + u * stride;
+      if (i < n) {
+        ddv r = binary_op<OP>(mk(va[u].x, va[u].y), mk(vb[u].x, vb[u].y));
+        __builtin_nontemporal_store(v2d{r.hi, r.lo}, (v2d *)c + i);
+      }
+    }
+  }
+}
+
+// Grid: enough workgroups to cover n once with U elements per lane, capped so
+// that every CU holds several resident groups (256 CUs on MI355X).
+unsigned grid_for(int64_t n, int U, int max_blocks) {
+  int64_t per = (int64_t)kThreads * U;
+  int64_t g = (n + per - 1) / per;
+  if (g > max_blocks) g = max_blocks;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+template <int OP>
+hipError_t run_unary(const double2 *a, double2 *c, int64_t n, hipStream_t s) {
+  if (OP == mfhip::OP_SQRT || OP == mfhip::OP_NEG) {
+    hipLaunchKernelGGL((k_unary<OP, 4>), dim3(grid_for(n, 4, 8192)), dim3(kThreads), 0, s, a, c, n);
+  } else {
+    hipLaunchKernelGGL((k_unary<OP, 1>), dim3(grid_for(n, 1, 1 << 20)), dim3(kThreads), 0, s, a, c, n);
+  }
+  return hipGetLastError();
+}
+
+template <int OP>
+hipError_t run_binary(const double2 *a, const double2 *b, double2 *c, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL((k_binary<OP, 4>), dim3(grid_for(n, 4, 8192)), dim3(kThreads), 0, s, a, b, c, n);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+namespace mfhip {
+
+hipError_t unary(int op, const double2 *a, double2 *c, int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  switch (op) {
+    case OP_SQRT: return run_unary<OP_SQRT>(a, c, n, s);
+    case OP_EXP: return run_unary<OP_EXP>(a, c, n, s);
+    case OP_LOG: return run_unary<OP_LOG>(a, c, n, s);
+    case OP_SIN: return run_unary<OP_SIN>(a, c, n, s);
+    case OP_ERF: return run_unary<OP_ERF>(a, c, n, s);
+    case OP_NEG: return run_unary<OP_NEG>(a, c, n, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t binary(int op, const double2 *a, const double2 *b, double2 *c, int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  switch (op) {
+    case OP_ADD: return run_binary<OP_ADD>(a, b, c, n, s);
+    case OP_SUB: return run_binary<OP_SUB>(a, b, c, n, s);
+    case OP_MUL: return run_binary<OP_MUL>(a, b, c, n, s);
+    case OP_DIV: return run_binary<OP_DIV>(a, b, c, n, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace mfhip

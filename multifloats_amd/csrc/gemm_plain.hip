@@ -1,0 +1,264 @@
+// Plain (reference-order) double-double GEMM / GEMV kernels for gfx950.
+//
+// Semantics follow src/float64x2/matmul.inc:47-239 of the reference: every
+// output C(i,j) is one sequential chain over p = 0..k-1 of mac() with A(i,p)
+// as the "a" operand, a full TwoSum renormalisation after every complete chunk
+// of `ri` MACs that is not the last (when ri > 0 and k > ri), then a final
+// TwoSum. The reference's MR x NR blocking never changes that per-output
+// order, so any kernel that keeps the chain sequential per output -- as these
+// do -- returns the reference's bits.
+//
+// GEMM layout: a 256-thread workgroup owns a (16*TM) x (16*TN) tile of C.
+// Per K-block of BK, A and B tiles are de-interleaved from the AoS {hi,lo}
+// HBM layout into separate hi and lo LDS arrays (SoA), so each thread reads
+// its TM A-values and TN B-values with 16-byte LDS loads and runs TM*TN
+// independent MAC chains out of registers: the FP64 VALU, not memory, is the
+// bound (13 FP64 instructions per MAC). The next K-block is prefetched into
+// registers while the current one is consumed.
+#include "dd_core.h"
+#include "mf_internal.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+
+template <int TM, int TN, int BK, int WPE>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WPE))) void k_gemm_plain(const double2 *__restrict__ A,
+                                                         const double2 *__restrict__ B,
+                                                         double2 *__restrict__ C, int64_t m,
+                                                         int64_t k, int64_t n, int64_t ri,
+                                                         int tiles_m) {
+  constexpr int BM = 16 * TM, BN = 16 * TN;
+  constexpr int A_PER = BM * BK / kThreads;
+  constexpr int B_PER = BN * BK / kThreads;
+  static_assert(A_PER * kThreads == BM * BK && B_PER * kThreads == BN * BK, "tile");
+  __shared__ double As_h[BK][BM], As_l[BK][BM];
+  __shared__ double Bs_h[BK][BN], Bs_l[BK][BN];
+
+  const int tid = threadIdx.x;
+  const int tx = tid & 15, ty = tid >> 4;
+  const int tile = blockIdx.x;
+  const int64_t m0 = (int64_t)(tile % tiles_m) * BM;
+  const int64_t n0 = (int64_t)(tile / tiles_m) * BN;
+
+  // Global -> register staging coordinates.
+  // A tile (BM rows x BK cols): consecutive threads walk down a column of A.
+  // B tile (BK rows x BN cols): consecutive threads walk down a column of B.
+  double2 ra[A_PER], rb[B_PER];
+  auto load = [&](int64_t k0) {
+#pragma unroll
+    for (int q = 0; q < A_PER; ++q) {
+      int e = tid + q * kThreads;
+      int r = e % BM, c = e / BM;
+      int64_t gi = m0 + r, gp = k0 + c;
+      ra[q] = (gi < m && gp < k) ? A[gi + gp * m] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int q = 0; q < B_PER; ++q) {
+      int e = tid + q * kThreads;
+      int r = e % BK, c = e / BK;
+      int64_t gp = k0 + r, gj = n0 + c;
+      rb[q] = (gp < k && gj < n) ? B[gp + gj * k] : make_double2(0.0, 0.0);
+    }
+  };
+  auto stash = [&]() {
+#pragma unroll
+    for (int q = 0; q < A_PER; ++q) {
+      int e = tid + q * kThreads;
+      As_h[e / BM][e % BM] = ra[q].x;
+      As_l[e / BM][e % BM] = ra[q].y;
+    }
+#pragma unroll
+    for (int q = 0; q < B_PER; ++q) {
+      int e = tid + q * kThreads;
+      Bs_h[e % BK][e / BK] = rb[q].x;
+      Bs_l[e % BK][e / BK] = rb[q].y;
+    }
+  };
+
+  double sh[TM][TN], sl[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) sh[i][j] = sl[i][j] = 0.0;
+
+  const int64_t rint = (ri > 0 && k > ri) ? ri : 0;
+  int64_t until = rint;  // MACs left before the next renormalisation point
+  int64_t done = 0;
+
+  auto step = [&](int kk) {
+    double ah[TM], al[TM], bh[TN], bl[TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      ah[i] = As_h[kk][tx * TM + i];
+      al[i] = As_l[kk][tx * TM + i];
+    }
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      bh[j] = Bs_h[kk][ty * TN + j];
+      bl[j] = Bs_l[kk][ty * TN + j];
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) mac(ah[i], al[i], bh[j], bl[j], sh[i][j], sl[i][j]);
+    ++done;
+    if (rint && --until == 0) {
+      until = rint;
+      if (done < k) {
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) renorm(sh[i][j], sl[i][j]);
+      }
+    }
+  };
+
+  load(0);
+  for (int64_t k0 = 0; k0 < k; k0 += BK) {
+    __syncthreads();
+    stash();
+    __syncthreads();
+    if (k0 + BK < k) load(k0 + BK);
+    const int64_t kmax = (k - k0) < BK ? (k - k0) : BK;
+    if (kmax == BK) {
+#pragma unroll
+      for (int kk = 0; kk < BK; ++kk) step(kk);
+    } else {
+      for (int kk = 0; kk < (int)kmax; ++kk) step(kk);
+    }
+  }
+
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    const int64_t gj = n0 + ty * TN + j;
+    if (gj >= n) continue;
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      const int64_t gi = m0 + tx * TM + i;
+      if (gi >= m) continue;
+      double h = sh[i][j], l = sl[i][j];
+      renorm(h, l);
+      C[gi + gj * m] = make_double2(h, l);
+    }
+  }
+}
+
+// y(m) = A(m,k) x(k): one thread per row, A read down its columns (coalesced),
+// x broadcast. Reference matmul.inc:207-211 via gaxpy_mv_panel (47-88).
+__global__ __launch_bounds__(kThreads) void k_gemv(const double2 *__restrict__ A,
+                                                   const double2 *__restrict__ x,
+                                                   double2 *__restrict__ y, int64_t m, int64_t k,
+                                                   int64_t ri) {
+  const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= m) return;
+  const int64_t rint = (ri > 0 && k > ri) ? ri : 0;
+  double sh = 0.0, sl = 0.0;
+  int64_t until = rint;
+  for (int64_t p = 0; p < k; ++p) {
+    double2 a = A[i + p * m];
+    double2 b = x[p];
+    mac(a.x, a.y, b.x, b.y, sh, sl);
+    if (rint && --until == 0) {
+      until = rint;
+      if (p + 1 < k) renorm(sh, sl);
+    }
+  }
+  renorm(sh, sl);
+  y[i] = make_double2(sh, sl);
+}
+
+// y(n) = x(k) B(k,n): x is the "a" operand (reference matmul.inc:213-239).
+// One thread per column; an 8-deep slab of B is transposed through LDS so the
+// HBM reads stay coalesced along B's columns.
+constexpr int kVmK = 8;
+__global__ __launch_bounds__(kThreads) void k_gevm(const double2 *__restrict__ x,
+                                                   const double2 *__restrict__ B,
+                                                   double2 *__restrict__ y, int64_t k, int64_t n,
+                                                   int64_t ri) {
+  __shared__ double Bh[kVmK][kThreads], Bl[kVmK][kThreads];
+  const int tid = threadIdx.x;
+  const int64_t n0 = (int64_t)blockIdx.x * kThreads;
+  const int64_t j = n0 + tid;
+  const int64_t rint = (ri > 0 && k > ri) ? ri : 0;
+  double sh = 0.0, sl = 0.0;
+  int64_t until = rint, done = 0;
+  for (int64_t p0 = 0; p0 < k; p0 += kVmK) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kVmK; ++q) {
+      int e = tid + q * kThreads;
+      int r = e % kVmK, c = e / kVmK;
+      int64_t gp = p0 + r, gj = n0 + c;
+      double2 v = (gp < k && gj < n) ? B[gp + gj * k] : make_double2(0.0, 0.0);
+      Bh[r][c] = v.x;
+      Bl[r][c] = v.y;
+    }
+    __syncthreads();
+    const int kmax = (k - p0) < kVmK ? (int)(k - p0) : kVmK;
+    for (int r = 0; r < kmax; ++r) {
+      double2 a = x[p0 + r];
+      mac(a.x, a.y, Bh[r][tid], Bl[r][tid], sh, sl);
+      ++done;
+      if (rint && --until == 0) {
+        until = rint;
+        if (done < k) renorm(sh, sl);
+      }
+    }
+  }
+  if (j < n) {
+    renorm(sh, sl);
+    y[j] = make_double2(sh, sl);
+  }
+}
+
+template <int TM, int TN, int BK, int WPE>
+hipError_t launch_gemm(const double2 *A, const double2 *B, double2 *C, int64_t m, int64_t k,
+                       int64_t n, int64_t ri, hipStream_t s) {
+  constexpr int BM = 16 * TM, BN = 16 * TN;
+  const int64_t tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
+  if (tm * tn > 0x7fffffff) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_gemm_plain<TM, TN, BK, WPE>), dim3((unsigned)(tm * tn)), dim3(kThreads), 0, s,
+                     A, B, C, m, k, n, ri, (int)tm);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+namespace mfhip {
+
+int gemm_plain_variants() { return 7; }
+
+hipError_t gemm_plain(const double2 *A, const double2 *B, double2 *C, int64_t m, int64_t k,
+                      int64_t n, int64_t ri, int variant, hipStream_t s) {
+  if (m <= 0 || n <= 0) return hipSuccess;
+  if (k <= 0) return hipMemsetAsync(C, 0, sizeof(double2) * m * n, s);
+  if (variant < 0) variant = (m * n >= (int64_t)256 * 256) ? 1 : 0;
+  switch (variant) {
+    case 0: return launch_gemm<2, 2, 16, 4>(A, B, C, m, k, n, ri, s);
+    case 1: return launch_gemm<4, 4, 16, 2>(A, B, C, m, k, n, ri, s);
+    case 2: return launch_gemm<4, 4, 16, 3>(A, B, C, m, k, n, ri, s);
+    case 3: return launch_gemm<4, 4, 16, 1>(A, B, C, m, k, n, ri, s);
+    case 4: return launch_gemm<4, 8, 16, 1>(A, B, C, m, k, n, ri, s);
+    case 5: return launch_gemm<8, 4, 16, 1>(A, B, C, m, k, n, ri, s);
+    default: return launch_gemm<4, 8, 16, 2>(A, B, C, m, k, n, ri, s);
+  }
+}
+
+hipError_t gemv(const double2 *A, const double2 *x, double2 *y, int64_t m, int64_t k, int64_t ri,
+                hipStream_t s) {
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gemv, dim3((unsigned)((m + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                     s, A, x, y, m, k, ri);
+  return hipGetLastError();
+}
+
+hipError_t gevm(const double2 *x, const double2 *B, double2 *y, int64_t k, int64_t n, int64_t ri,
+                hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gevm, dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                     s, x, B, y, k, n, ri);
+  return hipGetLastError();
+}
+
+}  // namespace mfhip

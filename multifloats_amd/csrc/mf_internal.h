@@ -1,0 +1,35 @@
+// Internal launcher interface shared by the kernel translation units and the
+// C-ABI layer. Every launcher enqueues on the given stream and returns the
+// launch status; none of them synchronises or allocates.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+typedef double v2d __attribute__((ext_vector_type(2)));
+
+namespace mfhip {
+
+enum { OP_SQRT = 0, OP_EXP = 1, OP_LOG = 2, OP_SIN = 3, OP_ERF = 4, OP_NEG = 5 };
+enum { OP_ADD = 0, OP_SUB = 1, OP_MUL = 2, OP_DIV = 3 };
+enum { RED_DOT = 0, RED_SUM = 1, RED_NORM2 = 2 };
+
+// Plain reference-order GEMM; variant < 0 picks by size, otherwise one of
+// gemm_plain_variants() tile shapes (index order is stable for the tuner).
+int gemm_plain_variants();
+hipError_t gemm_plain(const double2 *A, const double2 *B, double2 *C, int64_t m, int64_t k,
+                      int64_t n, int64_t ri, int variant, hipStream_t s);
+hipError_t gemv(const double2 *A, const double2 *x, double2 *y, int64_t m, int64_t k, int64_t ri,
+                hipStream_t s);
+hipError_t gevm(const double2 *x, const double2 *B, double2 *y, int64_t k, int64_t n, int64_t ri,
+                hipStream_t s);
+
+hipError_t unary(int op, const double2 *a, double2 *c, int64_t n, hipStream_t s);
+hipError_t binary(int op, const double2 *a, const double2 *b, double2 *c, int64_t n, hipStream_t s);
+
+int64_t reduce_groups(int64_t n);
+size_t reduce_workspace_bytes(int64_t n);
+hipError_t reduce(int kind, const double2 *a, const double2 *b, int64_t n, int64_t ri,
+                  double2 *out, double2 *work, hipStream_t s);
+
+}  // namespace mfhip

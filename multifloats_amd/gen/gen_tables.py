@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Generate the constant tables used by the DD kernels (multifloats_amd/csrc/dd_tables.h).
+
+Every table here is a closed form evaluated with mpmath at 400 bits and rounded
+limb by limb (hi = RN(v), lo = RN(v - hi), third limb likewise). Nothing is
+taken from another project's tables.
+
+Tables that the reference's exp/log/sin algorithms need and whose values are
+fixed by mathematics (so a bit-exact restatement must use the same values):
+  - exp: ln2 Cody-Waite split with a 33-bit leading piece (reference
+    src/float64x2/exp_log.inc:247-268), exp2 Taylor coefficients ln2^j/j!
+    and the 257-entry table 2^(k/256) (exp_log.inc:26-72);
+  - log: c_k = 2/((2k+1) ln2) and the 32 centres 1+(2i+1)/64 with log2 values
+    (exp_log.inc:280-308);
+  - sin: pi/2 three-piece split, Taylor coefficients of sin(x)/x and cos(x),
+    pi/4 and 1/sqrt(2) (src/float64x2/trig.inc:114-187).
+
+The erf table is this project's own design (the reference fits erf with
+rational approximations; see csrc/dd_math.h for the piecewise Taylor scheme):
+  - row 0:   erf(x) = x + x*Q(x^2) for |x| < 1/4, Q(z) = (2/sqrt(pi) - 1)
+             + sum_{n>=1} (2/sqrt(pi)) (-1)^n z^n / (n! (2n+1));
+  - row i>0: Taylor expansion of erf about the centre c_i = 1/4 + (i - 1/2)/16,
+             erf(c_i + t) = a0 + t*(a1 + a2 t + ...), |t| <= 1/32, with a0
+             stored as three limbs.
+
+Run:  python3 multifloats_amd/gen/gen_tables.py   (rewrites csrc/dd_tables.h)
+"""
+import os
+import sys
+
+import mpmath
+from mpmath import mp, mpf
+
+mp.prec = 400
+
+ERF_ND = 10       # Horner coefficients kept as double-double
+ERF_NT = 18       # total Horner coefficients (the rest are plain doubles)
+ERF_X0 = mpf(1) / 4
+ERF_H = mpf(1) / 16
+ERF_XMAX = 9      # |x| >= 9 rounds to +-1 (erfc(9) < 2^-121)
+ERF_ROWS = 1 + int((ERF_XMAX - ERF_X0) / ERF_H)
+
+
+def limbs(v, n=2):
+    out = []
+    r = mpf(v)
+    for _ in range(n):
+        d = float(r)
+        out.append(d)
+        r = r - mpf(d)
+    return out
+
+
+def hx(d):
+    if d == 0.0:
+        return "0.0" if str(d)[0] != "-" else "-0.0"
+    return float.hex(d)
+
+
+def arr(name, vals, per_line=3):
+    lines = []
+    for i in range(0, len(vals), per_line):
+        lines.append("    " + ", ".join(hx(v) for v in vals[i:i + per_line]) + ",")
+    return "MF_TABLE double %s[%d] = {\n%s\n};\n" % (name, len(vals), "\n".join(lines))
+
+
+def scalar(name, v):
+    return "MF_SCALAR double %s = %s;\n" % (name, hx(v))
+
+
+def dd_pair(name, v):
+    h, l = limbs(v)
+    return scalar(name + "_hi", h) + scalar(name + "_lo", l)
+
+
+def main():
+    ln2 = mpmath.log(2)
+    pi = mpmath.pi
+    out = []
+    out.append("/* Generated by multifloats_amd/gen/gen_tables.py -- do not edit. */\n")
+    out.append("#pragma once\n")
+    out.append("#if defined(__HIPCC__)\n#define MF_TABLE static __constant__ const\n"
+               "#define MF_SCALAR static constexpr\n#elif defined(__cplusplus)\n"
+               "#define MF_TABLE static const\n#define MF_SCALAR static constexpr\n#else\n"
+               "#define MF_TABLE static const\n#define MF_SCALAR static const\n#endif\n\n")
+
+    out.append("/* log2(e), ln(2), pi/4, 1/sqrt(2) as double-doubles */\n")
+    out.append(dd_pair("mf_log2e", 1 / ln2))
+    out.append(dd_pair("mf_ln2", ln2))
+    out.append(dd_pair("mf_pi_quarter", pi / 4))
+    out.append(dd_pair("mf_inv_sqrt2", 1 / mpmath.sqrt(2)))
+
+    # ln2 split: leading piece keeps the top 33 significand bits of RN(ln2).
+    import struct
+    ln2_d = float(ln2)
+    bits = struct.unpack("<Q", struct.pack("<d", ln2_d))[0]
+    cw1 = struct.unpack("<d", struct.pack("<Q", bits & ~((1 << 20) - 1)))[0]
+    cw2 = float(ln2 - mpf(cw1))
+    cw3 = float(ln2 - mpf(cw1) - mpf(cw2))
+    out.append("\n/* ln2 = cw1 + cw2 + cw3; cw1 has a 33-bit significand so n*cw1 is exact */\n")
+    out.append(scalar("mf_ln2_cw1", cw1) + scalar("mf_ln2_cw2", cw2) + scalar("mf_ln2_cw3", cw3))
+    # exp clamps: the largest x below the overflow edge is RN(1024*ln2) rounded
+    # down to the double 1024*RN(ln2); below -750 every result underflows to 0.
+    out.append(scalar("mf_exp_max", 1024.0 * ln2_d))
+    out.append(scalar("mf_exp_min", -750.0))
+
+    ph = limbs(pi / 2, 3)
+    out.append("\n/* pi/2 = cw1 + cw2 + cw3 (three round-to-nearest limbs) */\n")
+    out.append(scalar("mf_pio2_cw1", ph[0]) + scalar("mf_pio2_cw2", ph[1]) + scalar("mf_pio2_cw3", ph[2]))
+
+    c = [ln2 ** j / mpmath.factorial(j) for j in range(10)]
+    out.append("\n/* 2^s - 1 = s*(c1 + c2 s + ... + c9 s^8), c_j = ln2^j / j! */\n")
+    out.append(arr("mf_exp2c_hi", [limbs(v)[0] for v in c]))
+    out.append(arr("mf_exp2c_lo", [limbs(v)[1] for v in c]))
+    t = [mpf(2) ** (mpf(k - 128) / 256) for k in range(257)]
+    out.append("\n/* T[k] = 2^((k-128)/256), k = 0..256 */\n")
+    out.append(arr("mf_exp2t_hi", [limbs(v)[0] for v in t]))
+    out.append(arr("mf_exp2t_lo", [limbs(v)[1] for v in t]))
+
+    lc = [2 / ((2 * k + 1) * ln2) for k in range(11)]
+    out.append("\n/* log2 series in t^2: c_k = 2/((2k+1) ln2) */\n")
+    out.append(arr("mf_log2c_hi", [limbs(v)[0] for v in lc]))
+    out.append(arr("mf_log2c_lo", [limbs(v)[1] for v in lc]))
+    centres = [1.0 + (2 * i + 1) / 64.0 for i in range(32)]
+    vals = [mpmath.log(mpf(x)) / ln2 for x in centres]
+    out.append("\n/* centres 1 + (2i+1)/64 and log2(centre) */\n")
+    out.append(arr("mf_log2_centre", centres))
+    out.append(arr("mf_log2v_hi", [limbs(v)[0] for v in vals]))
+    out.append(arr("mf_log2v_lo", [limbs(v)[1] for v in vals]))
+
+    st = [(-1) ** k / mpmath.factorial(2 * k + 1) for k in range(13)]
+    ct = [(-1) ** k / mpmath.factorial(2 * k) for k in range(13)]
+    out.append("\n/* sin(x)/x and cos(x) Taylor coefficients in x^2 */\n")
+    out.append(arr("mf_sinc_hi", [limbs(v)[0] for v in st]))
+    out.append(arr("mf_sinc_lo", [limbs(v)[1] for v in st]))
+    out.append(arr("mf_cosc_hi", [limbs(v)[0] for v in ct]))
+    out.append(arr("mf_cosc_lo", [limbs(v)[1] for v in ct]))
+
+    # ---- erf ----
+    two_rtpi = 2 / mpmath.sqrt(pi)
+    rows_base = []      # three limbs of a0 (row 0: unused)
+    rows_dd_hi, rows_dd_lo, rows_d = [], [], []
+    centres_e = []
+    for r in range(ERF_ROWS):
+        if r == 0:
+            p = [two_rtpi - 1] + [two_rtpi * (-1) ** n / (mpmath.factorial(n) * (2 * n + 1))
+                                  for n in range(1, ERF_NT)]
+            base = [0.0, 0.0, 0.0]
+            centres_e.append(0.0)
+        else:
+            x0 = ERF_X0 + (r - mpf(1) / 2) * ERF_H
+            assert float(x0) == x0
+            g = two_rtpi * mpmath.exp(-x0 * x0)
+            a = [g * (-1) ** (n - 1) * mpmath.hermite(n - 1, x0) / mpmath.factorial(n)
+                 for n in range(1, ERF_NT + 1)]
+            p = a
+            base = limbs(mpmath.erf(x0), 3)
+            centres_e.append(float(x0))
+        rows_base.extend(base)
+        for j in range(ERF_ND):
+            h, l = limbs(p[j])
+            rows_dd_hi.append(h)
+            rows_dd_lo.append(l)
+        for j in range(ERF_ND, ERF_NT):
+            rows_d.append(float(p[j]))
+    out.append("\n/* erf: row 0 is x + x*Q(x^2) for |x| < 1/4; row i is the Taylor expansion\n"
+               "   about mf_erf_centre[i] = 1/4 + (i - 1/2)/16 for 1/4 <= |x| < 9 */\n")
+    out.append("#define MF_ERF_ND %d\n#define MF_ERF_NT %d\n#define MF_ERF_ROWS %d\n" % (ERF_ND, ERF_NT, ERF_ROWS))
+    out.append(scalar("mf_erf_x0", float(ERF_X0)) + scalar("mf_erf_inv_h", float(1 / ERF_H))
+               + scalar("mf_erf_xmax", float(ERF_XMAX)))
+    out.append(arr("mf_erf_centre", centres_e))
+    out.append(arr("mf_erf_base", rows_base))
+    out.append(arr("mf_erf_ddhi", rows_dd_hi))
+    out.append(arr("mf_erf_ddlo", rows_dd_lo))
+    out.append(arr("mf_erf_d", rows_d))
+
+    dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "csrc", "dd_tables.h")
+    with open(dst, "w") as f:
+        f.write("".join(out))
+    print("wrote", os.path.normpath(dst), file=sys.stderr)
+
+
+if __name__ == "__main__":
+    main()

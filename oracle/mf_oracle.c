@@ -1,0 +1,496 @@
+/* CPU oracle for the multifloats DD hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+ * this library, and only as the checker or the timed CPU baseline. The product
+ * (multifloats_amd/) never links or calls it.
+ *
+ * A plain-C restatement of the reference's CPU algorithms, each function citing
+ * the reference file:line it follows (paths relative to the reference root).
+ * Built with -O2 -ffp-contract=off (oracle/Makefile) so that no multiply-add is
+ * fused behind our back; fma() is the correctly rounded libm call.
+ *
+ * Pinning (what checks each function, besides agreement with the GPU):
+ *   or_add/sub/mul/div/sqrt, or_exp/log/sin, or_matmul_*: bit-exact golden
+ *       vectors from the compiled reference (tests/golden), closed forms
+ *       (A = I, small integers) and exact-accumulator bounds.
+ *   or_sum/or_dot/or_norm2 (Fortran-only in the reference, which cannot be
+ *       built here: no fypp, no gfortran): exact-accumulator bounds, invariants
+ *       (integer sums exact, Pythagorean triples), and planted-mistake tests.
+ *   or_exact_*: brute force against Python integers/fractions on small inputs.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../multifloats_amd/csrc/dd_tables.h"
+
+typedef struct { double hi, lo; } dd;
+
+static inline dd mkdd(double h, double l) { dd r; r.hi = h; r.lo = l; return r; }
+
+/* include/multifloats/float64x2.h:83-107 */
+static inline void tsum(double a, double b, double *s, double *e) {
+  double t = a + b, ap = t - b, bp = t - ap;
+  *e = (a - ap) + (b - bp);
+  *s = t;
+}
+static inline void ftsum(double a, double b, double *s, double *e) {
+  double t = a + b, bp = t - a;
+  *e = b - bp;
+  *s = t;
+}
+static inline void tprod(double a, double b, double *p, double *e) {
+  double q = a * b;
+  *e = fma(a, b, -q);
+  *p = q;
+}
+
+/* float64x2.h:203-229 (operator+) */
+static dd o_add(dd x, dd y) {
+  double s = x.hi + y.hi, a, b, c, d;
+  if (!isfinite(s)) return mkdd(s, 0.0);
+  if (x.hi == 0.0 && y.hi == 0.0) return mkdd(s, x.lo + y.lo);
+  tsum(x.hi, y.hi, &a, &b);
+  tsum(x.lo, y.lo, &c, &d);
+  ftsum(a, c, &a, &c);
+  b += d;
+  b += c;
+  dd r;
+  ftsum(a, b, &r.hi, &r.lo);
+  return r;
+}
+/* float64x2.h:232-250 (operator-) */
+static dd o_sub(dd x, dd y) {
+  double s = x.hi - y.hi, a, b, c, d;
+  if (!isfinite(s)) return mkdd(s, 0.0);
+  if (x.hi == 0.0 && y.hi == 0.0) return mkdd(s, x.lo - y.lo);
+  tsum(x.hi, -y.hi, &a, &b);
+  tsum(x.lo, -y.lo, &c, &d);
+  ftsum(a, c, &a, &c);
+  b += d;
+  b += c;
+  dd r;
+  ftsum(a, b, &r.hi, &r.lo);
+  return r;
+}
+/* float64x2.h:253-271 (operator*) */
+static dd o_mul(dd x, dd y) {
+  double p, e;
+  tprod(x.hi, y.hi, &p, &e);
+  if (!isfinite(p)) return mkdd(p, 0.0);
+  double p01 = x.hi * y.lo, p10 = x.lo * y.hi;
+  p01 += p10;
+  e += p01;
+  dd r;
+  ftsum(p, e, &r.hi, &r.lo);
+  return r;
+}
+/* float64x2.h:274-311 (operator/) */
+static dd o_div(dd x, dd y) {
+  double q1 = x.hi / y.hi;
+  if (!isfinite(q1)) return mkdd(q1, q1);
+  if (!isfinite(y.hi)) return mkdd(q1, 0.0);
+  double p, e, r0, r0e, rh, rl;
+  tprod(q1, y.hi, &p, &e);
+  double qlo = e + q1 * y.lo;
+  tsum(x.hi, -p, &r0, &r0e);
+  double r1 = (x.lo - qlo) + r0e;
+  ftsum(r0, r1, &rh, &rl);
+  double q2 = rh / y.hi;
+  dd r;
+  ftsum(q1, q2, &r.hi, &r.lo);
+  return r;
+}
+/* float64x2.h:2190-2213 (sqrt) */
+static dd o_sqrt(dd x) {
+  double c = sqrt(x.hi);
+  if (!(x.hi > 0.0) || !isfinite(c)) return mkdd(c, 0.0);
+  double u, uu;
+  tprod(c, c, &u, &uu);
+  double cc = ((x.hi - u) - uu + x.lo) / (2.0 * c);
+  double h = c + cc;
+  return mkdd(h, (c - h) + cc);
+}
+
+#define ARR1(name, fn)                                              \
+  void name(const dd *a, dd *c, int64_t n) {                        \
+    for (int64_t i = 0; i < n; ++i) c[i] = fn(a[i]);                \
+  }
+#define ARR2(name, fn)                                              \
+  void name(const dd *a, const dd *b, dd *c, int64_t n) {           \
+    for (int64_t i = 0; i < n; ++i) c[i] = fn(a[i], b[i]);          \
+  }
+ARR2(or_add, o_add)
+ARR2(or_sub, o_sub)
+ARR2(or_mul, o_mul)
+ARR2(or_div, o_div)
+ARR1(or_sqrt, o_sqrt)
+
+/* ---------------------------------------------------------------- GEMM */
+/* src/float64x2/matmul.inc:6-19 (mac_inl) */
+static inline void o_mac(double ah, double al, double bh, double bl, double *sh, double *sl) {
+  double p = ah * bh;
+  double e = fma(ah, bh, -p);
+  double cross = fma(ah, bl, al * bh);
+  double t = *sh + p, bp = t - *sh, ap = t - bp;
+  double aerr = *sh - ap, berr = p - bp;
+  *sl += (e + cross) + (aerr + berr);
+  *sh = t;
+}
+/* matmul.inc:24-39 (renorm_inl / finalize_inl) */
+static inline void o_renorm(double *sh, double *sl) {
+  double t = *sh + *sl, bp = t - *sh, ap = t - bp;
+  double aerr = *sh - ap, berr = *sl - bp;
+  *sl = aerr + berr;
+  *sh = t;
+}
+/* One output of matmul.inc's panels: sequential p, renorm after every full
+ * chunk of ri MACs that is not the last (matmul.inc:55-87, 153-172). The
+ * reference's MR x NR blocking never changes this per-output order. */
+static dd o_chain(const dd *a, int64_t sa, const dd *b, int64_t sb, int64_t k, int64_t ri) {
+  double sh = 0.0, sl = 0.0;
+  int chunked = !(ri <= 0 || k <= ri);
+  for (int64_t p = 0; p < k; ++p) {
+    o_mac(a[p * sa].hi, a[p * sa].lo, b[p * sb].hi, b[p * sb].lo, &sh, &sl);
+    if (chunked && (p + 1) % ri == 0 && p + 1 < k) o_renorm(&sh, &sl);
+  }
+  o_renorm(&sh, &sl);
+  return mkdd(sh, sl);
+}
+/* matmul.inc:182-205 (mm_impl): C(m,n) = A(m,k) B(k,n), column-major */
+void or_matmul_mm(const dd *a, const dd *b, dd *c, int64_t m, int64_t k, int64_t n, int64_t ri) {
+  for (int64_t j = 0; j < n; ++j)
+    for (int64_t i = 0; i < m; ++i) c[i + j * m] = o_chain(a + i, m, b + j * k, 1, k, ri);
+}
+/* matmul.inc:207-211 (mv_impl) */
+void or_matmul_mv(const dd *a, const dd *x, dd *y, int64_t m, int64_t k, int64_t ri) {
+  for (int64_t i = 0; i < m; ++i) y[i] = o_chain(a + i, m, x, 1, k, ri);
+}
+/* matmul.inc:213-239 (vm_impl): x is the "a" operand of the MAC */
+void or_matmul_vm(const dd *x, const dd *b, dd *y, int64_t k, int64_t n, int64_t ri) {
+  for (int64_t j = 0; j < n; ++j) y[j] = o_chain(x, 1, b + j * k, 1, k, ri);
+}
+/* fsrc/multifloats.fypp:2609-2621 (cdd_matmul_mm): four real GEMMs, then the
+ * native DD - and + (bit-identical to the C++ operators, fypp:2772-2790).
+ * Complex arrays are interleaved {re.hi, re.lo, im.hi, im.lo}. */
+void or_cmatmul_mm(const dd *a, const dd *b, dd *c, int64_t m, int64_t k, int64_t n, int64_t ri) {
+  dd *ar = malloc(sizeof(dd) * m * k), *ai = malloc(sizeof(dd) * m * k);
+  dd *br = malloc(sizeof(dd) * k * n), *bi = malloc(sizeof(dd) * k * n);
+  dd *t1 = malloc(sizeof(dd) * m * n), *t2 = malloc(sizeof(dd) * m * n);
+  for (int64_t i = 0; i < m * k; ++i) { ar[i] = a[2 * i]; ai[i] = a[2 * i + 1]; }
+  for (int64_t i = 0; i < k * n; ++i) { br[i] = b[2 * i]; bi[i] = b[2 * i + 1]; }
+  or_matmul_mm(ar, br, t1, m, k, n, ri);
+  or_matmul_mm(ai, bi, t2, m, k, n, ri);
+  for (int64_t i = 0; i < m * n; ++i) c[2 * i] = o_sub(t1[i], t2[i]);
+  or_matmul_mm(ar, bi, t1, m, k, n, ri);
+  or_matmul_mm(ai, br, t2, m, k, n, ri);
+  for (int64_t i = 0; i < m * n; ++i) c[2 * i + 1] = o_add(t1[i], t2[i]);
+  free(ar); free(ai); free(br); free(bi); free(t1); free(t2);
+}
+
+/* ----------------------------------------------------------- reductions */
+/* fypp:2254-2308 (dd_dot_product): note cross = ah*bl + al*bh without fma,
+ * s_lo += (((e + cross) + aerr) + berr), and a fast_two_sum after EVERY chunk
+ * of ri elements including the last, then a final fast_two_sum. */
+dd or_dot(const dd *a, const dd *b, int64_t n, int64_t ri) {
+  double sh = 0.0, sl = 0.0;
+  int64_t step = ri <= 0 ? (n > 0 ? n : 1) : ri;
+  for (int64_t kk = 0; kk < n; kk += step) {
+    int64_t ke = kk + step < n ? kk + step : n;
+    for (int64_t k = kk; k < ke; ++k) {
+      double ah = a[k].hi, al = a[k].lo, bh = b[k].hi, bl = b[k].lo;
+      double p = ah * bh;
+      double e = fma(ah, bh, -p);
+      double cross = ah * bl + al * bh;
+      double t = sh + p, bp = t - sh, ap = t - bp;
+      double aerr = sh - ap, berr = p - bp;
+      sl = sl + (((e + cross) + aerr) + berr);
+      sh = t;
+    }
+    if (ri > 0) {
+      double t = sh + sl, bp = t - sh;
+      sh = t;
+      sl = sl - bp;
+    }
+  }
+  double t = sh + sl, bp = t - sh;
+  return mkdd(t, sl - bp);
+}
+/* fypp:1980-2020 (dd_sum_1d, no mask) */
+dd or_sum(const dd *a, int64_t n) {
+  double sh = 0.0, sl = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    double xh = a[i].hi, xl = a[i].lo;
+    double t = sh + xh, bp = t - sh, ap = t - bp;
+    double aerr = sh - ap, berr = xh - bp;
+    sl = sl + ((xl + aerr) + berr);
+    sh = t;
+  }
+  double t = sh + sl, bp = t - sh;
+  return mkdd(t, sl - bp);
+}
+/* fypp:2338-2352 (dd_norm2_1d): s = s + x*x with the DD operators, then the
+ * DD sqrt (sqrtdd, float64x2.h:2190-2213). No overflow scaling. */
+dd or_norm2(const dd *a, int64_t n) {
+  dd s = mkdd(0.0, 0.0);
+  for (int64_t i = 0; i < n; ++i) s = o_add(s, o_mul(a[i], a[i]));
+  return o_sqrt(s);
+}
+
+/* ------------------------------------------------------ transcendentals */
+static inline dd cdd(const double *hi, const double *lo, int i) { return mkdd(hi[i], lo[i]); }
+
+/* src/float64x2/poly.inc:69-79 (neval, degree 10) */
+static dd o_estrin10(dd x, const double *h, const double *l) {
+  dd x2 = o_mul(x, x), x4 = o_mul(x2, x2), x8 = o_mul(x4, x4);
+  dd p01 = o_add(cdd(h, l, 0), o_mul(cdd(h, l, 1), x));
+  dd p23 = o_add(cdd(h, l, 2), o_mul(cdd(h, l, 3), x));
+  dd p45 = o_add(cdd(h, l, 4), o_mul(cdd(h, l, 5), x));
+  dd p67 = o_add(cdd(h, l, 6), o_mul(cdd(h, l, 7), x));
+  dd p89 = o_add(cdd(h, l, 8), o_mul(cdd(h, l, 9), x));
+  dd p03 = o_add(p01, o_mul(p23, x2));
+  dd p47 = o_add(p45, o_mul(p67, x2));
+  dd p810 = o_add(p89, o_mul(cdd(h, l, 10), x2));
+  return o_add(o_add(p03, o_mul(p47, x4)), o_mul(p810, x8));
+}
+/* poly.inc:92-104 (neval, degree 12) */
+static dd o_estrin12(dd x, const double *h, const double *l) {
+  dd x2 = o_mul(x, x), x4 = o_mul(x2, x2), x8 = o_mul(x4, x4);
+  dd p01 = o_add(cdd(h, l, 0), o_mul(cdd(h, l, 1), x));
+  dd p23 = o_add(cdd(h, l, 2), o_mul(cdd(h, l, 3), x));
+  dd p45 = o_add(cdd(h, l, 4), o_mul(cdd(h, l, 5), x));
+  dd p67 = o_add(cdd(h, l, 6), o_mul(cdd(h, l, 7), x));
+  dd p89 = o_add(cdd(h, l, 8), o_mul(cdd(h, l, 9), x));
+  dd p1011 = o_add(cdd(h, l, 10), o_mul(cdd(h, l, 11), x));
+  dd p03 = o_add(p01, o_mul(p23, x2));
+  dd p47 = o_add(p45, o_mul(p67, x2));
+  dd p811 = o_add(p89, o_mul(p1011, x2));
+  dd top = o_add(p811, o_mul(cdd(h, l, 12), x4));
+  return o_add(o_add(p03, o_mul(p47, x4)), o_mul(top, x8));
+}
+
+/* src/float64x2/exp_log.inc:26-72 (exp2_from_reduced) */
+static dd o_exp2_reduced(double n_d, double rh, double rl) {
+  if (n_d > 1100.0) return mkdd(INFINITY, 0.0);
+  if (n_d < -1100.0) return mkdd(0.0, 0.0);
+  int k = (int)nearbyint(rh * 256.0);
+  if (k > 128) k = 128;
+  if (k < -128) k = -128;
+  double sh = rh - (double)k / 256.0, sl = rl;
+  ftsum(sh, sl, &sh, &sl);
+  dd s = mkdd(sh, sl);
+  dd q = cdd(mf_exp2c_hi, mf_exp2c_lo, 9);
+  for (int j = 8; j >= 1; --j) q = o_add(o_mul(q, s), cdd(mf_exp2c_hi, mf_exp2c_lo, j));
+  dd pow2s = o_add(mkdd(1.0, 0.0), o_mul(s, q));
+  dd mant = o_mul(cdd(mf_exp2t_hi, mf_exp2t_lo, k + 128), pow2s);
+  int n = (int)n_d;
+  return mkdd(ldexp(mant.hi, n), ldexp(mant.lo, n));
+}
+/* exp_log.inc:237-277 (exp) */
+static dd o_exp(dd x) {
+  if (!isfinite(x.hi)) return mkdd(x.hi > 0.0 ? x.hi : (x.hi < 0.0 ? 0.0 : x.hi), 0.0);
+  if (x.hi > mf_exp_max) return mkdd(INFINITY, 0.0);
+  if (x.hi < mf_exp_min) return mkdd(0.0, 0.0);
+  double n_d = nearbyint(x.hi * mf_log2e_hi), rh, rl, t;
+  tsum(x.hi, -(n_d * mf_ln2_cw1), &rh, &t);
+  rl = t + x.lo;
+  ftsum(rh, rl, &rh, &rl);
+  double p_hi = n_d * mf_ln2_cw2, p_lo = fma(n_d, mf_ln2_cw2, -p_hi);
+  tsum(rh, -p_hi, &rh, &t);
+  rl += t - p_lo;
+  ftsum(rh, rl, &rh, &rl);
+  tsum(rh, -(n_d * mf_ln2_cw3), &rh, &t);
+  rl += t;
+  dd rr;
+  ftsum(rh, rl, &rr.hi, &rr.lo);
+  dd y = o_mul(rr, mkdd(mf_log2e_hi, mf_log2e_lo));
+  return o_exp2_reduced(n_d, y.hi, y.lo);
+}
+/* exp_log.inc:280-308 (log2_kernel) */
+static dd o_log2_kernel(dd x) {
+  double hi = x.hi;
+  if (hi > 15.0 / 16.0 && hi < 17.0 / 16.0) {
+    dd t = o_div(o_sub(x, mkdd(1.0, 0.0)), o_add(x, mkdd(1.0, 0.0)));
+    return o_mul(t, o_estrin10(o_mul(t, t), mf_log2c_hi, mf_log2c_lo));
+  }
+  int e = ilogb(hi);
+  dd m = mkdd(ldexp(x.hi, -e), ldexp(x.lo, -e));
+  int idx = (int)((m.hi - 1.0) * 32.0);
+  if (idx < 0) idx = 0;
+  if (idx > 31) idx = 31;
+  dd c = mkdd(mf_log2_centre[idx], 0.0);
+  dd v = cdd(mf_log2v_hi, mf_log2v_lo, idx);
+  dd t = o_div(o_sub(m, c), o_add(m, c));
+  dd ts = o_mul(t, t);
+  dd p = cdd(mf_log2c_hi, mf_log2c_lo, 6);  /* horner, poly.inc:9-16 */
+  for (int i = 5; i >= 0; --i) p = o_add(o_mul(p, ts), cdd(mf_log2c_hi, mf_log2c_lo, i));
+  return o_add(o_add(mkdd((double)e, 0.0), v), o_mul(t, p));
+}
+/* exp_log.inc:311-341 (log2, log) */
+static dd o_log(dd x) {
+  if (!isfinite(x.hi)) return mkdd(log(x.hi), 0.0);
+  if (isnan(x.hi) || x.hi < 0.0) return mkdd(NAN, 0.0);
+  if (x.hi == 0.0) return mkdd(-INFINITY, 0.0);
+  dd l2 = o_log2_kernel(x);
+  if (!isfinite(l2.hi)) return l2;
+  return o_mul(l2, mkdd(mf_ln2_hi, mf_ln2_lo));
+}
+/* src/float64x2/trig.inc:114-147 (reduce_pi_half) */
+static void o_reduce(dd x, dd *r, int *q) {
+  double n = nearbyint(x.hi * 0.6366197723675814), rh = x.hi, rl = x.lo;
+  double cw[3] = {mf_pio2_cw1, mf_pio2_cw2, mf_pio2_cw3};
+  for (int s = 0; s < 3; ++s) {
+    double ph = n * cw[s], pl = s < 2 ? fma(n, cw[s], -ph) : 0.0, sh, sl, tl, tll;
+    tsum(rh, -ph, &sh, &sl);
+    tsum(rl, -pl, &tl, &tll);
+    sl += tl;
+    ftsum(sh, sl, &rh, &rl);
+    rl += tll;
+    ftsum(rh, rl, &rh, &rl);
+  }
+  *r = mkdd(rh, rl);
+  long long nn = (long long)fabs(n) & 3LL;
+  if (n < 0.0) nn = (4 - nn) & 3;
+  *q = (int)nn;
+}
+/* trig.inc:151-187 (sin_kernel, cos_kernel, sin_eval, cos_eval) */
+static dd o_sink(dd x) { return o_mul(o_estrin12(o_mul(x, x), mf_sinc_hi, mf_sinc_lo), x); }
+static dd o_cosk(dd x) { return o_estrin12(o_mul(x, x), mf_cosc_hi, mf_cosc_lo); }
+static dd o_eval(dd r, int want_cos) {
+  if (fabs(r.hi) <= 0.392699081698724) return want_cos ? o_cosk(r) : o_sink(r);
+  dd pi4 = mkdd(mf_pi_quarter_hi, mf_pi_quarter_lo), is2 = mkdd(mf_inv_sqrt2_hi, mf_inv_sqrt2_lo);
+  int pos = r.hi > 0.0;
+  dd rp = pos ? o_sub(r, pi4) : o_add(r, pi4);
+  dd sk = o_sink(rp), ck = o_cosk(rp);
+  if (!want_cos) return o_mul(pos ? o_add(sk, ck) : o_sub(sk, ck), is2);
+  return o_mul(pos ? o_sub(ck, sk) : o_add(ck, sk), is2);
+}
+/* trig.inc:241-253 (sin) */
+static dd o_sin(dd x) {
+  if (x.hi == 0.0 && x.lo == 0.0) return x;
+  if (!isfinite(x.hi) || !(fabs(x.hi) < 0x1p55)) return mkdd(NAN, 0.0);
+  dd r;
+  int q;
+  o_reduce(x, &r, &q);
+  dd v = o_eval(r, q & 1);
+  return (q & 2) ? mkdd(-v.hi, -v.lo) : v;
+}
+ARR1(or_exp, o_exp)
+ARR1(or_log, o_log)
+ARR1(or_sin, o_sin)
+
+/* ------------------------------------------------- exact accumulator ---
+ * A fixed-point superaccumulator over 2^-1216 .. 2^1120 in 32-bit digits held
+ * in int64 (carry-save), enough for every product of two finite doubles and
+ * for sums of up to 2^30 of them. Exact: adding a double splits its 53-bit
+ * significand over at most three digits. Used only to measure errors. */
+#define XA_DIGITS 74
+#define XA_BIAS 1216 /* bit position of 2^0 */
+typedef struct { int64_t d[XA_DIGITS]; int count; } xacc;
+
+static void xa_norm(xacc *a) {
+  for (int i = 0; i < XA_DIGITS - 1; ++i) {
+    int64_t c = a->d[i] >> 32; /* arithmetic shift: floor division */
+    a->d[i] -= c * ((int64_t)1 << 32);
+    a->d[i + 1] += c;
+  }
+  a->count = 0;
+}
+static void xa_add(xacc *a, double x) {
+  if (x == 0.0) return;
+  int e;
+  double m = frexp(x, &e); /* x = m 2^e, 0.5 <= |m| < 1 */
+  int64_t im = (int64_t)ldexp(m, 53); /* exact, |im| < 2^53 */
+  int pos = e - 53 + XA_BIAS;         /* bit position of im's lsb */
+  if (pos < 0) { /* only reachable by subnormal inputs below 2^-1216: impossible for doubles */
+    im >>= -pos; pos = 0;
+  }
+  int di = pos >> 5, sh = pos & 31;
+  /* im * 2^sh spans < 85 bits: split into three 32-bit digits */
+  int neg = im < 0;
+  uint64_t u = neg ? (uint64_t)(-im) : (uint64_t)im;
+  unsigned __int128 v = (unsigned __int128)u << sh;
+  int64_t d0 = (int64_t)(uint32_t)(v), d1 = (int64_t)(uint32_t)(v >> 32), d2 = (int64_t)(uint32_t)(v >> 64);
+  if (neg) { d0 = -d0; d1 = -d1; d2 = -d2; }
+  a->d[di] += d0;
+  a->d[di + 1] += d1;
+  a->d[di + 2] += d2;
+  if (++a->count >= (1 << 29)) xa_norm(a);
+}
+/* Round the accumulator to the nearest double (correct to within one ulp,
+ * which is all an error measurement needs). */
+static double xa_value(xacc *a) {
+  xa_norm(a);
+  /* after normalisation the top digit carries the sign: work on |value| so
+   * that every digit is in [0, 2^32), then sum from the top down */
+  int neg = a->d[XA_DIGITS - 1] < 0;
+  if (neg) {
+    for (int i = 0; i < XA_DIGITS; ++i) a->d[i] = -a->d[i];
+    xa_norm(a);
+  }
+  double r = 0.0;
+  int top = XA_DIGITS - 1;
+  while (top > 0 && a->d[top] == 0) --top;
+  for (int i = top; i >= 0 && i >= top - 3; --i) r += ldexp((double)a->d[i], 32 * i - XA_BIAS);
+  return neg ? -r : r;
+}
+static void xa_add_dd_prod(xacc *a, dd x, dd y) {
+  double p, e;
+  tprod(x.hi, y.hi, &p, &e); xa_add(a, p); xa_add(a, e);
+  tprod(x.hi, y.lo, &p, &e); xa_add(a, p); xa_add(a, e);
+  tprod(x.lo, y.hi, &p, &e); xa_add(a, p); xa_add(a, e);
+  tprod(x.lo, y.lo, &p, &e); xa_add(a, p); xa_add(a, e);
+}
+
+/* err = |sum_p a[p*sa] b[p*sb] - c| exactly (then rounded); mag = sum |a||b|. */
+void or_exact_dot_err(const dd *a, int64_t sa, const dd *b, int64_t sb, int64_t k, const dd *c,
+                      double *err, double *mag) {
+  xacc acc;
+  memset(&acc, 0, sizeof acc);
+  double m = 0.0;
+  for (int64_t p = 0; p < k; ++p) {
+    xa_add_dd_prod(&acc, a[p * sa], b[p * sb]);
+    m += fabs(a[p * sa].hi * b[p * sb].hi);
+  }
+  xa_add(&acc, -c->hi);
+  xa_add(&acc, -c->lo);
+  *err = fabs(xa_value(&acc));
+  *mag = m;
+}
+/* Exact value of sum_p a_p b_p rounded to a double-double (hi = RN(sum)). */
+void or_exact_dot(const dd *a, int64_t sa, const dd *b, int64_t sb, int64_t k, dd *out) {
+  xacc acc;
+  memset(&acc, 0, sizeof acc);
+  for (int64_t p = 0; p < k; ++p) xa_add_dd_prod(&acc, a[p * sa], b[p * sb]);
+  xacc acc2 = acc;
+  double h = xa_value(&acc);
+  xa_add(&acc2, -h);
+  out->hi = h;
+  out->lo = xa_value(&acc2);
+}
+/* err = |sum_i x_i - c|, mag = sum |x_i| */
+void or_exact_sum_err(const dd *x, int64_t n, const dd *c, double *err, double *mag) {
+  xacc acc;
+  memset(&acc, 0, sizeof acc);
+  double m = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    xa_add(&acc, x[i].hi);
+    xa_add(&acc, x[i].lo);
+    m += fabs(x[i].hi);
+  }
+  xa_add(&acc, -c->hi);
+  xa_add(&acc, -c->lo);
+  *err = fabs(xa_value(&acc));
+  *mag = m;
+}
+/* GEMM error on a sampled set of outputs: for each listed (i,j), the exact
+ * error of c(i,j) and sum_p |a_ip||b_pj|. Column-major, lda = m, ldb = k. */
+void or_gemm_err(const dd *a, const dd *b, const dd *c, int64_t m, int64_t k, int64_t n,
+                 const int64_t *rows, const int64_t *cols, int64_t npts, double *err, double *mag) {
+  (void)n;
+  for (int64_t t = 0; t < npts; ++t) {
+    int64_t i = rows[t], j = cols[t];
+    or_exact_dot_err(a + i, m, b + j * k, 1, k, &c[i + j * m], &err[t], &mag[t]);
+  }
+}

@@ -1,0 +1,168 @@
+"""ctypes access to the oracle (test infrastructure) and helpers for tests.
+
+Only tests, __graft_entry__.smoke() and bench.py's CPU-baseline leg load the
+oracle; the product never does.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from multifloats_amd import datagen
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+ORACLE = os.path.join(ROOT, "oracle", "liboracle.so")
+REF = os.path.join(ROOT, "oracle", "_ref", "libmfref.so")
+
+P, I64 = ctypes.c_void_p, ctypes.c_int64
+
+
+class F2(ctypes.Structure):
+    _fields_ = [("limbs", ctypes.c_double * 2)]
+
+
+_or = None
+
+
+def oracle():
+    global _or
+    if _or is None:
+        L = ctypes.CDLL(ORACLE)
+        for nm in ("or_add", "or_sub", "or_mul", "or_div"):
+            getattr(L, nm).argtypes = [P, P, P, I64]
+        for nm in ("or_sqrt", "or_exp", "or_log", "or_sin"):
+            getattr(L, nm).argtypes = [P, P, I64]
+        L.or_matmul_mm.argtypes = [P, P, P, I64, I64, I64, I64]
+        L.or_cmatmul_mm.argtypes = [P, P, P, I64, I64, I64, I64]
+        L.or_matmul_mv.argtypes = [P, P, P, I64, I64, I64]
+        L.or_matmul_vm.argtypes = [P, P, P, I64, I64, I64]
+        L.or_dot.argtypes = [P, P, I64, I64]
+        L.or_dot.restype = F2
+        L.or_sum.argtypes = [P, I64]
+        L.or_sum.restype = F2
+        L.or_norm2.argtypes = [P, I64]
+        L.or_norm2.restype = F2
+        D = ctypes.POINTER(ctypes.c_double)
+        L.or_exact_dot_err.argtypes = [P, I64, P, I64, I64, P, D, D]
+        L.or_exact_dot.argtypes = [P, I64, P, I64, I64, P]
+        L.or_exact_sum_err.argtypes = [P, I64, P, D, D]
+        L.or_gemm_err.argtypes = [P, P, P, I64, I64, I64, P, P, I64, P, P]
+        _or = L
+    return _or
+
+
+def ptr(a):
+    assert a.flags.c_contiguous or a.flags.f_contiguous
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def dd(a, order="C"):
+    return np.require(np.asarray(a, dtype=np.float64), requirements=[order, "A", "W"])
+
+
+def or_unary(nm, a):
+    a = dd(a)
+    c = np.empty_like(a)
+    getattr(oracle(), "or_" + nm)(ptr(a), ptr(c), a.shape[0])
+    return c
+
+
+def or_binary(nm, a, b):
+    a, b = dd(a), dd(b)
+    c = np.empty_like(a)
+    getattr(oracle(), "or_" + nm)(ptr(a), ptr(b), ptr(c), a.shape[0])
+    return c
+
+
+def cm(a):
+    """Column-major float64x2 layout of an (m, n, 2) DD matrix."""
+    return datagen.as_colmajor(a)
+
+
+def mptr(a):
+    assert datagen.is_colmajor(a)
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def or_matmul(A, B, ri):
+    m, k = A.shape[:2]
+    n = B.shape[1]
+    A, B = cm(A), cm(B)
+    C = cm(np.zeros((m, n, 2)))
+    oracle().or_matmul_mm(mptr(A), mptr(B), mptr(C), m, k, n, ri)
+    return C
+
+
+def res(r):
+    return np.array([r.limbs[0], r.limbs[1]])
+
+
+def exact_sum_err(x, c):
+    x = dd(x)
+    c = dd(np.asarray(c))
+    e, m = ctypes.c_double(), ctypes.c_double()
+    oracle().or_exact_sum_err(ptr(x), x.shape[0], ptr(c), ctypes.byref(e), ctypes.byref(m))
+    return e.value, m.value
+
+
+def exact_dot_err(a, b, c):
+    a, b, c = dd(a), dd(b), dd(np.asarray(c))
+    e, m = ctypes.c_double(), ctypes.c_double()
+    oracle().or_exact_dot_err(ptr(a), 1, ptr(b), 1, a.shape[0], ptr(c), ctypes.byref(e), ctypes.byref(m))
+    return e.value, m.value
+
+
+def exact_dot(a, b):
+    a, b = dd(a), dd(b)
+    out = np.zeros(2)
+    oracle().or_exact_dot(ptr(a), 1, ptr(b), 1, a.shape[0], ptr(out))
+    return out
+
+
+def gemm_err(A, B, C, rows, cols):
+    m, k = A.shape[:2]
+    n = B.shape[1]
+    A, B, C = cm(A), cm(B), cm(C)
+    rows = np.ascontiguousarray(rows, dtype=np.int64)
+    cols = np.ascontiguousarray(cols, dtype=np.int64)
+    err = np.zeros(rows.size)
+    mag = np.zeros(rows.size)
+    oracle().or_gemm_err(mptr(A), mptr(B), mptr(C), m, k, n, ptr(rows), ptr(cols), rows.size, ptr(err), ptr(mag))
+    return err, mag
+
+
+def bits_equal(x, y):
+    """Bit-for-bit equality of DD arrays, with any NaN matching any NaN."""
+    x = np.asarray(x, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    nx, ny = np.isnan(x), np.isnan(y)
+    same = (x.view(np.uint64) == y.view(np.uint64)) | (nx & ny)
+    return same
+
+
+def assert_bits(x, y, what=""):
+    same = bits_equal(x, y)
+    if not same.all():
+        bad = np.argwhere(~same)[:5]
+        raise AssertionError("%s: %d limbs differ; first %s: got %r want %r" % (
+            what, int((~same).sum()), bad.tolist(),
+            [np.asarray(x)[tuple(b)] for b in bad], [np.asarray(y)[tuple(b)] for b in bad]))
+
+
+def rel_err_vs_exact(got, exact3):
+    """|got - exact| / |exact| with exact as three limbs, for finite exact values.
+
+    Computed in exact rational arithmetic via Python floats -> fractions would
+    be slow; limb-wise subtraction is exact enough: (got.hi - e0) is exact when
+    got.hi and e0 are within a factor 2 (Sterbenz), then the small terms.
+    """
+    g = np.asarray(got, dtype=np.float64)
+    e = np.asarray(exact3, dtype=np.float64)
+    d = (g[:, 0] - e[:, 0]) + (g[:, 1] - e[:, 1]) - e[:, 2]
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return np.abs(d) / np.abs(e[:, 0])
+
+
+def load_golden(name):
+    return np.load(os.path.join(GOLDEN, name), allow_pickle=False)

@@ -1,0 +1,202 @@
+#!/usr/bin/env python3
+"""Write the golden fixtures under tests/golden/ (run in the build container).
+
+Inputs are seeded draws from multifloats_amd.datagen plus the special and
+known-answer values the reference's own tests use; expected outputs come from
+  - the reference's compiled C++ path (oracle/_ref/libmfref.so, built from the
+    reference sources by oracle/Makefile), called through its C ABI: adddd,
+    subdd, muldd, divdd, sqrtdd, expdd, logdd, sindd, erfdd, matmuldd_mm/mv/vm;
+  - mpmath at 320 bits for the transcendental values (stored as three limbs so
+    relative errors down to 1e-40 can be measured).
+Every fixture is data only (inputs and expected outputs).
+
+Usage: python3 tests/golden/make_golden.py
+"""
+import ctypes
+import os
+import sys
+
+import numpy as np
+import mpmath
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.normpath(os.path.join(HERE, "..", ".."))
+sys.path.insert(0, ROOT)
+from multifloats_amd import datagen  # noqa: E402
+
+mpmath.mp.prec = 320
+
+
+class F2(ctypes.Structure):
+    _fields_ = [("limbs", ctypes.c_double * 2)]
+
+
+def load_ref():
+    path = os.path.join(ROOT, "oracle", "_ref", "libmfref.so")
+    if not os.path.exists(path):
+        sys.exit("build the reference first: make -C oracle ref")
+    L = ctypes.CDLL(path)
+    for nm in ("adddd", "subdd", "muldd", "divdd"):
+        getattr(L, nm).argtypes = [F2, F2]
+        getattr(L, nm).restype = F2
+    for nm in ("sqrtdd", "expdd", "logdd", "sindd", "erfdd"):
+        getattr(L, nm).argtypes = [F2]
+        getattr(L, nm).restype = F2
+    P, I64 = ctypes.c_void_p, ctypes.c_int64
+    L.matmuldd_mm.argtypes = [P, P, P, I64, I64, I64, I64]
+    L.matmuldd_mv.argtypes = [P, P, P, I64, I64, I64]
+    L.matmuldd_vm.argtypes = [P, P, P, I64, I64, I64]
+    return L
+
+
+def f2(v):
+    r = F2()
+    r.limbs[0], r.limbs[1] = float(v[0]), float(v[1])
+    return r
+
+
+def call1(L, nm, a):
+    fn = getattr(L, nm)
+    out = np.empty_like(a)
+    for i in range(a.shape[0]):
+        r = fn(f2(a[i]))
+        out[i] = (r.limbs[0], r.limbs[1])
+    return out
+
+
+def call2(L, nm, a, b):
+    fn = getattr(L, nm)
+    out = np.empty_like(a)
+    for i in range(a.shape[0]):
+        r = fn(f2(a[i]), f2(b[i]))
+        out[i] = (r.limbs[0], r.limbs[1])
+    return out
+
+
+def limbs3(v):
+    out = []
+    for _ in range(3):
+        d = float(v)
+        out.append(d)
+        v = v - mpmath.mpf(d)
+    return out
+
+
+def mp_of(x):
+    return mpmath.mpf(float(x[0])) + mpmath.mpf(float(x[1]))
+
+
+SPECIALS = [(0.0, 0.0), (-0.0, 0.0), (1.0, 0.0), (-1.0, 0.0), (2.0, 0.0), (0.5, 1e-17),
+            (np.inf, 0.0), (-np.inf, 0.0), (np.nan, 0.0), (1e300, 1e283), (-1e300, 0.0),
+            (1e-300, 0.0), (5e-324, 0.0), (2.2250738585072014e-308, 0.0),
+            (1.7976931348623157e308, 0.0), (3.0, -1e-16), (1e16, 1.0), (-7.5, 2e-16)]
+
+
+def arith(L, rng):
+    n = 3000
+    a = np.concatenate([datagen.config0(rng, n), datagen.spread(rng, n, 30),
+                        np.array(SPECIALS * 2)])
+    b = np.concatenate([datagen.config0(rng, n), datagen.spread(rng, n, 30),
+                        np.array(SPECIALS[::-1] + SPECIALS)])
+    # exact cancellations and equal-hi cases
+    a[:50] = b[:50]
+    a[50:100, 0] = -b[50:100, 0]
+    out = dict(a=a, b=b)
+    for nm in ("adddd", "subdd", "muldd", "divdd"):
+        out[nm] = call2(L, nm, a, b)
+    sq_in = np.abs(a)
+    sq_in[-len(SPECIALS):] = a[-len(SPECIALS):]  # keep negatives and NaN among specials
+    out["sqrt_in"] = sq_in
+    out["sqrtdd"] = call1(L, "sqrtdd", sq_in)
+    np.savez_compressed(os.path.join(HERE, "arith.npz"), **out)
+
+
+def transc(L, rng):
+    n = 1500
+    cases = {
+        "exp": np.concatenate([datagen.exp_inputs(rng, n), datagen.config0(rng, 200),
+                               datagen._with_lo(rng, rng.uniform(-745.5, -700.0, 100)),
+                               datagen._with_lo(rng, rng.uniform(708.0, 709.78, 50)),
+                               np.array(SPECIALS)]),
+        "log": np.concatenate([datagen.log_inputs(rng, n),
+                               datagen._with_lo(rng, rng.uniform(15 / 16, 17 / 16, 300)),
+                               np.abs(datagen.spread(rng, 200, 40)),
+                               np.array([(2.0 ** k, 0.0) for k in range(-1074, 1024, 37)]),
+                               np.array(SPECIALS)]),
+        "sin": np.concatenate([datagen.sin_inputs(rng, n, near_frac=0.2),
+                               datagen.config0(rng, 200),
+                               np.array([(2 * np.pi * 2.0 ** k, 0.0) for k in range(0, 40)]),
+                               np.array(SPECIALS)]),
+        "erf": np.concatenate([datagen.erf_inputs(rng, n), datagen.config0(rng, 200),
+                               datagen._with_lo(rng, rng.uniform(-0.3, 0.3, 200)),
+                               datagen._with_lo(rng, 10.0 ** rng.uniform(-30, 0, 100)),
+                               datagen._with_lo(rng, rng.uniform(5.5, 10.0, 100)),
+                               np.array(SPECIALS)]),
+    }
+    fn = {"exp": mpmath.exp, "log": mpmath.log, "sin": mpmath.sin, "erf": mpmath.erf}
+    out = {}
+    for nm, x in cases.items():
+        out[nm + "_in"] = x
+        out[nm + "_ref"] = call1(L, nm + "dd", x)
+        exact = np.zeros((x.shape[0], 3))
+        for i in range(x.shape[0]):
+            if not np.isfinite(x[i, 0]):
+                exact[i] = (np.nan, 0, 0)
+                continue
+            v = mp_of(x[i])
+            if nm == "log" and v <= 0:
+                exact[i] = (np.nan, 0, 0)
+                continue
+            try:
+                exact[i] = limbs3(fn[nm](v))
+            except (OverflowError, ValueError):
+                exact[i] = (np.nan, 0, 0)
+        out[nm + "_exact"] = exact
+    np.savez_compressed(os.path.join(HERE, "transc.npz"), **out)
+
+
+def matmul(L, rng):
+    shapes = [(1, 1, 1), (7, 1, 9), (3, 9, 1), (2, 20, 5), (16, 16, 16), (4, 32, 6),
+              (8, 8, 8), (33, 17, 31), (65, 70, 66)]
+    for k in (1, 7, 8, 9, 16, 17, 64, 1000):
+        shapes.append((5, k, 3))
+    out = {}
+    idx = 0
+    for (m, k, n) in shapes:
+        for ri in (0, 8, 64):
+            for e in (None, 20):
+                A = datagen.matrix(rng, m, k, e)
+                B = datagen.matrix(rng, k, n, e)
+                assert datagen.is_colmajor(A) and datagen.is_colmajor(B)
+                C = datagen.as_colmajor(np.zeros((m, n, 2)))
+                L.matmuldd_mm(A.ctypes.data, B.ctypes.data, C.ctypes.data, m, k, n, ri)
+                x = datagen.config0(rng, k)
+                y = np.zeros((m, 2))
+                L.matmuldd_mv(A.ctypes.data, x.ctypes.data, y.ctypes.data, m, k, ri)
+                z = np.zeros((n, 2))
+                L.matmuldd_vm(x.ctypes.data, B.ctypes.data, z.ctypes.data, k, n, ri)
+                key = "c%03d" % idx
+                out[key + "_shape"] = np.array([m, k, n, ri])
+                out[key + "_A"] = datagen.flat(A)
+                out[key + "_B"] = datagen.flat(B)
+                out[key + "_C"] = datagen.flat(C)
+                out[key + "_x"] = x
+                out[key + "_mv"] = y
+                out[key + "_vm"] = z
+                idx += 1
+    out["count"] = np.array([idx])
+    np.savez_compressed(os.path.join(HERE, "matmul.npz"), **out)
+
+
+def main():
+    L = load_ref()
+    rng = np.random.default_rng(20261203)
+    arith(L, rng)
+    transc(L, rng)
+    matmul(L, rng)
+    for f in ("arith.npz", "transc.npz", "matmul.npz"):
+        print(f, os.path.getsize(os.path.join(HERE, f)))
+
+
+if __name__ == "__main__":
+    main()

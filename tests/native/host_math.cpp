@@ -1,0 +1,30 @@
+// Host (g++) build of the kernels' DD headers, for CPU tests only: checks the
+// device algorithms (multifloats_amd/csrc/dd_core.h, dd_math.h) against the
+// oracle and the 320-bit fixture values before they run on a GPU.
+#include "../../multifloats_amd/csrc/dd_math.h"
+
+#define H1(name, fn)                                                   \
+  extern "C" void name(const double *a, double *c, long n) {           \
+    for (long i = 0; i < n; ++i) {                                     \
+      ddv r = fn(mk(a[2 * i], a[2 * i + 1]));                          \
+      c[2 * i] = r.hi;                                                 \
+      c[2 * i + 1] = r.lo;                                             \
+    }                                                                  \
+  }
+#define H2(name, fn)                                                          \
+  extern "C" void name(const double *a, const double *b, double *c, long n) { \
+    for (long i = 0; i < n; ++i) {                                            \
+      ddv r = fn(mk(a[2 * i], a[2 * i + 1]), mk(b[2 * i], b[2 * i + 1]));     \
+      c[2 * i] = r.hi;                                                        \
+      c[2 * i + 1] = r.lo;                                                    \
+    }                                                                         \
+  }
+H1(h_exp, dd_exp)
+H1(h_log, dd_log)
+H1(h_sin, dd_sin)
+H1(h_erf, dd_erf)
+H1(h_sqrt, dd_sqrt)
+H2(h_add, dd_add)
+H2(h_sub, dd_sub)
+H2(h_mul, dd_mul)
+H2(h_div, dd_div)

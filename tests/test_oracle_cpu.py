@@ -1,0 +1,173 @@
+"""CPU tests pinning the oracle: golden vectors from the compiled reference,
+closed forms, invariants, brute force, and planted mistakes."""
+import ctypes
+from fractions import Fraction
+
+import mpmath
+import numpy as np
+import pytest
+
+from multifloats_amd import datagen
+from tests import _native as nt
+
+
+def test_oracle_arith_matches_reference_bits():
+    g = nt.load_golden("arith.npz")
+    for nm, ref in (("add", "adddd"), ("sub", "subdd"), ("mul", "muldd"), ("div", "divdd")):
+        nt.assert_bits(nt.or_binary(nm, g["a"], g["b"]), g[ref], nm)
+    nt.assert_bits(nt.or_unary("sqrt", g["sqrt_in"]), g["sqrtdd"], "sqrt")
+
+
+@pytest.mark.parametrize("fn", ["exp", "log", "sin"])
+def test_oracle_transcendentals_match_reference_bits(fn):
+    g = nt.load_golden("transc.npz")
+    nt.assert_bits(nt.or_unary(fn, g[fn + "_in"]), g[fn + "_ref"], fn)
+
+
+def test_oracle_matmul_matches_reference_bits():
+    g = nt.load_golden("matmul.npz")
+    O = nt.oracle()
+    for i in range(int(g["count"][0])):
+        key = "c%03d" % i
+        m, k, n, ri = (int(v) for v in g[key + "_shape"])
+        A = datagen.unflat(g[key + "_A"], m, k)
+        B = datagen.unflat(g[key + "_B"], k, n)
+        C = nt.or_matmul(A, B, ri)
+        nt.assert_bits(datagen.flat(C), g[key + "_C"], "mm %s" % key)
+        x = nt.dd(g[key + "_x"])
+        y = np.zeros((m, 2))
+        O.or_matmul_mv(nt.mptr(nt.cm(A)), nt.ptr(x), nt.ptr(y), m, k, ri)
+        nt.assert_bits(y, g[key + "_mv"], "mv %s" % key)
+        z = np.zeros((n, 2))
+        O.or_matmul_vm(nt.ptr(x), nt.mptr(nt.cm(B)), nt.ptr(z), k, n, ri)
+        nt.assert_bits(z, g[key + "_vm"], "vm %s" % key)
+
+
+def test_tables_are_the_closed_forms():
+    """Every DD table entry equals its closed form to < 2^-105 relative."""
+    import importlib.util, os, re
+    mpmath.mp.prec = 300
+    path = os.path.join(nt.ROOT, "multifloats_amd", "csrc", "dd_tables.h")
+    text = open(path).read()
+
+    def arr(name):
+        m = re.search(r"double %s\[\d+\] = \{(.*?)\};" % name, text, re.S)
+        return [float.fromhex(t.strip()) if "0x" in t else float(t) for t in m.group(1).split(",") if t.strip()]
+
+    ln2 = mpmath.log(2)
+    checks = {
+        "mf_exp2c": [ln2 ** j / mpmath.factorial(j) for j in range(10)],
+        "mf_exp2t": [mpmath.mpf(2) ** (mpmath.mpf(k - 128) / 256) for k in range(257)],
+        "mf_log2c": [2 / ((2 * k + 1) * ln2) for k in range(11)],
+        "mf_log2v": [mpmath.log(1 + mpmath.mpf(2 * i + 1) / 64) / ln2 for i in range(32)],
+        "mf_sinc": [(-1) ** k / mpmath.factorial(2 * k + 1) for k in range(13)],
+        "mf_cosc": [(-1) ** k / mpmath.factorial(2 * k) for k in range(13)],
+    }
+    for name, vals in checks.items():
+        hi, lo = arr(name + "_hi"), arr(name + "_lo")
+        assert len(hi) == len(vals)
+        for h, l, v in zip(hi, lo, vals):
+            if v == 0:
+                continue
+            assert abs((mpmath.mpf(h) + mpmath.mpf(l) - v) / v) < mpmath.mpf(2) ** -105, name
+            assert abs(l) <= abs(h) * 2.0 ** -52, name
+
+
+def _frac(x):
+    return Fraction(float(x[0])) + Fraction(float(x[1]))
+
+
+def test_exact_accumulator_brute_force():
+    rng = np.random.default_rng(1)
+    for trial in range(20):
+        k = int(rng.integers(1, 40))
+        e = [None, 10, 60][trial % 3]
+        a = datagen.spread(rng, k, e) if e else datagen.config0(rng, k)
+        b = datagen.spread(rng, k, e) if e else datagen.config0(rng, k)
+        exact = sum((_frac(a[i]) * _frac(b[i]) for i in range(k)), Fraction(0))
+        c = nt.exact_dot(a, b)
+        assert abs(_frac(c) - exact) <= abs(exact) * Fraction(2) ** -100 + Fraction(2) ** -1000
+        # error of a deliberately perturbed value is measured exactly
+        cp = c.copy()
+        cp[1] += abs(c[0]) * 2.0 ** -90
+        err, mag = nt.exact_dot_err(a, b, cp)
+        true = float(abs(_frac(cp) - exact))
+        assert abs(err - true) <= true * 1e-10 + 1e-300
+
+
+def test_exact_sum_negative_and_cancelling():
+    x = np.array([[1.0, 1e-20], [-1.0, 0.0], [1e-30, 0.0], [-3.5, 1e-17]])
+    exact = sum((_frac(v) for v in x), Fraction(0))
+    c = np.array([float(exact), 0.0])
+    c[1] = float(exact - Fraction(c[0]))
+    err, mag = nt.exact_sum_err(x, c)
+    assert err <= abs(float(exact)) * 2.0 ** -100
+
+
+def test_reductions_invariants():
+    O = nt.oracle()
+    # integer sums are exact
+    x = np.stack([np.arange(1, 1001, dtype=np.float64), np.zeros(1000)], axis=-1)
+    r = nt.res(O.or_sum(nt.ptr(x), 1000))
+    assert r[0] == 500500.0 and r[1] == 0.0
+    # norm2 of a Pythagorean pair
+    for k in (-500, 0, 100):
+        v = np.array([[3.0 * 2.0 ** k, 0.0], [4.0 * 2.0 ** k, 0.0]])
+        r = nt.res(O.or_norm2(nt.ptr(v), 2))
+        assert r[0] == 5.0 * 2.0 ** k and r[1] == 0.0
+    # empty inputs
+    e = np.zeros((1, 2))
+    assert nt.res(O.or_sum(nt.ptr(e), 0)).tolist() == [0.0, 0.0]
+    assert nt.res(O.or_dot(nt.ptr(e), nt.ptr(e), 0, 8)).tolist() == [0.0, 0.0]
+
+
+@pytest.mark.parametrize("ri", [0, 1, 8, 64])
+def test_oracle_dot_sum_within_bound(ri):
+    rng = np.random.default_rng(7 + ri)
+    n = 20000
+    a, b = datagen.cancel_pairs(rng, n)
+    O = nt.oracle()
+    r = nt.res(O.or_dot(nt.ptr(nt.dd(a)), nt.ptr(nt.dd(b)), n, ri))
+    err, mag = nt.exact_dot_err(a, b, r)
+    assert err <= 4 * n * 2.0 ** -104 * mag
+    x = datagen.sum_inputs(rng, n)
+    s = nt.res(O.or_sum(nt.ptr(nt.dd(x)), n))
+    err, mag = nt.exact_sum_err(x, s)
+    assert err <= 4 * n * 2.0 ** -104 * mag
+
+
+def test_planted_mistake_is_caught():
+    """A dot product that drops the cross terms (a plausible slip) must fail
+    the bound the GPU reductions are held to."""
+    rng = np.random.default_rng(3)
+    n = 5000
+    a, b = datagen.cancel_pairs(rng, n)
+    sh = sl = 0.0
+    for i in range(n):  # the oracle's update with the cross term dropped
+        p = a[i, 0] * b[i, 0]
+        e = float(Fraction(a[i, 0]) * Fraction(b[i, 0]) - Fraction(p))
+        t = sh + p
+        bp = t - sh
+        ap = t - bp
+        sl = sl + ((e + (sh - ap)) + (p - bp))
+        sh = t
+    bad = np.array([sh + sl, sl - ((sh + sl) - sh)])
+    err, mag = nt.exact_dot_err(a, b, bad)
+    assert err > 4 * n * 2.0 ** -104 * mag
+
+
+def test_oracle_gemm_closed_forms():
+    rng = np.random.default_rng(5)
+    m = 12
+    Id = np.zeros((m, m, 2))
+    for i in range(m):
+        Id[i, i, 0] = 1.0
+    B = datagen.matrix(rng, m, 9)
+    C = nt.or_matmul(Id, B, 8)
+    nt.assert_bits(C, B, "I*B")
+    Ai = np.zeros((6, 5, 2))
+    Bi = np.zeros((5, 4, 2))
+    Ai[..., 0] = rng.integers(-9, 10, (6, 5))
+    Bi[..., 0] = rng.integers(-9, 10, (5, 4))
+    C = nt.or_matmul(Ai, Bi, 0)
+    assert np.array_equal(C[..., 0], Ai[..., 0] @ Bi[..., 0]) and not C[..., 1].any()
