@@ -32,4 +32,10 @@ size_t reduce_workspace_bytes(int64_t n);
 hipError_t reduce(int kind, const double2 *a, const double2 *b, int64_t n, int64_t ri,
                   double2 *out, double2 *work, hipStream_t s);
 
+// Ozaki split-precision GEMM with S int8 slices per operand (8 <= S <= 16).
+int ozaki_max_slices();
+size_t ozaki_workspace_bytes(int64_t m, int64_t k, int64_t n, int S);
+hipError_t ozaki_gemm(const double2 *A, const double2 *B, double2 *C, int64_t m, int64_t k,
+                      int64_t n, int S, void *work, size_t work_bytes, hipStream_t s);
+
 }  // namespace mfhip
