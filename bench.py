@@ -2,9 +2,15 @@
 """Benchmark of the DD hot path on one MI355X.
 
 Default (the headline): float64x2 GEMM C = A*B at m = n = k = 4096, entries
-s*2^U(-20,20)*u with lo per the config-0 rule, renorm interval 8, inputs and
-output resident in HBM. A step is one GEMM launch over the whole matrix.
-Prints ONE JSON line with value = DD-GFLOP/s (2*m*n*k per step).
+s*2^U(-20,20)*u with lo per the config-0 rule, inputs and output resident in
+HBM, in two forms on the same inputs:
+  ozaki  the INT8-MFMA split-precision GEMM (16 slices), whole call per step
+         (exponent scan, slicing, products, recombination);
+  plain  the reference-order FP64 GEMM (bit-identical to the reference),
+         renorm interval 8.
+Prints ONE JSON line: value = Ozaki DD-GFLOP/s (2*m*n*k per step) when its
+measured error on a sampled block is within 2^-100 (|A||B|) elementwise, with
+the plain form, the error figures and the speed-up in the same line.
 
   python bench.py [--steps K] [--warmup W] [--workload gemm|elementwise|reduce]
                   [--no-cpu] [--variant V]
@@ -31,6 +37,7 @@ sys.path.insert(0, ROOT)
 FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 vector (and matrix) peak
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
 FP64_FLOPS_PER_DDMAC = 15.75  # 2 MUL + 2 FMA(x2) + 9 ADD + 6/8 renorm at ri = 8
+INT8_PEAK_TOPS = 5000.0      # MI355X dense INT8 MFMA
 
 
 def parse():
@@ -40,6 +47,10 @@ def parse():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--workload", default="gemm", choices=["gemm", "elementwise", "reduce"])
     p.add_argument("--size", type=int, default=4096)
+    p.add_argument("--spread", type=float, default=20.0)
+    p.add_argument("--slices", type=int, default=16)
+    p.add_argument("--form", default="both", choices=["both", "plain", "ozaki"])
+    p.add_argument("--err-sample", type=int, default=32)
     p.add_argument("--variant", type=int, default=-1)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-cols", type=int, default=48)
@@ -86,40 +97,98 @@ def cpu_gemm_baseline(A, B, C_gpu, cols, ri):
                 % (cols, m, k, cols, dt, same)), same
 
 
+def sampled_error(A, B, C, rows, cols):
+    from tests import _native as nt  # exact accumulator (test infrastructure)
+    rr, cc = np.meshgrid(rows, cols, indexing="ij")
+    err, mag = nt.gemm_err(A, B, C, rr.ravel(), cc.ravel())
+    rel = err / mag
+    return float(rel.max()), float(rel.mean())
+
+
 def bench_gemm(args, torch, mf):
     from multifloats_amd import datagen
     m = n = k = args.size
     ri = 8
+    S = args.slices
     rng = np.random.default_rng(2026)
-    A = datagen.matrix(rng, m, k, 20)
-    B = datagen.matrix(rng, k, n, 20)
+    A = datagen.matrix(rng, m, k, args.spread)
+    B = datagen.matrix(rng, k, n, args.spread)
     dA = torch.from_numpy(datagen.flat(A)).cuda()
     dB = torch.from_numpy(datagen.flat(B)).cuda()
     dC = torch.empty((m * n, 2), dtype=torch.float64, device="cuda")
     stream = torch.cuda.current_stream().cuda_stream
-
-    def step():
-        mf.dev_matmul(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), m, k, n, ri, stream, args.variant)
-
-    ms = timed(torch, step, args.steps, args.warmup)
     flops = 2.0 * m * n * k
-    value = flops / (ms * 1e-3) / 1e9
-    achieved = FP64_FLOPS_PER_DDMAC * m * n * k / (ms * 1e-3) / 1e12
-    out = {
-        "metric": "DD-GEMM DD-GFLOP/s at 4096^3 (plain reference-order float64x2 GEMM)",
-        "value": round(value, 2), "unit": "DD-GFLOP/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "float64x2 GEMM m=n=k=%d, exponent spread 2^+-20, renorm interval 8, "
-                               "plain form (bit-identical to the reference)" % m,
-                   "m": m, "n": n, "k": k, "renorm_interval": ri, "variant": args.variant},
-        "roofline": {"bound": "fp64", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
-                     "note": "FP64 VALU bound: %.2f FP64 flops per DD-MAC (FMA = 2)" % FP64_FLOPS_PER_DDMAC},
-    }
-    if not args.no_cpu:
+    es = min(args.err_sample, m)
+    rows = rng.choice(m, es, replace=False)
+    cols = rng.choice(n, es, replace=False)
+    res = {}
+
+    if args.form in ("both", "plain"):
+        def step_plain():
+            mf.dev_matmul(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), m, k, n, ri, stream, args.variant)
+        ms = timed(torch, step_plain, args.steps, args.warmup)
         C = datagen.unflat(dC.cpu().numpy(), m, n)
-        out["cpu_baseline"], same = cpu_gemm_baseline(A, B, C, args.cpu_cols, ri)
+        emax, emean = sampled_error(A, B, C, rows, cols)
+        achieved = FP64_FLOPS_PER_DDMAC * m * n * k / (ms * 1e-3) / 1e12
+        res["plain"] = dict(ms=ms, value=flops / (ms * 1e-3) / 1e9, err_max=emax, err_mean=emean,
+                            roofline={"bound": "fp64", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
+                                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                                      "traffic": None,
+                                      "note": "k_gemm_plain, FP64 VALU: %.2f FP64 flops per DD-MAC (FMA = 2)"
+                                              % FP64_FLOPS_PER_DDMAC})
+        if not args.no_cpu:
+            res["cpu"], _ = cpu_gemm_baseline(A, B, C, args.cpu_cols, ri)
+        del C
+
+    if args.form in ("both", "ozaki"):
+        wb = mf.ozaki_workspace_bytes(m, k, n, S)
+        work = torch.empty(wb, dtype=torch.uint8, device="cuda")
+
+        def step_oz():
+            mf.dev_matmul_ozaki(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), m, k, n, S, work.data_ptr(), wb, stream)
+
+        def step_mult():
+            mf.dev_ozaki_multiply(dC.data_ptr(), m, k, n, S, work.data_ptr(), wb, stream)
+        ms = timed(torch, step_oz, args.steps, args.warmup)
+        ms_mult = timed(torch, step_mult, args.steps, 1)
+        C = datagen.unflat(dC.cpu().numpy(), m, n)
+        emax, emean = sampled_error(A, B, C, rows, cols)
+        P = S * (S + 1) // 2
+        tops = P * 2.0 * m * n * k / (ms_mult * 1e-3) / 1e12
+        res["ozaki"] = dict(ms=ms, ms_multiply=ms_mult, value=flops / (ms * 1e-3) / 1e9, err_max=emax,
+                            err_mean=emean, slices=S, slice_pairs=P,
+                            roofline={"bound": "mfma", "achieved": round(tops, 2), "peak": INT8_PEAK_TOPS,
+                                      "unit": "TOP/s", "frac": round(tops / INT8_PEAK_TOPS, 4), "traffic": None,
+                                      "note": "k_ozaki INT8 MFMA: %d slice pairs x 2mnk int8 ops per launch" % P})
+        del C
+
+    head = res.get("ozaki") if args.form != "plain" else res["plain"]
+    ok = head["err_max"] <= 2.0 ** -100 if args.form != "plain" else True
+    out = {
+        "metric": "DD-GEMM DD-GFLOP/s at 4096^3: Ozaki-INT8 vs plain FP64, err<=2^-100*|A||B|",
+        "value": round(head["value"], 2) if ok else None, "unit": "DD-GFLOP/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(head["ms"], 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int8-mfma+f64" if args.form != "plain" else "f64", "data": "synthetic",
+        "config": {"workload": "float64x2 GEMM m=n=k=%d, entries s*2^U(-%g,%g)*u, lo per config-0; %s"
+                               % (m, args.spread, args.spread,
+                                  "Ozaki INT8 (%d slices) and plain reference-order (renorm 8) forms" % S
+                                  if args.form == "both" else args.form),
+                   "m": m, "n": n, "k": k, "slices": S, "renorm_interval": ri,
+                   "error_sample": "%dx%d outputs vs exact accumulator" % (es, es)},
+        "roofline": head["roofline"],
+    }
+    for form in ("ozaki", "plain"):
+        if form in res:
+            r = res[form]
+            out[form] = {kk: (round(v, 4) if isinstance(v, float) and kk.startswith("ms") else
+                              round(v, 2) if kk == "value" else v)
+                         for kk, v in r.items() if kk != "roofline"}
+            out[form]["roofline"] = r["roofline"]
+    if "ozaki" in res and "plain" in res:
+        out["speedup_ozaki_vs_plain"] = round(res["plain"]["ms"] / res["ozaki"]["ms"], 3)
+    if "cpu" in res:
+        out["cpu_baseline"] = res["cpu"]
     return out
 
 

@@ -121,6 +121,18 @@ MULTIFLOATS_API float64x2 dotdd(const float64x2 *a, const float64x2 *b, int64_t 
 /* Fortran norm2(a) (fypp:2338-2352); like the reference, no overflow scaling. */
 MULTIFLOATS_API float64x2 norm2dd(const float64x2 *a, int64_t n);
 
+/* Precision form of matmuldd_mm (no counterpart in the reference): the Ozaki
+ * split-precision GEMM on the INT8 matrix cores. A and B are cut into
+ * `slices` int8 digit slices (7 bits each) per row / column scale; slice
+ * pairs whose scales sum to fewer than `slices` levels are multiplied with
+ * exact INT32 accumulation and recombined in DD. Not bit-identical to
+ * matmuldd_mm: held to |C - AB| <= 2^-100 (|A||B|) elementwise at the default
+ * 16 slices for inputs whose exponents span up to 2^+-30 per row/column.
+ * slices <= 0 selects 16; valid range 8..16; k <= 32768. Rows of A or columns
+ * of B holding Inf/NaN give NaN outputs. */
+MULTIFLOATS_API void matmuldd_mm_ozaki(const float64x2 *a, const float64x2 *b, float64x2 *c,
+                                       int64_t m, int64_t k, int64_t n, int slices);
+
 /* ---- 3. device-pointer API ------------------------------------------- */
 /* All pointers below are device pointers (16-byte aligned) and `stream` is a
  * hipStream_t (NULL = the default stream). Returns 0 on success, otherwise a
@@ -156,6 +168,22 @@ MULTIFLOATS_API size_t mfhip_reduce_workspace_bytes(int64_t n);
 MULTIFLOATS_API int mfhip_reduce(int kind, const float64x2 *a, const float64x2 *b, int64_t n,
                                  int64_t renorm_interval, float64x2 *out, void *work,
                                  void *stream);
+
+/* Ozaki GEMM on device pointers; `work` is caller-owned device scratch of at
+ * least mfhip_ozaki_workspace_bytes(m, k, n, slices) bytes. */
+MULTIFLOATS_API size_t mfhip_ozaki_workspace_bytes(int64_t m, int64_t k, int64_t n, int slices);
+MULTIFLOATS_API int mfhip_ozaki_matmul_mm(const float64x2 *a, const float64x2 *b, float64x2 *c,
+                                          int64_t m, int64_t k, int64_t n, int slices, void *work,
+                                          size_t work_bytes, void *stream);
+/* mfhip_ozaki_matmul_mm in two steps: prepare scales and slices A and B into
+ * `work`; multiply forms C from `work` (same m, k, n, slices, work). A
+ * prepared `work` may be multiplied again, e.g. to time the product alone. */
+MULTIFLOATS_API int mfhip_ozaki_prepare(const float64x2 *a, const float64x2 *b, int64_t m,
+                                        int64_t k, int64_t n, int slices, void *work,
+                                        size_t work_bytes, void *stream);
+MULTIFLOATS_API int mfhip_ozaki_multiply(float64x2 *c, int64_t m, int64_t k, int64_t n,
+                                         int slices, void *work, size_t work_bytes,
+                                         void *stream);
 
 /* Last HIP error recorded on the calling thread by any entry point (0 = none). */
 MULTIFLOATS_API int mfhip_last_error(void);

@@ -79,6 +79,13 @@ def lib():
         L.mfhip_reduce_workspace_bytes.argtypes = [I64]
         L.mfhip_reduce_workspace_bytes.restype = ctypes.c_size_t
         L.mfhip_reduce.argtypes = [I, P, P, I64, I64, P, P, VP]
+        L.matmuldd_mm_ozaki.argtypes = [P, P, P, I64, I64, I64, I]
+        L.matmuldd_mm_ozaki.restype = None
+        L.mfhip_ozaki_workspace_bytes.argtypes = [I64, I64, I64, I]
+        L.mfhip_ozaki_workspace_bytes.restype = ctypes.c_size_t
+        L.mfhip_ozaki_matmul_mm.argtypes = [P, P, P, I64, I64, I64, I, P, ctypes.c_size_t, VP]
+        L.mfhip_ozaki_prepare.argtypes = [P, P, I64, I64, I64, I, P, ctypes.c_size_t, VP]
+        L.mfhip_ozaki_multiply.argtypes = [P, I64, I64, I64, I, P, ctypes.c_size_t, VP]
         L.mfhip_last_error.argtypes = []
         L.mfhip_error_string.argtypes = [I]
         L.mfhip_error_string.restype = ctypes.c_char_p
@@ -141,6 +148,19 @@ def matmul(a, b, renorm_interval=DD_FMA_RENORM_INTERVAL):
         lib().matmuldd_vm(_ptr(a), _ptr(b), _ptr(c), k, n, renorm_interval)
     else:
         raise ValueError("matmul: need (matrix, matrix), (matrix, vector) or (vector, matrix)")
+    _check_status()
+    return c
+
+
+def matmul_ozaki(a, b, slices=16):
+    """Ozaki split-precision DD GEMM on INT8 matrix cores (matmuldd_mm_ozaki)."""
+    a, b = _mat(a), _mat(b)
+    m, k = a.shape[:2]
+    if b.shape[0] != k:
+        raise ValueError("matmul_ozaki: inner dimensions differ")
+    n = b.shape[1]
+    c = _mat(np.zeros((m, n, 2)))
+    lib().matmuldd_mm_ozaki(_ptr(a), _ptr(b), _ptr(c), m, k, n, slices)
     _check_status()
     return c
 
@@ -210,6 +230,22 @@ def _dev(rc):
 
 def dev_matmul(a, b, c, m, k, n, renorm_interval=DD_FMA_RENORM_INTERVAL, stream=0, variant=-1):
     _dev(lib().mfhip_matmul_mm_variant(a, b, c, m, k, n, renorm_interval, variant, stream))
+
+
+def ozaki_workspace_bytes(m, k, n, slices=16):
+    return lib().mfhip_ozaki_workspace_bytes(m, k, n, slices)
+
+
+def dev_matmul_ozaki(a, b, c, m, k, n, slices, work, work_bytes, stream=0):
+    _dev(lib().mfhip_ozaki_matmul_mm(a, b, c, m, k, n, slices, work, work_bytes, stream))
+
+
+def dev_ozaki_prepare(a, b, m, k, n, slices, work, work_bytes, stream=0):
+    _dev(lib().mfhip_ozaki_prepare(a, b, m, k, n, slices, work, work_bytes, stream))
+
+
+def dev_ozaki_multiply(c, m, k, n, slices, work, work_bytes, stream=0):
+    _dev(lib().mfhip_ozaki_multiply(c, m, k, n, slices, work, work_bytes, stream))
 
 
 def dev_unary(op, a, c, n, stream=0):

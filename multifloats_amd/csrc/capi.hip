@@ -182,6 +182,29 @@ hipError_t host_reduce(int kind, const float64x2 *a, const float64x2 *b, int64_t
   return hipStreamSynchronize(tl_stage.stream);
 }
 
+int oz_slices(int s) { return s <= 0 ? 16 : s; }
+
+hipError_t host_mm_ozaki(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t m,
+                         int64_t k, int64_t n, int slices) {
+  tl_err = 0;
+  if (m <= 0 || n <= 0) return hipSuccess;
+  slices = oz_slices(slices);
+  if (slices < 8 || slices > mfhip::ozaki_max_slices()) return hipErrorInvalidValue;
+  TRY(stage_init());
+  const size_t kk = k > 0 ? (size_t)k : 0;
+  void *da, *db, *dc, *dw;
+  const size_t wb = mfhip::ozaki_workspace_bytes(m, k, n, slices);
+  TRY(stage_buf(0, 16 * (size_t)m * kk, &da));
+  TRY(stage_buf(1, 16 * kk * (size_t)n, &db));
+  TRY(stage_buf(2, 16 * (size_t)m * n, &dc));
+  TRY(stage_buf(3, wb, &dw));
+  TRY(h2d(da, a, 16 * (size_t)m * kk));
+  TRY(h2d(db, b, 16 * kk * (size_t)n));
+  TRY(mfhip::ozaki_gemm(D(da), D(db), W(dc), m, k, n, slices, dw, wb, tl_stage.stream));
+  TRY(d2h(c, dc, 16 * (size_t)m * n));
+  return hipStreamSynchronize(tl_stage.stream);
+}
+
 float64x2 nan_dd() {
   float64x2 r;
   r.limbs[0] = r.limbs[1] = NAN;
@@ -215,6 +238,14 @@ void matmuldd_vm(const float64x2 *x, const float64x2 *b, float64x2 *y, int64_t k
   if (e != hipSuccess) {
     report("matmuldd_vm", e);
     nan_fill(y, n > 0 ? n : 0);
+  }
+}
+void matmuldd_mm_ozaki(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t m,
+                       int64_t k, int64_t n, int slices) {
+  hipError_t e = host_mm_ozaki(a, b, c, m, k, n, slices);
+  if (e != hipSuccess) {
+    report("matmuldd_mm_ozaki", e);
+    nan_fill(c, (m > 0 && n > 0) ? m * n : 0);
   }
 }
 void matmulf64x2_mm(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t m, int64_t k,
@@ -335,6 +366,30 @@ int mfhip_reduce(int kind, const float64x2 *a, const float64x2 *b, int64_t n, in
   CHECK_ALIGN(a, b, out, work);
   if (!work || !out) return status(hipErrorInvalidValue);
   return status(mfhip::reduce(kind, D(a), D(b), n, ri, W(out), W(work), (hipStream_t)stream));
+}
+
+size_t mfhip_ozaki_workspace_bytes(int64_t m, int64_t k, int64_t n, int slices) {
+  return mfhip::ozaki_workspace_bytes(m, k, n, oz_slices(slices));
+}
+int mfhip_ozaki_matmul_mm(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t m,
+                          int64_t k, int64_t n, int slices, void *work, size_t work_bytes,
+                          void *stream) {
+  CHECK_ALIGN(a, b, c, work);
+  return status(mfhip::ozaki_gemm(D(a), D(b), W(c), m, k, n, oz_slices(slices), work, work_bytes,
+                                  (hipStream_t)stream));
+}
+
+int mfhip_ozaki_prepare(const float64x2 *a, const float64x2 *b, int64_t m, int64_t k, int64_t n,
+                        int slices, void *work, size_t work_bytes, void *stream) {
+  CHECK_ALIGN(a, b, work);
+  return status(mfhip::ozaki_prepare(D(a), D(b), m, k, n, oz_slices(slices), work, work_bytes,
+                                     (hipStream_t)stream));
+}
+int mfhip_ozaki_multiply(float64x2 *c, int64_t m, int64_t k, int64_t n, int slices, void *work,
+                         size_t work_bytes, void *stream) {
+  CHECK_ALIGN(c, work);
+  return status(mfhip::ozaki_multiply(W(c), m, k, n, oz_slices(slices), work, work_bytes,
+                                      (hipStream_t)stream));
 }
 
 int mfhip_last_error(void) { return tl_err; }

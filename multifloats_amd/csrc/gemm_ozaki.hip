@@ -21,13 +21,15 @@
 //    additions (two_sum), then scaled by 2^(E_i + F_j - 10).
 //
 // Kernel shape. A workgroup owns a 64x64 tile of C and all S level
-// accumulators of it: 4 waves, each holding up to 4 levels of the whole tile
-// (4 x 4 x 16 = 256 accumulator registers per lane), levels assigned to waves
-// by longest-processing-time so each wave issues the same number of MFMAs.
-// Per 32-deep K block the S slices of both operand tiles (S x 4 KiB) are
-// staged once in LDS (48-byte row pitch: conflict-free 16-byte fragment
-// reads) and every slice pair of a wave's levels is multiplied out of LDS, so
-// HBM/L2 see each slice tile once per output tile rather than once per pair.
+// accumulators of it: 8 waves (2 per SIMD), each holding 2 levels of the
+// whole tile (2 x 4 x 16 = 128 accumulator registers per lane), levels
+// assigned to waves by longest-processing-time so each wave issues the same
+// number of MFMAs. Per 32-deep K block the S slices of both operand tiles
+// (S x 4 KiB) are copied once into LDS by asynchronous 16-byte direct-to-LDS
+// loads (double-buffered, no register staging), already in a swizzled layout
+// that makes the 16-byte fragment reads bank-conflict free; every slice pair
+// of a wave's levels is then multiplied out of LDS, so HBM/L2 see each slice
+// tile once per output tile rather than once per pair.
 #include "dd_core.h"
 #include "mf_internal.h"
 
@@ -39,9 +41,11 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 constexpr int kT = 64;        // output tile edge (rows of A / cols of B per block)
 constexpr int kK = 32;        // K block (one MFMA K step)
 constexpr int kChunk = kT * kK;  // bytes of one slice tile (64 x 32)
-constexpr int kPitch = 48;    // LDS bytes per tile row
-constexpr int kLdsSlice = kT * kPitch;
-constexpr int kThreads = 256;
+constexpr int kLdsSlice = kChunk;  // LDS image of a slice tile = its global image
+constexpr int kThreads = 256;  // pre-pass kernels
+constexpr int kWaves = 8;       // main kernel: 8 waves, 2 per SIMD
+constexpr int kLPW = 2;         // levels per wave
+constexpr int kMainThreads = 64 * kWaves;
 constexpr int kEmpty = -2147483647 - 1;  // exponent of an all-zero row/column
 constexpr int kNonFinite = 2147483647;   // a row/column holding Inf or NaN
 
@@ -115,8 +119,17 @@ __device__ __forceinline__ void digits(double h, double l, int S, signed char *d
 
 __device__ __forceinline__ int scale_of(int e) { return (e == kEmpty || e == kNonFinite) ? 0 : e; }
 
+// Byte offset of element (r, c) in a 64 x 32 slice tile. Row r is 32 bytes;
+// its two 16-byte halves are swapped when bit 2 xor bit 3 of r is set, so the
+// 16-byte fragment reads of 8 or 16 consecutive lanes (rows r..r+7 or
+// r..r+15, same half) land on distinct LDS banks.
+__device__ __forceinline__ int swz(int r) { return ((r >> 2) ^ (r >> 3)) & 1; }
+__device__ __forceinline__ int tile_off(int r, int c) {
+  return r * kK + ((((c >> 4) ^ swz(r))) << 4) + (c & 15);
+}
+
 // Slice a 64-row x 32-k tile of A. Element (r, c) of the tile is A(i0+r, p0+c);
-// tile bytes of slice s go to out[((ib*KB + kb)*S + s)*2048 + r*32 + c].
+// its digit s goes to out[((ib*KB + kb)*S + s)*2048 + tile_off(r, c)].
 template <int MAXS>
 __global__ __launch_bounds__(kThreads) void k_slice_a(const double2 *__restrict__ A, int64_t m,
                                                       int64_t k, const int *__restrict__ E,
@@ -147,7 +160,7 @@ __global__ __launch_bounds__(kThreads) void k_slice_a(const double2 *__restrict_
     digits<MAXS>(h, l, S, d);
 #pragma unroll
     for (int s = 0; s < MAXS; ++s)
-      if (s < S) tile[s * kChunk + r * kK + c] = d[s];
+      if (s < S) tile[s * kChunk + tile_off(r, c)] = d[s];
   }
   __syncthreads();
   const int nvec = S * kChunk / 16;
@@ -156,7 +169,7 @@ __global__ __launch_bounds__(kThreads) void k_slice_a(const double2 *__restrict_
   for (int v = tid; v < nvec; v += kThreads) dst[v] = src[v];
 }
 
-// Slice a 32-k x 64-col tile of B; bytes of slice t at [c*32 + p] (k fastest).
+// Slice a 32-k x 64-col tile of B; digit t of B(p0+p, j0+c) at tile_off(c, p).
 template <int MAXS>
 __global__ __launch_bounds__(kThreads) void k_slice_b(const double2 *__restrict__ B, int64_t k,
                                                       int64_t n, const int *__restrict__ F,
@@ -187,7 +200,7 @@ __global__ __launch_bounds__(kThreads) void k_slice_b(const double2 *__restrict_
     digits<MAXS>(h, l, S, d);
 #pragma unroll
     for (int s = 0; s < MAXS; ++s)
-      if (s < S) tile[s * kChunk + c * kK + p] = d[s];
+      if (s < S) tile[s * kChunk + tile_off(c, p)] = d[s];
   }
   __syncthreads();
   const int nvec = S * kChunk / 16;
@@ -197,19 +210,20 @@ __global__ __launch_bounds__(kThreads) void k_slice_b(const double2 *__restrict_
 }
 
 // Level -> wave assignment: longest processing time first (level L has L+1
-// slice pairs), at most 4 levels per wave.
+// slice pairs), at most kLPW levels per wave. For S = 16 this pairs levels
+// (15,0), (14,1), ..., (8,7): 17 slice pairs on every wave.
 template <int S>
 struct Plan {
-  int lv[4][4];
-  int cnt[4];
+  int lv[kWaves][kLPW];
+  int cnt[kWaves];
   constexpr Plan() : lv{}, cnt{} {
-    int load[4] = {0, 0, 0, 0};
-    for (int w = 0; w < 4; ++w)
-      for (int j = 0; j < 4; ++j) lv[w][j] = -1;
+    int load[kWaves] = {};
+    for (int w = 0; w < kWaves; ++w)
+      for (int j = 0; j < kLPW; ++j) lv[w][j] = -1;
     for (int L = S - 1; L >= 0; --L) {
       int best = -1;
-      for (int w = 0; w < 4; ++w)
-        if (cnt[w] < 4 && (best < 0 || load[w] < load[best])) best = w;
+      for (int w = 0; w < kWaves; ++w)
+        if (cnt[w] < kLPW && (best < 0 || load[w] < load[best])) best = w;
       lv[best][cnt[best]++] = L;
       load[best] += L + 1;
     }
@@ -217,19 +231,19 @@ struct Plan {
 };
 
 __device__ __forceinline__ v4i lds_frag(const signed char *base, int row, int half) {
-  return *(const v4i *)(base + row * kPitch + half * 16);
+  return *(const v4i *)(base + row * kK + ((half ^ swz(row)) << 4));
 }
 
 template <int S, int W>
 __device__ __forceinline__ void compute_block(const signed char *sa, const signed char *sb,
-                                              v16i (&acc)[4][2][2], int lane) {
+                                              v16i (&acc)[kLPW][2][2], int lane) {
   constexpr Plan<S> P{};
   const int r = lane & 31, half = lane >> 5;
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     bool any = false;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < kLPW; ++j) {
       const int L = P.lv[W][j];
       any = any || (L >= s && L - s < S);
     }
@@ -237,7 +251,7 @@ __device__ __forceinline__ void compute_block(const signed char *sa, const signe
     const v4i a0 = lds_frag(sa + s * kLdsSlice, r, half);
     const v4i a1 = lds_frag(sa + s * kLdsSlice, r + 32, half);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < kLPW; ++j) {
       const int L = P.lv[W][j];
       const int t = L - s;
       if (L >= 0 && t >= 0 && t < S) {
@@ -252,132 +266,139 @@ __device__ __forceinline__ void compute_block(const signed char *sa, const signe
   }
 }
 
-// Sum of this wave's levels for one accumulator slot position, in DD,
-// smallest-scale level first.
-template <int S, int W>
-__device__ __forceinline__ ddv wave_partial(const v16i (&acc)[4][2][2], int bi, int bj, int r) {
-  constexpr Plan<S> P{};
-  ddv sum = mk(0.0, 0.0);
+// Dump one wave's raw level accumulators to LDS in register order:
+// X[slot][bi][bj][lane][r], 32 KiB per wave. Every index is static.
+__device__ __forceinline__ void dump_acc(const v16i (&acc)[kLPW][2][2], int *X, int lane) {
 #pragma unroll
-  for (int j = 3; j >= 0; --j) {
-    const int L = P.lv[W][j];
-    if (L >= 0) {
-      double v = __builtin_ldexp((double)acc[j][bi][bj][r], -7 * L);
-      sum = dd_add(sum, mk(v, 0.0));
-    }
-  }
-  return sum;
+  for (int j = 0; j < kLPW; ++j)
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj) {
+        v4i *dst = (v4i *)(X + (((j * 2 + bi) * 2 + bj) * 64 + lane) * 16);
+        const v16i &a = acc[j][bi][bj];
+        dst[0] = v4i{a[0], a[1], a[2], a[3]};
+        dst[1] = v4i{a[4], a[5], a[6], a[7]};
+        dst[2] = v4i{a[8], a[9], a[10], a[11]};
+        dst[3] = v4i{a[12], a[13], a[14], a[15]};
+      }
+}
+
+// Where output (row, col) of the 64x64 tile sits in a dump: the 32x32 MFMA
+// result layout puts column col%32 in lane (col%32) + 32*((row%8)/4) and row
+// 8*(r/4) + 4*(lane/32) + r%4 in accumulator element r.
+__device__ __forceinline__ int dump_index(int slot, int row, int col) {
+  const int bi = row >> 5, bj = col >> 5, rr = row & 31;
+  const int lane = (col & 31) + 32 * ((rr & 7) >> 2);
+  const int r = 4 * (rr >> 3) + (rr & 3);
+  return (((slot * 2 + bi) * 2 + bj) * 64 + lane) * 16 + r;
 }
 
 template <int S>
-__global__ __launch_bounds__(kThreads) void k_ozaki(const signed char *__restrict__ As,
+__global__ __launch_bounds__(kMainThreads) void k_ozaki(const signed char *__restrict__ As,
                                                     const signed char *__restrict__ Bs,
                                                     const int *__restrict__ E,
                                                     const int *__restrict__ F,
                                                     double2 *__restrict__ C, int64_t m, int64_t n,
                                                     int64_t KB, int tiles_m) {
   extern __shared__ __attribute__((aligned(16))) signed char lds[];
-  signed char *sa = lds;
-  signed char *sb = lds + S * kLdsSlice;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int ib = blockIdx.x % tiles_m, jb = blockIdx.x / tiles_m;
-  const v4i *ga = (const v4i *)(As + (int64_t)ib * KB * S * kChunk);
-  const v4i *gb = (const v4i *)(Bs + (int64_t)jb * KB * S * kChunk);
-  constexpr int V = S * kChunk / 16;  // 16-byte vectors per operand chunk
-  constexpr int PER = (V + kThreads - 1) / kThreads;
+  const signed char *ga = As + (int64_t)ib * KB * S * kChunk;
+  const signed char *gb = Bs + (int64_t)jb * KB * S * kChunk;
+  constexpr int kBuf = 2 * S * kChunk;         // one stage: A chunk then B chunk
+  constexpr int NV = S * kChunk / 1024;        // 1 KiB wave-loads per operand chunk
 
-  v16i acc[4][2][2];
+  v16i acc[kLPW][2][2];
 #pragma unroll
-  for (int j = 0; j < 4; ++j)
+  for (int j = 0; j < kLPW; ++j)
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
       for (int b = 0; b < 2; ++b) acc[j][a][b] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 
-  v4i pa[PER], pb[PER];
-  auto fetch = [&](int64_t kb) {
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      int v = tid + q * kThreads;
-      if (v < V) {
-        pa[q] = ga[kb * V + v];
-        pb[q] = gb[kb * V + v];
-      }
-    }
-  };
-  // 16-byte vector v of a chunk: slice v/128, tile row (v%128)/2, half v%2.
-  auto stash = [&]() {
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      int v = tid + q * kThreads;
-      if (v < V) {
-        int s = v >> 7, row = (v & 127) >> 1, hf = v & 1;
-        int off = s * kLdsSlice + row * kPitch + hf * 16;
-        *(v4i *)(sa + off) = pa[q];
-        *(v4i *)(sb + off) = pb[q];
-      }
+  // Asynchronous HBM -> LDS copy of the K block kb into stage st: each wave
+  // moves 1 KiB per instruction (16 bytes per lane, consecutive in LDS).
+  auto issue = [&](int64_t kb, int st) {
+    signed char *dst = lds + st * kBuf;
+    const signed char *srca = ga + kb * S * kChunk;
+    const signed char *srcb = gb + kb * S * kChunk;
+    for (int v = wave; v < NV; v += kWaves) {
+      __builtin_amdgcn_global_load_lds((const void *)(srca + v * 1024 + lane * 16),
+                                       (__attribute__((address_space(3))) void *)(dst + v * 1024),
+                                       16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void *)(srcb + v * 1024 + lane * 16),
+                                       (__attribute__((address_space(3))) void *)(dst + S * kChunk + v * 1024),
+                                       16, 0, 0);
     }
   };
 
-  fetch(0);
+  issue(0, 0);
   for (int64_t kb = 0; kb < KB; ++kb) {
-    __syncthreads();
-    stash();
-    __syncthreads();
-    if (kb + 1 < KB) fetch(kb + 1);
+    const int st = (int)(kb & 1);
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of block kb have landed
+    __syncthreads();                // ...and every wave's; stage st^1 is free
+    if (kb + 1 < KB) issue(kb + 1, st ^ 1);
+    const signed char *sa = lds + st * kBuf;
+    const signed char *sb = sa + S * kChunk;
     switch (wave) {
       case 0: compute_block<S, 0>(sa, sb, acc, lane); break;
       case 1: compute_block<S, 1>(sa, sb, acc, lane); break;
       case 2: compute_block<S, 2>(sa, sb, acc, lane); break;
-      default: compute_block<S, 3>(sa, sb, acc, lane); break;
+      case 3: compute_block<S, 3>(sa, sb, acc, lane); break;
+      case 4: compute_block<S, 4>(sa, sb, acc, lane); break;
+      case 5: compute_block<S, 5>(sa, sb, acc, lane); break;
+      case 6: compute_block<S, 6>(sa, sb, acc, lane); break;
+      default: compute_block<S, 7>(sa, sb, acc, lane); break;
     }
   }
+  __builtin_amdgcn_s_waitcnt(0);
 
-  // Recombine: wave partials meet in LDS in wave order 0,1,2,3 (fixed, so
-  // the result is deterministic); wave 3 scales and stores.
-  __syncthreads();
-  double *red = (double *)lds;  // 64 x 64 DD = 64 KiB
-  for (int w = 0; w < 4; ++w) {
-    if (wave == w) {
+  // Recombine. Thread t owns outputs (row t%64, cols 8*(t/64) .. +7) and
+  // keeps their running DD sums in registers. Waves dump their level
+  // accumulators to LDS two at a time (waves 0-1 first, ...; within a wave
+  // the slots in order), so the summation order is fixed.
+  constexpr Plan<S> P{};
+  int *X = (int *)lds;
+  constexpr int kDump = kLPW * 4 * 64 * 16;  // ints per wave dump
+  const int orow = tid & 63, ocol0 = (tid >> 6) * 8;
+  ddv part[8];
 #pragma unroll
-      for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-        for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-          for (int r = 0; r < 16; ++r) {
-            // 32x32 MFMA result layout: lane holds column (lane % 32) and
-            // rows 8*(r/4) + 4*(lane/32) + r%4.
-            const int row = bi * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            const int col = bj * 32 + (lane & 31);
-            ddv part;
-            switch (w) {
-              case 0: part = wave_partial<S, 0>(acc, bi, bj, r); break;
-              case 1: part = wave_partial<S, 1>(acc, bi, bj, r); break;
-              case 2: part = wave_partial<S, 2>(acc, bi, bj, r); break;
-              default: part = wave_partial<S, 3>(acc, bi, bj, r); break;
-            }
-            double *slot = red + 2 * (row * kT + col);
-            if (w > 0) part = dd_add(mk(slot[0], slot[1]), part);
-            if (w < 3) {
-              slot[0] = part.hi;
-              slot[1] = part.lo;
-            } else {
-              const int64_t gi = (int64_t)ib * kT + row, gj = (int64_t)jb * kT + col;
-              if (gi < m && gj < n) {
-                const int ei = E[gi], fj = F[gj];
-                double2 out;
-                if (ei == kNonFinite || fj == kNonFinite) {
-                  out = make_double2(__builtin_nan(""), 0.0);
-                } else {
-                  const int sc = scale_of(ei) + scale_of(fj) - 10;
-                  out = make_double2(__builtin_ldexp(part.hi, sc), __builtin_ldexp(part.lo, sc));
-                }
-                C[gi + gj * m] = out;
-              }
-            }
-          }
-    }
+  for (int q = 0; q < 8; ++q) part[q] = mk(0.0, 0.0);
+  for (int w0 = 0; w0 < kWaves; w0 += 2) {
     __syncthreads();
+    if ((wave >> 1) == (w0 >> 1)) dump_acc(acc, X + (wave & 1) * kDump, lane);
+    __syncthreads();
+    for (int w = w0; w < w0 + 2; ++w)
+      for (int j = 0; j < kLPW; ++j) {
+        const int L = P.lv[w][j];
+        if (L < 0) continue;
+        const int *Xw = X + (w & 1) * kDump;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const double v = __builtin_ldexp((double)Xw[dump_index(j, orow, ocol0 + q)], -7 * L);
+          part[q] = dd_add(part[q], mk(v, 0.0));
+        }
+      }
+  }
+  const int64_t gi = (int64_t)ib * kT + orow;
+  if (gi < m) {
+    const int ei = E[gi];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int64_t gj = (int64_t)jb * kT + ocol0 + q;
+      if (gj < n) {
+        const int fj = F[gj];
+        double2 out;
+        if (ei == kNonFinite || fj == kNonFinite) {
+          out = make_double2(__builtin_nan(""), 0.0);
+        } else {
+          const int sc = scale_of(ei) + scale_of(fj) - 10;
+          out = make_double2(__builtin_ldexp(part[q].hi, sc), __builtin_ldexp(part[q].lo, sc));
+        }
+        C[gi + gj * m] = out;
+      }
+    }
   }
 }
 
@@ -385,12 +406,12 @@ template <int S>
 hipError_t launch_main(const signed char *As, const signed char *Bs, const int *E, const int *F,
                        double2 *C, int64_t m, int64_t n, int64_t KB, int64_t tm, int64_t tn,
                        hipStream_t s) {
-  // operand staging, reused by the 64 x 64 DD recombination buffer (64 KiB)
-  const size_t lds = 2 * (size_t)S * kLdsSlice > 65536 ? 2 * (size_t)S * kLdsSlice : 65536;
+  // two operand stages (2 x 2 x S x 2 KiB), reused by the 64 KiB recombination dump
+  const size_t lds = 4 * (size_t)S * kChunk > 65536 ? 4 * (size_t)S * kChunk : 65536;
   hipError_t e = hipFuncSetAttribute((const void *)k_ozaki<S>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_ozaki<S>, dim3((unsigned)(tm * tn)), dim3(kThreads), lds, s, As, Bs, E, F,
+  hipLaunchKernelGGL(k_ozaki<S>, dim3((unsigned)(tm * tn)), dim3(kMainThreads), lds, s, As, Bs, E, F,
                      C, m, n, KB, (int)tm);
   return hipGetLastError();
 }
@@ -409,43 +430,69 @@ size_t ozaki_workspace_bytes(int64_t m, int64_t k, int64_t n, int S) {
          align256(sizeof(int) * (size_t)m) + align256(sizeof(int) * (size_t)n);
 }
 
-hipError_t ozaki_gemm(const double2 *A, const double2 *B, double2 *C, int64_t m, int64_t k,
-                      int64_t n, int S, void *work, size_t work_bytes, hipStream_t s) {
-  if (m <= 0 || n <= 0) return hipSuccess;
-  if (k <= 0) return hipMemsetAsync(C, 0, sizeof(double2) * m * n, s);
-  // int32 level sums stay exact while k * S * 64 * 64 < 2^31
-  if (S < 8 || S > ozaki_max_slices() || k > 32768) return hipErrorInvalidValue;
-  if (work_bytes < ozaki_workspace_bytes(m, k, n, S)) return hipErrorInvalidValue;
-  const int64_t tm = (m + kT - 1) / kT, tn = (n + kT - 1) / kT, KB = (k + kK - 1) / kK;
-  char *w = (char *)work;
-  signed char *As = (signed char *)w;
-  w += align256((size_t)tm * KB * S * kChunk);
-  signed char *Bs = (signed char *)w;
-  w += align256((size_t)tn * KB * S * kChunk);
-  int *E = (int *)w;
-  w += align256(sizeof(int) * (size_t)m);
-  int *F = (int *)w;
+namespace {
+struct OzWork {
+  signed char *As, *Bs;
+  int *E, *F;
+  int64_t tm, tn, KB;
+};
+OzWork carve(void *work, int64_t m, int64_t k, int64_t n, int S) {
+  OzWork w;
+  w.tm = (m + kT - 1) / kT;
+  w.tn = (n + kT - 1) / kT;
+  w.KB = (k + kK - 1) / kK;
+  char *p = (char *)work;
+  w.As = (signed char *)p;
+  p += align256((size_t)w.tm * w.KB * S * kChunk);
+  w.Bs = (signed char *)p;
+  p += align256((size_t)w.tn * w.KB * S * kChunk);
+  w.E = (int *)p;
+  p += align256(sizeof(int) * (size_t)m);
+  w.F = (int *)p;
+  return w;
+}
+bool oz_args_ok(int64_t k, int S) { return S >= 8 && S <= ozaki_max_slices() && k <= 32768; }
+}  // namespace
 
-  hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, E, m, kEmpty);
+hipError_t ozaki_prepare(const double2 *A, const double2 *B, int64_t m, int64_t k, int64_t n,
+                         int S, void *work, size_t work_bytes, hipStream_t s) {
+  if (m <= 0 || n <= 0 || k <= 0) return hipSuccess;
+  if (!oz_args_ok(k, S) || work_bytes < ozaki_workspace_bytes(m, k, n, S)) return hipErrorInvalidValue;
+  OzWork w = carve(work, m, k, n, S);
+  hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, w.E, m, kEmpty);
   const int64_t kchunk = 64;
   hipLaunchKernelGGL(k_row_exp, dim3((unsigned)((m + kThreads - 1) / kThreads), (unsigned)((k + kchunk - 1) / kchunk)),
-                     dim3(kThreads), 0, s, A, m, k, kchunk, E);
-  hipLaunchKernelGGL(k_col_exp, dim3((unsigned)n), dim3(kThreads), 0, s, B, k, F);
-  hipLaunchKernelGGL(k_slice_a<16>, dim3((unsigned)tm, (unsigned)KB), dim3(kThreads), 0, s, A, m, k, E, S, KB, As);
-  hipLaunchKernelGGL(k_slice_b<16>, dim3((unsigned)tn, (unsigned)KB), dim3(kThreads), 0, s, B, k, n, F, S, KB, Bs);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+                     dim3(kThreads), 0, s, A, m, k, kchunk, w.E);
+  hipLaunchKernelGGL(k_col_exp, dim3((unsigned)n), dim3(kThreads), 0, s, B, k, w.F);
+  hipLaunchKernelGGL(k_slice_a<16>, dim3((unsigned)w.tm, (unsigned)w.KB), dim3(kThreads), 0, s, A, m, k, w.E, S, w.KB, w.As);
+  hipLaunchKernelGGL(k_slice_b<16>, dim3((unsigned)w.tn, (unsigned)w.KB), dim3(kThreads), 0, s, B, k, n, w.F, S, w.KB, w.Bs);
+  return hipGetLastError();
+}
+
+hipError_t ozaki_multiply(double2 *C, int64_t m, int64_t k, int64_t n, int S, void *work,
+                          size_t work_bytes, hipStream_t s) {
+  if (m <= 0 || n <= 0) return hipSuccess;
+  if (k <= 0) return hipMemsetAsync(C, 0, sizeof(double2) * m * n, s);
+  if (!oz_args_ok(k, S) || work_bytes < ozaki_workspace_bytes(m, k, n, S)) return hipErrorInvalidValue;
+  OzWork w = carve(work, m, k, n, S);
   switch (S) {
-    case 8: return launch_main<8>(As, Bs, E, F, C, m, n, KB, tm, tn, s);
-    case 9: return launch_main<9>(As, Bs, E, F, C, m, n, KB, tm, tn, s);
-    case 10: return launch_main<10>(As, Bs, E, F, C, m, n, KB, tm, tn, s);
-    case 11: return launch_main<11>(As, Bs, E, F, C, m, n, KB, tm, tn, s);
-    case 12: return launch_main<12>(As, Bs, E, F, C, m, n, KB, tm, tn, s);
-    case 13: return launch_main<13>(As, Bs, E, F, C, m, n, KB, tm, tn, s);
-    case 14: return launch_main<14>(As, Bs, E, F, C, m, n, KB, tm, tn, s);
-    case 15: return launch_main<15>(As, Bs, E, F, C, m, n, KB, tm, tn, s);
-    default: return launch_main<16>(As, Bs, E, F, C, m, n, KB, tm, tn, s);
+    case 8: return launch_main<8>(w.As, w.Bs, w.E, w.F, C, m, n, w.KB, w.tm, w.tn, s);
+    case 9: return launch_main<9>(w.As, w.Bs, w.E, w.F, C, m, n, w.KB, w.tm, w.tn, s);
+    case 10: return launch_main<10>(w.As, w.Bs, w.E, w.F, C, m, n, w.KB, w.tm, w.tn, s);
+    case 11: return launch_main<11>(w.As, w.Bs, w.E, w.F, C, m, n, w.KB, w.tm, w.tn, s);
+    case 12: return launch_main<12>(w.As, w.Bs, w.E, w.F, C, m, n, w.KB, w.tm, w.tn, s);
+    case 13: return launch_main<13>(w.As, w.Bs, w.E, w.F, C, m, n, w.KB, w.tm, w.tn, s);
+    case 14: return launch_main<14>(w.As, w.Bs, w.E, w.F, C, m, n, w.KB, w.tm, w.tn, s);
+    case 15: return launch_main<15>(w.As, w.Bs, w.E, w.F, C, m, n, w.KB, w.tm, w.tn, s);
+    default: return launch_main<16>(w.As, w.Bs, w.E, w.F, C, m, n, w.KB, w.tm, w.tn, s);
   }
+}
+
+hipError_t ozaki_gemm(const double2 *A, const double2 *B, double2 *C, int64_t m, int64_t k,
+                      int64_t n, int S, void *work, size_t work_bytes, hipStream_t s) {
+  hipError_t e = ozaki_prepare(A, B, m, k, n, S, work, work_bytes, s);
+  if (e != hipSuccess) return e;
+  return ozaki_multiply(C, m, k, n, S, work, work_bytes, s);
 }
 
 }  // namespace mfhip

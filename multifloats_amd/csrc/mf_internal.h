@@ -37,5 +37,11 @@ int ozaki_max_slices();
 size_t ozaki_workspace_bytes(int64_t m, int64_t k, int64_t n, int S);
 hipError_t ozaki_gemm(const double2 *A, const double2 *B, double2 *C, int64_t m, int64_t k,
                       int64_t n, int S, void *work, size_t work_bytes, hipStream_t s);
+// The two halves of ozaki_gemm: scaling + slicing into `work`, then the
+// INT8 MFMA products and DD recombination from `work` into C.
+hipError_t ozaki_prepare(const double2 *A, const double2 *B, int64_t m, int64_t k, int64_t n,
+                         int S, void *work, size_t work_bytes, hipStream_t s);
+hipError_t ozaki_multiply(double2 *C, int64_t m, int64_t k, int64_t n, int S, void *work,
+                          size_t work_bytes, hipStream_t s);
 
 }  // namespace mfhip

@@ -1,0 +1,96 @@
+"""GPU tests of the Ozaki INT8 split-precision DD GEMM: elementwise error
+against the exact accumulator, |C - AB|_ij <= 2^-100 (|A||B|)_ij."""
+import numpy as np
+import pytest
+
+import multifloats_amd as mf
+from multifloats_amd import datagen
+from tests import _native as nt
+
+pytestmark = pytest.mark.gpu
+
+BOUND = 2.0 ** -100
+
+
+def mats(rng, m, k, n, spread):
+    e = spread if spread else None
+    return datagen.matrix(rng, m, k, e), datagen.matrix(rng, k, n, e)
+
+
+def all_err(A, B, C):
+    m, n = A.shape[0], B.shape[1]
+    rows, cols = np.meshgrid(np.arange(m), np.arange(n), indexing="ij")
+    return nt.gemm_err(A, B, C, rows.ravel(), cols.ravel())
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (64, 32, 64), (70, 45, 130), (200, 300, 100), (5, 1000, 3)])
+@pytest.mark.parametrize("spread", [0, 10, 20, 30])
+def test_ozaki_error_bound(gpu, shape, spread):
+    m, k, n = shape
+    rng = np.random.default_rng(m + 7 * k + 13 * n + spread)
+    A, B = mats(rng, m, k, n, spread)
+    C = mf.matmul_ozaki(A, B, 16)
+    err, mag = all_err(A, B, C)
+    ratio = err / mag
+    assert ratio.max() <= BOUND, (ratio.max(), np.log2(ratio.max()))
+
+
+def test_ozaki_error_falls_with_slices(gpu):
+    rng = np.random.default_rng(3)
+    A, B = mats(rng, 96, 200, 80, 20)
+    worst = []
+    for S in (8, 10, 12, 14, 16):
+        err, mag = all_err(A, B, mf.matmul_ozaki(A, B, S))
+        worst.append(np.log2((err / mag).max()))
+        # the dropped levels L >= S bound the error: a loose a-priori check
+        assert worst[-1] <= -7 * S + 24, (S, worst[-1])
+    assert all(a > b for a, b in zip(worst, worst[1:])), worst
+
+
+def test_ozaki_special_rows_and_columns(gpu):
+    rng = np.random.default_rng(4)
+    A, B = mats(rng, 70, 40, 66, 10)
+    A = A.copy()
+    B = B.copy()
+    A[3, :, :] = 0.0            # zero row -> zero outputs
+    A[5, 7, 0] = np.inf         # row with Inf -> NaN outputs
+    B[:, 9, :] = 0.0            # zero column
+    B[11, 20, 0] = np.nan       # column with NaN
+    C = mf.matmul_ozaki(datagen.as_colmajor(A), datagen.as_colmajor(B), 16)
+    assert not np.delete(C[3], 20, axis=0).any() and np.isnan(C[3, 20, 0])
+    assert np.isnan(C[5, :, 0]).all()
+    assert not C[np.arange(70) != 5, 9].any()
+    assert np.isnan(C[:, 20, 0]).all()
+    ok = np.ones((70, 66), bool)
+    ok[[3, 5], :] = False
+    ok[:, [9, 20]] = False
+    rows, cols = np.nonzero(ok)
+    err, mag = nt.gemm_err(A, B, C, rows, cols)
+    assert (err <= BOUND * mag).all()
+
+
+def test_ozaki_empty_inner_dimension(gpu):
+    C = mf.matmul_ozaki(np.zeros((4, 0, 2)), np.zeros((0, 3, 2)), 16)
+    assert C.shape == (4, 3, 2) and not C.any()
+
+
+def test_ozaki_4096_sampled(gpu):
+    """4096^3 with exponent spread 2^+-20: sampled outputs within the bound."""
+    import torch
+    m = k = n = 4096
+    rng = np.random.default_rng(42)
+    A, B = mats(rng, m, k, n, 20)
+    dA = torch.from_numpy(datagen.flat(A)).cuda()
+    dB = torch.from_numpy(datagen.flat(B)).cuda()
+    dC = torch.empty((m * n, 2), dtype=torch.float64, device="cuda")
+    wb = mf.ozaki_workspace_bytes(m, k, n, 16)
+    work = torch.empty(wb, dtype=torch.uint8, device="cuda")
+    s = torch.cuda.current_stream().cuda_stream
+    mf.dev_matmul_ozaki(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), m, k, n, 16, work.data_ptr(), wb, s)
+    torch.cuda.synchronize()
+    C = datagen.unflat(dC.cpu().numpy(), m, n)
+    rows = rng.choice(m, 24, replace=False)
+    cols = rng.choice(n, 24, replace=False)
+    rr, cc = np.meshgrid(rows, cols, indexing="ij")
+    err, mag = nt.gemm_err(A, B, C, rr.ravel(), cc.ravel())
+    assert (err <= BOUND * mag).all(), np.log2((err / mag).max())
