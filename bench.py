@@ -48,7 +48,7 @@ def parse():
     p.add_argument("--workload", default="gemm", choices=["gemm", "elementwise", "reduce"])
     p.add_argument("--size", type=int, default=4096)
     p.add_argument("--spread", type=float, default=20.0)
-    p.add_argument("--slices", type=int, default=16)
+    p.add_argument("--slices", type=int, default=0, help="0 = chosen from the data")
     p.add_argument("--form", default="both", choices=["both", "plain", "ozaki"])
     p.add_argument("--err-sample", type=int, default=32)
     p.add_argument("--variant", type=int, default=-1)
@@ -150,6 +150,10 @@ def bench_gemm(args, torch, mf):
         def step_mult():
             mf.dev_ozaki_multiply(dC.data_ptr(), m, k, n, S, work.data_ptr(), wb, stream)
         ms = timed(torch, step_oz, args.steps, args.warmup)
+        S = mf.ozaki_slices_used()
+        if S == 0:
+            raise SystemExit("bench: the slice rule chose the reference-order GEMM for this input")
+        mf.dev_ozaki_prepare(dA.data_ptr(), dB.data_ptr(), m, k, n, S, work.data_ptr(), wb, stream)
         ms_mult = timed(torch, step_mult, args.steps, 1)
         C = datagen.unflat(dC.cpu().numpy(), m, n)
         emax, emean = sampled_error(A, B, C, rows, cols)
@@ -172,7 +176,8 @@ def bench_gemm(args, torch, mf):
         "dtype": "int8-mfma+f64" if args.form != "plain" else "f64", "data": "synthetic",
         "config": {"workload": "float64x2 GEMM m=n=k=%d, entries s*2^U(-%g,%g)*u, lo per config-0; %s"
                                % (m, args.spread, args.spread,
-                                  "Ozaki INT8 (%d slices) and plain reference-order (renorm 8) forms" % S
+                                  "Ozaki INT8 (slices chosen from the data: %d) and plain reference-order "
+                                  "(renorm 8) forms" % S
                                   if args.form == "both" else args.form),
                    "m": m, "n": n, "k": k, "slices": S, "renorm_interval": ri,
                    "error_sample": "%dx%d outputs vs exact accumulator" % (es, es)},

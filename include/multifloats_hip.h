@@ -126,10 +126,11 @@ MULTIFLOATS_API float64x2 norm2dd(const float64x2 *a, int64_t n);
  * `slices` int8 digit slices (7 bits each) per row / column scale; slice
  * pairs whose scales sum to fewer than `slices` levels are multiplied with
  * exact INT32 accumulation and recombined in DD. Not bit-identical to
- * matmuldd_mm: held to |C - AB| <= 2^-100 (|A||B|) elementwise at the default
- * 16 slices for inputs whose exponents span up to 2^+-30 per row/column.
- * slices <= 0 selects 16; valid range 8..16; k <= 32768. Rows of A or columns
- * of B holding Inf/NaN give NaN outputs. */
+ * matmuldd_mm; the target is |C - AB| <= 2^-100 (|A||B|) elementwise.
+ * slices <= 0 (recommended) chooses the count from k and the exponent ranges
+ * of A's rows and B's columns, and when more than 16 would be needed runs the
+ * reference-order GEMM instead; explicit counts must lie in 8..16.
+ * k <= 32768. Rows of A or columns of B holding Inf/NaN give NaN outputs. */
 MULTIFLOATS_API void matmuldd_mm_ozaki(const float64x2 *a, const float64x2 *b, float64x2 *c,
                                        int64_t m, int64_t k, int64_t n, int slices);
 
@@ -170,12 +171,18 @@ MULTIFLOATS_API int mfhip_reduce(int kind, const float64x2 *a, const float64x2 *
                                  void *stream);
 
 /* Ozaki GEMM on device pointers; `work` is caller-owned device scratch of at
- * least mfhip_ozaki_workspace_bytes(m, k, n, slices) bytes. */
+ * least mfhip_ozaki_workspace_bytes(m, k, n, slices) bytes. With slices <= 0
+ * the call synchronises `stream` once to read the exponent ranges. */
 MULTIFLOATS_API size_t mfhip_ozaki_workspace_bytes(int64_t m, int64_t k, int64_t n, int slices);
 MULTIFLOATS_API int mfhip_ozaki_matmul_mm(const float64x2 *a, const float64x2 *b, float64x2 *c,
                                           int64_t m, int64_t k, int64_t n, int slices, void *work,
                                           size_t work_bytes, void *stream);
-/* mfhip_ozaki_matmul_mm in two steps: prepare scales and slices A and B into
+/* Slice count used by the calling thread's last Ozaki call (0 = it ran the
+ * reference-order GEMM instead), and the rule behind slices <= 0. */
+MULTIFLOATS_API int mfhip_ozaki_slices_used(void);
+MULTIFLOATS_API int mfhip_ozaki_choose_slices(int64_t k, int range_a, int range_b);
+/* mfhip_ozaki_matmul_mm in two steps with an explicit slice count (8..16):
+ * prepare scales and slices A and B into
  * `work`; multiply forms C from `work` (same m, k, n, slices, work). A
  * prepared `work` may be multiplied again, e.g. to time the product alone. */
 MULTIFLOATS_API int mfhip_ozaki_prepare(const float64x2 *a, const float64x2 *b, int64_t m,

@@ -84,6 +84,8 @@ def lib():
         L.mfhip_ozaki_workspace_bytes.argtypes = [I64, I64, I64, I]
         L.mfhip_ozaki_workspace_bytes.restype = ctypes.c_size_t
         L.mfhip_ozaki_matmul_mm.argtypes = [P, P, P, I64, I64, I64, I, P, ctypes.c_size_t, VP]
+        L.mfhip_ozaki_slices_used.argtypes = []
+        L.mfhip_ozaki_choose_slices.argtypes = [I64, I, I]
         L.mfhip_ozaki_prepare.argtypes = [P, P, I64, I64, I64, I, P, ctypes.c_size_t, VP]
         L.mfhip_ozaki_multiply.argtypes = [P, I64, I64, I64, I, P, ctypes.c_size_t, VP]
         L.mfhip_last_error.argtypes = []
@@ -152,8 +154,11 @@ def matmul(a, b, renorm_interval=DD_FMA_RENORM_INTERVAL):
     return c
 
 
-def matmul_ozaki(a, b, slices=16):
-    """Ozaki split-precision DD GEMM on INT8 matrix cores (matmuldd_mm_ozaki)."""
+def matmul_ozaki(a, b, slices=0):
+    """Ozaki split-precision DD GEMM on INT8 matrix cores (matmuldd_mm_ozaki).
+
+    slices=0 chooses the slice count from the data (ozaki_slices_used() tells
+    which; 0 there means the reference-order GEMM ran instead)."""
     a, b = _mat(a), _mat(b)
     m, k = a.shape[:2]
     if b.shape[0] != k:
@@ -232,7 +237,15 @@ def dev_matmul(a, b, c, m, k, n, renorm_interval=DD_FMA_RENORM_INTERVAL, stream=
     _dev(lib().mfhip_matmul_mm_variant(a, b, c, m, k, n, renorm_interval, variant, stream))
 
 
-def ozaki_workspace_bytes(m, k, n, slices=16):
+def ozaki_slices_used():
+    return lib().mfhip_ozaki_slices_used()
+
+
+def ozaki_choose_slices(k, range_a, range_b):
+    return lib().mfhip_ozaki_choose_slices(k, range_a, range_b)
+
+
+def ozaki_workspace_bytes(m, k, n, slices=0):
     return lib().mfhip_ozaki_workspace_bytes(m, k, n, slices)
 
 

@@ -182,25 +182,26 @@ hipError_t host_reduce(int kind, const float64x2 *a, const float64x2 *b, int64_t
   return hipStreamSynchronize(tl_stage.stream);
 }
 
-int oz_slices(int s) { return s <= 0 ? 16 : s; }
+thread_local int tl_oz_used = 0;
+// workspace is always sized for the largest slice count the call may use
+int oz_ws_slices(int s) { return s <= 0 ? mfhip::ozaki_max_slices() : s; }
 
 hipError_t host_mm_ozaki(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t m,
                          int64_t k, int64_t n, int slices) {
   tl_err = 0;
   if (m <= 0 || n <= 0) return hipSuccess;
-  slices = oz_slices(slices);
-  if (slices < 8 || slices > mfhip::ozaki_max_slices()) return hipErrorInvalidValue;
+  if (slices > mfhip::ozaki_max_slices() || (slices > 0 && slices < 8)) return hipErrorInvalidValue;
   TRY(stage_init());
   const size_t kk = k > 0 ? (size_t)k : 0;
   void *da, *db, *dc, *dw;
-  const size_t wb = mfhip::ozaki_workspace_bytes(m, k, n, slices);
+  const size_t wb = mfhip::ozaki_workspace_bytes(m, k, n, oz_ws_slices(slices));
   TRY(stage_buf(0, 16 * (size_t)m * kk, &da));
   TRY(stage_buf(1, 16 * kk * (size_t)n, &db));
   TRY(stage_buf(2, 16 * (size_t)m * n, &dc));
   TRY(stage_buf(3, wb, &dw));
   TRY(h2d(da, a, 16 * (size_t)m * kk));
   TRY(h2d(db, b, 16 * kk * (size_t)n));
-  TRY(mfhip::ozaki_gemm(D(da), D(db), W(dc), m, k, n, slices, dw, wb, tl_stage.stream));
+  TRY(mfhip::ozaki_gemm(D(da), D(db), W(dc), m, k, n, slices, dw, wb, tl_stage.stream, &tl_oz_used));
   TRY(d2h(c, dc, 16 * (size_t)m * n));
   return hipStreamSynchronize(tl_stage.stream);
 }
@@ -369,26 +370,31 @@ int mfhip_reduce(int kind, const float64x2 *a, const float64x2 *b, int64_t n, in
 }
 
 size_t mfhip_ozaki_workspace_bytes(int64_t m, int64_t k, int64_t n, int slices) {
-  return mfhip::ozaki_workspace_bytes(m, k, n, oz_slices(slices));
+  return mfhip::ozaki_workspace_bytes(m, k, n, oz_ws_slices(slices));
 }
 int mfhip_ozaki_matmul_mm(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t m,
                           int64_t k, int64_t n, int slices, void *work, size_t work_bytes,
                           void *stream) {
   CHECK_ALIGN(a, b, c, work);
-  return status(mfhip::ozaki_gemm(D(a), D(b), W(c), m, k, n, oz_slices(slices), work, work_bytes,
-                                  (hipStream_t)stream));
+  if (slices > mfhip::ozaki_max_slices() || (slices > 0 && slices < 8))
+    return status(hipErrorInvalidValue);
+  return status(mfhip::ozaki_gemm(D(a), D(b), W(c), m, k, n, slices, work, work_bytes,
+                                  (hipStream_t)stream, &tl_oz_used));
 }
-
+int mfhip_ozaki_slices_used(void) { return tl_oz_used; }
+int mfhip_ozaki_choose_slices(int64_t k, int range_a, int range_b) {
+  return mfhip::ozaki_choose_slices(k, range_a, range_b);
+}
 int mfhip_ozaki_prepare(const float64x2 *a, const float64x2 *b, int64_t m, int64_t k, int64_t n,
                         int slices, void *work, size_t work_bytes, void *stream) {
   CHECK_ALIGN(a, b, work);
-  return status(mfhip::ozaki_prepare(D(a), D(b), m, k, n, oz_slices(slices), work, work_bytes,
+  return status(mfhip::ozaki_prepare(D(a), D(b), m, k, n, slices, work, work_bytes,
                                      (hipStream_t)stream));
 }
 int mfhip_ozaki_multiply(float64x2 *c, int64_t m, int64_t k, int64_t n, int slices, void *work,
                          size_t work_bytes, void *stream) {
   CHECK_ALIGN(c, work);
-  return status(mfhip::ozaki_multiply(W(c), m, k, n, oz_slices(slices), work, work_bytes,
+  return status(mfhip::ozaki_multiply(W(c), m, k, n, slices, work, work_bytes,
                                       (hipStream_t)stream));
 }
 

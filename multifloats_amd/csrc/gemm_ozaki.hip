@@ -33,6 +33,8 @@
 #include "dd_core.h"
 #include "mf_internal.h"
 
+#include <math.h>
+
 namespace {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
@@ -55,45 +57,72 @@ __device__ __forceinline__ int dexp(double x) {
   return e - 1;
 }
 
-// Row exponents of A (column-major m x k): thread = row, block.y = k chunk.
+// Row exponent maxima and minima of A (column-major m x k): thread = row,
+// block.y = k chunk. Zeros are skipped; a non-finite entry marks the row.
 __global__ __launch_bounds__(kThreads) void k_row_exp(const double2 *__restrict__ A, int64_t m,
                                                       int64_t k, int64_t kchunk,
-                                                      int *__restrict__ E) {
+                                                      int *__restrict__ E, int *__restrict__ Emin) {
   const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   if (i >= m) return;
   const int64_t p0 = (int64_t)blockIdx.y * kchunk;
   const int64_t p1 = p0 + kchunk < k ? p0 + kchunk : k;
-  int e = kEmpty;
+  int e = kEmpty, lo = kNonFinite;
   bool bad = false;
   for (int64_t p = p0; p < p1; ++p) {
     double h = A[i + p * m].x;
     if (!mf_isfinite(h)) bad = true;
-    else if (h != 0.0) e = max(e, dexp(h));
+    else if (h != 0.0) {
+      const int x = dexp(h);
+      e = max(e, x);
+      lo = min(lo, x);
+    }
   }
   if (bad) e = kNonFinite;
   if (e != kEmpty) atomicMax(&E[i], e);
+  if (lo != kNonFinite) atomicMin(&Emin[i], lo);
 }
 
-// Column exponents of B (column-major k x n): block = column, threads walk p.
+// Largest exponent range max - min over the finite, nonzero rows.
+__global__ void k_row_range(const int *__restrict__ E, const int *__restrict__ Emin, int64_t m,
+                            int *__restrict__ range) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const int e = E[i];
+  if (e != kEmpty && e != kNonFinite) atomicMax(range, e - Emin[i]);
+}
+
+// Column exponent maxima of B (column-major k x n): block = column, threads
+// walk p; also folds the column's exponent range into *range.
 __global__ __launch_bounds__(kThreads) void k_col_exp(const double2 *__restrict__ B, int64_t k,
-                                                      int *__restrict__ F) {
+                                                      int *__restrict__ F, int *__restrict__ range) {
   const int64_t j = blockIdx.x;
-  int e = kEmpty;
+  int e = kEmpty, lo = kNonFinite;
   bool bad = false;
   for (int64_t p = threadIdx.x; p < k; p += kThreads) {
     double h = B[p + j * k].x;
     if (!mf_isfinite(h)) bad = true;
-    else if (h != 0.0) e = max(e, dexp(h));
+    else if (h != 0.0) {
+      const int x = dexp(h);
+      e = max(e, x);
+      lo = min(lo, x);
+    }
   }
   if (bad) e = kNonFinite;
-  __shared__ int red[kThreads];
-  red[threadIdx.x] = e;
+  __shared__ int rmax[kThreads], rmin[kThreads];
+  rmax[threadIdx.x] = e;
+  rmin[threadIdx.x] = lo;
   __syncthreads();
   for (int s = kThreads / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + s]);
+    if (threadIdx.x < s) {
+      rmax[threadIdx.x] = max(rmax[threadIdx.x], rmax[threadIdx.x + s]);
+      rmin[threadIdx.x] = min(rmin[threadIdx.x], rmin[threadIdx.x + s]);
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) F[j] = red[0];
+  if (threadIdx.x == 0) {
+    F[j] = rmax[0];
+    if (rmax[0] != kEmpty && rmax[0] != kNonFinite) atomicMax(range, rmax[0] - rmin[0]);
+  }
 }
 
 __global__ void k_fill_int(int *__restrict__ p, int64_t n, int v) {
@@ -427,13 +456,13 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 size_t ozaki_workspace_bytes(int64_t m, int64_t k, int64_t n, int S) {
   const int64_t tm = (m + kT - 1) / kT, tn = (n + kT - 1) / kT, KB = (k + kK - 1) / kK;
   return align256((size_t)tm * KB * S * kChunk) + align256((size_t)tn * KB * S * kChunk) +
-         align256(sizeof(int) * (size_t)m) + align256(sizeof(int) * (size_t)n);
+         2 * align256(sizeof(int) * (size_t)m) + align256(sizeof(int) * (size_t)n) + 256;
 }
 
 namespace {
 struct OzWork {
   signed char *As, *Bs;
-  int *E, *F;
+  int *E, *Emin, *F, *range;  // range[0] rows of A, range[1] columns of B
   int64_t tm, tn, KB;
 };
 OzWork carve(void *work, int64_t m, int64_t k, int64_t n, int S) {
@@ -441,32 +470,76 @@ OzWork carve(void *work, int64_t m, int64_t k, int64_t n, int S) {
   w.tm = (m + kT - 1) / kT;
   w.tn = (n + kT - 1) / kT;
   w.KB = (k + kK - 1) / kK;
+  // exponent arrays first, at offsets independent of S, then the slices
   char *p = (char *)work;
+  w.E = (int *)p;
+  p += align256(sizeof(int) * (size_t)m);
+  w.Emin = (int *)p;
+  p += align256(sizeof(int) * (size_t)m);
+  w.F = (int *)p;
+  p += align256(sizeof(int) * (size_t)n);
+  w.range = (int *)p;
+  p += 256;
   w.As = (signed char *)p;
   p += align256((size_t)w.tm * w.KB * S * kChunk);
   w.Bs = (signed char *)p;
-  p += align256((size_t)w.tn * w.KB * S * kChunk);
-  w.E = (int *)p;
-  p += align256(sizeof(int) * (size_t)m);
-  w.F = (int *)p;
   return w;
 }
 bool oz_args_ok(int64_t k, int S) { return S >= 8 && S <= ozaki_max_slices() && k <= 32768; }
 }  // namespace
 
-hipError_t ozaki_prepare(const double2 *A, const double2 *B, int64_t m, int64_t k, int64_t n,
-                         int S, void *work, size_t work_bytes, hipStream_t s) {
+hipError_t ozaki_scan(const double2 *A, const double2 *B, int64_t m, int64_t k, int64_t n,
+                      int S, void *work, size_t work_bytes, hipStream_t s) {
   if (m <= 0 || n <= 0 || k <= 0) return hipSuccess;
   if (!oz_args_ok(k, S) || work_bytes < ozaki_workspace_bytes(m, k, n, S)) return hipErrorInvalidValue;
   OzWork w = carve(work, m, k, n, S);
   hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, w.E, m, kEmpty);
+  hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, w.Emin, m, kNonFinite);
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, s, w.range, (int64_t)2, 0);
   const int64_t kchunk = 64;
   hipLaunchKernelGGL(k_row_exp, dim3((unsigned)((m + kThreads - 1) / kThreads), (unsigned)((k + kchunk - 1) / kchunk)),
-                     dim3(kThreads), 0, s, A, m, k, kchunk, w.E);
-  hipLaunchKernelGGL(k_col_exp, dim3((unsigned)n), dim3(kThreads), 0, s, B, k, w.F);
+                     dim3(kThreads), 0, s, A, m, k, kchunk, w.E, w.Emin);
+  hipLaunchKernelGGL(k_row_range, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, w.E, w.Emin, m, w.range);
+  hipLaunchKernelGGL(k_col_exp, dim3((unsigned)n), dim3(kThreads), 0, s, B, k, w.F, w.range + 1);
+  return hipGetLastError();
+}
+
+hipError_t ozaki_read_ranges(void *work, int64_t m, int64_t k, int64_t n, int S, int *ranges,
+                             hipStream_t s) {
+  OzWork w = carve(work, m, k, n, S);
+  hipError_t e = hipMemcpyAsync(ranges, w.range, 2 * sizeof(int), hipMemcpyDeviceToHost, s);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(s);
+}
+
+int ozaki_choose_slices(int64_t k, int range_a, int range_b) {
+  // Empirical error model of the triangular truncation (see DESIGN notes in
+  // the accuracy tests): |err_ij| ~ 2^(E_i + F_j - 7S - 1) sqrt(k) while
+  // (|A||B|)_ij / 2^(E_i + F_j) falls below 1 by up to about
+  // (range_a + range_b) * min(1, 16/k) binades. Pick the smallest S with
+  // 2^-101 headroom under 2^-100 (|A||B|)_ij.
+  if (k <= 0) return 8;
+  const double need = 101.0 + 0.5 * log2((double)k) +
+                      (double)(range_a + range_b) * (k >= 16 ? 16.0 / (double)k : 1.0);
+  int S = (int)ceil(need / 7.0);
+  return S < 8 ? 8 : S;
+}
+
+hipError_t ozaki_slice(const double2 *A, const double2 *B, int64_t m, int64_t k, int64_t n, int S,
+                       void *work, size_t work_bytes, hipStream_t s) {
+  if (m <= 0 || n <= 0 || k <= 0) return hipSuccess;
+  if (!oz_args_ok(k, S) || work_bytes < ozaki_workspace_bytes(m, k, n, S)) return hipErrorInvalidValue;
+  OzWork w = carve(work, m, k, n, S);
   hipLaunchKernelGGL(k_slice_a<16>, dim3((unsigned)w.tm, (unsigned)w.KB), dim3(kThreads), 0, s, A, m, k, w.E, S, w.KB, w.As);
   hipLaunchKernelGGL(k_slice_b<16>, dim3((unsigned)w.tn, (unsigned)w.KB), dim3(kThreads), 0, s, B, k, n, w.F, S, w.KB, w.Bs);
   return hipGetLastError();
+}
+
+hipError_t ozaki_prepare(const double2 *A, const double2 *B, int64_t m, int64_t k, int64_t n,
+                         int S, void *work, size_t work_bytes, hipStream_t s) {
+  hipError_t e = ozaki_scan(A, B, m, k, n, S, work, work_bytes, s);
+  if (e != hipSuccess) return e;
+  return ozaki_slice(A, B, m, k, n, S, work, work_bytes, s);
 }
 
 hipError_t ozaki_multiply(double2 *C, int64_t m, int64_t k, int64_t n, int S, void *work,
@@ -489,8 +562,27 @@ hipError_t ozaki_multiply(double2 *C, int64_t m, int64_t k, int64_t n, int S, vo
 }
 
 hipError_t ozaki_gemm(const double2 *A, const double2 *B, double2 *C, int64_t m, int64_t k,
-                      int64_t n, int S, void *work, size_t work_bytes, hipStream_t s) {
-  hipError_t e = ozaki_prepare(A, B, m, k, n, S, work, work_bytes, s);
+                      int64_t n, int S, void *work, size_t work_bytes, hipStream_t s, int *used) {
+  if (used) *used = S;
+  if (m <= 0 || n <= 0) return hipSuccess;
+  if (k <= 0) return hipMemsetAsync(C, 0, sizeof(double2) * m * n, s);
+  const int Sw = S > 0 ? S : ozaki_max_slices();
+  hipError_t e = ozaki_scan(A, B, m, k, n, Sw, work, work_bytes, s);
+  if (e != hipSuccess) return e;
+  if (S <= 0) {
+    int r[2];
+    e = ozaki_read_ranges(work, m, k, n, Sw, r, s);
+    if (e != hipSuccess) return e;
+    S = ozaki_choose_slices(k, r[0], r[1]);
+    if (used) *used = S > ozaki_max_slices() ? 0 : S;
+    // More slices than the fused kernel holds: the reference-order GEMM
+    // (FP64 VALU, bit-identical to the reference) is both exact enough and,
+    // at the small k that needs it, cheap.
+    if (S > ozaki_max_slices()) return gemm_plain(A, B, C, m, k, n, 8, -1, s);
+    e = ozaki_slice(A, B, m, k, n, S, work, work_bytes, s);
+  } else {
+    e = ozaki_slice(A, B, m, k, n, S, work, work_bytes, s);
+  }
   if (e != hipSuccess) return e;
   return ozaki_multiply(C, m, k, n, S, work, work_bytes, s);
 }

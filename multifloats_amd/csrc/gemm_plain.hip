@@ -22,12 +22,16 @@ namespace {
 
 constexpr int kThreads = 256;
 
-template <int TM, int TN, int BK, int WPE>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WPE))) void k_gemm_plain(const double2 *__restrict__ A,
-                                                         const double2 *__restrict__ B,
-                                                         double2 *__restrict__ C, int64_t m,
-                                                         int64_t k, int64_t n, int64_t ri,
-                                                         int tiles_m) {
+// R > 0: the renorm interval ri equals R and divides BK, so in every K block
+// that is not the last one the renormalisation points sit at the static
+// steps (kk+1) % R == 0 and the block is unrolled without branches (the
+// per-step branch otherwise costs a register copy per accumulator at each
+// join). R == 0: ri is arbitrary and checked per step.
+template <int TM, int TN, int BK, int WPE, int R>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WPE))) void k_gemm_plain(
+    const double2 *__restrict__ A, const double2 *__restrict__ B, double2 *__restrict__ C,
+    int64_t m, int64_t k, int64_t n, int64_t ri, int tiles_m) {
+  static_assert(R == 0 || BK % R == 0, "static renorm interval must divide the K block");
   constexpr int BM = 16 * TM, BN = 16 * TN;
   constexpr int A_PER = BM * BK / kThreads;
   constexpr int B_PER = BN * BK / kThreads;
@@ -86,7 +90,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WPE)))
   int64_t until = rint;  // MACs left before the next renormalisation point
   int64_t done = 0;
 
-  auto step = [&](int kk) {
+  auto macs = [&](int kk) {
     double ah[TM], al[TM], bh[TN], bl[TN];
 #pragma unroll
     for (int i = 0; i < TM; ++i) {
@@ -102,15 +106,19 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WPE)))
     for (int i = 0; i < TM; ++i)
 #pragma unroll
       for (int j = 0; j < TN; ++j) mac(ah[i], al[i], bh[j], bl[j], sh[i][j], sl[i][j]);
+  };
+  auto renorm_all = [&]() {
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) renorm(sh[i][j], sl[i][j]);
+  };
+  auto step = [&](int kk) {
+    macs(kk);
     ++done;
     if (rint && --until == 0) {
       until = rint;
-      if (done < k) {
-#pragma unroll
-        for (int i = 0; i < TM; ++i)
-#pragma unroll
-          for (int j = 0; j < TN; ++j) renorm(sh[i][j], sl[i][j]);
-      }
+      if (done < k) renorm_all();
     }
   };
 
@@ -121,7 +129,17 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WPE)))
     __syncthreads();
     if (k0 + BK < k) load(k0 + BK);
     const int64_t kmax = (k - k0) < BK ? (k - k0) : BK;
-    if (kmax == BK) {
+    if (R > 0 && rint == R && k0 + BK < k) {
+      // static renormalisation points; `until` stays in phase since R | BK
+      constexpr int RR = R > 0 ? R : 1;
+#pragma unroll 1
+      for (int c = 0; c < BK; c += RR) {
+#pragma unroll
+        for (int kk = 0; kk < RR; ++kk) macs(c + kk);
+        renorm_all();
+      }
+      done += BK;
+    } else if (kmax == BK) {
 #pragma unroll
       for (int kk = 0; kk < BK; ++kk) step(kk);
     } else {
@@ -218,8 +236,13 @@ hipError_t launch_gemm(const double2 *A, const double2 *B, double2 *C, int64_t m
   constexpr int BM = 16 * TM, BN = 16 * TN;
   const int64_t tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
   if (tm * tn > 0x7fffffff) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((k_gemm_plain<TM, TN, BK, WPE>), dim3((unsigned)(tm * tn)), dim3(kThreads), 0, s,
-                     A, B, C, m, k, n, ri, (int)tm);
+  const bool r8 = ri == 8 && k > 8;
+  if (r8)
+    hipLaunchKernelGGL((k_gemm_plain<TM, TN, BK, WPE, 8>), dim3((unsigned)(tm * tn)), dim3(kThreads), 0,
+                       s, A, B, C, m, k, n, ri, (int)tm);
+  else
+    hipLaunchKernelGGL((k_gemm_plain<TM, TN, BK, WPE, 0>), dim3((unsigned)(tm * tn)), dim3(kThreads), 0,
+                       s, A, B, C, m, k, n, ri, (int)tm);
   return hipGetLastError();
 }
 

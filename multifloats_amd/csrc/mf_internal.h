@@ -35,8 +35,12 @@ hipError_t reduce(int kind, const double2 *a, const double2 *b, int64_t n, int64
 // Ozaki split-precision GEMM with S int8 slices per operand (8 <= S <= 16).
 int ozaki_max_slices();
 size_t ozaki_workspace_bytes(int64_t m, int64_t k, int64_t n, int S);
+// S <= 0 chooses the slice count from the data (synchronises the stream to
+// read the exponent ranges) and falls back to gemm_plain when the choice
+// exceeds ozaki_max_slices(); *used receives the count (0 = plain fallback).
 hipError_t ozaki_gemm(const double2 *A, const double2 *B, double2 *C, int64_t m, int64_t k,
-                      int64_t n, int S, void *work, size_t work_bytes, hipStream_t s);
+                      int64_t n, int S, void *work, size_t work_bytes, hipStream_t s, int *used);
+int ozaki_choose_slices(int64_t k, int range_a, int range_b);
 // The two halves of ozaki_gemm: scaling + slicing into `work`, then the
 // INT8 MFMA products and DD recombination from `work` into C.
 hipError_t ozaki_prepare(const double2 *A, const double2 *B, int64_t m, int64_t k, int64_t n,

@@ -23,16 +23,20 @@ def all_err(A, B, C):
     return nt.gemm_err(A, B, C, rows.ravel(), cols.ravel())
 
 
-@pytest.mark.parametrize("shape", [(1, 1, 1), (64, 32, 64), (70, 45, 130), (200, 300, 100), (5, 1000, 3)])
+@pytest.mark.parametrize("shape", [(1, 1, 1), (64, 32, 64), (70, 45, 130), (200, 300, 100), (5, 1000, 3),
+                                   (64, 128, 64), (100, 2048, 90)])
 @pytest.mark.parametrize("spread", [0, 10, 20, 30])
 def test_ozaki_error_bound(gpu, shape, spread):
+    """Automatic slice count: the componentwise 2^-100 target on every output."""
     m, k, n = shape
     rng = np.random.default_rng(m + 7 * k + 13 * n + spread)
     A, B = mats(rng, m, k, n, spread)
-    C = mf.matmul_ozaki(A, B, 16)
+    C = mf.matmul_ozaki(A, B, 0)
+    used = mf.ozaki_slices_used()
+    assert used == 0 or 8 <= used <= 16
     err, mag = all_err(A, B, C)
     ratio = err / mag
-    assert ratio.max() <= BOUND, (ratio.max(), np.log2(ratio.max()))
+    assert ratio.max() <= BOUND, (used, ratio.max(), np.log2(ratio.max()))
 
 
 def test_ozaki_error_falls_with_slices(gpu):
@@ -57,6 +61,7 @@ def test_ozaki_special_rows_and_columns(gpu):
     B[:, 9, :] = 0.0            # zero column
     B[11, 20, 0] = np.nan       # column with NaN
     C = mf.matmul_ozaki(datagen.as_colmajor(A), datagen.as_colmajor(B), 16)
+    assert mf.ozaki_slices_used() == 16
     assert not np.delete(C[3], 20, axis=0).any() and np.isnan(C[3, 20, 0])
     assert np.isnan(C[5, :, 0]).all()
     assert not C[np.arange(70) != 5, 9].any()
@@ -69,8 +74,14 @@ def test_ozaki_special_rows_and_columns(gpu):
     assert (err <= BOUND * mag).all()
 
 
+def test_slice_rule():
+    assert mf.ozaki_choose_slices(4096, 40, 40) == 16
+    assert mf.ozaki_choose_slices(8192, 60, 60) == 16
+    assert mf.ozaki_choose_slices(32, 20, 20) > 16  # runs the reference-order GEMM
+
+
 def test_ozaki_empty_inner_dimension(gpu):
-    C = mf.matmul_ozaki(np.zeros((4, 0, 2)), np.zeros((0, 3, 2)), 16)
+    C = mf.matmul_ozaki(np.zeros((4, 0, 2)), np.zeros((0, 3, 2)), 0)
     assert C.shape == (4, 3, 2) and not C.any()
 
 
@@ -83,11 +94,12 @@ def test_ozaki_4096_sampled(gpu):
     dA = torch.from_numpy(datagen.flat(A)).cuda()
     dB = torch.from_numpy(datagen.flat(B)).cuda()
     dC = torch.empty((m * n, 2), dtype=torch.float64, device="cuda")
-    wb = mf.ozaki_workspace_bytes(m, k, n, 16)
+    wb = mf.ozaki_workspace_bytes(m, k, n, 0)
     work = torch.empty(wb, dtype=torch.uint8, device="cuda")
     s = torch.cuda.current_stream().cuda_stream
-    mf.dev_matmul_ozaki(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), m, k, n, 16, work.data_ptr(), wb, s)
+    mf.dev_matmul_ozaki(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), m, k, n, 0, work.data_ptr(), wb, s)
     torch.cuda.synchronize()
+    assert mf.ozaki_slices_used() == 16
     C = datagen.unflat(dC.cpu().numpy(), m, n)
     rows = rng.choice(m, 24, replace=False)
     cols = rng.choice(n, 24, replace=False)
