@@ -166,15 +166,21 @@ MF_HD ddv sin_kernel(ddv x) { return dd_mul(estrin12(dd_mul(x, x), mf_sinc_hi, m
 MF_HD ddv cos_kernel(ddv x) { return estrin12(dd_mul(x, x), mf_cosc_hi, mf_cosc_lo); }
 
 // want_cos = 0: sin(r); 1: cos(r), for |r| <= pi/4.
+// Every lane evaluates the sin and the cos Taylor kernel exactly once, on r
+// itself when |r| <= pi/8 and on r -+ pi/4 otherwise, and selects afterwards:
+// the same operations per lane as the reference's branches (so the same
+// bits), but a wavefront whose lanes take different branches runs two
+// polynomial evaluations instead of up to four.
 MF_HD ddv sincos_eval(ddv r, int want_cos) {
   const double pi8 = 0.392699081698724;
-  if (__builtin_fabs(r.hi) <= pi8) return want_cos ? cos_kernel(r) : sin_kernel(r);
+  const bool small = __builtin_fabs(r.hi) <= pi8;
+  const bool pos = r.hi > 0.0;
   ddv pi4 = mk(mf_pi_quarter_hi, mf_pi_quarter_lo);
-  ddv is2 = mk(mf_inv_sqrt2_hi, mf_inv_sqrt2_lo);
-  bool pos = r.hi > 0.0;
-  ddv rp = pos ? dd_sub(r, pi4) : dd_add(r, pi4);
+  ddv rp = small ? r : (pos ? dd_sub(r, pi4) : dd_add(r, pi4));
   ddv sk = sin_kernel(rp);
   ddv ck = cos_kernel(rp);
+  if (small) return want_cos ? ck : sk;
+  ddv is2 = mk(mf_inv_sqrt2_hi, mf_inv_sqrt2_lo);
   ddv comb;
   if (!want_cos) comb = pos ? dd_add(sk, ck) : dd_sub(sk, ck);
   else comb = pos ? dd_sub(ck, sk) : dd_add(ck, sk);
