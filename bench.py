@@ -45,7 +45,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=2)
-    p.add_argument("--workload", default="gemm", choices=["gemm", "elementwise", "reduce"])
+    p.add_argument("--workload", default="gemm", choices=["gemm", "cgemm", "elementwise", "reduce"])
     p.add_argument("--size", type=int, default=4096)
     p.add_argument("--spread", type=float, default=20.0)
     p.add_argument("--slices", type=int, default=0, help="0 = chosen from the data")
@@ -197,6 +197,32 @@ def bench_gemm(args, torch, mf):
     return out
 
 
+def bench_cgemm(args, torch, mf):
+    """complex64x2 GEMM m = n = k = 2048 (4 real chains per output, fused)."""
+    from multifloats_amd import datagen
+    m = n = k = 2048 if args.size == 4096 else args.size
+    rng = np.random.default_rng(3)
+
+    def cm(r, c):
+        re, im = datagen.matrix(rng, r, c, args.spread), datagen.matrix(rng, r, c, args.spread)
+        return np.ascontiguousarray(np.concatenate([re, im], axis=2).transpose(1, 0, 2)).reshape(-1, 4)
+    dA = torch.from_numpy(cm(m, k)).cuda()
+    dB = torch.from_numpy(cm(k, n)).cuda()
+    dC = torch.empty((m * n, 4), dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    ms = timed(torch, lambda: mf.dev_cmatmul(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), m, k, n, 8, stream),
+               args.steps, args.warmup)
+    flops = 8.0 * m * n * k
+    achieved = 4 * FP64_FLOPS_PER_DDMAC * m * n * k / (ms * 1e-3) / 1e12
+    return {"metric": "complex DD-GEMM DD-GFLOP/s", "value": round(flops / (ms * 1e-3) / 1e9, 2),
+            "unit": "DD-GFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(ms, 4), "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "complex64x2 GEMM m=n=k=%d, spread 2^+-%g, renorm 8, bit-identical to the "
+                                   "reference's four-GEMM composition" % (m, args.spread)},
+            "roofline": {"bound": "fp64", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None}}
+
+
 def bench_elementwise(args, torch, mf):
     from multifloats_amd import datagen
     n = 1 << 26
@@ -261,7 +287,8 @@ def main():
     import multifloats_amd as mf
     if not torch.cuda.is_available():
         sys.exit("bench.py: no GPU visible")
-    fn = {"gemm": bench_gemm, "elementwise": bench_elementwise, "reduce": bench_reduce}[args.workload]
+    fn = {"gemm": bench_gemm, "cgemm": bench_cgemm, "elementwise": bench_elementwise,
+          "reduce": bench_reduce}[args.workload]
     print(json.dumps(fn(args, torch, mf)), flush=True)
 
 

@@ -93,6 +93,19 @@ MULTIFLOATS_API void matmulf64x2_mv(const float64x2 *a, const float64x2 *x, floa
 MULTIFLOATS_API void matmulf64x2_vm(const float64x2 *x, const float64x2 *b, float64x2 *y,
                                     int64_t k, int64_t n, int64_t renorm_interval);
 
+/* Complex C(m,n) = A(m,k) * B(k,n) on complex64x2 (re then im), the Fortran
+ * matmul on cmplx64x2 (cdd_matmul_mm, fsrc/multifloats.fypp:2609-2621, which
+ * has no C symbol in the reference). Bit-identical to that composition:
+ * re = mm(a_re,b_re) - mm(a_im,b_im), im = mm(a_re,b_im) + mm(a_im,b_re),
+ * each mm a reference-order chain with the given renorm interval. */
+MULTIFLOATS_API void cmatmuldd_mm(const complex64x2 *a, const complex64x2 *b, complex64x2 *c,
+                                  int64_t m, int64_t k, int64_t n, int64_t renorm_interval);
+/* cdd_matmul_mv / cdd_matmul_vm (fypp:2623-2649). */
+MULTIFLOATS_API void cmatmuldd_mv(const complex64x2 *a, const complex64x2 *x, complex64x2 *y,
+                                  int64_t m, int64_t k, int64_t renorm_interval);
+MULTIFLOATS_API void cmatmuldd_vm(const complex64x2 *x, const complex64x2 *b, complex64x2 *y,
+                                  int64_t k, int64_t n, int64_t renorm_interval);
+
 /* ---- 2. array entry points -------------------------------------------- */
 
 /* c[i] = a[i] op b[i], bit-identical to the reference operators
@@ -154,6 +167,9 @@ MULTIFLOATS_API int mfhip_matmul_variants(void);
 MULTIFLOATS_API int mfhip_matmul_mm_variant(const float64x2 *a, const float64x2 *b, float64x2 *c,
                                             int64_t m, int64_t k, int64_t n,
                                             int64_t renorm_interval, int variant, void *stream);
+MULTIFLOATS_API int mfhip_cmatmul_mm(const complex64x2 *a, const complex64x2 *b, complex64x2 *c,
+                                     int64_t m, int64_t k, int64_t n, int64_t renorm_interval,
+                                     void *stream);
 MULTIFLOATS_API int mfhip_matmul_mv(const float64x2 *a, const float64x2 *x, float64x2 *y,
                                     int64_t m, int64_t k, int64_t renorm_interval, void *stream);
 MULTIFLOATS_API int mfhip_matmul_vm(const float64x2 *x, const float64x2 *b, float64x2 *y,

@@ -84,6 +84,10 @@ def lib():
         L.mfhip_ozaki_workspace_bytes.argtypes = [I64, I64, I64, I]
         L.mfhip_ozaki_workspace_bytes.restype = ctypes.c_size_t
         L.mfhip_ozaki_matmul_mm.argtypes = [P, P, P, I64, I64, I64, I, P, ctypes.c_size_t, VP]
+        L.cmatmuldd_mm.argtypes = [P, P, P, I64, I64, I64, I64]
+        L.cmatmuldd_mv.argtypes = [P, P, P, I64, I64, I64]
+        L.cmatmuldd_vm.argtypes = [P, P, P, I64, I64, I64]
+        L.mfhip_cmatmul_mm.argtypes = [P, P, P, I64, I64, I64, I64, VP]
         L.mfhip_ozaki_slices_used.argtypes = []
         L.mfhip_ozaki_choose_slices.argtypes = [I64, I, I]
         L.mfhip_ozaki_prepare.argtypes = [P, P, I64, I64, I64, I, P, ctypes.c_size_t, VP]
@@ -152,6 +156,48 @@ def matmul(a, b, renorm_interval=DD_FMA_RENORM_INTERVAL):
         raise ValueError("matmul: need (matrix, matrix), (matrix, vector) or (vector, matrix)")
     _check_status()
     return c
+
+
+def _cmat(a):
+    """(m, n, 4) complex DD matrix {re.hi, re.lo, im.hi, im.lo}, column-major."""
+    a = np.asarray(a, dtype=np.float64)
+    if a.ndim != 3 or a.shape[2] != 4:
+        raise ValueError("complex DD matrices are (m, n, 4) arrays")
+    return np.ascontiguousarray(a.transpose(1, 0, 2)).transpose(1, 0, 2)
+
+
+def cmatmul(a, b, renorm_interval=DD_FMA_RENORM_INTERVAL):
+    """Fortran matmul on cmplx64x2 (cdd_matmul_mm/mv/vm, fypp:2609-2649).
+
+    Matrices are (m, n, 4) column-major; vectors (k, 4)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    if a.ndim == 3 and b.ndim == 3:
+        m, k = a.shape[:2]
+        n = b.shape[1]
+        if b.shape[0] != k:
+            raise ValueError("cmatmul: inner dimensions differ")
+        a, b = _cmat(a), _cmat(b)
+        c = _cmat(np.zeros((m, n, 4)))
+        lib().cmatmuldd_mm(_ptr(a), _ptr(b), _ptr(c), m, k, n, renorm_interval)
+    elif a.ndim == 3 and b.ndim == 2:
+        m, k = a.shape[:2]
+        a, b = _cmat(a), np.ascontiguousarray(b)
+        c = np.zeros((m, 4))
+        lib().cmatmuldd_mv(_ptr(a), _ptr(b), _ptr(c), m, k, renorm_interval)
+    elif a.ndim == 2 and b.ndim == 3:
+        k, n = b.shape[:2]
+        a, b = np.ascontiguousarray(a), _cmat(b)
+        c = np.zeros((n, 4))
+        lib().cmatmuldd_vm(_ptr(a), _ptr(b), _ptr(c), k, n, renorm_interval)
+    else:
+        raise ValueError("cmatmul: need (matrix, matrix), (matrix, vector) or (vector, matrix)")
+    _check_status()
+    return c
+
+
+def dev_cmatmul(a, b, c, m, k, n, renorm_interval=DD_FMA_RENORM_INTERVAL, stream=0):
+    _dev(lib().mfhip_cmatmul_mm(a, b, c, m, k, n, renorm_interval, stream))
 
 
 def matmul_ozaki(a, b, slices=0):

@@ -14,6 +14,7 @@
 #include "../../include/multifloats_hip.h"
 #include "mf_internal.h"
 
+using multifloats::complex64x2;
 using multifloats::float64x2;
 
 namespace {
@@ -183,7 +184,32 @@ hipError_t host_reduce(int kind, const float64x2 *a, const float64x2 *b, int64_t
 }
 
 thread_local int tl_oz_used = 0;
-// workspace is always sized for the largest slice count the call may use
+
+hipError_t host_cmm(const complex64x2 *a, const complex64x2 *b, complex64x2 *c, int64_t m,
+                    int64_t k, int64_t n, int64_t ri) {
+  tl_err = 0;
+  if (m <= 0 || n <= 0) return hipSuccess;
+  TRY(stage_init());
+  const size_t kk = k > 0 ? (size_t)k : 0;
+  void *da, *db, *dc;
+  TRY(stage_buf(0, 32 * (size_t)m * kk, &da));
+  TRY(stage_buf(1, 32 * kk * (size_t)n, &db));
+  TRY(stage_buf(2, 32 * (size_t)m * n, &dc));
+  TRY(h2d(da, a, 32 * (size_t)m * kk));
+  TRY(h2d(db, b, 32 * kk * (size_t)n));
+  TRY(mfhip::gemm_cplx((const double4 *)da, (const double4 *)db, (double4 *)dc, m, k, n, ri,
+                       tl_stage.stream));
+  TRY(d2h(c, dc, 32 * (size_t)m * n));
+  return hipStreamSynchronize(tl_stage.stream);
+}
+
+void nan_fill_c(complex64x2 *out, int64_t n) {
+  if (!out) return;
+  for (int64_t i = 0; i < n; ++i)
+    out[i].re.limbs[0] = out[i].re.limbs[1] = out[i].im.limbs[0] = out[i].im.limbs[1] = NAN;
+}
+
+// the Ozaki workspace is always sized for the largest slice count a call may use
 int oz_ws_slices(int s) { return s <= 0 ? mfhip::ozaki_max_slices() : s; }
 
 hipError_t host_mm_ozaki(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t m,
@@ -247,6 +273,30 @@ void matmuldd_mm_ozaki(const float64x2 *a, const float64x2 *b, float64x2 *c, int
   if (e != hipSuccess) {
     report("matmuldd_mm_ozaki", e);
     nan_fill(c, (m > 0 && n > 0) ? m * n : 0);
+  }
+}
+void cmatmuldd_mm(const complex64x2 *a, const complex64x2 *b, complex64x2 *c, int64_t m,
+                  int64_t k, int64_t n, int64_t ri) {
+  hipError_t e = host_cmm(a, b, c, m, k, n, ri);
+  if (e != hipSuccess) {
+    report("cmatmuldd_mm", e);
+    nan_fill_c(c, (m > 0 && n > 0) ? m * n : 0);
+  }
+}
+void cmatmuldd_mv(const complex64x2 *a, const complex64x2 *x, complex64x2 *y, int64_t m,
+                  int64_t k, int64_t ri) {
+  hipError_t e = host_cmm(a, x, y, m, k, 1, ri);
+  if (e != hipSuccess) {
+    report("cmatmuldd_mv", e);
+    nan_fill_c(y, m > 0 ? m : 0);
+  }
+}
+void cmatmuldd_vm(const complex64x2 *x, const complex64x2 *b, complex64x2 *y, int64_t k,
+                  int64_t n, int64_t ri) {
+  hipError_t e = host_cmm(x, b, y, 1, k, n, ri);
+  if (e != hipSuccess) {
+    report("cmatmuldd_vm", e);
+    nan_fill_c(y, n > 0 ? n : 0);
   }
 }
 void matmulf64x2_mm(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t m, int64_t k,
@@ -341,6 +391,12 @@ int mfhip_matmul_mm_variant(const float64x2 *a, const float64x2 *b, float64x2 *c
 int mfhip_matmul_mm(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t m, int64_t k,
                     int64_t n, int64_t ri, void *stream) {
   return mfhip_matmul_mm_variant(a, b, c, m, k, n, ri, -1, stream);
+}
+int mfhip_cmatmul_mm(const complex64x2 *a, const complex64x2 *b, complex64x2 *c, int64_t m,
+                     int64_t k, int64_t n, int64_t ri, void *stream) {
+  CHECK_ALIGN(a, b, c);
+  return status(mfhip::gemm_cplx((const double4 *)a, (const double4 *)b, (double4 *)c, m, k, n, ri,
+                                 (hipStream_t)stream));
 }
 int mfhip_matmul_mv(const float64x2 *a, const float64x2 *x, float64x2 *y, int64_t m, int64_t k,
                     int64_t ri, void *stream) {

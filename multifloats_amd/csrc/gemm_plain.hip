@@ -230,6 +230,153 @@ __global__ __launch_bounds__(kThreads) void k_gevm(const double2 *__restrict__ x
   }
 }
 
+// Complex C(m,n) = A(m,k) B(k,n) on complex64x2 {re.hi, re.lo, im.hi, im.lo}.
+// The reference forms it from four real GEMMs and the DD - and + operators
+// (fsrc/multifloats.fypp:2609-2621):
+//   re = mm(a_re, b_re) - mm(a_im, b_im),  im = mm(a_re, b_im) + mm(a_im, b_re).
+// Here one pass runs those four sequential chains per output side by side
+// (same MAC operand order, same renormalisation schedule), then applies the
+// same subtraction and addition: identical bits, inputs read once, no
+// temporaries. mv and vm are the n = 1 and m = 1 cases (fypp:2623-2649).
+template <int TM, int TN, int BK>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_gemm_cplx(
+    const double4 *__restrict__ A, const double4 *__restrict__ B, double4 *__restrict__ C,
+    int64_t m, int64_t k, int64_t n, int64_t ri, int tiles_m) {
+  constexpr int BM = 16 * TM, BN = 16 * TN;
+  constexpr int A_PER = BM * BK / kThreads;
+  constexpr int B_PER = BN * BK / kThreads;
+  static_assert(A_PER * kThreads == BM * BK && B_PER * kThreads == BN * BK, "tile");
+  __shared__ double As[4][BK][BM];  // re.hi, re.lo, im.hi, im.lo
+  __shared__ double Bs[4][BK][BN];
+
+  const int tid = threadIdx.x;
+  const int tx = tid & 15, ty = tid >> 4;
+  const int tile = blockIdx.x;
+  const int64_t m0 = (int64_t)(tile % tiles_m) * BM;
+  const int64_t n0 = (int64_t)(tile / tiles_m) * BN;
+
+  double4 ra[A_PER], rb[B_PER];
+  auto load = [&](int64_t k0) {
+#pragma unroll
+    for (int q = 0; q < A_PER; ++q) {
+      int e = tid + q * kThreads;
+      int r = e % BM, c = e / BM;
+      int64_t gi = m0 + r, gp = k0 + c;
+      ra[q] = (gi < m && gp < k) ? A[gi + gp * m] : make_double4(0.0, 0.0, 0.0, 0.0);
+    }
+#pragma unroll
+    for (int q = 0; q < B_PER; ++q) {
+      int e = tid + q * kThreads;
+      int r = e % BK, c = e / BK;
+      int64_t gp = k0 + r, gj = n0 + c;
+      rb[q] = (gp < k && gj < n) ? B[gp + gj * k] : make_double4(0.0, 0.0, 0.0, 0.0);
+    }
+  };
+  auto stash = [&]() {
+#pragma unroll
+    for (int q = 0; q < A_PER; ++q) {
+      int e = tid + q * kThreads;
+      int r = e % BM, c = e / BM;
+      As[0][c][r] = ra[q].x;
+      As[1][c][r] = ra[q].y;
+      As[2][c][r] = ra[q].z;
+      As[3][c][r] = ra[q].w;
+    }
+#pragma unroll
+    for (int q = 0; q < B_PER; ++q) {
+      int e = tid + q * kThreads;
+      int r = e % BK, c = e / BK;
+      Bs[0][r][c] = rb[q].x;
+      Bs[1][r][c] = rb[q].y;
+      Bs[2][r][c] = rb[q].z;
+      Bs[3][r][c] = rb[q].w;
+    }
+  };
+
+  // chain 0: ar*br, 1: ai*bi, 2: ar*bi, 3: ai*br
+  double sh[4][TM][TN], sl[4][TM][TN];
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) sh[c][i][j] = sl[c][i][j] = 0.0;
+
+  const int64_t rint = (ri > 0 && k > ri) ? ri : 0;
+  int64_t until = rint, done = 0;
+
+  auto step = [&](int kk) {
+    double arh[TM], arl[TM], aih[TM], ail[TM], brh[TN], brl[TN], bih[TN], bil[TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      arh[i] = As[0][kk][tx * TM + i];
+      arl[i] = As[1][kk][tx * TM + i];
+      aih[i] = As[2][kk][tx * TM + i];
+      ail[i] = As[3][kk][tx * TM + i];
+    }
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      brh[j] = Bs[0][kk][ty * TN + j];
+      brl[j] = Bs[1][kk][ty * TN + j];
+      bih[j] = Bs[2][kk][ty * TN + j];
+      bil[j] = Bs[3][kk][ty * TN + j];
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        mac(arh[i], arl[i], brh[j], brl[j], sh[0][i][j], sl[0][i][j]);
+        mac(aih[i], ail[i], bih[j], bil[j], sh[1][i][j], sl[1][i][j]);
+        mac(arh[i], arl[i], bih[j], bil[j], sh[2][i][j], sl[2][i][j]);
+        mac(aih[i], ail[i], brh[j], brl[j], sh[3][i][j], sl[3][i][j]);
+      }
+    ++done;
+    if (rint && --until == 0) {
+      until = rint;
+      if (done < k) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+          for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j) renorm(sh[c][i][j], sl[c][i][j]);
+      }
+    }
+  };
+
+  load(0);
+  for (int64_t k0 = 0; k0 < k; k0 += BK) {
+    __syncthreads();
+    stash();
+    __syncthreads();
+    if (k0 + BK < k) load(k0 + BK);
+    const int kmax = (k - k0) < BK ? (int)(k - k0) : BK;
+#pragma unroll 1
+    for (int kk = 0; kk < kmax; ++kk) step(kk);
+  }
+
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    const int64_t gj = n0 + ty * TN + j;
+    if (gj >= n) continue;
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      const int64_t gi = m0 + tx * TM + i;
+      if (gi >= m) continue;
+      ddv p[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        double h = sh[c][i][j], l = sl[c][i][j];
+        renorm(h, l);
+        p[c] = mk(h, l);
+      }
+      const ddv re = dd_sub(p[0], p[1]);
+      const ddv im = dd_add(p[2], p[3]);
+      C[gi + gj * m] = make_double4(re.hi, re.lo, im.hi, im.lo);
+    }
+  }
+}
+
 template <int TM, int TN, int BK, int WPE>
 hipError_t launch_gemm(const double2 *A, const double2 *B, double2 *C, int64_t m, int64_t k,
                        int64_t n, int64_t ri, hipStream_t s) {
@@ -266,6 +413,18 @@ hipError_t gemm_plain(const double2 *A, const double2 *B, double2 *C, int64_t m,
     case 5: return launch_gemm<8, 4, 16, 1>(A, B, C, m, k, n, ri, s);
     default: return launch_gemm<4, 8, 16, 2>(A, B, C, m, k, n, ri, s);
   }
+}
+
+hipError_t gemm_cplx(const double4 *A, const double4 *B, double4 *C, int64_t m, int64_t k,
+                     int64_t n, int64_t ri, hipStream_t s) {
+  if (m <= 0 || n <= 0) return hipSuccess;
+  if (k <= 0) return hipMemsetAsync(C, 0, sizeof(double4) * m * n, s);
+  constexpr int TM = 2, TN = 2, BK = 16;
+  const int64_t tm = (m + 16 * TM - 1) / (16 * TM), tn = (n + 16 * TN - 1) / (16 * TN);
+  if (tm * tn > 0x7fffffff) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_gemm_cplx<TM, TN, BK>), dim3((unsigned)(tm * tn)), dim3(kThreads), 0, s, A, B,
+                     C, m, k, n, ri, (int)tm);
+  return hipGetLastError();
 }
 
 hipError_t gemv(const double2 *A, const double2 *x, double2 *y, int64_t m, int64_t k, int64_t ri,

@@ -19,6 +19,10 @@ enum { RED_DOT = 0, RED_SUM = 1, RED_NORM2 = 2 };
 int gemm_plain_variants();
 hipError_t gemm_plain(const double2 *A, const double2 *B, double2 *C, int64_t m, int64_t k,
                       int64_t n, int64_t ri, int variant, hipStream_t s);
+// Complex GEMM (complex64x2 as double4), bit-identical to the reference's
+// four-real-GEMM composition; also serves mv (n = 1) and vm (m = 1).
+hipError_t gemm_cplx(const double4 *A, const double4 *B, double4 *C, int64_t m, int64_t k,
+                     int64_t n, int64_t ri, hipStream_t s);
 hipError_t gemv(const double2 *A, const double2 *x, double2 *y, int64_t m, int64_t k, int64_t ri,
                 hipStream_t s);
 hipError_t gevm(const double2 *x, const double2 *B, double2 *y, int64_t k, int64_t n, int64_t ri,

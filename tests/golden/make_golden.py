@@ -188,13 +188,42 @@ def matmul(L, rng):
     np.savez_compressed(os.path.join(HERE, "matmul.npz"), **out)
 
 
+def cmatmul(L, rng):
+    """Complex products through the reference's own composition
+    (fsrc/multifloats.fypp:2609-2621): four matmuldd_mm calls on the split
+    real/imaginary parts, then subdd / adddd elementwise."""
+    out = {}
+    shapes = [(1, 1, 1), (7, 5, 3), (33, 17, 31), (16, 64, 8), (1, 40, 9), (9, 40, 1)]
+    idx = 0
+    for (m, k, n) in shapes:
+        for ri in (0, 8):
+            Ar, Ai = datagen.matrix(rng, m, k, 10), datagen.matrix(rng, m, k, 10)
+            Br, Bi = datagen.matrix(rng, k, n, 10), datagen.matrix(rng, k, n, 10)
+
+            def mm(X, Y):
+                Z = datagen.as_colmajor(np.zeros((m, n, 2)))
+                L.matmuldd_mm(X.ctypes.data, Y.ctypes.data, Z.ctypes.data, m, k, n, ri)
+                return datagen.flat(Z)
+            re = call2(L, "subdd", mm(Ar, Br), mm(Ai, Bi))
+            im = call2(L, "adddd", mm(Ar, Bi), mm(Ai, Br))
+            key = "z%03d" % idx
+            out[key + "_shape"] = np.array([m, k, n, ri])
+            out[key + "_A"] = np.concatenate([datagen.flat(Ar), datagen.flat(Ai)], axis=1)
+            out[key + "_B"] = np.concatenate([datagen.flat(Br), datagen.flat(Bi)], axis=1)
+            out[key + "_C"] = np.concatenate([re, im], axis=1)
+            idx += 1
+    out["count"] = np.array([idx])
+    np.savez_compressed(os.path.join(HERE, "cmatmul.npz"), **out)
+
+
 def main():
     L = load_ref()
     rng = np.random.default_rng(20261203)
     arith(L, rng)
     transc(L, rng)
     matmul(L, rng)
-    for f in ("arith.npz", "transc.npz", "matmul.npz"):
+    cmatmul(L, np.random.default_rng(777))
+    for f in ("arith.npz", "transc.npz", "matmul.npz", "cmatmul.npz"):
         print(f, os.path.getsize(os.path.join(HERE, f)))
 
 

@@ -188,3 +188,49 @@ def test_reductions_deterministic(gpu):
     r1 = mf.dot_product(a, b)
     r2 = mf.dot_product(a, b)
     nt.assert_bits(r1, r2, "dot rerun")
+
+
+def _cunflat(f, m, n):
+    return np.asarray(f).reshape(n, m, 4).transpose(1, 0, 2)
+
+
+def test_complex_matmul_bits_golden(gpu):
+    g = nt.load_golden("cmatmul.npz")
+    for i in range(int(g["count"][0])):
+        key = "z%03d" % i
+        m, k, n, ri = (int(v) for v in g[key + "_shape"])
+        A, B = _cunflat(g[key + "_A"], m, k), _cunflat(g[key + "_B"], k, n)
+        C = mf.cmatmul(A, B, ri)
+        got = np.ascontiguousarray(C.transpose(1, 0, 2)).reshape(-1, 4)
+        nt.assert_bits(got, g[key + "_C"], "cmm " + key)
+        if n == 1:
+            nt.assert_bits(mf.cmatmul(A, B[:, 0, :], ri), g[key + "_C"], "cmv " + key)
+        if m == 1:
+            nt.assert_bits(mf.cmatmul(A[0], B, ri), g[key + "_C"], "cvm " + key)
+
+
+def test_complex_matmul_2048_sampled(torch_cuda):
+    """complex 2048^3 (the at-scale complex workload): sampled outputs equal
+    the oracle's composition bit for bit."""
+    torch = torch_cuda
+    m = k = n = 2048
+    rng = np.random.default_rng(21)
+
+    def cm(r, c):
+        re, im = datagen.matrix(rng, r, c, 20), datagen.matrix(rng, r, c, 20)
+        return np.concatenate([re, im], axis=2)  # (r, c, 4) logical
+    A, B = cm(m, k), cm(k, n)
+    flat = lambda X: np.ascontiguousarray(X.transpose(1, 0, 2)).reshape(-1, 4)
+    dA, dB = to_dev(torch, flat(A)), to_dev(torch, flat(B))
+    dC = torch.full((m * n, 4), float("nan"), dtype=torch.float64, device="cuda")
+    mf.dev_cmatmul(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), m, k, n, 8,
+                   torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    C = _cunflat(dC.cpu().numpy(), m, n)
+    rows = rng.choice(m, 8, replace=False)
+    cols = rng.choice(n, 8, replace=False)
+    As, Bs = flat(A[rows]), flat(B[:, cols])
+    want = np.zeros((8 * 8, 4))
+    nt.oracle().or_cmatmul_mm(nt.ptr(As), nt.ptr(Bs), nt.ptr(want), 8, k, 8, 8)
+    got = np.ascontiguousarray(C[np.ix_(rows, cols)].transpose(1, 0, 2)).reshape(-1, 4)
+    nt.assert_bits(got, want, "complex 2048")

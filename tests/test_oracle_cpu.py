@@ -171,3 +171,17 @@ def test_oracle_gemm_closed_forms():
     Bi[..., 0] = rng.integers(-9, 10, (5, 4))
     C = nt.or_matmul(Ai, Bi, 0)
     assert np.array_equal(C[..., 0], Ai[..., 0] @ Bi[..., 0]) and not C[..., 1].any()
+
+
+def test_oracle_complex_matmul_matches_reference_composition():
+    """The oracle's complex GEMM equals the reference's four-matmuldd_mm +
+    subdd/adddd composition (fixtures from the compiled reference)."""
+    g = nt.load_golden("cmatmul.npz")
+    for i in range(int(g["count"][0])):
+        key = "z%03d" % i
+        m, k, n, ri = (int(v) for v in g[key + "_shape"])
+        A = np.ascontiguousarray(g[key + "_A"])  # (m*k, 4) column-major complex
+        B = np.ascontiguousarray(g[key + "_B"])
+        C = np.zeros((m * n, 4))
+        nt.oracle().or_cmatmul_mm(nt.ptr(A), nt.ptr(B), nt.ptr(C), m, k, n, ri)
+        nt.assert_bits(C, g[key + "_C"], "cmm " + key)
