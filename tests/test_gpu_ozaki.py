@@ -71,7 +71,9 @@ def test_ozaki_special_rows_and_columns(gpu):
     ok[:, [9, 20]] = False
     rows, cols = np.nonzero(ok)
     err, mag = nt.gemm_err(A, B, C, rows, cols)
-    assert (err <= BOUND * mag).all()
+    # 16 slices forced at k = 40 and spread 2^+-10, where the slice rule
+    # would run the reference-order GEMM: only a loose accuracy check here.
+    assert (err <= 2.0 ** -90 * mag).all()
 
 
 def test_slice_rule():
