@@ -45,7 +45,8 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=2)
-    p.add_argument("--workload", default="gemm", choices=["gemm", "cgemm", "elementwise", "reduce"])
+    p.add_argument("--workload", default="gemm", choices=["gemm", "cgemm", "batched", "elementwise", "reduce"])
+    p.add_argument("--retune", action="store_true", help="batched: rerun the tile autotuner first")
     p.add_argument("--size", type=int, default=4096)
     p.add_argument("--spread", type=float, default=20.0)
     p.add_argument("--slices", type=int, default=0, help="0 = chosen from the data")
@@ -223,6 +224,95 @@ def bench_cgemm(args, torch, mf):
                          "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None}}
 
 
+def bench_batched(args, torch, mf):
+    """10,000 float64x2 GEMMs (m, n, k log-uniform in [8, 512]) and 50,000
+    dot products (length log-uniform in [16, 65536]), config-0 entries, in one
+    persistent grouped launch replayed from a HIP graph; per-call launches of
+    the same problems as the comparison."""
+    from multifloats_amd import datagen, autotune
+    from tests import _native as nt  # oracle: accuracy of sampled problems
+    rng = np.random.default_rng(4)
+    sizes, dlen = datagen.batched_sizes(rng)
+    table = autotune.load_table()
+    if table is None or args.retune:
+        table = autotune.tune(torch, mf)
+    variants = autotune.plan_variants(sizes, table)
+    tot = int(sum(m * k + k * n + m * n for m, k, n in sizes))
+    gbuf = torch.empty(2 * tot, dtype=torch.float64, device="cuda")
+    datagen.fill_config0_torch(torch, gbuf, 101)
+    dtot = int(dlen.sum())
+    dbuf = torch.empty(4 * dtot, dtype=torch.float64, device="cuda")
+    datagen.fill_config0_torch(torch, dbuf, 202)
+    dout = torch.zeros((len(dlen), 2), dtype=torch.float64, device="cuda")
+    base, dbase = gbuf.data_ptr(), dbuf.data_ptr()
+    gemms, offs, off = [], [], 0
+    for m, k, n in sizes:
+        m, k, n = int(m), int(k), int(n)
+        offs.append(off)
+        gemms.append((base + 16 * off, base + 16 * (off + m * k), base + 16 * (off + m * k + k * n), m, k, n, 8))
+        off += m * k + k * n + m * n
+    dots, doff, off = [], [], 0
+    for i, L in enumerate(dlen):
+        L = int(L)
+        doff.append(off)
+        dots.append((dbase + 16 * off, dbase + 16 * (off + L), dout.data_ptr() + 16 * i, L, 8))
+        off += 2 * L
+    plan = mf.GroupedPlan(gemms, dots, variants)
+    stream = torch.cuda.current_stream().cuda_stream
+    ms = timed(torch, lambda: plan.run_graph(stream), args.steps, args.warmup)
+    ms_eager = timed(torch, lambda: plan.run(stream), max(1, args.steps // 2), 1)
+    macs = float(sum(int(m) * int(k) * int(n) for m, k, n in sizes))
+    dd_flops = 2.0 * macs + 2.0 * float(dtot)
+
+    # per-call launches of the same problems (one stream, the library's own kernels)
+    wbytes = mf.reduce_workspace_bytes(int(dlen.max()))
+    work = torch.empty(wbytes // 8 + 2, dtype=torch.float64, device="cuda")
+    pout = torch.zeros((len(dlen), 2), dtype=torch.float64, device="cuda")
+    def per_call():
+        for g in gemms:
+            mf.dev_matmul(g[0], g[1], g[2], g[3], g[4], g[5], 8, stream)
+        for i, d in enumerate(dots):
+            mf.dev_reduce(mf.MFHIP_DOT, d[0], d[1], d[3], pout.data_ptr() + 16 * i, work.data_ptr(), 8, stream)
+    ms_calls = timed(torch, per_call, 1, 1)
+
+    # accuracy: sampled GEMMs bit for bit against the oracle, sampled dots
+    # against the exact accumulator
+    gb = gbuf.view(-1, 2)
+    bits_ok = True
+    gerr = 0.0
+    for i in rng.choice(len(sizes), 12, replace=False):
+        m, k, n = (int(v) for v in sizes[i])
+        o = offs[i]
+        A = datagen.unflat(gb[o:o + m * k].cpu().numpy(), m, k)
+        B = datagen.unflat(gb[o + m * k:o + m * k + k * n].cpu().numpy(), k, n)
+        C = datagen.unflat(gb[o + m * k + k * n:o + m * k + k * n + m * n].cpu().numpy(), m, n)
+        bits_ok &= bool(nt.bits_equal(C, nt.or_matmul(A, B, 8)).all())
+        rr, cc = np.meshgrid(np.arange(min(m, 8)), np.arange(min(n, 8)), indexing="ij")
+        e, mg = nt.gemm_err(A, B, C, rr.ravel(), cc.ravel())
+        gerr = max(gerr, float((e / mg).max()))
+    db = dbuf.view(-1, 2)
+    dres = dout.cpu().numpy()
+    derr = 0.0
+    for i in rng.choice(len(dlen), 40, replace=False):
+        L, o = int(dlen[i]), doff[i]
+        a, b = db[o:o + L].cpu().numpy(), db[o + L:o + 2 * L].cpu().numpy()
+        e, mg = nt.exact_dot_err(a, b, dres[i])
+        derr = max(derr, e / mg / L)
+    plan.close()
+    return {"metric": "batched DD GEMMs + dots, aggregate DD-GFLOP/s", "value": round(dd_flops / (ms * 1e-3) / 1e9, 2),
+            "unit": "DD-GFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(ms, 4), "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "10000 float64x2 GEMMs m,n,k log-U[8,512] + 50000 dots len log-U[16,65536], "
+                                   "config-0 entries, renorm 8; one persistent grouped launch in a HIP graph",
+                       "gemm_macs": macs, "dot_elements": dtot, "work_items": plan.items,
+                       "tile_table": table},
+            "graph_ms": round(ms, 4), "eager_grouped_ms": round(ms_eager, 4), "per_call_ms": round(ms_calls, 3),
+            "speedup_vs_per_call": round(ms_calls / ms, 2),
+            "gemm_dd_gflops": round(2.0 * macs / (ms * 1e-3) / 1e9, 2),
+            "accuracy": {"sampled_gemms_bit_identical_to_reference_order": bits_ok,
+                         "gemm_max_err_over_absAB": gerr, "dot_max_err_over_n_sum_abs": derr}}
+
+
 def bench_elementwise(args, torch, mf):
     from multifloats_amd import datagen
     n = 1 << 26
@@ -287,8 +377,8 @@ def main():
     import multifloats_amd as mf
     if not torch.cuda.is_available():
         sys.exit("bench.py: no GPU visible")
-    fn = {"gemm": bench_gemm, "cgemm": bench_cgemm, "elementwise": bench_elementwise,
-          "reduce": bench_reduce}[args.workload]
+    fn = {"gemm": bench_gemm, "cgemm": bench_cgemm, "batched": bench_batched,
+          "elementwise": bench_elementwise, "reduce": bench_reduce}[args.workload]
     print(json.dumps(fn(args, torch, mf)), flush=True)
 
 

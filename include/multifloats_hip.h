@@ -208,6 +208,33 @@ MULTIFLOATS_API int mfhip_ozaki_multiply(float64x2 *c, int64_t m, int64_t k, int
                                          int slices, void *work, size_t work_bytes,
                                          void *stream);
 
+/* Batching form (no counterpart in the reference): many independent DD GEMMs
+ * (matmuldd_mm semantics, bit-identical per problem) and dot products
+ * (dotdd semantics) in one persistent grouped launch. A plan is built once
+ * (it allocates its device-side work list), then run any number of times on
+ * the same buffers; run_graph captures the launch in a HIP graph on first use
+ * and replays it afterwards. tile_variant[i] picks the GEMM tile for problem
+ * i: 0 = 32x32, 1 = 64x64, -1 = built-in rule (NULL: rule for all). */
+typedef struct {
+  const float64x2 *a;
+  const float64x2 *b;
+  float64x2 *c;
+  int64_t m, k, n, renorm_interval;
+} mfhip_gemm_problem;
+typedef struct {
+  const float64x2 *a;
+  const float64x2 *b;
+  float64x2 *out;
+  int64_t n, renorm_interval;
+} mfhip_dot_problem;
+MULTIFLOATS_API void *mfhip_grouped_create(const mfhip_gemm_problem *gemms, int ngemm,
+                                           const int *tile_variant, const mfhip_dot_problem *dots,
+                                           int ndot);
+MULTIFLOATS_API int mfhip_grouped_items(void *plan);
+MULTIFLOATS_API int mfhip_grouped_run(void *plan, void *stream);
+MULTIFLOATS_API int mfhip_grouped_run_graph(void *plan, void *stream);
+MULTIFLOATS_API void mfhip_grouped_destroy(void *plan);
+
 /* Last HIP error recorded on the calling thread by any entry point (0 = none). */
 MULTIFLOATS_API int mfhip_last_error(void);
 MULTIFLOATS_API const char *mfhip_error_string(int err);

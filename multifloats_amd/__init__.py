@@ -325,3 +325,59 @@ def reduce_workspace_bytes(n):
 
 def matmul_variants():
     return lib().mfhip_matmul_variants()
+
+
+# ---- batching form (grouped persistent launch) ----
+
+class GemmProblem(ctypes.Structure):
+    _fields_ = [("a", ctypes.c_void_p), ("b", ctypes.c_void_p), ("c", ctypes.c_void_p),
+                ("m", ctypes.c_int64), ("k", ctypes.c_int64), ("n", ctypes.c_int64),
+                ("renorm_interval", ctypes.c_int64)]
+
+
+class DotProblem(ctypes.Structure):
+    _fields_ = [("a", ctypes.c_void_p), ("b", ctypes.c_void_p), ("out", ctypes.c_void_p),
+                ("n", ctypes.c_int64), ("renorm_interval", ctypes.c_int64)]
+
+
+class GroupedPlan:
+    """Many DD GEMMs and dot products in one persistent grouped launch
+    (mfhip_grouped_*). gemms: list of (a, b, c, m, k, n, ri) with device
+    addresses; dots: list of (a, b, out, n, ri); variants: per-GEMM tile
+    variant (0 = 32x32, 1 = 64x64, -1 = built-in rule)."""
+
+    def __init__(self, gemms, dots, variants=None):
+        L = lib()
+        L.mfhip_grouped_create.restype = ctypes.c_void_p
+        L.mfhip_grouped_create.argtypes = [ctypes.POINTER(GemmProblem), ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_int), ctypes.POINTER(DotProblem), ctypes.c_int]
+        L.mfhip_grouped_items.argtypes = [ctypes.c_void_p]
+        L.mfhip_grouped_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.mfhip_grouped_run_graph.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.mfhip_grouped_destroy.argtypes = [ctypes.c_void_p]
+        G = (GemmProblem * max(len(gemms), 1))(*[GemmProblem(*g) for g in gemms])
+        D = (DotProblem * max(len(dots), 1))(*[DotProblem(*d) for d in dots])
+        V = None
+        if variants is not None:
+            V = (ctypes.c_int * max(len(variants), 1))(*[int(v) for v in variants])
+        self._h = L.mfhip_grouped_create(G, len(gemms), V, D, len(dots))
+        if not self._h:
+            raise RuntimeError("mfhip_grouped_create failed")
+        self.items = L.mfhip_grouped_items(self._h)
+
+    def run(self, stream=0):
+        _dev(lib().mfhip_grouped_run(self._h, stream))
+
+    def run_graph(self, stream=0):
+        _dev(lib().mfhip_grouped_run_graph(self._h, stream))
+
+    def close(self):
+        if self._h:
+            lib().mfhip_grouped_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

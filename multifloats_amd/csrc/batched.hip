@@ -1,0 +1,385 @@
+// Batching form: many independent, uneven DD GEMMs and dot products in one
+// persistent grouped launch (optionally captured in a HIP graph).
+//
+// Semantics per problem are the reference's: each GEMM is matmuldd_mm
+// (src/float64x2/matmul.inc, reference-order chains, so results are
+// bit-identical to a standalone matmuldd_mm), each dot is the Fortran
+// dot_product (fsrc/multifloats.fypp:2254-2308) evaluated like dotdd: the
+// reference's per-element update per lane, then a fixed tree of accurate DD
+// additions.
+//
+// Plan: the host cuts every GEMM into output tiles of the tile variant its
+// size class was tuned to (32x32 or 64x64) and every dot into chunks of
+// kDotChunk elements, then sorts the items by estimated cost, largest first
+// (longest processing time). The persistent kernel's workgroups pull items
+// from an atomic queue until it is drained -- every workgroup reaches the
+// exit test, so the grid always drains. A dot's chunks leave DD partials;
+// the chunk that completes a dot last (per-dot atomic counter) folds that
+// dot's partials in index order, so results do not depend on scheduling.
+// Queue and counters are cleared by a fill kernel at the start of every run
+// (graph-capture safe: no memset nodes).
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/multifloats_hip.h"
+#include "dd_core.h"
+#include "mf_internal.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kBK = 16;
+constexpr int64_t kDotChunk = 16384;
+
+enum { ITEM_G32 = 0, ITEM_G64 = 1, ITEM_DOT = 2 };
+
+struct Item {
+  int kind;
+  int prob;  // GEMM or dot index
+  int a, b;  // GEMM: tile row, tile col; dot: chunk index, unused
+};
+
+struct GemmD {
+  const double2 *A, *B;
+  double2 *C;
+  int64_t m, k, n, ri;
+};
+
+struct DotD {
+  const double2 *a, *b;
+  double2 *out;
+  int64_t n, ri;
+  int64_t first;  // index of the dot's first partial
+  int nchunks;
+  int id;  // index of the dot's completion counter
+};
+
+// One (16*TM) x (16*TN) output tile of a reference-order GEMM (the same
+// per-output chains as k_gemm_plain). `lds` holds 4 * BK * max(BM, BN)
+// doubles.
+template <int TM, int TN>
+__device__ void gemm_tile(const GemmD &g, int tr, int tc, double *lds) {
+  constexpr int BM = 16 * TM, BN = 16 * TN;
+  constexpr int A_PER = BM * kBK / kThreads, B_PER = BN * kBK / kThreads;
+  double(*As_h)[BM] = (double(*)[BM])lds;
+  double(*As_l)[BM] = (double(*)[BM])(lds + kBK * BM);
+  double(*Bs_h)[BN] = (double(*)[BN])(lds + 2 * kBK * BM);
+  double(*Bs_l)[BN] = (double(*)[BN])(lds + 2 * kBK * BM + kBK * BN);
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  const int64_t m = g.m, k = g.k, n = g.n;
+  const int64_t m0 = (int64_t)tr * BM, n0 = (int64_t)tc * BN;
+  double sh[TM][TN], sl[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) sh[i][j] = sl[i][j] = 0.0;
+  const int64_t rint = (g.ri > 0 && k > g.ri) ? g.ri : 0;
+  int64_t until = rint, done = 0;
+  for (int64_t k0 = 0; k0 < k; k0 += kBK) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < A_PER; ++q) {
+      int e = tid + q * kThreads, r = e % BM, c = e / BM;
+      int64_t gi = m0 + r, gp = k0 + c;
+      double2 v = (gi < m && gp < k) ? g.A[gi + gp * m] : make_double2(0.0, 0.0);
+      As_h[c][r] = v.x;
+      As_l[c][r] = v.y;
+    }
+#pragma unroll
+    for (int q = 0; q < B_PER; ++q) {
+      int e = tid + q * kThreads, r = e % kBK, c = e / kBK;
+      int64_t gp = k0 + r, gj = n0 + c;
+      double2 v = (gp < k && gj < n) ? g.B[gp + gj * k] : make_double2(0.0, 0.0);
+      Bs_h[r][c] = v.x;
+      Bs_l[r][c] = v.y;
+    }
+    __syncthreads();
+    const int kmax = (k - k0) < kBK ? (int)(k - k0) : kBK;
+    for (int kk = 0; kk < kmax; ++kk) {
+      double ah[TM], al[TM], bh[TN], bl[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        ah[i] = As_h[kk][tx * TM + i];
+        al[i] = As_l[kk][tx * TM + i];
+      }
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        bh[j] = Bs_h[kk][ty * TN + j];
+        bl[j] = Bs_l[kk][ty * TN + j];
+      }
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) mac(ah[i], al[i], bh[j], bl[j], sh[i][j], sl[i][j]);
+      ++done;
+      if (rint && --until == 0) {
+        until = rint;
+        if (done < k) {
+#pragma unroll
+          for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j) renorm(sh[i][j], sl[i][j]);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    const int64_t gj = n0 + ty * TN + j;
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      const int64_t gi = m0 + tx * TM + i;
+      if (gi < m && gj < n) {
+        double h = sh[i][j], l = sl[i][j];
+        renorm(h, l);
+        g.C[gi + gj * m] = make_double2(h, l);
+      }
+    }
+  }
+}
+
+// Block tree over 256 normalised pairs, fixed shape (the same as reduce.hip).
+__device__ ddv tree256(ddv v, double *th, double *tl) {
+  const int tid = threadIdx.x;
+  __syncthreads();
+  th[tid] = v.hi;
+  tl[tid] = v.lo;
+  __syncthreads();
+#pragma unroll
+  for (int s = kThreads / 2; s > 0; s >>= 1) {
+    if (tid < s) {
+      ddv r = dd_add(mk(th[tid], tl[tid]), mk(th[tid + s], tl[tid + s]));
+      th[tid] = r.hi;
+      tl[tid] = r.lo;
+    }
+    __syncthreads();
+  }
+  return mk(th[0], tl[0]);
+}
+
+__device__ void dot_chunk(const DotD &d, int c, double2 *partials, int *dot_done, double *lds) {
+  const int tid = threadIdx.x;
+  const int64_t p0 = (int64_t)c * kDotChunk;
+  const int64_t p1 = p0 + kDotChunk < d.n ? p0 + kDotChunk : d.n;
+  double sh = 0.0, sl = 0.0;
+  int64_t until = d.ri;
+  for (int64_t i = p0 + tid; i < p1; i += kThreads) {
+    const double2 a = d.a[i], b = d.b[i];
+    // Fortran dot update (fypp:2286-2302)
+    const double p = a.x * b.x;
+    const double e = __builtin_fma(a.x, b.x, -p);
+    const double cross = a.x * b.y + a.y * b.x;
+    const double t = sh + p, bp = t - sh, ap = t - bp;
+    sl = sl + (((e + cross) + (sh - ap)) + (p - bp));
+    sh = t;
+    if (d.ri > 0 && --until == 0) {
+      until = d.ri;
+      fast_two_sum(sh, sl, sh, sl);
+    }
+  }
+  renorm(sh, sl);
+  double *th = lds, *tl = lds + kThreads;
+  const ddv r = tree256(mk(sh, sl), th, tl);
+  __shared__ int last;
+  if (tid == 0) {
+    partials[d.first + c] = make_double2(r.hi, r.lo);
+    __threadfence();  // release: the partial is visible before the count
+    last = atomicAdd(&dot_done[d.id], 1) == d.nchunks - 1;
+  }
+  __syncthreads();
+  if (last) {
+    __threadfence();  // acquire: see every other chunk's partial
+    ddv acc = mk(0.0, 0.0);
+    for (int q = tid; q < d.nchunks; q += kThreads) {
+      const double2 v = partials[d.first + q];
+      acc = dd_add(acc, mk(v.x, v.y));
+    }
+    const ddv s = tree256(acc, th, tl);
+    if (tid == 0) *d.out = make_double2(s.hi, s.lo);
+  }
+}
+
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_grouped(
+    const Item *__restrict__ items, int nitems, const GemmD *__restrict__ gemms,
+    const DotD *__restrict__ dots, double2 *__restrict__ partials, int *__restrict__ dot_done,
+    int *__restrict__ queue) {
+  __shared__ __attribute__((aligned(16))) double lds[4 * kBK * 64];
+  __shared__ int slot;
+  for (;;) {
+    if (threadIdx.x == 0) slot = atomicAdd(queue, 1);
+    __syncthreads();
+    const int it = slot;
+    __syncthreads();  // everyone has read `slot` before thread 0 overwrites it
+    if (it >= nitems) break;
+    const Item w = items[it];
+    if (w.kind == ITEM_G64) gemm_tile<4, 4>(gemms[w.prob], w.a, w.b, lds);
+    else if (w.kind == ITEM_G32) gemm_tile<2, 2>(gemms[w.prob], w.a, w.b, lds);
+    else dot_chunk(dots[w.prob], w.a, partials, dot_done, lds);
+  }
+}
+
+__global__ void k_zero_ints(int *__restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- host side
+namespace {
+
+struct Plan {
+  int device = -1;
+  Item *items = nullptr;
+  GemmD *gemms = nullptr;
+  DotD *dots = nullptr;
+  double2 *partials = nullptr;
+  int *counters = nullptr;  // [0] queue, [1 .. ndot] per-dot completion
+  int nitems = 0, ndot = 0, grid = 0;
+  int64_t ncounters = 0;
+  hipStream_t gstream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+};
+
+hipError_t run_plan(Plan *p, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_ints, dim3((unsigned)((p->ncounters + 255) / 256)), dim3(256), 0, s,
+                     p->counters, p->ncounters);
+  hipLaunchKernelGGL(k_grouped, dim3((unsigned)p->grid), dim3(kThreads), 0, s, p->items, p->nitems,
+                     p->gemms, p->dots, p->partials, p->counters + 1, p->counters);
+  return hipGetLastError();
+}
+
+void free_plan(Plan *p) {
+  if (!p) return;
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->ev_in) (void)hipEventDestroy(p->ev_in);
+  if (p->ev_out) (void)hipEventDestroy(p->ev_out);
+  if (p->gstream) (void)hipStreamDestroy(p->gstream);
+  (void)hipFree(p->items);
+  (void)hipFree(p->gemms);
+  (void)hipFree(p->dots);
+  (void)hipFree(p->partials);
+  (void)hipFree(p->counters);
+  delete p;
+}
+
+}  // namespace
+
+namespace multifloats {
+extern "C" {
+
+void *mfhip_grouped_create(const mfhip_gemm_problem *gemms, int ngemm, const int *tile_variant,
+                           const mfhip_dot_problem *dots, int ndot) {
+  if (ngemm < 0 || ndot < 0 || (ngemm > 0 && !gemms) || (ndot > 0 && !dots)) return nullptr;
+  std::vector<GemmD> hg((size_t)ngemm);
+  std::vector<DotD> hd((size_t)ndot);
+  std::vector<Item> items;
+  std::vector<double> cost;
+  for (int i = 0; i < ngemm; ++i) {
+    const mfhip_gemm_problem &g = gemms[i];
+    hg[i] = GemmD{(const double2 *)g.a, (const double2 *)g.b, (double2 *)g.c, g.m, g.k, g.n,
+                  g.renorm_interval};
+    if (g.m <= 0 || g.n <= 0) continue;
+    int v = tile_variant ? tile_variant[i] : -1;
+    if (v < 0) v = (g.m > 32 && g.n > 32) ? ITEM_G64 : ITEM_G32;
+    const int64_t T = v == ITEM_G64 ? 64 : 32;
+    const int64_t tm = (g.m + T - 1) / T, tn = (g.n + T - 1) / T;
+    for (int64_t a = 0; a < tm; ++a)
+      for (int64_t b = 0; b < tn; ++b) {
+        items.push_back(Item{v, i, (int)a, (int)b});
+        // MAC count of the padded tile (k = 0 still writes zeros: count 1)
+        cost.push_back((double)(T * T) * (double)(g.k > 0 ? g.k : 1));
+      }
+  }
+  int64_t nparts = 0;
+  for (int i = 0; i < ndot; ++i) {
+    const mfhip_dot_problem &d = dots[i];
+    const int64_t n = d.n > 0 ? d.n : 0;
+    const int nch = (int)((n + kDotChunk - 1) / kDotChunk);
+    const int chunks = nch > 0 ? nch : 1;
+    hd[i] = DotD{(const double2 *)d.a, (const double2 *)d.b, (double2 *)d.out, n,
+                 d.renorm_interval, nparts, chunks, i};
+    for (int c = 0; c < chunks; ++c) {
+      items.push_back(Item{ITEM_DOT, i, c, 0});
+      const int64_t len = std::min<int64_t>(kDotChunk, n - (int64_t)c * kDotChunk);
+      // one element streams 32 bytes: about 14 DD-MACs of VALU time
+      cost.push_back(14.0 * (double)(len > 0 ? len : 1));
+    }
+    nparts += chunks;
+  }
+  // longest processing time first (stable for determinism of the plan)
+  std::vector<size_t> order(items.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
+  std::vector<Item> sorted(items.size());
+  for (size_t i = 0; i < order.size(); ++i) sorted[i] = items[order[i]];
+
+  Plan *p = new Plan();
+  bool ok = hipGetDevice(&p->device) == hipSuccess;
+  int cus = 0;
+  ok = ok && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device) == hipSuccess;
+  p->grid = 2 * (cus > 0 ? cus : 1);
+  p->nitems = (int)sorted.size();
+  p->ndot = ndot;
+  p->ncounters = 1 + (int64_t)ndot;
+  auto upload = [&](void **dst, const void *src, size_t bytes) {
+    if (!ok) return;
+    ok = hipMalloc(dst, bytes ? bytes : 16) == hipSuccess;
+    if (ok && bytes) ok = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+  };
+  upload((void **)&p->items, sorted.data(), sizeof(Item) * sorted.size());
+  upload((void **)&p->gemms, hg.data(), sizeof(GemmD) * hg.size());
+  upload((void **)&p->dots, hd.data(), sizeof(DotD) * hd.size());
+  ok = ok && hipMalloc((void **)&p->partials, sizeof(double2) * (size_t)(nparts > 0 ? nparts : 1)) == hipSuccess;
+  ok = ok && hipMalloc((void **)&p->counters, sizeof(int) * (size_t)p->ncounters) == hipSuccess;
+  if (!ok) {
+    free_plan(p);
+    return nullptr;
+  }
+  return p;
+}
+
+int mfhip_grouped_items(void *plan) { return plan ? ((Plan *)plan)->nitems : 0; }
+
+int mfhip_grouped_run(void *plan, void *stream) {
+  if (!plan) return (int)hipErrorInvalidValue;
+  return (int)run_plan((Plan *)plan, (hipStream_t)stream);
+}
+
+int mfhip_grouped_run_graph(void *plan, void *stream) {
+  if (!plan) return (int)hipErrorInvalidValue;
+  Plan *p = (Plan *)plan;
+  hipError_t e = hipSuccess;
+  if (!p->exec) {
+    // capture on the plan's own stream (capturing the legacy default stream
+    // is not allowed); ordered against the caller's stream with events
+    e = hipStreamCreateWithFlags(&p->gstream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_in, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_out, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipStreamBeginCapture(p->gstream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      hipError_t le = run_plan(p, p->gstream);
+      e = hipStreamEndCapture(p->gstream, &p->graph);
+      if (e == hipSuccess) e = le;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) return (int)e;
+  }
+  e = hipEventRecord(p->ev_in, (hipStream_t)stream);
+  if (e == hipSuccess) e = hipStreamWaitEvent(p->gstream, p->ev_in, 0);
+  if (e == hipSuccess) e = hipGraphLaunch(p->exec, p->gstream);
+  if (e == hipSuccess) e = hipEventRecord(p->ev_out, p->gstream);
+  if (e == hipSuccess) e = hipStreamWaitEvent((hipStream_t)stream, p->ev_out, 0);
+  return (int)e;
+}
+
+void mfhip_grouped_destroy(void *plan) { free_plan((Plan *)plan); }
+
+}  // extern "C"
+}  // namespace multifloats

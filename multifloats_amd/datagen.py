@@ -162,3 +162,25 @@ def sum_inputs(rng, n, frac=0.10):
     d = np.stack([np.ones(sel.size), rng.uniform(-1e-20, 1e-20, size=sel.size)], axis=-1)
     x[2 * sel + 1] = -dd_mul_np(x[2 * sel], d)
     return x
+
+
+def batched_sizes(rng, ngemm=10000, ndot=50000, lo=8, hi=512, dlo=16, dhi=65536):
+    """Problem sizes of the batching workload: m, n, k of each GEMM and the
+    length of each dot are log-uniform integers in [lo, hi] / [dlo, dhi]."""
+    def logu(a, b, size):
+        return np.clip(np.round(np.exp(rng.uniform(np.log(a), np.log(b), size))), a, b).astype(np.int64)
+    return logu(lo, hi, (ngemm, 3)), logu(dlo, dhi, ndot)
+
+
+def fill_config0_torch(torch, t, seed):
+    """Fill a float64 CUDA tensor viewed as (..., 2) with config-0 DD values
+    on the device (hi ~ U(-1,1), lo = hi 2^-53 U(-1,1), then Fast2Sum)."""
+    g = torch.Generator(device=t.device)
+    g.manual_seed(seed)
+    v = t.view(-1, 2)
+    hi = torch.rand(v.shape[0], generator=g, device=t.device, dtype=torch.float64) * 2.0 - 1.0
+    lo = hi * (2.0 ** -53)
+    lo = lo * (torch.rand(v.shape[0], generator=g, device=t.device, dtype=torch.float64) * 2.0 - 1.0)
+    s = hi + lo
+    v[:, 1] = lo - (s - hi)
+    v[:, 0] = s
