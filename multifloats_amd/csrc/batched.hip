@@ -11,12 +11,12 @@
 // Plan: the host cuts every GEMM into output tiles of the tile variant its
 // size class was tuned to (32x32 or 64x64) and every dot into chunks of
 // kDotChunk elements, then sorts the items by estimated cost, largest first
-// (longest processing time). The persistent kernel's workgroups pull items
-// from an atomic queue until it is drained -- every workgroup reaches the
-// exit test, so the grid always drains. A dot's chunks leave DD partials;
-// the chunk that completes a dot last (per-dot atomic counter) folds that
-// dot's partials in index order, so results do not depend on scheduling.
-// Queue and counters are cleared by a fill kernel at the start of every run
+// (longest processing time). The persistent kernel's workgroups take the
+// sorted items round robin; every workgroup's loop is bounded by the item
+// count, so the grid always drains. A dot's chunks leave DD partials; the
+// chunk that completes a dot last (per-dot atomic counter) folds that dot's
+// partials in index order, so results do not depend on scheduling. The
+// counters are cleared by a fill kernel at the start of every run
 // (graph-capture safe: no memset nodes).
 #include <stdlib.h>
 #include <string.h>
@@ -190,7 +190,9 @@ __device__ void dot_chunk(const DotD &d, int c, double2 *partials, int *dot_done
     last = atomicAdd(&dot_done[d.id], 1) == d.nchunks - 1;
   }
   __syncthreads();
-  if (last) {
+  // broadcast through LDS, then made wave-uniform so the branch around the
+  // barriers of tree256 is a scalar branch
+  if (__builtin_amdgcn_readfirstlane(last)) {
     __threadfence();  // acquire: see every other chunk's partial
     ddv acc = mk(0.0, 0.0);
     for (int q = tid; q < d.nchunks; q += kThreads) {
@@ -202,18 +204,15 @@ __device__ void dot_chunk(const DotD &d, int c, double2 *partials, int *dot_done
   }
 }
 
+// Static longest-processing-time round robin: workgroup g takes items g,
+// g + G, g + 2G, ... of the cost-sorted list. The item index derives from
+// blockIdx only, so every branch and loop around the barriers inside the item
+// bodies is uniform.
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_grouped(
     const Item *__restrict__ items, int nitems, const GemmD *__restrict__ gemms,
-    const DotD *__restrict__ dots, double2 *__restrict__ partials, int *__restrict__ dot_done,
-    int *__restrict__ queue) {
+    const DotD *__restrict__ dots, double2 *__restrict__ partials, int *__restrict__ dot_done) {
   __shared__ __attribute__((aligned(16))) double lds[4 * kBK * 64];
-  __shared__ int slot;
-  for (;;) {
-    if (threadIdx.x == 0) slot = atomicAdd(queue, 1);
-    __syncthreads();
-    const int it = slot;
-    __syncthreads();  // everyone has read `slot` before thread 0 overwrites it
-    if (it >= nitems) break;
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const Item w = items[it];
     if (w.kind == ITEM_G64) gemm_tile<4, 4>(gemms[w.prob], w.a, w.b, lds);
     else if (w.kind == ITEM_G32) gemm_tile<2, 2>(gemms[w.prob], w.a, w.b, lds);
@@ -237,7 +236,7 @@ struct Plan {
   GemmD *gemms = nullptr;
   DotD *dots = nullptr;
   double2 *partials = nullptr;
-  int *counters = nullptr;  // [0] queue, [1 .. ndot] per-dot completion
+  int *counters = nullptr;  // per-dot completion counts
   int nitems = 0, ndot = 0, grid = 0;
   int64_t ncounters = 0;
   hipStream_t gstream = nullptr;
@@ -250,7 +249,7 @@ hipError_t run_plan(Plan *p, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_ints, dim3((unsigned)((p->ncounters + 255) / 256)), dim3(256), 0, s,
                      p->counters, p->ncounters);
   hipLaunchKernelGGL(k_grouped, dim3((unsigned)p->grid), dim3(kThreads), 0, s, p->items, p->nitems,
-                     p->gemms, p->dots, p->partials, p->counters + 1, p->counters);
+                     p->gemms, p->dots, p->partials, p->counters);
   return hipGetLastError();
 }
 
@@ -327,7 +326,7 @@ void *mfhip_grouped_create(const mfhip_gemm_problem *gemms, int ngemm, const int
   p->grid = 2 * (cus > 0 ? cus : 1);
   p->nitems = (int)sorted.size();
   p->ndot = ndot;
-  p->ncounters = 1 + (int64_t)ndot;
+  p->ncounters = ndot > 0 ? (int64_t)ndot : 1;
   auto upload = [&](void **dst, const void *src, size_t bytes) {
     if (!ok) return;
     ok = hipMalloc(dst, bytes ? bytes : 16) == hipSuccess;
