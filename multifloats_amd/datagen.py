@@ -80,7 +80,11 @@ def as_colmajor(a):
 
 
 def is_colmajor(a):
-    if a.ndim != 3 or a.shape[2] != 2 or a.dtype != np.float64 or a.strides[2] != 8:
+    if a.ndim != 3 or a.shape[2] != 2 or a.dtype != np.float64:
+        return False
+    if a.size == 0:
+        return True
+    if a.strides[2] != 8:
         return False
     m, n = a.shape[:2]
     return (m <= 1 or a.strides[0] == 16) and (n <= 1 or a.strides[1] == 16 * m)
