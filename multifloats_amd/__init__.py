@@ -355,8 +355,8 @@ class GroupedPlan:
         L.mfhip_grouped_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         L.mfhip_grouped_run_graph.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         L.mfhip_grouped_destroy.argtypes = [ctypes.c_void_p]
-        G = (GemmProblem * max(len(gemms), 1))(*[GemmProblem(*g) for g in gemms])
-        D = (DotProblem * max(len(dots), 1))(*[DotProblem(*d) for d in dots])
+        G = (GemmProblem * max(len(gemms), 1))(*[GemmProblem(*[int(x) for x in g]) for g in gemms])
+        D = (DotProblem * max(len(dots), 1))(*[DotProblem(*[int(x) for x in d]) for d in dots])
         V = None
         if variants is not None:
             V = (ctypes.c_int * max(len(variants), 1))(*[int(v) for v in variants])
