@@ -39,6 +39,16 @@ HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
 FP64_FLOPS_PER_DDMAC = 15.75  # 2 MUL + 2 FMA(x2) + 9 ADD + 6/8 renorm at ri = 8
 INT8_PEAK_TOPS = 5000.0      # MI355X dense INT8 MFMA
 
+# HBM-side bytes per launch from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+# (KiB per dispatch) on the 4096^3 workload, scaled by what the same counters
+# read for a 1 GiB device copy: FETCH_SIZE x 2.0 (it reads 524,300 KiB for
+# 1 GiB fetched), WRITE_SIZE x 1.0 (1,048,576 KiB). Summary and commands:
+# profiles/pmc_traffic_gemm_4096.md. Keyed by (kernel, size, slices).
+PMC_TRAFFIC = {
+    ("k_ozaki", 4096, 16): 3146263.5 * 2.0 * 1024 + 262144.0 * 1024,
+    ("k_gemm_plain", 4096, 0): 3144165.19 * 2.0 * 1024 + 286727.06 * 1024,
+}
+
 
 def parse():
     p = argparse.ArgumentParser()
@@ -134,7 +144,8 @@ def bench_gemm(args, torch, mf):
         res["plain"] = dict(ms=ms, value=flops / (ms * 1e-3) / 1e9, err_max=emax, err_mean=emean,
                             roofline={"bound": "fp64", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                                       "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                                      "traffic": None,
+                                      "traffic": PMC_TRAFFIC.get(("k_gemm_plain", args.size, 0))
+                                      if args.variant < 0 else None,
                                       "note": "k_gemm_plain, FP64 VALU: %.2f FP64 flops per DD-MAC (FMA = 2)"
                                               % FP64_FLOPS_PER_DDMAC})
         if not args.no_cpu:
@@ -163,7 +174,8 @@ def bench_gemm(args, torch, mf):
         res["ozaki"] = dict(ms=ms, ms_multiply=ms_mult, value=flops / (ms * 1e-3) / 1e9, err_max=emax,
                             err_mean=emean, slices=S, slice_pairs=P,
                             roofline={"bound": "mfma", "achieved": round(tops, 2), "peak": INT8_PEAK_TOPS,
-                                      "unit": "TOP/s", "frac": round(tops / INT8_PEAK_TOPS, 4), "traffic": None,
+                                      "unit": "TOP/s", "frac": round(tops / INT8_PEAK_TOPS, 4),
+                                      "traffic": PMC_TRAFFIC.get(("k_ozaki", args.size, S)),
                                       "note": "k_ozaki INT8 MFMA: %d slice pairs x 2mnk int8 ops per launch" % P})
         del C
 
