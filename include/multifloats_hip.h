@@ -133,6 +133,25 @@ MULTIFLOATS_API float64x2 dotdd(const float64x2 *a, const float64x2 *b, int64_t 
                                 int64_t renorm_interval);
 /* Fortran norm2(a) (fypp:2338-2352); like the reference, no overflow scaling. */
 MULTIFLOATS_API float64x2 norm2dd(const float64x2 *a, int64_t n);
+/* Fortran sum(array, mask) (fypp:1980-2020): elements whose mask entry
+ * (a default-kind Fortran LOGICAL, i.e. a 4-byte integer, nonzero = true) is
+ * false are skipped; mask == NULL is sumdd. */
+MULTIFLOATS_API float64x2 sumdd_mask(const float64x2 *a, const int32_t *mask, int64_t n);
+/* Fortran dot_product(a, b) on cmplx64x2 = sum(conj(a) * b) (fypp:2321-2331:
+ * four real dot chains combined with DD + and -). */
+MULTIFLOATS_API complex64x2 cdotdd(const complex64x2 *a, const complex64x2 *b, int64_t n,
+                                   int64_t renorm_interval);
+/* Fortran sum(array, dim[, mask]) and norm2(array, dim) (fypp:2035-2090,
+ * 2356-2377) for a contiguous column-major array viewed as (inner, len,
+ * outer), with len the extent of dimension `dim`, inner the product of the
+ * extents before it and outer the product of those after it. out receives
+ * the inner x outer result in column-major order. Each output is the
+ * reference's own sequential chain, so the results are bit-identical to it.
+ * mask may be NULL. */
+MULTIFLOATS_API void sumdd_dim(const float64x2 *a, const int32_t *mask, int64_t inner, int64_t len,
+                               int64_t outer, float64x2 *out);
+MULTIFLOATS_API void norm2dd_dim(const float64x2 *a, int64_t inner, int64_t len, int64_t outer,
+                                 float64x2 *out);
 
 /* Precision form of matmuldd_mm (no counterpart in the reference): the Ozaki
  * split-precision GEMM on the INT8 matrix cores. A and B are cut into
@@ -185,6 +204,18 @@ MULTIFLOATS_API size_t mfhip_reduce_workspace_bytes(int64_t n);
 MULTIFLOATS_API int mfhip_reduce(int kind, const float64x2 *a, const float64x2 *b, int64_t n,
                                  int64_t renorm_interval, float64x2 *out, void *work,
                                  void *stream);
+
+/* Device forms of sumdd_mask (work as for mfhip_reduce), cdotdd (work of
+ * mfhip_cdot_workspace_bytes(n) bytes; *out is one complex64x2) and
+ * sumdd_dim / norm2dd_dim (kind MFHIP_SUM or MFHIP_NORM2; no workspace;
+ * mask only with MFHIP_SUM, may be NULL). */
+MULTIFLOATS_API int mfhip_sum_masked(const float64x2 *a, const int32_t *mask, int64_t n,
+                                     float64x2 *out, void *work, void *stream);
+MULTIFLOATS_API size_t mfhip_cdot_workspace_bytes(int64_t n);
+MULTIFLOATS_API int mfhip_cdot(const complex64x2 *a, const complex64x2 *b, int64_t n,
+                               int64_t renorm_interval, complex64x2 *out, void *work, void *stream);
+MULTIFLOATS_API int mfhip_reduce_dim(int kind, const float64x2 *a, const int32_t *mask, int64_t inner,
+                                     int64_t len, int64_t outer, float64x2 *out, void *stream);
 
 /* Ozaki GEMM on device pointers; `work` is caller-owned device scratch of at
  * least mfhip_ozaki_workspace_bytes(m, k, n, slices) bytes. With slices <= 0

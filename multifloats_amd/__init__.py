@@ -40,6 +40,10 @@ class float64x2(ctypes.Structure):
     _fields_ = [("limbs", ctypes.c_double * 2)]
 
 
+class complex64x2(ctypes.Structure):
+    _fields_ = [("re", float64x2), ("im", float64x2)]
+
+
 _lib = None
 
 
@@ -69,6 +73,19 @@ def lib():
         L.norm2dd.restype = float64x2
         L.dotdd.argtypes = [P, P, I64, I64]
         L.dotdd.restype = float64x2
+        L.sumdd_mask.argtypes = [P, P, I64]
+        L.sumdd_mask.restype = float64x2
+        L.cdotdd.argtypes = [P, P, I64, I64]
+        L.cdotdd.restype = complex64x2
+        L.sumdd_dim.argtypes = [P, P, I64, I64, I64, P]
+        L.sumdd_dim.restype = None
+        L.norm2dd_dim.argtypes = [P, I64, I64, I64, P]
+        L.norm2dd_dim.restype = None
+        L.mfhip_sum_masked.argtypes = [P, P, I64, P, P, VP]
+        L.mfhip_cdot_workspace_bytes.argtypes = [I64]
+        L.mfhip_cdot_workspace_bytes.restype = ctypes.c_size_t
+        L.mfhip_cdot.argtypes = [P, P, I64, I64, P, P, VP]
+        L.mfhip_reduce_dim.argtypes = [I, P, P, I64, I64, I64, P, VP]
         L.mfhip_matmul_mm.argtypes = [P, P, P, I64, I64, I64, I64, VP]
         L.mfhip_matmul_mm_variant.argtypes = [P, P, P, I64, I64, I64, I64, I, VP]
         L.mfhip_matmul_variants.argtypes = []
@@ -248,26 +265,90 @@ def _res(r):
     return np.array([r.limbs[0], r.limbs[1]])
 
 
-def sum(a):  # noqa: A001 - mirrors the Fortran intrinsic name
-    """Fortran sum over every element of a real64x2 array (storage order)."""
-    a = np.ascontiguousarray(_dd(a)).reshape(-1, 2)
-    r = lib().sumdd(_ptr(a), a.shape[0])
+def _fortran_order(a):
+    """Buffer of a (n1, ..., nr, 2) DD array in Fortran element order
+    (first index fastest), limbs innermost: the layout of a real64x2 array."""
+    r = a.ndim - 1
+    return np.ascontiguousarray(a.transpose(list(range(r - 1, -1, -1)) + [r]))
+
+
+def _from_fortran_order(buf, shape):
+    """Inverse of _fortran_order for a result of logical shape `shape` + (2,)."""
+    r = len(shape)
+    return buf.reshape(tuple(reversed(shape)) + (2,)).transpose(list(range(r - 1, -1, -1)) + [r])
+
+
+def _mask_buf(mask, shape):
+    m = np.asarray(mask, dtype=bool)
+    if m.shape != tuple(shape):
+        raise ValueError("mask must have the array's shape")
+    r = m.ndim
+    return np.ascontiguousarray(m.transpose(list(range(r - 1, -1, -1)))).astype(np.int32)
+
+
+def _reduce_dim(kind, a, dim, mask):
+    shape = a.shape[:-1]
+    r = len(shape)
+    if not 1 <= dim <= r:
+        raise ValueError("dim must lie in 1..%d" % r)
+    inner = int(np.prod(shape[:dim - 1], dtype=np.int64))
+    ln = int(shape[dim - 1])
+    outer = int(np.prod(shape[dim:], dtype=np.int64))
+    rest = shape[:dim - 1] + shape[dim:]
+    buf = _fortran_order(a)
+    out = np.empty((max(inner * outer, 1), 2))
+    if kind == "sum":
+        mb = _mask_buf(mask, shape) if mask is not None else None
+        lib().sumdd_dim(_ptr(buf), _ptr(mb) if mb is not None else None, inner, ln, outer, _ptr(out))
+    else:
+        lib().norm2dd_dim(_ptr(buf), inner, ln, outer, _ptr(out))
+    _check_status()
+    return _from_fortran_order(out[:inner * outer], rest)
+
+
+def sum(a, dim=None, mask=None):  # noqa: A001 - mirrors the Fortran intrinsic name
+    """Fortran sum(array[, dim][, mask]) on a real64x2 array of shape (..., 2)
+    (fsrc/multifloats.fypp:1980-2090). Without dim: every element in Fortran
+    storage order (first index fastest); with dim (1-based): the reduced
+    array, bit-identical to the reference's per-output chains."""
+    a = _dd(a)
+    if dim is not None and a.ndim > 2:
+        return _reduce_dim("sum", a, dim, mask)
+    buf = _fortran_order(a).reshape(-1, 2)
+    if mask is None:
+        r = lib().sumdd(_ptr(buf), buf.shape[0])
+    else:
+        mb = _mask_buf(mask, a.shape[:-1]).reshape(-1)
+        r = lib().sumdd_mask(_ptr(buf), _ptr(mb), buf.shape[0])
     _check_status()
     return _res(r)
 
 
 def dot_product(a, b, renorm_interval=DD_FMA_RENORM_INTERVAL):
-    a, b = _dd(a), _dd(b)
-    if a.shape != b.shape or a.ndim != 2:
-        raise ValueError("dot_product: rank-1 arrays of equal size")
+    """Fortran dot_product on rank-1 real64x2 arrays (n, 2) (fypp:2254-2308)
+    or on cmplx64x2 arrays (n, 4) = {re.hi, re.lo, im.hi, im.lo}, where it is
+    sum(conj(a) * b) (fypp:2321-2331)."""
+    a = np.require(np.asarray(a, dtype=np.float64), requirements=["C", "A"])
+    b = np.require(np.asarray(b, dtype=np.float64), requirements=["C", "A"])
+    if a.shape != b.shape or a.ndim != 2 or a.shape[1] not in (2, 4):
+        raise ValueError("dot_product: rank-1 arrays of equal size, (n, 2) or (n, 4)")
+    if a.shape[1] == 4:
+        r = lib().cdotdd(_ptr(a), _ptr(b), a.shape[0], renorm_interval)
+        _check_status()
+        return np.array([r.re.limbs[0], r.re.limbs[1], r.im.limbs[0], r.im.limbs[1]])
     r = lib().dotdd(_ptr(a), _ptr(b), a.shape[0], renorm_interval)
     _check_status()
     return _res(r)
 
 
-def norm2(a):
-    a = np.ascontiguousarray(_dd(a)).reshape(-1, 2)
-    r = lib().norm2dd(_ptr(a), a.shape[0])
+def norm2(a, dim=None):
+    """Fortran norm2(array[, dim]) (fypp:2338-2377); with dim (1-based) the
+    results are bit-identical to the reference's per-output chains."""
+    a = _dd(a)
+    if dim is not None and a.ndim > 2:
+        return _reduce_dim("norm2", a, dim, None)
+    buf = _fortran_order(a).reshape(-1, 2)
+    r = lib().norm2dd(_ptr(buf), buf.shape[0])
     _check_status()
     return _res(r)
 
@@ -321,6 +402,22 @@ def dev_reduce(kind, a, b, n, out, work, renorm_interval=DD_FMA_RENORM_INTERVAL,
 
 def reduce_workspace_bytes(n):
     return lib().mfhip_reduce_workspace_bytes(n)
+
+
+def dev_sum_masked(a, mask, n, out, work, stream=0):
+    _dev(lib().mfhip_sum_masked(a, mask, n, out, work, stream))
+
+
+def cdot_workspace_bytes(n):
+    return lib().mfhip_cdot_workspace_bytes(n)
+
+
+def dev_cdot(a, b, n, out, work, renorm_interval=DD_FMA_RENORM_INTERVAL, stream=0):
+    _dev(lib().mfhip_cdot(a, b, n, renorm_interval, out, work, stream))
+
+
+def dev_reduce_dim(kind, a, mask, inner, length, outer, out, stream=0):
+    _dev(lib().mfhip_reduce_dim(kind, a, mask, inner, length, outer, out, stream))
 
 
 def matmul_variants():

@@ -183,6 +183,60 @@ hipError_t host_reduce(int kind, const float64x2 *a, const float64x2 *b, int64_t
   return hipStreamSynchronize(tl_stage.stream);
 }
 
+hipError_t host_sum_mask(const float64x2 *a, const int32_t *mask, int64_t n, float64x2 *out) {
+  tl_err = 0;
+  TRY(stage_init());
+  const size_t nn = n > 0 ? (size_t)n : 0;
+  void *da, *dm, *dw;
+  TRY(stage_buf(0, 16 * nn + 16, &da));
+  TRY(stage_buf(1, 4 * nn + 16, &dm));
+  TRY(stage_buf(3, mfhip::reduce_workspace_bytes(n) + 16, &dw));
+  TRY(h2d(da, a, 16 * nn));
+  TRY(h2d(dm, mask, 4 * nn));
+  double2 *dout = W(dw);
+  TRY(mfhip::reduce(mfhip::RED_SUM, D(da), nullptr, n, 0, dout, dout + 1, tl_stage.stream,
+                    static_cast<const int32_t *>(dm)));
+  TRY(d2h(out, dout, 16));
+  return hipStreamSynchronize(tl_stage.stream);
+}
+
+hipError_t host_cdot(const complex64x2 *a, const complex64x2 *b, int64_t n, int64_t ri,
+                     complex64x2 *out) {
+  tl_err = 0;
+  TRY(stage_init());
+  const size_t nn = n > 0 ? (size_t)n : 0;
+  void *da, *db, *dw;
+  TRY(stage_buf(0, 32 * nn + 32, &da));
+  TRY(stage_buf(1, 32 * nn + 32, &db));
+  TRY(stage_buf(3, 2 * mfhip::reduce_workspace_bytes(n) + 32, &dw));
+  TRY(h2d(da, a, 32 * nn));
+  TRY(h2d(db, b, 32 * nn));
+  double4 *dout = static_cast<double4 *>(dw);
+  TRY(mfhip::cdot((const double4 *)da, (const double4 *)db, n, ri, dout, dout + 1, tl_stage.stream));
+  TRY(d2h(out, dout, 32));
+  return hipStreamSynchronize(tl_stage.stream);
+}
+
+hipError_t host_dim(int kind, const float64x2 *a, const int32_t *mask, int64_t inner, int64_t len,
+                    int64_t outer, float64x2 *out) {
+  tl_err = 0;
+  if (inner <= 0 || outer <= 0) return hipSuccess;
+  if (len < 0) len = 0;
+  TRY(stage_init());
+  const size_t tot = (size_t)inner * (size_t)len * (size_t)outer;
+  const size_t nout = (size_t)inner * (size_t)outer;
+  void *da, *dm = nullptr, *dc;
+  TRY(stage_buf(0, 16 * tot, &da));
+  if (mask) TRY(stage_buf(1, 4 * tot, &dm));
+  TRY(stage_buf(2, 16 * nout, &dc));
+  TRY(h2d(da, a, 16 * tot));
+  if (mask) TRY(h2d(dm, mask, 4 * tot));
+  TRY(mfhip::reduce_dim(kind, D(da), static_cast<const int32_t *>(dm), inner, len, outer, W(dc),
+                        tl_stage.stream));
+  TRY(d2h(out, dc, 16 * nout));
+  return hipStreamSynchronize(tl_stage.stream);
+}
+
 thread_local int tl_oz_used = 0;
 
 hipError_t host_cmm(const complex64x2 *a, const complex64x2 *b, complex64x2 *c, int64_t m,
@@ -366,6 +420,40 @@ float64x2 norm2dd(const float64x2 *a, int64_t n) {
   return r;
 }
 
+float64x2 sumdd_mask(const float64x2 *a, const int32_t *mask, int64_t n) {
+  float64x2 r;
+  hipError_t e = mask ? host_sum_mask(a, mask, n, &r) : host_reduce(mfhip::RED_SUM, a, nullptr, n, 0, &r);
+  if (e != hipSuccess) {
+    report("sumdd_mask", e);
+    return nan_dd();
+  }
+  return r;
+}
+complex64x2 cdotdd(const complex64x2 *a, const complex64x2 *b, int64_t n, int64_t ri) {
+  complex64x2 r;
+  hipError_t e = host_cdot(a, b, n, ri, &r);
+  if (e != hipSuccess) {
+    report("cdotdd", e);
+    nan_fill_c(&r, 1);
+  }
+  return r;
+}
+void sumdd_dim(const float64x2 *a, const int32_t *mask, int64_t inner, int64_t len, int64_t outer,
+               float64x2 *out) {
+  hipError_t e = host_dim(mfhip::RED_SUM, a, mask, inner, len, outer, out);
+  if (e != hipSuccess) {
+    report("sumdd_dim", e);
+    nan_fill(out, (inner > 0 && outer > 0) ? inner * outer : 0);
+  }
+}
+void norm2dd_dim(const float64x2 *a, int64_t inner, int64_t len, int64_t outer, float64x2 *out) {
+  hipError_t e = host_dim(mfhip::RED_NORM2, a, nullptr, inner, len, outer, out);
+  if (e != hipSuccess) {
+    report("norm2dd_dim", e);
+    nan_fill(out, (inner > 0 && outer > 0) ? inner * outer : 0);
+  }
+}
+
 // ---- device-pointer API ----
 
 static int status(hipError_t e) {
@@ -423,6 +511,27 @@ int mfhip_reduce(int kind, const float64x2 *a, const float64x2 *b, int64_t n, in
   CHECK_ALIGN(a, b, out, work);
   if (!work || !out) return status(hipErrorInvalidValue);
   return status(mfhip::reduce(kind, D(a), D(b), n, ri, W(out), W(work), (hipStream_t)stream));
+}
+
+int mfhip_sum_masked(const float64x2 *a, const int32_t *mask, int64_t n, float64x2 *out, void *work,
+                     void *stream) {
+  CHECK_ALIGN(a, out, work);
+  if (!work || !out || !mask) return status(hipErrorInvalidValue);
+  return status(mfhip::reduce(mfhip::RED_SUM, D(a), nullptr, n, 0, W(out), W(work), (hipStream_t)stream,
+                              mask));
+}
+size_t mfhip_cdot_workspace_bytes(int64_t n) { return 2 * mfhip::reduce_workspace_bytes(n); }
+int mfhip_cdot(const complex64x2 *a, const complex64x2 *b, int64_t n, int64_t ri, complex64x2 *out,
+               void *work, void *stream) {
+  CHECK_ALIGN(a, b, out, work);
+  if (!work || !out) return status(hipErrorInvalidValue);
+  return status(mfhip::cdot((const double4 *)a, (const double4 *)b, n, ri, (double4 *)out,
+                            (double4 *)work, (hipStream_t)stream));
+}
+int mfhip_reduce_dim(int kind, const float64x2 *a, const int32_t *mask, int64_t inner, int64_t len,
+                     int64_t outer, float64x2 *out, void *stream) {
+  CHECK_ALIGN(a, out);
+  return status(mfhip::reduce_dim(kind, D(a), mask, inner, len, outer, W(out), (hipStream_t)stream));
 }
 
 size_t mfhip_ozaki_workspace_bytes(int64_t m, int64_t k, int64_t n, int slices) {

@@ -33,8 +33,16 @@ hipError_t binary(int op, const double2 *a, const double2 *b, double2 *c, int64_
 
 int64_t reduce_groups(int64_t n);
 size_t reduce_workspace_bytes(int64_t n);
+// mask (sum only, may be null): elements with mask[i] == 0 are skipped.
 hipError_t reduce(int kind, const double2 *a, const double2 *b, int64_t n, int64_t ri,
-                  double2 *out, double2 *work, hipStream_t s);
+                  double2 *out, double2 *work, hipStream_t s, const int32_t *mask = nullptr);
+// Complex conj(a).b; work holds 2 * reduce_workspace_bytes(n).
+hipError_t cdot(const double4 *a, const double4 *b, int64_t n, int64_t ri, double4 *out, double4 *work,
+                hipStream_t s);
+// Bit-exact sum / norm2 along one dimension of an (inner, len, outer)
+// column-major array into out(inner, outer); needs no workspace.
+hipError_t reduce_dim(int kind, const double2 *a, const int32_t *mask, int64_t inner, int64_t len,
+                      int64_t outer, double2 *out, hipStream_t s);
 
 // Ozaki split-precision GEMM with S int8 slices per operand (8 <= S <= 16).
 int ozaki_max_slices();

@@ -1,0 +1,391 @@
+#!/usr/bin/env python3
+"""Writes multifloats_hip.F90, the Fortran module that routes the reference's
+array generics (matmul, dot_product, sum, norm2, exp, log, sin, erf, sqrt and
+the array + - * /) to libmultifloats_hip.so through bind(c) interfaces.
+
+The reference generates its module with fypp (fsrc/multifloats.fypp); fypp is
+not part of this toolchain, so the rank 1..7 specifics are written out here.
+
+    python multifloats_amd/fortran/gen_bridge.py   # rewrites multifloats_hip.F90
+"""
+import os
+
+MAX_RANK = 7
+HERE = os.path.dirname(os.path.abspath(__file__))
+UNARY = [("exp", "expdd_v"), ("log", "logdd_v"), ("sin", "sindd_v"), ("erf", "erfdd_v"), ("sqrt", "sqrtdd_v")]
+BINARY = [("+", "add", "adddd_v"), ("-", "sub", "subdd_v"), ("*", "mul", "muldd_v"), ("/", "div", "divdd_v")]
+
+
+def colons(r):
+    return ",".join([":"] * r)
+
+
+def shape_of(name, r):
+    return ", ".join("size(%s,%d)" % (name, d) for d in range(1, r + 1))
+
+
+def reduced_shape(r):
+    # extents of `array` without dimension `dim`
+    return ", ".join("size(array, %d + merge(1, 0, %d >= dim))" % (j, j) for j in range(1, r))
+
+
+def gen():
+    o = []
+    w = o.append
+    w("""! GENERATED by gen_bridge.py -- edit that file, not this one.
+!
+! multifloats_hip: Fortran bridge from the reference's array generics to the
+! MI355X library libmultifloats_hip.so. Every specific here is a non-elemental
+! array procedure that hands a contiguous array to one C-ABI entry point
+! (include/multifloats_hip.h), which runs the hand-written HIP kernels and
+! returns when the result is back in host memory.
+!
+! Reference interfaces replaced (fsrc/multifloats.fypp):
+!   matmul          dd_matmul_mm/mv/vm 2568-2593, cdd_matmul_* 2609-2649
+!   dot_product     dd_dot_product 2254-2308, cdd_dot_product 2321-2331
+!   sum             dd_sum_Nd / dd_sum_Nd_dim 1980-2090 (with mask)
+!   norm2           dd_norm2_Nd / dd_norm2_Nd_dim 2338-2377
+!   exp log sin erf sqrt   elemental delegates 709-736, 1407-1412
+!   + - * /         elemental native operators 2746-2837
+!   DD_FMA_RENORM_INTERVAL and dd_set_fma_renorm_interval 664, 2677-2680
+!
+! Types: real64x2 and cmplx64x2 are SEQUENCE types with the reference's names
+! and components (fypp:170-178), so values of the reference's types are values
+! of these. Built with -DMULTIFLOATS_HIP_USE_REFERENCE_TYPES the module takes
+! the types from the reference's module instead and only adds the specifics.
+module multifloats_hip
+    use, intrinsic :: iso_c_binding
+#ifdef MULTIFLOATS_HIP_USE_REFERENCE_TYPES
+    use multifloats, only: real64x2, cmplx64x2
+#endif
+    implicit none
+    private
+
+    integer, parameter :: dp = 8
+
+#ifndef MULTIFLOATS_HIP_USE_REFERENCE_TYPES
+    type, public :: real64x2
+        sequence
+        real(dp) :: limbs(2)
+    end type real64x2
+
+    type, public :: cmplx64x2
+        sequence
+        type(real64x2) :: re, im
+    end type cmplx64x2
+#else
+    public :: real64x2, cmplx64x2
+#endif
+
+    ! by-value twins of the C structs (include/multifloats_hip.h)
+    type, bind(c) :: float64x2
+        real(c_double) :: limbs(2)
+    end type float64x2
+
+    type, bind(c) :: complex64x2
+        type(float64x2) :: re, im
+    end type complex64x2
+
+    ! Renormalisation interval of matmul and dot_product (reference default 8).
+    integer, public, protected :: DD_FMA_RENORM_INTERVAL = 8
+    public :: dd_set_fma_renorm_interval
+    public :: dd_matmul_ozaki
+    public :: matmul, dot_product, sum, norm2
+    public :: exp, log, sin, erf, sqrt
+    public :: operator(+), operator(-), operator(*), operator(/)
+
+    interface
+        pure subroutine c_matmuldd_mm(a, b, c, m, k, n, ri) bind(c, name='matmuldd_mm')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, b, c
+            integer(c_int64_t), value :: m, k, n, ri
+        end subroutine
+        pure subroutine c_matmuldd_mv(a, x, y, m, k, ri) bind(c, name='matmuldd_mv')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, x, y
+            integer(c_int64_t), value :: m, k, ri
+        end subroutine
+        pure subroutine c_matmuldd_vm(x, b, y, k, n, ri) bind(c, name='matmuldd_vm')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: x, b, y
+            integer(c_int64_t), value :: k, n, ri
+        end subroutine
+        pure subroutine c_cmatmuldd_mm(a, b, c, m, k, n, ri) bind(c, name='cmatmuldd_mm')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, b, c
+            integer(c_int64_t), value :: m, k, n, ri
+        end subroutine
+        pure subroutine c_cmatmuldd_mv(a, x, y, m, k, ri) bind(c, name='cmatmuldd_mv')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, x, y
+            integer(c_int64_t), value :: m, k, ri
+        end subroutine
+        pure subroutine c_cmatmuldd_vm(x, b, y, k, n, ri) bind(c, name='cmatmuldd_vm')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: x, b, y
+            integer(c_int64_t), value :: k, n, ri
+        end subroutine
+        subroutine c_matmuldd_mm_ozaki(a, b, c, m, k, n, slices) bind(c, name='matmuldd_mm_ozaki')
+            import :: c_ptr, c_int64_t, c_int
+            type(c_ptr), value :: a, b, c
+            integer(c_int64_t), value :: m, k, n
+            integer(c_int), value :: slices
+        end subroutine
+        pure type(float64x2) function c_dotdd(a, b, n, ri) bind(c, name='dotdd')
+            import :: c_ptr, c_int64_t, float64x2
+            type(c_ptr), value :: a, b
+            integer(c_int64_t), value :: n, ri
+        end function
+        pure type(complex64x2) function c_cdotdd(a, b, n, ri) bind(c, name='cdotdd')
+            import :: c_ptr, c_int64_t, complex64x2
+            type(c_ptr), value :: a, b
+            integer(c_int64_t), value :: n, ri
+        end function
+        pure type(float64x2) function c_sumdd_mask(a, mask, n) bind(c, name='sumdd_mask')
+            import :: c_ptr, c_int64_t, float64x2
+            type(c_ptr), value :: a, mask
+            integer(c_int64_t), value :: n
+        end function
+        pure type(float64x2) function c_norm2dd(a, n) bind(c, name='norm2dd')
+            import :: c_ptr, c_int64_t, float64x2
+            type(c_ptr), value :: a
+            integer(c_int64_t), value :: n
+        end function
+        pure subroutine c_sumdd_dim(a, mask, inner, len, outer, out) bind(c, name='sumdd_dim')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, mask, out
+            integer(c_int64_t), value :: inner, len, outer
+        end subroutine
+        pure subroutine c_norm2dd_dim(a, inner, len, outer, out) bind(c, name='norm2dd_dim')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, out
+            integer(c_int64_t), value :: inner, len, outer
+        end subroutine
+""")
+    for f, c in UNARY:
+        w("""        pure subroutine c_%s(a, c, n) bind(c, name='%s')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, c
+            integer(c_int64_t), value :: n
+        end subroutine
+""" % (c, c))
+    for _, nm, c in BINARY:
+        w("""        pure subroutine c_%s(a, b, c, n) bind(c, name='%s')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, b, c
+            integer(c_int64_t), value :: n
+        end subroutine
+""" % (c, c))
+    w("    end interface\n")
+
+    # generic interfaces
+    w("""    interface matmul
+        module procedure dd_matmul_mm, dd_matmul_mv, dd_matmul_vm
+        module procedure cdd_matmul_mm, cdd_matmul_mv, cdd_matmul_vm
+    end interface
+
+    interface dot_product
+        module procedure dd_dot_product, cdd_dot_product
+    end interface
+""")
+    w("    interface sum\n")
+    for r in range(1, MAX_RANK + 1):
+        w("        module procedure dd_sum_%dd\n" % r)
+        if r > 1:
+            w("        module procedure dd_sum_%dd_dim\n" % r)
+    w("    end interface\n\n    interface norm2\n")
+    for r in range(1, MAX_RANK + 1):
+        w("        module procedure dd_norm2_%dd\n" % r)
+        if r > 1:
+            w("        module procedure dd_norm2_%dd_dim\n" % r)
+    w("    end interface\n\n")
+    for f, _ in UNARY:
+        w("    interface %s\n" % f)
+        for r in range(1, MAX_RANK + 1):
+            w("        module procedure dd_%s_%dd\n" % (f, r))
+        w("    end interface\n\n")
+    for op, nm, _ in BINARY:
+        w("    interface operator(%s)\n" % op)
+        for r in range(1, MAX_RANK + 1):
+            w("        module procedure dd_%s_%dd\n" % (nm, r))
+        w("    end interface\n\n")
+
+    w("contains\n\n")
+    w("""    subroutine dd_set_fma_renorm_interval(n)
+        integer, intent(in) :: n
+        DD_FMA_RENORM_INTERVAL = n
+    end subroutine
+
+    pure function ri() result(r)
+        integer(c_int64_t) :: r
+        r = int(DD_FMA_RENORM_INTERVAL, c_int64_t)
+    end function
+
+    pure function dd_matmul_mm(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:), b(:,:)
+        type(real64x2), target :: res(size(a,1), size(b,2))
+        call c_matmuldd_mm(c_loc(a), c_loc(b), c_loc(res), int(size(a,1), c_int64_t), &
+            int(size(a,2), c_int64_t), int(size(b,2), c_int64_t), ri())
+    end function
+
+    pure function dd_matmul_mv(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:), b(:)
+        type(real64x2), target :: res(size(a,1))
+        call c_matmuldd_mv(c_loc(a), c_loc(b), c_loc(res), int(size(a,1), c_int64_t), &
+            int(size(a,2), c_int64_t), ri())
+    end function
+
+    pure function dd_matmul_vm(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:), b(:,:)
+        type(real64x2), target :: res(size(b,2))
+        call c_matmuldd_vm(c_loc(a), c_loc(b), c_loc(res), int(size(a), c_int64_t), &
+            int(size(b,2), c_int64_t), ri())
+    end function
+
+    ! Complex matmul: one fused kernel, bit-identical to the reference's
+    ! four-real-matmul composition (fypp:2609-2621).
+    pure function cdd_matmul_mm(a, b) result(res)
+        type(cmplx64x2), intent(in), target, contiguous :: a(:,:), b(:,:)
+        type(cmplx64x2), target :: res(size(a,1), size(b,2))
+        call c_cmatmuldd_mm(c_loc(a), c_loc(b), c_loc(res), int(size(a,1), c_int64_t), &
+            int(size(a,2), c_int64_t), int(size(b,2), c_int64_t), ri())
+    end function
+
+    pure function cdd_matmul_mv(a, b) result(res)
+        type(cmplx64x2), intent(in), target, contiguous :: a(:,:), b(:)
+        type(cmplx64x2), target :: res(size(a,1))
+        call c_cmatmuldd_mv(c_loc(a), c_loc(b), c_loc(res), int(size(a,1), c_int64_t), &
+            int(size(a,2), c_int64_t), ri())
+    end function
+
+    pure function cdd_matmul_vm(a, b) result(res)
+        type(cmplx64x2), intent(in), target, contiguous :: a(:), b(:,:)
+        type(cmplx64x2), target :: res(size(b,2))
+        call c_cmatmuldd_vm(c_loc(a), c_loc(b), c_loc(res), int(size(a), c_int64_t), &
+            int(size(b,2), c_int64_t), ri())
+    end function
+
+    ! Precision form (no reference counterpart): Ozaki INT8 split GEMM,
+    ! |C - AB| <= 2^-100 (|A||B|); slices absent or <= 0 lets the library
+    ! choose from the data.
+    function dd_matmul_ozaki(a, b, slices) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:), b(:,:)
+        integer, intent(in), optional :: slices
+        type(real64x2), target :: res(size(a,1), size(b,2))
+        integer(c_int) :: s
+        s = 0
+        if (present(slices)) s = int(slices, c_int)
+        call c_matmuldd_mm_ozaki(c_loc(a), c_loc(b), c_loc(res), int(size(a,1), c_int64_t), &
+            int(size(a,2), c_int64_t), int(size(b,2), c_int64_t), s)
+    end function
+
+    pure function dd_dot_product(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:), b(:)
+        type(real64x2) :: res
+        type(float64x2) :: r
+        r = c_dotdd(c_loc(a), c_loc(b), int(min(size(a), size(b)), c_int64_t), ri())
+        res%limbs = r%limbs
+    end function
+
+    pure function cdd_dot_product(a, b) result(res)
+        type(cmplx64x2), intent(in), target, contiguous :: a(:), b(:)
+        type(cmplx64x2) :: res
+        type(complex64x2) :: r
+        r = c_cdotdd(c_loc(a), c_loc(b), int(min(size(a), size(b)), c_int64_t), ri())
+        res%re%limbs = r%re%limbs
+        res%im%limbs = r%im%limbs
+    end function
+
+""")
+    for r in range(1, MAX_RANK + 1):
+        C = colons(r)
+        w("""    pure function dd_sum_%(r)dd(array, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(%(C)s)
+        logical, intent(in), target, contiguous, optional :: mask(%(C)s)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        if (present(mask)) then
+            s = c_sumdd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
+        else
+            s = c_sumdd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
+        end if
+        res%%limbs = s%%limbs
+    end function
+
+    pure function dd_norm2_%(r)dd(array) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(%(C)s)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        s = c_norm2dd(c_loc(array), int(size(array), c_int64_t))
+        res%%limbs = s%%limbs
+    end function
+
+""" % dict(r=r, C=C))
+        if r > 1:
+            w("""    pure function dd_sum_%(r)dd_dim(array, dim, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(%(C)s)
+        integer, intent(in) :: dim
+        logical, intent(in), target, contiguous, optional :: mask(%(C)s)
+        type(real64x2), target :: res(%(RS)s)
+        integer(c_int64_t) :: inner, outer
+        integer :: d_
+        inner = 1
+        do d_ = 1, dim - 1
+            inner = inner * size(array, d_)
+        end do
+        outer = 1
+        do d_ = dim + 1, %(r)d
+            outer = outer * size(array, d_)
+        end do
+        if (present(mask)) then
+            call c_sumdd_dim(c_loc(array), c_loc(mask), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        else
+            call c_sumdd_dim(c_loc(array), c_null_ptr, inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        end if
+    end function
+
+    pure function dd_norm2_%(r)dd_dim(array, dim) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(%(C)s)
+        integer, intent(in) :: dim
+        type(real64x2), target :: res(%(RS)s)
+        integer(c_int64_t) :: inner, outer
+        integer :: d_
+        inner = 1
+        do d_ = 1, dim - 1
+            inner = inner * size(array, d_)
+        end do
+        outer = 1
+        do d_ = dim + 1, %(r)d
+            outer = outer * size(array, d_)
+        end do
+        call c_norm2dd_dim(c_loc(array), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+    end function
+
+""" % dict(r=r, C=C, RS=reduced_shape(r)))
+    for f, c in UNARY:
+        for r in range(1, MAX_RANK + 1):
+            C = colons(r)
+            w("""    pure function dd_%(f)s_%(r)dd(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(%(C)s)
+        type(real64x2), target :: res(%(S)s)
+        call c_%(c)s(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+""" % dict(f=f, r=r, C=C, c=c, S=shape_of("x", r)))
+    for op, nm, c in BINARY:
+        for r in range(1, MAX_RANK + 1):
+            C = colons(r)
+            w("""    pure function dd_%(nm)s_%(r)dd(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(%(C)s), b(%(C)s)
+        type(real64x2), target :: res(%(S)s)
+        call c_%(c)s(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+""" % dict(nm=nm, r=r, C=C, c=c, S=shape_of("a", r)))
+    w("end module multifloats_hip\n")
+    return "".join(o)
+
+
+if __name__ == "__main__":
+    with open(os.path.join(HERE, "multifloats_hip.F90"), "w") as fh:
+        fh.write(gen())

@@ -1,0 +1,1152 @@
+! GENERATED by gen_bridge.py -- edit that file, not this one.
+!
+! multifloats_hip: Fortran bridge from the reference's array generics to the
+! MI355X library libmultifloats_hip.so. Every specific here is a non-elemental
+! array procedure that hands a contiguous array to one C-ABI entry point
+! (include/multifloats_hip.h), which runs the hand-written HIP kernels and
+! returns when the result is back in host memory.
+!
+! Reference interfaces replaced (fsrc/multifloats.fypp):
+!   matmul          dd_matmul_mm/mv/vm 2568-2593, cdd_matmul_* 2609-2649
+!   dot_product     dd_dot_product 2254-2308, cdd_dot_product 2321-2331
+!   sum             dd_sum_Nd / dd_sum_Nd_dim 1980-2090 (with mask)
+!   norm2           dd_norm2_Nd / dd_norm2_Nd_dim 2338-2377
+!   exp log sin erf sqrt   elemental delegates 709-736, 1407-1412
+!   + - * /         elemental native operators 2746-2837
+!   DD_FMA_RENORM_INTERVAL and dd_set_fma_renorm_interval 664, 2677-2680
+!
+! Types: real64x2 and cmplx64x2 are SEQUENCE types with the reference's names
+! and components (fypp:170-178), so values of the reference's types are values
+! of these. Built with -DMULTIFLOATS_HIP_USE_REFERENCE_TYPES the module takes
+! the types from the reference's module instead and only adds the specifics.
+module multifloats_hip
+    use, intrinsic :: iso_c_binding
+#ifdef MULTIFLOATS_HIP_USE_REFERENCE_TYPES
+    use multifloats, only: real64x2, cmplx64x2
+#endif
+    implicit none
+    private
+
+    integer, parameter :: dp = 8
+
+#ifndef MULTIFLOATS_HIP_USE_REFERENCE_TYPES
+    type, public :: real64x2
+        sequence
+        real(dp) :: limbs(2)
+    end type real64x2
+
+    type, public :: cmplx64x2
+        sequence
+        type(real64x2) :: re, im
+    end type cmplx64x2
+#else
+    public :: real64x2, cmplx64x2
+#endif
+
+    ! by-value twins of the C structs (include/multifloats_hip.h)
+    type, bind(c) :: float64x2
+        real(c_double) :: limbs(2)
+    end type float64x2
+
+    type, bind(c) :: complex64x2
+        type(float64x2) :: re, im
+    end type complex64x2
+
+    ! Renormalisation interval of matmul and dot_product (reference default 8).
+    integer, public, protected :: DD_FMA_RENORM_INTERVAL = 8
+    public :: dd_set_fma_renorm_interval
+    public :: dd_matmul_ozaki
+    public :: matmul, dot_product, sum, norm2
+    public :: exp, log, sin, erf, sqrt
+    public :: operator(+), operator(-), operator(*), operator(/)
+
+    interface
+        pure subroutine c_matmuldd_mm(a, b, c, m, k, n, ri) bind(c, name='matmuldd_mm')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, b, c
+            integer(c_int64_t), value :: m, k, n, ri
+        end subroutine
+        pure subroutine c_matmuldd_mv(a, x, y, m, k, ri) bind(c, name='matmuldd_mv')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, x, y
+            integer(c_int64_t), value :: m, k, ri
+        end subroutine
+        pure subroutine c_matmuldd_vm(x, b, y, k, n, ri) bind(c, name='matmuldd_vm')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: x, b, y
+            integer(c_int64_t), value :: k, n, ri
+        end subroutine
+        pure subroutine c_cmatmuldd_mm(a, b, c, m, k, n, ri) bind(c, name='cmatmuldd_mm')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, b, c
+            integer(c_int64_t), value :: m, k, n, ri
+        end subroutine
+        pure subroutine c_cmatmuldd_mv(a, x, y, m, k, ri) bind(c, name='cmatmuldd_mv')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, x, y
+            integer(c_int64_t), value :: m, k, ri
+        end subroutine
+        pure subroutine c_cmatmuldd_vm(x, b, y, k, n, ri) bind(c, name='cmatmuldd_vm')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: x, b, y
+            integer(c_int64_t), value :: k, n, ri
+        end subroutine
+        subroutine c_matmuldd_mm_ozaki(a, b, c, m, k, n, slices) bind(c, name='matmuldd_mm_ozaki')
+            import :: c_ptr, c_int64_t, c_int
+            type(c_ptr), value :: a, b, c
+            integer(c_int64_t), value :: m, k, n
+            integer(c_int), value :: slices
+        end subroutine
+        pure type(float64x2) function c_dotdd(a, b, n, ri) bind(c, name='dotdd')
+            import :: c_ptr, c_int64_t, float64x2
+            type(c_ptr), value :: a, b
+            integer(c_int64_t), value :: n, ri
+        end function
+        pure type(complex64x2) function c_cdotdd(a, b, n, ri) bind(c, name='cdotdd')
+            import :: c_ptr, c_int64_t, complex64x2
+            type(c_ptr), value :: a, b
+            integer(c_int64_t), value :: n, ri
+        end function
+        pure type(float64x2) function c_sumdd_mask(a, mask, n) bind(c, name='sumdd_mask')
+            import :: c_ptr, c_int64_t, float64x2
+            type(c_ptr), value :: a, mask
+            integer(c_int64_t), value :: n
+        end function
+        pure type(float64x2) function c_norm2dd(a, n) bind(c, name='norm2dd')
+            import :: c_ptr, c_int64_t, float64x2
+            type(c_ptr), value :: a
+            integer(c_int64_t), value :: n
+        end function
+        pure subroutine c_sumdd_dim(a, mask, inner, len, outer, out) bind(c, name='sumdd_dim')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, mask, out
+            integer(c_int64_t), value :: inner, len, outer
+        end subroutine
+        pure subroutine c_norm2dd_dim(a, inner, len, outer, out) bind(c, name='norm2dd_dim')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, out
+            integer(c_int64_t), value :: inner, len, outer
+        end subroutine
+        pure subroutine c_expdd_v(a, c, n) bind(c, name='expdd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_logdd_v(a, c, n) bind(c, name='logdd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_sindd_v(a, c, n) bind(c, name='sindd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_erfdd_v(a, c, n) bind(c, name='erfdd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_sqrtdd_v(a, c, n) bind(c, name='sqrtdd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_adddd_v(a, b, c, n) bind(c, name='adddd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, b, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_subdd_v(a, b, c, n) bind(c, name='subdd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, b, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_muldd_v(a, b, c, n) bind(c, name='muldd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, b, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_divdd_v(a, b, c, n) bind(c, name='divdd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, b, c
+            integer(c_int64_t), value :: n
+        end subroutine
+    end interface
+    interface matmul
+        module procedure dd_matmul_mm, dd_matmul_mv, dd_matmul_vm
+        module procedure cdd_matmul_mm, cdd_matmul_mv, cdd_matmul_vm
+    end interface
+
+    interface dot_product
+        module procedure dd_dot_product, cdd_dot_product
+    end interface
+    interface sum
+        module procedure dd_sum_1d
+        module procedure dd_sum_2d
+        module procedure dd_sum_2d_dim
+        module procedure dd_sum_3d
+        module procedure dd_sum_3d_dim
+        module procedure dd_sum_4d
+        module procedure dd_sum_4d_dim
+        module procedure dd_sum_5d
+        module procedure dd_sum_5d_dim
+        module procedure dd_sum_6d
+        module procedure dd_sum_6d_dim
+        module procedure dd_sum_7d
+        module procedure dd_sum_7d_dim
+    end interface
+
+    interface norm2
+        module procedure dd_norm2_1d
+        module procedure dd_norm2_2d
+        module procedure dd_norm2_2d_dim
+        module procedure dd_norm2_3d
+        module procedure dd_norm2_3d_dim
+        module procedure dd_norm2_4d
+        module procedure dd_norm2_4d_dim
+        module procedure dd_norm2_5d
+        module procedure dd_norm2_5d_dim
+        module procedure dd_norm2_6d
+        module procedure dd_norm2_6d_dim
+        module procedure dd_norm2_7d
+        module procedure dd_norm2_7d_dim
+    end interface
+
+    interface exp
+        module procedure dd_exp_1d
+        module procedure dd_exp_2d
+        module procedure dd_exp_3d
+        module procedure dd_exp_4d
+        module procedure dd_exp_5d
+        module procedure dd_exp_6d
+        module procedure dd_exp_7d
+    end interface
+
+    interface log
+        module procedure dd_log_1d
+        module procedure dd_log_2d
+        module procedure dd_log_3d
+        module procedure dd_log_4d
+        module procedure dd_log_5d
+        module procedure dd_log_6d
+        module procedure dd_log_7d
+    end interface
+
+    interface sin
+        module procedure dd_sin_1d
+        module procedure dd_sin_2d
+        module procedure dd_sin_3d
+        module procedure dd_sin_4d
+        module procedure dd_sin_5d
+        module procedure dd_sin_6d
+        module procedure dd_sin_7d
+    end interface
+
+    interface erf
+        module procedure dd_erf_1d
+        module procedure dd_erf_2d
+        module procedure dd_erf_3d
+        module procedure dd_erf_4d
+        module procedure dd_erf_5d
+        module procedure dd_erf_6d
+        module procedure dd_erf_7d
+    end interface
+
+    interface sqrt
+        module procedure dd_sqrt_1d
+        module procedure dd_sqrt_2d
+        module procedure dd_sqrt_3d
+        module procedure dd_sqrt_4d
+        module procedure dd_sqrt_5d
+        module procedure dd_sqrt_6d
+        module procedure dd_sqrt_7d
+    end interface
+
+    interface operator(+)
+        module procedure dd_add_1d
+        module procedure dd_add_2d
+        module procedure dd_add_3d
+        module procedure dd_add_4d
+        module procedure dd_add_5d
+        module procedure dd_add_6d
+        module procedure dd_add_7d
+    end interface
+
+    interface operator(-)
+        module procedure dd_sub_1d
+        module procedure dd_sub_2d
+        module procedure dd_sub_3d
+        module procedure dd_sub_4d
+        module procedure dd_sub_5d
+        module procedure dd_sub_6d
+        module procedure dd_sub_7d
+    end interface
+
+    interface operator(*)
+        module procedure dd_mul_1d
+        module procedure dd_mul_2d
+        module procedure dd_mul_3d
+        module procedure dd_mul_4d
+        module procedure dd_mul_5d
+        module procedure dd_mul_6d
+        module procedure dd_mul_7d
+    end interface
+
+    interface operator(/)
+        module procedure dd_div_1d
+        module procedure dd_div_2d
+        module procedure dd_div_3d
+        module procedure dd_div_4d
+        module procedure dd_div_5d
+        module procedure dd_div_6d
+        module procedure dd_div_7d
+    end interface
+
+contains
+
+    subroutine dd_set_fma_renorm_interval(n)
+        integer, intent(in) :: n
+        DD_FMA_RENORM_INTERVAL = n
+    end subroutine
+
+    pure function ri() result(r)
+        integer(c_int64_t) :: r
+        r = int(DD_FMA_RENORM_INTERVAL, c_int64_t)
+    end function
+
+    pure function dd_matmul_mm(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:), b(:,:)
+        type(real64x2), target :: res(size(a,1), size(b,2))
+        call c_matmuldd_mm(c_loc(a), c_loc(b), c_loc(res), int(size(a,1), c_int64_t), &
+            int(size(a,2), c_int64_t), int(size(b,2), c_int64_t), ri())
+    end function
+
+    pure function dd_matmul_mv(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:), b(:)
+        type(real64x2), target :: res(size(a,1))
+        call c_matmuldd_mv(c_loc(a), c_loc(b), c_loc(res), int(size(a,1), c_int64_t), &
+            int(size(a,2), c_int64_t), ri())
+    end function
+
+    pure function dd_matmul_vm(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:), b(:,:)
+        type(real64x2), target :: res(size(b,2))
+        call c_matmuldd_vm(c_loc(a), c_loc(b), c_loc(res), int(size(a), c_int64_t), &
+            int(size(b,2), c_int64_t), ri())
+    end function
+
+    ! Complex matmul: one fused kernel, bit-identical to the reference's
+    ! four-real-matmul composition (fypp:2609-2621).
+    pure function cdd_matmul_mm(a, b) result(res)
+        type(cmplx64x2), intent(in), target, contiguous :: a(:,:), b(:,:)
+        type(cmplx64x2), target :: res(size(a,1), size(b,2))
+        call c_cmatmuldd_mm(c_loc(a), c_loc(b), c_loc(res), int(size(a,1), c_int64_t), &
+            int(size(a,2), c_int64_t), int(size(b,2), c_int64_t), ri())
+    end function
+
+    pure function cdd_matmul_mv(a, b) result(res)
+        type(cmplx64x2), intent(in), target, contiguous :: a(:,:), b(:)
+        type(cmplx64x2), target :: res(size(a,1))
+        call c_cmatmuldd_mv(c_loc(a), c_loc(b), c_loc(res), int(size(a,1), c_int64_t), &
+            int(size(a,2), c_int64_t), ri())
+    end function
+
+    pure function cdd_matmul_vm(a, b) result(res)
+        type(cmplx64x2), intent(in), target, contiguous :: a(:), b(:,:)
+        type(cmplx64x2), target :: res(size(b,2))
+        call c_cmatmuldd_vm(c_loc(a), c_loc(b), c_loc(res), int(size(a), c_int64_t), &
+            int(size(b,2), c_int64_t), ri())
+    end function
+
+    ! Precision form (no reference counterpart): Ozaki INT8 split GEMM,
+    ! |C - AB| <= 2^-100 (|A||B|); slices absent or <= 0 lets the library
+    ! choose from the data.
+    function dd_matmul_ozaki(a, b, slices) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:), b(:,:)
+        integer, intent(in), optional :: slices
+        type(real64x2), target :: res(size(a,1), size(b,2))
+        integer(c_int) :: s
+        s = 0
+        if (present(slices)) s = int(slices, c_int)
+        call c_matmuldd_mm_ozaki(c_loc(a), c_loc(b), c_loc(res), int(size(a,1), c_int64_t), &
+            int(size(a,2), c_int64_t), int(size(b,2), c_int64_t), s)
+    end function
+
+    pure function dd_dot_product(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:), b(:)
+        type(real64x2) :: res
+        type(float64x2) :: r
+        r = c_dotdd(c_loc(a), c_loc(b), int(min(size(a), size(b)), c_int64_t), ri())
+        res%limbs = r%limbs
+    end function
+
+    pure function cdd_dot_product(a, b) result(res)
+        type(cmplx64x2), intent(in), target, contiguous :: a(:), b(:)
+        type(cmplx64x2) :: res
+        type(complex64x2) :: r
+        r = c_cdotdd(c_loc(a), c_loc(b), int(min(size(a), size(b)), c_int64_t), ri())
+        res%re%limbs = r%re%limbs
+        res%im%limbs = r%im%limbs
+    end function
+
+    pure function dd_sum_1d(array, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:)
+        logical, intent(in), target, contiguous, optional :: mask(:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        if (present(mask)) then
+            s = c_sumdd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
+        else
+            s = c_sumdd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
+        end if
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_norm2_1d(array) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        s = c_norm2dd(c_loc(array), int(size(array), c_int64_t))
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_sum_2d(array, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:)
+        logical, intent(in), target, contiguous, optional :: mask(:,:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        if (present(mask)) then
+            s = c_sumdd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
+        else
+            s = c_sumdd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
+        end if
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_norm2_2d(array) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        s = c_norm2dd(c_loc(array), int(size(array), c_int64_t))
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_sum_2d_dim(array, dim, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:)
+        integer, intent(in) :: dim
+        logical, intent(in), target, contiguous, optional :: mask(:,:)
+        type(real64x2), target :: res(size(array, 1 + merge(1, 0, 1 >= dim)))
+        integer(c_int64_t) :: inner, outer
+        integer :: d_
+        inner = 1
+        do d_ = 1, dim - 1
+            inner = inner * size(array, d_)
+        end do
+        outer = 1
+        do d_ = dim + 1, 2
+            outer = outer * size(array, d_)
+        end do
+        if (present(mask)) then
+            call c_sumdd_dim(c_loc(array), c_loc(mask), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        else
+            call c_sumdd_dim(c_loc(array), c_null_ptr, inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        end if
+    end function
+
+    pure function dd_norm2_2d_dim(array, dim) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:)
+        integer, intent(in) :: dim
+        type(real64x2), target :: res(size(array, 1 + merge(1, 0, 1 >= dim)))
+        integer(c_int64_t) :: inner, outer
+        integer :: d_
+        inner = 1
+        do d_ = 1, dim - 1
+            inner = inner * size(array, d_)
+        end do
+        outer = 1
+        do d_ = dim + 1, 2
+            outer = outer * size(array, d_)
+        end do
+        call c_norm2dd_dim(c_loc(array), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+    end function
+
+    pure function dd_sum_3d(array, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:)
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        if (present(mask)) then
+            s = c_sumdd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
+        else
+            s = c_sumdd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
+        end if
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_norm2_3d(array) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        s = c_norm2dd(c_loc(array), int(size(array), c_int64_t))
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_sum_3d_dim(array, dim, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:)
+        integer, intent(in) :: dim
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:)
+        type(real64x2), target :: res(size(array, 1 + merge(1, 0, 1 >= dim)), size(array, 2 + merge(1, 0, 2 >= dim)))
+        integer(c_int64_t) :: inner, outer
+        integer :: d_
+        inner = 1
+        do d_ = 1, dim - 1
+            inner = inner * size(array, d_)
+        end do
+        outer = 1
+        do d_ = dim + 1, 3
+            outer = outer * size(array, d_)
+        end do
+        if (present(mask)) then
+            call c_sumdd_dim(c_loc(array), c_loc(mask), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        else
+            call c_sumdd_dim(c_loc(array), c_null_ptr, inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        end if
+    end function
+
+    pure function dd_norm2_3d_dim(array, dim) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:)
+        integer, intent(in) :: dim
+        type(real64x2), target :: res(size(array, 1 + merge(1, 0, 1 >= dim)), size(array, 2 + merge(1, 0, 2 >= dim)))
+        integer(c_int64_t) :: inner, outer
+        integer :: d_
+        inner = 1
+        do d_ = 1, dim - 1
+            inner = inner * size(array, d_)
+        end do
+        outer = 1
+        do d_ = dim + 1, 3
+            outer = outer * size(array, d_)
+        end do
+        call c_norm2dd_dim(c_loc(array), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+    end function
+
+    pure function dd_sum_4d(array, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:)
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:,:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        if (present(mask)) then
+            s = c_sumdd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
+        else
+            s = c_sumdd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
+        end if
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_norm2_4d(array) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        s = c_norm2dd(c_loc(array), int(size(array), c_int64_t))
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_sum_4d_dim(array, dim, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:)
+        integer, intent(in) :: dim
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:,:)
+        type(real64x2), target :: res(size(array, 1 + merge(1, 0, 1 >= dim)), size(array, 2 + merge(1, 0, 2 >= dim)), size(array, 3 + merge(1, 0, 3 >= dim)))
+        integer(c_int64_t) :: inner, outer
+        integer :: d_
+        inner = 1
+        do d_ = 1, dim - 1
+            inner = inner * size(array, d_)
+        end do
+        outer = 1
+        do d_ = dim + 1, 4
+            outer = outer * size(array, d_)
+        end do
+        if (present(mask)) then
+            call c_sumdd_dim(c_loc(array), c_loc(mask), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        else
+            call c_sumdd_dim(c_loc(array), c_null_ptr, inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        end if
+    end function
+
+    pure function dd_norm2_4d_dim(array, dim) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:)
+        integer, intent(in) :: dim
+        type(real64x2), target :: res(size(array, 1 + merge(1, 0, 1 >= dim)), size(array, 2 + merge(1, 0, 2 >= dim)), size(array, 3 + merge(1, 0, 3 >= dim)))
+        integer(c_int64_t) :: inner, outer
+        integer :: d_
+        inner = 1
+        do d_ = 1, dim - 1
+            inner = inner * size(array, d_)
+        end do
+        outer = 1
+        do d_ = dim + 1, 4
+            outer = outer * size(array, d_)
+        end do
+        call c_norm2dd_dim(c_loc(array), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+    end function
+
+    pure function dd_sum_5d(array, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:,:)
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:,:,:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        if (present(mask)) then
+            s = c_sumdd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
+        else
+            s = c_sumdd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
+        end if
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_norm2_5d(array) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:,:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        s = c_norm2dd(c_loc(array), int(size(array), c_int64_t))
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_sum_5d_dim(array, dim, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:,:)
+        integer, intent(in) :: dim
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:,:,:)
+        type(real64x2), target :: res(size(array, 1 + merge(1, 0, 1 >= dim)), size(array, 2 + merge(1, 0, 2 >= dim)), size(array, 3 + merge(1, 0, 3 >= dim)), size(array, 4 + merge(1, 0, 4 >= dim)))
+        integer(c_int64_t) :: inner, outer
+        integer :: d_
+        inner = 1
+        do d_ = 1, dim - 1
+            inner = inner * size(array, d_)
+        end do
+        outer = 1
+        do d_ = dim + 1, 5
+            outer = outer * size(array, d_)
+        end do
+        if (present(mask)) then
+            call c_sumdd_dim(c_loc(array), c_loc(mask), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        else
+            call c_sumdd_dim(c_loc(array), c_null_ptr, inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        end if
+    end function
+
+    pure function dd_norm2_5d_dim(array, dim) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:,:)
+        integer, intent(in) :: dim
+        type(real64x2), target :: res(size(array, 1 + merge(1, 0, 1 >= dim)), size(array, 2 + merge(1, 0, 2 >= dim)), size(array, 3 + merge(1, 0, 3 >= dim)), size(array, 4 + merge(1, 0, 4 >= dim)))
+        integer(c_int64_t) :: inner, outer
+        integer :: d_
+        inner = 1
+        do d_ = 1, dim - 1
+            inner = inner * size(array, d_)
+        end do
+        outer = 1
+        do d_ = dim + 1, 5
+            outer = outer * size(array, d_)
+        end do
+        call c_norm2dd_dim(c_loc(array), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+    end function
+
+    pure function dd_sum_6d(array, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:,:,:)
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:,:,:,:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        if (present(mask)) then
+            s = c_sumdd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
+        else
+            s = c_sumdd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
+        end if
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_norm2_6d(array) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:,:,:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        s = c_norm2dd(c_loc(array), int(size(array), c_int64_t))
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_sum_6d_dim(array, dim, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:,:,:)
+        integer, intent(in) :: dim
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(array, 1 + merge(1, 0, 1 >= dim)), size(array, 2 + merge(1, 0, 2 >= dim)), size(array, 3 + merge(1, 0, 3 >= dim)), size(array, 4 + merge(1, 0, 4 >= dim)), size(array, 5 + merge(1, 0, 5 >= dim)))
+        integer(c_int64_t) :: inner, outer
+        integer :: d_
+        inner = 1
+        do d_ = 1, dim - 1
+            inner = inner * size(array, d_)
+        end do
+        outer = 1
+        do d_ = dim + 1, 6
+            outer = outer * size(array, d_)
+        end do
+        if (present(mask)) then
+            call c_sumdd_dim(c_loc(array), c_loc(mask), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        else
+            call c_sumdd_dim(c_loc(array), c_null_ptr, inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        end if
+    end function
+
+    pure function dd_norm2_6d_dim(array, dim) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:,:,:)
+        integer, intent(in) :: dim
+        type(real64x2), target :: res(size(array, 1 + merge(1, 0, 1 >= dim)), size(array, 2 + merge(1, 0, 2 >= dim)), size(array, 3 + merge(1, 0, 3 >= dim)), size(array, 4 + merge(1, 0, 4 >= dim)), size(array, 5 + merge(1, 0, 5 >= dim)))
+        integer(c_int64_t) :: inner, outer
+        integer :: d_
+        inner = 1
+        do d_ = 1, dim - 1
+            inner = inner * size(array, d_)
+        end do
+        outer = 1
+        do d_ = dim + 1, 6
+            outer = outer * size(array, d_)
+        end do
+        call c_norm2dd_dim(c_loc(array), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+    end function
+
+    pure function dd_sum_7d(array, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:,:,:,:)
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:,:,:,:,:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        if (present(mask)) then
+            s = c_sumdd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
+        else
+            s = c_sumdd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
+        end if
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_norm2_7d(array) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:,:,:,:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        s = c_norm2dd(c_loc(array), int(size(array), c_int64_t))
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_sum_7d_dim(array, dim, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:,:,:,:)
+        integer, intent(in) :: dim
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(array, 1 + merge(1, 0, 1 >= dim)), size(array, 2 + merge(1, 0, 2 >= dim)), size(array, 3 + merge(1, 0, 3 >= dim)), size(array, 4 + merge(1, 0, 4 >= dim)), size(array, 5 + merge(1, 0, 5 >= dim)), size(array, 6 + merge(1, 0, 6 >= dim)))
+        integer(c_int64_t) :: inner, outer
+        integer :: d_
+        inner = 1
+        do d_ = 1, dim - 1
+            inner = inner * size(array, d_)
+        end do
+        outer = 1
+        do d_ = dim + 1, 7
+            outer = outer * size(array, d_)
+        end do
+        if (present(mask)) then
+            call c_sumdd_dim(c_loc(array), c_loc(mask), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        else
+            call c_sumdd_dim(c_loc(array), c_null_ptr, inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        end if
+    end function
+
+    pure function dd_norm2_7d_dim(array, dim) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:,:,:,:)
+        integer, intent(in) :: dim
+        type(real64x2), target :: res(size(array, 1 + merge(1, 0, 1 >= dim)), size(array, 2 + merge(1, 0, 2 >= dim)), size(array, 3 + merge(1, 0, 3 >= dim)), size(array, 4 + merge(1, 0, 4 >= dim)), size(array, 5 + merge(1, 0, 5 >= dim)), size(array, 6 + merge(1, 0, 6 >= dim)))
+        integer(c_int64_t) :: inner, outer
+        integer :: d_
+        inner = 1
+        do d_ = 1, dim - 1
+            inner = inner * size(array, d_)
+        end do
+        outer = 1
+        do d_ = dim + 1, 7
+            outer = outer * size(array, d_)
+        end do
+        call c_norm2dd_dim(c_loc(array), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+    end function
+
+    pure function dd_exp_1d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:)
+        type(real64x2), target :: res(size(x,1))
+        call c_expdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_exp_2d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2))
+        call c_expdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_exp_3d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3))
+        call c_expdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_exp_4d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4))
+        call c_expdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_exp_5d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5))
+        call c_expdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_exp_6d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6))
+        call c_expdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_exp_7d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6), size(x,7))
+        call c_expdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_log_1d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:)
+        type(real64x2), target :: res(size(x,1))
+        call c_logdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_log_2d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2))
+        call c_logdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_log_3d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3))
+        call c_logdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_log_4d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4))
+        call c_logdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_log_5d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5))
+        call c_logdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_log_6d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6))
+        call c_logdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_log_7d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6), size(x,7))
+        call c_logdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_sin_1d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:)
+        type(real64x2), target :: res(size(x,1))
+        call c_sindd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_sin_2d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2))
+        call c_sindd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_sin_3d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3))
+        call c_sindd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_sin_4d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4))
+        call c_sindd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_sin_5d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5))
+        call c_sindd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_sin_6d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6))
+        call c_sindd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_sin_7d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6), size(x,7))
+        call c_sindd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_erf_1d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:)
+        type(real64x2), target :: res(size(x,1))
+        call c_erfdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_erf_2d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2))
+        call c_erfdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_erf_3d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3))
+        call c_erfdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_erf_4d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4))
+        call c_erfdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_erf_5d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5))
+        call c_erfdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_erf_6d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6))
+        call c_erfdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_erf_7d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6), size(x,7))
+        call c_erfdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_sqrt_1d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:)
+        type(real64x2), target :: res(size(x,1))
+        call c_sqrtdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_sqrt_2d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2))
+        call c_sqrtdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_sqrt_3d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3))
+        call c_sqrtdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_sqrt_4d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4))
+        call c_sqrtdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_sqrt_5d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5))
+        call c_sqrtdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_sqrt_6d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6))
+        call c_sqrtdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_sqrt_7d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6), size(x,7))
+        call c_sqrtdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_add_1d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:), b(:)
+        type(real64x2), target :: res(size(a,1))
+        call c_adddd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_add_2d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:), b(:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2))
+        call c_adddd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_add_3d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:), b(:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3))
+        call c_adddd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_add_4d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:), b(:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4))
+        call c_adddd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_add_5d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:), b(:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5))
+        call c_adddd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_add_6d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:,:), b(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5), size(a,6))
+        call c_adddd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_add_7d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:,:,:), b(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5), size(a,6), size(a,7))
+        call c_adddd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_sub_1d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:), b(:)
+        type(real64x2), target :: res(size(a,1))
+        call c_subdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_sub_2d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:), b(:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2))
+        call c_subdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_sub_3d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:), b(:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3))
+        call c_subdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_sub_4d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:), b(:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4))
+        call c_subdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_sub_5d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:), b(:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5))
+        call c_subdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_sub_6d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:,:), b(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5), size(a,6))
+        call c_subdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_sub_7d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:,:,:), b(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5), size(a,6), size(a,7))
+        call c_subdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_mul_1d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:), b(:)
+        type(real64x2), target :: res(size(a,1))
+        call c_muldd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_mul_2d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:), b(:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2))
+        call c_muldd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_mul_3d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:), b(:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3))
+        call c_muldd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_mul_4d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:), b(:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4))
+        call c_muldd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_mul_5d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:), b(:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5))
+        call c_muldd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_mul_6d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:,:), b(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5), size(a,6))
+        call c_muldd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_mul_7d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:,:,:), b(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5), size(a,6), size(a,7))
+        call c_muldd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_div_1d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:), b(:)
+        type(real64x2), target :: res(size(a,1))
+        call c_divdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_div_2d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:), b(:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2))
+        call c_divdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_div_3d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:), b(:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3))
+        call c_divdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_div_4d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:), b(:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4))
+        call c_divdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_div_5d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:), b(:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5))
+        call c_divdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_div_6d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:,:), b(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5), size(a,6))
+        call c_divdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_div_7d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:,:,:), b(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5), size(a,6), size(a,7))
+        call c_divdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+end module multifloats_hip

@@ -238,6 +238,74 @@ dd or_norm2(const dd *a, int64_t n) {
   return o_sqrt(s);
 }
 
+/* fypp:1980-2020 with mask present: masked-out elements are skipped
+ * (`cycle`), not added. */
+dd or_sum_mask(const dd *a, const int32_t *mask, int64_t n) {
+  double sh = 0.0, sl = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    if (!mask[i]) continue;
+    double xh = a[i].hi, xl = a[i].lo;
+    double t = sh + xh, bp = t - sh, ap = t - bp;
+    double aerr = sh - ap, berr = xh - bp;
+    sl = sl + ((xl + aerr) + berr);
+    sh = t;
+  }
+  double t = sh + sl, bp = t - sh;
+  return mkdd(t, sl - bp);
+}
+
+/* fypp:2321-2331 (cdd_dot_product): conj(a).b as four real dot_products,
+ * re = dot(ar,br) + dot(ai,bi), im = dot(ar,bi) - dot(ai,br). Complex
+ * elements are {re, im} pairs of dd. */
+void or_cdot(const dd *a, const dd *b, int64_t n, int64_t ri, dd *out) {
+  size_t nn = n > 0 ? (size_t)n : 1;
+  dd *ar = malloc(nn * sizeof(dd)), *ai = malloc(nn * sizeof(dd));
+  dd *br = malloc(nn * sizeof(dd)), *bi = malloc(nn * sizeof(dd));
+  for (int64_t i = 0; i < n; ++i) {
+    ar[i] = a[2 * i]; ai[i] = a[2 * i + 1];
+    br[i] = b[2 * i]; bi[i] = b[2 * i + 1];
+  }
+  out[0] = o_add(or_dot(ar, br, n, ri), or_dot(ai, bi, n, ri));
+  out[1] = o_sub(or_dot(ar, bi, n, ri), or_dot(ai, br, n, ri));
+  free(ar); free(ai); free(br); free(bi);
+}
+
+/* fypp:2035-2090 (dd_sum_Nd_dim) on an (inner, len, outer) column-major view:
+ * out(i, o) is the Neumaier chain over j (masked elements skipped), closed by
+ * the full TwoSum in the reference's order (t, bp, ap, aerr, berr). */
+void or_sum_dim(const dd *a, const int32_t *mask, int64_t inner, int64_t len, int64_t outer, dd *out) {
+  for (int64_t o = 0; o < outer; ++o)
+    for (int64_t i = 0; i < inner; ++i) {
+      double sh = 0.0, sl = 0.0;
+      for (int64_t j = 0; j < len; ++j) {
+        int64_t e = i + inner * (j + len * o);
+        if (mask && !mask[e]) continue;
+        double xh = a[e].hi, xl = a[e].lo;
+        double t = sh + xh, bp = t - sh, ap = t - bp;
+        double aerr = sh - ap, berr = xh - bp;
+        sl = sl + ((xl + aerr) + berr);
+        sh = t;
+      }
+      double t = sh + sl, bp = t - sh, ap = t - bp;
+      double aerr = sh - ap, berr = sl - bp;
+      out[i + inner * o] = mkdd(t, aerr + berr);
+    }
+}
+
+/* fypp:2356-2377 (dd_norm2_Nd_dim): cur = cur + x*x with the DD operators
+ * along dimension `dim`, then DD sqrt. */
+void or_norm2_dim(const dd *a, int64_t inner, int64_t len, int64_t outer, dd *out) {
+  for (int64_t o = 0; o < outer; ++o)
+    for (int64_t i = 0; i < inner; ++i) {
+      dd s = mkdd(0.0, 0.0);
+      for (int64_t j = 0; j < len; ++j) {
+        dd x = a[i + inner * (j + len * o)];
+        s = o_add(s, o_mul(x, x));
+      }
+      out[i + inner * o] = o_sqrt(s);
+    }
+}
+
 /* ------------------------------------------------------ transcendentals */
 static inline dd cdd(const double *hi, const double *lo, int i) { return mkdd(hi[i], lo[i]); }
 

@@ -43,6 +43,11 @@ def oracle():
         L.or_sum.restype = F2
         L.or_norm2.argtypes = [P, I64]
         L.or_norm2.restype = F2
+        L.or_sum_mask.argtypes = [P, P, I64]
+        L.or_sum_mask.restype = F2
+        L.or_cdot.argtypes = [P, P, I64, I64, P]
+        L.or_sum_dim.argtypes = [P, P, I64, I64, I64, P]
+        L.or_norm2_dim.argtypes = [P, I64, I64, I64, P]
         D = ctypes.POINTER(ctypes.c_double)
         L.or_exact_dot_err.argtypes = [P, I64, P, I64, I64, P, D, D]
         L.or_exact_dot.argtypes = [P, I64, P, I64, I64, P]
@@ -130,6 +135,33 @@ def gemm_err(A, B, C, rows, cols):
     mag = np.zeros(rows.size)
     oracle().or_gemm_err(mptr(A), mptr(B), mptr(C), m, k, n, ptr(rows), ptr(cols), rows.size, ptr(err), ptr(mag))
     return err, mag
+
+
+def or_sum_mask(x, mask):
+    x = dd(x)
+    m = np.ascontiguousarray(mask, dtype=np.int32)
+    return res(oracle().or_sum_mask(ptr(x), ptr(m), x.shape[0]))
+
+
+def or_cdot(a, b, ri):
+    """conj(a).b for (n, 4) complex DD vectors -> [re.hi, re.lo, im.hi, im.lo]."""
+    a, b = dd(a), dd(b)
+    out = np.zeros(4)
+    oracle().or_cdot(ptr(a), ptr(b), a.shape[0], ri, ptr(out))
+    return out
+
+
+def or_reduce_dim(kind, buf, mask, inner, length, outer):
+    """Oracle sum / norm2 along the middle axis of an (inner, len, outer)
+    column-major DD buffer (flat (inner*len*outer, 2)); returns (inner*outer, 2)."""
+    buf = dd(buf)
+    out = np.zeros((max(inner * outer, 1), 2))
+    if kind == "sum":
+        mb = None if mask is None else np.ascontiguousarray(mask, dtype=np.int32)
+        oracle().or_sum_dim(ptr(buf), None if mb is None else ptr(mb), inner, length, outer, ptr(out))
+    else:
+        oracle().or_norm2_dim(ptr(buf), inner, length, outer, ptr(out))
+    return out[:inner * outer]
 
 
 def bits_equal(x, y):

@@ -234,3 +234,92 @@ def test_complex_matmul_2048_sampled(torch_cuda):
     nt.oracle().or_cmatmul_mm(nt.ptr(As), nt.ptr(Bs), nt.ptr(want), 8, k, 8, 8)
     got = np.ascontiguousarray(C[np.ix_(rows, cols)].transpose(1, 0, 2)).reshape(-1, 4)
     nt.assert_bits(got, want, "complex 2048")
+
+
+# ---- sum / norm2 along a dimension, masked sum, complex dot ----
+
+DIM_SHAPES = [((1000, 37), 1), ((1000, 37), 2), ((5, 7, 300), 1), ((5, 7, 300), 2), ((5, 7, 300), 3),
+              ((3, 0, 4), 2), ((1, 1), 1), ((257, 3), 1), ((64, 65, 2, 3), 3)]
+
+
+@pytest.mark.parametrize("shape,dim", DIM_SHAPES)
+@pytest.mark.parametrize("masked", [False, True])
+def test_sum_norm2_dim_bits_vs_oracle(gpu, shape, dim, masked):
+    rng = np.random.default_rng(hash((shape, dim, masked)) % (1 << 32))
+    nel = int(np.prod(shape))
+    x = datagen.config0(rng, max(nel, 1))[:nel].reshape(tuple(reversed(shape)) + (2,))
+    x = x.transpose(list(range(len(shape) - 1, -1, -1)) + [len(shape)])  # logical shape, Fortran memory
+    mask = (rng.random(shape) < 0.6) if masked else None
+    inner = int(np.prod(shape[:dim - 1]))
+    ln = shape[dim - 1]
+    outer = int(np.prod(shape[dim:]))
+    buf = mf._fortran_order(x).reshape(-1, 2)
+    mbuf = None if mask is None else mf._mask_buf(mask, shape).reshape(-1)
+    got = mf.sum(x, dim=dim, mask=mask) if len(shape) > 1 else None
+    want = nt.or_reduce_dim("sum", buf, mbuf, inner, ln, outer)
+    nt.assert_bits(mf._fortran_order(got).reshape(-1, 2), want, "sum dim=%d %s" % (dim, shape))
+    if not masked:
+        got = mf.norm2(x, dim=dim)
+        want = nt.or_reduce_dim("norm2", buf, None, inner, ln, outer)
+        nt.assert_bits(mf._fortran_order(got).reshape(-1, 2), want, "norm2 dim=%d %s" % (dim, shape))
+
+
+@pytest.mark.parametrize("n", [0, 1, 1000, 65537, 1 << 21])
+def test_masked_sum_bound(gpu, n):
+    rng = np.random.default_rng(n + 5)
+    x = datagen.sum_inputs(rng, max(n, 1))[:n]
+    mask = rng.random(n) < 0.5
+    s = mf.sum(x, mask=mask)
+    err, mag = nt.exact_sum_err(np.ascontiguousarray(x[mask]) if n else x, s)
+    assert err <= 2 * max(n, 1) * 2.0 ** -104 * mag + 1e-300
+    assert mf.sum(x, mask=np.zeros(n, bool)).tolist() == [0.0, 0.0]
+    if n:
+        nt.assert_bits(mf.sum(x, mask=np.ones(n, bool)), mf.sum(x), "all-true mask = no mask")
+
+
+@pytest.mark.parametrize("n", [0, 1, 3000, 1 << 20])
+@pytest.mark.parametrize("ri", [0, 8])
+def test_complex_dot_bound(gpu, n, ri):
+    rng = np.random.default_rng(n + 11 + ri)
+    N = max(n, 1)
+    a = np.concatenate([datagen.config0(rng, N), datagen.config0(rng, N)], axis=1)[:n]
+    b = np.concatenate([datagen.config0(rng, N), datagen.config0(rng, N)], axis=1)[:n]
+    r = mf.dot_product(a, b, ri)
+    if n == 0:
+        assert r.tolist() == [0.0] * 4
+        return
+    ar, ai, br, bi = a[:, :2], a[:, 2:], b[:, :2], b[:, 2:]
+    e, m = nt.exact_dot_err(np.concatenate([ar, ai]), np.concatenate([br, bi]), r[:2])
+    assert e <= 4 * n * 2.0 ** -104 * m
+    e, m = nt.exact_dot_err(np.concatenate([ar, -ai]), np.concatenate([bi, br]), r[2:])
+    assert e <= 4 * n * 2.0 ** -104 * m
+    r2 = mf.dot_product(a, a, ri)
+    assert r2[2] == 0.0 and r2[3] == 0.0
+
+
+def test_device_reduction_entry_points(torch_cuda):
+    torch = torch_cuda
+    rng = np.random.default_rng(12)
+    n = 100_003
+    x = datagen.sum_inputs(rng, n)
+    mask = rng.random(n) < 0.5
+    dx = to_dev(torch, x)
+    dm = to_dev(torch, mask.astype(np.int32))
+    out = torch.zeros(4, dtype=torch.float64, device="cuda")
+    work = torch.empty(mf.cdot_workspace_bytes(n) // 8 + 4, dtype=torch.float64, device="cuda")
+    mf.dev_sum_masked(dx.data_ptr(), dm.data_ptr(), n, out.data_ptr(), work.data_ptr())
+    torch.cuda.synchronize()
+    nt.assert_bits(out[:2].cpu().numpy(), mf.sum(x, mask=mask), "device masked sum = host entry")
+    a = np.concatenate([datagen.config0(rng, n), datagen.config0(rng, n)], axis=1)
+    b = np.concatenate([datagen.config0(rng, n), datagen.config0(rng, n)], axis=1)
+    da, db = to_dev(torch, a), to_dev(torch, b)
+    mf.dev_cdot(da.data_ptr(), db.data_ptr(), n, out.data_ptr(), work.data_ptr())
+    torch.cuda.synchronize()
+    nt.assert_bits(out.cpu().numpy(), mf.dot_product(a, b), "device cdot = host entry")
+    m_, k_ = 300, 41
+    A = datagen.config0(rng, m_ * k_)
+    dA = to_dev(torch, A)
+    dout = torch.zeros((k_, 2), dtype=torch.float64, device="cuda")
+    mf.dev_reduce_dim(mf.MFHIP_SUM, dA.data_ptr(), None, 1, m_, k_, dout.data_ptr())
+    torch.cuda.synchronize()
+    nt.assert_bits(dout.cpu().numpy(), nt.or_reduce_dim("sum", A, None, 1, m_, k_), "device sum dim")

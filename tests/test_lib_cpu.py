@@ -2,6 +2,7 @@
 import ctypes
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -9,6 +10,8 @@ import pytest
 import multifloats_amd as mf
 from multifloats_amd import datagen
 from tests import _native as nt
+
+ROOT = nt.ROOT
 
 HEADER = os.path.join(nt.ROOT, "include", "multifloats_hip.h")
 
@@ -105,3 +108,54 @@ def test_hostmath_erf_accuracy():
     L.h_erf(nt.ptr(x), nt.ptr(got), ctypes.c_long(x.shape[0]))
     fin = np.isfinite(ex[:, 0]) & (np.abs(ex[:, 0]) > 1e-290)
     assert nt.rel_err_vs_exact(got[fin], ex[fin]).max() <= 1.55e-32
+
+
+FORTRAN_DIR = os.path.join(ROOT, "multifloats_amd", "fortran")
+
+
+def test_fortran_bridge_source_is_generated():
+    """multifloats_hip.F90 is exactly what gen_bridge.py writes."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("gen_bridge", os.path.join(FORTRAN_DIR, "gen_bridge.py"))
+    g = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(g)
+    with open(os.path.join(FORTRAN_DIR, "multifloats_hip.F90")) as fh:
+        assert fh.read() == g.gen()
+
+
+def test_fortran_bridge_binds_only_exported_symbols():
+    """Every bind(c, name=...) in the bridge names a symbol the header declares
+    and the library exports."""
+    import re
+    with open(os.path.join(FORTRAN_DIR, "multifloats_hip.F90")) as fh:
+        names = set(re.findall(r"bind\(c, name='([a-z0-9_]+)'\)", fh.read()))
+    assert names
+    with open(os.path.join(ROOT, "include", "multifloats_hip.h")) as fh:
+        hdr = fh.read()
+    L = ctypes.CDLL(os.path.join(ROOT, "multifloats_amd", "libmultifloats_hip.so"))
+    for nm in names:
+        assert re.search(r"\b%s\(" % nm, hdr), nm
+        assert hasattr(L, nm), nm
+
+
+def test_fortran_bridge_fails_loudly_without_gpu(tmp_path):
+    """With no GPU the bridge's calls report the HIP error and return NaN
+    (no CPU fallback)."""
+    exe = os.path.join(ROOT, "tests", "fortran", "_build", "bridge_check")
+    if not os.path.exists(exe):
+        pytest.skip("bridge_check not built")
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    m, k, n, nv = 2, 3, 2, 4
+    fin = tmp_path / "in.bin"
+    with open(fin, "wb") as fh:
+        np.array([m, k, n, nv], dtype=np.int64).tofile(fh)
+        np.ones(2 * (m * k + k * n) + 4 * (m * k + k * n) + 2 * 2 * nv + 4 * 2 * nv).tofile(fh)
+        np.ones(nv, dtype=np.int32).tofile(fh)
+    out = tmp_path / "out.bin"
+    p = subprocess.run([exe, str(fin), str(out)], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0
+    assert "multifloats_hip: matmuldd_mm failed" in p.stderr
+    vals = np.fromfile(out, dtype=np.float64)
+    assert vals.size > 0 and np.isnan(vals).all()

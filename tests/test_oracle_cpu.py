@@ -185,3 +185,100 @@ def test_oracle_complex_matmul_matches_reference_composition():
         C = np.zeros((m * n, 4))
         nt.oracle().or_cmatmul_mm(nt.ptr(A), nt.ptr(B), nt.ptr(C), m, k, n, ri)
         nt.assert_bits(C, g[key + "_C"], "cmm " + key)
+
+
+def _colsum_fast2sum(col):
+    """dd_sum_1d on one column (the oracle's rank-1 sum)."""
+    return nt.res(nt.oracle().or_sum(nt.ptr(nt.dd(col)), col.shape[0]))
+
+
+def test_oracle_sum_dim_matches_rank1_chains():
+    """Each output of sum(array, dim) is the rank-1 chain over that fibre; the
+    closing TwoSum equals the rank-1 Fast2Sum whenever |s_hi| >= |s_lo|, which
+    holds for these inputs, so the bits must agree fibre by fibre."""
+    rng = np.random.default_rng(21)
+    shape = (5, 7, 33)
+    x = datagen.config0(rng, int(np.prod(shape))).reshape(shape[::-1] + (2,)).transpose(2, 1, 0, 3)
+    buf = np.ascontiguousarray(x.transpose(2, 1, 0, 3)).reshape(-1, 2)  # Fortran order
+    for dim in (1, 2, 3):
+        inner = int(np.prod(shape[:dim - 1]))
+        ln = shape[dim - 1]
+        outer = int(np.prod(shape[dim:]))
+        got = nt.or_reduce_dim("sum", buf, None, inner, ln, outer)
+        v = buf.reshape(outer, ln, inner, 2)
+        for o in range(outer):
+            for i in range(inner):
+                nt.assert_bits(got[i + inner * o], _colsum_fast2sum(np.ascontiguousarray(v[o, :, i])),
+                               "sum dim=%d fibre (%d,%d)" % (dim, i, o))
+        got2 = nt.or_reduce_dim("norm2", buf, None, inner, ln, outer)
+        for o in range(0, outer, 3):
+            for i in range(inner):
+                col = np.ascontiguousarray(v[o, :, i])
+                nt.assert_bits(got2[i + inner * o], nt.res(nt.oracle().or_norm2(nt.ptr(col), ln)),
+                               "norm2 dim=%d" % dim)
+
+
+def test_oracle_sum_mask_is_compaction():
+    rng = np.random.default_rng(22)
+    n = 4000
+    x = datagen.sum_inputs(rng, n)
+    mask = rng.random(n) < 0.3
+    nt.assert_bits(nt.or_sum_mask(x, mask), _colsum_fast2sum(np.ascontiguousarray(x[mask])), "mask")
+    nt.assert_bits(nt.or_sum_mask(x, np.ones(n, bool)), _colsum_fast2sum(x), "all true")
+    assert nt.or_sum_mask(x, np.zeros(n, bool)).tolist() == [0.0, 0.0]
+    # masked dim sum skips the same elements
+    buf = x.reshape(n, 2)
+    got = nt.or_reduce_dim("sum", buf, mask.astype(np.int32), 1, n, 1)
+    nt.assert_bits(got[0], _colsum_fast2sum(np.ascontiguousarray(x[mask])), "masked dim sum")
+
+
+def test_oracle_dim_closed_forms_and_planted_transpose():
+    # integer columns sum exactly; Pythagorean rows give exact norms
+    m, n = 6, 50
+    A = np.zeros((m * n, 2))
+    A[:, 0] = np.arange(m * n)  # A(i, j) = i + m*j, column-major
+    s1 = nt.or_reduce_dim("sum", A, None, 1, m, n)   # dim=1: column sums
+    want1 = np.array([sum(i + m * j for i in range(m)) for j in range(n)], dtype=np.float64)
+    assert (s1[:, 0] == want1).all() and (s1[:, 1] == 0).all()
+    s2 = nt.or_reduce_dim("sum", A, None, m, n, 1)   # dim=2: row sums
+    want2 = np.array([sum(i + m * j for j in range(n)) for i in range(m)], dtype=np.float64)
+    assert (s2[:, 0] == want2).all()
+    # a plausible slip -- swapping inner and outer -- changes the answer
+    bad = nt.or_reduce_dim("sum", A, None, n, m, 1)
+    assert bad.shape[0] != s2.shape[0] or not (bad[:, 0] == want2).all()
+    P = np.array([[3.0, 0.0], [4.0, 0.0], [6.0, 0.0], [8.0, 0.0]])  # columns (3,4), (6,8)
+    r = nt.or_reduce_dim("norm2", P, None, 1, 2, 2)
+    assert r.tolist() == [[5.0, 0.0], [10.0, 0.0]]
+    # empty fibres: zero
+    assert nt.or_reduce_dim("sum", np.zeros((1, 2)), None, 3, 0, 2).tolist() == [[0.0, 0.0]] * 6
+
+
+def test_oracle_complex_dot():
+    rng = np.random.default_rng(23)
+    n = 3000
+    a = np.concatenate([datagen.config0(rng, n), datagen.config0(rng, n)], axis=1)
+    b = np.concatenate([datagen.config0(rng, n), datagen.config0(rng, n)], axis=1)
+    O = nt.oracle()
+    r = nt.or_cdot(a, b, 8)
+    # real and imaginary parts against the exact accumulator:
+    # re = ar.br + ai.bi, im = ar.bi - ai.br
+    ar, ai, br, bi = a[:, :2], a[:, 2:], b[:, :2], b[:, 2:]
+    e, m = nt.exact_dot_err(np.concatenate([ar, ai]), np.concatenate([br, bi]), r[:2])
+    assert e <= 4 * n * 2.0 ** -104 * m
+    e, m = nt.exact_dot_err(np.concatenate([ar, -ai]), np.concatenate([bi, br]), r[2:])
+    assert e <= 4 * n * 2.0 ** -104 * m
+    # conj(a).a is real: the two cross chains are the same chain commuted
+    r2 = nt.or_cdot(a, a, 8)
+    assert r2[2] == 0.0 and r2[3] == 0.0 and r2[0] > 0
+    # imaginary parts zero: the real dot, exactly
+    z = a.copy()
+    z[:, 2:] = 0
+    w = b.copy()
+    w[:, 2:] = 0
+    r3 = nt.or_cdot(z, w, 8)
+    nt.assert_bits(r3[:2], nt.res(O.or_dot(nt.ptr(nt.dd(ar)), nt.ptr(nt.dd(br)), n, 8)), "real case")
+    assert r3[2] == 0.0
+    # a slip that drops one of the four chains breaks the bound on the imaginary part
+    slip_im = nt.res(O.or_dot(nt.ptr(nt.dd(ar)), nt.ptr(nt.dd(bi)), n, 8))
+    e, m = nt.exact_dot_err(np.concatenate([ar, -ai]), np.concatenate([bi, br]), slip_im)
+    assert e > 4 * n * 2.0 ** -104 * m
