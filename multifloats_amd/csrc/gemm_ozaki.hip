@@ -45,8 +45,7 @@ constexpr int kK = 32;        // K block (one MFMA K step)
 constexpr int kChunk = kT * kK;  // bytes of one slice tile (64 x 32)
 constexpr int kLdsSlice = kChunk;  // LDS image of a slice tile = its global image
 constexpr int kThreads = 256;  // pre-pass kernels
-constexpr int kWaves = 8;       // main kernel: 8 waves, 2 per SIMD
-constexpr int kLPW = 2;         // levels per wave
+constexpr int kWaves = 4;       // main kernel: 4 waves, 1 per SIMD
 constexpr int kMainThreads = 64 * kWaves;
 constexpr int kEmpty = -2147483647 - 1;  // exponent of an all-zero row/column
 constexpr int kNonFinite = 2147483647;   // a row/column holding Inf or NaN
@@ -238,195 +237,114 @@ __global__ __launch_bounds__(kThreads) void k_slice_b(const double2 *__restrict_
   for (int v = tid; v < nvec; v += kThreads) dst[v] = src[v];
 }
 
-// Level -> wave assignment: longest processing time first (level L has L+1
-// slice pairs), at most kLPW levels per wave. For S = 16 this pairs levels
-// (15,0), (14,1), ..., (8,7): 17 slice pairs on every wave.
-template <int S>
-struct Plan {
-  int lv[kWaves][kLPW];
-  int cnt[kWaves];
-  constexpr Plan() : lv{}, cnt{} {
-    int load[kWaves] = {};
-    for (int w = 0; w < kWaves; ++w)
-      for (int j = 0; j < kLPW; ++j) lv[w][j] = -1;
-    for (int L = S - 1; L >= 0; --L) {
-      int best = -1;
-      for (int w = 0; w < kWaves; ++w)
-        if (cnt[w] < kLPW && (best < 0 || load[w] < load[best])) best = w;
-      lv[best][cnt[best]++] = L;
-      load[best] += L + 1;
-    }
-  }
-};
-
 __device__ __forceinline__ v4i lds_frag(const signed char *base, int row, int half) {
   return *(const v4i *)(base + row * kK + ((half ^ swz(row)) << 4));
 }
 
-template <int S, int W>
-__device__ __forceinline__ void compute_block(const signed char *sa, const signed char *sb,
-                                              v16i (&acc)[kLPW][2][2], int lane) {
-  constexpr Plan<S> P{};
-  const int r = lane & 31, half = lane >> 5;
-#pragma unroll
-  for (int s = 0; s < S; ++s) {
-    bool any = false;
-#pragma unroll
-    for (int j = 0; j < kLPW; ++j) {
-      const int L = P.lv[W][j];
-      any = any || (L >= s && L - s < S);
-    }
-    if (!any) continue;
-    const v4i a0 = lds_frag(sa + s * kLdsSlice, r, half);
-    const v4i a1 = lds_frag(sa + s * kLdsSlice, r + 32, half);
-#pragma unroll
-    for (int j = 0; j < kLPW; ++j) {
-      const int L = P.lv[W][j];
-      const int t = L - s;
-      if (L >= 0 && t >= 0 && t < S) {
-        const v4i b0 = lds_frag(sb + t * kLdsSlice, r, half);
-        const v4i b1 = lds_frag(sb + t * kLdsSlice, r + 32, half);
-        acc[j][0][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b0, acc[j][0][0], 0, 0, 0);
-        acc[j][0][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b1, acc[j][0][1], 0, 0, 0);
-        acc[j][1][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b0, acc[j][1][0], 0, 0, 0);
-        acc[j][1][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b1, acc[j][1][1], 0, 0, 0);
-      }
-    }
-  }
-}
-
-// Dump one wave's raw level accumulators to LDS in register order:
-// X[slot][bi][bj][lane][r], 32 KiB per wave. Every index is static.
-__device__ __forceinline__ void dump_acc(const v16i (&acc)[kLPW][2][2], int *X, int lane) {
-#pragma unroll
-  for (int j = 0; j < kLPW; ++j)
-#pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-      for (int bj = 0; bj < 2; ++bj) {
-        v4i *dst = (v4i *)(X + (((j * 2 + bi) * 2 + bj) * 64 + lane) * 16);
-        const v16i &a = acc[j][bi][bj];
-        dst[0] = v4i{a[0], a[1], a[2], a[3]};
-        dst[1] = v4i{a[4], a[5], a[6], a[7]};
-        dst[2] = v4i{a[8], a[9], a[10], a[11]};
-        dst[3] = v4i{a[12], a[13], a[14], a[15]};
-      }
-}
-
-// Where output (row, col) of the 64x64 tile sits in a dump: the 32x32 MFMA
-// result layout puts column col%32 in lane (col%32) + 32*((row%8)/4) and row
-// 8*(r/4) + 4*(lane/32) + r%4 in accumulator element r.
-__device__ __forceinline__ int dump_index(int slot, int row, int col) {
-  const int bi = row >> 5, bj = col >> 5, rr = row & 31;
-  const int lane = (col & 31) + 32 * ((rr & 7) >> 2);
-  const int r = 4 * (rr >> 3) + (rr & 3);
-  return (((slot * 2 + bi) * 2 + bj) * 64 + lane) * 16 + r;
-}
-
+// Main kernel: a workgroup of 4 waves (one per SIMD) owns a 64x64 tile of C;
+// wave w owns its 32x32 block (w & 1, w >> 1) with ALL S level accumulators
+// (S x 16 registers per lane, held in the accumulation registers). Per K
+// block the wave reads its S A-fragments once into registers and streams the
+// S B-fragments: B fragment t meets A fragments s < S - t, so every fragment
+// read from LDS feeds up to S MFMAs (LDS traffic 32 fragments per 136 MFMAs
+// at S = 16). Consecutive MFMAs update distinct levels, so the matrix core
+// never waits on its own previous result. The S level sums of an output are
+// then all in the owning lane: recombination needs no data exchange.
 template <int S>
-__global__ __launch_bounds__(kMainThreads) void k_ozaki(const signed char *__restrict__ As,
-                                                    const signed char *__restrict__ Bs,
-                                                    const int *__restrict__ E,
-                                                    const int *__restrict__ F,
-                                                    double2 *__restrict__ C, int64_t m, int64_t n,
-                                                    int64_t KB, int tiles_m) {
+__global__ __launch_bounds__(kMainThreads) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_ozaki(
+    const signed char *__restrict__ As, const signed char *__restrict__ Bs, const int *__restrict__ E,
+    const int *__restrict__ F, double2 *__restrict__ C, int64_t m, int64_t n, int64_t KB, int tiles_m) {
   extern __shared__ __attribute__((aligned(16))) signed char lds[];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar address math
   const int ib = blockIdx.x % tiles_m, jb = blockIdx.x / tiles_m;
+  const int bi = wave & 1, bj = wave >> 1;
+  const int r = lane & 31, half = lane >> 5;
   const signed char *ga = As + (int64_t)ib * KB * S * kChunk;
   const signed char *gb = Bs + (int64_t)jb * KB * S * kChunk;
-  constexpr int kBuf = 2 * S * kChunk;         // one stage: A chunk then B chunk
-  constexpr int NV = S * kChunk / 1024;        // 1 KiB wave-loads per operand chunk
+  constexpr int kBuf = 2 * S * kChunk;                          // one stage: A chunk then B chunk
+  constexpr int NV = S * kChunk / 1024;                         // 1 KiB wave-loads per operand chunk
+  constexpr int NLOAD = 2 * ((NV + kWaves - 1) / kWaves);      // loads per wave per K block
+  constexpr int kSpread = 4;                                    // one load per 4 MFMAs
 
-  v16i acc[kLPW][2][2];
+  v16i acc[S];
 #pragma unroll
-  for (int j = 0; j < kLPW; ++j)
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-      for (int b = 0; b < 2; ++b) acc[j][a][b] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int L = 0; L < S; ++L) acc[L] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 
-  // Asynchronous HBM -> LDS copy of the K block kb into stage st: each wave
-  // moves 1 KiB per instruction (16 bytes per lane, consecutive in LDS).
-  auto issue = [&](int64_t kb, int st) {
-    signed char *dst = lds + st * kBuf;
-    const signed char *srca = ga + kb * S * kChunk;
-    const signed char *srcb = gb + kb * S * kChunk;
-    for (int v = wave; v < NV; v += kWaves) {
-      __builtin_amdgcn_global_load_lds((const void *)(srca + v * 1024 + lane * 16),
-                                       (__attribute__((address_space(3))) void *)(dst + v * 1024),
-                                       16, 0, 0);
-      __builtin_amdgcn_global_load_lds((const void *)(srcb + v * 1024 + lane * 16),
-                                       (__attribute__((address_space(3))) void *)(dst + S * kChunk + v * 1024),
-                                       16, 0, 0);
+  // Load j of a wave's share of the asynchronous HBM -> LDS copy of K block
+  // kb into stage st: operand j & 1, 1 KiB piece wave + 4 (j >> 1) (16 bytes
+  // per lane, consecutive in LDS).
+  auto issue_one = [&](int64_t kb, int st, int j) {
+    const int op = j & 1, v = wave + kWaves * (j >> 1);
+    if (v < NV) {
+      signed char *dst = lds + st * kBuf + op * S * kChunk + v * 1024;
+      const signed char *src = (op ? gb : ga) + kb * S * kChunk + v * 1024 + lane * 16;
+      __builtin_amdgcn_global_load_lds((const void *)src, (__attribute__((address_space(3))) void *)dst, 16,
+                                       0, 0);
     }
   };
 
-  issue(0, 0);
+#pragma unroll
+  for (int j = 0; j < NLOAD; ++j) issue_one(0, 0, j);
   for (int64_t kb = 0; kb < KB; ++kb) {
     const int st = (int)(kb & 1);
     __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of block kb have landed
     __syncthreads();                // ...and every wave's; stage st^1 is free
-    if (kb + 1 < KB) issue(kb + 1, st ^ 1);
+    const bool more = kb + 1 < KB;
     const signed char *sa = lds + st * kBuf;
     const signed char *sb = sa + S * kChunk;
-    switch (wave) {
-      case 0: compute_block<S, 0>(sa, sb, acc, lane); break;
-      case 1: compute_block<S, 1>(sa, sb, acc, lane); break;
-      case 2: compute_block<S, 2>(sa, sb, acc, lane); break;
-      case 3: compute_block<S, 3>(sa, sb, acc, lane); break;
-      case 4: compute_block<S, 4>(sa, sb, acc, lane); break;
-      case 5: compute_block<S, 5>(sa, sb, acc, lane); break;
-      case 6: compute_block<S, 6>(sa, sb, acc, lane); break;
-      default: compute_block<S, 7>(sa, sb, acc, lane); break;
+    v4i a[S];
+    v4i b = lds_frag(sb, 32 * bj + r, half);
+#pragma unroll
+    for (int s = 0; s < S; ++s) a[s] = lds_frag(sa + s * kLdsSlice, 32 * bi + r, half);
+    v4i bn = lds_frag(sb + kLdsSlice, 32 * bj + r, half);
+    // The copy of block kb+1 is spread over the first MFMAs of this block
+    // (one load per kSpread MFMAs; the instruction order is pinned by
+    // scheduling barriers): a burst of loads at the block start stalls the
+    // MFMA stream measurably.
+    int slot = 0;
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+      if (t > 0) {
+        b = bn;
+        if (t + 1 < S) bn = lds_frag(sb + (t + 1) * kLdsSlice, 32 * bj + r, half);
+      }
+#pragma unroll
+      for (int s = 0; s + t < S; ++s) {
+        acc[s + t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[s], b, acc[s + t], 0, 0, 0);
+        if (more && slot % kSpread == 0 && slot / kSpread < NLOAD) issue_one(kb + 1, st ^ 1, slot / kSpread);
+        __builtin_amdgcn_sched_barrier(0);
+        ++slot;
+      }
     }
+#pragma unroll
+    for (int j = (S * (S + 1) / 2 + kSpread - 1) / kSpread; j < NLOAD; ++j)
+      if (more) issue_one(kb + 1, st ^ 1, j);
   }
   __builtin_amdgcn_s_waitcnt(0);
 
-  // Recombine. Thread t owns outputs (row t%64, cols 8*(t/64) .. +7) and
-  // keeps their running DD sums in registers. Waves dump their level
-  // accumulators to LDS two at a time (waves 0-1 first, ...; within a wave
-  // the slots in order), so the summation order is fixed.
-  constexpr Plan<S> P{};
-  int *X = (int *)lds;
-  constexpr int kDump = kLPW * 4 * 64 * 16;  // ints per wave dump
-  const int orow = tid & 63, ocol0 = (tid >> 6) * 8;
-  ddv part[8];
+  // Recombine in the owning lane. The 32x32 MFMA result layout puts column r
+  // of the block in lane r + 32*h and row 8*(e/4) + 4*h + e%4 in element e.
+  // Levels are summed smallest first with accurate DD additions (fixed order).
+  const int64_t gj = (int64_t)jb * kT + 32 * bj + r;
+  const int fj = gj < n ? F[gj] : 0;
 #pragma unroll
-  for (int q = 0; q < 8; ++q) part[q] = mk(0.0, 0.0);
-  for (int w0 = 0; w0 < kWaves; w0 += 2) {
-    __syncthreads();
-    if ((wave >> 1) == (w0 >> 1)) dump_acc(acc, X + (wave & 1) * kDump, lane);
-    __syncthreads();
-    for (int w = w0; w < w0 + 2; ++w)
-      for (int j = 0; j < kLPW; ++j) {
-        const int L = P.lv[w][j];
-        if (L < 0) continue;
-        const int *Xw = X + (w & 1) * kDump;
+  for (int e = 0; e < 16; ++e) {
+    const int64_t gi = (int64_t)ib * kT + 32 * bi + 8 * (e >> 2) + 4 * half + (e & 3);
+    ddv part = mk(0.0, 0.0);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          const double v = __builtin_ldexp((double)Xw[dump_index(j, orow, ocol0 + q)], -7 * L);
-          part[q] = dd_add(part[q], mk(v, 0.0));
-        }
+    for (int L = S - 1; L >= 0; --L)
+      part = dd_add(part, mk(__builtin_ldexp((double)acc[L][e], -7 * L), 0.0));
+    if (gi < m && gj < n) {
+      const int ei = E[gi];
+      double2 out;
+      if (ei == kNonFinite || fj == kNonFinite) {
+        out = make_double2(__builtin_nan(""), 0.0);
+      } else {
+        const int sc = scale_of(ei) + scale_of(fj) - 10;
+        out = make_double2(__builtin_ldexp(part.hi, sc), __builtin_ldexp(part.lo, sc));
       }
-  }
-  const int64_t gi = (int64_t)ib * kT + orow;
-  if (gi < m) {
-    const int ei = E[gi];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int64_t gj = (int64_t)jb * kT + ocol0 + q;
-      if (gj < n) {
-        const int fj = F[gj];
-        double2 out;
-        if (ei == kNonFinite || fj == kNonFinite) {
-          out = make_double2(__builtin_nan(""), 0.0);
-        } else {
-          const int sc = scale_of(ei) + scale_of(fj) - 10;
-          out = make_double2(__builtin_ldexp(part[q].hi, sc), __builtin_ldexp(part[q].lo, sc));
-        }
-        C[gi + gj * m] = out;
-      }
+      C[gi + gj * m] = out;
     }
   }
 }
@@ -435,8 +353,8 @@ template <int S>
 hipError_t launch_main(const signed char *As, const signed char *Bs, const int *E, const int *F,
                        double2 *C, int64_t m, int64_t n, int64_t KB, int64_t tm, int64_t tn,
                        hipStream_t s) {
-  // two operand stages (2 x 2 x S x 2 KiB), reused by the 64 KiB recombination dump
-  const size_t lds = 4 * (size_t)S * kChunk > 65536 ? 4 * (size_t)S * kChunk : 65536;
+  // two operand stages of 2 x S x 2 KiB each
+  const size_t lds = 4 * (size_t)S * kChunk;
   hipError_t e = hipFuncSetAttribute((const void *)k_ozaki<S>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
