@@ -325,16 +325,32 @@ __global__ __launch_bounds__(kMainThreads) __attribute__((amdgpu_waves_per_eu(1,
 
   // Recombine in the owning lane. The 32x32 MFMA result layout puts column r
   // of the block in lane r + 32*h and row 8*(e/4) + 4*h + e%4 in element e.
-  // Levels are summed smallest first with accurate DD additions (fixed order).
+  // Levels 4g..4g+3 are first combined exactly in 64-bit integers (|G_L| <
+  // 2^31, so a group stays below 2^53 and converts to double exactly); the
+  // group values are then summed smallest first with TwoSum, their rounding
+  // errors collected, and closed by a Fast2Sum. The order is fixed.
+  constexpr int NG = (S + 3) / 4;
   const int64_t gj = (int64_t)jb * kT + 32 * bj + r;
   const int fj = gj < n ? F[gj] : 0;
 #pragma unroll
   for (int e = 0; e < 16; ++e) {
     const int64_t gi = (int64_t)ib * kT + 32 * bi + 8 * (e >> 2) + 4 * half + (e & 3);
-    ddv part = mk(0.0, 0.0);
+    double hi = 0.0, lo = 0.0;
 #pragma unroll
-    for (int L = S - 1; L >= 0; --L)
-      part = dd_add(part, mk(__builtin_ldexp((double)acc[L][e], -7 * L), 0.0));
+    for (int g = NG - 1; g >= 0; --g) {
+      constexpr int kLast = S - 1;
+      const int L1 = 4 * g + 3 < kLast ? 4 * g + 3 : kLast;  // last level of the group
+      int64_t v = 0;
+#pragma unroll
+      for (int L = 4 * g; L <= L1; ++L) v = v * 128 + (int64_t)acc[L][e];
+      const double x = __builtin_ldexp((double)v, -7 * L1);
+      double t, err;
+      two_sum(hi, x, t, err);
+      hi = t;
+      lo += err;
+    }
+    ddv part;
+    fast_two_sum(hi, lo, part.hi, part.lo);
     if (gi < m && gj < n) {
       const int ei = E[gi];
       double2 out;
