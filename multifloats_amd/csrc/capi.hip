@@ -208,7 +208,7 @@ hipError_t host_cdot(const complex64x2 *a, const complex64x2 *b, int64_t n, int6
   void *da, *db, *dw;
   TRY(stage_buf(0, 32 * nn + 32, &da));
   TRY(stage_buf(1, 32 * nn + 32, &db));
-  TRY(stage_buf(3, 2 * mfhip::reduce_workspace_bytes(n) + 32, &dw));
+  TRY(stage_buf(3, mfhip::cdot_workspace_bytes(n) + 32, &dw));
   TRY(h2d(da, a, 32 * nn));
   TRY(h2d(db, b, 32 * nn));
   double4 *dout = static_cast<double4 *>(dw);
@@ -520,7 +520,7 @@ int mfhip_sum_masked(const float64x2 *a, const int32_t *mask, int64_t n, float64
   return status(mfhip::reduce(mfhip::RED_SUM, D(a), nullptr, n, 0, W(out), W(work), (hipStream_t)stream,
                               mask));
 }
-size_t mfhip_cdot_workspace_bytes(int64_t n) { return 2 * mfhip::reduce_workspace_bytes(n); }
+size_t mfhip_cdot_workspace_bytes(int64_t n) { return mfhip::cdot_workspace_bytes(n); }
 int mfhip_cdot(const complex64x2 *a, const complex64x2 *b, int64_t n, int64_t ri, complex64x2 *out,
                void *work, void *stream) {
   CHECK_ALIGN(a, b, out, work);

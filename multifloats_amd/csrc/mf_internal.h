@@ -36,7 +36,8 @@ size_t reduce_workspace_bytes(int64_t n);
 // mask (sum only, may be null): elements with mask[i] == 0 are skipped.
 hipError_t reduce(int kind, const double2 *a, const double2 *b, int64_t n, int64_t ri,
                   double2 *out, double2 *work, hipStream_t s, const int32_t *mask = nullptr);
-// Complex conj(a).b; work holds 2 * reduce_workspace_bytes(n).
+// Complex conj(a).b; work holds cdot_workspace_bytes(n).
+size_t cdot_workspace_bytes(int64_t n);
 hipError_t cdot(const double4 *a, const double4 *b, int64_t n, int64_t ri, double4 *out, double4 *work,
                 hipStream_t s);
 // Bit-exact sum / norm2 along one dimension of an (inner, len, outer)

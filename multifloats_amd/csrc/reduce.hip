@@ -1,16 +1,21 @@
-// DD reductions: dot_product, sum, norm2.
+// DD reductions: dot_product, sum, norm2 (rank 1, and along one dimension).
 //
 // The reference computes these as single sequential Fortran loops
 // (fsrc/multifloats.fypp: dd_dot_product 2254-2308, dd_sum_1d 1980-2020,
-// dd_norm2_1d 2338-2352), a latency-bound chain through s_hi. Here every lane
-// runs the reference's own per-element update on a strided slice of the
-// input (so each partial carries the same compensation the reference's
-// accumulator does), then the partials are combined by a fixed binary tree of
-// accurate DD additions -- first across the workgroup through LDS, then across
-// workgroups in a second single-workgroup pass. The tree shape depends only on
-// n, so results are reproducible run to run. The reassociation means results
-// are not the sequential loop's bits; they are held to the c*n*2^-104*sum|term|
-// bound against an exact accumulator instead.
+// dd_norm2_1d 2338-2352), a latency-bound chain through s_hi.
+//
+// Rank 1: pass 1 gives every workgroup a contiguous span of 256*E elements;
+// lane l runs the reference's own per-element update over elements
+// l, l+256, ... of the span (E of them, all loaded up front with
+// nontemporal 16-byte loads, so each lane's partial carries the same
+// compensation the reference's accumulator does); the lanes meet in a fixed
+// LDS tree of accurate DD additions. Fold passes then combine 2048 partials
+// per workgroup the same way until one remains. Every launch shape depends
+// only on n, so results are reproducible run to run. The reassociation means
+// results are not the sequential loop's bits; they are held to the
+// c*n*2^-104*sum|term| bound against an exact accumulator instead.
+// (Measured on MI355X, N = 2^26 dot: ~6.7 TB/s; the earlier grid-stride form
+// with a 2048-workgroup cap reached ~4.7 TB/s.)
 #include "dd_core.h"
 #include "mf_internal.h"
 
@@ -67,7 +72,17 @@ __device__ ddv block_tree(ddv v) {
 
 // Pass 1: lane-local reference accumulation over a strided slice, then the
 // workgroup tree; one normalised DD partial per workgroup.
-// R_SUM_MASK: the sum update runs only where mask[i] != 0 (Fortran
+// Elements per lane in pass 1 (16-byte loads in flight per lane: dot 2E,
+// the single-operand reductions E).
+template <int KIND>
+constexpr int elems_per_lane() {
+  return KIND == R_DOT ? 8 : 16;
+}
+constexpr int kCdotE = 4;   // complex dot: 2 x 32 bytes per element
+constexpr int kFold = 8;    // partials per lane in a fold pass
+constexpr int kFoldSpan = kThreads * kFold;
+
+// Pass 1. R_SUM_MASK: the sum update runs only where mask[i] != 0 (Fortran
 // sum(array, mask), fypp:1993-1995: masked elements are skipped, not added).
 template <int KIND>
 __global__ __launch_bounds__(kThreads) void k_reduce_partial(const double2 *__restrict__ a,
@@ -75,95 +90,109 @@ __global__ __launch_bounds__(kThreads) void k_reduce_partial(const double2 *__re
                                                              int64_t n, int64_t ri,
                                                              double2 *__restrict__ partial,
                                                              const int32_t *__restrict__ mask) {
-  const int64_t stride = (int64_t)gridDim.x * kThreads;
-  int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  constexpr int E = elems_per_lane<KIND>();
+  const int64_t base = (int64_t)blockIdx.x * kThreads * E + threadIdx.x;
+  v2d va[E], vb[E];
+  int32_t vm[E];
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const int64_t i = base + (int64_t)kThreads * j;
+    const bool ok = i < n;
+    va[j] = ok ? __builtin_nontemporal_load((const v2d *)a + i) : v2d{0.0, 0.0};
+    if constexpr (KIND == R_DOT) vb[j] = ok ? __builtin_nontemporal_load((const v2d *)b + i) : v2d{0.0, 0.0};
+    if constexpr (KIND == R_SUM_MASK) vm[j] = ok ? __builtin_nontemporal_load(mask + i) : 0;
+  }
   ddv acc;
   if constexpr (KIND == R_NORM2) {
     acc = mk(0.0, 0.0);
-    for (; i < n; i += stride) {
-      double2 x = a[i];
-      ddv xx = mk(x.x, x.y);
-      acc = dd_add(acc, dd_mul(xx, xx));
-    }
+#pragma unroll
+    for (int j = 0; j < E; ++j)
+      if (base + (int64_t)kThreads * j < n) {
+        const ddv x = mk(va[j].x, va[j].y);
+        acc = dd_add(acc, dd_mul(x, x));
+      }
   } else {
     double sh = 0.0, sl = 0.0;
     int64_t until = ri;
-    constexpr int U = 4;
-    // U independent loads in flight per lane; the update order per lane is
-    // still strictly increasing in i.
-    for (; i + (U - 1) * stride < n; i += U * stride) {
-      double2 va[U], vb[U];
-      int32_t vm[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        va[u] = a[i + u * stride];
-        if constexpr (KIND == R_DOT) vb[u] = b[i + u * stride];
-        if constexpr (KIND == R_SUM_MASK) vm[u] = mask[i + u * stride];
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        if constexpr (KIND == R_DOT) {
-          dot_step(va[u], vb[u], sh, sl);
-          if (ri > 0 && --until == 0) {
-            until = ri;
-            fast_two_sum(sh, sl, sh, sl);
-          }
-        } else if constexpr (KIND == R_SUM_MASK) {
-          if (vm[u]) sum_step(va[u], sh, sl);
-        } else {
-          sum_step(va[u], sh, sl);
-        }
-      }
-    }
-    for (; i < n; i += stride) {
+    for (int j = 0; j < E; ++j) {
+      if (base + (int64_t)kThreads * j >= n) break;
       if constexpr (KIND == R_DOT) {
-        dot_step(a[i], b[i], sh, sl);
+        dot_step(make_double2(va[j].x, va[j].y), make_double2(vb[j].x, vb[j].y), sh, sl);
         if (ri > 0 && --until == 0) {
           until = ri;
           fast_two_sum(sh, sl, sh, sl);
         }
       } else if constexpr (KIND == R_SUM_MASK) {
-        if (mask[i]) sum_step(a[i], sh, sl);
+        if (vm[j]) sum_step(make_double2(va[j].x, va[j].y), sh, sl);
       } else {
-        sum_step(a[i], sh, sl);
+        sum_step(make_double2(va[j].x, va[j].y), sh, sl);
       }
     }
     renorm(sh, sl);
     acc = mk(sh, sl);
   }
-  ddv r = block_tree(acc);
+  const ddv r = block_tree(acc);
   if (threadIdx.x == 0) partial[blockIdx.x] = make_double2(r.hi, r.lo);
 }
 
-// Pass 2: one workgroup folds the partials (strided, then the same tree).
-template <int KIND>
-__global__ __launch_bounds__(kThreads) void k_reduce_final(const double2 *__restrict__ partial,
-                                                           int64_t g, double2 *__restrict__ out) {
-  ddv acc = mk(0.0, 0.0);
-  for (int64_t i = threadIdx.x; i < g; i += kThreads) acc = dd_add(acc, mk(partial[i].x, partial[i].y));
-  ddv r = block_tree(acc);
-  if (threadIdx.x == 0) {
-    if constexpr (KIND == R_NORM2) r = dd_sqrt(r);
-    out[0] = make_double2(r.hi, r.lo);
+// Fold pass: workgroup w combines partials [w*2048, (w+1)*2048) of NC-component
+// values (NC = 1 real, 2 complex) through the same tree; the last pass writes
+// the result (norm2 takes the DD square root there).
+template <int KIND, int NC>
+__global__ __launch_bounds__(kThreads) void k_fold(const double2 *__restrict__ in, int64_t g,
+                                                   double2 *__restrict__ out, int last) {
+  const int64_t base = (int64_t)blockIdx.x * kFoldSpan + threadIdx.x;
+  ddv acc[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) acc[c] = mk(0.0, 0.0);
+#pragma unroll
+  for (int j = 0; j < kFold; ++j) {
+    const int64_t i = base + (int64_t)kThreads * j;
+    if (i < g) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) acc[c] = dd_add(acc[c], mk(in[i * NC + c].x, in[i * NC + c].y));
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    if (c) __syncthreads();
+    ddv r = block_tree(acc[c]);
+    if (threadIdx.x == 0) {
+      if (KIND == R_NORM2 && last) r = dd_sqrt(r);
+      out[(int64_t)blockIdx.x * NC + c] = make_double2(r.hi, r.lo);
+    }
   }
 }
-
 
 // Complex dot_product(a, b) = sum(conj(a) * b) (fypp:2321-2331): the reference
 // runs four real dot chains, re = dot(ar, br) + dot(ai, bi) and
 // im = dot(ar, bi) - dot(ai, br), and combines them with DD + and -. Here
-// each lane keeps the four chains for its slice in registers (one pass over
-// both inputs), forms its (re, im) with the same DD + / -, and the lanes'
-// complex partials meet in the fixed tree.
-__device__ __forceinline__ void cdot_lane(const double4 *__restrict__ a, const double4 *__restrict__ b,
-                                          int64_t n, int64_t ri, ddv &re, ddv &im) {
-  const int64_t stride = (int64_t)gridDim.x * kThreads;
+// each lane keeps the four chains for its elements in registers (one pass
+// over both inputs), forms its (re, im) with the same DD + / -, and the
+// lanes' complex partials meet in the fixed tree.
+__global__ __launch_bounds__(kThreads) void k_cdot_partial(const double4 *__restrict__ a,
+                                                           const double4 *__restrict__ b, int64_t n,
+                                                           int64_t ri, double2 *__restrict__ partial) {
+  const int64_t base = (int64_t)blockIdx.x * kThreads * kCdotE + threadIdx.x;
+  v2d x[kCdotE][2], y[kCdotE][2];
+#pragma unroll
+  for (int j = 0; j < kCdotE; ++j) {
+    const int64_t i = base + (int64_t)kThreads * j;
+    const bool ok = i < n;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      x[j][h] = ok ? __builtin_nontemporal_load((const v2d *)a + 2 * i + h) : v2d{0.0, 0.0};
+      y[j][h] = ok ? __builtin_nontemporal_load((const v2d *)b + 2 * i + h) : v2d{0.0, 0.0};
+    }
+  }
   double h[4] = {0.0, 0.0, 0.0, 0.0}, l[4] = {0.0, 0.0, 0.0, 0.0};
   int64_t until = ri;
-  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
-    const double4 x = a[i], y = b[i];
-    const double2 ar = make_double2(x.x, x.y), ai = make_double2(x.z, x.w);
-    const double2 br = make_double2(y.x, y.y), bi = make_double2(y.z, y.w);
+#pragma unroll
+  for (int j = 0; j < kCdotE; ++j) {
+    if (base + (int64_t)kThreads * j >= n) break;
+    const double2 ar = make_double2(x[j][0].x, x[j][0].y), ai = make_double2(x[j][1].x, x[j][1].y);
+    const double2 br = make_double2(y[j][0].x, y[j][0].y), bi = make_double2(y[j][1].x, y[j][1].y);
     dot_step(ar, br, h[0], l[0]);
     dot_step(ai, bi, h[1], l[1]);
     dot_step(ar, bi, h[2], l[2]);
@@ -176,33 +205,15 @@ __device__ __forceinline__ void cdot_lane(const double4 *__restrict__ a, const d
   }
 #pragma unroll
   for (int q = 0; q < 4; ++q) renorm(h[q], l[q]);
-  re = dd_add(mk(h[0], l[0]), mk(h[1], l[1]));
-  im = dd_sub(mk(h[2], l[2]), mk(h[3], l[3]));
-}
-
-__global__ __launch_bounds__(kThreads) void k_cdot_partial(const double4 *__restrict__ a,
-                                                           const double4 *__restrict__ b, int64_t n,
-                                                           int64_t ri, double4 *__restrict__ partial) {
-  ddv re, im;
-  cdot_lane(a, b, n, ri, re, im);
-  ddv r = block_tree(re);
+  const ddv re = dd_add(mk(h[0], l[0]), mk(h[1], l[1]));
+  const ddv im = dd_sub(mk(h[2], l[2]), mk(h[3], l[3]));
+  const ddv r = block_tree(re);
   __syncthreads();
-  ddv q = block_tree(im);
-  if (threadIdx.x == 0) partial[blockIdx.x] = make_double4(r.hi, r.lo, q.hi, q.lo);
-}
-
-__global__ __launch_bounds__(kThreads) void k_cdot_final(const double4 *__restrict__ partial, int64_t g,
-                                                         double4 *__restrict__ out) {
-  ddv re = mk(0.0, 0.0), im = mk(0.0, 0.0);
-  for (int64_t i = threadIdx.x; i < g; i += kThreads) {
-    const double4 p = partial[i];
-    re = dd_add(re, mk(p.x, p.y));
-    im = dd_add(im, mk(p.z, p.w));
+  const ddv q = block_tree(im);
+  if (threadIdx.x == 0) {
+    partial[2 * (int64_t)blockIdx.x] = make_double2(r.hi, r.lo);
+    partial[2 * (int64_t)blockIdx.x + 1] = make_double2(q.hi, q.lo);
   }
-  ddv r = block_tree(re);
-  __syncthreads();
-  ddv q = block_tree(im);
-  if (threadIdx.x == 0) out[0] = make_double4(r.hi, r.lo, q.hi, q.lo);
 }
 
 // Reductions along one dimension, bit-identical to the reference: the array
@@ -304,55 +315,79 @@ __global__ __launch_bounds__(kThreads) void k_dim_cols(const double2 *__restrict
 
 namespace mfhip {
 
-// Number of pass-1 workgroups for n elements: a function of n only (the tree
-// shape, hence the result bits, must not depend on the device), capped at 2048
-// (8 per CU on a 256-CU part), at least 2048 elements per workgroup.
-int64_t reduce_groups(int64_t n) {
-  int64_t g = (n + 2047) / 2048;
-  if (g > 2048) g = 2048;
-  if (g < 1) g = 1;
-  return g;
+namespace {
+int64_t groups_for(int64_t n, int per_lane) {
+  const int64_t span = (int64_t)kThreads * per_lane;
+  const int64_t g = (n + span - 1) / span;
+  return g < 1 ? 1 : g;
+}
+// Partials held in the workspace: every level of the fold chain except the
+// last, which goes straight to the output.
+int64_t chain_partials(int64_t g) {
+  int64_t tot = 0;
+  while (true) {
+    tot += g;
+    const int64_t g2 = (g + kFoldSpan - 1) / kFoldSpan;
+    if (g2 <= 1) break;
+    g = g2;
+  }
+  return tot;
+}
+template <int KIND, int NC>
+hipError_t fold_chain(double2 *work, int64_t g, double2 *out, hipStream_t s) {
+  double2 *src = work, *dst = work + g * NC;
+  while (true) {
+    const int64_t g2 = (g + kFoldSpan - 1) / kFoldSpan;
+    const int last = g2 <= 1;
+    hipLaunchKernelGGL((k_fold<KIND, NC>), dim3((unsigned)g2), dim3(kThreads), 0, s, src, g, last ? out : dst,
+                       last);
+    if (last) break;
+    src = dst;
+    dst += g2 * NC;
+    g = g2;
+  }
+  return hipGetLastError();
+}
+template <int KIND>
+hipError_t reduce_kind(const double2 *a, const double2 *b, int64_t n, int64_t ri, double2 *out, double2 *work,
+                       hipStream_t s, const int32_t *mask) {
+  const int64_t g = groups_for(n, elems_per_lane<KIND>());
+  hipLaunchKernelGGL(k_reduce_partial<KIND>, dim3((unsigned)g), dim3(kThreads), 0, s, a, b, n, ri, work, mask);
+  return fold_chain<KIND == R_SUM_MASK ? R_SUM : KIND, 1>(work, g, out, s);
+}
+}  // namespace
+
+int64_t reduce_groups(int64_t n) { return groups_for(n, elems_per_lane<R_DOT>()); }
+
+// Sized for the kind with the most partials (dot: 8 elements per lane).
+size_t reduce_workspace_bytes(int64_t n) {
+  return sizeof(double2) * (size_t)chain_partials(groups_for(n < 0 ? 0 : n, elems_per_lane<R_DOT>()));
 }
 
-size_t reduce_workspace_bytes(int64_t n) { return sizeof(double2) * (size_t)reduce_groups(n); }
+size_t cdot_workspace_bytes(int64_t n) {
+  return 2 * sizeof(double2) * (size_t)chain_partials(groups_for(n < 0 ? 0 : n, kCdotE));
+}
 
 hipError_t reduce(int kind, const double2 *a, const double2 *b, int64_t n, int64_t ri,
                   double2 *out, double2 *work, hipStream_t s, const int32_t *mask) {
   if (n < 0) n = 0;
-  const int64_t g = reduce_groups(n);
   switch (kind) {
-    case RED_DOT:
-      hipLaunchKernelGGL(k_reduce_partial<R_DOT>, dim3((unsigned)g), dim3(kThreads), 0, s, a, b, n, ri, work,
-                         nullptr);
-      hipLaunchKernelGGL(k_reduce_final<R_DOT>, dim3(1), dim3(kThreads), 0, s, work, g, out);
-      break;
+    case RED_DOT: return reduce_kind<R_DOT>(a, b, n, ri, out, work, s, nullptr);
     case RED_SUM:
-      if (mask) {
-        hipLaunchKernelGGL(k_reduce_partial<R_SUM_MASK>, dim3((unsigned)g), dim3(kThreads), 0, s, a, b, n, ri,
-                           work, mask);
-      } else {
-        hipLaunchKernelGGL(k_reduce_partial<R_SUM>, dim3((unsigned)g), dim3(kThreads), 0, s, a, b, n, ri, work,
-                           nullptr);
-      }
-      hipLaunchKernelGGL(k_reduce_final<R_SUM>, dim3(1), dim3(kThreads), 0, s, work, g, out);
-      break;
-    case RED_NORM2:
-      hipLaunchKernelGGL(k_reduce_partial<R_NORM2>, dim3((unsigned)g), dim3(kThreads), 0, s, a, b, n, ri, work,
-                         nullptr);
-      hipLaunchKernelGGL(k_reduce_final<R_NORM2>, dim3(1), dim3(kThreads), 0, s, work, g, out);
-      break;
+      return mask ? reduce_kind<R_SUM_MASK>(a, b, n, ri, out, work, s, mask)
+                  : reduce_kind<R_SUM>(a, b, n, ri, out, work, s, nullptr);
+    case RED_NORM2: return reduce_kind<R_NORM2>(a, b, n, ri, out, work, s, nullptr);
     default: return hipErrorInvalidValue;
   }
-  return hipGetLastError();
 }
 
 hipError_t cdot(const double4 *a, const double4 *b, int64_t n, int64_t ri, double4 *out, double4 *work,
                 hipStream_t s) {
   if (n < 0) n = 0;
-  const int64_t g = reduce_groups(n);
-  hipLaunchKernelGGL(k_cdot_partial, dim3((unsigned)g), dim3(kThreads), 0, s, a, b, n, ri, work);
-  hipLaunchKernelGGL(k_cdot_final, dim3(1), dim3(kThreads), 0, s, work, g, out);
-  return hipGetLastError();
+  const int64_t g = groups_for(n, kCdotE);
+  double2 *w = reinterpret_cast<double2 *>(work);
+  hipLaunchKernelGGL(k_cdot_partial, dim3((unsigned)g), dim3(kThreads), 0, s, a, b, n, ri, w);
+  return fold_chain<R_DOT, 2>(w, g, reinterpret_cast<double2 *>(out), s);
 }
 
 hipError_t reduce_dim(int kind, const double2 *a, const int32_t *mask, int64_t inner, int64_t len,
