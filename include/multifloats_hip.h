@@ -155,13 +155,13 @@ MULTIFLOATS_API void norm2dd_dim(const float64x2 *a, int64_t inner, int64_t len,
 
 /* Precision form of matmuldd_mm (no counterpart in the reference): the Ozaki
  * split-precision GEMM on the INT8 matrix cores. A and B are cut into
- * `slices` int8 digit slices (7 bits each) per row / column scale; slice
+ * `slices` int8 digit slices (8 bits each) per row / column scale; slice
  * pairs whose scales sum to fewer than `slices` levels are multiplied with
  * exact INT32 accumulation and recombined in DD. Not bit-identical to
  * matmuldd_mm; the target is |C - AB| <= 2^-100 (|A||B|) elementwise.
  * slices <= 0 (recommended) chooses the count from k and the exponent ranges
  * of A's rows and B's columns, and when more than 16 would be needed runs the
- * reference-order GEMM instead; explicit counts must lie in 8..16.
+ * reference-order GEMM instead; explicit counts must lie in 7..16.
  * k <= 32768. Rows of A or columns of B holding Inf/NaN give NaN outputs. */
 MULTIFLOATS_API void matmuldd_mm_ozaki(const float64x2 *a, const float64x2 *b, float64x2 *c,
                                        int64_t m, int64_t k, int64_t n, int slices);
@@ -228,7 +228,7 @@ MULTIFLOATS_API int mfhip_ozaki_matmul_mm(const float64x2 *a, const float64x2 *b
  * reference-order GEMM instead), and the rule behind slices <= 0. */
 MULTIFLOATS_API int mfhip_ozaki_slices_used(void);
 MULTIFLOATS_API int mfhip_ozaki_choose_slices(int64_t k, int range_a, int range_b);
-/* mfhip_ozaki_matmul_mm in two steps with an explicit slice count (8..16):
+/* mfhip_ozaki_matmul_mm in two steps with an explicit slice count (7..16):
  * prepare scales and slices A and B into
  * `work`; multiply forms C from `work` (same m, k, n, slices, work). A
  * prepared `work` may be multiplied again, e.g. to time the product alone. */

@@ -270,7 +270,7 @@ hipError_t host_mm_ozaki(const float64x2 *a, const float64x2 *b, float64x2 *c, i
                          int64_t k, int64_t n, int slices) {
   tl_err = 0;
   if (m <= 0 || n <= 0) return hipSuccess;
-  if (slices > mfhip::ozaki_max_slices() || (slices > 0 && slices < 8)) return hipErrorInvalidValue;
+  if (slices > mfhip::ozaki_max_slices() || (slices > 0 && slices < 7)) return hipErrorInvalidValue;
   TRY(stage_init());
   const size_t kk = k > 0 ? (size_t)k : 0;
   void *da, *db, *dc, *dw;
@@ -541,7 +541,7 @@ int mfhip_ozaki_matmul_mm(const float64x2 *a, const float64x2 *b, float64x2 *c, 
                           int64_t k, int64_t n, int slices, void *work, size_t work_bytes,
                           void *stream) {
   CHECK_ALIGN(a, b, c, work);
-  if (slices > mfhip::ozaki_max_slices() || (slices > 0 && slices < 8))
+  if (slices > mfhip::ozaki_max_slices() || (slices > 0 && slices < 7))
     return status(hipErrorInvalidValue);
   return status(mfhip::ozaki_gemm(D(a), D(b), W(c), m, k, n, slices, work, work_bytes,
                                   (hipStream_t)stream, &tl_oz_used));

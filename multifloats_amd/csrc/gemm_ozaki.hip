@@ -4,32 +4,23 @@
 // same inputs and layout, results held to an error bound instead of the
 // reference's bits.
 //
-// 1. Scaling. E_i = max_p ilogb(A(i,p).hi) per row of A, F_j likewise per
-//    column of B. x = a * 2^(5 - E_i) then satisfies |x| <= 64.
-// 2. Slicing. x = sum_s d_s 2^(-7 s) with balanced digits d_s = RN(x_s) in
-//    [-64, 64], x_{s+1} = 128 (x_s - d_s). Every step is exact in DD
-//    arithmetic, so the S digits are the leading 7S bits of x with round-to-
-//    nearest digit boundaries. Digits are written as int8 in a tiled layout:
+// 1. Scaling. E_i = max_p ilogb(A(i,p).hi) per row of A, plus one when the
+//    row's largest |hi| reaches 127/128 of 2^(E_i+1); F_j likewise per column
+//    of B. x = a * 2^(6 - E_i) then satisfies |x| < 127.
+// 2. Slicing. x = sum_s d_s 2^(-8 s): digits d_s = RN(x_s),
+//    x_{s+1} = 256 (x_s - d_s), every step exact in DD arithmetic; a digit
+//    that rounds to +128 becomes -128 with a carry of one into the digit
+//    above (carries ripple upward; |x| < 127 keeps the top digit <= 127). So
+//    the S digits are int8 in [-128, 127] and hold the leading 8S bits of x
+//    with a round-to-nearest last digit. They are written in a tiled layout:
 //    one contiguous (S x 64 x 32)-byte chunk per (64-row block, 32-k block).
-// 3. Products. With G_L = sum_p sum_{s+t=L} d^A_s d^B_t (exact in int32:
-//    |d d| <= 4096, so k * S * 4096 < 2^31 for k <= 32768, S <= 16),
-//        C_ij = 2^(E_i + F_j - 10) * sum_{L < S} 2^(-7 L) G_L(i,j).
+// 3. Products. With G_L = sum_p sum_{s+t=L} d^A_s d^B_t,
+//        C_ij = 2^(E_i + F_j - 12) * sum_{L < S} 2^(-8 L) G_L(i,j).
 //    Levels L >= S are dropped (the triangular Ozaki truncation); each G_L is
-//    accumulated by v_mfma_i32_32x32x32_i8 with exact INT32 accumulation.
-// 4. Recombination. The S integers of an output are converted exactly to
-//    double, scaled by exact powers of two and summed with accurate DD
-//    additions (two_sum), then scaled by 2^(E_i + F_j - 10).
-//
-// Kernel shape. A workgroup owns a 64x64 tile of C and all S level
-// accumulators of it: 8 waves (2 per SIMD), each holding 2 levels of the
-// whole tile (2 x 4 x 16 = 128 accumulator registers per lane), levels
-// assigned to waves by longest-processing-time so each wave issues the same
-// number of MFMAs. Per 32-deep K block the S slices of both operand tiles
-// (S x 4 KiB) are copied once into LDS by asynchronous 16-byte direct-to-LDS
-// loads (double-buffered, no register staging), already in a swizzled layout
-// that makes the 16-byte fragment reads bank-conflict free; every slice pair
-// of a wave's levels is then multiplied out of LDS, so HBM/L2 see each slice
-// tile once per output tile rather than once per pair.
+//    accumulated by v_mfma_i32_32x32x32_i8 with exact INT32 accumulation,
+//    which holds while k * S * 2^14 < 2^31: longer K ranges are cut into
+//    chunks whose DD results are added into C.
+// 4. Recombination, in the lane that owns the output (see k_ozaki).
 #include "dd_core.h"
 #include "mf_internal.h"
 
@@ -58,14 +49,18 @@ __device__ __forceinline__ int dexp(double x) {
 
 // Row exponent maxima and minima of A (column-major m x k): thread = row,
 // block.y = k chunk. Zeros are skipped; a non-finite entry marks the row.
+// Mbits collects the largest |hi| of the row (the bit patterns of
+// non-negative doubles order like the values).
 __global__ __launch_bounds__(kThreads) void k_row_exp(const double2 *__restrict__ A, int64_t m,
                                                       int64_t k, int64_t kchunk,
-                                                      int *__restrict__ E, int *__restrict__ Emin) {
+                                                      int *__restrict__ E, int *__restrict__ Emin,
+                                                      unsigned long long *__restrict__ Mbits) {
   const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   if (i >= m) return;
   const int64_t p0 = (int64_t)blockIdx.y * kchunk;
   const int64_t p1 = p0 + kchunk < k ? p0 + kchunk : k;
   int e = kEmpty, lo = kNonFinite;
+  double mx = 0.0;
   bool bad = false;
   for (int64_t p = p0; p < p1; ++p) {
     double h = A[i + p * m].x;
@@ -74,28 +69,44 @@ __global__ __launch_bounds__(kThreads) void k_row_exp(const double2 *__restrict_
       const int x = dexp(h);
       e = max(e, x);
       lo = min(lo, x);
+      mx = fmax(mx, fabs(h));
     }
   }
   if (bad) e = kNonFinite;
   if (e != kEmpty) atomicMax(&E[i], e);
   if (lo != kNonFinite) atomicMin(&Emin[i], lo);
+  if (mx > 0.0) atomicMax(&Mbits[i], (unsigned long long)__double_as_longlong(mx));
 }
 
-// Largest exponent range max - min over the finite, nonzero rows.
-__global__ void k_row_range(const int *__restrict__ E, const int *__restrict__ Emin, int64_t m,
-                            int *__restrict__ range) {
+// One more bit of headroom for a row or column whose largest |hi| reaches
+// 127/128 of 2^(e+1), so that |x| < 127 after scaling by 2^(6-e).
+__device__ __forceinline__ int headroom(int e, double mx) {
+  if (e == kEmpty || e == kNonFinite) return e;
+  return mx >= __builtin_ldexp(127.0, e - 6) ? e + 1 : e;
+}
+
+// Finish the row exponents (headroom) and fold the largest exponent range
+// max - min over the finite, nonzero rows into *range.
+__global__ void k_row_finish(int *__restrict__ E, const int *__restrict__ Emin,
+                             const unsigned long long *__restrict__ Mbits, int64_t m, int *__restrict__ range) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
   const int e = E[i];
-  if (e != kEmpty && e != kNonFinite) atomicMax(range, e - Emin[i]);
+  if (e != kEmpty && e != kNonFinite) {
+    const int eh = headroom(e, __longlong_as_double((long long)Mbits[i]));
+    E[i] = eh;
+    atomicMax(range, eh - Emin[i]);
+  }
 }
 
-// Column exponent maxima of B (column-major k x n): block = column, threads
-// walk p; also folds the column's exponent range into *range.
+// Column exponent maxima of B (column-major k x n) with the same headroom
+// rule: block = column, threads walk p; also folds the column's exponent
+// range into *range.
 __global__ __launch_bounds__(kThreads) void k_col_exp(const double2 *__restrict__ B, int64_t k,
                                                       int *__restrict__ F, int *__restrict__ range) {
   const int64_t j = blockIdx.x;
   int e = kEmpty, lo = kNonFinite;
+  double mx = 0.0;
   bool bad = false;
   for (int64_t p = threadIdx.x; p < k; p += kThreads) {
     double h = B[p + j * k].x;
@@ -104,23 +115,28 @@ __global__ __launch_bounds__(kThreads) void k_col_exp(const double2 *__restrict_
       const int x = dexp(h);
       e = max(e, x);
       lo = min(lo, x);
+      mx = fmax(mx, fabs(h));
     }
   }
   if (bad) e = kNonFinite;
   __shared__ int rmax[kThreads], rmin[kThreads];
+  __shared__ double rmx[kThreads];
   rmax[threadIdx.x] = e;
   rmin[threadIdx.x] = lo;
+  rmx[threadIdx.x] = mx;
   __syncthreads();
   for (int s = kThreads / 2; s > 0; s >>= 1) {
     if (threadIdx.x < s) {
       rmax[threadIdx.x] = max(rmax[threadIdx.x], rmax[threadIdx.x + s]);
       rmin[threadIdx.x] = min(rmin[threadIdx.x], rmin[threadIdx.x + s]);
+      rmx[threadIdx.x] = fmax(rmx[threadIdx.x], rmx[threadIdx.x + s]);
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    F[j] = rmax[0];
-    if (rmax[0] != kEmpty && rmax[0] != kNonFinite) atomicMax(range, rmax[0] - rmin[0]);
+    const int eh = headroom(rmax[0], rmx[0]);
+    F[j] = eh;
+    if (eh != kEmpty && eh != kNonFinite) atomicMax(range, eh - rmin[0]);
   }
 }
 
@@ -129,18 +145,27 @@ __global__ void k_fill_int(int *__restrict__ p, int64_t n, int v) {
   if (i < n) p[i] = v;
 }
 
-// Balanced base-128 digits of x (|x| <= 64, exact DD), MAXS of them.
+// Base-256 digits of x (|x| < 127, exact DD), S <= MAXS of them, in
+// [-128, 127]: round to nearest digit by digit, then turn every +128 into
+// -128 with a carry into the digit above (bottom-up, so carries ripple).
 template <int MAXS>
-__device__ __forceinline__ void digits(double h, double l, int S, signed char *d) {
+__device__ __forceinline__ void digits(double h, double l, int S, int *d) {
 #pragma unroll
   for (int s = 0; s < MAXS; ++s) {
     if (s < S) {
       double q = __builtin_rint(h);
-      d[s] = (signed char)(int)q;
+      d[s] = (int)q;
       h = h - q;  // exact
       fast_two_sum(h, l, h, l);
-      h *= 128.0;
-      l *= 128.0;
+      h *= 256.0;
+      l *= 256.0;
+    }
+  }
+#pragma unroll
+  for (int s = MAXS - 1; s > 0; --s) {
+    if (s < S && d[s] == 128) {
+      d[s] = -128;
+      d[s - 1] += 1;
     }
   }
 }
@@ -179,16 +204,16 @@ __global__ __launch_bounds__(kThreads) void k_slice_a(const double2 *__restrict_
       if (Ei != kEmpty && Ei != kNonFinite) {
         h = v.x;
         l = v.y;
-        ex = 5 - Ei;
+        ex = 6 - Ei;
       }
     }
     h = __builtin_ldexp(h, ex);
     l = __builtin_ldexp(l, ex);
-    signed char d[MAXS];
+    int d[MAXS];
     digits<MAXS>(h, l, S, d);
 #pragma unroll
     for (int s = 0; s < MAXS; ++s)
-      if (s < S) tile[s * kChunk + tile_off(r, c)] = d[s];
+      if (s < S) tile[s * kChunk + tile_off(r, c)] = (signed char)d[s];
   }
   __syncthreads();
   const int nvec = S * kChunk / 16;
@@ -219,16 +244,16 @@ __global__ __launch_bounds__(kThreads) void k_slice_b(const double2 *__restrict_
       if (Fj != kEmpty && Fj != kNonFinite) {
         h = v.x;
         l = v.y;
-        ex = 5 - Fj;
+        ex = 6 - Fj;
       }
     }
     h = __builtin_ldexp(h, ex);
     l = __builtin_ldexp(l, ex);
-    signed char d[MAXS];
+    int d[MAXS];
     digits<MAXS>(h, l, S, d);
 #pragma unroll
     for (int s = 0; s < MAXS; ++s)
-      if (s < S) tile[s * kChunk + tile_off(c, p)] = d[s];
+      if (s < S) tile[s * kChunk + tile_off(c, p)] = (signed char)d[s];
   }
   __syncthreads();
   const int nvec = S * kChunk / 16;
@@ -250,18 +275,23 @@ __device__ __forceinline__ v4i lds_frag(const signed char *base, int row, int ha
 // at S = 16). Consecutive MFMAs update distinct levels, so the matrix core
 // never waits on its own previous result. The S level sums of an output are
 // then all in the owning lane: recombination needs no data exchange.
+//
+// K range: blocks [kb0, kb0 + nkb) of the KB blocks in the workspace. With
+// accumulate set, the result is added (DD) to what C holds (later chunks of a
+// K range too long for exact INT32 accumulation).
 template <int S>
 __global__ __launch_bounds__(kMainThreads) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_ozaki(
     const signed char *__restrict__ As, const signed char *__restrict__ Bs, const int *__restrict__ E,
-    const int *__restrict__ F, double2 *__restrict__ C, int64_t m, int64_t n, int64_t KB, int tiles_m) {
+    const int *__restrict__ F, double2 *__restrict__ C, int64_t m, int64_t n, int64_t KB, int tiles_m,
+    int64_t kb0, int64_t nkb, int accumulate) {
   extern __shared__ __attribute__((aligned(16))) signed char lds[];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar address math
   const int ib = blockIdx.x % tiles_m, jb = blockIdx.x / tiles_m;
   const int bi = wave & 1, bj = wave >> 1;
   const int r = lane & 31, half = lane >> 5;
-  const signed char *ga = As + (int64_t)ib * KB * S * kChunk;
-  const signed char *gb = Bs + (int64_t)jb * KB * S * kChunk;
+  const signed char *ga = As + ((int64_t)ib * KB + kb0) * S * kChunk;
+  const signed char *gb = Bs + ((int64_t)jb * KB + kb0) * S * kChunk;
   constexpr int kBuf = 2 * S * kChunk;                          // one stage: A chunk then B chunk
   constexpr int NV = S * kChunk / 1024;                         // 1 KiB wave-loads per operand chunk
   constexpr int NLOAD = 2 * ((NV + kWaves - 1) / kWaves);      // loads per wave per K block
@@ -286,11 +316,11 @@ __global__ __launch_bounds__(kMainThreads) __attribute__((amdgpu_waves_per_eu(1,
 
 #pragma unroll
   for (int j = 0; j < NLOAD; ++j) issue_one(0, 0, j);
-  for (int64_t kb = 0; kb < KB; ++kb) {
+  for (int64_t kb = 0; kb < nkb; ++kb) {
     const int st = (int)(kb & 1);
     __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of block kb have landed
     __syncthreads();                // ...and every wave's; stage st^1 is free
-    const bool more = kb + 1 < KB;
+    const bool more = kb + 1 < nkb;
     const signed char *sa = lds + st * kBuf;
     const signed char *sb = sa + S * kChunk;
     v4i a[S];
@@ -325,11 +355,11 @@ __global__ __launch_bounds__(kMainThreads) __attribute__((amdgpu_waves_per_eu(1,
 
   // Recombine in the owning lane. The 32x32 MFMA result layout puts column r
   // of the block in lane r + 32*h and row 8*(e/4) + 4*h + e%4 in element e.
-  // Levels 4g..4g+3 are first combined exactly in 64-bit integers (|G_L| <
-  // 2^31, so a group stays below 2^53 and converts to double exactly); the
+  // Levels 3g..3g+2 are first combined exactly in 64-bit integers (|G_L| <
+  // 2^31, so a group stays below 2^48 and converts to double exactly); the
   // group values are then summed smallest first with TwoSum, their rounding
   // errors collected, and closed by a Fast2Sum. The order is fixed.
-  constexpr int NG = (S + 3) / 4;
+  constexpr int NG = (S + 2) / 3;
   const int64_t gj = (int64_t)jb * kT + 32 * bj + r;
   const int fj = gj < n ? F[gj] : 0;
 #pragma unroll
@@ -339,11 +369,11 @@ __global__ __launch_bounds__(kMainThreads) __attribute__((amdgpu_waves_per_eu(1,
 #pragma unroll
     for (int g = NG - 1; g >= 0; --g) {
       constexpr int kLast = S - 1;
-      const int L1 = 4 * g + 3 < kLast ? 4 * g + 3 : kLast;  // last level of the group
+      const int L1 = 3 * g + 2 < kLast ? 3 * g + 2 : kLast;  // last level of the group
       int64_t v = 0;
 #pragma unroll
-      for (int L = 4 * g; L <= L1; ++L) v = v * 128 + (int64_t)acc[L][e];
-      const double x = __builtin_ldexp((double)v, -7 * L1);
+      for (int L = 3 * g; L <= L1; ++L) v = v * 256 + (int64_t)acc[L][e];
+      const double x = __builtin_ldexp((double)v, -8 * L1);
       double t, err;
       two_sum(hi, x, t, err);
       hi = t;
@@ -357,13 +387,22 @@ __global__ __launch_bounds__(kMainThreads) __attribute__((amdgpu_waves_per_eu(1,
       if (ei == kNonFinite || fj == kNonFinite) {
         out = make_double2(__builtin_nan(""), 0.0);
       } else {
-        const int sc = scale_of(ei) + scale_of(fj) - 10;
+        const int sc = scale_of(ei) + scale_of(fj) - 12;
         out = make_double2(__builtin_ldexp(part.hi, sc), __builtin_ldexp(part.lo, sc));
+        if (accumulate) {
+          const double2 c0 = C[gi + gj * m];
+          const ddv sum = dd_add(mk(c0.x, c0.y), mk(out.x, out.y));
+          out = make_double2(sum.hi, sum.lo);
+        }
       }
       C[gi + gj * m] = out;
     }
   }
 }
+
+// Longest K range (in 32-deep blocks) whose level sums stay exact in INT32:
+// (L+1) k 2^14 < 2^31 for every level L < S.
+int64_t max_exact_kblocks(int S) { return ((2147483647LL / ((int64_t)S << 14)) / kK); }
 
 template <int S>
 hipError_t launch_main(const signed char *As, const signed char *Bs, const int *E, const int *F,
@@ -374,8 +413,12 @@ hipError_t launch_main(const signed char *As, const signed char *Bs, const int *
   hipError_t e = hipFuncSetAttribute((const void *)k_ozaki<S>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_ozaki<S>, dim3((unsigned)(tm * tn)), dim3(kMainThreads), lds, s, As, Bs, E, F,
-                     C, m, n, KB, (int)tm);
+  const int64_t span = max_exact_kblocks(S);
+  for (int64_t kb0 = 0; kb0 < KB; kb0 += span) {
+    const int64_t nkb = KB - kb0 < span ? KB - kb0 : span;
+    hipLaunchKernelGGL(k_ozaki<S>, dim3((unsigned)(tm * tn)), dim3(kMainThreads), lds, s, As, Bs, E, F, C, m,
+                       n, KB, (int)tm, kb0, nkb, kb0 > 0 ? 1 : 0);
+  }
   return hipGetLastError();
 }
 
@@ -384,19 +427,22 @@ hipError_t launch_main(const signed char *As, const signed char *Bs, const int *
 namespace mfhip {
 
 int ozaki_max_slices() { return 16; }
+constexpr int kMinSlices = 7;
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 size_t ozaki_workspace_bytes(int64_t m, int64_t k, int64_t n, int S) {
   const int64_t tm = (m + kT - 1) / kT, tn = (n + kT - 1) / kT, KB = (k + kK - 1) / kK;
   return align256((size_t)tm * KB * S * kChunk) + align256((size_t)tn * KB * S * kChunk) +
-         2 * align256(sizeof(int) * (size_t)m) + align256(sizeof(int) * (size_t)n) + 256;
+         2 * align256(sizeof(int) * (size_t)m) + align256(sizeof(int) * (size_t)n) + 256 +
+         align256(sizeof(unsigned long long) * (size_t)m);
 }
 
 namespace {
 struct OzWork {
   signed char *As, *Bs;
   int *E, *Emin, *F, *range;  // range[0] rows of A, range[1] columns of B
+  unsigned long long *Mbits;  // largest |hi| per row of A (bit pattern)
   int64_t tm, tn, KB;
 };
 OzWork carve(void *work, int64_t m, int64_t k, int64_t n, int S) {
@@ -414,12 +460,14 @@ OzWork carve(void *work, int64_t m, int64_t k, int64_t n, int S) {
   p += align256(sizeof(int) * (size_t)n);
   w.range = (int *)p;
   p += 256;
+  w.Mbits = (unsigned long long *)p;
+  p += align256(sizeof(unsigned long long) * (size_t)m);
   w.As = (signed char *)p;
   p += align256((size_t)w.tm * w.KB * S * kChunk);
   w.Bs = (signed char *)p;
   return w;
 }
-bool oz_args_ok(int64_t k, int S) { return S >= 8 && S <= ozaki_max_slices() && k <= 32768; }
+bool oz_args_ok(int64_t k, int S) { return S >= kMinSlices && S <= ozaki_max_slices() && k <= 32768; }
 }  // namespace
 
 hipError_t ozaki_scan(const double2 *A, const double2 *B, int64_t m, int64_t k, int64_t n,
@@ -430,10 +478,12 @@ hipError_t ozaki_scan(const double2 *A, const double2 *B, int64_t m, int64_t k, 
   hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, w.E, m, kEmpty);
   hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, w.Emin, m, kNonFinite);
   hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, s, w.range, (int64_t)2, 0);
+  hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((2 * m + 255) / 256)), dim3(256), 0, s, (int *)w.Mbits, 2 * m, 0);
   const int64_t kchunk = 64;
   hipLaunchKernelGGL(k_row_exp, dim3((unsigned)((m + kThreads - 1) / kThreads), (unsigned)((k + kchunk - 1) / kchunk)),
-                     dim3(kThreads), 0, s, A, m, k, kchunk, w.E, w.Emin);
-  hipLaunchKernelGGL(k_row_range, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, w.E, w.Emin, m, w.range);
+                     dim3(kThreads), 0, s, A, m, k, kchunk, w.E, w.Emin, w.Mbits);
+  hipLaunchKernelGGL(k_row_finish, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, w.E, w.Emin, w.Mbits, m,
+                     w.range);
   hipLaunchKernelGGL(k_col_exp, dim3((unsigned)n), dim3(kThreads), 0, s, B, k, w.F, w.range + 1);
   return hipGetLastError();
 }
@@ -448,15 +498,18 @@ hipError_t ozaki_read_ranges(void *work, int64_t m, int64_t k, int64_t n, int S,
 
 int ozaki_choose_slices(int64_t k, int range_a, int range_b) {
   // Empirical model, fitted on MI355X and on the input distributions of the
-  // workloads (entries s*2^U(-e,e)*u):
-  //  * truncation error |err_ij| ~ 2^(E_i + F_j - 7S - 1) sqrt(k);
+  // workloads (entries s*2^U(-e,e)*u), in bits of digit precision b = 8S:
+  //  * truncation error |err_ij| ~ 2^(E_i + F_j - b - 1) sqrt(k);
   //  * (|A||B|)_ij / 2^(E_i + F_j) falls below 1 by at most W binades, with
   //    W = (range_a + range_b) * f(k), f the measured envelope below (max
   //    over e = 10, 20, 30 of the worst output among 512 x 512), interpolated
   //    in log2 k; W can never exceed range_a + range_b + 2;
   //  * 3 binades of margin (model mismatch seen at k = 300).
   // The smallest S with the 2^-100 (|A||B|)_ij target met under that model.
-  if (k <= 0) return 8;
+  // (The model was first fitted to 7-bit digits, S' = need / 7; 8-bit digits
+  // with the headroom rule above reach the same errors at 8S = 7S', checked
+  // with an exact model of the pipeline at k = 4096, spreads 2^+-20/30.)
+  if (k <= 0) return kMinSlices;
   static const double lk[] = {4, 5, 6, 7, 8.23, 9.97, 11, 12, 13};
   static const double fr[] = {0.42, 0.354, 0.28, 0.20, 0.12, 0.05, 0.0275, 0.004, 0.002};
   const double x = log2((double)k);
@@ -472,8 +525,8 @@ int ozaki_choose_slices(int64_t k, int range_a, int range_b) {
       }
   }
   const double need = 101.0 + 0.5 * x + (double)(range_a + range_b) * f + 3.0;
-  int S = (int)ceil(need / 7.0);
-  return S < 8 ? 8 : S;
+  int S = (int)ceil(need / 8.0);
+  return S < kMinSlices ? kMinSlices : S;
 }
 
 hipError_t ozaki_slice(const double2 *A, const double2 *B, int64_t m, int64_t k, int64_t n, int S,
@@ -500,6 +553,7 @@ hipError_t ozaki_multiply(double2 *C, int64_t m, int64_t k, int64_t n, int S, vo
   if (!oz_args_ok(k, S) || work_bytes < ozaki_workspace_bytes(m, k, n, S)) return hipErrorInvalidValue;
   OzWork w = carve(work, m, k, n, S);
   switch (S) {
+    case 7: return launch_main<7>(w.As, w.Bs, w.E, w.F, C, m, n, w.KB, w.tm, w.tn, s);
     case 8: return launch_main<8>(w.As, w.Bs, w.E, w.F, C, m, n, w.KB, w.tm, w.tn, s);
     case 9: return launch_main<9>(w.As, w.Bs, w.E, w.F, C, m, n, w.KB, w.tm, w.tn, s);
     case 10: return launch_main<10>(w.As, w.Bs, w.E, w.F, C, m, n, w.KB, w.tm, w.tn, s);
