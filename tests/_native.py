@@ -125,7 +125,9 @@ def exact_dot(a, b):
     return out
 
 
-def gemm_err(A, B, C, rows, cols):
+def gemm_err(A, B, C, rows, cols, threads=1):
+    """Exact error |C - AB|_ij and (|A||B|)_ij at the listed outputs; the
+    points are split over `threads` host threads (the C routine is pure)."""
     m, k = A.shape[:2]
     n = B.shape[1]
     A, B, C = cm(A), cm(B), cm(C)
@@ -133,7 +135,23 @@ def gemm_err(A, B, C, rows, cols):
     cols = np.ascontiguousarray(cols, dtype=np.int64)
     err = np.zeros(rows.size)
     mag = np.zeros(rows.size)
-    oracle().or_gemm_err(mptr(A), mptr(B), mptr(C), m, k, n, ptr(rows), ptr(cols), rows.size, ptr(err), ptr(mag))
+    O = oracle()
+
+    def part(lo, hi):
+        lo, hi = int(lo), int(hi)
+        if hi > lo:
+            O.or_gemm_err(mptr(A), mptr(B), mptr(C), m, k, n,
+                          ctypes.c_void_p(rows.ctypes.data + 8 * lo), ctypes.c_void_p(cols.ctypes.data + 8 * lo),
+                          hi - lo, ctypes.c_void_p(err.ctypes.data + 8 * lo),
+                          ctypes.c_void_p(mag.ctypes.data + 8 * lo))
+    threads = max(1, min(int(threads), rows.size))
+    if threads == 1:
+        part(0, rows.size)
+    else:
+        from concurrent.futures import ThreadPoolExecutor
+        cuts = np.linspace(0, rows.size, threads + 1).astype(int)
+        with ThreadPoolExecutor(threads) as ex:
+            list(ex.map(lambda t: part(cuts[t], cuts[t + 1]), range(threads)))
     return err, mag
 
 
