@@ -26,6 +26,7 @@
 
 #include "../../include/multifloats_hip.h"
 #include "dd_core.h"
+#include "gemm_tile.h"
 #include "mf_internal.h"
 
 namespace {
@@ -57,88 +58,16 @@ struct DotD {
   int id;  // index of the dot's completion counter
 };
 
-// One (16*TM) x (16*TN) output tile of a reference-order GEMM (the same
-// per-output chains as k_gemm_plain). `lds` holds 4 * BK * max(BM, BN)
-// doubles.
+// One output tile of a reference-order GEMM: the tile body of the standalone
+// GEMM (gemm_tile.h), with the static renormalisation schedule when the
+// problem's interval is 8 (the Fortran default).
 template <int TM, int TN>
-__device__ void gemm_tile(const GemmD &g, int tr, int tc, double *lds) {
-  constexpr int BM = 16 * TM, BN = 16 * TN;
-  constexpr int A_PER = BM * kBK / kThreads, B_PER = BN * kBK / kThreads;
-  double(*As_h)[BM] = (double(*)[BM])lds;
-  double(*As_l)[BM] = (double(*)[BM])(lds + kBK * BM);
-  double(*Bs_h)[BN] = (double(*)[BN])(lds + 2 * kBK * BM);
-  double(*Bs_l)[BN] = (double(*)[BN])(lds + 2 * kBK * BM + kBK * BN);
-  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-  const int64_t m = g.m, k = g.k, n = g.n;
-  const int64_t m0 = (int64_t)tr * BM, n0 = (int64_t)tc * BN;
-  double sh[TM][TN], sl[TM][TN];
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j) sh[i][j] = sl[i][j] = 0.0;
-  const int64_t rint = (g.ri > 0 && k > g.ri) ? g.ri : 0;
-  int64_t until = rint, done = 0;
-  for (int64_t k0 = 0; k0 < k; k0 += kBK) {
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < A_PER; ++q) {
-      int e = tid + q * kThreads, r = e % BM, c = e / BM;
-      int64_t gi = m0 + r, gp = k0 + c;
-      double2 v = (gi < m && gp < k) ? g.A[gi + gp * m] : make_double2(0.0, 0.0);
-      As_h[c][r] = v.x;
-      As_l[c][r] = v.y;
-    }
-#pragma unroll
-    for (int q = 0; q < B_PER; ++q) {
-      int e = tid + q * kThreads, r = e % kBK, c = e / kBK;
-      int64_t gp = k0 + r, gj = n0 + c;
-      double2 v = (gp < k && gj < n) ? g.B[gp + gj * k] : make_double2(0.0, 0.0);
-      Bs_h[r][c] = v.x;
-      Bs_l[r][c] = v.y;
-    }
-    __syncthreads();
-    const int kmax = (k - k0) < kBK ? (int)(k - k0) : kBK;
-    for (int kk = 0; kk < kmax; ++kk) {
-      double ah[TM], al[TM], bh[TN], bl[TN];
-#pragma unroll
-      for (int i = 0; i < TM; ++i) {
-        ah[i] = As_h[kk][tx * TM + i];
-        al[i] = As_l[kk][tx * TM + i];
-      }
-#pragma unroll
-      for (int j = 0; j < TN; ++j) {
-        bh[j] = Bs_h[kk][ty * TN + j];
-        bl[j] = Bs_l[kk][ty * TN + j];
-      }
-#pragma unroll
-      for (int i = 0; i < TM; ++i)
-#pragma unroll
-        for (int j = 0; j < TN; ++j) mac(ah[i], al[i], bh[j], bl[j], sh[i][j], sl[i][j]);
-      ++done;
-      if (rint && --until == 0) {
-        until = rint;
-        if (done < k) {
-#pragma unroll
-          for (int i = 0; i < TM; ++i)
-#pragma unroll
-            for (int j = 0; j < TN; ++j) renorm(sh[i][j], sl[i][j]);
-        }
-      }
-    }
-  }
-#pragma unroll
-  for (int j = 0; j < TN; ++j) {
-    const int64_t gj = n0 + ty * TN + j;
-#pragma unroll
-    for (int i = 0; i < TM; ++i) {
-      const int64_t gi = m0 + tx * TM + i;
-      if (gi < m && gj < n) {
-        double h = sh[i][j], l = sl[i][j];
-        renorm(h, l);
-        g.C[gi + gj * m] = make_double2(h, l);
-      }
-    }
-  }
+__device__ __forceinline__ void gemm_tile(const GemmD &g, int tr, int tc, double *lds) {
+  const int64_t m0 = (int64_t)tr * (16 * TM), n0 = (int64_t)tc * (16 * TN);
+  if (g.ri == 8)
+    mfhip_tile::plain_tile<TM, TN, kBK, 8>(g.A, g.B, g.C, g.m, g.k, g.n, g.ri, m0, n0, lds);
+  else
+    mfhip_tile::plain_tile<TM, TN, kBK, 0>(g.A, g.B, g.C, g.m, g.k, g.n, g.ri, m0, n0, lds);
 }
 
 // Block tree over 256 normalised pairs, fixed shape (the same as reduce.hip).

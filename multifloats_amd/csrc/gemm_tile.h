@@ -1,0 +1,167 @@
+// One (16*TM) x (16*TN) output tile of the reference-order DD GEMM, run by a
+// 256-lane workgroup; shared by the standalone GEMM kernel and the grouped
+// (batched) kernel.
+//
+// Semantics follow src/float64x2/matmul.inc:47-239 of the reference: every
+// output C(i,j) is one sequential chain over p = 0..k-1 of mac() with A(i,p)
+// as the "a" operand, a full TwoSum renormalisation after every complete chunk
+// of `ri` MACs that is not the last (when ri > 0 and k > ri), then a final
+// TwoSum. The reference's MR x NR blocking never changes that per-output
+// order, so any kernel that keeps the chain sequential per output -- as this
+// one does -- returns the reference's bits.
+//
+// Per K-block of BK, A and B tiles are de-interleaved from the AoS {hi,lo}
+// HBM layout into separate hi and lo LDS arrays (SoA, carved from `lds`,
+// 2*BK*(16*TM + 16*TN) doubles), so each lane reads its TM A-values and TN
+// B-values with 16-byte LDS loads and runs TM*TN independent MAC chains out
+// of registers: the FP64 VALU, not memory, is the bound (13 FP64 instructions
+// per MAC). The next K-block is prefetched into registers while the current
+// one is consumed.
+//
+// R > 0: when the renorm interval ri equals R and R divides BK, in every K
+// block that is not the last one the renormalisation points sit at the
+// static steps (kk+1) % R == 0 and the block is unrolled without branches
+// (the per-step branch otherwise costs a register copy per accumulator at
+// each join). R == 0 (or ri != R): ri is checked per step.
+#pragma once
+#include "dd_core.h"
+
+namespace mfhip_tile {
+
+constexpr int kThreads = 256;
+
+template <int TM, int TN, int BK, int R>
+__device__ __forceinline__ void plain_tile(const double2 *__restrict__ A, const double2 *__restrict__ B,
+                                           double2 *__restrict__ C, int64_t m, int64_t k, int64_t n,
+                                           int64_t ri, int64_t m0, int64_t n0, double *lds) {
+  static_assert(R == 0 || BK % R == 0, "static renorm interval must divide the K block");
+  constexpr int BM = 16 * TM, BN = 16 * TN;
+  constexpr int A_PER = BM * BK / kThreads;
+  constexpr int B_PER = BN * BK / kThreads;
+  static_assert(A_PER * kThreads == BM * BK && B_PER * kThreads == BN * BK, "tile");
+  double(*As_h)[BM] = (double(*)[BM])lds;
+  double(*As_l)[BM] = (double(*)[BM])(lds + BK * BM);
+  double(*Bs_h)[BN] = (double(*)[BN])(lds + 2 * BK * BM);
+  double(*Bs_l)[BN] = (double(*)[BN])(lds + 2 * BK * BM + BK * BN);
+
+  const int tid = threadIdx.x;
+  const int tx = tid & 15, ty = tid >> 4;
+
+  // Global -> register staging coordinates.
+  // A tile (BM rows x BK cols): consecutive threads walk down a column of A.
+  // B tile (BK rows x BN cols): consecutive threads walk down a column of B.
+  double2 ra[A_PER], rb[B_PER];
+  auto load = [&](int64_t k0) {
+#pragma unroll
+    for (int q = 0; q < A_PER; ++q) {
+      int e = tid + q * kThreads;
+      int r = e % BM, c = e / BM;
+      int64_t gi = m0 + r, gp = k0 + c;
+      ra[q] = (gi < m && gp < k) ? A[gi + gp * m] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int q = 0; q < B_PER; ++q) {
+      int e = tid + q * kThreads;
+      int r = e % BK, c = e / BK;
+      int64_t gp = k0 + r, gj = n0 + c;
+      rb[q] = (gp < k && gj < n) ? B[gp + gj * k] : make_double2(0.0, 0.0);
+    }
+  };
+  auto stash = [&]() {
+#pragma unroll
+    for (int q = 0; q < A_PER; ++q) {
+      int e = tid + q * kThreads;
+      As_h[e / BM][e % BM] = ra[q].x;
+      As_l[e / BM][e % BM] = ra[q].y;
+    }
+#pragma unroll
+    for (int q = 0; q < B_PER; ++q) {
+      int e = tid + q * kThreads;
+      Bs_h[e % BK][e / BK] = rb[q].x;
+      Bs_l[e % BK][e / BK] = rb[q].y;
+    }
+  };
+
+  double sh[TM][TN], sl[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) sh[i][j] = sl[i][j] = 0.0;
+
+  const int64_t rint = (ri > 0 && k > ri) ? ri : 0;
+  int64_t until = rint;  // MACs left before the next renormalisation point
+  int64_t done = 0;
+
+  auto macs = [&](int kk) {
+    double ah[TM], al[TM], bh[TN], bl[TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      ah[i] = As_h[kk][tx * TM + i];
+      al[i] = As_l[kk][tx * TM + i];
+    }
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      bh[j] = Bs_h[kk][ty * TN + j];
+      bl[j] = Bs_l[kk][ty * TN + j];
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) mac(ah[i], al[i], bh[j], bl[j], sh[i][j], sl[i][j]);
+  };
+  auto renorm_all = [&]() {
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) renorm(sh[i][j], sl[i][j]);
+  };
+  auto step = [&](int kk) {
+    macs(kk);
+    ++done;
+    if (rint && --until == 0) {
+      until = rint;
+      if (done < k) renorm_all();
+    }
+  };
+
+  load(0);
+  for (int64_t k0 = 0; k0 < k; k0 += BK) {
+    __syncthreads();
+    stash();
+    __syncthreads();
+    if (k0 + BK < k) load(k0 + BK);
+    const int64_t kmax = (k - k0) < BK ? (k - k0) : BK;
+    if (R > 0 && rint == R && k0 + BK < k) {
+      // static renormalisation points; `until` stays in phase since R | BK
+      constexpr int RR = R > 0 ? R : 1;
+#pragma unroll 1
+      for (int c = 0; c < BK; c += RR) {
+#pragma unroll
+        for (int kk = 0; kk < RR; ++kk) macs(c + kk);
+        renorm_all();
+      }
+      done += BK;
+    } else if (kmax == BK) {
+#pragma unroll
+      for (int kk = 0; kk < BK; ++kk) step(kk);
+    } else {
+      for (int kk = 0; kk < (int)kmax; ++kk) step(kk);
+    }
+  }
+
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    const int64_t gj = n0 + ty * TN + j;
+    if (gj >= n) continue;
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      const int64_t gi = m0 + tx * TM + i;
+      if (gi >= m) continue;
+      double h = sh[i][j], l = sl[i][j];
+      renorm(h, l);
+      C[gi + gj * m] = make_double2(h, l);
+    }
+  }
+}
+
+}  // namespace mfhip_tile
