@@ -10,13 +10,17 @@
 //
 // Plan: the host cuts every GEMM into output tiles of the tile variant its
 // size class was tuned to (32x32 or 64x64) and every dot into chunks of
-// kDotChunk elements, then sorts the items by estimated cost, largest first
-// (longest processing time). The persistent kernel's workgroups take the
-// sorted items round robin; every workgroup's loop is bounded by the item
-// count, so the grid always drains. A dot's chunks leave DD partials; the
-// chunk that completes a dot last (per-dot atomic counter) folds that dot's
-// partials in index order, so results do not depend on scheduling. The
-// counters are cleared by a fill kernel at the start of every run
+// kDotChunk (2^20) elements, then sorts the items by estimated cost, largest
+// first (longest processing time). The persistent kernel's workgroups take
+// the sorted items round robin; every workgroup's loop is bounded by the item
+// count, so the grid always drains. A dot of one chunk is a single item that
+// writes its result directly. A longer dot's chunks leave DD partials; the
+// chunk that completes the dot last (per-dot atomic counter) folds the
+// partials in index order, so results do not depend on scheduling. That
+// hand-off needs device-scope release/acquire fences, which on this part
+// write back / invalidate an XCD's whole L2: with 16 Ki-element chunks the
+// fences of ~64 Ki dot items held 50 000 dots to ~1 TB/s, hence the large
+// chunk. The counters are cleared by a fill kernel at the start of every run
 // (graph-capture safe: no memset nodes).
 #include <stdlib.h>
 #include <string.h>
@@ -33,7 +37,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kBK = 16;
-constexpr int64_t kDotChunk = 16384;
+constexpr int64_t kDotChunk = 1 << 20;  // elements per work item of one dot
 
 enum { ITEM_G32 = 0, ITEM_G64 = 1, ITEM_DOT = 2 };
 
@@ -95,23 +99,55 @@ __device__ void dot_chunk(const DotD &d, int c, double2 *partials, int *dot_done
   const int64_t p1 = p0 + kDotChunk < d.n ? p0 + kDotChunk : d.n;
   double sh = 0.0, sl = 0.0;
   int64_t until = d.ri;
-  for (int64_t i = p0 + tid; i < p1; i += kThreads) {
-    const double2 a = d.a[i], b = d.b[i];
-    // Fortran dot update (fypp:2286-2302)
-    const double p = a.x * b.x;
-    const double e = __builtin_fma(a.x, b.x, -p);
-    const double cross = a.x * b.y + a.y * b.x;
-    const double t = sh + p, bp = t - sh, ap = t - bp;
-    sl = sl + (((e + cross) + (sh - ap)) + (p - bp));
-    sh = t;
-    if (d.ri > 0 && --until == 0) {
-      until = d.ri;
-      fast_two_sum(sh, sl, sh, sl);
+  // Lane tid takes elements p0 + tid + 256 j in order. They are loaded eight
+  // at a time with the next eight already in flight (nontemporal 16-byte
+  // loads, two register buffers), so the latency-bound accumulation chain
+  // does not starve the memory system.
+  constexpr int kE = 8;
+  constexpr int64_t kRound = (int64_t)kThreads * kE;
+  v2d xa[kE], xb[kE], ya[kE], yb[kE];  // two register buffers, indexed statically
+  auto fetch = [&](v2d (&va)[kE], v2d (&vb)[kE], int64_t i0) {
+#pragma unroll
+    for (int j = 0; j < kE; ++j) {
+      const int64_t i = i0 + (int64_t)kThreads * j;
+      const bool ok = i < p1;
+      va[j] = ok ? __builtin_nontemporal_load((const v2d *)d.a + i) : v2d{0.0, 0.0};
+      vb[j] = ok ? __builtin_nontemporal_load((const v2d *)d.b + i) : v2d{0.0, 0.0};
     }
+  };
+  auto consume = [&](const v2d (&va)[kE], const v2d (&vb)[kE], int64_t i0) {
+#pragma unroll
+    for (int j = 0; j < kE; ++j) {
+      if (i0 + (int64_t)kThreads * j >= p1) break;
+      const v2d a = va[j], b = vb[j];
+      // Fortran dot update (fypp:2286-2302)
+      const double p = a.x * b.x;
+      const double e = __builtin_fma(a.x, b.x, -p);
+      const double cross = a.x * b.y + a.y * b.x;
+      const double t = sh + p, bp = t - sh, ap = t - bp;
+      sl = sl + (((e + cross) + (sh - ap)) + (p - bp));
+      sh = t;
+      if (d.ri > 0 && --until == 0) {
+        until = d.ri;
+        fast_two_sum(sh, sl, sh, sl);
+      }
+    }
+  };
+  fetch(xa, xb, p0 + tid);
+  for (int64_t i0 = p0 + tid; i0 < p1; i0 += 2 * kRound) {
+    if (i0 + kRound < p1) fetch(ya, yb, i0 + kRound);
+    consume(xa, xb, i0);
+    if (i0 + kRound >= p1) break;
+    if (i0 + 2 * kRound < p1) fetch(xa, xb, i0 + 2 * kRound);
+    consume(ya, yb, i0 + kRound);
   }
   renorm(sh, sl);
   double *th = lds, *tl = lds + kThreads;
   const ddv r = tree256(mk(sh, sl), th, tl);
+  if (d.nchunks == 1) {  // the whole dot: no partials, no cross-workgroup hand-off
+    if (tid == 0) *d.out = make_double2(r.hi, r.lo);
+    return;
+  }
   __shared__ int last;
   if (tid == 0) {
     partials[d.first + c] = make_double2(r.hi, r.lo);

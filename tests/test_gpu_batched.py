@@ -11,7 +11,7 @@ from tests import _native as nt
 pytestmark = pytest.mark.gpu
 
 
-def build(torch, rng, ngemm, ndot, variants=None, hi=96, dhi=40000):
+def build(torch, rng, ngemm, ndot, variants=None, hi=96, dhi=40000, long_dot=0):
     sizes, dlen = datagen.batched_sizes(rng, ngemm, ndot, hi=hi, dhi=dhi)
     sizes[0] = (1, 1, 1)
     sizes[1] = (70, 0, 33)          # k = 0 -> zeros
@@ -19,6 +19,8 @@ def build(torch, rng, ngemm, ndot, variants=None, hi=96, dhi=40000):
     gbuf = torch.empty(2 * tot, dtype=torch.float64, device="cuda")
     datagen.fill_config0_torch(torch, gbuf, 5)
     dlen[0] = 1
+    if long_dot:
+        dlen[1] = long_dot          # longer than one work item: chunks + partials
     dtot = int(dlen.sum())
     dbuf = torch.empty(4 * dtot, dtype=torch.float64, device="cuda")
     datagen.fill_config0_torch(torch, dbuf, 6)
@@ -44,7 +46,8 @@ def build(torch, rng, ngemm, ndot, variants=None, hi=96, dhi=40000):
 def test_grouped_gemm_and_dot_parity(gpu, variant):
     import torch
     rng = np.random.default_rng(3 if variant is None else variant)
-    sizes, dlen, gbuf, dbuf, dout, gemms, dots, offs, doff = build(torch, rng, 300, 200)
+    sizes, dlen, gbuf, dbuf, dout, gemms, dots, offs, doff = build(torch, rng, 300, 200,
+                                                                   long_dot=2_500_000 if variant is None else 0)
     var = None if variant is None else [variant] * len(gemms)
     plan = mf.GroupedPlan(gemms, dots, var)
     s = torch.cuda.current_stream().cuda_stream
@@ -70,7 +73,7 @@ def test_grouped_gemm_and_dot_parity(gpu, variant):
 def test_grouped_graph_replays_match_eager(gpu):
     import torch
     rng = np.random.default_rng(8)
-    sizes, dlen, gbuf, dbuf, dout, gemms, dots, offs, doff = build(torch, rng, 120, 120)
+    sizes, dlen, gbuf, dbuf, dout, gemms, dots, offs, doff = build(torch, rng, 120, 120, long_dot=3_000_001)
     plan = mf.GroupedPlan(gemms, dots)
     s = torch.cuda.current_stream().cuda_stream
     plan.run(s)
