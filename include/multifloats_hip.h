@@ -245,7 +245,9 @@ MULTIFLOATS_API int mfhip_ozaki_multiply(float64x2 *c, int64_t m, int64_t k, int
  * (it allocates its device-side work list), then run any number of times on
  * the same buffers; run_graph captures the launch in a HIP graph on first use
  * and replays it afterwards. tile_variant[i] picks the GEMM tile for problem
- * i: 0 = 32x32, 1 = 64x64, -1 = built-in rule (NULL: rule for all). */
+ * i: 0 = 32x32, 1 = 64x64, 2 = 128x16, 3 = 16x128, 4 = 64x32, 5 = 32x64,
+ * -1 = built-in rule: least padded work weighted by a per-shape cost (NULL:
+ * rule for all). */
 typedef struct {
   const float64x2 *a;
   const float64x2 *b;

@@ -8,8 +8,9 @@
 // reference's per-element update per lane, then a fixed tree of accurate DD
 // additions.
 //
-// Plan: the host cuts every GEMM into output tiles of the tile variant its
-// size class was tuned to (32x32 or 64x64) and every dot into chunks of
+// Plan: the host cuts every GEMM into output tiles of its tile variant (one
+// of six shapes, chosen by the caller's tuned table or by least padded work)
+// and every dot into chunks of
 // kDotChunk (2^20) elements, then sorts the items by estimated cost, largest
 // first (longest processing time). The persistent kernel's workgroups take
 // the sorted items round robin; every workgroup's loop is bounded by the item
@@ -39,7 +40,19 @@ constexpr int kThreads = 256;
 constexpr int kBK = 16;
 constexpr int64_t kDotChunk = 1 << 20;  // elements per work item of one dot
 
-enum { ITEM_G32 = 0, ITEM_G64 = 1, ITEM_DOT = 2 };
+// GEMM tile variants (kind of a GEMM work item): (TM, TN) outputs per lane,
+// (16 TM) x (16 TN) tiles. 0: 32x32, 1: 64x64, 2: 128x16, 3: 16x128,
+// 4: 64x32, 5: 32x64. The skinny shapes keep the padding of tall or wide
+// problems small.
+constexpr int kNumTileVariants = 6;
+constexpr int kTileTM[kNumTileVariants] = {2, 4, 8, 1, 4, 2};
+constexpr int kTileTN[kNumTileVariants] = {2, 4, 1, 8, 2, 4};
+// Relative cost per padded MAC of each variant (fewer chains per lane or
+// more LDS reads per MAC cost more); used by the built-in choice.
+constexpr double kTileCost[kNumTileVariants] = {1.3, 1.0, 1.15, 1.15, 1.08, 1.08};
+enum { ITEM_DOT = kNumTileVariants };
+// LDS of the largest tile (2 * 16 * (16 TM + 16 TN) doubles).
+constexpr int kGroupedLds = 2 * 16 * (16 * 8 + 16 * 1);
 
 struct Item {
   int kind;
@@ -176,12 +189,18 @@ __device__ void dot_chunk(const DotD &d, int c, double2 *partials, int *dot_done
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_grouped(
     const Item *__restrict__ items, int nitems, const GemmD *__restrict__ gemms,
     const DotD *__restrict__ dots, double2 *__restrict__ partials, int *__restrict__ dot_done) {
-  __shared__ __attribute__((aligned(16))) double lds[4 * kBK * 64];
+  __shared__ __attribute__((aligned(16))) double lds[kGroupedLds];
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const Item w = items[it];
-    if (w.kind == ITEM_G64) gemm_tile<4, 4>(gemms[w.prob], w.a, w.b, lds);
-    else if (w.kind == ITEM_G32) gemm_tile<2, 2>(gemms[w.prob], w.a, w.b, lds);
-    else dot_chunk(dots[w.prob], w.a, partials, dot_done, lds);
+    switch (w.kind) {
+      case 0: gemm_tile<2, 2>(gemms[w.prob], w.a, w.b, lds); break;
+      case 1: gemm_tile<4, 4>(gemms[w.prob], w.a, w.b, lds); break;
+      case 2: gemm_tile<8, 1>(gemms[w.prob], w.a, w.b, lds); break;
+      case 3: gemm_tile<1, 8>(gemms[w.prob], w.a, w.b, lds); break;
+      case 4: gemm_tile<4, 2>(gemms[w.prob], w.a, w.b, lds); break;
+      case 5: gemm_tile<2, 4>(gemms[w.prob], w.a, w.b, lds); break;
+      default: dot_chunk(dots[w.prob], w.a, partials, dot_done, lds); break;
+    }
   }
 }
 
@@ -251,14 +270,25 @@ void *mfhip_grouped_create(const mfhip_gemm_problem *gemms, int ngemm, const int
                   g.renorm_interval};
     if (g.m <= 0 || g.n <= 0) continue;
     int v = tile_variant ? tile_variant[i] : -1;
-    if (v < 0) v = (g.m > 32 && g.n > 32) ? ITEM_G64 : ITEM_G32;
-    const int64_t T = v == ITEM_G64 ? 64 : 32;
-    const int64_t tm = (g.m + T - 1) / T, tn = (g.n + T - 1) / T;
+    if (v >= kNumTileVariants) v = -1;
+    if (v < 0) {  // least padded work, weighted by the variant's relative cost
+      double best = 0.0;
+      for (int q = 0; q < kNumTileVariants; ++q) {
+        const int64_t bm = 16 * kTileTM[q], bn = 16 * kTileTN[q];
+        const double c = (double)((g.m + bm - 1) / bm * bm) * (double)((g.n + bn - 1) / bn * bn) * kTileCost[q];
+        if (v < 0 || c < best) {
+          v = q;
+          best = c;
+        }
+      }
+    }
+    const int64_t bm = 16 * kTileTM[v], bn = 16 * kTileTN[v];
+    const int64_t tm = (g.m + bm - 1) / bm, tn = (g.n + bn - 1) / bn;
     for (int64_t a = 0; a < tm; ++a)
       for (int64_t b = 0; b < tn; ++b) {
         items.push_back(Item{v, i, (int)a, (int)b});
         // MAC count of the padded tile (k = 0 still writes zeros: count 1)
-        cost.push_back((double)(T * T) * (double)(g.k > 0 ? g.k : 1));
+        cost.push_back((double)(bm * bn) * (double)(g.k > 0 ? g.k : 1) * kTileCost[v]);
       }
   }
   int64_t nparts = 0;

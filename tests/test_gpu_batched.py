@@ -42,7 +42,7 @@ def build(torch, rng, ngemm, ndot, variants=None, hi=96, dhi=40000, long_dot=0):
     return sizes, dlen, gbuf, dbuf, dout, gemms, dots, offs, doff
 
 
-@pytest.mark.parametrize("variant", [None, 0, 1])
+@pytest.mark.parametrize("variant", [None, 0, 1, 2, 3, 4, 5])
 def test_grouped_gemm_and_dot_parity(gpu, variant):
     import torch
     rng = np.random.default_rng(3 if variant is None else variant)

@@ -159,3 +159,21 @@ def test_fortran_bridge_fails_loudly_without_gpu(tmp_path):
     assert "multifloats_hip: matmuldd_mm failed" in p.stderr
     vals = np.fromfile(out, dtype=np.float64)
     assert vals.size > 0 and np.isnan(vals).all()
+
+
+def test_grouped_tile_choice_minimises_padded_work():
+    """The tuned planner gives tall, wide and small problems the tile shapes
+    that waste the fewest lanes (equal rates: pure padding)."""
+    from multifloats_amd import autotune
+    table = {"variant_rates": [1.0] * len(autotune.TILES)}
+    sizes = np.array([[400, 64, 10], [10, 64, 400], [64, 64, 64], [20, 64, 20], [120, 30, 60]])
+    v = autotune.plan_variants(sizes, table)
+    assert autotune.TILES[v[0]] == (128, 16)
+    assert autotune.TILES[v[1]] == (16, 128)
+    assert autotune.TILES[v[2]] == (64, 64) or autotune.padded_macs(64, 64, 64, v[2]) == 64 ** 3
+    assert autotune.TILES[v[3]] == (32, 32)
+    for (m, k, n), q in zip(sizes, v):
+        assert autotune.padded_macs(m, k, n, q) == min(autotune.padded_macs(m, k, n, w) for w in autotune.VARIANTS)
+    # a slower variant loses ties
+    table = {"variant_rates": [1.0, 2.0, 1.0, 1.0, 1.0, 1.0]}
+    assert autotune.plan_variants(np.array([[64, 64, 64]]), table) == [1]
