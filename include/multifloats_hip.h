@@ -153,6 +153,19 @@ MULTIFLOATS_API void sumdd_dim(const float64x2 *a, const int32_t *mask, int64_t 
 MULTIFLOATS_API void norm2dd_dim(const float64x2 *a, int64_t inner, int64_t len, int64_t outer,
                                  float64x2 *out);
 
+/* BLAS-style C = alpha op(A) op(B) + beta C on float64x2 (the interface of
+ * the reference's xGEMM shim test/integration/blas/wgemm.f90:1-14; SURVEY's
+ * "next" GEMM interface). transa/transb: 'N', 'T' or 'C' (= 'T' for real);
+ * column-major with leading dimensions lda, ldb, ldc; argument checks and
+ * quick returns as wgemm.f90:16-69 (alpha == 0 or k == 0: C = beta C without
+ * reading A, B; beta == 0: C is not read). op(A) op(B) is formed by the
+ * reference-order chains of matmuldd_mm with the given renorm interval, then
+ * alpha T + beta C with the DD operators. Invalid arguments write NaN to C
+ * (as a failed call does) instead of calling xerbla. */
+MULTIFLOATS_API void wgemmdd(char transa, char transb, int64_t m, int64_t n, int64_t k, float64x2 alpha,
+                             const float64x2 *a, int64_t lda, const float64x2 *b, int64_t ldb,
+                             float64x2 beta, float64x2 *c, int64_t ldc, int64_t renorm_interval);
+
 /* Precision form of matmuldd_mm (no counterpart in the reference): the Ozaki
  * split-precision GEMM on the INT8 matrix cores. A and B are cut into
  * `slices` int8 digit slices (8 bits each) per row / column scale; slice
@@ -216,6 +229,12 @@ MULTIFLOATS_API int mfhip_cdot(const complex64x2 *a, const complex64x2 *b, int64
                                int64_t renorm_interval, complex64x2 *out, void *work, void *stream);
 MULTIFLOATS_API int mfhip_reduce_dim(int kind, const float64x2 *a, const int32_t *mask, int64_t inner,
                                      int64_t len, int64_t outer, float64x2 *out, void *stream);
+
+/* Device form of wgemmdd. */
+MULTIFLOATS_API int mfhip_gemm(char transa, char transb, int64_t m, int64_t n, int64_t k, float64x2 alpha,
+                               const float64x2 *a, int64_t lda, const float64x2 *b, int64_t ldb,
+                               float64x2 beta, float64x2 *c, int64_t ldc, int64_t renorm_interval,
+                               void *stream);
 
 /* Ozaki GEMM on device pointers; `work` is caller-owned device scratch of at
  * least mfhip_ozaki_workspace_bytes(m, k, n, slices) bytes. With slices <= 0

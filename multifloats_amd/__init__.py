@@ -109,6 +109,10 @@ def lib():
         L.mfhip_ozaki_choose_slices.argtypes = [I64, I, I]
         L.mfhip_ozaki_prepare.argtypes = [P, P, I64, I64, I64, I, P, ctypes.c_size_t, VP]
         L.mfhip_ozaki_multiply.argtypes = [P, I64, I64, I64, I, P, ctypes.c_size_t, VP]
+        C = ctypes.c_char
+        L.wgemmdd.argtypes = [C, C, I64, I64, I64, float64x2, P, I64, P, I64, float64x2, P, I64, I64]
+        L.wgemmdd.restype = None
+        L.mfhip_gemm.argtypes = [C, C, I64, I64, I64, float64x2, P, I64, P, I64, float64x2, P, I64, I64, VP]
         L.mfhip_last_error.argtypes = []
         L.mfhip_error_string.argtypes = [I]
         L.mfhip_error_string.restype = ctypes.c_char_p
@@ -171,6 +175,56 @@ def matmul(a, b, renorm_interval=DD_FMA_RENORM_INTERVAL):
         lib().matmuldd_vm(_ptr(a), _ptr(b), _ptr(c), k, n, renorm_interval)
     else:
         raise ValueError("matmul: need (matrix, matrix), (matrix, vector) or (vector, matrix)")
+    _check_status()
+    return c
+
+
+def _f2(x):
+    """A DD scalar as float64x2: a float, or a (hi, lo) pair."""
+    v = np.asarray(x, dtype=np.float64).reshape(-1)
+    if v.size == 1:
+        return float64x2((ctypes.c_double * 2)(float(v[0]), 0.0))
+    if v.size != 2:
+        raise ValueError("a DD scalar is a float or a (hi, lo) pair")
+    return float64x2((ctypes.c_double * 2)(float(v[0]), float(v[1])))
+
+
+def _trans(t):
+    if not isinstance(t, str) or len(t) != 1:
+        raise ValueError("transa/transb: one of 'N', 'T', 'C'")
+    return t.encode()
+
+
+def gemm(transa, transb, alpha, a, b, beta=0.0, c=None, m=None, n=None, k=None,
+         renorm_interval=DD_FMA_RENORM_INTERVAL):
+    """BLAS xGEMM on real64x2: C = alpha op(A) op(B) + beta C (the interface of
+    the reference's shim test/integration/blas/wgemm.f90:1-14).
+
+    a, b, c are column-major storage arrays of shape (ld, cols, 2); their first
+    axis is the leading dimension, so op(A) may be the leading m x k block of a
+    larger array. m, n, k default to the full op(A), op(B). c is updated in
+    place when it is already column-major float64 (else a column-major copy is
+    made) and returned; c=None means beta C = 0 on a fresh (m, n) array.
+    alpha, beta: floats or (hi, lo) pairs. The argument errors of wgemm.f90:16-51
+    raise RuntimeError (C is then NaN)."""
+    ta, tb = _trans(transa), _trans(transb)
+    nota, notb = ta in (b"N", b"n"), tb in (b"N", b"n")
+    a, b = _mat(a), _mat(b)
+    if m is None:
+        m = a.shape[0] if nota else a.shape[1]
+    if k is None:
+        k = a.shape[1] if nota else a.shape[0]
+    if n is None:
+        n = b.shape[1] if notb else b.shape[0]
+    if (b.shape[0] if notb else b.shape[1]) < k:
+        raise ValueError("gemm: op(B) has fewer than k rows")
+    if c is None:
+        c = _mat(np.zeros((max(m, 1), n, 2)))
+        beta = 0.0
+    elif not datagen.is_colmajor(c):
+        c = _mat(c)
+    lib().wgemmdd(ta, tb, m, n, k, _f2(alpha), _ptr(a), max(a.shape[0], 1), _ptr(b), max(b.shape[0], 1),
+                  _f2(beta), _ptr(c), max(c.shape[0], 1), renorm_interval)
     _check_status()
     return c
 
@@ -362,6 +416,13 @@ def _dev(rc):
 
 def dev_matmul(a, b, c, m, k, n, renorm_interval=DD_FMA_RENORM_INTERVAL, stream=0, variant=-1):
     _dev(lib().mfhip_matmul_mm_variant(a, b, c, m, k, n, renorm_interval, variant, stream))
+
+
+def dev_gemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
+             renorm_interval=DD_FMA_RENORM_INTERVAL, stream=0):
+    """Device form of gemm: a, b, c are device addresses (column-major DD)."""
+    _dev(lib().mfhip_gemm(_trans(transa), _trans(transb), m, n, k, _f2(alpha), a, lda, b, ldb, _f2(beta),
+                          c, ldc, renorm_interval, stream))
 
 
 def ozaki_slices_used():

@@ -12,6 +12,7 @@
 
 
 #include "../../include/multifloats_hip.h"
+#include "dd_core.h"
 #include "mf_internal.h"
 
 using multifloats::complex64x2;
@@ -237,6 +238,30 @@ hipError_t host_dim(int kind, const float64x2 *a, const int32_t *mask, int64_t i
   return hipStreamSynchronize(tl_stage.stream);
 }
 
+hipError_t host_gemm_blas(char ta, char tb, int64_t m, int64_t n, int64_t k, float64x2 alpha,
+                          const float64x2 *a, int64_t lda, const float64x2 *b, int64_t ldb, float64x2 beta,
+                          float64x2 *c, int64_t ldc, int64_t ri) {
+  tl_err = 0;
+  TRY(mfhip::gemm_blas_check(ta, tb, m, n, k, lda, ldb, ldc));
+  if (m == 0 || n == 0) return hipSuccess;
+  TRY(stage_init());
+  const bool nota = ta == 'N' || ta == 'n', notb = tb == 'N' || tb == 'n';
+  // elements of A, B, C the call may touch (column-major with leading dims)
+  const size_t na = (size_t)lda * (size_t)(nota ? k : m), nb = (size_t)ldb * (size_t)(notb ? n : k);
+  const size_t nc = (size_t)ldc * (size_t)n;
+  void *da, *db, *dc;
+  TRY(stage_buf(0, 16 * na, &da));
+  TRY(stage_buf(1, 16 * nb, &db));
+  TRY(stage_buf(2, 16 * nc, &dc));
+  TRY(h2d(da, a, 16 * na));
+  TRY(h2d(db, b, 16 * nb));
+  TRY(h2d(dc, c, 16 * nc));
+  TRY(mfhip::gemm_blas(ta, tb, m, n, k, ddv{alpha.limbs[0], alpha.limbs[1]}, D(da), lda, D(db), ldb,
+                       ddv{beta.limbs[0], beta.limbs[1]}, W(dc), ldc, ri, tl_stage.stream));
+  TRY(d2h(c, dc, 16 * nc));
+  return hipStreamSynchronize(tl_stage.stream);
+}
+
 thread_local int tl_oz_used = 0;
 
 hipError_t host_cmm(const complex64x2 *a, const complex64x2 *b, complex64x2 *c, int64_t m,
@@ -454,6 +479,17 @@ void norm2dd_dim(const float64x2 *a, int64_t inner, int64_t len, int64_t outer, 
   }
 }
 
+void wgemmdd(char transa, char transb, int64_t m, int64_t n, int64_t k, float64x2 alpha, const float64x2 *a,
+             int64_t lda, const float64x2 *b, int64_t ldb, float64x2 beta, float64x2 *c, int64_t ldc,
+             int64_t renorm_interval) {
+  hipError_t e = host_gemm_blas(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, renorm_interval);
+  if (e != hipSuccess) {
+    report("wgemmdd", e);
+    if (c && m > 0 && n > 0 && ldc >= m)
+      for (int64_t j = 0; j < n; ++j) nan_fill(c + j * ldc, m);
+  }
+}
+
 // ---- device-pointer API ----
 
 static int status(hipError_t e) {
@@ -532,6 +568,15 @@ int mfhip_reduce_dim(int kind, const float64x2 *a, const int32_t *mask, int64_t 
                      int64_t outer, float64x2 *out, void *stream) {
   CHECK_ALIGN(a, out);
   return status(mfhip::reduce_dim(kind, D(a), mask, inner, len, outer, W(out), (hipStream_t)stream));
+}
+
+int mfhip_gemm(char transa, char transb, int64_t m, int64_t n, int64_t k, float64x2 alpha, const float64x2 *a,
+               int64_t lda, const float64x2 *b, int64_t ldb, float64x2 beta, float64x2 *c, int64_t ldc,
+               int64_t renorm_interval, void *stream) {
+  CHECK_ALIGN(a, b, c);
+  return status(mfhip::gemm_blas(transa, transb, m, n, k, ddv{alpha.limbs[0], alpha.limbs[1]}, D(a), lda, D(b), ldb,
+                                 ddv{beta.limbs[0], beta.limbs[1]}, W(c), ldc, renorm_interval,
+                                 (hipStream_t)stream));
 }
 
 size_t mfhip_ozaki_workspace_bytes(int64_t m, int64_t k, int64_t n, int slices) {

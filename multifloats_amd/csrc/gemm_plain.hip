@@ -28,6 +28,36 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WPE)))
                                          (int64_t)(tile / tiles_m) * (16 * TN), lds);
 }
 
+// BLAS-style C = alpha op(A) op(B) + beta C with leading dimensions
+// (op = transpose when TA / TB); the chains are the matmuldd_mm chains of
+// op(A) op(B), the epilogue applies alpha and beta with the DD operators.
+template <int TM, int TN, int BK, int WPE, int R, bool TA, bool TB>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WPE))) void k_gemm_blas(
+    const double2 *__restrict__ A, int64_t lda, const double2 *__restrict__ B, int64_t ldb,
+    double2 *__restrict__ C, int64_t ldc, int64_t m, int64_t k, int64_t n, int64_t ri, ddv alpha, ddv beta,
+    int tiles_m) {
+  __shared__ double lds[2 * BK * (16 * TM + 16 * TN)];
+  const int tile = blockIdx.x;
+  mfhip_tile::gemm_tile<TM, TN, BK, R, TA, TB, true>(A, lda, B, ldb, C, ldc, m, k, n, ri,
+                                                      (int64_t)(tile % tiles_m) * (16 * TM),
+                                                      (int64_t)(tile / tiles_m) * (16 * TN), lds, alpha, beta);
+}
+
+// C = beta C (beta == 0: zeros, C not read) for the BLAS quick paths.
+__global__ __launch_bounds__(kThreads) void k_scale_c(double2 *__restrict__ C, int64_t ldc, int64_t m, int64_t n,
+                                                      ddv beta) {
+  const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (t >= m * n) return;
+  const int64_t i = t % m, j = t / m;
+  double2 *c = C + i + j * ldc;
+  if (beta.hi == 0.0 && beta.lo == 0.0) {
+    *c = make_double2(0.0, 0.0);
+  } else {
+    const ddv r = dd_mul(beta, mk(c->x, c->y));
+    *c = make_double2(r.hi, r.lo);
+  }
+}
+
 // y(m) = A(m,k) x(k): one thread per row, A read down its columns (coalesced),
 // x broadcast. Reference matmul.inc:207-211 via gaxpy_mv_panel (47-88).
 __global__ __launch_bounds__(kThreads) void k_gemv(const double2 *__restrict__ A,
@@ -279,6 +309,62 @@ hipError_t gemm_plain(const double2 *A, const double2 *B, double2 *C, int64_t m,
     case 5: return launch_gemm<8, 4, 16, 1>(A, B, C, m, k, n, ri, s);
     default: return launch_gemm<4, 8, 16, 2>(A, B, C, m, k, n, ri, s);
   }
+}
+
+namespace {
+template <int TM, int TN, int WPE, bool TA, bool TB>
+hipError_t launch_blas(const double2 *A, int64_t lda, const double2 *B, int64_t ldb, double2 *C, int64_t ldc,
+                       int64_t m, int64_t k, int64_t n, int64_t ri, ddv alpha, ddv beta, hipStream_t s) {
+  constexpr int BM = 16 * TM, BN = 16 * TN;
+  const int64_t tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
+  if (tm * tn > 0x7fffffff) return hipErrorInvalidValue;
+  if (ri == 8 && k > 8)
+    hipLaunchKernelGGL((k_gemm_blas<TM, TN, 16, WPE, 8, TA, TB>), dim3((unsigned)(tm * tn)), dim3(kThreads), 0,
+                       s, A, lda, B, ldb, C, ldc, m, k, n, ri, alpha, beta, (int)tm);
+  else
+    hipLaunchKernelGGL((k_gemm_blas<TM, TN, 16, WPE, 0, TA, TB>), dim3((unsigned)(tm * tn)), dim3(kThreads), 0,
+                       s, A, lda, B, ldb, C, ldc, m, k, n, ri, alpha, beta, (int)tm);
+  return hipGetLastError();
+}
+template <bool TA, bool TB>
+hipError_t blas_by_size(const double2 *A, int64_t lda, const double2 *B, int64_t ldb, double2 *C, int64_t ldc,
+                        int64_t m, int64_t k, int64_t n, int64_t ri, ddv alpha, ddv beta, hipStream_t s) {
+  if (m * n >= (int64_t)256 * 256)
+    return launch_blas<4, 4, 2, TA, TB>(A, lda, B, ldb, C, ldc, m, k, n, ri, alpha, beta, s);
+  return launch_blas<2, 2, 4, TA, TB>(A, lda, B, ldb, C, ldc, m, k, n, ri, alpha, beta, s);
+}
+bool trans_ok(char t) { return t == 'N' || t == 'n' || t == 'T' || t == 't' || t == 'C' || t == 'c'; }
+}  // namespace
+
+// BLAS xGEMM argument rules (order of the reference's wgemm shim,
+// test/integration/blas/wgemm.f90:16-51) and quick returns (52-69).
+hipError_t gemm_blas_check(char ta, char tb, int64_t m, int64_t n, int64_t k, int64_t lda, int64_t ldb,
+                           int64_t ldc) {
+  const bool nota = ta == 'N' || ta == 'n', notb = tb == 'N' || tb == 'n';
+  const int64_t nrowa = nota ? m : k, nrowb = notb ? k : n;
+  if (!trans_ok(ta) || !trans_ok(tb) || m < 0 || n < 0 || k < 0 || lda < (nrowa > 1 ? nrowa : 1) ||
+      ldb < (nrowb > 1 ? nrowb : 1) || ldc < (m > 1 ? m : 1))
+    return hipErrorInvalidValue;
+  return hipSuccess;
+}
+
+hipError_t gemm_blas(char ta, char tb, int64_t m, int64_t n, int64_t k, ddv alpha, const double2 *A,
+                     int64_t lda, const double2 *B, int64_t ldb, ddv beta, double2 *C, int64_t ldc, int64_t ri,
+                     hipStream_t s) {
+  if (hipError_t e = gemm_blas_check(ta, tb, m, n, k, lda, ldb, ldc); e != hipSuccess) return e;
+  const bool nota = ta == 'N' || ta == 'n', notb = tb == 'N' || tb == 'n';
+  const bool alpha0 = alpha.hi == 0.0 && alpha.lo == 0.0;
+  const bool beta1 = beta.hi == 1.0 && beta.lo == 0.0;
+  if (m == 0 || n == 0 || ((alpha0 || k == 0) && beta1)) return hipSuccess;
+  if (alpha0 || k == 0) {
+    hipLaunchKernelGGL(k_scale_c, dim3((unsigned)((m * n + kThreads - 1) / kThreads)), dim3(kThreads), 0, s, C,
+                       ldc, m, n, beta);
+    return hipGetLastError();
+  }
+  if (nota && notb) return blas_by_size<false, false>(A, lda, B, ldb, C, ldc, m, k, n, ri, alpha, beta, s);
+  if (nota) return blas_by_size<false, true>(A, lda, B, ldb, C, ldc, m, k, n, ri, alpha, beta, s);
+  if (notb) return blas_by_size<true, false>(A, lda, B, ldb, C, ldc, m, k, n, ri, alpha, beta, s);
+  return blas_by_size<true, true>(A, lda, B, ldb, C, ldc, m, k, n, ri, alpha, beta, s);
 }
 
 hipError_t gemm_cplx(const double4 *A, const double4 *B, double4 *C, int64_t m, int64_t k,

@@ -30,10 +30,19 @@ namespace mfhip_tile {
 
 constexpr int kThreads = 256;
 
-template <int TM, int TN, int BK, int R>
-__device__ __forceinline__ void plain_tile(const double2 *__restrict__ A, const double2 *__restrict__ B,
-                                           double2 *__restrict__ C, int64_t m, int64_t k, int64_t n,
-                                           int64_t ri, int64_t m0, int64_t n0, double *lds) {
+// General form: C(m,n) = op(A) op(B) with BLAS-style leading dimensions.
+// TA: op(A) = A^T (element (i,p) at A[p + i*lda]), else A[i + p*lda]; TB
+// likewise for B ((p,j) at B[j + p*ldb] or B[p + j*ldb]); C(i,j) at
+// C[i + j*ldc]. AB: the result T of the chains is written as
+// alpha*T + beta*C (DD operators; C is not read when beta == 0).
+// Threads walk the contiguous dimension of each operand, so every layout
+// loads coalesced.
+template <int TM, int TN, int BK, int R, bool TA = false, bool TB = false, bool AB = false>
+__device__ __forceinline__ void gemm_tile(const double2 *__restrict__ A, int64_t lda,
+                                          const double2 *__restrict__ B, int64_t ldb,
+                                          double2 *__restrict__ C, int64_t ldc, int64_t m, int64_t k,
+                                          int64_t n, int64_t ri, int64_t m0, int64_t n0, double *lds,
+                                          ddv alpha = ddv{1.0, 0.0}, ddv beta = ddv{0.0, 0.0}) {
   static_assert(R == 0 || BK % R == 0, "static renorm interval must divide the K block");
   constexpr int BM = 16 * TM, BN = 16 * TN;
   constexpr int A_PER = BM * BK / kThreads;
@@ -47,38 +56,40 @@ __device__ __forceinline__ void plain_tile(const double2 *__restrict__ A, const 
   const int tid = threadIdx.x;
   const int tx = tid & 15, ty = tid >> 4;
 
-  // Global -> register staging coordinates.
-  // A tile (BM rows x BK cols): consecutive threads walk down a column of A.
-  // B tile (BK rows x BN cols): consecutive threads walk down a column of B.
+  // Global -> register staging coordinates: element e of the A tile
+  // (BM rows x BK cols) is (ar(e), ac(e)), of the B tile (BK x BN) (br, bc);
+  // consecutive threads walk the operand's contiguous dimension.
+  auto ar = [](int e) { return TA ? e / BK : e % BM; };
+  auto ac = [](int e) { return TA ? e % BK : e / BM; };
+  auto br = [](int e) { return TB ? e / BN : e % BK; };
+  auto bc = [](int e) { return TB ? e % BN : e / BK; };
   double2 ra[A_PER], rb[B_PER];
   auto load = [&](int64_t k0) {
 #pragma unroll
     for (int q = 0; q < A_PER; ++q) {
-      int e = tid + q * kThreads;
-      int r = e % BM, c = e / BM;
-      int64_t gi = m0 + r, gp = k0 + c;
-      ra[q] = (gi < m && gp < k) ? A[gi + gp * m] : make_double2(0.0, 0.0);
+      const int e = tid + q * kThreads;
+      const int64_t gi = m0 + ar(e), gp = k0 + ac(e);
+      ra[q] = (gi < m && gp < k) ? A[TA ? gp + gi * lda : gi + gp * lda] : make_double2(0.0, 0.0);
     }
 #pragma unroll
     for (int q = 0; q < B_PER; ++q) {
-      int e = tid + q * kThreads;
-      int r = e % BK, c = e / BK;
-      int64_t gp = k0 + r, gj = n0 + c;
-      rb[q] = (gp < k && gj < n) ? B[gp + gj * k] : make_double2(0.0, 0.0);
+      const int e = tid + q * kThreads;
+      const int64_t gp = k0 + br(e), gj = n0 + bc(e);
+      rb[q] = (gp < k && gj < n) ? B[TB ? gj + gp * ldb : gp + gj * ldb] : make_double2(0.0, 0.0);
     }
   };
   auto stash = [&]() {
 #pragma unroll
     for (int q = 0; q < A_PER; ++q) {
-      int e = tid + q * kThreads;
-      As_h[e / BM][e % BM] = ra[q].x;
-      As_l[e / BM][e % BM] = ra[q].y;
+      const int e = tid + q * kThreads;
+      As_h[ac(e)][ar(e)] = ra[q].x;
+      As_l[ac(e)][ar(e)] = ra[q].y;
     }
 #pragma unroll
     for (int q = 0; q < B_PER; ++q) {
-      int e = tid + q * kThreads;
-      Bs_h[e % BK][e / BK] = rb[q].x;
-      Bs_l[e % BK][e / BK] = rb[q].y;
+      const int e = tid + q * kThreads;
+      Bs_h[br(e)][bc(e)] = rb[q].x;
+      Bs_l[br(e)][bc(e)] = rb[q].y;
     }
   };
 
@@ -159,9 +170,26 @@ __device__ __forceinline__ void plain_tile(const double2 *__restrict__ A, const 
       if (gi >= m) continue;
       double h = sh[i][j], l = sl[i][j];
       renorm(h, l);
-      C[gi + gj * m] = make_double2(h, l);
+      if (AB) {
+        ddv o = dd_mul(alpha, mk(h, l));
+        if (beta.hi != 0.0 || beta.lo != 0.0) {
+          const double2 c0 = C[gi + gj * ldc];
+          o = dd_add(o, dd_mul(beta, mk(c0.x, c0.y)));
+        }
+        h = o.hi;
+        l = o.lo;
+      }
+      C[gi + gj * ldc] = make_double2(h, l);
     }
   }
+}
+
+// The standalone matmuldd_mm tile: contiguous column-major operands.
+template <int TM, int TN, int BK, int R>
+__device__ __forceinline__ void plain_tile(const double2 *__restrict__ A, const double2 *__restrict__ B,
+                                           double2 *__restrict__ C, int64_t m, int64_t k, int64_t n,
+                                           int64_t ri, int64_t m0, int64_t n0, double *lds) {
+  gemm_tile<TM, TN, BK, R>(A, m, B, k, C, m, m, k, n, ri, m0, n0, lds);
 }
 
 }  // namespace mfhip_tile

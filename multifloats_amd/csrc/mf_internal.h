@@ -8,6 +8,8 @@
 
 typedef double v2d __attribute__((ext_vector_type(2)));
 
+struct ddv;  // dd_core.h
+
 namespace mfhip {
 
 enum { OP_SQRT = 0, OP_EXP = 1, OP_LOG = 2, OP_SIN = 3, OP_ERF = 4, OP_NEG = 5 };
@@ -23,6 +25,14 @@ hipError_t gemm_plain(const double2 *A, const double2 *B, double2 *C, int64_t m,
 // four-real-GEMM composition; also serves mv (n = 1) and vm (m = 1).
 hipError_t gemm_cplx(const double4 *A, const double4 *B, double4 *C, int64_t m, int64_t k,
                      int64_t n, int64_t ri, hipStream_t s);
+// BLAS argument checks of wgemm.f90:16-51 (transposes, sizes, leading dimensions)
+hipError_t gemm_blas_check(char ta, char tb, int64_t m, int64_t n, int64_t k, int64_t lda, int64_t ldb,
+                           int64_t ldc);
+// BLAS-style C = alpha op(A) op(B) + beta C (ta/tb 'N', 'T' or 'C'; leading
+// dimensions lda/ldb/ldc); invalid arguments give hipErrorInvalidValue.
+hipError_t gemm_blas(char ta, char tb, int64_t m, int64_t n, int64_t k, ddv alpha, const double2 *A,
+                     int64_t lda, const double2 *B, int64_t ldb, ddv beta, double2 *C, int64_t ldc, int64_t ri,
+                     hipStream_t s);
 hipError_t gemv(const double2 *A, const double2 *x, double2 *y, int64_t m, int64_t k, int64_t ri,
                 hipStream_t s);
 hipError_t gevm(const double2 *x, const double2 *B, double2 *y, int64_t k, int64_t n, int64_t ri,

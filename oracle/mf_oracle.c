@@ -171,6 +171,45 @@ void or_matmul_mv(const dd *a, const dd *x, dd *y, int64_t m, int64_t k, int64_t
 void or_matmul_vm(const dd *x, const dd *b, dd *y, int64_t k, int64_t n, int64_t ri) {
   for (int64_t j = 0; j < n; ++j) y[j] = o_chain(x, 1, b + j * k, 1, k, ri);
 }
+/* BLAS-style C = alpha op(A) op(B) + beta C with the argument rules and quick
+ * returns of the reference's xGEMM shim (test/integration/blas/wgemm.f90:16-69),
+ * op(A) op(B) formed as matmul.inc's per-output chains (o_chain) and combined
+ * with the DD operators as o_add(o_mul(alpha, T), o_mul(beta, C)) (beta == 0:
+ * o_mul(alpha, T), C not read). alpha == 0 or k == 0: C = beta C (zeros when
+ * beta == 0). Returns the wgemm.f90 info code (0 = success, C untouched else). */
+int or_gemm_blas(char ta, char tb, int64_t m, int64_t n, int64_t k, const dd *alpha, const dd *a, int64_t lda,
+                 const dd *b, int64_t ldb, const dd *beta, dd *c, int64_t ldc, int64_t ri) {
+  int nota = ta == 'N' || ta == 'n', notb = tb == 'N' || tb == 'n';
+  int oka = nota || ta == 'T' || ta == 't' || ta == 'C' || ta == 'c';
+  int okb = notb || tb == 'T' || tb == 't' || tb == 'C' || tb == 'c';
+  int64_t nrowa = nota ? m : k, nrowb = notb ? k : n;
+  if (!oka) return 1;
+  if (!okb) return 2;
+  if (m < 0) return 3;
+  if (n < 0) return 4;
+  if (k < 0) return 5;
+  if (lda < (nrowa > 1 ? nrowa : 1)) return 8;
+  if (ldb < (nrowb > 1 ? nrowb : 1)) return 10;
+  if (ldc < (m > 1 ? m : 1)) return 13;
+  int alpha0 = alpha->hi == 0.0 && alpha->lo == 0.0, beta0 = beta->hi == 0.0 && beta->lo == 0.0;
+  int beta1 = beta->hi == 1.0 && beta->lo == 0.0;
+  if (m == 0 || n == 0 || ((alpha0 || k == 0) && beta1)) return 0;
+  for (int64_t j = 0; j < n; ++j)
+    for (int64_t i = 0; i < m; ++i) {
+      dd *cij = &c[i + j * ldc], o;
+      if (alpha0 || k == 0) {
+        o = beta0 ? mkdd(0.0, 0.0) : o_mul(*beta, *cij);
+      } else {
+        /* op(A)(i, p) and op(B)(p, j) as base pointer + stride over p */
+        dd t = o_chain(nota ? a + i : a + i * lda, nota ? lda : 1, notb ? b + j * ldb : b + j, notb ? 1 : ldb,
+                       k, ri);
+        o = o_mul(*alpha, t);
+        if (!beta0) o = o_add(o, o_mul(*beta, *cij));
+      }
+      *cij = o;
+    }
+  return 0;
+}
 /* fsrc/multifloats.fypp:2609-2621 (cdd_matmul_mm): four real GEMMs, then the
  * native DD - and + (bit-identical to the C++ operators, fypp:2772-2790).
  * Complex arrays are interleaved {re.hi, re.lo, im.hi, im.lo}. */

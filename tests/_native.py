@@ -48,6 +48,9 @@ def oracle():
         L.or_cdot.argtypes = [P, P, I64, I64, P]
         L.or_sum_dim.argtypes = [P, P, I64, I64, I64, P]
         L.or_norm2_dim.argtypes = [P, I64, I64, I64, P]
+        L.or_gemm_blas.argtypes = [ctypes.c_char, ctypes.c_char, I64, I64, I64, P, P, I64, P, I64, P, P, I64,
+                                   I64]
+        L.or_gemm_blas.restype = ctypes.c_int
         D = ctypes.POINTER(ctypes.c_double)
         L.or_exact_dot_err.argtypes = [P, I64, P, I64, I64, P, D, D]
         L.or_exact_dot.argtypes = [P, I64, P, I64, I64, P]
@@ -97,6 +100,22 @@ def or_matmul(A, B, ri):
     C = cm(np.zeros((m, n, 2)))
     oracle().or_matmul_mm(mptr(A), mptr(B), mptr(C), m, k, n, ri)
     return C
+
+
+def or_gemm_blas(ta, tb, m, n, k, alpha, A, B, beta, C, ri, lda=None, ldb=None, ldc=None):
+    """The oracle's BLAS-style GEMM on column-major storage arrays (ld, cols, 2);
+    C is updated in place. Returns wgemm.f90's info code. lda/ldb/ldc default
+    to the storage arrays' first extents."""
+    assert datagen.is_colmajor(A) and datagen.is_colmajor(B) and datagen.is_colmajor(C)
+    al = np.array(alpha, dtype=np.float64).reshape(-1)
+    be = np.array(beta, dtype=np.float64).reshape(-1)
+    al = np.array([al[0], al[1] if al.size > 1 else 0.0])
+    be = np.array([be[0], be[1] if be.size > 1 else 0.0])
+    lda = max(A.shape[0], 1) if lda is None else lda
+    ldb = max(B.shape[0], 1) if ldb is None else ldb
+    ldc = max(C.shape[0], 1) if ldc is None else ldc
+    return oracle().or_gemm_blas(ta.encode(), tb.encode(), m, n, k, ptr(al), mptr(A), lda, mptr(B), ldb, ptr(be),
+                                 mptr(C), ldc, ri)
 
 
 def res(r):
