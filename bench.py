@@ -12,7 +12,7 @@ Prints ONE JSON line: value = Ozaki DD-GFLOP/s (2*m*n*k per step) when its
 measured error on a sampled block is within 2^-100 (|A||B|) elementwise, with
 the plain form, the error figures and the speed-up in the same line.
 
-  python bench.py [--steps K] [--warmup W] [--workload gemm|elementwise|reduce]
+  python bench.py [--steps K] [--warmup W] [--workload gemm|cgemm|blas|batched|elementwise|reduce]
                   [--no-cpu] [--variant V]
 
 Timing: torch.cuda events recorded on the stream the kernels are launched on
@@ -55,7 +55,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=2)
-    p.add_argument("--workload", default="gemm", choices=["gemm", "cgemm", "batched", "elementwise", "reduce"])
+    p.add_argument("--workload", default="gemm", choices=["gemm", "cgemm", "blas", "batched", "elementwise", "reduce"])
     p.add_argument("--retune", action="store_true", help="batched: rerun the tile autotuner first")
     p.add_argument("--size", type=int, default=4096)
     p.add_argument("--spread", type=float, default=20.0)
@@ -236,6 +236,37 @@ def bench_cgemm(args, torch, mf):
                          "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None}}
 
 
+def bench_blas(args, torch, mf):
+    """BLAS-style C = alpha op(A) op(B) + beta C at m = n = k = size for the four
+    transpose pairs (general alpha and beta, renorm interval 8)."""
+    from multifloats_amd import datagen
+    m = n = k = args.size
+    rng = np.random.default_rng(5)
+    dev = {}
+    for nm in ("A", "B", "C"):
+        dev[nm] = torch.from_numpy(np.ascontiguousarray(
+            datagen.matrix(rng, m, m, args.spread).transpose(1, 0, 2)).reshape(-1, 2)).cuda()
+    stream = torch.cuda.current_stream().cuda_stream
+    alpha, beta = (1.5, 2.0 ** -60), (-0.75, 2.0 ** -58)
+    per = {}
+    for ta, tb in ("NN", "TN", "NT", "TT"):
+        def step():
+            mf.dev_gemm(ta, tb, m, n, k, alpha, dev["A"].data_ptr(), m, dev["B"].data_ptr(), k, beta,
+                        dev["C"].data_ptr(), m, 8, stream)
+        ms = timed(torch, step, args.steps, args.warmup)
+        per[ta + tb] = {"ms": round(ms, 4), "dd_gflops": round(2.0 * m * n * k / (ms * 1e-3) / 1e9, 2)}
+    worst = max(per.values(), key=lambda r: r["ms"])
+    achieved = FP64_FLOPS_PER_DDMAC * m * n * k / (worst["ms"] * 1e-3) / 1e12
+    return {"metric": "BLAS-style DD-GEMM DD-GFLOP/s (slowest transpose pair)", "value": worst["dd_gflops"],
+            "unit": "DD-GFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": worst["ms"], "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "float64x2 C = alpha op(A) op(B) + beta C, m=n=k=%d, NN/TN/NT/TT, "
+                                   "renorm 8" % m, "per_pair": per},
+            "roofline": {"bound": "fp64", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None},
+            "vs_baseline": None}
+
+
 def bench_batched(args, torch, mf):
     """10,000 float64x2 GEMMs (m, n, k log-uniform in [8, 512]) and 50,000
     dot products (length log-uniform in [16, 65536]), config-0 entries, in one
@@ -389,7 +420,7 @@ def main():
     import multifloats_amd as mf
     if not torch.cuda.is_available():
         sys.exit("bench.py: no GPU visible")
-    fn = {"gemm": bench_gemm, "cgemm": bench_cgemm, "batched": bench_batched,
+    fn = {"gemm": bench_gemm, "cgemm": bench_cgemm, "blas": bench_blas, "batched": bench_batched,
           "elementwise": bench_elementwise, "reduce": bench_reduce}[args.workload]
     print(json.dumps(fn(args, torch, mf)), flush=True)
 

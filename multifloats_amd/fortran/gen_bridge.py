@@ -90,6 +90,7 @@ module multifloats_hip
     integer, public, protected :: DD_FMA_RENORM_INTERVAL = 8
     public :: dd_set_fma_renorm_interval
     public :: dd_matmul_ozaki
+    public :: wgemm
     public :: matmul, dot_product, sum, norm2
     public :: exp, log, sin, erf, sqrt
     public :: operator(+), operator(-), operator(*), operator(/)
@@ -130,6 +131,18 @@ module multifloats_hip
             type(c_ptr), value :: a, b, c
             integer(c_int64_t), value :: m, k, n
             integer(c_int), value :: slices
+        end subroutine
+        ! alpha and beta are float64x2 by value in C. The x86-64 System V ABI
+        ! passes such a struct in two SSE registers, exactly as its two limbs;
+        ! flang passes a second by-value bind(c) struct on the stack instead,
+        ! so they are spelled here as their limbs.
+        subroutine c_wgemmdd(ta, tb, m, n, k, alpha_hi, alpha_lo, a, lda, b, ldb, beta_hi, beta_lo, c, ldc, ri) &
+                bind(c, name='wgemmdd')
+            import :: c_ptr, c_int64_t, c_char, c_double
+            character(kind=c_char), value :: ta, tb
+            integer(c_int64_t), value :: m, n, k, lda, ldb, ldc, ri
+            real(c_double), value :: alpha_hi, alpha_lo, beta_hi, beta_lo
+            type(c_ptr), value :: a, b, c
         end subroutine
         pure type(float64x2) function c_dotdd(a, b, n, ri) bind(c, name='dotdd')
             import :: c_ptr, c_int64_t, float64x2
@@ -278,6 +291,21 @@ module multifloats_hip
         call c_matmuldd_mm_ozaki(c_loc(a), c_loc(b), c_loc(res), int(size(a,1), c_int64_t), &
             int(size(a,2), c_int64_t), int(size(b,2), c_int64_t), s)
     end function
+
+    ! xGEMM on real64x2 with the argument list of the reference's test shim
+    ! (test/integration/blas/wgemm.f90:1-14): C = alpha op(A) op(B) + beta C,
+    ! transa/transb 'N', 'T' or 'C'. op(A) op(B) is formed like matmul (same
+    ! chains, DD_FMA_RENORM_INTERVAL); bad arguments leave NaN in C.
+    subroutine wgemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc)
+        character, intent(in) :: transa, transb
+        integer, intent(in) :: m, n, k, lda, ldb, ldc
+        type(real64x2), intent(in) :: alpha, beta
+        type(real64x2), intent(in), target :: a(lda, *), b(ldb, *)
+        type(real64x2), intent(inout), target :: c(ldc, *)
+        call c_wgemmdd(transa, transb, int(m, c_int64_t), int(n, c_int64_t), int(k, c_int64_t), &
+            alpha%limbs(1), alpha%limbs(2), c_loc(a(1, 1)), int(lda, c_int64_t), c_loc(b(1, 1)), &
+            int(ldb, c_int64_t), beta%limbs(1), beta%limbs(2), c_loc(c(1, 1)), int(ldc, c_int64_t), ri())
+    end subroutine
 
     pure function dd_dot_product(a, b) result(res)
         type(real64x2), intent(in), target, contiguous :: a(:), b(:)

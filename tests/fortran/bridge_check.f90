@@ -12,7 +12,7 @@ program bridge_check
     implicit none
     integer, parameter :: dp = 8
     integer(8) :: m, k, n, nv
-    type(real64x2), allocatable :: a(:,:), b(:,:), x(:), y(:)
+    type(real64x2), allocatable :: a(:,:), b(:,:), x(:), y(:), at(:,:), cc(:,:)
     type(cmplx64x2), allocatable :: ca(:,:), cb(:,:), cx(:), cy(:)
     integer(4), allocatable :: imask(:)
     logical, allocatable :: mask(:)
@@ -70,5 +70,12 @@ program bridge_check
     write(u) x / y
     write(u) exp(a)                     ! rank 2
     write(u) dd_matmul_ozaki(a, b)
+    ! xGEMM: C = x(1) A B + y(1) C with A passed transposed and C padded to m + 2 rows
+    at = transpose(a)
+    allocate(cc(m + 2, n))
+    cc(1:m, :) = a(:, 1:n)
+    cc(m + 1:m + 2, :) = x(1)
+    call wgemm('T', 'N', int(m), int(n), int(k), x(1), at, int(k), b, int(k), y(1), cc, int(m + 2))
+    write(u) cc
     close(u)
 end program bridge_check

@@ -91,4 +91,11 @@ def test_fortran_bridge_routes_to_the_library(gpu, tmp_path):
     rows, cols = np.meshgrid(np.arange(m), np.arange(n), indexing="ij")
     err, mag = nt.gemm_err(A, B, C, rows.ravel(), cols.ravel())
     assert (err <= 2.0 ** -100 * mag).all(), "ozaki through the bridge"
+    At = nt.cm(A.transpose(1, 0, 2))
+    Cw = np.empty((m + 2, n, 2))
+    Cw[:m] = A[:, :n]
+    Cw[m:] = x[0]
+    Cw = nt.cm(Cw)
+    assert nt.or_gemm_blas("T", "N", m, n, k, x[0], At, nt.cm(B), y[0], Cw, 8) == 0
+    nt.assert_bits(r.take((m + 2) * n), datagen.flat(Cw), "wgemm T N")
     assert r.pos == r.buf.size, "output size"

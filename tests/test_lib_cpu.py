@@ -158,7 +158,12 @@ def test_fortran_bridge_fails_loudly_without_gpu(tmp_path):
     assert p.returncode == 0
     assert "multifloats_hip: matmuldd_mm failed" in p.stderr
     vals = np.fromfile(out, dtype=np.float64)
-    assert vals.size > 0 and np.isnan(vals).all()
+    # the last record is wgemm's C, (m + 2) x n with two padding rows that are
+    # not outputs and keep their input value
+    wg = vals[-(m + 2) * n * 2:].reshape(n, m + 2, 2)
+    assert "multifloats_hip: wgemmdd failed" in p.stderr
+    assert vals.size > wg.size and np.isnan(vals[:-wg.size]).all()
+    assert np.isnan(wg[:, :m]).all() and (wg[:, m:] == 1.0).all()
 
 
 def test_grouped_tile_choice_minimises_padded_work():
