@@ -166,6 +166,16 @@ MULTIFLOATS_API void wgemmdd(char transa, char transb, int64_t m, int64_t n, int
                              const float64x2 *a, int64_t lda, const float64x2 *b, int64_t ldb,
                              float64x2 beta, float64x2 *c, int64_t ldc, int64_t renorm_interval);
 
+/* Triangular solve op(A) X = alpha B (side 'L') or X op(A) = alpha B (side
+ * 'R'), X overwriting B: the reference's xTRSM shim
+ * test/integration/blas/wtrsm.f90:1-10, bit for bit (its eight loop nests,
+ * 57-206, in the same operation order, including the skips of zero pivots
+ * and zero coefficients). uplo 'U'/'L', transa 'N'/'T'/'C', diag 'U' (unit,
+ * diagonal not read) or 'N'. Argument checks and quick returns as wtrsm.f90:
+ * 24-56 (alpha == 0: B = 0). Invalid arguments write NaN to B. */
+MULTIFLOATS_API void wtrsmdd(char side, char uplo, char transa, char diag, int64_t m, int64_t n, float64x2 alpha,
+                             const float64x2 *a, int64_t lda, float64x2 *b, int64_t ldb);
+
 /* Precision form of matmuldd_mm (no counterpart in the reference): the Ozaki
  * split-precision GEMM on the INT8 matrix cores. A and B are cut into
  * `slices` int8 digit slices (8 bits each) per row / column scale; slice
@@ -235,6 +245,13 @@ MULTIFLOATS_API int mfhip_gemm(char transa, char transb, int64_t m, int64_t n, i
                                const float64x2 *a, int64_t lda, const float64x2 *b, int64_t ldb,
                                float64x2 beta, float64x2 *c, int64_t ldc, int64_t renorm_interval,
                                void *stream);
+
+/* Device form of wtrsmdd; work holds mfhip_trsm_workspace_bytes(side, m, n)
+ * bytes (read and written; calls that share it must not overlap). */
+MULTIFLOATS_API size_t mfhip_trsm_workspace_bytes(char side, int64_t m, int64_t n);
+MULTIFLOATS_API int mfhip_trsm(char side, char uplo, char transa, char diag, int64_t m, int64_t n,
+                               float64x2 alpha, const float64x2 *a, int64_t lda, float64x2 *b, int64_t ldb,
+                               void *work, size_t work_bytes, void *stream);
 
 /* Ozaki GEMM on device pointers; `work` is caller-owned device scratch of at
  * least mfhip_ozaki_workspace_bytes(m, k, n, slices) bytes. With slices <= 0

@@ -113,6 +113,11 @@ def lib():
         L.wgemmdd.argtypes = [C, C, I64, I64, I64, float64x2, P, I64, P, I64, float64x2, P, I64, I64]
         L.wgemmdd.restype = None
         L.mfhip_gemm.argtypes = [C, C, I64, I64, I64, float64x2, P, I64, P, I64, float64x2, P, I64, I64, VP]
+        L.wtrsmdd.argtypes = [C, C, C, C, I64, I64, float64x2, P, I64, P, I64]
+        L.wtrsmdd.restype = None
+        L.mfhip_trsm_workspace_bytes.argtypes = [C, I64, I64]
+        L.mfhip_trsm_workspace_bytes.restype = ctypes.c_size_t
+        L.mfhip_trsm.argtypes = [C, C, C, C, I64, I64, float64x2, P, I64, P, I64, P, ctypes.c_size_t, VP]
         L.mfhip_last_error.argtypes = []
         L.mfhip_error_string.argtypes = [I]
         L.mfhip_error_string.restype = ctypes.c_char_p
@@ -190,8 +195,9 @@ def _f2(x):
 
 
 def _trans(t):
+    """A BLAS option letter (trans, side, uplo, diag) as a C char."""
     if not isinstance(t, str) or len(t) != 1:
-        raise ValueError("transa/transb: one of 'N', 'T', 'C'")
+        raise ValueError("BLAS options are single letters, e.g. 'N', 'T', 'L', 'U'")
     return t.encode()
 
 
@@ -227,6 +233,27 @@ def gemm(transa, transb, alpha, a, b, beta=0.0, c=None, m=None, n=None, k=None,
                   _f2(beta), _ptr(c), max(c.shape[0], 1), renorm_interval)
     _check_status()
     return c
+
+
+def trsm(side, uplo, transa, diag, alpha, a, b, m=None, n=None):
+    """BLAS xTRSM on real64x2, bit-identical to the reference's shim
+    test/integration/blas/wtrsm.f90: solves op(A) X = alpha B (side 'L') or
+    X op(A) = alpha B (side 'R') for X, which overwrites b.
+
+    a, b are column-major storage arrays (ld, cols, 2) (first axis = leading
+    dimension); m, n default to b's logical extent. b is updated in place when
+    it is already column-major float64 (else a column-major copy is made) and
+    returned. The argument errors of wtrsm.f90:24-47 raise RuntimeError (B is
+    then NaN)."""
+    a = _mat(a)
+    if not datagen.is_colmajor(b):
+        b = _mat(b)
+    m = b.shape[0] if m is None else m
+    n = b.shape[1] if n is None else n
+    lib().wtrsmdd(_trans(side), _trans(uplo), _trans(transa), _trans(diag), m, n, _f2(alpha), _ptr(a),
+                  max(a.shape[0], 1), _ptr(b), max(b.shape[0], 1))
+    _check_status()
+    return b
 
 
 def _cmat(a):
@@ -423,6 +450,16 @@ def dev_gemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
     """Device form of gemm: a, b, c are device addresses (column-major DD)."""
     _dev(lib().mfhip_gemm(_trans(transa), _trans(transb), m, n, k, _f2(alpha), a, lda, b, ldb, _f2(beta),
                           c, ldc, renorm_interval, stream))
+
+
+def trsm_workspace_bytes(side, m, n):
+    return lib().mfhip_trsm_workspace_bytes(_trans(side), m, n)
+
+
+def dev_trsm(side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb, work, work_bytes, stream=0):
+    """Device form of trsm: a, b, work are device addresses."""
+    _dev(lib().mfhip_trsm(_trans(side), _trans(uplo), _trans(transa), _trans(diag), m, n, _f2(alpha), a, lda, b,
+                          ldb, work, work_bytes, stream))
 
 
 def ozaki_slices_used():

@@ -262,6 +262,27 @@ hipError_t host_gemm_blas(char ta, char tb, int64_t m, int64_t n, int64_t k, flo
   return hipStreamSynchronize(tl_stage.stream);
 }
 
+hipError_t host_trsm(char side, char uplo, char ta, char diag, int64_t m, int64_t n, float64x2 alpha,
+                     const float64x2 *a, int64_t lda, float64x2 *b, int64_t ldb) {
+  tl_err = 0;
+  TRY(mfhip::trsm_check(side, uplo, ta, diag, m, n, lda, ldb));
+  if (m == 0 || n == 0) return hipSuccess;
+  TRY(stage_init());
+  const int64_t s = (side == 'L' || side == 'l') ? m : n;
+  const size_t na = (size_t)lda * (size_t)s, nb = (size_t)ldb * (size_t)n;
+  const size_t wb = mfhip::trsm_workspace_bytes(side, m, n);
+  void *da, *db, *dw;
+  TRY(stage_buf(0, 16 * na, &da));
+  TRY(stage_buf(1, 16 * nb, &db));
+  TRY(stage_buf(3, wb, &dw));
+  TRY(h2d(da, a, 16 * na));
+  TRY(h2d(db, b, 16 * nb));
+  TRY(mfhip::trsm(side, uplo, ta, diag, m, n, ddv{alpha.limbs[0], alpha.limbs[1]}, D(da), lda, W(db), ldb, dw, wb,
+                  tl_stage.stream));
+  TRY(d2h(b, db, 16 * nb));
+  return hipStreamSynchronize(tl_stage.stream);
+}
+
 thread_local int tl_oz_used = 0;
 
 hipError_t host_cmm(const complex64x2 *a, const complex64x2 *b, complex64x2 *c, int64_t m,
@@ -490,6 +511,16 @@ void wgemmdd(char transa, char transb, int64_t m, int64_t n, int64_t k, float64x
   }
 }
 
+void wtrsmdd(char side, char uplo, char transa, char diag, int64_t m, int64_t n, float64x2 alpha, const float64x2 *a,
+             int64_t lda, float64x2 *b, int64_t ldb) {
+  hipError_t e = host_trsm(side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb);
+  if (e != hipSuccess) {
+    report("wtrsmdd", e);
+    if (b && m > 0 && n > 0 && ldb >= m)
+      for (int64_t j = 0; j < n; ++j) nan_fill(b + j * ldb, m);
+  }
+}
+
 // ---- device-pointer API ----
 
 static int status(hipError_t e) {
@@ -577,6 +608,15 @@ int mfhip_gemm(char transa, char transb, int64_t m, int64_t n, int64_t k, float6
   return status(mfhip::gemm_blas(transa, transb, m, n, k, ddv{alpha.limbs[0], alpha.limbs[1]}, D(a), lda, D(b), ldb,
                                  ddv{beta.limbs[0], beta.limbs[1]}, W(c), ldc, renorm_interval,
                                  (hipStream_t)stream));
+}
+
+size_t mfhip_trsm_workspace_bytes(char side, int64_t m, int64_t n) { return mfhip::trsm_workspace_bytes(side, m, n); }
+
+int mfhip_trsm(char side, char uplo, char transa, char diag, int64_t m, int64_t n, float64x2 alpha, const float64x2 *a,
+               int64_t lda, float64x2 *b, int64_t ldb, void *work, size_t work_bytes, void *stream) {
+  CHECK_ALIGN(a, b, b);
+  return status(mfhip::trsm(side, uplo, transa, diag, m, n, ddv{alpha.limbs[0], alpha.limbs[1]}, D(a), lda, W(b),
+                            ldb, work, work_bytes, (hipStream_t)stream));
 }
 
 size_t mfhip_ozaki_workspace_bytes(int64_t m, int64_t k, int64_t n, int slices) {

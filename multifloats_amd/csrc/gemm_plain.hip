@@ -336,6 +336,13 @@ hipError_t blas_by_size(const double2 *A, int64_t lda, const double2 *B, int64_t
 bool trans_ok(char t) { return t == 'N' || t == 'n' || t == 'T' || t == 't' || t == 'C' || t == 'c'; }
 }  // namespace
 
+hipError_t scale_matrix(double2 *C, int64_t ldc, int64_t m, int64_t n, ddv beta, hipStream_t s) {
+  if (m <= 0 || n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scale_c, dim3((unsigned)((m * n + kThreads - 1) / kThreads)), dim3(kThreads), 0, s, C, ldc,
+                     m, n, beta);
+  return hipGetLastError();
+}
+
 // BLAS xGEMM argument rules (order of the reference's wgemm shim,
 // test/integration/blas/wgemm.f90:16-51) and quick returns (52-69).
 hipError_t gemm_blas_check(char ta, char tb, int64_t m, int64_t n, int64_t k, int64_t lda, int64_t ldb,
@@ -356,11 +363,7 @@ hipError_t gemm_blas(char ta, char tb, int64_t m, int64_t n, int64_t k, ddv alph
   const bool alpha0 = alpha.hi == 0.0 && alpha.lo == 0.0;
   const bool beta1 = beta.hi == 1.0 && beta.lo == 0.0;
   if (m == 0 || n == 0 || ((alpha0 || k == 0) && beta1)) return hipSuccess;
-  if (alpha0 || k == 0) {
-    hipLaunchKernelGGL(k_scale_c, dim3((unsigned)((m * n + kThreads - 1) / kThreads)), dim3(kThreads), 0, s, C,
-                       ldc, m, n, beta);
-    return hipGetLastError();
-  }
+  if (alpha0 || k == 0) return scale_matrix(C, ldc, m, n, beta, s);
   if (nota && notb) return blas_by_size<false, false>(A, lda, B, ldb, C, ldc, m, k, n, ri, alpha, beta, s);
   if (nota) return blas_by_size<false, true>(A, lda, B, ldb, C, ldc, m, k, n, ri, alpha, beta, s);
   if (notb) return blas_by_size<true, false>(A, lda, B, ldb, C, ldc, m, k, n, ri, alpha, beta, s);

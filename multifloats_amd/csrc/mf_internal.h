@@ -25,6 +25,15 @@ hipError_t gemm_plain(const double2 *A, const double2 *B, double2 *C, int64_t m,
 // four-real-GEMM composition; also serves mv (n = 1) and vm (m = 1).
 hipError_t gemm_cplx(const double4 *A, const double4 *B, double4 *C, int64_t m, int64_t k,
                      int64_t n, int64_t ri, hipStream_t s);
+// C = beta C on an m x n block with leading dimension ldc (beta == 0: zeros,
+// C not read).
+hipError_t scale_matrix(double2 *C, int64_t ldc, int64_t m, int64_t n, ddv beta, hipStream_t s);
+// xTRSM of wtrsm.f90 (side, uplo, transa, diag; X overwrites B); work holds
+// trsm_workspace_bytes(side, m, n) bytes (pivot masks of side 'L').
+size_t trsm_workspace_bytes(char side, int64_t m, int64_t n);
+hipError_t trsm_check(char side, char uplo, char ta, char diag, int64_t m, int64_t n, int64_t lda, int64_t ldb);
+hipError_t trsm(char side, char uplo, char ta, char diag, int64_t m, int64_t n, ddv alpha, const double2 *A,
+                int64_t lda, double2 *B, int64_t ldb, void *work, size_t work_bytes, hipStream_t st);
 // BLAS argument checks of wgemm.f90:16-51 (transposes, sizes, leading dimensions)
 hipError_t gemm_blas_check(char ta, char tb, int64_t m, int64_t n, int64_t k, int64_t lda, int64_t ldb,
                            int64_t ldc);

@@ -210,6 +210,101 @@ int or_gemm_blas(char ta, char tb, int64_t m, int64_t n, int64_t k, const dd *al
     }
   return 0;
 }
+/* DD comparisons of the Fortran module (fypp:2913-2940): both limbs equal. */
+static inline int o_ne(dd x, dd y) { return x.hi != y.hi || x.lo != y.lo; }
+
+/* Triangular solve op(A) X = alpha B (side 'L') or X op(A) = alpha B (side
+ * 'R'), X overwriting B: the reference's xTRSM shim
+ * test/integration/blas/wtrsm.f90, its argument checks (24-47), quick returns
+ * (48-56) and all eight loop nests (57-206) restated with the DD operators in
+ * the same order, including its skips (b(k,j) == 0, a(k,j) == 0). Returns the
+ * info code (0 = success; B untouched otherwise). */
+int or_trsm(char side, char uplo, char transa, char diag, int64_t m, int64_t n, const dd *alpha, const dd *a,
+            int64_t lda, dd *b, int64_t ldb) {
+#define A_(i, j) a[(i) + (j) * lda]
+#define B_(i, j) b[(i) + (j) * ldb]
+  int lside = side == 'L' || side == 'l', upper = uplo == 'U' || uplo == 'u';
+  int notr = transa == 'N' || transa == 'n', nounit = diag == 'N' || diag == 'n';
+  int64_t nrowa = lside ? m : n;
+  const dd zero = mkdd(0.0, 0.0), one = mkdd(1.0, 0.0);
+  if (!lside && !(side == 'R' || side == 'r')) return 1;
+  if (!upper && !(uplo == 'L' || uplo == 'l')) return 2;
+  if (!notr && !(transa == 'T' || transa == 't' || transa == 'C' || transa == 'c')) return 3;
+  if (!(diag == 'U' || diag == 'u') && !nounit) return 4;
+  if (m < 0) return 5;
+  if (n < 0) return 6;
+  if (lda < (nrowa > 1 ? nrowa : 1)) return 9;
+  if (ldb < (m > 1 ? m : 1)) return 11;
+  if (m == 0 || n == 0) return 0;
+  if (!o_ne(*alpha, zero)) {
+    for (int64_t j = 0; j < n; ++j)
+      for (int64_t i = 0; i < m; ++i) B_(i, j) = zero;
+    return 0;
+  }
+  const int scale = o_ne(*alpha, one);
+  if (lside && notr) { /* wtrsm.f90:58-101 */
+    for (int64_t j = 0; j < n; ++j) {
+      if (scale)
+        for (int64_t i = 0; i < m; ++i) B_(i, j) = o_mul(*alpha, B_(i, j));
+      for (int64_t t = 0; t < m; ++t) {
+        int64_t k = upper ? m - 1 - t : t;
+        if (!o_ne(B_(k, j), zero)) continue;
+        if (nounit) B_(k, j) = o_div(B_(k, j), A_(k, k));
+        if (upper)
+          for (int64_t i = 0; i < k; ++i) B_(i, j) = o_sub(B_(i, j), o_mul(B_(k, j), A_(i, k)));
+        else
+          for (int64_t i = k + 1; i < m; ++i) B_(i, j) = o_sub(B_(i, j), o_mul(B_(k, j), A_(i, k)));
+      }
+    }
+  } else if (lside) { /* wtrsm.f90:102-126 */
+    for (int64_t j = 0; j < n; ++j)
+      for (int64_t t = 0; t < m; ++t) {
+        int64_t i = upper ? t : m - 1 - t;
+        dd temp = o_mul(*alpha, B_(i, j));
+        if (upper)
+          for (int64_t k = 0; k < i; ++k) temp = o_sub(temp, o_mul(A_(k, i), B_(k, j)));
+        else
+          for (int64_t k = i + 1; k < m; ++k) temp = o_sub(temp, o_mul(A_(k, i), B_(k, j)));
+        if (nounit) temp = o_div(temp, A_(i, i));
+        B_(i, j) = temp;
+      }
+  } else if (notr) { /* wtrsm.f90:128-166 */
+    for (int64_t t = 0; t < n; ++t) {
+      int64_t j = upper ? t : n - 1 - t;
+      if (scale)
+        for (int64_t i = 0; i < m; ++i) B_(i, j) = o_mul(*alpha, B_(i, j));
+      int64_t k0 = upper ? 0 : j + 1, k1 = upper ? j : n;
+      for (int64_t k = k0; k < k1; ++k) {
+        if (!o_ne(A_(k, j), zero)) continue;
+        for (int64_t i = 0; i < m; ++i) B_(i, j) = o_sub(B_(i, j), o_mul(A_(k, j), B_(i, k)));
+      }
+      if (nounit) {
+        dd temp = o_div(one, A_(j, j));
+        for (int64_t i = 0; i < m; ++i) B_(i, j) = o_mul(temp, B_(i, j));
+      }
+    }
+  } else { /* wtrsm.f90:167-205 */
+    for (int64_t t = 0; t < n; ++t) {
+      int64_t k = upper ? n - 1 - t : t;
+      if (nounit) {
+        dd temp = o_div(one, A_(k, k));
+        for (int64_t i = 0; i < m; ++i) B_(i, k) = o_mul(temp, B_(i, k));
+      }
+      int64_t j0 = upper ? 0 : k + 1, j1 = upper ? k : n;
+      for (int64_t j = j0; j < j1; ++j) {
+        if (!o_ne(A_(j, k), zero)) continue;
+        dd temp = A_(j, k);
+        for (int64_t i = 0; i < m; ++i) B_(i, j) = o_sub(B_(i, j), o_mul(temp, B_(i, k)));
+      }
+      if (scale)
+        for (int64_t i = 0; i < m; ++i) B_(i, k) = o_mul(*alpha, B_(i, k));
+    }
+  }
+  return 0;
+#undef A_
+#undef B_
+}
+
 /* fsrc/multifloats.fypp:2609-2621 (cdd_matmul_mm): four real GEMMs, then the
  * native DD - and + (bit-identical to the C++ operators, fypp:2772-2790).
  * Complex arrays are interleaved {re.hi, re.lo, im.hi, im.lo}. */

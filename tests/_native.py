@@ -51,6 +51,9 @@ def oracle():
         L.or_gemm_blas.argtypes = [ctypes.c_char, ctypes.c_char, I64, I64, I64, P, P, I64, P, I64, P, P, I64,
                                    I64]
         L.or_gemm_blas.restype = ctypes.c_int
+        c1 = ctypes.c_char
+        L.or_trsm.argtypes = [c1, c1, c1, c1, I64, I64, P, P, I64, P, I64]
+        L.or_trsm.restype = ctypes.c_int
         D = ctypes.POINTER(ctypes.c_double)
         L.or_exact_dot_err.argtypes = [P, I64, P, I64, I64, P, D, D]
         L.or_exact_dot.argtypes = [P, I64, P, I64, I64, P]
@@ -116,6 +119,18 @@ def or_gemm_blas(ta, tb, m, n, k, alpha, A, B, beta, C, ri, lda=None, ldb=None, 
     ldc = max(C.shape[0], 1) if ldc is None else ldc
     return oracle().or_gemm_blas(ta.encode(), tb.encode(), m, n, k, ptr(al), mptr(A), lda, mptr(B), ldb, ptr(be),
                                  mptr(C), ldc, ri)
+
+
+def or_trsm(side, uplo, ta, diag, m, n, alpha, A, B, lda=None, ldb=None):
+    """The oracle's xTRSM on column-major storage arrays (ld, cols, 2); B is
+    overwritten with X. Returns wgemm-style info (wtrsm.f90:24-47)."""
+    assert datagen.is_colmajor(A) and datagen.is_colmajor(B)
+    al = np.array(alpha, dtype=np.float64).reshape(-1)
+    al = np.array([al[0], al[1] if al.size > 1 else 0.0])
+    lda = max(A.shape[0], 1) if lda is None else lda
+    ldb = max(B.shape[0], 1) if ldb is None else ldb
+    return oracle().or_trsm(side.encode(), uplo.encode(), ta.encode(), diag.encode(), m, n, ptr(al), mptr(A), lda,
+                            mptr(B), ldb)
 
 
 def res(r):
