@@ -12,7 +12,7 @@ Prints ONE JSON line: value = Ozaki DD-GFLOP/s (2*m*n*k per step) when its
 measured error on a sampled block is within 2^-100 (|A||B|) elementwise, with
 the plain form, the error figures and the speed-up in the same line.
 
-  python bench.py [--steps K] [--warmup W] [--workload gemm|cgemm|blas|batched|elementwise|reduce]
+  python bench.py [--steps K] [--warmup W] [--workload gemm|cgemm|blas|trsm|batched|elementwise|reduce]
                   [--no-cpu] [--variant V]
 
 Timing: torch.cuda events recorded on the stream the kernels are launched on
@@ -55,7 +55,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=2)
-    p.add_argument("--workload", default="gemm", choices=["gemm", "cgemm", "blas", "batched", "elementwise", "reduce"])
+    p.add_argument("--workload", default="gemm", choices=["gemm", "cgemm", "blas", "trsm", "batched", "elementwise", "reduce"])
     p.add_argument("--retune", action="store_true", help="batched: rerun the tile autotuner first")
     p.add_argument("--size", type=int, default=4096)
     p.add_argument("--spread", type=float, default=20.0)
@@ -267,6 +267,54 @@ def bench_blas(args, torch, mf):
             "vs_baseline": None}
 
 
+FP64_FLOPS_PER_TRSM_UPDATE = 29  # x - c*y: DD mul 9 + DD sub 20
+
+
+def bench_trsm(args, torch, mf):
+    """Triangular solves B <- op(A)^-1 alpha B / alpha B op(A)^-1, m = n = size,
+    all eight shim variants (non-unit diagonal). The six blocked variants run at
+    the full size; the two whose sums run against the pivot order (L/L/T,
+    R/L/N) are one dependent chain per vector and run at size / 4."""
+    from multifloats_amd import datagen
+    size = args.size
+    rng = np.random.default_rng(9)
+    stream = torch.cuda.current_stream().cuda_stream
+    per, blocked = {}, []
+    for side, uplo, ta in [(a, b, c) for a in "LR" for b in "UL" for c in "NT"]:
+        chain = uplo == "L" and ((side == "L") == (ta == "T"))
+        s = size // 4 if chain else size
+        A = datagen.matrix(rng, s, s)
+        A[np.arange(s), np.arange(s)] += np.array([4.0, 0.0])  # well-conditioned diagonal
+        dA = torch.from_numpy(np.ascontiguousarray(A.transpose(1, 0, 2))).cuda()
+        dB = torch.from_numpy(np.ascontiguousarray(datagen.matrix(rng, s, s).transpose(1, 0, 2))).cuda()
+        wb = mf.trsm_workspace_bytes(side, s, s)
+        work = torch.empty(wb, dtype=torch.uint8, device="cuda")
+        # every step solves the same system again (B is restored from a copy first)
+        dB0 = dB.clone()
+
+        def step():
+            dB.copy_(dB0)
+            mf.dev_trsm(side, uplo, ta, "N", s, s, 1.5, dA.data_ptr(), s, dB.data_ptr(), s, work.data_ptr(), wb,
+                        stream)
+        copy_ms = timed(torch, lambda: dB.copy_(dB0), args.steps, args.warmup)
+        ms = timed(torch, step, args.steps, args.warmup) - copy_ms
+        updates = s * (s - 1) / 2.0 * s
+        rec = {"size": s, "ms": round(ms, 4), "dd_gflops": round(2.0 * updates / (ms * 1e-3) / 1e9, 2)}
+        per[side + uplo + ta] = rec
+        if not chain:
+            blocked.append(rec)
+    worst = min(blocked, key=lambda r: r["dd_gflops"])
+    achieved = FP64_FLOPS_PER_TRSM_UPDATE * worst["dd_gflops"] / 2.0 / 1e3
+    return {"metric": "DD-TRSM DD-GFLOP/s (slowest blocked variant)", "value": worst["dd_gflops"],
+            "unit": "DD-GFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": worst["ms"], "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "float64x2 triangular solve m=n=%d (chain variants %d), alpha 1.5, non-unit, "
+                                   "bit-identical to the xTRSM shim" % (size, size // 4), "per_variant": per},
+            "roofline": {"bound": "fp64", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None},
+            "vs_baseline": None}
+
+
 def bench_batched(args, torch, mf):
     """10,000 float64x2 GEMMs (m, n, k log-uniform in [8, 512]) and 50,000
     dot products (length log-uniform in [16, 65536]), config-0 entries, in one
@@ -420,7 +468,7 @@ def main():
     import multifloats_amd as mf
     if not torch.cuda.is_available():
         sys.exit("bench.py: no GPU visible")
-    fn = {"gemm": bench_gemm, "cgemm": bench_cgemm, "blas": bench_blas, "batched": bench_batched,
+    fn = {"gemm": bench_gemm, "cgemm": bench_cgemm, "blas": bench_blas, "trsm": bench_trsm, "batched": bench_batched,
           "elementwise": bench_elementwise, "reduce": bench_reduce}[args.workload]
     print(json.dumps(fn(args, torch, mf)), flush=True)
 

@@ -41,7 +41,14 @@ enum Fin { FIN_NONE = 0, FIN_DIV = 1, FIN_RECIP = 2 };
 
 __device__ __forceinline__ bool dd_nonzero(ddv x) { return x.hi != 0.0 || x.lo != 0.0; }
 
-__device__ __forceinline__ ddv shfl_dd(ddv x, int lane) { return mk(__shfl(x.hi, lane), __shfl(x.lo, lane)); }
+// x from a wavefront-uniform lane (v_readlane: no LDS round trip)
+__device__ __forceinline__ double readlane_d(double x, int lane) {
+  const long long b = __double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, lane);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ ddv shfl_dd(ddv x, int lane) { return mk(readlane_d(x.hi, lane), readlane_d(x.lo, lane)); }
 
 // Actual index of block position q (pivot order) within [base, base + tb).
 template <bool DESC>
@@ -212,11 +219,16 @@ __global__ __launch_bounds__(kThreads) void k_trsm_update(const double2 *__restr
 //       x_j -= a(k,j) x_k for k = j+1..s-1 ascending with a(k,j) != 0, then
 //       x_j = (1 / a(j,j)) x_j; j descending.
 // Both read coefficients a(k, t), k > t (uniform across the wavefront).
+// The products do not depend on the chain, so each batch of kChainU terms is
+// loaded and multiplied ahead of its dependent subtractions.
+constexpr int kChainThreads = 64;
+constexpr int kChainU = 8;
+
 template <bool R, bool NOUNIT>
-__global__ __launch_bounds__(kThreads) void k_trsm_chain(const double2 *__restrict__ A, int64_t lda,
-                                                         double2 *__restrict__ B, int64_t ldb, int64_t nvec,
-                                                         int64_t s, ddv alpha, bool scale) {
-  const int64_t v = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+__global__ __launch_bounds__(kChainThreads) void k_trsm_chain(const double2 *__restrict__ A, int64_t lda,
+                                                              double2 *__restrict__ B, int64_t ldb, int64_t nvec,
+                                                              int64_t s, ddv alpha, bool scale) {
+  const int64_t v = (int64_t)blockIdx.x * kChainThreads + threadIdx.x;
   if (v >= nvec) return;
   auto xp = [&](int64_t t) -> double2 * { return R ? B + v + t * ldb : B + t + v * ldb; };
   for (int64_t t = s - 1; t >= 0; --t) {
@@ -224,7 +236,22 @@ __global__ __launch_bounds__(kThreads) void k_trsm_chain(const double2 *__restri
     ddv x = mk(b0.x, b0.y);
     if (!R || scale) x = dd_mul(alpha, x);
     const double2 *col = A + t * lda;  // a(k, t)
-    for (int64_t k = t + 1; k < s; ++k) {
+    int64_t k = t + 1;
+    for (; k + kChainU <= s; k += kChainU) {
+      double2 c[kChainU], xv[kChainU];
+#pragma unroll
+      for (int u = 0; u < kChainU; ++u) {
+        c[u] = col[k + u];
+        xv[u] = *xp(k + u);
+      }
+      ddv pr[kChainU];
+#pragma unroll
+      for (int u = 0; u < kChainU; ++u) pr[u] = dd_mul(mk(c[u].x, c[u].y), mk(xv[u].x, xv[u].y));
+#pragma unroll
+      for (int u = 0; u < kChainU; ++u)
+        if (!R || dd_nonzero(mk(c[u].x, c[u].y))) x = dd_sub(x, pr[u]);
+    }
+    for (; k < s; ++k) {
       const double2 c = col[k], xk = *xp(k);
       const ddv cd = mk(c.x, c.y);
       if (!R || dd_nonzero(cd)) x = dd_sub(x, dd_mul(cd, mk(xk.x, xk.y)));
@@ -298,10 +325,10 @@ hipError_t trsm(char side, char uplo, char ta, char diag, int64_t m, int64_t n, 
   uint64_t *mask = (uint64_t *)work;
   if (!upper && (R != T)) {  // L/T/L and R/N/L: per-vector chains
     const int64_t nvec = R ? m : n, s = R ? n : m;
-    const dim3 grid((unsigned)((nvec + kThreads - 1) / kThreads));
+    const dim3 grid((unsigned)((nvec + kChainThreads - 1) / kChainThreads));
     auto kern = R ? (unit ? k_trsm_chain<true, false> : k_trsm_chain<true, true>)
                   : (unit ? k_trsm_chain<false, false> : k_trsm_chain<false, true>);
-    hipLaunchKernelGGL(kern, grid, dim3(kThreads), 0, st, A, lda, B, ldb, nvec, s, alpha, scale);
+    hipLaunchKernelGGL(kern, grid, dim3(kChainThreads), 0, st, A, lda, B, ldb, nvec, s, alpha, scale);
     return hipGetLastError();
   }
   if (!R && !T && work_bytes < trsm_workspace_bytes(side, m, n)) return hipErrorInvalidValue;
