@@ -122,6 +122,21 @@ MULTIFLOATS_API void sqrtdd_v(const float64x2 *a, float64x2 *c, int64_t n);
 MULTIFLOATS_API void expdd_v(const float64x2 *a, float64x2 *c, int64_t n);
 MULTIFLOATS_API void logdd_v(const float64x2 *a, float64x2 *c, int64_t n);
 MULTIFLOATS_API void sindd_v(const float64x2 *a, float64x2 *c, int64_t n);
+/* Array forms of cosdd, tandd, exp2dd, expm1dd, log2dd, log10dd, log1pdd,
+ * sinhdd, coshdd, tanhdd (float64x2.h:455-517; src/float64x2/trig.inc:255-289,
+ * exp_log.inc:109-128, 311-353, 407-469, hyp.inc:2-87), bit-identical to them
+ * element by element (the Fortran generics cos, tan, expm1, log2, log10,
+ * log1p, sinh, cosh, tanh are elemental loops over these, fypp:57). */
+MULTIFLOATS_API void cosdd_v(const float64x2 *a, float64x2 *c, int64_t n);
+MULTIFLOATS_API void tandd_v(const float64x2 *a, float64x2 *c, int64_t n);
+MULTIFLOATS_API void exp2dd_v(const float64x2 *a, float64x2 *c, int64_t n);
+MULTIFLOATS_API void expm1dd_v(const float64x2 *a, float64x2 *c, int64_t n);
+MULTIFLOATS_API void log2dd_v(const float64x2 *a, float64x2 *c, int64_t n);
+MULTIFLOATS_API void log10dd_v(const float64x2 *a, float64x2 *c, int64_t n);
+MULTIFLOATS_API void log1pdd_v(const float64x2 *a, float64x2 *c, int64_t n);
+MULTIFLOATS_API void sinhdd_v(const float64x2 *a, float64x2 *c, int64_t n);
+MULTIFLOATS_API void coshdd_v(const float64x2 *a, float64x2 *c, int64_t n);
+MULTIFLOATS_API void tanhdd_v(const float64x2 *a, float64x2 *c, int64_t n);
 /* Array form of erfdd (float64x2.h:525; src/float64x2/special.inc:18-51): own
  * piecewise-Taylor algorithm held to the reference's max relative error. */
 MULTIFLOATS_API void erfdd_v(const float64x2 *a, float64x2 *c, int64_t n);
@@ -195,7 +210,9 @@ MULTIFLOATS_API void matmuldd_mm_ozaki(const float64x2 *a, const float64x2 *b, f
  * hipError_t value (see mfhip_error_string). Work is only enqueued. */
 
 enum {
-  MFHIP_SQRT = 0, MFHIP_EXP = 1, MFHIP_LOG = 2, MFHIP_SIN = 3, MFHIP_ERF = 4, MFHIP_NEG = 5
+  MFHIP_SQRT = 0, MFHIP_EXP = 1, MFHIP_LOG = 2, MFHIP_SIN = 3, MFHIP_ERF = 4, MFHIP_NEG = 5,
+  MFHIP_COS = 6, MFHIP_TAN = 7, MFHIP_EXP2 = 8, MFHIP_EXPM1 = 9, MFHIP_LOG2 = 10,
+  MFHIP_LOG10 = 11, MFHIP_LOG1P = 12, MFHIP_SINH = 13, MFHIP_COSH = 14, MFHIP_TANH = 15
 };
 enum { MFHIP_ADD = 0, MFHIP_SUB = 1, MFHIP_MUL = 2, MFHIP_DIV = 3 };
 enum { MFHIP_DOT = 0, MFHIP_SUM = 1, MFHIP_NORM2 = 2 };

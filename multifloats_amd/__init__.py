@@ -32,6 +32,8 @@ LIB_PATH = os.path.join(_HERE, "libmultifloats_hip.so")
 DD_FMA_RENORM_INTERVAL = 8
 
 MFHIP_SQRT, MFHIP_EXP, MFHIP_LOG, MFHIP_SIN, MFHIP_ERF, MFHIP_NEG = range(6)
+(MFHIP_COS, MFHIP_TAN, MFHIP_EXP2, MFHIP_EXPM1, MFHIP_LOG2, MFHIP_LOG10, MFHIP_LOG1P, MFHIP_SINH, MFHIP_COSH,
+ MFHIP_TANH) = range(6, 16)
 MFHIP_ADD, MFHIP_SUB, MFHIP_MUL, MFHIP_DIV = range(4)
 MFHIP_DOT, MFHIP_SUM, MFHIP_NORM2 = range(3)
 
@@ -45,6 +47,7 @@ class complex64x2(ctypes.Structure):
 
 
 _lib = None
+_TRANSC2 = ("cos", "tan", "exp2", "expm1", "log2", "log10", "log1p", "sinh", "cosh", "tanh")
 
 
 def lib():
@@ -64,7 +67,7 @@ def lib():
         for nm in ("adddd_v", "subdd_v", "muldd_v", "divdd_v"):
             getattr(L, nm).argtypes = [P, P, P, I64]
             getattr(L, nm).restype = None
-        for nm in ("sqrtdd_v", "expdd_v", "logdd_v", "sindd_v", "erfdd_v"):
+        for nm in ("sqrtdd_v", "expdd_v", "logdd_v", "sindd_v", "erfdd_v") + tuple(f + "dd_v" for f in _TRANSC2):
             getattr(L, nm).argtypes = [P, P, I64]
             getattr(L, nm).restype = None
         L.sumdd.argtypes = [P, I64]
@@ -336,6 +339,16 @@ def log(a): return _unary("logdd_v", a)
 def sin(a): return _unary("sindd_v", a)
 def erf(a): return _unary("erfdd_v", a)
 def sqrt(a): return _unary("sqrtdd_v", a)
+def cos(a): return _unary("cosdd_v", a)
+def tan(a): return _unary("tandd_v", a)
+def exp2(a): return _unary("exp2dd_v", a)
+def expm1(a): return _unary("expm1dd_v", a)
+def log2(a): return _unary("log2dd_v", a)
+def log10(a): return _unary("log10dd_v", a)
+def log1p(a): return _unary("log1pdd_v", a)
+def sinh(a): return _unary("sinhdd_v", a)
+def cosh(a): return _unary("coshdd_v", a)
+def tanh(a): return _unary("tanhdd_v", a)
 def add(a, b): return _binary("adddd_v", a, b)
 def sub(a, b): return _binary("subdd_v", a, b)
 def mul(a, b): return _binary("muldd_v", a, b)

@@ -437,6 +437,16 @@ MF_UNARY(expdd_v, mfhip::OP_EXP)
 MF_UNARY(logdd_v, mfhip::OP_LOG)
 MF_UNARY(sindd_v, mfhip::OP_SIN)
 MF_UNARY(erfdd_v, mfhip::OP_ERF)
+MF_UNARY(cosdd_v, mfhip::OP_COS)
+MF_UNARY(tandd_v, mfhip::OP_TAN)
+MF_UNARY(exp2dd_v, mfhip::OP_EXP2)
+MF_UNARY(expm1dd_v, mfhip::OP_EXPM1)
+MF_UNARY(log2dd_v, mfhip::OP_LOG2)
+MF_UNARY(log10dd_v, mfhip::OP_LOG10)
+MF_UNARY(log1pdd_v, mfhip::OP_LOG1P)
+MF_UNARY(sinhdd_v, mfhip::OP_SINH)
+MF_UNARY(coshdd_v, mfhip::OP_COSH)
+MF_UNARY(tanhdd_v, mfhip::OP_TANH)
 
 float64x2 sumdd(const float64x2 *a, int64_t n) {
   float64x2 r;

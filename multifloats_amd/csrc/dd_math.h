@@ -1,13 +1,17 @@
-// DD transcendentals for the elementwise kernels: exp, log, sin, erf.
+// DD transcendentals for the elementwise kernels.
 //
-// exp, log and sin keep the reference's algorithms step for step (the same
-// range reductions, table sizes, polynomial degrees and evaluation trees), so
-// their results are bit-identical to the reference CPU functions:
-//   dd_exp  src/float64x2/exp_log.inc:237-277 with exp2_from_reduced 26-72
-//   dd_log  exp_log.inc:329-341, log2 311-327, log2_kernel 280-308
-//   dd_sin  src/float64x2/trig.inc:241-253, reduce_pi_half 114-147,
-//           sin_eval/cos_eval 165-187, kernels 151-160
-//   estrin10 / estrin12 / horner7   src/float64x2/poly.inc:9-16, 69-104
+// exp, exp2, expm1, log, log2, log10, log1p, sin, cos, tan, sinh, cosh and
+// tanh keep the reference's algorithms step for step (the same range
+// reductions, table sizes, polynomial degrees, evaluation trees and special
+// cases), so their results are bit-identical to the reference CPU functions:
+//   dd_exp   src/float64x2/exp_log.inc:237-277 with exp2_from_reduced 26-72
+//   dd_exp2  exp_log.inc:109-128;  dd_expm1 407-427
+//   dd_log   exp_log.inc:329-341, log2 311-327, log2_kernel 280-308
+//   dd_log10 exp_log.inc:343-353;  dd_log1p 429-469
+//   dd_sin / dd_cos / dd_tan  src/float64x2/trig.inc:241-289,
+//           reduce_pi_half 114-147, sin_eval/cos_eval/sincos_eval 165-215
+//   dd_sinh / dd_cosh / dd_tanh  src/float64x2/hyp.inc:2-87
+//   estrin10/12/17/24  src/float64x2/poly.inc:33-217 (neval's trees)
 // The constant tables are closed forms produced by gen/gen_tables.py.
 //
 // dd_erf is this project's own design, not the reference's rational fits
@@ -53,6 +57,51 @@ MF_HD ddv estrin12(ddv x, const double *hi, const double *lo) {
   ddv p811 = dd_add(p89, dd_mul(p1011, x2));
   ddv top = dd_add(p811, dd_mul(cdd(hi, lo, 12), x4));
   return dd_add(dd_add(p03, dd_mul(p47, x4)), dd_mul(top, x8));
+}
+
+// c0 + ... + c15 x^15 by the degree-15 tree (shared by the 17 and 24 cases).
+MF_HD ddv estrin_0_15(ddv x, ddv x2, ddv x4, ddv x8, const double *hi, const double *lo) {
+  ddv p01 = dd_add(cdd(hi, lo, 0), dd_mul(cdd(hi, lo, 1), x));
+  ddv p23 = dd_add(cdd(hi, lo, 2), dd_mul(cdd(hi, lo, 3), x));
+  ddv p45 = dd_add(cdd(hi, lo, 4), dd_mul(cdd(hi, lo, 5), x));
+  ddv p67 = dd_add(cdd(hi, lo, 6), dd_mul(cdd(hi, lo, 7), x));
+  ddv p89 = dd_add(cdd(hi, lo, 8), dd_mul(cdd(hi, lo, 9), x));
+  ddv p1011 = dd_add(cdd(hi, lo, 10), dd_mul(cdd(hi, lo, 11), x));
+  ddv p1213 = dd_add(cdd(hi, lo, 12), dd_mul(cdd(hi, lo, 13), x));
+  ddv p1415 = dd_add(cdd(hi, lo, 14), dd_mul(cdd(hi, lo, 15), x));
+  ddv p03 = dd_add(p01, dd_mul(p23, x2));
+  ddv p47 = dd_add(p45, dd_mul(p67, x2));
+  ddv p811 = dd_add(p89, dd_mul(p1011, x2));
+  ddv p1215 = dd_add(p1213, dd_mul(p1415, x2));
+  return dd_add(dd_add(p03, dd_mul(p47, x4)), dd_mul(dd_add(p811, dd_mul(p1215, x4)), x8));
+}
+
+// Degree 17: the degree-15 tree plus (c16 + c17 x) x^16 (poly.inc:174-191).
+MF_HD ddv estrin17(ddv x, const double *hi, const double *lo) {
+  ddv x2 = dd_mul(x, x);
+  ddv x4 = dd_mul(x2, x2);
+  ddv x8 = dd_mul(x4, x4);
+  ddv x16 = dd_mul(x8, x8);
+  ddv p = estrin_0_15(x, x2, x4, x8, hi, lo);
+  ddv p1617 = dd_add(cdd(hi, lo, 16), dd_mul(cdd(hi, lo, 17), x));
+  return dd_add(p, dd_mul(p1617, x16));
+}
+
+// Degree 24: the degree-15 tree plus a degree-8 tree times x^16 (poly.inc:192-217).
+MF_HD ddv estrin24(ddv x, const double *hi, const double *lo) {
+  ddv x2 = dd_mul(x, x);
+  ddv x4 = dd_mul(x2, x2);
+  ddv x8 = dd_mul(x4, x4);
+  ddv x16 = dd_mul(x8, x8);
+  ddv p = estrin_0_15(x, x2, x4, x8, hi, lo);
+  ddv p1617 = dd_add(cdd(hi, lo, 16), dd_mul(cdd(hi, lo, 17), x));
+  ddv p1819 = dd_add(cdd(hi, lo, 18), dd_mul(cdd(hi, lo, 19), x));
+  ddv p2021 = dd_add(cdd(hi, lo, 20), dd_mul(cdd(hi, lo, 21), x));
+  ddv p2223 = dd_add(cdd(hi, lo, 22), dd_mul(cdd(hi, lo, 23), x));
+  ddv p1619 = dd_add(p1617, dd_mul(p1819, x2));
+  ddv p2023 = dd_add(p2021, dd_mul(p2223, x2));
+  ddv q = dd_add(dd_add(p1619, dd_mul(p2023, x4)), dd_mul(cdd(hi, lo, 24), x8));
+  return dd_add(p, dd_mul(q, x16));
 }
 
 // ---------------------------------------------------------------- exp
@@ -103,6 +152,27 @@ MF_HD ddv dd_exp(ddv x) {
   return exp2_reduced(n_d, y.hi, y.lo);
 }
 
+// exp_log.inc:119-128 (exp2) with exp2_kernel 109-116
+MF_HD ddv dd_exp2(ddv x) {
+  if (!mf_isfinite(x.hi)) return mk(x.hi > 0.0 ? x.hi : (x.hi < 0.0 ? 0.0 : x.hi), 0.0);
+  if (x.hi < mf_exp2_min) return mk(0.0, 0.0);
+  if (x.hi > mf_exp2_max) return mk(__builtin_inf(), 0.0);
+  double n_d = __builtin_rint(x.hi);
+  double rh, rl, t;
+  two_sum(x.hi, -n_d, rh, t);
+  rl = t + x.lo;
+  fast_two_sum(rh, rl, rh, rl);
+  return exp2_reduced(n_d, rh, rl);
+}
+
+// exp_log.inc:407-427: exp(x) - 1 for |x| >= 1/2, else x * P(x), P of degree 24
+MF_HD ddv dd_expm1(ddv x) {
+  if (x.hi == 0.0 && x.lo == 0.0) return x;
+  if (!mf_isfinite(x.hi)) return mk(x.hi > 0.0 ? x.hi : (x.hi < 0.0 ? -1.0 : x.hi), 0.0);
+  if (__builtin_fabs(x.hi) >= 0.5) return dd_sub(dd_exp(x), mk(1.0, 0.0));
+  return dd_mul(x, estrin24(x, mf_expm1c_hi, mf_expm1c_lo));
+}
+
 // ---------------------------------------------------------------- log
 MF_HD ddv log2_kernel(ddv x) {
   double hi = x.hi;
@@ -137,6 +207,68 @@ MF_HD ddv dd_log(ddv x) {
   l2 = log2_kernel(x);
   if (!mf_isfinite(l2.hi)) return l2;
   return dd_mul(l2, mk(mf_ln2_hi, mf_ln2_lo));
+}
+
+// exp_log.inc:311-327
+MF_HD ddv dd_log2(ddv x) {
+  if (x.hi != x.hi || x.hi < 0.0) return mk(__builtin_nan(""), 0.0);
+  if (x.hi == 0.0) return mk(-__builtin_inf(), 0.0);
+  if (!mf_isfinite(x.hi)) return mk(__builtin_inf(), 0.0);
+  return log2_kernel(x);
+}
+
+// exp_log.inc:343-353: log2(x) * log10(2)
+MF_HD ddv dd_log10(ddv x) {
+  if (!mf_isfinite(x.hi)) return mk(x.hi > 0.0 ? x.hi : __builtin_nan(""), 0.0);
+  ddv l2 = dd_log2(x);
+  if (!mf_isfinite(l2.hi)) return l2;
+  return dd_mul(l2, mk(mf_log10_2_hi, mf_log10_2_lo));
+}
+
+// exp_log.inc:429-469: log(1 + x) for |x| >= 1/4, else 2 atanh(x / (2 + x))
+// as t * P(t^2), P of degree 17
+MF_HD ddv dd_log1p(ddv x) {
+  if (x.hi == 0.0 && x.lo == 0.0) return x;
+  if (!mf_isfinite(x.hi)) return mk(x.hi > 0.0 ? x.hi : __builtin_nan(""), 0.0);
+  ddv arg = dd_add(mk(1.0, 0.0), x);
+  if (arg.hi <= 0.0) {
+    if (arg.hi == 0.0 && arg.lo == 0.0) return mk(-__builtin_inf(), 0.0);
+    if (arg.hi < 0.0 || (arg.hi == 0.0 && arg.lo < 0.0)) return mk(__builtin_nan(""), 0.0);
+    return dd_log(mk(arg.lo, 0.0));  // hi == 0 < lo: the value is the lo limb alone
+  }
+  if (__builtin_fabs(x.hi) >= 0.25) return dd_log(arg);
+  ddv t = dd_div(x, dd_add(mk(2.0, 0.0), x));
+  return dd_mul(t, estrin17(dd_mul(t, t), mf_log1pc_hi, mf_log1pc_lo));
+}
+
+// ---------------------------------------------------------------- sinh, cosh, tanh
+// src/float64x2/hyp.inc:2-18: x * P(x^2) for |x| < 1/2, else (e^x - e^-x) / 2
+MF_HD ddv dd_sinh(ddv x) {
+  if (x.hi == 0.0 && x.lo == 0.0) return x;
+  if (!mf_isfinite(x.hi)) return mk(x.hi, 0.0);  // sinh(+-inf) = +-inf, NaN stays
+  if (__builtin_fabs(x.hi) < 0.5) return dd_mul(x, estrin12(dd_mul(x, x), mf_sinhc_hi, mf_sinhc_lo));
+  return dd_mul(dd_sub(dd_exp(x), dd_exp(dd_neg(x))), mk(0.5, 0.0));
+}
+
+// hyp.inc:20-29: (e^x + e^-x) / 2
+MF_HD ddv dd_cosh(ddv x) {
+  if (!mf_isfinite(x.hi)) return mk(x.hi == x.hi ? __builtin_inf() : x.hi, 0.0);
+  return dd_mul(dd_add(dd_exp(x), dd_exp(dd_neg(x))), mk(0.5, 0.0));
+}
+
+// hyp.inc:62-87: +-1 beyond |x| >= 38, sinh/cosh below 1/2, else
+// (1 - e^-2|x|) / (1 + e^-2|x|) with the sign restored
+MF_HD ddv dd_tanh(ddv x) {
+  if (x.hi == 0.0 && x.lo == 0.0) return x;
+  if (!mf_isfinite(x.hi)) return mk(x.hi == x.hi ? (x.hi > 0.0 ? 1.0 : -1.0) : x.hi, 0.0);
+  if (x.hi >= 38.0) return mk(1.0, 0.0);
+  if (x.hi <= -38.0) return mk(-1.0, 0.0);
+  if (__builtin_fabs(x.hi) < 0.5) return dd_div(dd_sinh(x), dd_cosh(x));
+  const bool sign = x.hi < 0.0;
+  ddv m2 = sign ? mk(2.0 * x.hi, 2.0 * x.lo) : mk(-2.0 * x.hi, -2.0 * x.lo);
+  ddv em2 = dd_exp(m2);
+  ddv r = dd_div(dd_sub(mk(1.0, 0.0), em2), dd_add(mk(1.0, 0.0), em2));
+  return sign ? dd_neg(r) : r;
 }
 
 // ---------------------------------------------------------------- sin
@@ -195,6 +327,39 @@ MF_HD ddv dd_sin(ddv x) {
   reduce_pio2(x, r, q);
   ddv v = sincos_eval(r, q & 1);
   return (q & 2) ? dd_neg(v) : v;
+}
+
+// trig.inc:255-266
+MF_HD ddv dd_cos(ddv x) {
+  if (!mf_isfinite(x.hi) || !(__builtin_fabs(x.hi) < 0x1p55)) return mk(__builtin_nan(""), 0.0);
+  ddv r;
+  int q;
+  reduce_pio2(x, r, q);
+  ddv v = sincos_eval(r, !(q & 1));
+  return (q == 1 || q == 2) ? dd_neg(v) : v;
+}
+
+// trig.inc:268-289 with sincos_eval 192-215: s / c, or -c / s in odd quadrants
+MF_HD ddv dd_tan(ddv x) {
+  if (x.hi == 0.0 && x.lo == 0.0) return x;
+  if (!mf_isfinite(x.hi) || !(__builtin_fabs(x.hi) < 0x1p55)) return mk(__builtin_nan(""), 0.0);
+  ddv r;
+  int q;
+  reduce_pio2(x, r, q);
+  const double pi8 = 0.392699081698724;
+  const bool small = __builtin_fabs(r.hi) <= pi8;
+  const bool pos = r.hi > 0.0;
+  ddv pi4 = mk(mf_pi_quarter_hi, mf_pi_quarter_lo);
+  ddv rp = small ? r : (pos ? dd_sub(r, pi4) : dd_add(r, pi4));
+  ddv sk = sin_kernel(rp);
+  ddv ck = cos_kernel(rp);
+  ddv s = sk, c = ck;
+  if (!small) {
+    ddv is2 = mk(mf_inv_sqrt2_hi, mf_inv_sqrt2_lo);
+    s = dd_mul(pos ? dd_add(sk, ck) : dd_sub(sk, ck), is2);
+    c = dd_mul(pos ? dd_sub(ck, sk) : dd_add(ck, sk), is2);
+  }
+  return (q & 1) ? dd_div(dd_neg(c), s) : dd_div(s, c);
 }
 
 // ---------------------------------------------------------------- erf

@@ -27,6 +27,16 @@ __device__ __forceinline__ ddv unary_op(ddv x) {
   else if constexpr (OP == mfhip::OP_SIN) return dd_sin(x);
   else if constexpr (OP == mfhip::OP_ERF) return dd_erf(x);
   else if constexpr (OP == mfhip::OP_NEG) return dd_neg(x);
+  else if constexpr (OP == mfhip::OP_COS) return dd_cos(x);
+  else if constexpr (OP == mfhip::OP_TAN) return dd_tan(x);
+  else if constexpr (OP == mfhip::OP_EXP2) return dd_exp2(x);
+  else if constexpr (OP == mfhip::OP_EXPM1) return dd_expm1(x);
+  else if constexpr (OP == mfhip::OP_LOG2) return dd_log2(x);
+  else if constexpr (OP == mfhip::OP_LOG10) return dd_log10(x);
+  else if constexpr (OP == mfhip::OP_LOG1P) return dd_log1p(x);
+  else if constexpr (OP == mfhip::OP_SINH) return dd_sinh(x);
+  else if constexpr (OP == mfhip::OP_COSH) return dd_cosh(x);
+  else if constexpr (OP == mfhip::OP_TANH) return dd_tanh(x);
   else return x;
 }
 
@@ -156,6 +166,16 @@ hipError_t unary(int op, const double2 *a, double2 *c, int64_t n, hipStream_t s)
     case OP_SIN: return run_unary<OP_SIN>(a, c, n, s);
     case OP_ERF: return run_unary<OP_ERF>(a, c, n, s);
     case OP_NEG: return run_unary<OP_NEG>(a, c, n, s);
+    case OP_COS: return run_unary<OP_COS>(a, c, n, s);
+    case OP_TAN: return run_unary<OP_TAN>(a, c, n, s);
+    case OP_EXP2: return run_unary<OP_EXP2>(a, c, n, s);
+    case OP_EXPM1: return run_unary<OP_EXPM1>(a, c, n, s);
+    case OP_LOG2: return run_unary<OP_LOG2>(a, c, n, s);
+    case OP_LOG10: return run_unary<OP_LOG10>(a, c, n, s);
+    case OP_LOG1P: return run_unary<OP_LOG1P>(a, c, n, s);
+    case OP_SINH: return run_unary<OP_SINH>(a, c, n, s);
+    case OP_COSH: return run_unary<OP_COSH>(a, c, n, s);
+    case OP_TANH: return run_unary<OP_TANH>(a, c, n, s);
     default: return hipErrorInvalidValue;
   }
 }

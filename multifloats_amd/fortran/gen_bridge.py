@@ -1,7 +1,8 @@
 #!/usr/bin/env python3
 """Writes multifloats_hip.F90, the Fortran module that routes the reference's
-array generics (matmul, dot_product, sum, norm2, exp, log, sin, erf, sqrt and
-the array + - * /) to libmultifloats_hip.so through bind(c) interfaces.
+array generics (matmul, dot_product, sum, norm2, the elemental math functions
+exp log sin cos tan expm1 log2 log10 log1p sinh cosh tanh erf sqrt, and the
+array + - * /) to libmultifloats_hip.so through bind(c) interfaces.
 
 The reference generates its module with fypp (fsrc/multifloats.fypp); fypp is
 not part of this toolchain, so the rank 1..7 specifics are written out here.
@@ -12,7 +13,10 @@ import os
 
 MAX_RANK = 7
 HERE = os.path.dirname(os.path.abspath(__file__))
-UNARY = [("exp", "expdd_v"), ("log", "logdd_v"), ("sin", "sindd_v"), ("erf", "erfdd_v"), ("sqrt", "sqrtdd_v")]
+UNARY = [("exp", "expdd_v"), ("log", "logdd_v"), ("sin", "sindd_v"), ("erf", "erfdd_v"), ("sqrt", "sqrtdd_v"),
+         ("cos", "cosdd_v"), ("tan", "tandd_v"), ("expm1", "expm1dd_v"), ("log2", "log2dd_v"),
+         ("log10", "log10dd_v"), ("log1p", "log1pdd_v"), ("sinh", "sinhdd_v"), ("cosh", "coshdd_v"),
+         ("tanh", "tanhdd_v")]
 BINARY = [("+", "add", "adddd_v"), ("-", "sub", "subdd_v"), ("*", "mul", "muldd_v"), ("/", "div", "divdd_v")]
 
 
@@ -45,7 +49,8 @@ def gen():
 !   dot_product     dd_dot_product 2254-2308, cdd_dot_product 2321-2331
 !   sum             dd_sum_Nd / dd_sum_Nd_dim 1980-2090 (with mask)
 !   norm2           dd_norm2_Nd / dd_norm2_Nd_dim 2338-2377
-!   exp log sin erf sqrt   elemental delegates 709-736, 1407-1412
+!   exp log sin erf sqrt cos tan expm1 log2 log10 log1p sinh cosh tanh
+!                   elemental delegates 709-736, 1407-1412 (names fypp:57)
 !   + - * /         elemental native operators 2746-2837
 !   DD_FMA_RENORM_INTERVAL and dd_set_fma_renorm_interval 664, 2677-2680
 !
@@ -92,7 +97,7 @@ module multifloats_hip
     public :: dd_matmul_ozaki
     public :: wgemm
     public :: matmul, dot_product, sum, norm2
-    public :: exp, log, sin, erf, sqrt
+    public :: exp, log, sin, erf, sqrt, cos, tan, expm1, log2, log10, log1p, sinh, cosh, tanh
     public :: operator(+), operator(-), operator(*), operator(/)
 
     interface

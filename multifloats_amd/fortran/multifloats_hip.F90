@@ -11,7 +11,8 @@
 !   dot_product     dd_dot_product 2254-2308, cdd_dot_product 2321-2331
 !   sum             dd_sum_Nd / dd_sum_Nd_dim 1980-2090 (with mask)
 !   norm2           dd_norm2_Nd / dd_norm2_Nd_dim 2338-2377
-!   exp log sin erf sqrt   elemental delegates 709-736, 1407-1412
+!   exp log sin erf sqrt cos tan expm1 log2 log10 log1p sinh cosh tanh
+!                   elemental delegates 709-736, 1407-1412 (names fypp:57)
 !   + - * /         elemental native operators 2746-2837
 !   DD_FMA_RENORM_INTERVAL and dd_set_fma_renorm_interval 664, 2677-2680
 !
@@ -58,7 +59,7 @@ module multifloats_hip
     public :: dd_matmul_ozaki
     public :: wgemm
     public :: matmul, dot_product, sum, norm2
-    public :: exp, log, sin, erf, sqrt
+    public :: exp, log, sin, erf, sqrt, cos, tan, expm1, log2, log10, log1p, sinh, cosh, tanh
     public :: operator(+), operator(-), operator(*), operator(/)
 
     interface
@@ -161,6 +162,51 @@ module multifloats_hip
             integer(c_int64_t), value :: n
         end subroutine
         pure subroutine c_sqrtdd_v(a, c, n) bind(c, name='sqrtdd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_cosdd_v(a, c, n) bind(c, name='cosdd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_tandd_v(a, c, n) bind(c, name='tandd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_expm1dd_v(a, c, n) bind(c, name='expm1dd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_log2dd_v(a, c, n) bind(c, name='log2dd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_log10dd_v(a, c, n) bind(c, name='log10dd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_log1pdd_v(a, c, n) bind(c, name='log1pdd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_sinhdd_v(a, c, n) bind(c, name='sinhdd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_coshdd_v(a, c, n) bind(c, name='coshdd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_tanhdd_v(a, c, n) bind(c, name='tanhdd_v')
             import :: c_ptr, c_int64_t
             type(c_ptr), value :: a, c
             integer(c_int64_t), value :: n
@@ -274,6 +320,96 @@ module multifloats_hip
         module procedure dd_sqrt_5d
         module procedure dd_sqrt_6d
         module procedure dd_sqrt_7d
+    end interface
+
+    interface cos
+        module procedure dd_cos_1d
+        module procedure dd_cos_2d
+        module procedure dd_cos_3d
+        module procedure dd_cos_4d
+        module procedure dd_cos_5d
+        module procedure dd_cos_6d
+        module procedure dd_cos_7d
+    end interface
+
+    interface tan
+        module procedure dd_tan_1d
+        module procedure dd_tan_2d
+        module procedure dd_tan_3d
+        module procedure dd_tan_4d
+        module procedure dd_tan_5d
+        module procedure dd_tan_6d
+        module procedure dd_tan_7d
+    end interface
+
+    interface expm1
+        module procedure dd_expm1_1d
+        module procedure dd_expm1_2d
+        module procedure dd_expm1_3d
+        module procedure dd_expm1_4d
+        module procedure dd_expm1_5d
+        module procedure dd_expm1_6d
+        module procedure dd_expm1_7d
+    end interface
+
+    interface log2
+        module procedure dd_log2_1d
+        module procedure dd_log2_2d
+        module procedure dd_log2_3d
+        module procedure dd_log2_4d
+        module procedure dd_log2_5d
+        module procedure dd_log2_6d
+        module procedure dd_log2_7d
+    end interface
+
+    interface log10
+        module procedure dd_log10_1d
+        module procedure dd_log10_2d
+        module procedure dd_log10_3d
+        module procedure dd_log10_4d
+        module procedure dd_log10_5d
+        module procedure dd_log10_6d
+        module procedure dd_log10_7d
+    end interface
+
+    interface log1p
+        module procedure dd_log1p_1d
+        module procedure dd_log1p_2d
+        module procedure dd_log1p_3d
+        module procedure dd_log1p_4d
+        module procedure dd_log1p_5d
+        module procedure dd_log1p_6d
+        module procedure dd_log1p_7d
+    end interface
+
+    interface sinh
+        module procedure dd_sinh_1d
+        module procedure dd_sinh_2d
+        module procedure dd_sinh_3d
+        module procedure dd_sinh_4d
+        module procedure dd_sinh_5d
+        module procedure dd_sinh_6d
+        module procedure dd_sinh_7d
+    end interface
+
+    interface cosh
+        module procedure dd_cosh_1d
+        module procedure dd_cosh_2d
+        module procedure dd_cosh_3d
+        module procedure dd_cosh_4d
+        module procedure dd_cosh_5d
+        module procedure dd_cosh_6d
+        module procedure dd_cosh_7d
+    end interface
+
+    interface tanh
+        module procedure dd_tanh_1d
+        module procedure dd_tanh_2d
+        module procedure dd_tanh_3d
+        module procedure dd_tanh_4d
+        module procedure dd_tanh_5d
+        module procedure dd_tanh_6d
+        module procedure dd_tanh_7d
     end interface
 
     interface operator(+)
@@ -1007,6 +1143,384 @@ contains
         type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:,:)
         type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6), size(x,7))
         call c_sqrtdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_cos_1d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:)
+        type(real64x2), target :: res(size(x,1))
+        call c_cosdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_cos_2d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2))
+        call c_cosdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_cos_3d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3))
+        call c_cosdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_cos_4d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4))
+        call c_cosdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_cos_5d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5))
+        call c_cosdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_cos_6d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6))
+        call c_cosdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_cos_7d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6), size(x,7))
+        call c_cosdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_tan_1d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:)
+        type(real64x2), target :: res(size(x,1))
+        call c_tandd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_tan_2d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2))
+        call c_tandd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_tan_3d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3))
+        call c_tandd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_tan_4d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4))
+        call c_tandd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_tan_5d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5))
+        call c_tandd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_tan_6d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6))
+        call c_tandd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_tan_7d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6), size(x,7))
+        call c_tandd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_expm1_1d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:)
+        type(real64x2), target :: res(size(x,1))
+        call c_expm1dd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_expm1_2d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2))
+        call c_expm1dd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_expm1_3d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3))
+        call c_expm1dd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_expm1_4d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4))
+        call c_expm1dd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_expm1_5d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5))
+        call c_expm1dd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_expm1_6d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6))
+        call c_expm1dd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_expm1_7d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6), size(x,7))
+        call c_expm1dd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_log2_1d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:)
+        type(real64x2), target :: res(size(x,1))
+        call c_log2dd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_log2_2d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2))
+        call c_log2dd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_log2_3d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3))
+        call c_log2dd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_log2_4d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4))
+        call c_log2dd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_log2_5d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5))
+        call c_log2dd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_log2_6d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6))
+        call c_log2dd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_log2_7d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6), size(x,7))
+        call c_log2dd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_log10_1d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:)
+        type(real64x2), target :: res(size(x,1))
+        call c_log10dd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_log10_2d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2))
+        call c_log10dd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_log10_3d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3))
+        call c_log10dd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_log10_4d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4))
+        call c_log10dd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_log10_5d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5))
+        call c_log10dd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_log10_6d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6))
+        call c_log10dd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_log10_7d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6), size(x,7))
+        call c_log10dd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_log1p_1d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:)
+        type(real64x2), target :: res(size(x,1))
+        call c_log1pdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_log1p_2d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2))
+        call c_log1pdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_log1p_3d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3))
+        call c_log1pdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_log1p_4d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4))
+        call c_log1pdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_log1p_5d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5))
+        call c_log1pdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_log1p_6d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6))
+        call c_log1pdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_log1p_7d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6), size(x,7))
+        call c_log1pdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_sinh_1d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:)
+        type(real64x2), target :: res(size(x,1))
+        call c_sinhdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_sinh_2d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2))
+        call c_sinhdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_sinh_3d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3))
+        call c_sinhdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_sinh_4d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4))
+        call c_sinhdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_sinh_5d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5))
+        call c_sinhdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_sinh_6d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6))
+        call c_sinhdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_sinh_7d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6), size(x,7))
+        call c_sinhdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_cosh_1d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:)
+        type(real64x2), target :: res(size(x,1))
+        call c_coshdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_cosh_2d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2))
+        call c_coshdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_cosh_3d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3))
+        call c_coshdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_cosh_4d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4))
+        call c_coshdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_cosh_5d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5))
+        call c_coshdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_cosh_6d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6))
+        call c_coshdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_cosh_7d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6), size(x,7))
+        call c_coshdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_tanh_1d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:)
+        type(real64x2), target :: res(size(x,1))
+        call c_tanhdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_tanh_2d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2))
+        call c_tanhdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_tanh_3d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3))
+        call c_tanhdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_tanh_4d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4))
+        call c_tanhdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_tanh_5d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5))
+        call c_tanhdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_tanh_6d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6))
+        call c_tanhdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_tanh_7d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6), size(x,7))
+        call c_tanhdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
     end function
 
     pure function dd_add_1d(a, b) result(res)

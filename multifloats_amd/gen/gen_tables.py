@@ -13,7 +13,10 @@ fixed by mathematics (so a bit-exact restatement must use the same values):
   - log: c_k = 2/((2k+1) ln2) and the 32 centres 1+(2i+1)/64 with log2 values
     (exp_log.inc:280-308);
   - sin: pi/2 three-piece split, Taylor coefficients of sin(x)/x and cos(x),
-    pi/4 and 1/sqrt(2) (src/float64x2/trig.inc:114-187).
+    pi/4 and 1/sqrt(2) (src/float64x2/trig.inc:114-187);
+  - exp2/log10/expm1/log1p/sinh: log10(2), the exp2 clamps, and the Taylor
+    coefficients 1/(k+1)!, 2/(2k+1) and 1/(2k+1)! (exp_log.inc:119-128,
+    343-353, 407-469; hyp.inc:2-18).
 
 The erf table is this project's own design (the reference fits erf with
 rational approximations; see csrc/dd_math.h for the piecewise Taylor scheme):
@@ -135,6 +138,24 @@ def main():
     out.append(arr("mf_sinc_lo", [limbs(v)[1] for v in st]))
     out.append(arr("mf_cosc_hi", [limbs(v)[0] for v in ct]))
     out.append(arr("mf_cosc_lo", [limbs(v)[1] for v in ct]))
+
+    # ---- exp2 / log10 / expm1 / log1p / sinh ----
+    out.append("\n/* log10(2); exp2 clamps of the reference (exp_log.inc:119-128): below\n"
+               "   -1080 the result is 0, above 1024 - 2^-42 it is +inf */\n")
+    out.append(dd_pair("mf_log10_2", mpmath.log10(2)))
+    out.append(scalar("mf_exp2_min", -1080.0) + scalar("mf_exp2_max", 1024.0 - 2.0 ** -42))
+    em = [1 / mpmath.factorial(k + 1) for k in range(25)]
+    out.append("\n/* expm1(x)/x Taylor coefficients: 1/(k+1)!, k = 0..24 */\n")
+    out.append(arr("mf_expm1c_hi", [limbs(v)[0] for v in em]))
+    out.append(arr("mf_expm1c_lo", [limbs(v)[1] for v in em]))
+    lp = [mpf(2) / (2 * k + 1) for k in range(18)]
+    out.append("\n/* log1p(x) = t*P(t^2), t = x/(2+x): 2/(2k+1), k = 0..17 */\n")
+    out.append(arr("mf_log1pc_hi", [limbs(v)[0] for v in lp]))
+    out.append(arr("mf_log1pc_lo", [limbs(v)[1] for v in lp]))
+    sh = [1 / mpmath.factorial(2 * k + 1) for k in range(13)]
+    out.append("\n/* sinh(x)/x Taylor coefficients in x^2: 1/(2k+1)!, k = 0..12 */\n")
+    out.append(arr("mf_sinhc_hi", [limbs(v)[0] for v in sh]))
+    out.append(arr("mf_sinhc_lo", [limbs(v)[1] for v in sh]))
 
     # ---- erf ----
     two_rtpi = 2 / mpmath.sqrt(pi)

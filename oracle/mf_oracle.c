@@ -578,9 +578,147 @@ static dd o_sin(dd x) {
   dd v = o_eval(r, q & 1);
   return (q & 2) ? mkdd(-v.hi, -v.lo) : v;
 }
+/* poly.inc:33-217 (neval): degrees 17 and 24 extend the degree-15 tree by
+ * the top coefficients times x^16 */
+static dd o_estrin15(dd x, dd x2, dd x4, dd x8, const double *h, const double *l) {
+  dd p[8];
+  for (int i = 0; i < 8; ++i) p[i] = o_add(cdd(h, l, 2 * i), o_mul(cdd(h, l, 2 * i + 1), x));
+  dd p03 = o_add(p[0], o_mul(p[1], x2)), p47 = o_add(p[2], o_mul(p[3], x2));
+  dd p811 = o_add(p[4], o_mul(p[5], x2)), p1215 = o_add(p[6], o_mul(p[7], x2));
+  return o_add(o_add(p03, o_mul(p47, x4)), o_mul(o_add(p811, o_mul(p1215, x4)), x8));
+}
+static dd o_estrin17(dd x, const double *h, const double *l) {
+  dd x2 = o_mul(x, x), x4 = o_mul(x2, x2), x8 = o_mul(x4, x4), x16 = o_mul(x8, x8);
+  dd p1617 = o_add(cdd(h, l, 16), o_mul(cdd(h, l, 17), x));
+  return o_add(o_estrin15(x, x2, x4, x8, h, l), o_mul(p1617, x16));
+}
+static dd o_estrin24(dd x, const double *h, const double *l) {
+  dd x2 = o_mul(x, x), x4 = o_mul(x2, x2), x8 = o_mul(x4, x4), x16 = o_mul(x8, x8);
+  dd lo = o_estrin15(x, x2, x4, x8, h, l);
+  dd q[4];
+  for (int i = 0; i < 4; ++i) q[i] = o_add(cdd(h, l, 16 + 2 * i), o_mul(cdd(h, l, 17 + 2 * i), x));
+  dd p1619 = o_add(q[0], o_mul(q[1], x2)), p2023 = o_add(q[2], o_mul(q[3], x2));
+  dd hi = o_add(o_add(p1619, o_mul(p2023, x4)), o_mul(cdd(h, l, 24), x8));
+  return o_add(lo, o_mul(hi, x16));
+}
+static dd o_neg(dd x) { return mkdd(-x.hi, -x.lo); }
+/* trig.inc:255-266 (cos) */
+static dd o_cos(dd x) {
+  if (!isfinite(x.hi) || !(fabs(x.hi) < 0x1p55)) return mkdd(NAN, 0.0);
+  dd r;
+  int q;
+  o_reduce(x, &r, &q);
+  switch (q) {
+    case 0: return o_eval(r, 1);
+    case 1: return o_neg(o_eval(r, 0));
+    case 2: return o_neg(o_eval(r, 1));
+    default: return o_eval(r, 0);
+  }
+}
+/* trig.inc:268-289 (tan) with sincos_eval 192-215 */
+static dd o_tan(dd x) {
+  if (x.hi == 0.0 && x.lo == 0.0) return x;
+  if (!isfinite(x.hi) || !(fabs(x.hi) < 0x1p55)) return mkdd(NAN, 0.0);
+  dd r, s, c;
+  int q;
+  o_reduce(x, &r, &q);
+  if (fabs(r.hi) <= 0.392699081698724) {
+    s = o_sink(r);
+    c = o_cosk(r);
+  } else {
+    dd pi4 = mkdd(mf_pi_quarter_hi, mf_pi_quarter_lo), is2 = mkdd(mf_inv_sqrt2_hi, mf_inv_sqrt2_lo);
+    int pos = r.hi > 0.0;
+    dd rp = pos ? o_sub(r, pi4) : o_add(r, pi4);
+    dd sk = o_sink(rp), ck = o_cosk(rp);
+    s = o_mul(pos ? o_add(sk, ck) : o_sub(sk, ck), is2);
+    c = o_mul(pos ? o_sub(ck, sk) : o_add(ck, sk), is2);
+  }
+  return (q == 0 || q == 2) ? o_div(s, c) : o_div(o_neg(c), s);
+}
+/* exp_log.inc:109-128 (exp2, exp2_kernel) */
+static dd o_exp2(dd x) {
+  if (!isfinite(x.hi)) return mkdd(x.hi > 0.0 ? x.hi : (x.hi < 0.0 ? 0.0 : x.hi), 0.0);
+  if (x.hi < mf_exp2_min) return mkdd(0.0, 0.0);
+  if (x.hi > mf_exp2_max) return mkdd(INFINITY, 0.0);
+  double n_d = nearbyint(x.hi), rh, rl, t;
+  tsum(x.hi, -n_d, &rh, &t);
+  rl = t + x.lo;
+  ftsum(rh, rl, &rh, &rl);
+  return o_exp2_reduced(n_d, rh, rl);
+}
+/* exp_log.inc:407-427 (expm1) */
+static dd o_expm1(dd x) {
+  if (x.hi == 0.0 && x.lo == 0.0) return x;
+  if (!isfinite(x.hi)) return mkdd(x.hi > 0.0 ? x.hi : (x.hi < 0.0 ? -1.0 : x.hi), 0.0);
+  if (fabs(x.hi) >= 0.5) return o_sub(o_exp(x), mkdd(1.0, 0.0));
+  return o_mul(x, o_estrin24(x, mf_expm1c_hi, mf_expm1c_lo));
+}
+/* exp_log.inc:311-327 (log2) */
+static dd o_log2(dd x) {
+  if (isnan(x.hi) || x.hi < 0.0) return mkdd(NAN, 0.0);
+  if (x.hi == 0.0) return mkdd(-INFINITY, 0.0);
+  if (!isfinite(x.hi)) return mkdd(INFINITY, 0.0);
+  return o_log2_kernel(x);
+}
+/* exp_log.inc:343-353 (log10) */
+static dd o_log10(dd x) {
+  if (!isfinite(x.hi)) return mkdd(log10(x.hi), 0.0);
+  dd l2 = o_log2(x);
+  if (!isfinite(l2.hi)) return l2;
+  return o_mul(l2, mkdd(mf_log10_2_hi, mf_log10_2_lo));
+}
+/* exp_log.inc:429-469 (log1p) */
+static dd o_log1p(dd x) {
+  if (x.hi == 0.0 && x.lo == 0.0) return x;
+  if (!isfinite(x.hi)) return mkdd(log1p(x.hi), 0.0);
+  dd arg = o_add(mkdd(1.0, 0.0), x);
+  if (arg.hi <= 0.0) {
+    if (arg.hi == 0.0 && arg.lo == 0.0) return mkdd(-INFINITY, 0.0);
+    if (arg.hi < 0.0 || (arg.hi == 0.0 && arg.lo < 0.0)) return mkdd(NAN, 0.0);
+    return o_log(mkdd(arg.lo, 0.0));
+  }
+  if (fabs(x.hi) >= 0.25) return o_log(arg);
+  dd t = o_div(x, o_add(mkdd(2.0, 0.0), x));
+  return o_mul(t, o_estrin17(o_mul(t, t), mf_log1pc_hi, mf_log1pc_lo));
+}
+/* src/float64x2/hyp.inc:2-18 (sinh) */
+static dd o_sinh(dd x) {
+  if (x.hi == 0.0 && x.lo == 0.0) return x;
+  if (!isfinite(x.hi)) return mkdd(sinh(x.hi), 0.0);
+  if (fabs(x.hi) < 0.5) return o_mul(x, o_estrin12(o_mul(x, x), mf_sinhc_hi, mf_sinhc_lo));
+  return o_mul(o_sub(o_exp(x), o_exp(o_neg(x))), mkdd(0.5, 0.0));
+}
+/* hyp.inc:20-29 (cosh) */
+static dd o_cosh(dd x) {
+  if (!isfinite(x.hi)) return mkdd(cosh(x.hi), 0.0);
+  return o_mul(o_add(o_exp(x), o_exp(o_neg(x))), mkdd(0.5, 0.0));
+}
+/* hyp.inc:62-87 (tanh) */
+static dd o_tanh(dd x) {
+  if (x.hi == 0.0 && x.lo == 0.0) return x;
+  if (!isfinite(x.hi)) return mkdd(tanh(x.hi), 0.0);
+  if (x.hi >= 38.0) return mkdd(1.0, 0.0);
+  if (x.hi <= -38.0) return mkdd(-1.0, 0.0);
+  if (fabs(x.hi) < 0.5) return o_div(o_sinh(x), o_cosh(x));
+  int sign = x.hi < 0.0;
+  dd m2 = sign ? mkdd(2.0 * x.hi, 2.0 * x.lo) : mkdd(-2.0 * x.hi, -2.0 * x.lo);
+  dd em2 = o_exp(m2);
+  dd res = o_div(o_sub(mkdd(1.0, 0.0), em2), o_add(mkdd(1.0, 0.0), em2));
+  return sign ? o_neg(res) : res;
+}
 ARR1(or_exp, o_exp)
 ARR1(or_log, o_log)
 ARR1(or_sin, o_sin)
+ARR1(or_cos, o_cos)
+ARR1(or_tan, o_tan)
+ARR1(or_exp2, o_exp2)
+ARR1(or_expm1, o_expm1)
+ARR1(or_log2, o_log2)
+ARR1(or_log10, o_log10)
+ARR1(or_log1p, o_log1p)
+ARR1(or_sinh, o_sinh)
+ARR1(or_cosh, o_cosh)
+ARR1(or_tanh, o_tanh)
 
 /* ------------------------------------------------- exact accumulator ---
  * A fixed-point superaccumulator over 2^-1216 .. 2^1120 in 32-bit digits held

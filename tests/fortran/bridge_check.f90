@@ -70,6 +70,15 @@ program bridge_check
     write(u) x / y
     write(u) exp(a)                     ! rank 2
     write(u) dd_matmul_ozaki(a, b)
+    write(u) cos(x)
+    write(u) tan(x)
+    write(u) expm1(x)
+    write(u) log2(x * x)
+    write(u) log10(x * x)
+    write(u) log1p(x)
+    write(u) sinh(x)
+    write(u) cosh(x)
+    write(u) tanh(x)
     ! xGEMM: C = x(1) A B + y(1) C with A passed transposed and C padded to m + 2 rows
     at = transpose(a)
     allocate(cc(m + 2, n))

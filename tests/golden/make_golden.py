@@ -5,7 +5,9 @@ Inputs are seeded draws from multifloats_amd.datagen plus the special and
 known-answer values the reference's own tests use; expected outputs come from
   - the reference's compiled C++ path (oracle/_ref/libmfref.so, built from the
     reference sources by oracle/Makefile), called through its C ABI: adddd,
-    subdd, muldd, divdd, sqrtdd, expdd, logdd, sindd, erfdd, matmuldd_mm/mv/vm;
+    subdd, muldd, divdd, sqrtdd, expdd, logdd, sindd, erfdd, matmuldd_mm/mv/vm,
+    and cosdd, tandd, exp2dd, expm1dd, log2dd, log10dd, log1pdd, sinhdd,
+    coshdd, tanhdd;
   - mpmath at 320 bits for the transcendental values (stored as three limbs so
     relative errors down to 1e-40 can be measured).
 Every fixture is data only (inputs and expected outputs).
@@ -39,7 +41,7 @@ def load_ref():
     for nm in ("adddd", "subdd", "muldd", "divdd"):
         getattr(L, nm).argtypes = [F2, F2]
         getattr(L, nm).restype = F2
-    for nm in ("sqrtdd", "expdd", "logdd", "sindd", "erfdd"):
+    for nm in ("sqrtdd", "expdd", "logdd", "sindd", "erfdd") + tuple(f + "dd" for f in TRANSC2):
         getattr(L, nm).argtypes = [F2]
         getattr(L, nm).restype = F2
     P, I64 = ctypes.c_void_p, ctypes.c_int64
@@ -155,6 +157,64 @@ def transc(L, rng):
     np.savez_compressed(os.path.join(HERE, "transc.npz"), **out)
 
 
+TRANSC2 = ("cos", "tan", "exp2", "expm1", "log2", "log10", "log1p", "sinh", "cosh", "tanh")
+
+
+def transc2(L, rng):
+    """cos, tan, exp2, expm1, log2, log10, log1p, sinh, cosh, tanh: seeded
+    draws over each function's regimes (the branch points of the reference's
+    algorithms included), special values, and 320-bit values for accuracy."""
+    n = 1500
+    w = datagen._with_lo
+    U = rng.uniform
+    near_half = np.array([(0.5 * s, 0.0) for s in (1, -1)] + [(np.nextafter(0.5, 0) * s, 0.0) for s in (1, -1)])
+    cases = {
+        "cos": np.concatenate([datagen.sin_inputs(rng, n, near_frac=0.2), datagen.config0(rng, 200),
+                               np.array([(2 * np.pi * 2.0 ** k, 0.0) for k in range(0, 40)])]),
+        "tan": np.concatenate([datagen.sin_inputs(rng, n, near_frac=0.2), datagen.config0(rng, 200)]),
+        "exp2": np.concatenate([w(rng, U(-1100, 1100, n)), datagen.config0(rng, 200),
+                                w(rng, U(1020, 1024, 50)), w(rng, U(-1080, -1070, 50)),
+                                np.array([(1023.9999999999998, 0.0), (1024.0, 0.0), (-1080.0, 0.0),
+                                          (-1080.0000000000002, 0.0), (0.5, 0.0), (10.0, 1e-16)])]),
+        "expm1": np.concatenate([w(rng, U(-0.5, 0.5, n)), w(rng, 10.0 ** U(-30, -1, 200)),
+                                 w(rng, U(-745, 709, 300)), near_half]),
+        "log2": np.concatenate([datagen.log_inputs(rng, n), w(rng, U(15 / 16, 17 / 16, 300)),
+                                np.array([(2.0 ** k, 0.0) for k in range(-1074, 1024, 37)])]),
+        "log10": np.concatenate([datagen.log_inputs(rng, n), w(rng, U(15 / 16, 17 / 16, 300)),
+                                 np.array([(10.0 ** k, 0.0) for k in range(-300, 300, 13)])]),
+        "log1p": np.concatenate([w(rng, U(-0.25, 0.25, n)), w(rng, 10.0 ** U(-30, -1, 200)),
+                                 w(rng, -(10.0 ** U(-30, -1, 100))), w(rng, U(-0.999, 3, 300)),
+                                 w(rng, 10.0 ** U(0, 300, 100)), w(rng, -1.0 - 10.0 ** U(-10, 2, 30)),
+                                 np.array([(-1.0, 0.0), (-1.0, 1e-20), (-1.0, -1e-20), (0.25, 0.0),
+                                           (-0.25, 0.0)])]),
+        "sinh": np.concatenate([w(rng, U(-40, 40, n)), w(rng, U(-0.5, 0.5, 300)),
+                                w(rng, U(-715, 715, 100)), near_half]),
+        "cosh": np.concatenate([w(rng, U(-40, 40, n)), w(rng, U(-0.5, 0.5, 300)), w(rng, U(-715, 715, 100))]),
+        "tanh": np.concatenate([w(rng, U(-40, 40, n)), w(rng, U(-0.5, 0.5, 300)),
+                                np.array([(38.0, 0.0), (-38.0, 0.0), (37.99999999999999, 0.0)]), near_half]),
+    }
+    mpfn = {"cos": mpmath.cos, "tan": mpmath.tan, "exp2": lambda v: mpmath.power(2, v), "expm1": mpmath.expm1,
+            "log2": lambda v: mpmath.log(v, 2), "log10": mpmath.log10, "log1p": mpmath.log1p,
+            "sinh": mpmath.sinh, "cosh": mpmath.cosh, "tanh": mpmath.tanh}
+    out = {}
+    for nm, x in cases.items():
+        x = np.concatenate([x, np.array(SPECIALS)])
+        out[nm + "_in"] = x
+        out[nm + "_ref"] = call1(L, nm + "dd", x)
+        exact = np.full((x.shape[0], 3), np.nan)
+        for i in range(x.shape[0]):
+            if not np.isfinite(x[i, 0]):
+                continue
+            try:
+                v = mpfn[nm](mp_of(x[i]))
+                if mpmath.im(v) == 0:
+                    exact[i] = limbs3(mpmath.re(v))
+            except (OverflowError, ValueError, ZeroDivisionError):
+                pass
+        out[nm + "_exact"] = exact
+    np.savez_compressed(os.path.join(HERE, "transc2.npz"), **out)
+
+
 def matmul(L, rng):
     shapes = [(1, 1, 1), (7, 1, 9), (3, 9, 1), (2, 20, 5), (16, 16, 16), (4, 32, 6),
               (8, 8, 8), (33, 17, 31), (65, 70, 66)]
@@ -223,7 +283,8 @@ def main():
     transc(L, rng)
     matmul(L, rng)
     cmatmul(L, np.random.default_rng(777))
-    for f in ("arith.npz", "transc.npz", "matmul.npz", "cmatmul.npz"):
+    transc2(L, np.random.default_rng(4242))
+    for f in ("arith.npz", "transc.npz", "matmul.npz", "cmatmul.npz", "transc2.npz"):
         print(f, os.path.getsize(os.path.join(HERE, f)))
 
 

@@ -91,6 +91,12 @@ def test_fortran_bridge_routes_to_the_library(gpu, tmp_path):
     rows, cols = np.meshgrid(np.arange(m), np.arange(n), indexing="ij")
     err, mag = nt.gemm_err(A, B, C, rows.ravel(), cols.ravel())
     assert (err <= 2.0 ** -100 * mag).all(), "ozaki through the bridge"
+    for nm in ("cos", "tan", "expm1"):
+        nt.assert_bits(r.take(nv), getattr(mf, nm)(x), nm)
+    for nm in ("log2", "log10"):
+        nt.assert_bits(r.take(nv), getattr(mf, nm)(mf.mul(x, x)), nm)
+    for nm in ("log1p", "sinh", "cosh", "tanh"):
+        nt.assert_bits(r.take(nv), getattr(mf, nm)(x), nm)
     At = nt.cm(A.transpose(1, 0, 2))
     Cw = np.empty((m + 2, n, 2))
     Cw[:m] = A[:, :n]

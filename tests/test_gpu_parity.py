@@ -58,6 +58,48 @@ def test_transcendental_bits_vs_oracle(gpu, fn):
     nt.assert_bits(getattr(mf, fn)(x), nt.or_unary(fn, x), fn)
 
 
+def _transc2_inputs(rng, fn, n):
+    w, U = datagen._with_lo, rng.uniform
+    if fn in ("cos", "tan"):
+        return datagen.sin_inputs(rng, n, near_frac=0.1)
+    if fn == "exp2":
+        return w(rng, U(-1100, 1100, n))
+    if fn == "expm1":
+        return np.concatenate([w(rng, U(-0.5, 0.5, n // 2)), w(rng, U(-745, 709, n - n // 2))])
+    if fn in ("log2", "log10"):
+        return np.concatenate([datagen.log_inputs(rng, n // 2), w(rng, U(15 / 16, 17 / 16, n - n // 2))])
+    if fn == "log1p":
+        return np.concatenate([w(rng, U(-0.25, 0.25, n // 2)), w(rng, U(-1.0, 10.0, n - n // 2))])
+    return np.concatenate([w(rng, U(-0.5, 0.5, n // 2)), w(rng, U(-60, 60, n - n // 2))])
+
+
+@pytest.mark.parametrize("fn", nt.TRANSC2)
+def test_transcendental2_bits_golden(gpu, fn):
+    g = nt.load_golden("transc2.npz")
+    nt.assert_bits(getattr(mf, fn)(g[fn + "_in"]), g[fn + "_ref"], fn)
+
+
+@pytest.mark.parametrize("fn", nt.TRANSC2)
+def test_transcendental2_bits_vs_oracle(gpu, fn):
+    rng = np.random.default_rng(sum(map(ord, fn)))
+    x = _transc2_inputs(rng, fn, 200000)
+    nt.assert_bits(getattr(mf, fn)(x), nt.or_unary(fn, x), fn)
+
+
+def test_transcendental2_device_entry(torch_cuda):
+    """mfhip_unary with the new op codes reaches the same kernels."""
+    torch = torch_cuda
+    rng = np.random.default_rng(8)
+    stream = torch.cuda.current_stream().cuda_stream
+    for fn in nt.TRANSC2:
+        x = _transc2_inputs(rng, fn, 5000)
+        dx = to_dev(torch, x)
+        dy = torch.empty_like(dx)
+        mf.dev_unary(getattr(mf, "MFHIP_" + fn.upper()), dx.data_ptr(), dy.data_ptr(), x.shape[0], stream)
+        torch.cuda.synchronize()
+        nt.assert_bits(dy.cpu().numpy(), nt.or_unary(fn, x), "device " + fn)
+
+
 def test_erf_accuracy_golden(gpu):
     g = nt.load_golden("transc.npz")
     x, ex = g["erf_in"], g["erf_exact"]

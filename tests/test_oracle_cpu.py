@@ -24,6 +24,12 @@ def test_oracle_transcendentals_match_reference_bits(fn):
     nt.assert_bits(nt.or_unary(fn, g[fn + "_in"]), g[fn + "_ref"], fn)
 
 
+@pytest.mark.parametrize("fn", nt.TRANSC2)
+def test_oracle_transcendentals2_match_reference_bits(fn):
+    g = nt.load_golden("transc2.npz")
+    nt.assert_bits(nt.or_unary(fn, g[fn + "_in"]), g[fn + "_ref"], fn)
+
+
 def test_oracle_matmul_matches_reference_bits():
     g = nt.load_golden("matmul.npz")
     O = nt.oracle()
