@@ -168,6 +168,28 @@ MULTIFLOATS_API void sumdd_dim(const float64x2 *a, const int32_t *mask, int64_t 
 MULTIFLOATS_API void norm2dd_dim(const float64x2 *a, int64_t inner, int64_t len, int64_t outer,
                                  float64x2 *out);
 
+/* Fortran product / maxval / minval on real64x2 (fypp:1783-1862, 1975-2090):
+ *   <f>dd(a, n), <f>dd_mask(a, mask, n) (mask of int32, nonzero = included,
+ *   NULL = all), <f>dd_dim(a, mask, inner, len, outer, out) as sumdd_dim.
+ * product: rank 1 is a DD-multiply tree from 1 (not the sequential loop's
+ * bits; relative error within 4 n 2^-104 for results clear of overflow and
+ * underflow); dim = sequential chains, bit-identical. maxval / minval: the
+ * reference's scan (first element, replaced by any later x > res / x < res,
+ * lexicographic on (hi, lo)), bit-identical in every form including NaN
+ * limbs; 0 when no element is selected. */
+MULTIFLOATS_API float64x2 productdd(const float64x2 *a, int64_t n);
+MULTIFLOATS_API float64x2 productdd_mask(const float64x2 *a, const int32_t *mask, int64_t n);
+MULTIFLOATS_API void productdd_dim(const float64x2 *a, const int32_t *mask, int64_t inner, int64_t len,
+                                   int64_t outer, float64x2 *out);
+MULTIFLOATS_API float64x2 maxvaldd(const float64x2 *a, int64_t n);
+MULTIFLOATS_API float64x2 maxvaldd_mask(const float64x2 *a, const int32_t *mask, int64_t n);
+MULTIFLOATS_API void maxvaldd_dim(const float64x2 *a, const int32_t *mask, int64_t inner, int64_t len,
+                                  int64_t outer, float64x2 *out);
+MULTIFLOATS_API float64x2 minvaldd(const float64x2 *a, int64_t n);
+MULTIFLOATS_API float64x2 minvaldd_mask(const float64x2 *a, const int32_t *mask, int64_t n);
+MULTIFLOATS_API void minvaldd_dim(const float64x2 *a, const int32_t *mask, int64_t inner, int64_t len,
+                                  int64_t outer, float64x2 *out);
+
 /* BLAS-style C = alpha op(A) op(B) + beta C on float64x2 (the interface of
  * the reference's xGEMM shim test/integration/blas/wgemm.f90:1-14; SURVEY's
  * "next" GEMM interface). transa/transb: 'N', 'T' or 'C' (= 'T' for real);
@@ -215,7 +237,7 @@ enum {
   MFHIP_LOG10 = 11, MFHIP_LOG1P = 12, MFHIP_SINH = 13, MFHIP_COSH = 14, MFHIP_TANH = 15
 };
 enum { MFHIP_ADD = 0, MFHIP_SUB = 1, MFHIP_MUL = 2, MFHIP_DIV = 3 };
-enum { MFHIP_DOT = 0, MFHIP_SUM = 1, MFHIP_NORM2 = 2 };
+enum { MFHIP_DOT = 0, MFHIP_SUM = 1, MFHIP_NORM2 = 2, MFHIP_PRODUCT = 3, MFHIP_MAXVAL = 4, MFHIP_MINVAL = 5 };
 
 MULTIFLOATS_API int mfhip_matmul_mm(const float64x2 *a, const float64x2 *b, float64x2 *c,
                                     int64_t m, int64_t k, int64_t n, int64_t renorm_interval,
@@ -246,11 +268,16 @@ MULTIFLOATS_API int mfhip_reduce(int kind, const float64x2 *a, const float64x2 *
                                  void *stream);
 
 /* Device forms of sumdd_mask (work as for mfhip_reduce), cdotdd (work of
- * mfhip_cdot_workspace_bytes(n) bytes; *out is one complex64x2) and
- * sumdd_dim / norm2dd_dim (kind MFHIP_SUM or MFHIP_NORM2; no workspace;
- * mask only with MFHIP_SUM, may be NULL). */
+ * mfhip_cdot_workspace_bytes(n) bytes; *out is one complex64x2) and the
+ * _dim reductions (kind MFHIP_SUM, MFHIP_NORM2, MFHIP_PRODUCT, MFHIP_MAXVAL
+ * or MFHIP_MINVAL; no workspace; mask may be NULL and is refused with
+ * MFHIP_NORM2). mfhip_reduce takes MFHIP_PRODUCT / MAXVAL / MINVAL too. */
 MULTIFLOATS_API int mfhip_sum_masked(const float64x2 *a, const int32_t *mask, int64_t n,
                                      float64x2 *out, void *work, void *stream);
+/* Masked rank-1 reduction of any kind but MFHIP_DOT / MFHIP_NORM2 (sum,
+ * product, maxval, minval); work as for mfhip_reduce. */
+MULTIFLOATS_API int mfhip_reduce_masked(int kind, const float64x2 *a, const int32_t *mask, int64_t n,
+                                        float64x2 *out, void *work, void *stream);
 MULTIFLOATS_API size_t mfhip_cdot_workspace_bytes(int64_t n);
 MULTIFLOATS_API int mfhip_cdot(const complex64x2 *a, const complex64x2 *b, int64_t n,
                                int64_t renorm_interval, complex64x2 *out, void *work, void *stream);

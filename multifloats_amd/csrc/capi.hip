@@ -184,7 +184,7 @@ hipError_t host_reduce(int kind, const float64x2 *a, const float64x2 *b, int64_t
   return hipStreamSynchronize(tl_stage.stream);
 }
 
-hipError_t host_sum_mask(const float64x2 *a, const int32_t *mask, int64_t n, float64x2 *out) {
+hipError_t host_reduce_mask(int kind, const float64x2 *a, const int32_t *mask, int64_t n, float64x2 *out) {
   tl_err = 0;
   TRY(stage_init());
   const size_t nn = n > 0 ? (size_t)n : 0;
@@ -195,8 +195,7 @@ hipError_t host_sum_mask(const float64x2 *a, const int32_t *mask, int64_t n, flo
   TRY(h2d(da, a, 16 * nn));
   TRY(h2d(dm, mask, 4 * nn));
   double2 *dout = W(dw);
-  TRY(mfhip::reduce(mfhip::RED_SUM, D(da), nullptr, n, 0, dout, dout + 1, tl_stage.stream,
-                    static_cast<const int32_t *>(dm)));
+  TRY(mfhip::reduce(kind, D(da), nullptr, n, 0, dout, dout + 1, tl_stage.stream, static_cast<const int32_t *>(dm)));
   TRY(d2h(out, dout, 16));
   return hipStreamSynchronize(tl_stage.stream);
 }
@@ -478,7 +477,8 @@ float64x2 norm2dd(const float64x2 *a, int64_t n) {
 
 float64x2 sumdd_mask(const float64x2 *a, const int32_t *mask, int64_t n) {
   float64x2 r;
-  hipError_t e = mask ? host_sum_mask(a, mask, n, &r) : host_reduce(mfhip::RED_SUM, a, nullptr, n, 0, &r);
+  hipError_t e = mask ? host_reduce_mask(mfhip::RED_SUM, a, mask, n, &r)
+                      : host_reduce(mfhip::RED_SUM, a, nullptr, n, 0, &r);
   if (e != hipSuccess) {
     report("sumdd_mask", e);
     return nan_dd();
@@ -502,6 +502,37 @@ void sumdd_dim(const float64x2 *a, const int32_t *mask, int64_t inner, int64_t l
     nan_fill(out, (inner > 0 && outer > 0) ? inner * outer : 0);
   }
 }
+#define MF_REDUCE_FAMILY(name, kind)                                                              \
+  float64x2 name##dd(const float64x2 *a, int64_t n) {                                             \
+    float64x2 r;                                                                                  \
+    hipError_t e = host_reduce(kind, a, nullptr, n, 0, &r);                                       \
+    if (e != hipSuccess) {                                                                        \
+      report(#name "dd", e);                                                                      \
+      return nan_dd();                                                                            \
+    }                                                                                             \
+    return r;                                                                                     \
+  }                                                                                               \
+  float64x2 name##dd_mask(const float64x2 *a, const int32_t *mask, int64_t n) {                  \
+    float64x2 r;                                                                                  \
+    hipError_t e = mask ? host_reduce_mask(kind, a, mask, n, &r) : host_reduce(kind, a, nullptr, n, 0, &r); \
+    if (e != hipSuccess) {                                                                        \
+      report(#name "dd_mask", e);                                                                 \
+      return nan_dd();                                                                            \
+    }                                                                                             \
+    return r;                                                                                     \
+  }                                                                                               \
+  void name##dd_dim(const float64x2 *a, const int32_t *mask, int64_t inner, int64_t len, int64_t outer, \
+                    float64x2 *out) {                                                             \
+    hipError_t e = host_dim(kind, a, mask, inner, len, outer, out);                               \
+    if (e != hipSuccess) {                                                                        \
+      report(#name "dd_dim", e);                                                                  \
+      nan_fill(out, (inner > 0 && outer > 0) ? inner * outer : 0);                                \
+    }                                                                                             \
+  }
+MF_REDUCE_FAMILY(product, mfhip::RED_PROD)
+MF_REDUCE_FAMILY(maxval, mfhip::RED_MAXVAL)
+MF_REDUCE_FAMILY(minval, mfhip::RED_MINVAL)
+
 void norm2dd_dim(const float64x2 *a, int64_t inner, int64_t len, int64_t outer, float64x2 *out) {
   hipError_t e = host_dim(mfhip::RED_NORM2, a, nullptr, inner, len, outer, out);
   if (e != hipSuccess) {
@@ -596,6 +627,13 @@ int mfhip_sum_masked(const float64x2 *a, const int32_t *mask, int64_t n, float64
   if (!work || !out || !mask) return status(hipErrorInvalidValue);
   return status(mfhip::reduce(mfhip::RED_SUM, D(a), nullptr, n, 0, W(out), W(work), (hipStream_t)stream,
                               mask));
+}
+int mfhip_reduce_masked(int kind, const float64x2 *a, const int32_t *mask, int64_t n, float64x2 *out, void *work,
+                        void *stream) {
+  CHECK_ALIGN(a, out, work);
+  if (!work || !out || !mask || kind == mfhip::RED_DOT || kind == mfhip::RED_NORM2)
+    return status(hipErrorInvalidValue);
+  return status(mfhip::reduce(kind, D(a), nullptr, n, 0, W(out), W(work), (hipStream_t)stream, mask));
 }
 size_t mfhip_cdot_workspace_bytes(int64_t n) { return mfhip::cdot_workspace_bytes(n); }
 int mfhip_cdot(const complex64x2 *a, const complex64x2 *b, int64_t n, int64_t ri, complex64x2 *out,

@@ -18,7 +18,7 @@ enum {
   OP_LOG10 = 11, OP_LOG1P = 12, OP_SINH = 13, OP_COSH = 14, OP_TANH = 15
 };
 enum { OP_ADD = 0, OP_SUB = 1, OP_MUL = 2, OP_DIV = 3 };
-enum { RED_DOT = 0, RED_SUM = 1, RED_NORM2 = 2 };
+enum { RED_DOT = 0, RED_SUM = 1, RED_NORM2 = 2, RED_PROD = 3, RED_MAXVAL = 4, RED_MINVAL = 5 };
 
 // Plain reference-order GEMM; variant < 0 picks by size, otherwise one of
 // gemm_plain_variants() tile shapes (index order is stable for the tuner).

@@ -17,6 +17,8 @@ UNARY = [("exp", "expdd_v"), ("log", "logdd_v"), ("sin", "sindd_v"), ("erf", "er
          ("cos", "cosdd_v"), ("tan", "tandd_v"), ("expm1", "expm1dd_v"), ("log2", "log2dd_v"),
          ("log10", "log10dd_v"), ("log1p", "log1pdd_v"), ("sinh", "sinhdd_v"), ("cosh", "coshdd_v"),
          ("tanh", "tanhdd_v")]
+# reductions with mask= and dim= (fypp:1783-2090): generic, C entry stem
+MASKED_REDUCE = [("sum", "sumdd"), ("product", "productdd"), ("maxval", "maxvaldd"), ("minval", "minvaldd")]
 BINARY = [("+", "add", "adddd_v"), ("-", "sub", "subdd_v"), ("*", "mul", "muldd_v"), ("/", "div", "divdd_v")]
 
 
@@ -47,7 +49,8 @@ def gen():
 ! Reference interfaces replaced (fsrc/multifloats.fypp):
 !   matmul          dd_matmul_mm/mv/vm 2568-2593, cdd_matmul_* 2609-2649
 !   dot_product     dd_dot_product 2254-2308, cdd_dot_product 2321-2331
-!   sum             dd_sum_Nd / dd_sum_Nd_dim 1980-2090 (with mask)
+!   sum product     dd_sum_Nd / dd_product_Nd (+ _dim) 1975-2090 (with mask)
+!   maxval minval   dd_maxval_Nd / dd_minval_Nd (+ _dim) 1783-1862 (with mask)
 !   norm2           dd_norm2_Nd / dd_norm2_Nd_dim 2338-2377
 !   exp log sin erf sqrt cos tan expm1 log2 log10 log1p sinh cosh tanh
 !                   elemental delegates 709-736, 1407-1412 (names fypp:57)
@@ -96,7 +99,7 @@ module multifloats_hip
     public :: dd_set_fma_renorm_interval
     public :: dd_matmul_ozaki
     public :: wgemm
-    public :: matmul, dot_product, sum, norm2
+    public :: matmul, dot_product, sum, product, maxval, minval, norm2
     public :: exp, log, sin, erf, sqrt, cos, tan, expm1, log2, log10, log1p, sinh, cosh, tanh
     public :: operator(+), operator(-), operator(*), operator(/)
 
@@ -159,21 +162,24 @@ module multifloats_hip
             type(c_ptr), value :: a, b
             integer(c_int64_t), value :: n, ri
         end function
-        pure type(float64x2) function c_sumdd_mask(a, mask, n) bind(c, name='sumdd_mask')
+""")
+    for f, c in MASKED_REDUCE:
+        w("""        pure type(float64x2) function c_%(c)s_mask(a, mask, n) bind(c, name='%(c)s_mask')
             import :: c_ptr, c_int64_t, float64x2
             type(c_ptr), value :: a, mask
             integer(c_int64_t), value :: n
         end function
-        pure type(float64x2) function c_norm2dd(a, n) bind(c, name='norm2dd')
-            import :: c_ptr, c_int64_t, float64x2
-            type(c_ptr), value :: a
-            integer(c_int64_t), value :: n
-        end function
-        pure subroutine c_sumdd_dim(a, mask, inner, len, outer, out) bind(c, name='sumdd_dim')
+        pure subroutine c_%(c)s_dim(a, mask, inner, len, outer, out) bind(c, name='%(c)s_dim')
             import :: c_ptr, c_int64_t
             type(c_ptr), value :: a, mask, out
             integer(c_int64_t), value :: inner, len, outer
         end subroutine
+""" % dict(c=c))
+    w("""        pure type(float64x2) function c_norm2dd(a, n) bind(c, name='norm2dd')
+            import :: c_ptr, c_int64_t, float64x2
+            type(c_ptr), value :: a
+            integer(c_int64_t), value :: n
+        end function
         pure subroutine c_norm2dd_dim(a, inner, len, outer, out) bind(c, name='norm2dd_dim')
             import :: c_ptr, c_int64_t
             type(c_ptr), value :: a, out
@@ -206,12 +212,14 @@ module multifloats_hip
         module procedure dd_dot_product, cdd_dot_product
     end interface
 """)
-    w("    interface sum\n")
-    for r in range(1, MAX_RANK + 1):
-        w("        module procedure dd_sum_%dd\n" % r)
-        if r > 1:
-            w("        module procedure dd_sum_%dd_dim\n" % r)
-    w("    end interface\n\n    interface norm2\n")
+    for f, _ in MASKED_REDUCE:
+        w("    interface %s\n" % f)
+        for r in range(1, MAX_RANK + 1):
+            w("        module procedure dd_%s_%dd\n" % (f, r))
+            if r > 1:
+                w("        module procedure dd_%s_%dd_dim\n" % (f, r))
+        w("    end interface\n\n")
+    w("    interface norm2\n")
     for r in range(1, MAX_RANK + 1):
         w("        module procedure dd_norm2_%dd\n" % r)
         if r > 1:
@@ -332,20 +340,22 @@ module multifloats_hip
 """)
     for r in range(1, MAX_RANK + 1):
         C = colons(r)
-        w("""    pure function dd_sum_%(r)dd(array, mask) result(res)
+        for f, c in MASKED_REDUCE:
+            w("""    pure function dd_%(f)s_%(r)dd(array, mask) result(res)
         type(real64x2), intent(in), target, contiguous :: array(%(C)s)
         logical, intent(in), target, contiguous, optional :: mask(%(C)s)
         type(real64x2) :: res
         type(float64x2) :: s
         if (present(mask)) then
-            s = c_sumdd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
+            s = c_%(c)s_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
         else
-            s = c_sumdd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
+            s = c_%(c)s_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
         end if
         res%%limbs = s%%limbs
     end function
 
-    pure function dd_norm2_%(r)dd(array) result(res)
+""" % dict(r=r, C=C, f=f, c=c))
+        w("""    pure function dd_norm2_%(r)dd(array) result(res)
         type(real64x2), intent(in), target, contiguous :: array(%(C)s)
         type(real64x2) :: res
         type(float64x2) :: s
@@ -355,7 +365,8 @@ module multifloats_hip
 
 """ % dict(r=r, C=C))
         if r > 1:
-            w("""    pure function dd_sum_%(r)dd_dim(array, dim, mask) result(res)
+          for f, c in MASKED_REDUCE:
+            w("""    pure function dd_%(f)s_%(r)dd_dim(array, dim, mask) result(res)
         type(real64x2), intent(in), target, contiguous :: array(%(C)s)
         integer, intent(in) :: dim
         logical, intent(in), target, contiguous, optional :: mask(%(C)s)
@@ -371,13 +382,14 @@ module multifloats_hip
             outer = outer * size(array, d_)
         end do
         if (present(mask)) then
-            call c_sumdd_dim(c_loc(array), c_loc(mask), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+            call c_%(c)s_dim(c_loc(array), c_loc(mask), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
         else
-            call c_sumdd_dim(c_loc(array), c_null_ptr, inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+            call c_%(c)s_dim(c_loc(array), c_null_ptr, inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
         end if
     end function
 
-    pure function dd_norm2_%(r)dd_dim(array, dim) result(res)
+""" % dict(r=r, C=C, RS=reduced_shape(r), f=f, c=c))
+          w("""    pure function dd_norm2_%(r)dd_dim(array, dim) result(res)
         type(real64x2), intent(in), target, contiguous :: array(%(C)s)
         integer, intent(in) :: dim
         type(real64x2), target :: res(%(RS)s)

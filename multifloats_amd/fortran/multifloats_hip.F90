@@ -9,7 +9,8 @@
 ! Reference interfaces replaced (fsrc/multifloats.fypp):
 !   matmul          dd_matmul_mm/mv/vm 2568-2593, cdd_matmul_* 2609-2649
 !   dot_product     dd_dot_product 2254-2308, cdd_dot_product 2321-2331
-!   sum             dd_sum_Nd / dd_sum_Nd_dim 1980-2090 (with mask)
+!   sum product     dd_sum_Nd / dd_product_Nd (+ _dim) 1975-2090 (with mask)
+!   maxval minval   dd_maxval_Nd / dd_minval_Nd (+ _dim) 1783-1862 (with mask)
 !   norm2           dd_norm2_Nd / dd_norm2_Nd_dim 2338-2377
 !   exp log sin erf sqrt cos tan expm1 log2 log10 log1p sinh cosh tanh
 !                   elemental delegates 709-736, 1407-1412 (names fypp:57)
@@ -58,7 +59,7 @@ module multifloats_hip
     public :: dd_set_fma_renorm_interval
     public :: dd_matmul_ozaki
     public :: wgemm
-    public :: matmul, dot_product, sum, norm2
+    public :: matmul, dot_product, sum, product, maxval, minval, norm2
     public :: exp, log, sin, erf, sqrt, cos, tan, expm1, log2, log10, log1p, sinh, cosh, tanh
     public :: operator(+), operator(-), operator(*), operator(/)
 
@@ -126,16 +127,46 @@ module multifloats_hip
             type(c_ptr), value :: a, mask
             integer(c_int64_t), value :: n
         end function
-        pure type(float64x2) function c_norm2dd(a, n) bind(c, name='norm2dd')
-            import :: c_ptr, c_int64_t, float64x2
-            type(c_ptr), value :: a
-            integer(c_int64_t), value :: n
-        end function
         pure subroutine c_sumdd_dim(a, mask, inner, len, outer, out) bind(c, name='sumdd_dim')
             import :: c_ptr, c_int64_t
             type(c_ptr), value :: a, mask, out
             integer(c_int64_t), value :: inner, len, outer
         end subroutine
+        pure type(float64x2) function c_productdd_mask(a, mask, n) bind(c, name='productdd_mask')
+            import :: c_ptr, c_int64_t, float64x2
+            type(c_ptr), value :: a, mask
+            integer(c_int64_t), value :: n
+        end function
+        pure subroutine c_productdd_dim(a, mask, inner, len, outer, out) bind(c, name='productdd_dim')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, mask, out
+            integer(c_int64_t), value :: inner, len, outer
+        end subroutine
+        pure type(float64x2) function c_maxvaldd_mask(a, mask, n) bind(c, name='maxvaldd_mask')
+            import :: c_ptr, c_int64_t, float64x2
+            type(c_ptr), value :: a, mask
+            integer(c_int64_t), value :: n
+        end function
+        pure subroutine c_maxvaldd_dim(a, mask, inner, len, outer, out) bind(c, name='maxvaldd_dim')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, mask, out
+            integer(c_int64_t), value :: inner, len, outer
+        end subroutine
+        pure type(float64x2) function c_minvaldd_mask(a, mask, n) bind(c, name='minvaldd_mask')
+            import :: c_ptr, c_int64_t, float64x2
+            type(c_ptr), value :: a, mask
+            integer(c_int64_t), value :: n
+        end function
+        pure subroutine c_minvaldd_dim(a, mask, inner, len, outer, out) bind(c, name='minvaldd_dim')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, mask, out
+            integer(c_int64_t), value :: inner, len, outer
+        end subroutine
+        pure type(float64x2) function c_norm2dd(a, n) bind(c, name='norm2dd')
+            import :: c_ptr, c_int64_t, float64x2
+            type(c_ptr), value :: a
+            integer(c_int64_t), value :: n
+        end function
         pure subroutine c_norm2dd_dim(a, inner, len, outer, out) bind(c, name='norm2dd_dim')
             import :: c_ptr, c_int64_t
             type(c_ptr), value :: a, out
@@ -254,6 +285,54 @@ module multifloats_hip
         module procedure dd_sum_6d_dim
         module procedure dd_sum_7d
         module procedure dd_sum_7d_dim
+    end interface
+
+    interface product
+        module procedure dd_product_1d
+        module procedure dd_product_2d
+        module procedure dd_product_2d_dim
+        module procedure dd_product_3d
+        module procedure dd_product_3d_dim
+        module procedure dd_product_4d
+        module procedure dd_product_4d_dim
+        module procedure dd_product_5d
+        module procedure dd_product_5d_dim
+        module procedure dd_product_6d
+        module procedure dd_product_6d_dim
+        module procedure dd_product_7d
+        module procedure dd_product_7d_dim
+    end interface
+
+    interface maxval
+        module procedure dd_maxval_1d
+        module procedure dd_maxval_2d
+        module procedure dd_maxval_2d_dim
+        module procedure dd_maxval_3d
+        module procedure dd_maxval_3d_dim
+        module procedure dd_maxval_4d
+        module procedure dd_maxval_4d_dim
+        module procedure dd_maxval_5d
+        module procedure dd_maxval_5d_dim
+        module procedure dd_maxval_6d
+        module procedure dd_maxval_6d_dim
+        module procedure dd_maxval_7d
+        module procedure dd_maxval_7d_dim
+    end interface
+
+    interface minval
+        module procedure dd_minval_1d
+        module procedure dd_minval_2d
+        module procedure dd_minval_2d_dim
+        module procedure dd_minval_3d
+        module procedure dd_minval_3d_dim
+        module procedure dd_minval_4d
+        module procedure dd_minval_4d_dim
+        module procedure dd_minval_5d
+        module procedure dd_minval_5d_dim
+        module procedure dd_minval_6d
+        module procedure dd_minval_6d_dim
+        module procedure dd_minval_7d
+        module procedure dd_minval_7d_dim
     end interface
 
     interface norm2
@@ -567,6 +646,45 @@ contains
         res%limbs = s%limbs
     end function
 
+    pure function dd_product_1d(array, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:)
+        logical, intent(in), target, contiguous, optional :: mask(:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        if (present(mask)) then
+            s = c_productdd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
+        else
+            s = c_productdd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
+        end if
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_maxval_1d(array, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:)
+        logical, intent(in), target, contiguous, optional :: mask(:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        if (present(mask)) then
+            s = c_maxvaldd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
+        else
+            s = c_maxvaldd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
+        end if
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_minval_1d(array, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:)
+        logical, intent(in), target, contiguous, optional :: mask(:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        if (present(mask)) then
+            s = c_minvaldd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
+        else
+            s = c_minvaldd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
+        end if
+        res%limbs = s%limbs
+    end function
+
     pure function dd_norm2_1d(array) result(res)
         type(real64x2), intent(in), target, contiguous :: array(:)
         type(real64x2) :: res
@@ -584,6 +702,45 @@ contains
             s = c_sumdd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
         else
             s = c_sumdd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
+        end if
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_product_2d(array, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:)
+        logical, intent(in), target, contiguous, optional :: mask(:,:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        if (present(mask)) then
+            s = c_productdd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
+        else
+            s = c_productdd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
+        end if
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_maxval_2d(array, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:)
+        logical, intent(in), target, contiguous, optional :: mask(:,:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        if (present(mask)) then
+            s = c_maxvaldd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
+        else
+            s = c_maxvaldd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
+        end if
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_minval_2d(array, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:)
+        logical, intent(in), target, contiguous, optional :: mask(:,:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        if (present(mask)) then
+            s = c_minvaldd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
+        else
+            s = c_minvaldd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
         end if
         res%limbs = s%limbs
     end function
@@ -618,6 +775,72 @@ contains
         end if
     end function
 
+    pure function dd_product_2d_dim(array, dim, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:)
+        integer, intent(in) :: dim
+        logical, intent(in), target, contiguous, optional :: mask(:,:)
+        type(real64x2), target :: res(size(array, 1 + merge(1, 0, 1 >= dim)))
+        integer(c_int64_t) :: inner, outer
+        integer :: d_
+        inner = 1
+        do d_ = 1, dim - 1
+            inner = inner * size(array, d_)
+        end do
+        outer = 1
+        do d_ = dim + 1, 2
+            outer = outer * size(array, d_)
+        end do
+        if (present(mask)) then
+            call c_productdd_dim(c_loc(array), c_loc(mask), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        else
+            call c_productdd_dim(c_loc(array), c_null_ptr, inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        end if
+    end function
+
+    pure function dd_maxval_2d_dim(array, dim, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:)
+        integer, intent(in) :: dim
+        logical, intent(in), target, contiguous, optional :: mask(:,:)
+        type(real64x2), target :: res(size(array, 1 + merge(1, 0, 1 >= dim)))
+        integer(c_int64_t) :: inner, outer
+        integer :: d_
+        inner = 1
+        do d_ = 1, dim - 1
+            inner = inner * size(array, d_)
+        end do
+        outer = 1
+        do d_ = dim + 1, 2
+            outer = outer * size(array, d_)
+        end do
+        if (present(mask)) then
+            call c_maxvaldd_dim(c_loc(array), c_loc(mask), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        else
+            call c_maxvaldd_dim(c_loc(array), c_null_ptr, inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        end if
+    end function
+
+    pure function dd_minval_2d_dim(array, dim, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:)
+        integer, intent(in) :: dim
+        logical, intent(in), target, contiguous, optional :: mask(:,:)
+        type(real64x2), target :: res(size(array, 1 + merge(1, 0, 1 >= dim)))
+        integer(c_int64_t) :: inner, outer
+        integer :: d_
+        inner = 1
+        do d_ = 1, dim - 1
+            inner = inner * size(array, d_)
+        end do
+        outer = 1
+        do d_ = dim + 1, 2
+            outer = outer * size(array, d_)
+        end do
+        if (present(mask)) then
+            call c_minvaldd_dim(c_loc(array), c_loc(mask), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        else
+            call c_minvaldd_dim(c_loc(array), c_null_ptr, inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        end if
+    end function
+
     pure function dd_norm2_2d_dim(array, dim) result(res)
         type(real64x2), intent(in), target, contiguous :: array(:,:)
         integer, intent(in) :: dim
@@ -644,6 +867,45 @@ contains
             s = c_sumdd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
         else
             s = c_sumdd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
+        end if
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_product_3d(array, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:)
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        if (present(mask)) then
+            s = c_productdd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
+        else
+            s = c_productdd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
+        end if
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_maxval_3d(array, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:)
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        if (present(mask)) then
+            s = c_maxvaldd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
+        else
+            s = c_maxvaldd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
+        end if
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_minval_3d(array, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:)
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        if (present(mask)) then
+            s = c_minvaldd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
+        else
+            s = c_minvaldd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
         end if
         res%limbs = s%limbs
     end function
@@ -678,6 +940,72 @@ contains
         end if
     end function
 
+    pure function dd_product_3d_dim(array, dim, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:)
+        integer, intent(in) :: dim
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:)
+        type(real64x2), target :: res(size(array, 1 + merge(1, 0, 1 >= dim)), size(array, 2 + merge(1, 0, 2 >= dim)))
+        integer(c_int64_t) :: inner, outer
+        integer :: d_
+        inner = 1
+        do d_ = 1, dim - 1
+            inner = inner * size(array, d_)
+        end do
+        outer = 1
+        do d_ = dim + 1, 3
+            outer = outer * size(array, d_)
+        end do
+        if (present(mask)) then
+            call c_productdd_dim(c_loc(array), c_loc(mask), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        else
+            call c_productdd_dim(c_loc(array), c_null_ptr, inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        end if
+    end function
+
+    pure function dd_maxval_3d_dim(array, dim, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:)
+        integer, intent(in) :: dim
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:)
+        type(real64x2), target :: res(size(array, 1 + merge(1, 0, 1 >= dim)), size(array, 2 + merge(1, 0, 2 >= dim)))
+        integer(c_int64_t) :: inner, outer
+        integer :: d_
+        inner = 1
+        do d_ = 1, dim - 1
+            inner = inner * size(array, d_)
+        end do
+        outer = 1
+        do d_ = dim + 1, 3
+            outer = outer * size(array, d_)
+        end do
+        if (present(mask)) then
+            call c_maxvaldd_dim(c_loc(array), c_loc(mask), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        else
+            call c_maxvaldd_dim(c_loc(array), c_null_ptr, inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        end if
+    end function
+
+    pure function dd_minval_3d_dim(array, dim, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:)
+        integer, intent(in) :: dim
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:)
+        type(real64x2), target :: res(size(array, 1 + merge(1, 0, 1 >= dim)), size(array, 2 + merge(1, 0, 2 >= dim)))
+        integer(c_int64_t) :: inner, outer
+        integer :: d_
+        inner = 1
+        do d_ = 1, dim - 1
+            inner = inner * size(array, d_)
+        end do
+        outer = 1
+        do d_ = dim + 1, 3
+            outer = outer * size(array, d_)
+        end do
+        if (present(mask)) then
+            call c_minvaldd_dim(c_loc(array), c_loc(mask), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        else
+            call c_minvaldd_dim(c_loc(array), c_null_ptr, inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        end if
+    end function
+
     pure function dd_norm2_3d_dim(array, dim) result(res)
         type(real64x2), intent(in), target, contiguous :: array(:,:,:)
         integer, intent(in) :: dim
@@ -704,6 +1032,45 @@ contains
             s = c_sumdd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
         else
             s = c_sumdd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
+        end if
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_product_4d(array, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:)
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:,:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        if (present(mask)) then
+            s = c_productdd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
+        else
+            s = c_productdd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
+        end if
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_maxval_4d(array, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:)
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:,:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        if (present(mask)) then
+            s = c_maxvaldd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
+        else
+            s = c_maxvaldd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
+        end if
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_minval_4d(array, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:)
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:,:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        if (present(mask)) then
+            s = c_minvaldd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
+        else
+            s = c_minvaldd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
         end if
         res%limbs = s%limbs
     end function
@@ -738,6 +1105,72 @@ contains
         end if
     end function
 
+    pure function dd_product_4d_dim(array, dim, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:)
+        integer, intent(in) :: dim
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:,:)
+        type(real64x2), target :: res(size(array, 1 + merge(1, 0, 1 >= dim)), size(array, 2 + merge(1, 0, 2 >= dim)), size(array, 3 + merge(1, 0, 3 >= dim)))
+        integer(c_int64_t) :: inner, outer
+        integer :: d_
+        inner = 1
+        do d_ = 1, dim - 1
+            inner = inner * size(array, d_)
+        end do
+        outer = 1
+        do d_ = dim + 1, 4
+            outer = outer * size(array, d_)
+        end do
+        if (present(mask)) then
+            call c_productdd_dim(c_loc(array), c_loc(mask), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        else
+            call c_productdd_dim(c_loc(array), c_null_ptr, inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        end if
+    end function
+
+    pure function dd_maxval_4d_dim(array, dim, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:)
+        integer, intent(in) :: dim
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:,:)
+        type(real64x2), target :: res(size(array, 1 + merge(1, 0, 1 >= dim)), size(array, 2 + merge(1, 0, 2 >= dim)), size(array, 3 + merge(1, 0, 3 >= dim)))
+        integer(c_int64_t) :: inner, outer
+        integer :: d_
+        inner = 1
+        do d_ = 1, dim - 1
+            inner = inner * size(array, d_)
+        end do
+        outer = 1
+        do d_ = dim + 1, 4
+            outer = outer * size(array, d_)
+        end do
+        if (present(mask)) then
+            call c_maxvaldd_dim(c_loc(array), c_loc(mask), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        else
+            call c_maxvaldd_dim(c_loc(array), c_null_ptr, inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        end if
+    end function
+
+    pure function dd_minval_4d_dim(array, dim, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:)
+        integer, intent(in) :: dim
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:,:)
+        type(real64x2), target :: res(size(array, 1 + merge(1, 0, 1 >= dim)), size(array, 2 + merge(1, 0, 2 >= dim)), size(array, 3 + merge(1, 0, 3 >= dim)))
+        integer(c_int64_t) :: inner, outer
+        integer :: d_
+        inner = 1
+        do d_ = 1, dim - 1
+            inner = inner * size(array, d_)
+        end do
+        outer = 1
+        do d_ = dim + 1, 4
+            outer = outer * size(array, d_)
+        end do
+        if (present(mask)) then
+            call c_minvaldd_dim(c_loc(array), c_loc(mask), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        else
+            call c_minvaldd_dim(c_loc(array), c_null_ptr, inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        end if
+    end function
+
     pure function dd_norm2_4d_dim(array, dim) result(res)
         type(real64x2), intent(in), target, contiguous :: array(:,:,:,:)
         integer, intent(in) :: dim
@@ -764,6 +1197,45 @@ contains
             s = c_sumdd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
         else
             s = c_sumdd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
+        end if
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_product_5d(array, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:,:)
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:,:,:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        if (present(mask)) then
+            s = c_productdd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
+        else
+            s = c_productdd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
+        end if
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_maxval_5d(array, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:,:)
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:,:,:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        if (present(mask)) then
+            s = c_maxvaldd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
+        else
+            s = c_maxvaldd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
+        end if
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_minval_5d(array, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:,:)
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:,:,:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        if (present(mask)) then
+            s = c_minvaldd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
+        else
+            s = c_minvaldd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
         end if
         res%limbs = s%limbs
     end function
@@ -798,6 +1270,72 @@ contains
         end if
     end function
 
+    pure function dd_product_5d_dim(array, dim, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:,:)
+        integer, intent(in) :: dim
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:,:,:)
+        type(real64x2), target :: res(size(array, 1 + merge(1, 0, 1 >= dim)), size(array, 2 + merge(1, 0, 2 >= dim)), size(array, 3 + merge(1, 0, 3 >= dim)), size(array, 4 + merge(1, 0, 4 >= dim)))
+        integer(c_int64_t) :: inner, outer
+        integer :: d_
+        inner = 1
+        do d_ = 1, dim - 1
+            inner = inner * size(array, d_)
+        end do
+        outer = 1
+        do d_ = dim + 1, 5
+            outer = outer * size(array, d_)
+        end do
+        if (present(mask)) then
+            call c_productdd_dim(c_loc(array), c_loc(mask), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        else
+            call c_productdd_dim(c_loc(array), c_null_ptr, inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        end if
+    end function
+
+    pure function dd_maxval_5d_dim(array, dim, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:,:)
+        integer, intent(in) :: dim
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:,:,:)
+        type(real64x2), target :: res(size(array, 1 + merge(1, 0, 1 >= dim)), size(array, 2 + merge(1, 0, 2 >= dim)), size(array, 3 + merge(1, 0, 3 >= dim)), size(array, 4 + merge(1, 0, 4 >= dim)))
+        integer(c_int64_t) :: inner, outer
+        integer :: d_
+        inner = 1
+        do d_ = 1, dim - 1
+            inner = inner * size(array, d_)
+        end do
+        outer = 1
+        do d_ = dim + 1, 5
+            outer = outer * size(array, d_)
+        end do
+        if (present(mask)) then
+            call c_maxvaldd_dim(c_loc(array), c_loc(mask), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        else
+            call c_maxvaldd_dim(c_loc(array), c_null_ptr, inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        end if
+    end function
+
+    pure function dd_minval_5d_dim(array, dim, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:,:)
+        integer, intent(in) :: dim
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:,:,:)
+        type(real64x2), target :: res(size(array, 1 + merge(1, 0, 1 >= dim)), size(array, 2 + merge(1, 0, 2 >= dim)), size(array, 3 + merge(1, 0, 3 >= dim)), size(array, 4 + merge(1, 0, 4 >= dim)))
+        integer(c_int64_t) :: inner, outer
+        integer :: d_
+        inner = 1
+        do d_ = 1, dim - 1
+            inner = inner * size(array, d_)
+        end do
+        outer = 1
+        do d_ = dim + 1, 5
+            outer = outer * size(array, d_)
+        end do
+        if (present(mask)) then
+            call c_minvaldd_dim(c_loc(array), c_loc(mask), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        else
+            call c_minvaldd_dim(c_loc(array), c_null_ptr, inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        end if
+    end function
+
     pure function dd_norm2_5d_dim(array, dim) result(res)
         type(real64x2), intent(in), target, contiguous :: array(:,:,:,:,:)
         integer, intent(in) :: dim
@@ -824,6 +1362,45 @@ contains
             s = c_sumdd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
         else
             s = c_sumdd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
+        end if
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_product_6d(array, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:,:,:)
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:,:,:,:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        if (present(mask)) then
+            s = c_productdd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
+        else
+            s = c_productdd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
+        end if
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_maxval_6d(array, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:,:,:)
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:,:,:,:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        if (present(mask)) then
+            s = c_maxvaldd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
+        else
+            s = c_maxvaldd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
+        end if
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_minval_6d(array, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:,:,:)
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:,:,:,:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        if (present(mask)) then
+            s = c_minvaldd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
+        else
+            s = c_minvaldd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
         end if
         res%limbs = s%limbs
     end function
@@ -858,6 +1435,72 @@ contains
         end if
     end function
 
+    pure function dd_product_6d_dim(array, dim, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:,:,:)
+        integer, intent(in) :: dim
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(array, 1 + merge(1, 0, 1 >= dim)), size(array, 2 + merge(1, 0, 2 >= dim)), size(array, 3 + merge(1, 0, 3 >= dim)), size(array, 4 + merge(1, 0, 4 >= dim)), size(array, 5 + merge(1, 0, 5 >= dim)))
+        integer(c_int64_t) :: inner, outer
+        integer :: d_
+        inner = 1
+        do d_ = 1, dim - 1
+            inner = inner * size(array, d_)
+        end do
+        outer = 1
+        do d_ = dim + 1, 6
+            outer = outer * size(array, d_)
+        end do
+        if (present(mask)) then
+            call c_productdd_dim(c_loc(array), c_loc(mask), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        else
+            call c_productdd_dim(c_loc(array), c_null_ptr, inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        end if
+    end function
+
+    pure function dd_maxval_6d_dim(array, dim, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:,:,:)
+        integer, intent(in) :: dim
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(array, 1 + merge(1, 0, 1 >= dim)), size(array, 2 + merge(1, 0, 2 >= dim)), size(array, 3 + merge(1, 0, 3 >= dim)), size(array, 4 + merge(1, 0, 4 >= dim)), size(array, 5 + merge(1, 0, 5 >= dim)))
+        integer(c_int64_t) :: inner, outer
+        integer :: d_
+        inner = 1
+        do d_ = 1, dim - 1
+            inner = inner * size(array, d_)
+        end do
+        outer = 1
+        do d_ = dim + 1, 6
+            outer = outer * size(array, d_)
+        end do
+        if (present(mask)) then
+            call c_maxvaldd_dim(c_loc(array), c_loc(mask), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        else
+            call c_maxvaldd_dim(c_loc(array), c_null_ptr, inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        end if
+    end function
+
+    pure function dd_minval_6d_dim(array, dim, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:,:,:)
+        integer, intent(in) :: dim
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(array, 1 + merge(1, 0, 1 >= dim)), size(array, 2 + merge(1, 0, 2 >= dim)), size(array, 3 + merge(1, 0, 3 >= dim)), size(array, 4 + merge(1, 0, 4 >= dim)), size(array, 5 + merge(1, 0, 5 >= dim)))
+        integer(c_int64_t) :: inner, outer
+        integer :: d_
+        inner = 1
+        do d_ = 1, dim - 1
+            inner = inner * size(array, d_)
+        end do
+        outer = 1
+        do d_ = dim + 1, 6
+            outer = outer * size(array, d_)
+        end do
+        if (present(mask)) then
+            call c_minvaldd_dim(c_loc(array), c_loc(mask), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        else
+            call c_minvaldd_dim(c_loc(array), c_null_ptr, inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        end if
+    end function
+
     pure function dd_norm2_6d_dim(array, dim) result(res)
         type(real64x2), intent(in), target, contiguous :: array(:,:,:,:,:,:)
         integer, intent(in) :: dim
@@ -884,6 +1527,45 @@ contains
             s = c_sumdd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
         else
             s = c_sumdd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
+        end if
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_product_7d(array, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:,:,:,:)
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:,:,:,:,:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        if (present(mask)) then
+            s = c_productdd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
+        else
+            s = c_productdd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
+        end if
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_maxval_7d(array, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:,:,:,:)
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:,:,:,:,:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        if (present(mask)) then
+            s = c_maxvaldd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
+        else
+            s = c_maxvaldd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
+        end if
+        res%limbs = s%limbs
+    end function
+
+    pure function dd_minval_7d(array, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:,:,:,:)
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:,:,:,:,:)
+        type(real64x2) :: res
+        type(float64x2) :: s
+        if (present(mask)) then
+            s = c_minvaldd_mask(c_loc(array), c_loc(mask), int(size(array), c_int64_t))
+        else
+            s = c_minvaldd_mask(c_loc(array), c_null_ptr, int(size(array), c_int64_t))
         end if
         res%limbs = s%limbs
     end function
@@ -915,6 +1597,72 @@ contains
             call c_sumdd_dim(c_loc(array), c_loc(mask), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
         else
             call c_sumdd_dim(c_loc(array), c_null_ptr, inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        end if
+    end function
+
+    pure function dd_product_7d_dim(array, dim, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:,:,:,:)
+        integer, intent(in) :: dim
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(array, 1 + merge(1, 0, 1 >= dim)), size(array, 2 + merge(1, 0, 2 >= dim)), size(array, 3 + merge(1, 0, 3 >= dim)), size(array, 4 + merge(1, 0, 4 >= dim)), size(array, 5 + merge(1, 0, 5 >= dim)), size(array, 6 + merge(1, 0, 6 >= dim)))
+        integer(c_int64_t) :: inner, outer
+        integer :: d_
+        inner = 1
+        do d_ = 1, dim - 1
+            inner = inner * size(array, d_)
+        end do
+        outer = 1
+        do d_ = dim + 1, 7
+            outer = outer * size(array, d_)
+        end do
+        if (present(mask)) then
+            call c_productdd_dim(c_loc(array), c_loc(mask), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        else
+            call c_productdd_dim(c_loc(array), c_null_ptr, inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        end if
+    end function
+
+    pure function dd_maxval_7d_dim(array, dim, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:,:,:,:)
+        integer, intent(in) :: dim
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(array, 1 + merge(1, 0, 1 >= dim)), size(array, 2 + merge(1, 0, 2 >= dim)), size(array, 3 + merge(1, 0, 3 >= dim)), size(array, 4 + merge(1, 0, 4 >= dim)), size(array, 5 + merge(1, 0, 5 >= dim)), size(array, 6 + merge(1, 0, 6 >= dim)))
+        integer(c_int64_t) :: inner, outer
+        integer :: d_
+        inner = 1
+        do d_ = 1, dim - 1
+            inner = inner * size(array, d_)
+        end do
+        outer = 1
+        do d_ = dim + 1, 7
+            outer = outer * size(array, d_)
+        end do
+        if (present(mask)) then
+            call c_maxvaldd_dim(c_loc(array), c_loc(mask), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        else
+            call c_maxvaldd_dim(c_loc(array), c_null_ptr, inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        end if
+    end function
+
+    pure function dd_minval_7d_dim(array, dim, mask) result(res)
+        type(real64x2), intent(in), target, contiguous :: array(:,:,:,:,:,:,:)
+        integer, intent(in) :: dim
+        logical, intent(in), target, contiguous, optional :: mask(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(array, 1 + merge(1, 0, 1 >= dim)), size(array, 2 + merge(1, 0, 2 >= dim)), size(array, 3 + merge(1, 0, 3 >= dim)), size(array, 4 + merge(1, 0, 4 >= dim)), size(array, 5 + merge(1, 0, 5 >= dim)), size(array, 6 + merge(1, 0, 6 >= dim)))
+        integer(c_int64_t) :: inner, outer
+        integer :: d_
+        inner = 1
+        do d_ = 1, dim - 1
+            inner = inner * size(array, d_)
+        end do
+        outer = 1
+        do d_ = dim + 1, 7
+            outer = outer * size(array, d_)
+        end do
+        if (present(mask)) then
+            call c_minvaldd_dim(c_loc(array), c_loc(mask), inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
+        else
+            call c_minvaldd_dim(c_loc(array), c_null_ptr, inner, int(size(array, dim), c_int64_t), outer, c_loc(res))
         end if
     end function
 

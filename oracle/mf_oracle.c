@@ -407,6 +407,38 @@ void or_cdot(const dd *a, const dd *b, int64_t n, int64_t ri, dd *out) {
 /* fypp:2035-2090 (dd_sum_Nd_dim) on an (inner, len, outer) column-major view:
  * out(i, o) is the Neumaier chain over j (masked elements skipped), closed by
  * the full TwoSum in the reference's order (t, bp, ap, aerr, berr). */
+/* product / maxval / minval over a strided fibre: the reference's loops
+ * (fypp:1786-1810 maxval/minval, 2005-2016 product; the _dim forms
+ * 1825-1862, 2076-2087 run the same loop per output). kind 0 product
+ * (res = res * x from 1), 1 maxval, 2 minval (res = first element, replaced
+ * by x when x > res / x < res, lexicographic on (hi, lo), fypp:2960-2998;
+ * 0 when nothing is selected). Masked-out elements are skipped. */
+static int o_gt(dd x, dd y) { return y.hi < x.hi || (x.hi == y.hi && y.lo < x.lo); }
+static dd o_fibre(int kind, const dd *a, const int32_t *mask, int64_t start, int64_t stride, int64_t len) {
+  dd res = mkdd(kind == 0 ? 1.0 : 0.0, 0.0);
+  int have = 0;
+  for (int64_t j = 0; j < len; ++j) {
+    int64_t e = start + stride * j;
+    if (mask && !mask[e]) continue;
+    if (kind == 0) {
+      res = o_mul(res, a[e]);
+    } else if (!have) {
+      res = a[e];
+      have = 1;
+    } else if (kind == 1 ? o_gt(a[e], res) : o_gt(res, a[e])) {
+      res = a[e];
+    }
+  }
+  return res;
+}
+void or_fibre_reduce(int kind, const dd *a, const int32_t *mask, int64_t n, dd *out) {
+  *out = o_fibre(kind, a, mask, 0, 1, n);
+}
+void or_fibre_reduce_dim(int kind, const dd *a, const int32_t *mask, int64_t inner, int64_t len, int64_t outer,
+                         dd *out) {
+  for (int64_t o = 0; o < outer; ++o)
+    for (int64_t i = 0; i < inner; ++i) out[i + inner * o] = o_fibre(kind, a, mask, i + inner * len * o, inner, len);
+}
 void or_sum_dim(const dd *a, const int32_t *mask, int64_t inner, int64_t len, int64_t outer, dd *out) {
   for (int64_t o = 0; o < outer; ++o)
     for (int64_t i = 0; i < inner; ++i) {

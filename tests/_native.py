@@ -51,6 +51,8 @@ def oracle():
         L.or_cdot.argtypes = [P, P, I64, I64, P]
         L.or_sum_dim.argtypes = [P, P, I64, I64, I64, P]
         L.or_norm2_dim.argtypes = [P, I64, I64, I64, P]
+        L.or_fibre_reduce.argtypes = [ctypes.c_int, P, P, I64, P]
+        L.or_fibre_reduce_dim.argtypes = [ctypes.c_int, P, P, I64, I64, I64, P]
         L.or_gemm_blas.argtypes = [ctypes.c_char, ctypes.c_char, I64, I64, I64, P, P, I64, P, I64, P, P, I64,
                                    I64]
         L.or_gemm_blas.restype = ctypes.c_int
@@ -217,6 +219,27 @@ def or_reduce_dim(kind, buf, mask, inner, length, outer):
     else:
         oracle().or_norm2_dim(ptr(buf), inner, length, outer, ptr(out))
     return out[:inner * outer]
+
+
+FIBRE_KIND = {"product": 0, "maxval": 1, "minval": 2}
+
+
+def or_fibre(kind, x, mask=None):
+    """The oracle's sequential product / maxval / minval of a rank-1 array."""
+    x = dd(x)
+    m = None if mask is None else np.ascontiguousarray(mask, dtype=np.int32)
+    out = np.zeros(2)
+    oracle().or_fibre_reduce(FIBRE_KIND[kind], ptr(x), None if m is None else ptr(m), x.shape[0], ptr(out))
+    return out
+
+
+def or_fibre_dim(kind, buf, mask, inner, length, outer):
+    buf = dd(buf)
+    m = None if mask is None else np.ascontiguousarray(mask, dtype=np.int32)
+    out = np.zeros((inner * outer, 2))
+    oracle().or_fibre_reduce_dim(FIBRE_KIND[kind], ptr(buf), None if m is None else ptr(m), inner, length, outer,
+                                 ptr(out))
+    return out
 
 
 def bits_equal(x, y):

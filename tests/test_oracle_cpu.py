@@ -288,3 +288,78 @@ def test_oracle_complex_dot():
     slip_im = nt.res(O.or_dot(nt.ptr(nt.dd(ar)), nt.ptr(nt.dd(bi)), n, 8))
     e, m = nt.exact_dot_err(np.concatenate([ar, -ai]), np.concatenate([bi, br]), slip_im)
     assert e > 4 * n * 2.0 ** -104 * m
+
+
+def _lex_pick(a, idx, best):
+    """First index of the lexicographic (hi, lo) max / min among idx (Python's
+    max/min return the first extremal element)."""
+    if not idx:
+        return None
+    return best(idx, key=lambda i: (a[i, 0], a[i, 1]))
+
+
+def test_oracle_maxval_minval_are_lexicographic_extrema():
+    rng = np.random.default_rng(31)
+    for trial in range(20):
+        n = int(rng.integers(0, 300))
+        a = datagen.config0(rng, n)
+        if trial % 2:  # many ties, signed zeros in both limbs
+            a = rng.choice(np.array([[1.0, 0.0], [1.0, 1e-17], [-0.0, 0.0], [0.0, -0.0], [0.0, 0.0], [2.0, -1e-16]]),
+                           size=n)
+        mask = rng.random(n) < 0.6
+        for kind, best in (("maxval", max), ("minval", min)):
+            for m in (None, mask):
+                idx = [i for i in range(n) if m is None or m[i]]
+                k = _lex_pick(a, idx, best)
+                want = np.zeros(2) if k is None else a[k]
+                nt.assert_bits(nt.or_fibre(kind, a, m), want, kind)
+    # +0 / -0 tie: the first one stays
+    z = np.array([[-0.0, 0.0], [0.0, 0.0]])
+    assert np.signbit(nt.or_fibre("maxval", z)[0]) and np.signbit(nt.or_fibre("minval", z)[0])
+
+
+def test_oracle_extrema_nan_rules():
+    """A NaN hi limb first: the result is that element; anywhere else it is
+    never selected (removing it changes nothing)."""
+    rng = np.random.default_rng(32)
+    a = datagen.config0(rng, 50)
+    b = a.copy()
+    b[17] = (np.nan, 0.0)
+    b[33] = (np.nan, 1.0)
+    keep = [i for i in range(50) if i not in (17, 33)]
+    for kind in ("maxval", "minval"):
+        nt.assert_bits(nt.or_fibre(kind, b), nt.or_fibre(kind, a[keep]), kind)
+        c = b.copy()
+        c[0] = (np.nan, 5.0)
+        r = nt.or_fibre(kind, c)
+        assert np.isnan(r[0]) and r[1] == 5.0
+
+
+def test_oracle_product_within_bound_of_exact():
+    rng = np.random.default_rng(33)
+    for n in (1, 2, 7, 40):
+        a = datagen.config0(rng, n) + np.array([1.5, 0.0])  # away from 0, no cancellation
+        exact = Fraction(1)
+        for x in a:
+            exact *= Fraction(float(x[0])) + Fraction(float(x[1]))
+        r = nt.or_fibre("product", a)
+        got = Fraction(float(r[0])) + Fraction(float(r[1]))
+        assert abs(got - exact) <= 4 * n * Fraction(2) ** -104 * abs(exact)
+    p2 = np.array([[2.0 ** k, 0.0] for k in (-3, 5, 11, -7)])
+    nt.assert_bits(nt.or_fibre("product", p2), np.array([2.0 ** 6, 0.0]), "powers of two")
+    assert (nt.or_fibre("product", np.zeros((0, 2))) == np.array([1.0, 0.0])).all()
+
+
+def test_oracle_fibre_dim_matches_rank1():
+    rng = np.random.default_rng(34)
+    inner, length, outer = 3, 11, 4
+    buf = datagen.config0(rng, inner * length * outer)
+    mask = (rng.random(inner * length * outer) < 0.5).astype(np.int32)
+    for kind in ("product", "maxval", "minval"):
+        for m in (None, mask):
+            out = nt.or_fibre_dim(kind, buf, m, inner, length, outer)
+            for o in range(outer):
+                for i in range(inner):
+                    sel = [i + inner * (j + length * o) for j in range(length)]
+                    want = nt.or_fibre(kind, buf[sel], None if m is None else m[sel])
+                    nt.assert_bits(out[i + inner * o], want, kind)
