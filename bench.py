@@ -420,8 +420,12 @@ def bench_elementwise(args, torch, mf):
     a.abs_()
     ms = timed(torch, lambda: mf.dev_unary(mf.MFHIP_SQRT, a.data_ptr(), c.data_ptr(), n, stream), args.steps, args.warmup)
     res["sqrt"] = dict(ms=round(ms, 4), GBs=round(32.0 * n / ms / 1e6, 1))
-    gens = {"exp": datagen.exp_inputs, "log": datagen.log_inputs, "sin": datagen.sin_inputs, "erf": datagen.erf_inputs}
-    ops = {"exp": mf.MFHIP_EXP, "log": mf.MFHIP_LOG, "sin": mf.MFHIP_SIN, "erf": mf.MFHIP_ERF}
+    gens = {"exp": datagen.exp_inputs, "log": datagen.log_inputs, "sin": datagen.sin_inputs, "erf": datagen.erf_inputs,
+            "cos": datagen.sin_inputs, "tan": datagen.sin_inputs, "exp2": datagen.exp_inputs,
+            "expm1": datagen.config0, "log2": datagen.log_inputs, "log10": datagen.log_inputs,
+            "log1p": datagen.config0, "sinh": datagen.erf_inputs, "cosh": datagen.erf_inputs,
+            "tanh": datagen.erf_inputs}
+    ops = {nm: getattr(mf, "MFHIP_" + nm.upper()) for nm in gens}
     for nm, gen in gens.items():
         x = torch.from_numpy(gen(rng, n)).cuda()
         ms = timed(torch, lambda: mf.dev_unary(ops[nm], x.data_ptr(), c.data_ptr(), n, stream), args.steps, args.warmup)
@@ -446,14 +450,16 @@ def bench_reduce(args, torch, mf):
     work = torch.empty(mf.reduce_workspace_bytes(n) // 8 + 2, dtype=torch.float64, device="cuda")
     stream = torch.cuda.current_stream().cuda_stream
     res = {}
-    for nm, kind, bytes_per in (("dot", mf.MFHIP_DOT, 32), ("sum", mf.MFHIP_SUM, 16), ("norm2", mf.MFHIP_NORM2, 16)):
+    for nm, kind, bytes_per in (("dot", mf.MFHIP_DOT, 32), ("sum", mf.MFHIP_SUM, 16), ("norm2", mf.MFHIP_NORM2, 16),
+                                ("product", mf.MFHIP_PRODUCT, 16), ("maxval", mf.MFHIP_MAXVAL, 16),
+                                ("minval", mf.MFHIP_MINVAL, 16)):
         ms = timed(torch, lambda: mf.dev_reduce(kind, a.data_ptr(), b.data_ptr(), n, out.data_ptr(),
                                                 work.data_ptr(), 8, stream), args.steps, args.warmup)
         res[nm] = dict(ms=round(ms, 4), GBs=round(bytes_per * n / ms / 1e6, 1))
     best = res["dot"]["GBs"]
     return {"metric": "DD dot_product N=2^26", "value": best, "unit": "GB/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "dtype": "f64",
-            "data": "synthetic", "config": {"workload": "dot/sum/norm2 float64x2 N=2^26"},
+            "data": "synthetic", "config": {"workload": "dot/sum/norm2/product/maxval/minval float64x2 N=2^26"},
             "roofline": {"bound": "hbm", "achieved": best, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(best / HBM_PEAK_GBS, 4), "traffic": None}, "ops": res}
 
