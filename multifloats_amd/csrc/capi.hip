@@ -446,6 +446,8 @@ MF_UNARY(log1pdd_v, mfhip::OP_LOG1P)
 MF_UNARY(sinhdd_v, mfhip::OP_SINH)
 MF_UNARY(coshdd_v, mfhip::OP_COSH)
 MF_UNARY(tanhdd_v, mfhip::OP_TANH)
+MF_UNARY(erfcdd_v, mfhip::OP_ERFC)
+MF_UNARY(erfcxdd_v, mfhip::OP_ERFCX)
 
 float64x2 sumdd(const float64x2 *a, int64_t n) {
   float64x2 r;

@@ -14,11 +14,10 @@
 //   estrin10/12/17/24  src/float64x2/poly.inc:33-217 (neval's trees)
 // The constant tables are closed forms produced by gen/gen_tables.py.
 //
-// dd_erf is this project's own design, not the reference's rational fits
-// (src/float64x2/special.inc:18-139): one branch-free piecewise Taylor
-// evaluation whose coefficient row is picked per lane, so a wavefront never
-// diverges between regimes. Its accuracy target is the reference's own
-// measured max relative error against a 200-bit oracle.
+// dd_erf, dd_erfc and dd_erfcx are this project's own design, not the
+// reference's rational fits (src/float64x2/special.inc:18-188): piecewise
+// Taylor evaluations whose coefficient row is picked per lane. Their accuracy
+// target is the reference's own error on the same inputs.
 #pragma once
 #include "dd_core.h"
 #include "dd_tables.h"
@@ -363,6 +362,30 @@ MF_HD ddv dd_tan(ddv x) {
 }
 
 // ---------------------------------------------------------------- erf
+// base (three limbs) + corr, rounded once to a double-double
+MF_HD ddv sum3_round(double b0, double b1, double b2, ddv corr) {
+  double s, e1, u1, v1, w, z;
+  two_sum(b0, corr.hi, s, e1);
+  two_sum(b1, corr.lo, u1, v1);
+  two_sum(e1, u1, w, z);
+  double lo = w + (z + (v1 + b2));
+  ddv r;
+  fast_two_sum(s, lo, r.hi, r.lo);
+  return r;
+}
+
+// P(u) of erf's row (ND double-double coefficients, the tail in double)
+MF_HD ddv erf_poly(int row, ddv u) {
+  const double *dtail = mf_erf_d + row * (MF_ERF_NT - MF_ERF_ND);
+  double d = dtail[MF_ERF_NT - MF_ERF_ND - 1];
+  for (int j = MF_ERF_NT - MF_ERF_ND - 2; j >= 0; --j) d = __builtin_fma(d, u.hi, dtail[j]);
+  const double *ph = mf_erf_ddhi + row * MF_ERF_ND;
+  const double *pl = mf_erf_ddlo + row * MF_ERF_ND;
+  ddv p = dd_add(mk(ph[MF_ERF_ND - 1], pl[MF_ERF_ND - 1]), dd_mul(u, mk(d, 0.0)));
+  for (int j = MF_ERF_ND - 2; j >= 0; --j) p = dd_add(mk(ph[j], pl[j]), dd_mul(u, p));
+  return p;
+}
+
 // Row 0 (|x| < 1/4):   erf(x) = x + x * Q(x^2)
 // Row i (1/4 <= |x| < 9): erf(c_i + t) = a0 + t * P(t), |t| <= 1/32
 // P and Q share one evaluator: the first MF_ERF_ND coefficients are
@@ -402,20 +425,134 @@ MF_HD ddv dd_erf(ddv x) {
     b1 = mf_erf_base[3 * row + 1];
     b2 = mf_erf_base[3 * row + 2];
   }
-  const double *dtail = mf_erf_d + row * (MF_ERF_NT - MF_ERF_ND);
-  double d = dtail[MF_ERF_NT - MF_ERF_ND - 1];
-  for (int j = MF_ERF_NT - MF_ERF_ND - 2; j >= 0; --j) d = __builtin_fma(d, u.hi, dtail[j]);
-  const double *ph = mf_erf_ddhi + row * MF_ERF_ND;
-  const double *pl = mf_erf_ddlo + row * MF_ERF_ND;
-  ddv p = dd_add(mk(ph[MF_ERF_ND - 1], pl[MF_ERF_ND - 1]), dd_mul(u, mk(d, 0.0)));
-  for (int j = MF_ERF_ND - 2; j >= 0; --j) p = dd_add(mk(ph[j], pl[j]), dd_mul(u, p));
-  ddv corr = dd_mul(mult, p);
-  double s, e1, u1, v1, w, z;
-  two_sum(b0, corr.hi, s, e1);
-  two_sum(b1, corr.lo, u1, v1);
-  two_sum(e1, u1, w, z);
-  double lo = w + (z + (v1 + b2));
-  ddv r;
-  fast_two_sum(s, lo, r.hi, r.lo);
+  ddv r = sum3_round(b0, b1, b2, dd_mul(mult, erf_poly(row, u)));
   return neg ? dd_neg(r) : r;
+}
+
+// ---------------------------------------------------------------- erfc, erfcx
+// This project's own design (the reference uses libquadmath's rational fits,
+// src/float64x2/special.inc:56-188), with the reference's regime split:
+//   erfc(x), x >= 0:
+//     x < 1/4       1 - erf(x) from erf's row 0, base 1 - x as three limbs;
+//     1/4 <= x < 4  Taylor rows of erfc itself (all coefficients double-
+//                   double, relative accuracy), base erfc(c) in three limbs;
+//     4 <= x < 27.3 exp(-x^2) * erfcx(x), x^2 = s^2 + (x - s)(x + s) with s the
+//                   leading 26 bits of x (s^2 exact), exp(-(x-s)(x+s)) by its
+//                   series; beyond 27.3 erfc underflows to 0;
+//   erfc(x), x < 0: 2 - erfc(-x) (1 + erf(-x) below 1/4);
+//   erfcx(x), x >= -2: Taylor rows of erfcx (ODE-generated coefficients) up
+//                   to 32, then the asymptotic series in 1/(2x^2);
+//   erfcx(x), x < -2: 2 exp(x^2) - erfcx(-x), +inf past exp's overflow.
+// Each row result is rounded once from a three-limb base plus a DD
+// correction, as erf.
+
+// 1/16-wide rows of erfcx on [0, 4), 32 rows per binade on [4, 32), and
+// rows 160..191 for [-2, 0).
+MF_HD int erfcx_row(double x) {
+  if (x < 0.0) {
+    int r = (int)(-x * 16.0);
+    return 160 + (r > 31 ? 31 : r);
+  }
+  if (x < 4.0) {
+    int r = (int)(x * 16.0);
+    return r > 63 ? 63 : r;
+  }
+  int fe;
+  const double m = __builtin_frexp(x, &fe);  // x = m 2^fe, m in [1/2, 1)
+  int j = (int)((2.0 * m - 1.0) * 32.0);
+  j = j > 31 ? 31 : j;
+  return 64 + 32 * (fe - 3) + j;
+}
+
+// DD Horner over nt DD coefficients c1..c_nt: c1 + c2 t + ... (all DD)
+MF_HD ddv horner_dd(ddv t, const double *hi, const double *lo, int nt) {
+  ddv p = mk(hi[nt - 1], lo[nt - 1]);
+  for (int j = nt - 2; j >= 0; --j) p = dd_add(mk(hi[j], lo[j]), dd_mul(t, p));
+  return p;
+}
+
+// t = x - c as a double-double (x.hi - c taken exactly by TwoSum: near 0 the
+// centre can exceed 2 x.hi)
+MF_HD ddv row_offset(ddv x, double c) {
+  double th, te;
+  two_sum(x.hi, -c, th, te);
+  double tl = te + x.lo;
+  fast_two_sum(th, tl, th, tl);
+  return mk(th, tl);
+}
+
+// exp(sg * x^2) for x >= 0, sg = +1 or -1: x^2 = s^2 + e with s the top 26
+// significand bits of x.hi (s^2 exact) and e = (x - s)(x + s) in DD, so
+// exp(sg x^2) = exp(sg s^2) * exp(sg e), the second by its degree-8 series.
+MF_HD ddv exp_pm_sq(ddv x, double sg) {
+  const double s = __builtin_bit_cast(double, __builtin_bit_cast(long long, x.hi) & ~((1LL << 27) - 1));
+  double dh = x.hi - s, dl = x.lo;
+  fast_two_sum(dh, dl, dh, dl);
+  const ddv e = dd_mul(mk(dh, dl), dd_add(x, mk(s, 0.0)));
+  const ddv y = mk(sg * e.hi, sg * e.lo);
+  ddv q = cdd(mf_expm1c_hi, mf_expm1c_lo, 7);
+  for (int k = 6; k >= 0; --k) q = dd_add(cdd(mf_expm1c_hi, mf_expm1c_lo, k), dd_mul(y, q));
+  const ddv e1 = dd_exp(mk(sg * (s * s), 0.0));
+  return dd_add(e1, dd_mul(e1, dd_mul(y, q)));  // e1 (1 + expm1(sg e)), one rounding at the end
+}
+
+// erfcx(x) for finite x >= -2
+MF_HD ddv erfcx_pos(ddv x) {
+  if (x.hi >= 32.0) {
+    const ddv u = dd_div(mk(0.5, 0.0), dd_mul(x, x));  // 1/(2x^2)
+    ddv sum = mk(1.0, 0.0);
+    for (int k = MF_ERFCX_ASYM - 1; k >= 0; --k)
+      sum = dd_sub(mk(1.0, 0.0), dd_mul(mk(2.0 * k + 1.0, 0.0), dd_mul(u, sum)));
+    return dd_div(dd_mul(sum, mk(mf_inv_sqrtpi_hi, mf_inv_sqrtpi_lo)), x);
+  }
+  const int row = erfcx_row(x.hi);
+  const ddv t = row_offset(x, mf_erfcx_centre[row]);
+  const ddv p = horner_dd(t, mf_erfcx_hi + row * MF_ERFCX_NT, mf_erfcx_lo + row * MF_ERFCX_NT, MF_ERFCX_NT);
+  const double *b = mf_erfcx_base + 3 * row;
+  return sum3_round(b[0], b[1], b[2], dd_mul(t, p));
+}
+
+// erfc(x) for finite x >= 1/4
+MF_HD ddv erfc_pos(ddv x) {
+  if (x.hi >= 27.3) return mk(0.0, 0.0);
+  if (x.hi >= 4.0) return dd_mul(exp_pm_sq(x, -1.0), erfcx_pos(x));
+  int row = (int)((x.hi - mf_erf_x0) * mf_erf_inv_h);
+  row = row > MF_ERFC_ROWS - 1 ? MF_ERFC_ROWS - 1 : row;
+  const ddv t = row_offset(x, mf_erfc_centre[row]);
+  const ddv p = horner_dd(t, mf_erfc_hi + row * MF_ERFC_NT, mf_erfc_lo + row * MF_ERFC_NT, MF_ERFC_NT);
+  const double *b = mf_erfc_base + 3 * row;
+  return sum3_round(b[0], b[1], b[2], dd_mul(t, p));
+}
+
+MF_HD ddv dd_erfc(ddv x) {
+  if (!mf_isfinite(x.hi)) {
+    if (x.hi != x.hi) return mk(x.hi, 0.0);
+    return mk(x.hi > 0.0 ? 0.0 : 2.0, 0.0);
+  }
+  const bool neg = x.hi < 0.0;
+  const ddv ax = neg ? dd_neg(x) : x;
+  if (ax.hi < mf_erf_x0) {
+    // 1 - erf(x) = (1 - x) - x Q(x^2): the base 1 - x kept exactly in three limbs
+    const double sg = neg ? -1.0 : 1.0;
+    double b0, e0, b1, b2;
+    two_sum(1.0, -sg * ax.hi, b0, e0);
+    two_sum(e0, -sg * ax.lo, b1, b2);
+    const ddv c = dd_mul(ax, erf_poly(0, dd_mul(ax, ax)));
+    return sum3_round(b0, b1, b2, mk(-sg * c.hi, -sg * c.lo));
+  }
+  const ddv r = erfc_pos(ax);
+  return neg ? dd_sub(mk(2.0, 0.0), r) : r;
+}
+
+MF_HD ddv dd_erfcx(ddv x) {
+  if (!mf_isfinite(x.hi)) {
+    if (x.hi != x.hi) return mk(x.hi, 0.0);
+    return mk(x.hi > 0.0 ? 0.0 : __builtin_inf(), 0.0);
+  }
+  if (x.hi >= -2.0) return erfcx_pos(x);
+  const ddv ax = dd_neg(x);
+  if (ax.hi > 26.7) return mk(__builtin_inf(), 0.0);  // exp(x^2) overflows
+  const ddv ex2 = exp_pm_sq(ax, 1.0);
+  if (!mf_isfinite(ex2.hi)) return mk(__builtin_inf(), 0.0);
+  return dd_sub(dd_mul(mk(2.0, 0.0), ex2), erfcx_pos(ax));
 }

@@ -37,6 +37,8 @@ __device__ __forceinline__ ddv unary_op(ddv x) {
   else if constexpr (OP == mfhip::OP_SINH) return dd_sinh(x);
   else if constexpr (OP == mfhip::OP_COSH) return dd_cosh(x);
   else if constexpr (OP == mfhip::OP_TANH) return dd_tanh(x);
+  else if constexpr (OP == mfhip::OP_ERFC) return dd_erfc(x);
+  else if constexpr (OP == mfhip::OP_ERFCX) return dd_erfcx(x);
   else return x;
 }
 
@@ -176,6 +178,8 @@ hipError_t unary(int op, const double2 *a, double2 *c, int64_t n, hipStream_t s)
     case OP_SINH: return run_unary<OP_SINH>(a, c, n, s);
     case OP_COSH: return run_unary<OP_COSH>(a, c, n, s);
     case OP_TANH: return run_unary<OP_TANH>(a, c, n, s);
+    case OP_ERFC: return run_unary<OP_ERFC>(a, c, n, s);
+    case OP_ERFCX: return run_unary<OP_ERFCX>(a, c, n, s);
     default: return hipErrorInvalidValue;
   }
 }

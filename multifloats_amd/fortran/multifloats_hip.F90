@@ -13,6 +13,7 @@
 !   maxval minval   dd_maxval_Nd / dd_minval_Nd (+ _dim) 1783-1862 (with mask)
 !   norm2           dd_norm2_Nd / dd_norm2_Nd_dim 2338-2377
 !   exp log sin erf sqrt cos tan expm1 log2 log10 log1p sinh cosh tanh
+!   erfc erfc_scaled
 !                   elemental delegates 709-736, 1407-1412 (names fypp:57)
 !   + - * /         elemental native operators 2746-2837
 !   DD_FMA_RENORM_INTERVAL and dd_set_fma_renorm_interval 664, 2677-2680
@@ -61,6 +62,7 @@ module multifloats_hip
     public :: wgemm
     public :: matmul, dot_product, sum, product, maxval, minval, norm2
     public :: exp, log, sin, erf, sqrt, cos, tan, expm1, log2, log10, log1p, sinh, cosh, tanh
+    public :: erfc, erfc_scaled
     public :: operator(+), operator(-), operator(*), operator(/)
 
     interface
@@ -238,6 +240,16 @@ module multifloats_hip
             integer(c_int64_t), value :: n
         end subroutine
         pure subroutine c_tanhdd_v(a, c, n) bind(c, name='tanhdd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_erfcdd_v(a, c, n) bind(c, name='erfcdd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_erfcxdd_v(a, c, n) bind(c, name='erfcxdd_v')
             import :: c_ptr, c_int64_t
             type(c_ptr), value :: a, c
             integer(c_int64_t), value :: n
@@ -489,6 +501,26 @@ module multifloats_hip
         module procedure dd_tanh_5d
         module procedure dd_tanh_6d
         module procedure dd_tanh_7d
+    end interface
+
+    interface erfc
+        module procedure dd_erfc_1d
+        module procedure dd_erfc_2d
+        module procedure dd_erfc_3d
+        module procedure dd_erfc_4d
+        module procedure dd_erfc_5d
+        module procedure dd_erfc_6d
+        module procedure dd_erfc_7d
+    end interface
+
+    interface erfc_scaled
+        module procedure dd_erfc_scaled_1d
+        module procedure dd_erfc_scaled_2d
+        module procedure dd_erfc_scaled_3d
+        module procedure dd_erfc_scaled_4d
+        module procedure dd_erfc_scaled_5d
+        module procedure dd_erfc_scaled_6d
+        module procedure dd_erfc_scaled_7d
     end interface
 
     interface operator(+)
@@ -2269,6 +2301,90 @@ contains
         type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:,:)
         type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6), size(x,7))
         call c_tanhdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_erfc_1d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:)
+        type(real64x2), target :: res(size(x,1))
+        call c_erfcdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_erfc_2d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2))
+        call c_erfcdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_erfc_3d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3))
+        call c_erfcdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_erfc_4d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4))
+        call c_erfcdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_erfc_5d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5))
+        call c_erfcdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_erfc_6d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6))
+        call c_erfcdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_erfc_7d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6), size(x,7))
+        call c_erfcdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_erfc_scaled_1d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:)
+        type(real64x2), target :: res(size(x,1))
+        call c_erfcxdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_erfc_scaled_2d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2))
+        call c_erfcxdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_erfc_scaled_3d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3))
+        call c_erfcxdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_erfc_scaled_4d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4))
+        call c_erfcxdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_erfc_scaled_5d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5))
+        call c_erfcxdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_erfc_scaled_6d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6))
+        call c_erfcxdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_erfc_scaled_7d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6), size(x,7))
+        call c_erfcxdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
     end function
 
     pure function dd_add_1d(a, b) result(res)

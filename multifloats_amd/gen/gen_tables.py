@@ -18,13 +18,20 @@ fixed by mathematics (so a bit-exact restatement must use the same values):
     coefficients 1/(k+1)!, 2/(2k+1) and 1/(2k+1)! (exp_log.inc:119-128,
     343-353, 407-469; hyp.inc:2-18).
 
-The erf table is this project's own design (the reference fits erf with
+The erf, erfc and erfcx tables are this project's own design (the reference fits erf with
 rational approximations; see csrc/dd_math.h for the piecewise Taylor scheme):
   - row 0:   erf(x) = x + x*Q(x^2) for |x| < 1/4, Q(z) = (2/sqrt(pi) - 1)
              + sum_{n>=1} (2/sqrt(pi)) (-1)^n z^n / (n! (2n+1));
   - row i>0: Taylor expansion of erf about the centre c_i = 1/4 + (i - 1/2)/16,
              erf(c_i + t) = a0 + t*(a1 + a2 t + ...), |t| <= 1/32, with a0
-             stored as three limbs.
+             stored as three limbs;
+  - erfc on [1/4, 4): Taylor rows of erfc about the same centres, all
+             coefficients double-double (relative accuracy for small erfc);
+  - erfcx = exp(x^2) erfc(x): Taylor rows about (i + 1/2)/16 on [0, 4),
+             32 centres per binade on [4, 32) and -(i + 1/2)/16 on [-2, 0),
+             coefficients from the ODE
+             y' = 2xy - 2/sqrt(pi) at 450 bits; 1/sqrt(pi) for the
+             asymptotic series beyond 32.
 
 Run:  python3 multifloats_amd/gen/gen_tables.py   (rewrites csrc/dd_tables.h)
 """
@@ -42,6 +49,12 @@ ERF_X0 = mpf(1) / 4
 ERF_H = mpf(1) / 16
 ERF_XMAX = 9      # |x| >= 9 rounds to +-1 (erfc(9) < 2^-121)
 ERF_ROWS = 1 + int((ERF_XMAX - ERF_X0) / ERF_H)
+
+ERFC_NT = 22      # DD Taylor coefficients per erfc row on [1/4, 4)
+ERFC_ROWS = 60    # centres 1/4 + (i - 1/2)/16, i = 1..60
+ERFCX_NT = 20     # DD Taylor coefficients per erfcx row
+ERFCX_ROWS = 192  # 64 rows of width 1/16 on [0, 4), 32 per binade on [4, 32), 32 on [-2, 0)
+ERFCX_ASYM = 16   # terms of the asymptotic series for x >= 32
 
 
 def limbs(v, n=2):
@@ -194,6 +207,60 @@ def main():
     out.append(arr("mf_erf_ddhi", rows_dd_hi))
     out.append(arr("mf_erf_ddlo", rows_dd_lo))
     out.append(arr("mf_erf_d", rows_d))
+
+    # ---- erfc on [1/4, 4): Taylor rows of erfc itself (relative accuracy) ----
+    rtpi = 2 / mpmath.sqrt(pi)
+    ec_centre, ec_base, ec_hi, ec_lo = [], [], [], []
+    for i in range(1, ERFC_ROWS + 1):
+        c0 = ERF_X0 + (i - mpf(1) / 2) * ERF_H
+        g = -rtpi * mpmath.exp(-c0 * c0)
+        ec_centre.append(float(c0))
+        ec_base.extend(limbs(mpmath.erfc(c0), 3))
+        for n in range(1, ERFC_NT + 1):
+            v = g * (-1) ** (n - 1) * mpmath.hermite(n - 1, c0) / mpmath.factorial(n)
+            h, l = limbs(v)
+            ec_hi.append(h)
+            ec_lo.append(l)
+    out.append("\n/* erfc(c + t) = base + t*(a1 + a2 t + ... + a%d t^%d) about c = 1/4 + (i - 1/2)/16,\n"
+               "   i = 1..%d, |t| <= 1/32; a_n = erfc^(n)(c)/n! */\n" % (ERFC_NT, ERFC_NT - 1, ERFC_ROWS))
+    out.append("#define MF_ERFC_NT %d\n#define MF_ERFC_ROWS %d\n" % (ERFC_NT, ERFC_ROWS))
+    out.append(arr("mf_erfc_centre", ec_centre))
+    out.append(arr("mf_erfc_base", ec_base))
+    out.append(arr("mf_erfc_hi", ec_hi))
+    out.append(arr("mf_erfc_lo", ec_lo))
+
+    # ---- erfcx = exp(x^2) erfc(x): Taylor rows from y' = 2xy - 2/sqrt(pi) ----
+    def erfcx_coeffs(c0, nmax):
+        y = [mpmath.exp(c0 * c0) * mpmath.erfc(c0)]
+        y.append(2 * c0 * y[0] - rtpi)
+        for n in range(1, nmax):
+            y.append((2 * c0 * y[n] + 2 * y[n - 1]) / (n + 1))
+        return y
+    centres_x = [(mpf(i) + mpf(1) / 2) / 16 for i in range(64)]
+    for e in (2, 3, 4):
+        w = mpf(2) ** e / 32
+        centres_x += [mpf(2) ** e + (j + mpf(1) / 2) * w for j in range(32)]
+    centres_x += [-(mpf(i) + mpf(1) / 2) / 16 for i in range(32)]
+    assert len(centres_x) == ERFCX_ROWS
+    ex_base, ex_hi, ex_lo = [], [], []
+    for c0 in centres_x:
+        y = erfcx_coeffs(c0, ERFCX_NT)
+        ex_base.extend(limbs(y[0], 3))
+        for n in range(1, ERFCX_NT + 1):
+            h, l = limbs(y[n])
+            ex_hi.append(h)
+            ex_lo.append(l)
+    out.append("\n/* erfcx(c + t) = base + t*(b1 + b2 t + ... + b%d t^%d), b_n = erfcx^(n)(c)/n!, about\n"
+               "   centres (i + 1/2)/16 (i < 64), 2^e + (j + 1/2) 2^e/32 (e = 2, 3, 4) and\n"
+               "   -(i + 1/2)/16 (i < 32, rows 160..191) */\n"
+               % (ERFCX_NT, ERFCX_NT - 1))
+    out.append("#define MF_ERFCX_NT %d\n#define MF_ERFCX_ROWS %d\n#define MF_ERFCX_ASYM %d\n"
+               % (ERFCX_NT, ERFCX_ROWS, ERFCX_ASYM))
+    out.append(arr("mf_erfcx_centre", [float(c0) for c0 in centres_x]))
+    out.append(arr("mf_erfcx_base", ex_base))
+    out.append(arr("mf_erfcx_hi", ex_hi))
+    out.append(arr("mf_erfcx_lo", ex_lo))
+    out.append(dd_pair("mf_inv_sqrtpi", 1 / mpmath.sqrt(pi)))
 
     dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "csrc", "dd_tables.h")
     with open(dst, "w") as f:

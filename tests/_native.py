@@ -274,5 +274,20 @@ def rel_err_vs_exact(got, exact3):
         return np.abs(d) / np.abs(e[:, 0])
 
 
+# relative-error caps of dd_erfc / dd_erfcx where the result is a normal
+# double-double; the reference's own maxima on the same fixture are 6.2e-32 and
+# 6.1e-32 (erfcx's worst case, 1/(x sqrt(pi)) for large x, is the final DD
+# division in both)
+ERFC_FAMILY_CAP = {"erfc": 4.5e-32, "erfcx": 6.5e-32}
+
+
+def erfc_family_errors(got, g, nm):
+    """(our max rel error, the reference's max rel error) on the fixture's
+    inputs whose exact value is a normal double-double (|v| > 2^-960)."""
+    ex = g[nm + "_exact"]
+    fin = np.isfinite(ex[:, 0]) & (np.abs(ex[:, 0]) > 2.0 ** -960)
+    return rel_err_vs_exact(got[fin], ex[fin]).max(), rel_err_vs_exact(g[nm + "_ref"][fin], ex[fin]).max()
+
+
 def load_golden(name):
     return np.load(os.path.join(GOLDEN, name), allow_pickle=False)

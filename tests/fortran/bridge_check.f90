@@ -79,6 +79,8 @@ program bridge_check
     write(u) sinh(x)
     write(u) cosh(x)
     write(u) tanh(x)
+    write(u) erfc(x)
+    write(u) erfc_scaled(x)
     ! xGEMM: C = x(1) A B + y(1) C with A passed transposed and C padded to m + 2 rows
     at = transpose(a)
     allocate(cc(m + 2, n))

@@ -7,7 +7,7 @@ known-answer values the reference's own tests use; expected outputs come from
     reference sources by oracle/Makefile), called through its C ABI: adddd,
     subdd, muldd, divdd, sqrtdd, expdd, logdd, sindd, erfdd, matmuldd_mm/mv/vm,
     and cosdd, tandd, exp2dd, expm1dd, log2dd, log10dd, log1pdd, sinhdd,
-    coshdd, tanhdd;
+    coshdd, tanhdd, erfcdd, erfcxdd;
   - mpmath at 320 bits for the transcendental values (stored as three limbs so
     relative errors down to 1e-40 can be measured).
 Every fixture is data only (inputs and expected outputs).
@@ -41,7 +41,7 @@ def load_ref():
     for nm in ("adddd", "subdd", "muldd", "divdd"):
         getattr(L, nm).argtypes = [F2, F2]
         getattr(L, nm).restype = F2
-    for nm in ("sqrtdd", "expdd", "logdd", "sindd", "erfdd") + tuple(f + "dd" for f in TRANSC2):
+    for nm in ("sqrtdd", "expdd", "logdd", "sindd", "erfdd", "erfcdd", "erfcxdd") + tuple(f + "dd" for f in TRANSC2):
         getattr(L, nm).argtypes = [F2]
         getattr(L, nm).restype = F2
     P, I64 = ctypes.c_void_p, ctypes.c_int64
@@ -215,6 +215,38 @@ def transc2(L, rng):
     np.savez_compressed(os.path.join(HERE, "transc2.npz"), **out)
 
 
+def special2(L, rng):
+    """erfc and erfcx: per-regime draws (the branch points of both the
+    reference's fits and this project's rows), the reference's own outputs on
+    them, and 320-bit values. Accuracy is judged against the reference's error
+    on these same inputs."""
+    w = datagen._with_lo
+    U = rng.uniform
+    n = 600
+    erfc_x = np.concatenate([w(rng, U(lo, hi, n)) for lo, hi in
+                             [(0, 0.25), (0.25, 1.25), (1.25, 4), (4, 9), (9, 26.5), (26.5, 27.5),
+                              (-0.25, 0), (-2, -0.25), (-9, -2), (-12, -9)]] +
+                            [w(rng, 10.0 ** U(-30, -1, 100)), np.array(SPECIALS)])
+    erfcx_x = np.concatenate([w(rng, U(lo, hi, n)) for lo, hi in
+                              [(0, 1.25), (1.25, 4), (4, 32), (32, 1000), (-2, 0), (-5, -2), (-26.5, -5),
+                               (-27.5, -26.5)]] +
+                             [w(rng, 10.0 ** U(3, 300, 100)), w(rng, 10.0 ** U(-30, -1, 100)), np.array(SPECIALS)])
+    out = {}
+    for nm, x, f in (("erfc", erfc_x, mpmath.erfc),
+                     ("erfcx", erfcx_x, lambda v: mpmath.exp(v * v) * mpmath.erfc(v))):
+        out[nm + "_in"] = x
+        out[nm + "_ref"] = call1(L, nm + "dd", x)
+        exact = np.full((x.shape[0], 3), np.nan)
+        for i in range(x.shape[0]):
+            if np.isfinite(x[i, 0]):
+                try:
+                    exact[i] = limbs3(f(mp_of(x[i])))
+                except (OverflowError, ValueError):
+                    pass
+        out[nm + "_exact"] = exact
+    np.savez_compressed(os.path.join(HERE, "special2.npz"), **out)
+
+
 def matmul(L, rng):
     shapes = [(1, 1, 1), (7, 1, 9), (3, 9, 1), (2, 20, 5), (16, 16, 16), (4, 32, 6),
               (8, 8, 8), (33, 17, 31), (65, 70, 66)]
@@ -284,7 +316,8 @@ def main():
     matmul(L, rng)
     cmatmul(L, np.random.default_rng(777))
     transc2(L, np.random.default_rng(4242))
-    for f in ("arith.npz", "transc.npz", "matmul.npz", "cmatmul.npz", "transc2.npz"):
+    special2(L, np.random.default_rng(777777))
+    for f in ("arith.npz", "transc.npz", "matmul.npz", "cmatmul.npz", "transc2.npz", "special2.npz"):
         print(f, os.path.getsize(os.path.join(HERE, f)))
 
 

@@ -23,6 +23,8 @@ H1(h_exp, dd_exp)
 H1(h_log, dd_log)
 H1(h_sin, dd_sin)
 H1(h_erf, dd_erf)
+H1(h_erfc, dd_erfc)
+H1(h_erfcx, dd_erfcx)
 H1(h_sqrt, dd_sqrt)
 H2(h_add, dd_add)
 H2(h_sub, dd_sub)

@@ -95,7 +95,7 @@ def test_fortran_bridge_routes_to_the_library(gpu, tmp_path):
         nt.assert_bits(r.take(nv), getattr(mf, nm)(x), nm)
     for nm in ("log2", "log10"):
         nt.assert_bits(r.take(nv), getattr(mf, nm)(mf.mul(x, x)), nm)
-    for nm in ("log1p", "sinh", "cosh", "tanh"):
+    for nm in ("log1p", "sinh", "cosh", "tanh", "erfc", "erfcx"):
         nt.assert_bits(r.take(nv), getattr(mf, nm)(x), nm)
     At = nt.cm(A.transpose(1, 0, 2))
     Cw = np.empty((m + 2, n, 2))

@@ -122,6 +122,37 @@ def test_erf_device_matches_host_build(gpu):
     nt.assert_bits(mf.erf(x), want, "erf")
 
 
+@pytest.mark.parametrize("nm", ["erfc", "erfcx"])
+def test_erfc_family_accuracy_golden(gpu, nm):
+    """Own design (the reference fits these with rationals): no less accurate
+    than the reference's erfcdd / erfcxdd on the same inputs, below the caps
+    where the result is a normal double-double, the reference's special values,
+    and within a few subnormal ulps where erfc underflows."""
+    g = nt.load_golden("special2.npz")
+    x, ex = g[nm + "_in"], g[nm + "_exact"]
+    got = getattr(mf, nm)(x)
+    ours, ref = nt.erfc_family_errors(got, g, nm)
+    assert ours <= ref and ours <= nt.ERFC_FAMILY_CAP[nm], (ours, ref)
+    nf = ~np.isfinite(x[:, 0])
+    nt.assert_bits(got[nf], g[nm + "_ref"][nf], nm + " non-finite inputs")
+    sub = np.isfinite(ex[:, 0]) & (np.abs(ex[:, 0]) <= 2.0 ** -960) & np.isfinite(got[:, 0])
+    assert (np.abs((got[sub, 0] - ex[sub, 0]) + (got[sub, 1] - ex[sub, 1])) <= 2.0 ** -1060).all()
+    if nm == "erfc":
+        z = np.array([[0.0, 0.0], [-0.0, 0.0]])
+        assert (mf.erfc(z) == np.array([1.0, 0.0])).all()
+
+
+@pytest.mark.parametrize("nm", ["erfc", "erfcx"])
+def test_erfc_family_device_matches_host_build(gpu, nm):
+    L = ctypes.CDLL(nt.os.path.join(nt.ROOT, "tests", "native", "libhostmath.so"))
+    rng = np.random.default_rng(6)
+    x = np.concatenate([datagen._with_lo(rng, rng.uniform(-30, 30, 100000)), datagen.config0(rng, 10000),
+                        datagen._with_lo(rng, 10.0 ** rng.uniform(-5, 5, 10000))])
+    want = np.empty_like(x)
+    getattr(L, "h_" + nm)(nt.ptr(x), nt.ptr(want), ctypes.c_long(x.shape[0]))
+    nt.assert_bits(getattr(mf, nm)(x), want, nm)
+
+
 def test_matmul_bits_golden(gpu):
     g = nt.load_golden("matmul.npz")
     for i in range(int(g["count"][0])):

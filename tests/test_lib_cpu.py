@@ -110,6 +110,21 @@ def test_hostmath_erf_accuracy():
     assert nt.rel_err_vs_exact(got[fin], ex[fin]).max() <= 1.55e-32
 
 
+@pytest.mark.parametrize("nm", ["erfc", "erfcx"])
+def test_hostmath_erfc_family_accuracy(nm):
+    """The host build of dd_erfc / dd_erfcx is no less accurate than the
+    reference's erfcdd / erfcxdd on the same inputs (320-bit values), and
+    below 4.5e-32 (erfc) / 6.5e-32 (erfcx) relative wherever the result is a
+    normal double-double."""
+    L = ctypes.CDLL(os.path.join(nt.ROOT, "tests", "native", "libhostmath.so"))
+    g = nt.load_golden("special2.npz")
+    x = g[nm + "_in"]
+    got = np.empty_like(x)
+    getattr(L, "h_" + nm)(nt.ptr(x), nt.ptr(got), ctypes.c_long(x.shape[0]))
+    ours, ref = nt.erfc_family_errors(got, g, nm)
+    assert ours <= ref and ours <= nt.ERFC_FAMILY_CAP[nm], (ours, ref)
+
+
 FORTRAN_DIR = os.path.join(ROOT, "multifloats_amd", "fortran")
 
 
