@@ -169,14 +169,24 @@ def bench_gemm(args, torch, mf):
         ms_mult = timed(torch, step_mult, args.steps, 1)
         C = datagen.unflat(dC.cpu().numpy(), m, n)
         emax, emean = sampled_error(A, B, C, rows, cols)
-        P = S * (S + 1) // 2
+        N = mf.ozaki_moduli(m, k, n, S)
+        if N:
+            # CRT form: N residue GEMMs (k_oz2_gemm, one launch over all N
+            # planes) then the reconstruction (k_oz2_crt); the multiply stage
+            # timed here is both kernels
+            P, kern = N, "k_oz2_gemm"
+            note = ("multiply stage = k_oz2_gemm (%d residue INT8 GEMMs x 2mnk int8 ops, MFMA) + k_oz2_crt "
+                    "(reconstruction); achieved = %d x 2mnk / multiply-stage time" % (N, N))
+        else:
+            P, kern = S * (S + 1) // 2, "k_ozaki"
+            note = "k_ozaki INT8 MFMA: %d slice pairs x 2mnk int8 ops per launch" % P
         tops = P * 2.0 * m * n * k / (ms_mult * 1e-3) / 1e12
         res["ozaki"] = dict(ms=ms, ms_multiply=ms_mult, value=flops / (ms * 1e-3) / 1e9, err_max=emax,
-                            err_mean=emean, slices=S, slice_pairs=P,
+                            err_mean=emean, slices=S, form="crt" if N else "triangular",
+                            int8_products=P,
                             roofline={"bound": "mfma", "achieved": round(tops, 2), "peak": INT8_PEAK_TOPS,
                                       "unit": "TOP/s", "frac": round(tops / INT8_PEAK_TOPS, 4),
-                                      "traffic": PMC_TRAFFIC.get(("k_ozaki", args.size, S)),
-                                      "note": "k_ozaki INT8 MFMA: %d slice pairs x 2mnk int8 ops per launch" % P})
+                                      "traffic": PMC_TRAFFIC.get((kern, args.size, S)), "note": note})
         del C
 
     head = res.get("ozaki") if args.form != "plain" else res["plain"]

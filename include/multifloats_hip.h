@@ -318,6 +318,12 @@ MULTIFLOATS_API int mfhip_ozaki_matmul_mm(const float64x2 *a, const float64x2 *b
  * reference-order GEMM instead), and the rule behind slices <= 0. */
 MULTIFLOATS_API int mfhip_ozaki_slices_used(void);
 MULTIFLOATS_API int mfhip_ozaki_choose_slices(int64_t k, int range_a, int range_b);
+/* Products form of an Ozaki call with these sizes and slice count (7..16):
+ * the number N of pairwise-coprime moduli of the CRT form (N INT8 GEMMs with
+ * residue operands, then Chinese-remainder reconstruction), which runs for
+ * m, n >= 512; 0 when the triangular form runs (S(S+1)/2 digit-pair
+ * products in one fused kernel). No counterpart in the reference. */
+MULTIFLOATS_API int mfhip_ozaki_moduli(int64_t m, int64_t k, int64_t n, int slices);
 /* mfhip_ozaki_matmul_mm in two steps with an explicit slice count (7..16):
  * prepare scales and slices A and B into
  * `work`; multiply forms C from `work` (same m, k, n, slices, work). A

@@ -119,6 +119,7 @@ def lib():
         L.mfhip_cmatmul_mm.argtypes = [P, P, P, I64, I64, I64, I64, VP]
         L.mfhip_ozaki_slices_used.argtypes = []
         L.mfhip_ozaki_choose_slices.argtypes = [I64, I, I]
+        L.mfhip_ozaki_moduli.argtypes = [I64, I64, I64, I]
         L.mfhip_ozaki_prepare.argtypes = [P, P, I64, I64, I64, I, P, ctypes.c_size_t, VP]
         L.mfhip_ozaki_multiply.argtypes = [P, I64, I64, I64, I, P, ctypes.c_size_t, VP]
         C = ctypes.c_char
@@ -522,6 +523,11 @@ def ozaki_slices_used():
 
 def ozaki_choose_slices(k, range_a, range_b):
     return lib().mfhip_ozaki_choose_slices(k, range_a, range_b)
+
+
+def ozaki_moduli(m, k, n, slices):
+    """Moduli N of the CRT form for these sizes and slices (0: triangular form)."""
+    return lib().mfhip_ozaki_moduli(m, k, n, slices)
 
 
 def ozaki_workspace_bytes(m, k, n, slices=0):

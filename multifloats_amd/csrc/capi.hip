@@ -682,6 +682,9 @@ int mfhip_ozaki_matmul_mm(const float64x2 *a, const float64x2 *b, float64x2 *c, 
                                   (hipStream_t)stream, &tl_oz_used));
 }
 int mfhip_ozaki_slices_used(void) { return tl_oz_used; }
+int mfhip_ozaki_moduli(int64_t m, int64_t k, int64_t n, int slices) {
+  return mfhip::ozaki_moduli(m, k, n, slices);
+}
 int mfhip_ozaki_choose_slices(int64_t k, int range_a, int range_b) {
   return mfhip::ozaki_choose_slices(k, range_a, range_b);
 }

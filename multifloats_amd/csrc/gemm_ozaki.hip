@@ -22,6 +22,7 @@
 //    chunks whose DD results are added into C.
 // 4. Recombination, in the lane that owns the output (see k_ozaki).
 #include "dd_core.h"
+#include "dd_tables.h"
 #include "mf_internal.h"
 
 #include <math.h>
@@ -422,6 +423,284 @@ hipError_t launch_main(const signed char *As, const signed char *Bs, const int *
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Ozaki scheme II (CRT form). The S digits of an element are one integer
+// X = sum_s d_s 256^(S-1-s) (|X| < 2^(8S-1)), so C_int = A_int B_int is an
+// exact integer product. Instead of S(S+1)/2 digit-pair products it is formed
+// modulo N pairwise-coprime moduli m_t <= 256 (one plain INT8 GEMM each: the
+// balanced residues fit int8 and k 2^14 < 2^31) and reassembled by the
+// Chinese remainder theorem, with M = prod m_t > 2 k 2^(16S-2) > 2 |C_int|.
+// The truncated levels of the triangular form are not dropped here, so the
+// error is the digit representation's alone (no larger than the triangular
+// form's with the same S).
+constexpr int kO2Tile = 256;  // modular GEMM output tile edge (4 x 4 blocks of 64)
+constexpr int kO2KS = 4;      // K blocks per LDS stage
+constexpr int kO2Group = 8;   // moduli per residue pass
+
+struct Oz2Consts {
+  int y[MF_OZ2_MODULI];  // (M / m_t)^-1 mod m_t
+  double mp_hi, mp_lo;   // M 2^-(16(S-1)) as a double-double
+};
+
+// Balanced residue of v (|v| < 2^23) modulo m (m <= 256): [-(m-1)/2, (m-1)/2]
+// for odd m, [-128, 127] for 256. v and q m are exact in float.
+__device__ __forceinline__ int bal_mod(int v, int m) {
+  const float q = rintf((float)v * (1.0f / (float)m));
+  int r = v - (int)q * m;
+  const int hi = (m - 1) >> 1, lo = m == 256 ? -128 : -hi;
+  r = r > hi ? r - m : r;
+  r = r < lo ? r + m : r;
+  return r;
+}
+
+// Residues of one chunk of digits. dp packs the digits least significant
+// first (byte b of dp[w] is d_(S-1-4w-b), zero beyond S); mf_oz2_pw packs
+// 256^q mod m_t the same way, so X mod m_t = sum of four 4-way int8 dots.
+__device__ __forceinline__ int oz2_res(const int *dp, int t) {
+  const int *w = mf_oz2_pw + 4 * t;
+  int v = __builtin_amdgcn_sdot4(dp[0], w[0], 0, false);
+  v = __builtin_amdgcn_sdot4(dp[1], w[1], v, false);
+  v = __builtin_amdgcn_sdot4(dp[2], w[2], v, false);
+  v = __builtin_amdgcn_sdot4(dp[3], w[3], v, false);
+  return bal_mod(v, mf_oz2_m[t]);
+}
+
+__device__ __forceinline__ void pack_digits(const int *d, int S, int *dp) {
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    unsigned x = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int q = 4 * w + b;  // weight 256^q: digit S-1-q
+      const int dv = q < S ? d[(S - 1 - q) & 15] : 0;
+      x |= ((unsigned)dv & 255u) << (8 * b);
+    }
+    dp[w] = (int)x;
+  }
+}
+
+// Residue planes of A (or of B with TRANS): plane t holds the 64 x 32 chunks
+// of tile row block ib (TB of them, padded to whole 256-row tiles) and K
+// block kb at ((t TB + ib) KB + kb) 2048 in the slices' swizzled layout.
+template <int MAXS, bool TRANS>
+__global__ __launch_bounds__(kThreads) void k_oz2_resid(const double2 *__restrict__ X, int64_t rows,
+                                                        int64_t k, const int *__restrict__ Ex, int S, int N,
+                                                        int64_t KB, int64_t TB, signed char *__restrict__ out) {
+  __shared__ __attribute__((aligned(16))) signed char tile[kO2Group * kChunk];
+  const int ib = blockIdx.x, kb = blockIdx.y;
+  const int tid = threadIdx.x;
+  constexpr int E8 = kChunk / kThreads;
+  int dp[E8][4], off[E8];
+#pragma unroll
+  for (int q = 0; q < E8; ++q) {
+    const int e = tid + q * kThreads;
+    // A: consecutive threads walk down a column (rows); B: down a column of B (k)
+    const int rr = TRANS ? e / kK : e % kT, cc = TRANS ? e % kK : e / kT;
+    const int64_t i = (int64_t)ib * kT + rr, p = (int64_t)kb * kK + cc;
+    double h = 0.0, l = 0.0;
+    int ex = 0;
+    if (i < rows && p < k) {
+      const double2 v = TRANS ? X[p + i * k] : X[i + p * rows];
+      const int Ei = Ex[i];
+      if (Ei != kEmpty && Ei != kNonFinite) {
+        h = v.x;
+        l = v.y;
+        ex = 6 - Ei;
+      }
+    }
+    h = __builtin_ldexp(h, ex);
+    l = __builtin_ldexp(l, ex);
+    int d[MAXS];
+    digits<MAXS>(h, l, S, d);
+    pack_digits(d, S, dp[q]);
+    off[q] = tile_off(rr, cc);
+  }
+  for (int t0 = 0; t0 < N; t0 += kO2Group) {
+    const int G = N - t0 < kO2Group ? N - t0 : kO2Group;
+    if (t0) __syncthreads();
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+      for (int q = 0; q < E8; ++q) tile[g * kChunk + off[q]] = (signed char)oz2_res(dp[q], t0 + g);
+    __syncthreads();
+    for (int v = tid; v < G * kChunk / 16; v += kThreads) {
+      const int g = v / (kChunk / 16), w = v % (kChunk / 16);
+      v4i *dst = (v4i *)(out + (((int64_t)(t0 + g) * TB + ib) * KB + kb) * kChunk);
+      dst[w] = ((const v4i *)(tile + g * kChunk))[w];
+    }
+  }
+}
+
+// One modular INT8 GEMM tile per workgroup: plane t = blockIdx.y, a 256 x 256
+// output tile, 4 waves of 128 x 128 (4 x 4 v_mfma_i32_32x32x32_i8 blocks, all
+// 256 accumulators in AGPRs). Operands stream HBM -> LDS with asynchronous
+// 16-byte copies, kO2KS K blocks per stage, double-buffered, the next stage's
+// copies spread over the current stage's MFMAs. The epilogue reduces each
+// exact INT32 sum modulo m_t and writes the residue byte to plane t of R
+// (column-major, like C).
+__global__ __launch_bounds__(kMainThreads) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_oz2_gemm(
+    const signed char *__restrict__ As, const signed char *__restrict__ Bs, unsigned char *__restrict__ R,
+    int64_t m, int64_t n, int64_t KB, int64_t TBA, int64_t TBB, int tiles_m) {
+  extern __shared__ __attribute__((aligned(16))) signed char lds[];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int t = blockIdx.y;
+  const int ib = blockIdx.x % tiles_m, jb = blockIdx.x / tiles_m;
+  const int wi = wave & 1, wj = wave >> 1;
+  const int r = lane & 31, half = lane >> 5;
+  const signed char *ga = As + ((int64_t)t * TBA + 4 * ib) * KB * kChunk;
+  const signed char *gb = Bs + ((int64_t)t * TBB + 4 * jb) * KB * kChunk;
+  constexpr int kStage = kO2KS * 8 * kChunk;             // 4 A + 4 B chunks per K block
+  constexpr int NL = kO2KS * 8 * kChunk / 1024 / kWaves;  // 1 KiB copies per wave per stage
+  v16i acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  // copy v of a stage: K block q = v / 16, operand (v / 8) & 1, chunk (v / 2) & 3, half KiB v & 1
+  auto issue = [&](int64_t kbs, int st, int j) {
+    const int v = wave + kWaves * j;
+    const int q = v >> 4, op = (v >> 3) & 1, c = (v >> 1) & 3, hh = v & 1;
+    signed char *dst = lds + st * kStage + ((q * 2 + op) * 4 + c) * kChunk + hh * 1024;
+    const signed char *src = (op ? gb : ga) + ((int64_t)c * KB + kbs * kO2KS + q) * kChunk + hh * 1024 + lane * 16;
+    __builtin_amdgcn_global_load_lds((const void *)src, (__attribute__((address_space(3))) void *)dst, 16, 0, 0);
+  };
+  const int64_t NS = KB / kO2KS;  // KB is padded to a multiple of kO2KS
+#pragma unroll
+  for (int j = 0; j < NL; ++j) issue(0, 0, j);
+  v4i fa[2][4], fb[2][4];
+  for (int64_t s = 0; s < NS; ++s) {
+    const int st = (int)(s & 1);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    const bool more = s + 1 < NS;
+    int slot = 0;
+    {
+      const signed char *sa = lds + st * kStage;
+      const signed char *sb = sa + 4 * kChunk;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) fa[0][a] = lds_frag(sa + (2 * wi + (a >> 1)) * kChunk, 32 * (a & 1) + r, half);
+#pragma unroll
+      for (int b = 0; b < 4; ++b) fb[0][b] = lds_frag(sb + (2 * wj + (b >> 1)) * kChunk, 32 * (b & 1) + r, half);
+    }
+#pragma unroll
+    for (int q = 0; q < kO2KS; ++q) {
+      const int cur = q & 1;
+      if (q + 1 < kO2KS) {
+        const signed char *sa = lds + st * kStage + (q + 1) * 8 * kChunk;
+        const signed char *sb = sa + 4 * kChunk;
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+          fa[cur ^ 1][a] = lds_frag(sa + (2 * wi + (a >> 1)) * kChunk, 32 * (a & 1) + r, half);
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          fb[cur ^ 1][b] = lds_frag(sb + (2 * wj + (b >> 1)) * kChunk, 32 * (b & 1) + r, half);
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[cur][a], fb[cur][b], acc[a][b], 0, 0, 0);
+          if (more && (slot & 1) == 0 && (slot >> 1) < NL) issue(s + 1, st ^ 1, slot >> 1);
+          ++slot;
+        }
+    }
+  }
+  // epilogue: row 8(e/4) + 4 half + e%4 and column r of each 32 x 32 block
+  const int mt = mf_oz2_m[t];
+  const double inv = 1.0 / (double)mt;
+  unsigned char *gr = R + (int64_t)t * m * n;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int64_t col = (int64_t)kO2Tile * jb + 128 * wj + 32 * b + r;
+      if (col >= n) continue;
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int64_t row0 = (int64_t)kO2Tile * ib + 128 * wi + 32 * a + 8 * g + 4 * half;
+        unsigned word = 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          // exact INT32 sum (|v| <= 2^29) -> residue in [0, m): the double
+          // quotient is within one of round(v / m), which the steps below absorb
+          const int v = acc[a][b][4 * g + e];
+          int q = (int)__builtin_rint((double)v * inv);
+          int rr = v - q * mt;
+          rr = rr < 0 ? rr + mt : rr;
+          rr = rr >= mt ? rr - mt : rr;
+          rr = rr < 0 ? rr + mt : rr;
+          word |= (unsigned)rr << (8 * e);
+        }
+        if (row0 + 3 < m && (m & 3) == 0) {
+          *(unsigned *)(gr + col * m + row0) = word;
+        } else {
+          for (int e = 0; e < 4; ++e)
+            if (row0 + e < m) gr[col * m + row0 + e] = (unsigned char)(word >> (8 * e));
+        }
+      }
+    }
+}
+
+// CRT: x / M = frac(sum_t u_t / m_t), u_t = c_t y_t mod m_t, summed exactly in
+// eight 36-bit fixed-point limbs of 1/m_t (each u_t L_tj < 2^44; N <= 40 terms
+// stay below 2^50, so the double sums are exact), carried, centred into
+// [-1/2, 1/2), then scaled: C = (x / M) M 2^(E_i + F_j - 12 - 16(S - 1)).
+// The fraction's truncation (< 2^-278 of M) stays below 2^-101 (|A||B|)_ij
+// while (|A||B|)_ij is within 2^150 of the row maximum times the column
+// maximum -- far beyond what the slice count admits.
+constexpr int kO2Limbs = 8;
+__global__ __launch_bounds__(kThreads) void k_oz2_crt(const unsigned char *__restrict__ R, int N, int64_t m,
+                                                      int64_t n, const int *__restrict__ E,
+                                                      const int *__restrict__ F, Oz2Consts cs,
+                                                      double2 *__restrict__ C) {
+  const int64_t mn = m * n;
+  for (int64_t o = (int64_t)blockIdx.x * kThreads + threadIdx.x; o < mn; o += (int64_t)gridDim.x * kThreads) {
+    const int64_t i = o % m, j = o / m;
+    double L[kO2Limbs];
+#pragma unroll
+    for (int l = 0; l < kO2Limbs; ++l) L[l] = 0.0;
+    for (int t = 0; t < N; ++t) {
+      const int mt = mf_oz2_m[t];
+      const int cy = (int)R[t * mn + o] * cs.y[t];  // < 2^16
+      int q = (int)rintf((float)cy * (1.0f / (float)mt));
+      int u = cy - q * mt;
+      u = u < 0 ? u + mt : u;
+      u = u >= mt ? u - mt : u;
+      const double ud = (double)u;
+      const double *rl = mf_oz2_rlimb + kO2Limbs * t;
+#pragma unroll
+      for (int l = 0; l < kO2Limbs; ++l) L[l] = __builtin_fma(ud, rl[l], L[l]);
+    }
+    const double T36 = 0x1p36, iT36 = 0x1p-36;
+#pragma unroll
+    for (int l = kO2Limbs - 1; l > 0; --l) {
+      const double c = __builtin_floor(L[l] * iT36);
+      L[l] -= c * T36;
+      L[l - 1] += c;
+    }
+    L[0] -= __builtin_floor(L[0] * iT36) * T36;  // integer part dropped: L0 in [0, 2^36)
+    if (L[0] >= 0x1p35) L[0] -= T36;             // centre: x / M in [-1/2, 1/2)
+    // x / M = sum_l L_l 2^(-36 (l + 1)), every term exact
+    ddv f = mk(L[0] * iT36, 0.0);
+    double sc36 = iT36;
+#pragma unroll
+    for (int l = 1; l < kO2Limbs; ++l) {
+      sc36 *= iT36;
+      f = dd_add(f, mk(L[l] * sc36, 0.0));
+    }
+    const int ei = E[i], fj = F[j];
+    double2 out;
+    if (ei == kNonFinite || fj == kNonFinite) {
+      out = make_double2(__builtin_nan(""), 0.0);
+    } else {
+      const ddv v = dd_mul(f, mk(cs.mp_hi, cs.mp_lo));
+      const int sc = scale_of(ei) + scale_of(fj) - 12;
+      out = make_double2(__builtin_ldexp(v.hi, sc), __builtin_ldexp(v.lo, sc));
+    }
+    C[o] = out;
+  }
+}
+
 }  // namespace
 
 namespace mfhip {
@@ -431,10 +710,62 @@ constexpr int kMinSlices = 7;
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+namespace {
+// Scheme II: moduli needed for S digits and inner dimension k, i.e. the
+// smallest N with M = prod_{t<N} m_t > 2 k 2^(16S-2) (with margin); 0 if the
+// 40 moduli do not reach it.
+int oz2_count(int S, int64_t k) {
+  const double need = 16.0 * S - 1.0 + log2((double)(k > 0 ? k : 1)) + 0.05;
+  double have = 0.0;
+  for (int t = 0; t < MF_OZ2_MODULI; ++t) {
+    have += log2((double)mf_oz2_m_host[t]);
+    if (have > need) return t + 1;
+  }
+  return 0;
+}
+// Scheme II pays off once the 256 x 256 tiles are mostly full; below that,
+// and wherever N moduli are not enough, the triangular form runs.
+bool use_oz2(int64_t m, int64_t n, int64_t k, int S) {
+  return m >= 512 && n >= 512 && oz2_count(S, k) > 0;
+}
+struct Oz2Dims {
+  int N;
+  int64_t TBA, TBB, KBp;  // 64-row blocks of A / B padded to whole 256-tiles; K blocks padded to kO2KS
+};
+Oz2Dims oz2_dims(int64_t m, int64_t k, int64_t n, int S) {
+  Oz2Dims d;
+  d.N = oz2_count(S, k);
+  d.TBA = (m + kO2Tile - 1) / kO2Tile * 4;
+  d.TBB = (n + kO2Tile - 1) / kO2Tile * 4;
+  const int64_t KB = (k + kK - 1) / kK;
+  d.KBp = (KB + kO2KS - 1) / kO2KS * kO2KS;
+  return d;
+}
+size_t oz2_bytes(int64_t m, int64_t k, int64_t n, int S) {
+  const Oz2Dims d = oz2_dims(m, k, n, S);
+  return align256((size_t)d.N * d.TBA * d.KBp * kChunk) + align256((size_t)d.N * d.TBB * d.KBp * kChunk) +
+         align256((size_t)d.N * m * n);
+}
+// (M / m_t)^-1 mod m_t and M 2^-(16(S-1)) (M rounded to a double-double, so
+// within 2^-106 M) for the first N moduli, from the generated tables.
+Oz2Consts oz2_consts(int N, int S) {
+  Oz2Consts c{};
+  for (int t = 0; t < N; ++t) c.y[t] = mf_oz2_y_host[(N - 1) * MF_OZ2_MODULI + t];
+  c.mp_hi = ldexp(mf_oz2_mp_host[2 * (N - 1)], -16 * (S - 1));
+  c.mp_lo = ldexp(mf_oz2_mp_host[2 * (N - 1) + 1], -16 * (S - 1));
+  return c;
+}
+}  // namespace
+
+int ozaki_moduli(int64_t m, int64_t k, int64_t n, int S) {
+  return (S >= kMinSlices && S <= ozaki_max_slices() && use_oz2(m, n, k, S)) ? oz2_count(S, k) : 0;
+}
+
 size_t ozaki_workspace_bytes(int64_t m, int64_t k, int64_t n, int S) {
   const int64_t tm = (m + kT - 1) / kT, tn = (n + kT - 1) / kT, KB = (k + kK - 1) / kK;
-  return align256((size_t)tm * KB * S * kChunk) + align256((size_t)tn * KB * S * kChunk) +
-         2 * align256(sizeof(int) * (size_t)m) + align256(sizeof(int) * (size_t)n) + 256 +
+  const size_t one = align256((size_t)tm * KB * S * kChunk) + align256((size_t)tn * KB * S * kChunk);
+  const size_t two = oz2_count(S, k) ? oz2_bytes(m, k, n, S) : 0;
+  return (one > two ? one : two) + 2 * align256(sizeof(int) * (size_t)m) + align256(sizeof(int) * (size_t)n) + 256 +
          align256(sizeof(unsigned long long) * (size_t)m);
 }
 
@@ -466,6 +797,23 @@ OzWork carve(void *work, int64_t m, int64_t k, int64_t n, int S) {
   p += align256((size_t)w.tm * w.KB * S * kChunk);
   w.Bs = (signed char *)p;
   return w;
+}
+// scheme II operand and residue planes, in place of the slices
+struct Oz2Work {
+  signed char *As, *Bs;
+  unsigned char *R;
+  Oz2Dims d;
+};
+Oz2Work carve2(const OzWork &w, int64_t m, int64_t k, int64_t n, int S) {
+  Oz2Work v;
+  v.d = oz2_dims(m, k, n, S);
+  char *p = (char *)w.As;
+  v.As = (signed char *)p;
+  p += align256((size_t)v.d.N * v.d.TBA * v.d.KBp * kChunk);
+  v.Bs = (signed char *)p;
+  p += align256((size_t)v.d.N * v.d.TBB * v.d.KBp * kChunk);
+  v.R = (unsigned char *)p;
+  return v;
 }
 bool oz_args_ok(int64_t k, int S) { return S >= kMinSlices && S <= ozaki_max_slices() && k <= 32768; }
 }  // namespace
@@ -534,6 +882,14 @@ hipError_t ozaki_slice(const double2 *A, const double2 *B, int64_t m, int64_t k,
   if (m <= 0 || n <= 0 || k <= 0) return hipSuccess;
   if (!oz_args_ok(k, S) || work_bytes < ozaki_workspace_bytes(m, k, n, S)) return hipErrorInvalidValue;
   OzWork w = carve(work, m, k, n, S);
+  if (use_oz2(m, n, k, S)) {
+    const Oz2Work v = carve2(w, m, k, n, S);
+    hipLaunchKernelGGL((k_oz2_resid<16, false>), dim3((unsigned)v.d.TBA, (unsigned)v.d.KBp), dim3(kThreads), 0, s,
+                       A, m, k, w.E, S, v.d.N, v.d.KBp, v.d.TBA, v.As);
+    hipLaunchKernelGGL((k_oz2_resid<16, true>), dim3((unsigned)v.d.TBB, (unsigned)v.d.KBp), dim3(kThreads), 0, s,
+                       B, n, k, w.F, S, v.d.N, v.d.KBp, v.d.TBB, v.Bs);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(k_slice_a<16>, dim3((unsigned)w.tm, (unsigned)w.KB), dim3(kThreads), 0, s, A, m, k, w.E, S, w.KB, w.As);
   hipLaunchKernelGGL(k_slice_b<16>, dim3((unsigned)w.tn, (unsigned)w.KB), dim3(kThreads), 0, s, B, k, n, w.F, S, w.KB, w.Bs);
   return hipGetLastError();
@@ -552,6 +908,19 @@ hipError_t ozaki_multiply(double2 *C, int64_t m, int64_t k, int64_t n, int S, vo
   if (k <= 0) return hipMemsetAsync(C, 0, sizeof(double2) * m * n, s);
   if (!oz_args_ok(k, S) || work_bytes < ozaki_workspace_bytes(m, k, n, S)) return hipErrorInvalidValue;
   OzWork w = carve(work, m, k, n, S);
+  if (use_oz2(m, n, k, S)) {
+    const Oz2Work v = carve2(w, m, k, n, S);
+    const size_t lds = 2 * (size_t)kO2KS * 8 * kChunk;
+    hipError_t e = hipFuncSetAttribute((const void *)k_oz2_gemm, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    const int64_t tiles_m = v.d.TBA / 4, tiles_n = v.d.TBB / 4;
+    hipLaunchKernelGGL(k_oz2_gemm, dim3((unsigned)(tiles_m * tiles_n), (unsigned)v.d.N), dim3(kMainThreads), lds, s,
+                       v.As, v.Bs, v.R, m, n, v.d.KBp, v.d.TBA, v.d.TBB, (int)tiles_m);
+    const int64_t g = (m * n + kThreads - 1) / kThreads;
+    hipLaunchKernelGGL(k_oz2_crt, dim3((unsigned)(g < 65536 ? g : 65536)), dim3(kThreads), 0, s, v.R, v.d.N, m, n,
+                       w.E, w.F, oz2_consts(v.d.N, S), C);
+    return hipGetLastError();
+  }
   switch (S) {
     case 7: return launch_main<7>(w.As, w.Bs, w.E, w.F, C, m, n, w.KB, w.tm, w.tn, s);
     case 8: return launch_main<8>(w.As, w.Bs, w.E, w.F, C, m, n, w.KB, w.tm, w.tn, s);

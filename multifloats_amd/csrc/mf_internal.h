@@ -78,6 +78,9 @@ size_t ozaki_workspace_bytes(int64_t m, int64_t k, int64_t n, int S);
 hipError_t ozaki_gemm(const double2 *A, const double2 *B, double2 *C, int64_t m, int64_t k,
                       int64_t n, int S, void *work, size_t work_bytes, hipStream_t s, int *used);
 int ozaki_choose_slices(int64_t k, int range_a, int range_b);
+// Moduli of the CRT form that a call with these sizes and S slices runs
+// (0: the triangular form runs).
+int ozaki_moduli(int64_t m, int64_t k, int64_t n, int S);
 // The two halves of ozaki_gemm: scaling + slicing into `work`, then the
 // INT8 MFMA products and DD recombination from `work` into C.
 hipError_t ozaki_prepare(const double2 *A, const double2 *B, int64_t m, int64_t k, int64_t n,

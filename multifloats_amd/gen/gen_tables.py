@@ -31,7 +31,9 @@ rational approximations; see csrc/dd_math.h for the piecewise Taylor scheme):
              32 centres per binade on [4, 32) and -(i + 1/2)/16 on [-2, 0),
              coefficients from the ODE
              y' = 2xy - 2/sqrt(pi) at 450 bits; 1/sqrt(pi) for the
-             asymptotic series beyond 32.
+             asymptotic series beyond 32;
+  - Ozaki scheme II: 40 pairwise-coprime moduli <= 256, the balanced
+             residues of 256^q, and 1/m in 288-bit fixed point.
 
 Run:  python3 multifloats_amd/gen/gen_tables.py   (rewrites csrc/dd_tables.h)
 """
@@ -261,6 +263,55 @@ def main():
     out.append(arr("mf_erfcx_hi", ex_hi))
     out.append(arr("mf_erfcx_lo", ex_lo))
     out.append(dd_pair("mf_inv_sqrtpi", 1 / mpmath.sqrt(pi)))
+
+    # ---- Ozaki scheme II: moduli for the CRT form of the INT8 GEMM ----
+    mods = [256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193, 191, 181, 179,
+            173, 167, 163, 157, 151, 149, 139, 137, 131, 127, 113, 109, 107, 103, 101, 97, 89, 83, 79, 73, 71]
+    for a in range(len(mods)):
+        for b in range(a):
+            assert mpmath.mpf(mods[a]) and __import__("math").gcd(mods[a], mods[b]) == 1
+    pw = []        # 256^q mod m as balanced int8, q = 0..15, packed 4 per int (q = 4w..4w+3, low byte first)
+    for mm in mods:
+        vals = []
+        for q in range(16):
+            v = pow(256, q, mm)
+            if v > mm // 2 or (mm == 256 and v >= 128):
+                v -= mm
+            vals.append(v)
+        for w4 in range(4):
+            word = 0
+            for b in range(4):
+                word |= (vals[4 * w4 + b] & 255) << (8 * b)
+            pw.append(word - (1 << 32) if word >= (1 << 31) else word)
+    rl = []        # 1/m as eight 36-bit limbs: floor(2^288 / m) = sum L_j 2^(36 (7 - j))
+    for mm in mods:
+        big = (1 << 288) // mm
+        rl += [float((big >> (36 * (7 - j))) & ((1 << 36) - 1)) for j in range(8)]
+    out.append("\n/* Ozaki scheme II: pairwise-coprime moduli (<= 256, so balanced residues are int8),\n"
+               "   256^q mod m (balanced, packed 4 per int, q = 0..15) and 1/m as eight 36-bit limbs */\n")
+    out.append("#define MF_OZ2_MODULI %d\n" % len(mods))
+    out.append("#ifdef __cplusplus\nstatic constexpr int mf_oz2_m_host[%d] = {%s};\n#endif\n"
+               % (len(mods), ", ".join(map(str, mods))))
+    out.append("MF_TABLE int mf_oz2_m[%d] = {%s};\n" % (len(mods), ", ".join(map(str, mods))))
+    out.append("MF_TABLE int mf_oz2_pw[%d] = {%s};\n" % (len(pw), ", ".join(map(str, pw))))
+    out.append(arr("mf_oz2_rlimb", rl, per_line=4))
+    # CRT constants for the first N moduli, N = 1..40 (host side): y[N-1][t] =
+    # (M_N / m_t)^-1 mod m_t and M_N = prod_{t<N} m_t rounded to a double-double
+    ys, mp = [], []
+    for N in range(1, len(mods) + 1):
+        MN = 1
+        for mm in mods[:N]:
+            MN *= mm
+        ys += [pow(MN // mods[t] % mods[t], -1, mods[t]) for t in range(N)]
+        ys += [0] * (len(mods) - N)
+        h = float(MN)
+        mp += [h, float(MN - int(h))]
+    out.append("#ifdef __cplusplus\nstatic constexpr int mf_oz2_y_host[%d] = {\n%s\n};\n"
+               % (len(ys), ",\n".join("    " + ", ".join(map(str, ys[r:r + len(mods)]))
+                                      for r in range(0, len(ys), len(mods)))))
+    out.append("static constexpr double mf_oz2_mp_host[%d] = {\n%s\n};\n#endif\n"
+               % (len(mp), "\n".join("    " + ", ".join(hx(v) for v in mp[r:r + 4]) + ","
+                                     for r in range(0, len(mp), 4))))
 
     dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "csrc", "dd_tables.h")
     with open(dst, "w") as f:

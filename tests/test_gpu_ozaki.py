@@ -140,3 +140,89 @@ def test_ozaki_4096_sampled(gpu):
     rr, cc = np.meshgrid(rows, cols, indexing="ij")
     err, mag = nt.gemm_err(A, B, C, rr.ravel(), cc.ravel())
     assert (err <= BOUND * mag).all(), np.log2((err / mag).max())
+
+
+# Shapes with m, n >= 512 run the CRT form (residue GEMMs + Chinese remainder
+# reconstruction); the outputs checked are a row/column sample that always
+# holds the ragged last rows and columns of the 256 x 256 tiles.
+def _sample(rng, m, n, count=40):
+    rows = np.unique(np.concatenate([rng.choice(m, count, replace=False), np.arange(m - 3, m)]))
+    cols = np.unique(np.concatenate([rng.choice(n, count, replace=False), np.arange(n - 3, n)]))
+    rr, cc = np.meshgrid(rows, cols, indexing="ij")
+    return rr.ravel(), cc.ravel()
+
+
+@pytest.mark.parametrize("shape", [(512, 32, 512), (520, 45, 700), (777, 300, 515), (512, 2048, 640)])
+@pytest.mark.parametrize("spread", [0, 10, 20, 30])
+def test_ozaki_crt_form_error_bound(gpu, shape, spread):
+    m, k, n = shape
+    rng = np.random.default_rng(m + 7 * k + 13 * n + spread)
+    A, B = mats(rng, m, k, n, spread)
+    C = mf.matmul_ozaki(A, B, 0)
+    used = mf.ozaki_slices_used()
+    assert used == 0 or 7 <= used <= 16
+    rr, cc = _sample(rng, m, n)
+    err, mag = nt.gemm_err(A, B, C, rr, cc, threads=4)
+    assert (err <= BOUND * mag).all(), (used, np.log2((err / mag).max()))
+
+
+def test_ozaki_crt_form_error_falls_with_slices(gpu):
+    rng = np.random.default_rng(13)
+    A, B = mats(rng, 512, 200, 520, 20)
+    rr, cc = _sample(rng, 512, 520, 24)
+    worst = []
+    for S in (7, 8, 10, 12, 14, 16):
+        err, mag = nt.gemm_err(A, B, mf.matmul_ozaki(A, B, S), rr, cc, threads=4)
+        worst.append(np.log2((err / mag).max()))
+        assert worst[-1] <= -8 * S + 24, (S, worst[-1])
+    assert all(a > b for a, b in zip(worst, worst[1:])), worst
+
+
+def test_ozaki_crt_form_special_rows_and_columns(gpu):
+    rng = np.random.default_rng(14)
+    m, k, n = 530, 40, 520
+    A, B = mats(rng, m, k, n, 10)
+    A = A.copy()
+    B = B.copy()
+    A[3, :, :] = 0.0
+    A[300, 7, 0] = np.inf
+    B[:, 9, :] = 0.0
+    B[11, 400, 0] = np.nan
+    C = mf.matmul_ozaki(datagen.as_colmajor(A), datagen.as_colmajor(B), 16)
+    assert not np.delete(C[3], 400, axis=0).any() and np.isnan(C[3, 400, 0])
+    assert np.isnan(C[300, :, 0]).all()
+    assert not C[np.arange(m) != 300, 9].any()
+    assert np.isnan(C[:, 400, 0]).all()
+    rr, cc = _sample(rng, m, n)
+    ok = ~np.isin(rr, [3, 300]) & ~np.isin(cc, [9, 400])
+    err, mag = nt.gemm_err(A, B, C, rr[ok], cc[ok], threads=4)
+    assert (err <= 2.0 ** -90 * mag).all()
+
+
+def test_ozaki_crt_form_long_k(gpu):
+    rng = np.random.default_rng(15)
+    A, B = mats(rng, 512, 10000, 512, 10)
+    rr, cc = _sample(rng, 512, 512, 10)
+    for S in (0, 16):
+        C = mf.matmul_ozaki(A, B, S)
+        err, mag = nt.gemm_err(A, B, C, rr, cc, threads=4)
+        assert (err <= BOUND * mag).all(), (S, mf.ozaki_slices_used(), np.log2((err / mag).max()))
+
+
+def test_ozaki_crt_form_large_residue_sums(gpu):
+    """Constant operands at k = 32768: every residue product is the same, so
+    the INT32 sums reach k (m_t / 2)^2 ~ 2^29, beyond float's exact integers;
+    the residue reduction must still be exact. C = k a b exactly."""
+    from fractions import Fraction
+    m, k, n = 512, 32768, 512
+    a = np.array([1.0 / 3.0, (1.0 / 3.0) * 2.0 ** -54])
+    b = np.array([-5.0 / 7.0, (5.0 / 7.0) * 2.0 ** -55])
+    A = np.broadcast_to(a, (m, k, 2))
+    B = np.broadcast_to(b, (k, n, 2))
+    exact = k * (Fraction(a[0]) + Fraction(a[1])) * (Fraction(b[0]) + Fraction(b[1]))
+    for S in (0, 16):
+        C = mf.matmul_ozaki(datagen.as_colmajor(A), datagen.as_colmajor(B), S)
+        vals = {(float(h), float(l)) for h, l in C.reshape(-1, 2)}
+        assert len(vals) == 1, (S, len(vals))
+        h, l = vals.pop()
+        assert abs(Fraction(h) + Fraction(l) - exact) <= abs(exact) * Fraction(2) ** -100, S

@@ -79,6 +79,52 @@ def test_tables_are_the_closed_forms():
             assert abs(l) <= abs(h) * 2.0 ** -52, name
 
 
+
+def test_crt_form_tables():
+    """The Ozaki CRT-form tables, recomputed with exact integers: pairwise
+    coprime moduli <= 256 whose product covers the largest need
+    (2 k 2^(16 S - 2) at k = 32768, S = 16); balanced int8 residues of 256^q;
+    1/m as eight 36-bit limbs of floor(2^288 / m); the CRT inverses
+    y = (M_N / m_t)^-1 mod m_t and M_N as a double-double for every N."""
+    import math, os, re
+    text = open(os.path.join(nt.ROOT, "multifloats_amd", "csrc", "dd_tables.h")).read()
+
+    def vals(decl, conv):
+        body = re.search(decl + r"\[\d+\] = \{(.*?)\};", text, re.S).group(1)
+        return [conv(t.strip()) for t in body.split(",") if t.strip()]
+
+    fl = lambda t: float.fromhex(t) if "0x" in t else float(t)
+    mods = vals(r"MF_TABLE int mf_oz2_m", int)
+    assert vals(r"static constexpr int mf_oz2_m_host", int) == mods
+    assert len(mods) == 40 and all(2 <= q <= 256 for q in mods)
+    assert all(math.gcd(a, b) == 1 for i, a in enumerate(mods) for b in mods[:i])
+    assert math.prod(mods) > 2 * 32768 * 2 ** (16 * 16 - 2)
+    pw = vals(r"MF_TABLE int mf_oz2_pw", int)
+    for t, q in enumerate(mods):
+        for e in range(16):
+            byte = (pw[4 * t + e // 4] >> (8 * (e % 4))) & 255
+            r = byte - 256 if byte >= 128 else byte
+            assert (r - pow(256, e, q)) % q == 0 and -128 <= r <= 127
+            assert q == 256 or abs(r) <= (q - 1) // 2
+    rl = vals(r"MF_TABLE double mf_oz2_rlimb", fl)
+    for t, q in enumerate(mods):
+        limbs = [int(v) for v in rl[8 * t:8 * t + 8]]
+        assert all(0 <= v < 2 ** 36 for v in limbs)
+        assert sum(v << (36 * (7 - j)) for j, v in enumerate(limbs)) == (1 << 288) // q
+    ys = vals(r"static constexpr int mf_oz2_y_host", int)
+    mp = vals(r"static constexpr double mf_oz2_mp_host", fl)
+    for N in range(1, 41):
+        MN = math.prod(mods[:N])
+        for t in range(40):
+            y = ys[40 * (N - 1) + t]
+            if t < N:
+                assert (y * (MN // mods[t])) % mods[t] == 1 % mods[t] and 0 <= y < mods[t]
+            else:
+                assert y == 0
+        hi, lo = mp[2 * (N - 1)], mp[2 * N - 1]
+        err = abs(Fraction(hi) + Fraction(lo) - MN)
+        assert err <= Fraction(MN, 2 ** 106) and abs(lo) <= abs(hi) * 2.0 ** -53
+
 def _frac(x):
     return Fraction(float(x[0])) + Fraction(float(x[1]))
 
