@@ -434,109 +434,169 @@ hipError_t launch_main(const signed char *As, const signed char *Bs, const int *
 // error is the digit representation's alone (no larger than the triangular
 // form's with the same S).
 constexpr int kO2Tile = 256;  // modular GEMM output tile edge (4 x 4 blocks of 64)
-constexpr int kO2KS = 4;      // K blocks per LDS stage
-constexpr int kO2Group = 8;   // moduli per residue pass
+constexpr int kO2KS = 2;      // K blocks per stage
 
 struct Oz2Consts {
-  int y[MF_OZ2_MODULI];  // (M / m_t)^-1 mod m_t
-  double mp_hi, mp_lo;   // M 2^-(16(S-1)) as a double-double
+  double mp_hi, mp_lo;  // M 2^-(16(S-1)) as a double-double
 };
 
-// Balanced residue of v (|v| < 2^23) modulo m (m <= 256): [-(m-1)/2, (m-1)/2]
-// for odd m, [-128, 127] for 256. v and q m are exact in float.
-__device__ __forceinline__ int bal_mod(int v, int m) {
-  const float q = rintf((float)v * (1.0f / (float)m));
-  int r = v - (int)q * m;
-  const int hi = (m - 1) >> 1, lo = m == 256 ? -128 : -hi;
-  r = r > hi ? r - m : r;
-  r = r < lo ? r + m : r;
-  return r;
+// Weights of the residue pass: byte j of w[t] (16 bytes) is f_t 256^j mod m_t,
+// balanced into int8, and zero for j >= S, so that f_t X mod m_t ==
+// sum_j e_j w_tj for X = sum_j e_j 256^j. f_t = 1 for A;
+// for B, f_t = y_t = (M / m_t)^-1 mod m_t, so that the residue GEMM sums
+// straight to the CRT digit c_t y_t.
+struct Oz2Weights {
+  int w[MF_OZ2_MODULI][4];
+  float inv[MF_OZ2_MODULI];  // RN(1 / m_t)
+};
+
+// Balanced residue of v (|v| <= 2^18) modulo m <= 256: [-(m-1)/2, (m-1)/2]
+// for odd m, [-128, 127] for 256. v is exact in float, the rounded quotient
+// is within one of round(v / m) and the fma is exact (an integer below 2^24),
+// so one step on each side lands in range.
+__device__ __forceinline__ unsigned bal_mod_byte(int v, int m, float inv) {
+  const float vf = (float)v, mf = (float)m;
+  float r = __builtin_fmaf(-rintf(vf * inv), mf, vf);
+  const float hi = (float)((m - 1) >> 1), lo = m == 256 ? -128.0f : -hi;
+  r = r > hi ? r - mf : r;
+  r = r < lo ? r + mf : r;
+  return (unsigned)(int)r & 255u;
 }
 
-// Residues of one chunk of digits. dp packs the digits least significant
-// first (byte b of dp[w] is d_(S-1-4w-b), zero beyond S); mf_oz2_pw packs
-// 256^q mod m_t the same way, so X mod m_t = sum of four 4-way int8 dots.
-__device__ __forceinline__ int oz2_res(const int *dp, int t) {
-  const int *w = mf_oz2_pw + 4 * t;
-  int v = __builtin_amdgcn_sdot4(dp[0], w[0], 0, false);
-  v = __builtin_amdgcn_sdot4(dp[1], w[1], v, false);
-  v = __builtin_amdgcn_sdot4(dp[2], w[2], v, false);
-  v = __builtin_amdgcn_sdot4(dp[3], w[3], v, false);
-  return bal_mod(v, mf_oz2_m[t]);
-}
-
-__device__ __forceinline__ void pack_digits(const int *d, int S, int *dp) {
-#pragma unroll
-  for (int w = 0; w < 4; ++w) {
-    unsigned x = 0;
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int q = 4 * w + b;  // weight 256^q: digit S-1-q
-      const int dv = q < S ? d[(S - 1 - q) & 15] : 0;
-      x |= ((unsigned)dv & 255u) << (8 * b);
+// Two's complement 128-bit value (lo, hi) of an integer-valued double D,
+// |D| < 2^127: |D| = mant 2^sh with -52 <= sh <= 74, and the bits that a right
+// shift drops are zero.
+__device__ __forceinline__ void int_double_to_i128(double D, uint64_t &lo, uint64_t &hi) {
+  const uint64_t bits = (uint64_t)__double_as_longlong(D);
+  const int e = (int)((bits >> 52) & 0x7FF);
+  const uint64_t mant = (bits & 0xFFFFFFFFFFFFFull) | (1ull << 52);
+  const int sh = e - 1075;
+  uint64_t l = 0, h = 0;
+  if (e != 0) {
+    if (sh <= 0) {
+      l = mant >> (-sh);
+    } else if (sh < 64) {
+      l = mant << sh;
+      h = mant >> (64 - sh);
+    } else {
+      h = mant << (sh - 64);
     }
-    dp[w] = (int)x;
   }
+  if (bits >> 63) {
+    l = ~l + 1;
+    h = ~h + (l == 0 ? 1 : 0);
+  }
+  lo = l;
+  hi = h;
 }
 
 // Residue planes of A (or of B with TRANS): plane t holds the 64 x 32 chunks
 // of tile row block ib (TB of them, padded to whole 256-row tiles) and K
 // block kb at ((t TB + ib) KB + kb) 2048 in the slices' swizzled layout.
-template <int MAXS, bool TRANS>
+// Thread (lane, wave) owns row r = lane (a row of A, a column of B) and the 8
+// consecutive k-elements c = 8 wave .. 8 wave + 7 of the chunk, whose residues
+// are 8 adjacent bytes of the layout: one 8-byte store per modulus. A is read
+// directly (lanes walk down a column: coalesced); B's chunk is staged through
+// LDS (lanes walk down its columns) with a padded row of 33 elements.
+// Each element's integer X = RN(x 256^(S-1)) (x = a 2^(6-E), |x| < 127) is
+// formed exactly in 128 bits from the integer part of H = h 2^P and the rounded
+// rest RN(frac(H) + l 2^P), P = 6 - E + 8(S-1). Its balanced base-256 digits
+// e_j in [-128, 127] (X = sum_j e_j 256^j, j < S; |X| <= 127 256^(S-1) + 1/2
+// is within their range) are the bytes of (X + K) xor K, K = 128 sum_{j<S}
+// 256^j: X + K = sum_j (e_j + 128) 256^j is the plain base-256 expansion.
+// These are the digits of the triangular form's slicing (the same integer,
+// up to the rounding of an exact tie).
+template <bool TRANS>
 __global__ __launch_bounds__(kThreads) void k_oz2_resid(const double2 *__restrict__ X, int64_t rows,
                                                         int64_t k, const int *__restrict__ Ex, int S, int N,
-                                                        int64_t KB, int64_t TB, signed char *__restrict__ out) {
-  __shared__ __attribute__((aligned(16))) signed char tile[kO2Group * kChunk];
+                                                        int64_t KB, int64_t TB, Oz2Weights wt,
+                                                        signed char *__restrict__ out) {
+  constexpr int kPad = kK + 1;
+  __shared__ double2 stage[TRANS ? kT * kPad : 1];
   const int ib = blockIdx.x, kb = blockIdx.y;
-  const int tid = threadIdx.x;
-  constexpr int E8 = kChunk / kThreads;
-  int dp[E8][4], off[E8];
+  const int tid = threadIdx.x, r = tid & 63, c0 = 8 * (tid >> 6);
+  const int64_t i = (int64_t)ib * kT + r;
+  if (TRANS) {
 #pragma unroll
-  for (int q = 0; q < E8; ++q) {
-    const int e = tid + q * kThreads;
-    // A: consecutive threads walk down a column (rows); B: down a column of B (k)
-    const int rr = TRANS ? e / kK : e % kT, cc = TRANS ? e % kK : e / kT;
-    const int64_t i = (int64_t)ib * kT + rr, p = (int64_t)kb * kK + cc;
-    double h = 0.0, l = 0.0;
-    int ex = 0;
-    if (i < rows && p < k) {
-      const double2 v = TRANS ? X[p + i * k] : X[i + p * rows];
-      const int Ei = Ex[i];
-      if (Ei != kEmpty && Ei != kNonFinite) {
-        h = v.x;
-        l = v.y;
-        ex = 6 - Ei;
-      }
+    for (int q = 0; q < kChunk / kThreads; ++q) {
+      const int e = tid + q * kThreads, pp = e % kK, col = e / kK;
+      const int64_t gi = (int64_t)ib * kT + col, gp = (int64_t)kb * kK + pp;
+      stage[col * kPad + pp] = (gi < rows && gp < k) ? X[gp + gi * k] : make_double2(0.0, 0.0);
     }
-    h = __builtin_ldexp(h, ex);
-    l = __builtin_ldexp(l, ex);
-    int d[MAXS];
-    digits<MAXS>(h, l, S, d);
-    pack_digits(d, S, dp[q]);
-    off[q] = tile_off(rr, cc);
-  }
-  for (int t0 = 0; t0 < N; t0 += kO2Group) {
-    const int G = N - t0 < kO2Group ? N - t0 : kO2Group;
-    if (t0) __syncthreads();
-    for (int g = 0; g < G; ++g)
-#pragma unroll
-      for (int q = 0; q < E8; ++q) tile[g * kChunk + off[q]] = (signed char)oz2_res(dp[q], t0 + g);
     __syncthreads();
-    for (int v = tid; v < G * kChunk / 16; v += kThreads) {
-      const int g = v / (kChunk / 16), w = v % (kChunk / 16);
-      v4i *dst = (v4i *)(out + (((int64_t)(t0 + g) * TB + ib) * KB + kb) * kChunk);
-      dst[w] = ((const v4i *)(tile + g * kChunk))[w];
+  }
+  const int Ei = i < rows ? Ex[i] : kEmpty;
+  const bool live = Ei != kEmpty && Ei != kNonFinite;
+  const int P = live ? 6 - Ei + 8 * (S - 1) : 0;
+  const uint64_t k80 = 0x8080808080808080ull;
+  const uint64_t K0 = S >= 8 ? k80 : k80 & ((1ull << (8 * S)) - 1);
+  const uint64_t K1 = S <= 8 ? 0ull : S >= 16 ? k80 : k80 & ((1ull << (8 * (S - 8))) - 1);
+  int dp[8][4];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int64_t p = (int64_t)kb * kK + c0 + q;
+    double2 v = make_double2(0.0, 0.0);
+    if (TRANS) v = stage[r * kPad + c0 + q];
+    else if (i < rows && p < k) v = X[i + p * rows];
+    const double H = live ? __builtin_ldexp(v.x, P) : 0.0;
+    const double L = live ? __builtin_ldexp(v.y, P) : 0.0;
+    const double Hi = __builtin_trunc(H);
+    const double Lr = __builtin_rint((H - Hi) + L);
+    uint64_t a0, a1, b0, b1;
+    int_double_to_i128(Hi, a0, a1);
+    int_double_to_i128(Lr, b0, b1);
+    uint64_t lo = a0 + b0, hi = a1 + b1 + (lo < a0 ? 1 : 0);
+    const uint64_t lk = lo + K0;
+    hi = (hi + K1 + (lk < lo ? 1 : 0)) ^ K1;
+    lo = lk ^ K0;
+    dp[q][0] = (int)(uint32_t)lo;
+    dp[q][1] = (int)(uint32_t)(lo >> 32);
+    dp[q][2] = (int)(uint32_t)hi;
+    dp[q][3] = (int)(uint32_t)(hi >> 32);
+  }
+  signed char *dst0 = out + ((int64_t)ib * KB + kb) * kChunk + tile_off(r, c0);
+  const int64_t plane = TB * KB * kChunk;
+  for (int t = 0; t < N; ++t) {
+    const int mt = mf_oz2_m[t];
+    const float inv = wt.inv[t];
+    const int w0 = wt.w[t][0], w1 = wt.w[t][1], w2 = wt.w[t][2], w3 = wt.w[t][3];
+    unsigned lo = 0, hi = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      int v = __builtin_amdgcn_sdot4(dp[q][0], w0, 0, false);
+      v = __builtin_amdgcn_sdot4(dp[q][1], w1, v, false);
+      v = __builtin_amdgcn_sdot4(dp[q][2], w2, v, false);
+      v = __builtin_amdgcn_sdot4(dp[q][3], w3, v, false);
+      const unsigned byte = bal_mod_byte(v, mt, inv);
+      if (q < 4) lo |= byte << (8 * q);
+      else hi |= byte << (8 * (q - 4));
     }
+    *(uint2 *)(dst0 + t * plane) = make_uint2(lo, hi);
   }
 }
 
 // One modular INT8 GEMM tile per workgroup: plane t = blockIdx.y, a 256 x 256
-// output tile, 4 waves of 128 x 128 (4 x 4 v_mfma_i32_32x32x32_i8 blocks, all
-// 256 accumulators in AGPRs). Operands stream HBM -> LDS with asynchronous
-// 16-byte copies, kO2KS K blocks per stage, double-buffered, the next stage's
-// copies spread over the current stage's MFMAs. The epilogue reduces each
-// exact INT32 sum modulo m_t and writes the residue byte to plane t of R
-// (column-major, like C).
+// output tile, 4 waves (one per SIMD) of 128 x 128 (4 x 4
+// v_mfma_i32_32x32x32_i8 blocks, all 256 accumulators in AGPRs). Operands
+// stream HBM -> LDS with asynchronous 16-byte copies, kO2KS K blocks per
+// stage, in kO2Bufs stage buffers: the copies run kO2Bufs - 1 stages ahead
+// of the MFMAs (a load latency of several microseconds under full load
+// exceeds one stage of MFMAs), and each stage waits only for its own
+// copies. The epilogue reduces each exact INT32 sum v (== c_t y_t, the CRT
+// digit, mod m_t) to its balanced residue and writes the signed byte to plane
+// t of R (column-major, like C).
+constexpr int kO2Bufs = 4;
+constexpr int kO2Stage = kO2KS * 8 * kChunk;              // 4 A + 4 B chunks per K block
+constexpr int kO2NL = kO2Stage / 1024 / kWaves;            // 1 KiB copies per wave per stage
+static_assert(kO2NL * 1024 * kWaves == kO2Stage, "stage copies split evenly over the waves");
+static_assert(kO2Bufs * kO2Stage <= 160 * 1024, "stage buffers fit in LDS");
+// s_waitcnt immediate (gfx9 encoding) that waits for vmcnt <= n only
+constexpr int waitcnt_vm(int n) { return (n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8); }
+// Workgroup barrier for LDS hand-offs only. __syncthreads() adds a fence that
+// drains every outstanding global access (vmcnt(0)), which would also wait for
+// the copies of the stages still in flight; the copies a stage needs are
+// waited for explicitly before this barrier, and LDS needs no cache flush.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 __global__ __launch_bounds__(kMainThreads) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_oz2_gemm(
     const signed char *__restrict__ As, const signed char *__restrict__ Bs, unsigned char *__restrict__ R,
     int64_t m, int64_t n, int64_t KB, int64_t TBA, int64_t TBB, int tiles_m) {
@@ -549,65 +609,69 @@ __global__ __launch_bounds__(kMainThreads) __attribute__((amdgpu_waves_per_eu(1,
   const int r = lane & 31, half = lane >> 5;
   const signed char *ga = As + ((int64_t)t * TBA + 4 * ib) * KB * kChunk;
   const signed char *gb = Bs + ((int64_t)t * TBB + 4 * jb) * KB * kChunk;
-  constexpr int kStage = kO2KS * 8 * kChunk;             // 4 A + 4 B chunks per K block
-  constexpr int NL = kO2KS * 8 * kChunk / 1024 / kWaves;  // 1 KiB copies per wave per stage
   v16i acc[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  // copy v of a stage: K block q = v / 16, operand (v / 8) & 1, chunk (v / 2) & 3, half KiB v & 1
-  auto issue = [&](int64_t kbs, int st, int j) {
+  // copy v = wave + 4 j of stage kbs: K block q = v / 16, operand (v / 8) & 1,
+  // chunk (v / 2) & 3, half KiB v & 1
+  auto issue = [&](int64_t kbs, int j) {
     const int v = wave + kWaves * j;
     const int q = v >> 4, op = (v >> 3) & 1, c = (v >> 1) & 3, hh = v & 1;
-    signed char *dst = lds + st * kStage + ((q * 2 + op) * 4 + c) * kChunk + hh * 1024;
+    signed char *dst = lds + (int)(kbs % kO2Bufs) * kO2Stage + ((q * 2 + op) * 4 + c) * kChunk + hh * 1024;
     const signed char *src = (op ? gb : ga) + ((int64_t)c * KB + kbs * kO2KS + q) * kChunk + hh * 1024 + lane * 16;
     __builtin_amdgcn_global_load_lds((const void *)src, (__attribute__((address_space(3))) void *)dst, 16, 0, 0);
   };
   const int64_t NS = KB / kO2KS;  // KB is padded to a multiple of kO2KS
+  for (int p = 0; p < kO2Bufs - 1; ++p)
+    if (p < NS)
 #pragma unroll
-  for (int j = 0; j < NL; ++j) issue(0, 0, j);
+      for (int j = 0; j < kO2NL; ++j) issue(p, j);
   v4i fa[2][4], fb[2][4];
   for (int64_t s = 0; s < NS; ++s) {
-    const int st = (int)(s & 1);
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    const bool more = s + 1 < NS;
+    // stages s+1 .. s+kO2Bufs-2 may stay in flight
+    if (s + 2 < NS) __builtin_amdgcn_s_waitcnt(waitcnt_vm(2 * kO2NL));
+    else if (s + 1 < NS) __builtin_amdgcn_s_waitcnt(waitcnt_vm(kO2NL));
+    else __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+    lds_barrier();
+    const bool more = s + kO2Bufs - 1 < NS;
+    const signed char *sa0 = lds + (int)(s % kO2Bufs) * kO2Stage;
     int slot = 0;
-    {
-      const signed char *sa = lds + st * kStage;
-      const signed char *sb = sa + 4 * kChunk;
 #pragma unroll
-      for (int a = 0; a < 4; ++a) fa[0][a] = lds_frag(sa + (2 * wi + (a >> 1)) * kChunk, 32 * (a & 1) + r, half);
+    for (int a = 0; a < 4; ++a) fa[0][a] = lds_frag(sa0 + (2 * wi + (a >> 1)) * kChunk, 32 * (a & 1) + r, half);
 #pragma unroll
-      for (int b = 0; b < 4; ++b) fb[0][b] = lds_frag(sb + (2 * wj + (b >> 1)) * kChunk, 32 * (b & 1) + r, half);
-    }
+    for (int b = 0; b < 4; ++b) fb[0][b] = lds_frag(sa0 + (4 + 2 * wj + (b >> 1)) * kChunk, 32 * (b & 1) + r, half);
 #pragma unroll
     for (int q = 0; q < kO2KS; ++q) {
       const int cur = q & 1;
       if (q + 1 < kO2KS) {
-        const signed char *sa = lds + st * kStage + (q + 1) * 8 * kChunk;
-        const signed char *sb = sa + 4 * kChunk;
+        const signed char *sa = sa0 + (q + 1) * 8 * kChunk;
 #pragma unroll
         for (int a = 0; a < 4; ++a)
           fa[cur ^ 1][a] = lds_frag(sa + (2 * wi + (a >> 1)) * kChunk, 32 * (a & 1) + r, half);
 #pragma unroll
         for (int b = 0; b < 4; ++b)
-          fb[cur ^ 1][b] = lds_frag(sb + (2 * wj + (b >> 1)) * kChunk, 32 * (b & 1) + r, half);
+          fb[cur ^ 1][b] = lds_frag(sa + (4 + 2 * wj + (b >> 1)) * kChunk, 32 * (b & 1) + r, half);
       }
 #pragma unroll
       for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
           acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[cur][a], fb[cur][b], acc[a][b], 0, 0, 0);
-          if (more && (slot & 1) == 0 && (slot >> 1) < NL) issue(s + 1, st ^ 1, slot >> 1);
+          constexpr int every = 16 * kO2KS / kO2NL;
+          if (more && slot % every == 0) issue(s + kO2Bufs - 1, slot / every);
           ++slot;
         }
     }
   }
-  // epilogue: row 8(e/4) + 4 half + e%4 and column r of each 32 x 32 block
-  const int mt = mf_oz2_m[t];
-  const double inv = 1.0 / (double)mt;
+  // epilogue: row 8(e/4) + 4 half + e%4 and column r of each 32 x 32 block.
+  // B's residues carry the factor y_t, so v == c_t y_t (mod m_t). q = RN(v / m)
+  // is exact (v / m is an exact binary fraction for m = 256 and at least
+  // 1/(2m) from a half-integer otherwise), so the fma gives the balanced
+  // digit v - q m in [-m/2, m/2] exactly; it is stored as a signed byte
+  // (+128 for m = 256 wraps to -128, the same residue).
+  const double md = (double)mf_oz2_m[t], inv = 1.0 / md;
   unsigned char *gr = R + (int64_t)t * m * n;
 #pragma unroll
   for (int a = 0; a < 4; ++a)
@@ -621,15 +685,9 @@ __global__ __launch_bounds__(kMainThreads) __attribute__((amdgpu_waves_per_eu(1,
         unsigned word = 0;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          // exact INT32 sum (|v| <= 2^29) -> residue in [0, m): the double
-          // quotient is within one of round(v / m), which the steps below absorb
-          const int v = acc[a][b][4 * g + e];
-          int q = (int)__builtin_rint((double)v * inv);
-          int rr = v - q * mt;
-          rr = rr < 0 ? rr + mt : rr;
-          rr = rr >= mt ? rr - mt : rr;
-          rr = rr < 0 ? rr + mt : rr;
-          word |= (unsigned)rr << (8 * e);
+          const double v = (double)acc[a][b][4 * g + e];
+          const double u = __builtin_fma(-__builtin_rint(v * inv), md, v);
+          word |= ((unsigned)(int)u & 255u) << (8 * e);
         }
         if (row0 + 3 < m && (m & 3) == 0) {
           *(unsigned *)(gr + col * m + row0) = word;
@@ -641,63 +699,92 @@ __global__ __launch_bounds__(kMainThreads) __attribute__((amdgpu_waves_per_eu(1,
     }
 }
 
-// CRT: x / M = frac(sum_t u_t / m_t), u_t = c_t y_t mod m_t, summed exactly in
-// eight 36-bit fixed-point limbs of 1/m_t (each u_t L_tj < 2^44; N <= 40 terms
-// stay below 2^50, so the double sums are exact), carried, centred into
-// [-1/2, 1/2), then scaled: C = (x / M) M 2^(E_i + F_j - 12 - 16(S - 1)).
-// The fraction's truncation (< 2^-278 of M) stays below 2^-101 (|A||B|)_ij
-// while (|A||B|)_ij is within 2^150 of the row maximum times the column
-// maximum -- far beyond what the slice count admits.
-constexpr int kO2Limbs = 8;
+// CRT: x / M = frac(sum_t u_t / m_t) with the balanced digits
+// u_t == c_t y_t (mod m_t), |u_t| <= 128, that the GEMM epilogue stored, summed
+// exactly in five 36-bit fixed-point limbs of floor(2^180 / m_t) (each
+// |u_t L_tj| < 2^43; N <= 40 terms stay below 2^49, so the double sums are
+// exact), carried (floor, so negative limbs carry correctly), centred into
+// [-1/2, 1/2), then
+// scaled: C = (x / M) M 2^(E_i + F_j - 12 - 16(S - 1)).
+// The truncated 1/m_t put an error below N 2^(8 - 180) on x / M, i.e. below
+// 2^(16S + log2 k - 159) in units of x (M < 2^(16S + log2 k + 7.1)); the
+// digit representation's own error in x is of order sqrt(k) 2^(8S - 2), which
+// is larger by 2^21 or more for S <= 16 and k <= 32768.
+// Each thread finishes VEC consecutive outputs (column-major, like C): one
+// VEC-byte load per residue plane, eight planes in flight.
+constexpr int kO2Limbs = 5;
+template <int VEC>
 __global__ __launch_bounds__(kThreads) void k_oz2_crt(const unsigned char *__restrict__ R, int N, int64_t m,
                                                       int64_t n, const int *__restrict__ E,
                                                       const int *__restrict__ F, Oz2Consts cs,
                                                       double2 *__restrict__ C) {
-  const int64_t mn = m * n;
-  for (int64_t o = (int64_t)blockIdx.x * kThreads + threadIdx.x; o < mn; o += (int64_t)gridDim.x * kThreads) {
-    const int64_t i = o % m, j = o / m;
-    double L[kO2Limbs];
+  const int64_t mn = m * n, groups = mn / VEC;  // VEC divides m n
+  for (int64_t gi = (int64_t)blockIdx.x * kThreads + threadIdx.x; gi < groups;
+       gi += (int64_t)gridDim.x * kThreads) {
+    const int64_t o0 = gi * VEC;
+    double L[VEC][kO2Limbs];
 #pragma unroll
-    for (int l = 0; l < kO2Limbs; ++l) L[l] = 0.0;
-    for (int t = 0; t < N; ++t) {
-      const int mt = mf_oz2_m[t];
-      const int cy = (int)R[t * mn + o] * cs.y[t];  // < 2^16
-      int q = (int)rintf((float)cy * (1.0f / (float)mt));
-      int u = cy - q * mt;
-      u = u < 0 ? u + mt : u;
-      u = u >= mt ? u - mt : u;
-      const double ud = (double)u;
-      const double *rl = mf_oz2_rlimb + kO2Limbs * t;
+    for (int e = 0; e < VEC; ++e)
 #pragma unroll
-      for (int l = 0; l < kO2Limbs; ++l) L[l] = __builtin_fma(ud, rl[l], L[l]);
-    }
-    const double T36 = 0x1p36, iT36 = 0x1p-36;
+      for (int l = 0; l < kO2Limbs; ++l) L[e][l] = 0.0;
+    for (int t0 = 0; t0 < N; t0 += 8) {
+      unsigned w[8];
 #pragma unroll
-    for (int l = kO2Limbs - 1; l > 0; --l) {
-      const double c = __builtin_floor(L[l] * iT36);
-      L[l] -= c * T36;
-      L[l - 1] += c;
-    }
-    L[0] -= __builtin_floor(L[0] * iT36) * T36;  // integer part dropped: L0 in [0, 2^36)
-    if (L[0] >= 0x1p35) L[0] -= T36;             // centre: x / M in [-1/2, 1/2)
-    // x / M = sum_l L_l 2^(-36 (l + 1)), every term exact
-    ddv f = mk(L[0] * iT36, 0.0);
-    double sc36 = iT36;
+      for (int g = 0; g < 8; ++g) {
+        // planes past N re-read plane N - 1 (in bounds; their words are unused)
+        const unsigned char *src = R + (int64_t)(t0 + g < N ? t0 + g : N - 1) * mn + o0;
+        w[g] = VEC == 4 ? *(const unsigned *)src : (unsigned)*src;
+      }
 #pragma unroll
-    for (int l = 1; l < kO2Limbs; ++l) {
-      sc36 *= iT36;
-      f = dd_add(f, mk(L[l] * sc36, 0.0));
+      for (int g = 0; g < 8; ++g) {
+        const int t = t0 + g;
+        if (t < N) {
+          const double *rl = mf_oz2_rlimb + kO2Limbs * t;
+#pragma unroll
+          for (int e = 0; e < VEC; ++e) {
+            const double ud = (double)(signed char)((w[g] >> (8 * e)) & 255u);
+#pragma unroll
+            for (int l = 0; l < kO2Limbs; ++l) L[e][l] = __builtin_fma(ud, rl[l], L[e][l]);
+          }
+        }
+      }
     }
-    const int ei = E[i], fj = F[j];
-    double2 out;
-    if (ei == kNonFinite || fj == kNonFinite) {
-      out = make_double2(__builtin_nan(""), 0.0);
-    } else {
-      const ddv v = dd_mul(f, mk(cs.mp_hi, cs.mp_lo));
-      const int sc = scale_of(ei) + scale_of(fj) - 12;
-      out = make_double2(__builtin_ldexp(v.hi, sc), __builtin_ldexp(v.lo, sc));
+    int64_t j = o0 / m, i = o0 - j * m;
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) {
+      double *Lr = L[e];
+      const double T36 = 0x1p36, iT36 = 0x1p-36;
+#pragma unroll
+      for (int l = kO2Limbs - 1; l > 0; --l) {
+        const double c = __builtin_floor(Lr[l] * iT36);
+        Lr[l] -= c * T36;
+        Lr[l - 1] += c;
+      }
+      Lr[0] -= __builtin_floor(Lr[0] * iT36) * T36;  // integer part dropped: L0 in [0, 2^36)
+      if (Lr[0] >= 0x1p35) Lr[0] -= T36;             // centre: x / M in [-1/2, 1/2)
+      // x / M = sum_l L_l 2^(-36 (l + 1)), every term exact
+      ddv f = mk(Lr[0] * iT36, 0.0);
+      double sc36 = iT36;
+#pragma unroll
+      for (int l = 1; l < kO2Limbs; ++l) {
+        sc36 *= iT36;
+        f = dd_add(f, mk(Lr[l] * sc36, 0.0));
+      }
+      const int ei = E[i], fj = F[j];
+      double2 out;
+      if (ei == kNonFinite || fj == kNonFinite) {
+        out = make_double2(__builtin_nan(""), 0.0);
+      } else {
+        const ddv v = dd_mul(f, mk(cs.mp_hi, cs.mp_lo));
+        const int sc = scale_of(ei) + scale_of(fj) - 12;
+        out = make_double2(__builtin_ldexp(v.hi, sc), __builtin_ldexp(v.lo, sc));
+      }
+      C[o0 + e] = out;
+      if (++i == m) {
+        i = 0;
+        ++j;
+      }
     }
-    C[o] = out;
   }
 }
 
@@ -730,7 +817,7 @@ bool use_oz2(int64_t m, int64_t n, int64_t k, int S) {
 }
 struct Oz2Dims {
   int N;
-  int64_t TBA, TBB, KBp;  // 64-row blocks of A / B padded to whole 256-tiles; K blocks padded to kO2KS
+  int64_t TBA, TBB, KBp;  // 64-row blocks of A / B padded to whole tiles; K blocks padded to kO2KS
 };
 Oz2Dims oz2_dims(int64_t m, int64_t k, int64_t n, int S) {
   Oz2Dims d;
@@ -746,11 +833,25 @@ size_t oz2_bytes(int64_t m, int64_t k, int64_t n, int S) {
   return align256((size_t)d.N * d.TBA * d.KBp * kChunk) + align256((size_t)d.N * d.TBB * d.KBp * kChunk) +
          align256((size_t)d.N * m * n);
 }
-// (M / m_t)^-1 mod m_t and M 2^-(16(S-1)) (M rounded to a double-double, so
-// within 2^-106 M) for the first N moduli, from the generated tables.
+// Residue-pass weights for S slices: f_t 256^j mod m_t (j < S), balanced,
+// with f_t = 1 (A) or the CRT inverse y_t (B, with_y).
+Oz2Weights oz2_weights(int N, int S, bool with_y) {
+  Oz2Weights c{};
+  for (int t = 0; t < N; ++t) {
+    const int mt = mf_oz2_m_host[t];
+    int pw = with_y ? mf_oz2_y_host[(N - 1) * MF_OZ2_MODULI + t] % mt : 1 % mt;
+    for (int j = 0; j < S; ++j, pw = pw * 256 % mt) {
+      const int bal = (mt == 256 ? pw >= 128 : pw > (mt - 1) / 2) ? pw - mt : pw;
+      c.w[t][j / 4] = (int)((unsigned)c.w[t][j / 4] | (((unsigned)bal & 255u) << (8 * (j % 4))));
+    }
+    c.inv[t] = 1.0f / (float)mt;
+  }
+  return c;
+}
+// M 2^-(16(S-1)) for the product M of the first N moduli (M rounded to a
+// double-double, so within 2^-106 M), from the generated tables.
 Oz2Consts oz2_consts(int N, int S) {
   Oz2Consts c{};
-  for (int t = 0; t < N; ++t) c.y[t] = mf_oz2_y_host[(N - 1) * MF_OZ2_MODULI + t];
   c.mp_hi = ldexp(mf_oz2_mp_host[2 * (N - 1)], -16 * (S - 1));
   c.mp_lo = ldexp(mf_oz2_mp_host[2 * (N - 1) + 1], -16 * (S - 1));
   return c;
@@ -884,10 +985,10 @@ hipError_t ozaki_slice(const double2 *A, const double2 *B, int64_t m, int64_t k,
   OzWork w = carve(work, m, k, n, S);
   if (use_oz2(m, n, k, S)) {
     const Oz2Work v = carve2(w, m, k, n, S);
-    hipLaunchKernelGGL((k_oz2_resid<16, false>), dim3((unsigned)v.d.TBA, (unsigned)v.d.KBp), dim3(kThreads), 0, s,
-                       A, m, k, w.E, S, v.d.N, v.d.KBp, v.d.TBA, v.As);
-    hipLaunchKernelGGL((k_oz2_resid<16, true>), dim3((unsigned)v.d.TBB, (unsigned)v.d.KBp), dim3(kThreads), 0, s,
-                       B, n, k, w.F, S, v.d.N, v.d.KBp, v.d.TBB, v.Bs);
+    hipLaunchKernelGGL(k_oz2_resid<false>, dim3((unsigned)v.d.TBA, (unsigned)v.d.KBp), dim3(kThreads), 0, s,
+                       A, m, k, w.E, S, v.d.N, v.d.KBp, v.d.TBA, oz2_weights(v.d.N, S, false), v.As);
+    hipLaunchKernelGGL(k_oz2_resid<true>, dim3((unsigned)v.d.TBB, (unsigned)v.d.KBp), dim3(kThreads), 0, s,
+                       B, n, k, w.F, S, v.d.N, v.d.KBp, v.d.TBB, oz2_weights(v.d.N, S, true), v.Bs);
     return hipGetLastError();
   }
   hipLaunchKernelGGL(k_slice_a<16>, dim3((unsigned)w.tm, (unsigned)w.KB), dim3(kThreads), 0, s, A, m, k, w.E, S, w.KB, w.As);
@@ -910,15 +1011,22 @@ hipError_t ozaki_multiply(double2 *C, int64_t m, int64_t k, int64_t n, int S, vo
   OzWork w = carve(work, m, k, n, S);
   if (use_oz2(m, n, k, S)) {
     const Oz2Work v = carve2(w, m, k, n, S);
-    const size_t lds = 2 * (size_t)kO2KS * 8 * kChunk;
+    const size_t lds = (size_t)kO2Bufs * kO2Stage;
     hipError_t e = hipFuncSetAttribute((const void *)k_oz2_gemm, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     const int64_t tiles_m = v.d.TBA / 4, tiles_n = v.d.TBB / 4;
+    const Oz2Consts cs = oz2_consts(v.d.N, S);
     hipLaunchKernelGGL(k_oz2_gemm, dim3((unsigned)(tiles_m * tiles_n), (unsigned)v.d.N), dim3(kMainThreads), lds, s,
                        v.As, v.Bs, v.R, m, n, v.d.KBp, v.d.TBA, v.d.TBB, (int)tiles_m);
-    const int64_t g = (m * n + kThreads - 1) / kThreads;
-    hipLaunchKernelGGL(k_oz2_crt, dim3((unsigned)(g < 65536 ? g : 65536)), dim3(kThreads), 0, s, v.R, v.d.N, m, n,
-                       w.E, w.F, oz2_consts(v.d.N, S), C);
+    if ((m * n) % 4 == 0) {
+      const int64_t g = (m * n / 4 + kThreads - 1) / kThreads;
+      hipLaunchKernelGGL(k_oz2_crt<4>, dim3((unsigned)(g < 65536 ? g : 65536)), dim3(kThreads), 0, s, v.R, v.d.N,
+                         m, n, w.E, w.F, cs, C);
+    } else {
+      const int64_t g = (m * n + kThreads - 1) / kThreads;
+      hipLaunchKernelGGL(k_oz2_crt<1>, dim3((unsigned)(g < 65536 ? g : 65536)), dim3(kThreads), 0, s, v.R, v.d.N,
+                         m, n, w.E, w.F, cs, C);
+    }
     return hipGetLastError();
   }
   switch (S) {

@@ -32,8 +32,9 @@ rational approximations; see csrc/dd_math.h for the piecewise Taylor scheme):
              coefficients from the ODE
              y' = 2xy - 2/sqrt(pi) at 450 bits; 1/sqrt(pi) for the
              asymptotic series beyond 32;
-  - Ozaki scheme II: 40 pairwise-coprime moduli <= 256, the balanced
-             residues of 256^q, and 1/m in 288-bit fixed point.
+  - Ozaki CRT form: 40 pairwise-coprime moduli <= 256, 1/m in 180-bit
+             fixed point, and the CRT inverses and products for every
+             modulus count (exact integers).
 
 Run:  python3 multifloats_amd/gen/gen_tables.py   (rewrites csrc/dd_tables.h)
 """
@@ -270,30 +271,16 @@ def main():
     for a in range(len(mods)):
         for b in range(a):
             assert mpmath.mpf(mods[a]) and __import__("math").gcd(mods[a], mods[b]) == 1
-    pw = []        # 256^q mod m as balanced int8, q = 0..15, packed 4 per int (q = 4w..4w+3, low byte first)
+    rl = []        # 1/m as five 36-bit limbs: floor(2^180 / m) = sum L_j 2^(36 (4 - j))
     for mm in mods:
-        vals = []
-        for q in range(16):
-            v = pow(256, q, mm)
-            if v > mm // 2 or (mm == 256 and v >= 128):
-                v -= mm
-            vals.append(v)
-        for w4 in range(4):
-            word = 0
-            for b in range(4):
-                word |= (vals[4 * w4 + b] & 255) << (8 * b)
-            pw.append(word - (1 << 32) if word >= (1 << 31) else word)
-    rl = []        # 1/m as eight 36-bit limbs: floor(2^288 / m) = sum L_j 2^(36 (7 - j))
-    for mm in mods:
-        big = (1 << 288) // mm
-        rl += [float((big >> (36 * (7 - j))) & ((1 << 36) - 1)) for j in range(8)]
+        big = (1 << 180) // mm
+        rl += [float((big >> (36 * (4 - j))) & ((1 << 36) - 1)) for j in range(5)]
     out.append("\n/* Ozaki scheme II: pairwise-coprime moduli (<= 256, so balanced residues are int8),\n"
-               "   256^q mod m (balanced, packed 4 per int, q = 0..15) and 1/m as eight 36-bit limbs */\n")
+               "   and 1/m as five 36-bit limbs */\n")
     out.append("#define MF_OZ2_MODULI %d\n" % len(mods))
     out.append("#ifdef __cplusplus\nstatic constexpr int mf_oz2_m_host[%d] = {%s};\n#endif\n"
                % (len(mods), ", ".join(map(str, mods))))
     out.append("MF_TABLE int mf_oz2_m[%d] = {%s};\n" % (len(mods), ", ".join(map(str, mods))))
-    out.append("MF_TABLE int mf_oz2_pw[%d] = {%s};\n" % (len(pw), ", ".join(map(str, pw))))
     out.append(arr("mf_oz2_rlimb", rl, per_line=4))
     # CRT constants for the first N moduli, N = 1..40 (host side): y[N-1][t] =
     # (M_N / m_t)^-1 mod m_t and M_N = prod_{t<N} m_t rounded to a double-double

@@ -83,9 +83,9 @@ def test_tables_are_the_closed_forms():
 def test_crt_form_tables():
     """The Ozaki CRT-form tables, recomputed with exact integers: pairwise
     coprime moduli <= 256 whose product covers the largest need
-    (2 k 2^(16 S - 2) at k = 32768, S = 16); balanced int8 residues of 256^q;
-    1/m as eight 36-bit limbs of floor(2^288 / m); the CRT inverses
-    y = (M_N / m_t)^-1 mod m_t and M_N as a double-double for every N."""
+    (2 k 2^(16 S - 2) at k = 32768, S = 16); 1/m as five 36-bit limbs of
+    floor(2^180 / m); the CRT inverses y = (M_N / m_t)^-1 mod m_t and M_N as
+    a double-double for every N."""
     import math, os, re
     text = open(os.path.join(nt.ROOT, "multifloats_amd", "csrc", "dd_tables.h")).read()
 
@@ -99,18 +99,11 @@ def test_crt_form_tables():
     assert len(mods) == 40 and all(2 <= q <= 256 for q in mods)
     assert all(math.gcd(a, b) == 1 for i, a in enumerate(mods) for b in mods[:i])
     assert math.prod(mods) > 2 * 32768 * 2 ** (16 * 16 - 2)
-    pw = vals(r"MF_TABLE int mf_oz2_pw", int)
-    for t, q in enumerate(mods):
-        for e in range(16):
-            byte = (pw[4 * t + e // 4] >> (8 * (e % 4))) & 255
-            r = byte - 256 if byte >= 128 else byte
-            assert (r - pow(256, e, q)) % q == 0 and -128 <= r <= 127
-            assert q == 256 or abs(r) <= (q - 1) // 2
     rl = vals(r"MF_TABLE double mf_oz2_rlimb", fl)
     for t, q in enumerate(mods):
-        limbs = [int(v) for v in rl[8 * t:8 * t + 8]]
+        limbs = [int(v) for v in rl[5 * t:5 * t + 5]]
         assert all(0 <= v < 2 ** 36 for v in limbs)
-        assert sum(v << (36 * (7 - j)) for j, v in enumerate(limbs)) == (1 << 288) // q
+        assert sum(v << (36 * (4 - j)) for j, v in enumerate(limbs)) == (1 << 180) // q
     ys = vals(r"static constexpr int mf_oz2_y_host", int)
     mp = vals(r"static constexpr double mf_oz2_mp_host", fl)
     for N in range(1, 41):
