@@ -334,6 +334,14 @@ MULTIFLOATS_API int mfhip_ozaki_prepare(const float64x2 *a, const float64x2 *b, 
 MULTIFLOATS_API int mfhip_ozaki_multiply(float64x2 *c, int64_t m, int64_t k, int64_t n,
                                          int slices, void *work, size_t work_bytes,
                                          void *stream);
+/* The products stage of mfhip_ozaki_multiply in the CRT form
+ * (mfhip_ozaki_moduli(m, k, n, slices) > 0) alone: the residue INT8 GEMMs
+ * from a prepared `work` into `work`, without the reconstruction of C (e.g.
+ * to time or schedule the products separately). Returns an error for the
+ * triangular form, whose products and recombination are one kernel. No
+ * counterpart in the reference. */
+MULTIFLOATS_API int mfhip_ozaki_products(int64_t m, int64_t k, int64_t n, int slices, void *work,
+                                         size_t work_bytes, void *stream);
 
 /* Batching form (no counterpart in the reference): many independent DD GEMMs
  * (matmuldd_mm semantics, bit-identical per problem) and dot products

@@ -122,6 +122,7 @@ def lib():
         L.mfhip_ozaki_moduli.argtypes = [I64, I64, I64, I]
         L.mfhip_ozaki_prepare.argtypes = [P, P, I64, I64, I64, I, P, ctypes.c_size_t, VP]
         L.mfhip_ozaki_multiply.argtypes = [P, I64, I64, I64, I, P, ctypes.c_size_t, VP]
+        L.mfhip_ozaki_products.argtypes = [I64, I64, I64, I, P, ctypes.c_size_t, VP]
         C = ctypes.c_char
         L.wgemmdd.argtypes = [C, C, I64, I64, I64, float64x2, P, I64, P, I64, float64x2, P, I64, I64]
         L.wgemmdd.restype = None
@@ -544,6 +545,11 @@ def dev_ozaki_prepare(a, b, m, k, n, slices, work, work_bytes, stream=0):
 
 def dev_ozaki_multiply(c, m, k, n, slices, work, work_bytes, stream=0):
     _dev(lib().mfhip_ozaki_multiply(c, m, k, n, slices, work, work_bytes, stream))
+
+
+def dev_ozaki_products(m, k, n, slices, work, work_bytes, stream=0):
+    """CRT form only: the residue GEMMs of dev_ozaki_multiply without the reconstruction."""
+    _dev(lib().mfhip_ozaki_products(m, k, n, slices, work, work_bytes, stream))
 
 
 def dev_unary(op, a, c, n, stream=0):

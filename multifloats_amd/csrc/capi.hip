@@ -700,6 +700,11 @@ int mfhip_ozaki_multiply(float64x2 *c, int64_t m, int64_t k, int64_t n, int slic
   return status(mfhip::ozaki_multiply(W(c), m, k, n, slices, work, work_bytes,
                                       (hipStream_t)stream));
 }
+int mfhip_ozaki_products(int64_t m, int64_t k, int64_t n, int slices, void *work, size_t work_bytes,
+                         void *stream) {
+  CHECK_ALIGN(work);
+  return status(mfhip::ozaki_products(m, k, n, slices, work, work_bytes, (hipStream_t)stream));
+}
 
 int mfhip_last_error(void) { return tl_err; }
 const char *mfhip_error_string(int err) { return hipGetErrorString((hipError_t)err); }

@@ -450,17 +450,15 @@ struct Oz2Weights {
   float inv[MF_OZ2_MODULI];  // RN(1 / m_t)
 };
 
-// Balanced residue of v (|v| <= 2^18) modulo m <= 256: [-(m-1)/2, (m-1)/2]
-// for odd m, [-128, 127] for 256. v is exact in float, the rounded quotient
-// is within one of round(v / m) and the fma is exact (an integer below 2^24),
-// so one step on each side lands in range.
+// Balanced residue of v (|v| <= 2^18) modulo m <= 256 as a signed byte:
+// v - RN(v / m) m, in [-(m-1)/2, (m-1)/2] for odd m and [-128, 128] for 256
+// (+128 wraps to -128, the same residue). The float quotient is exact: v / m
+// is an exact binary fraction for m = 256 and lies at least 1/(2m) from a
+// half-integer otherwise, beyond the product's error (|v| 2^-23 / m); the fma
+// is exact (an integer below 2^24).
 __device__ __forceinline__ unsigned bal_mod_byte(int v, int m, float inv) {
-  const float vf = (float)v, mf = (float)m;
-  float r = __builtin_fmaf(-rintf(vf * inv), mf, vf);
-  const float hi = (float)((m - 1) >> 1), lo = m == 256 ? -128.0f : -hi;
-  r = r > hi ? r - mf : r;
-  r = r < lo ? r + mf : r;
-  return (unsigned)(int)r & 255u;
+  const float vf = (float)v;
+  return (unsigned)(int)__builtin_fmaf(-rintf(vf * inv), (float)m, vf) & 255u;
 }
 
 // Two's complement 128-bit value (lo, hi) of an integer-valued double D,
@@ -1003,6 +1001,26 @@ hipError_t ozaki_prepare(const double2 *A, const double2 *B, int64_t m, int64_t 
   return ozaki_slice(A, B, m, k, n, S, work, work_bytes, s);
 }
 
+namespace {
+// The N residue GEMMs of the CRT form, from the residue planes into R.
+hipError_t oz2_products(const Oz2Work &v, int64_t m, int64_t n, hipStream_t s) {
+  const size_t lds = (size_t)kO2Bufs * kO2Stage;
+  hipError_t e = hipFuncSetAttribute((const void *)k_oz2_gemm, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  const int64_t tiles_m = v.d.TBA / 4, tiles_n = v.d.TBB / 4;
+  hipLaunchKernelGGL(k_oz2_gemm, dim3((unsigned)(tiles_m * tiles_n), (unsigned)v.d.N), dim3(kMainThreads), lds, s,
+                     v.As, v.Bs, v.R, m, n, v.d.KBp, v.d.TBA, v.d.TBB, (int)tiles_m);
+  return hipGetLastError();
+}
+}  // namespace
+
+hipError_t ozaki_products(int64_t m, int64_t k, int64_t n, int S, void *work, size_t work_bytes, hipStream_t s) {
+  if (m <= 0 || n <= 0 || k <= 0) return hipSuccess;
+  if (!oz_args_ok(k, S) || work_bytes < ozaki_workspace_bytes(m, k, n, S) || !use_oz2(m, n, k, S))
+    return hipErrorInvalidValue;
+  return oz2_products(carve2(carve(work, m, k, n, S), m, k, n, S), m, n, s);
+}
+
 hipError_t ozaki_multiply(double2 *C, int64_t m, int64_t k, int64_t n, int S, void *work,
                           size_t work_bytes, hipStream_t s) {
   if (m <= 0 || n <= 0) return hipSuccess;
@@ -1011,13 +1029,9 @@ hipError_t ozaki_multiply(double2 *C, int64_t m, int64_t k, int64_t n, int S, vo
   OzWork w = carve(work, m, k, n, S);
   if (use_oz2(m, n, k, S)) {
     const Oz2Work v = carve2(w, m, k, n, S);
-    const size_t lds = (size_t)kO2Bufs * kO2Stage;
-    hipError_t e = hipFuncSetAttribute((const void *)k_oz2_gemm, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipError_t e = oz2_products(v, m, n, s);
     if (e != hipSuccess) return e;
-    const int64_t tiles_m = v.d.TBA / 4, tiles_n = v.d.TBB / 4;
     const Oz2Consts cs = oz2_consts(v.d.N, S);
-    hipLaunchKernelGGL(k_oz2_gemm, dim3((unsigned)(tiles_m * tiles_n), (unsigned)v.d.N), dim3(kMainThreads), lds, s,
-                       v.As, v.Bs, v.R, m, n, v.d.KBp, v.d.TBA, v.d.TBB, (int)tiles_m);
     if ((m * n) % 4 == 0) {
       const int64_t g = (m * n / 4 + kThreads - 1) / kThreads;
       hipLaunchKernelGGL(k_oz2_crt<4>, dim3((unsigned)(g < 65536 ? g : 65536)), dim3(kThreads), 0, s, v.R, v.d.N,

@@ -87,5 +87,9 @@ hipError_t ozaki_prepare(const double2 *A, const double2 *B, int64_t m, int64_t 
                          int S, void *work, size_t work_bytes, hipStream_t s);
 hipError_t ozaki_multiply(double2 *C, int64_t m, int64_t k, int64_t n, int S, void *work,
                           size_t work_bytes, hipStream_t s);
+// The products stage of the CRT form alone (ozaki_moduli(m, k, n, S) > 0):
+// the residue GEMMs into `work`; invalid for the triangular form.
+hipError_t ozaki_products(int64_t m, int64_t k, int64_t n, int S, void *work, size_t work_bytes,
+                          hipStream_t s);
 
 }  // namespace mfhip

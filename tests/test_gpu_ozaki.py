@@ -226,3 +226,33 @@ def test_ozaki_crt_form_large_residue_sums(gpu):
         assert len(vals) == 1, (S, len(vals))
         h, l = vals.pop()
         assert abs(Fraction(h) + Fraction(l) - exact) <= abs(exact) * Fraction(2) ** -100, S
+
+
+def test_ozaki_products_stage(gpu):
+    """mfhip_ozaki_products runs the CRT form's residue GEMMs alone: a
+    multiply after it gives the full call's bits; the triangular form has no
+    separate products stage and reports an error."""
+    import torch
+    rng = np.random.default_rng(16)
+    s = torch.cuda.current_stream().cuda_stream
+    for (m, k, n), crt in (((520, 70, 515), True), ((96, 70, 80), False)):
+        A, B = mats(rng, m, k, n, 10)
+        S = 12
+        assert (mf.ozaki_moduli(m, k, n, S) > 0) == crt
+        dA = torch.from_numpy(datagen.flat(A)).cuda()
+        dB = torch.from_numpy(datagen.flat(B)).cuda()
+        dC = torch.empty((m * n, 2), dtype=torch.float64, device="cuda")
+        wb = mf.ozaki_workspace_bytes(m, k, n, S)
+        work = torch.empty(wb, dtype=torch.uint8, device="cuda")
+        mf.dev_matmul_ozaki(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), m, k, n, S, work.data_ptr(), wb, s)
+        full = dC.cpu().numpy().copy()
+        dC.zero_()
+        mf.dev_ozaki_prepare(dA.data_ptr(), dB.data_ptr(), m, k, n, S, work.data_ptr(), wb, s)
+        if crt:
+            mf.dev_ozaki_products(m, k, n, S, work.data_ptr(), wb, s)
+        else:
+            with pytest.raises(RuntimeError):
+                mf.dev_ozaki_products(m, k, n, S, work.data_ptr(), wb, s)
+        mf.dev_ozaki_multiply(dC.data_ptr(), m, k, n, S, work.data_ptr(), wb, s)
+        torch.cuda.synchronize()
+        assert np.array_equal(dC.cpu().numpy().view(np.uint64), full.view(np.uint64))
