@@ -21,6 +21,11 @@
 //    which holds while k * S * 2^14 < 2^31: longer K ranges are cut into
 //    chunks whose DD results are added into C.
 // 4. Recombination, in the lane that owns the output (see k_ozaki).
+// That is the triangular form, run while m or n is below 512. From m, n >= 512
+// the CRT form runs instead (k_oz2_resid, k_oz2_gemm, k_oz2_crt below): the
+// S-digit integers are multiplied exactly modulo N pairwise-coprime moduli
+// (N plain INT8 GEMMs instead of S(S+1)/2) and C is rebuilt by the Chinese
+// remainder theorem, with the digit representation as the only error.
 #include "dd_core.h"
 #include "dd_tables.h"
 #include "mf_internal.h"
@@ -588,6 +593,21 @@ constexpr int kO2Stage = kO2KS * 8 * kChunk;              // 4 A + 4 B chunks pe
 constexpr int kO2NL = kO2Stage / 1024 / kWaves;            // 1 KiB copies per wave per stage
 static_assert(kO2NL * 1024 * kWaves == kO2Stage, "stage copies split evenly over the waves");
 static_assert(kO2Bufs * kO2Stage <= 160 * 1024, "stage buffers fit in LDS");
+// Tile of workgroup x in a TM x TN tile grid. Workgroups are dispatched
+// round-robin over the 8 XCDs by linear id, and each XCD has its own L2: when
+// the grid splits evenly, the 8 residues of x get one compact (TM/2) x (TN/4)
+// sub-grid each, so an operand panel is fetched by 4 (A) or 2 (B) XCDs rather
+// than by all 8. (TM TN % 8 == 0 keeps the split the same in every plane.)
+__device__ __forceinline__ void xcd_tile(int x, int TM, int TN, int &ib, int &jb) {
+  if (TM % 2 == 0 && TN % 4 == 0) {
+    const int xcd = x & 7, slot = x >> 3, bm = TM / 2, bn = TN / 4;
+    ib = (xcd >> 2) * bm + slot % bm;
+    jb = (xcd & 3) * bn + slot / bm;
+  } else {
+    ib = x % TM;
+    jb = x / TM;
+  }
+}
 // s_waitcnt immediate (gfx9 encoding) that waits for vmcnt <= n only
 constexpr int waitcnt_vm(int n) { return (n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8); }
 // Workgroup barrier for LDS hand-offs only. __syncthreads() adds a fence that
@@ -597,12 +617,13 @@ constexpr int waitcnt_vm(int n) { return (n & 15) | ((n >> 4) << 14) | (7 << 4) 
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 __global__ __launch_bounds__(kMainThreads) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_oz2_gemm(
     const signed char *__restrict__ As, const signed char *__restrict__ Bs, unsigned char *__restrict__ R,
-    int64_t m, int64_t n, int64_t KB, int64_t TBA, int64_t TBB, int tiles_m) {
+    int64_t m, int64_t n, int64_t KB, int64_t TBA, int64_t TBB, int tiles_m, int tiles_n) {
   extern __shared__ __attribute__((aligned(16))) signed char lds[];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int t = blockIdx.y;
-  const int ib = blockIdx.x % tiles_m, jb = blockIdx.x / tiles_m;
+  int ib, jb;
+  xcd_tile(blockIdx.x, tiles_m, tiles_n, ib, jb);
   const int wi = wave & 1, wj = wave >> 1;
   const int r = lane & 31, half = lane >> 5;
   const signed char *ga = As + ((int64_t)t * TBA + 4 * ib) * KB * kChunk;
@@ -1009,7 +1030,7 @@ hipError_t oz2_products(const Oz2Work &v, int64_t m, int64_t n, hipStream_t s) {
   if (e != hipSuccess) return e;
   const int64_t tiles_m = v.d.TBA / 4, tiles_n = v.d.TBB / 4;
   hipLaunchKernelGGL(k_oz2_gemm, dim3((unsigned)(tiles_m * tiles_n), (unsigned)v.d.N), dim3(kMainThreads), lds, s,
-                     v.As, v.Bs, v.R, m, n, v.d.KBp, v.d.TBA, v.d.TBB, (int)tiles_m);
+                     v.As, v.Bs, v.R, m, n, v.d.KBp, v.d.TBA, v.d.TBB, (int)tiles_m, (int)tiles_n);
   return hipGetLastError();
 }
 }  // namespace
