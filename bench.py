@@ -46,7 +46,7 @@ INT8_PEAK_TOPS = 5000.0      # MI355X dense INT8 MFMA
 # profiles/pmc_traffic_gemm_4096.md. Keyed by (kernel, size, slices).
 PMC_TRAFFIC = {
     ("k_ozaki", 4096, 14): 2765231.0 * 2.0 * 1024 + 286820.0 * 1024,
-    ("k_oz2_gemm", 4096, 14): 2361438.0 * 2.0 * 1024 + 559382.0 * 1024,
+    ("k_oz2_gemm", 4096, 14): 1682594.0 * 2.0 * 1024 + 599483.0 * 1024,
     ("k_gemm_plain", 4096, 0): 3144165.19 * 2.0 * 1024 + 286727.06 * 1024,
 }
 
