@@ -224,10 +224,13 @@ MULTIFLOATS_API void wtrsmdd(char side, char uplo, char transa, char diag, int64
 
 /* Precision form of matmuldd_mm (no counterpart in the reference): the Ozaki
  * split-precision GEMM on the INT8 matrix cores. A and B are cut into
- * `slices` int8 digit slices (8 bits each) per row / column scale; slice
- * pairs whose scales sum to fewer than `slices` levels are multiplied with
- * exact INT32 accumulation and recombined in DD. Not bit-identical to
- * matmuldd_mm; the target is |C - AB| <= 2^-100 (|A||B|) elementwise.
+ * `slices` int8 digits (8 bits each) per row / column scale. For m or n below
+ * 512, digit pairs whose levels sum to fewer than `slices` are multiplied
+ * with exact INT32 accumulation and recombined in DD (triangular form); from
+ * m, n >= 512 the digit integers are multiplied exactly modulo pairwise-coprime
+ * moduli (one INT8 GEMM each) and C is rebuilt by the Chinese remainder
+ * theorem (CRT form). Not bit-identical to matmuldd_mm; the target is
+ * |C - AB| <= 2^-100 (|A||B|) elementwise.
  * slices <= 0 (recommended) chooses the count from k and the exponent ranges
  * of A's rows and B's columns, and when more than 16 would be needed runs the
  * reference-order GEMM instead; explicit counts must lie in 7..16.
