@@ -68,6 +68,7 @@ __global__ __launch_bounds__(kThreads) void k_row_exp(const double2 *__restrict_
   int e = kEmpty, lo = kNonFinite;
   double mx = 0.0;
   bool bad = false;
+#pragma unroll 8
   for (int64_t p = p0; p < p1; ++p) {
     double h = A[i + p * m].x;
     if (!mf_isfinite(h)) bad = true;
@@ -106,49 +107,74 @@ __global__ void k_row_finish(int *__restrict__ E, const int *__restrict__ Emin,
 }
 
 // Column exponent maxima of B (column-major k x n) with the same headroom
-// rule: block = column, threads walk p; also folds the column's exponent
-// range into *range.
+// rule: block = column, threads walk p (all of a thread's loads issued before
+// its max/min chain); the block reduces within each wave by shuffles, then
+// across the four waves through LDS. Also folds the column's exponent range
+// into *range.
 __global__ __launch_bounds__(kThreads) void k_col_exp(const double2 *__restrict__ B, int64_t k,
                                                       int *__restrict__ F, int *__restrict__ range) {
   const int64_t j = blockIdx.x;
   int e = kEmpty, lo = kNonFinite;
   double mx = 0.0;
   bool bad = false;
-  for (int64_t p = threadIdx.x; p < k; p += kThreads) {
-    double h = B[p + j * k].x;
-    if (!mf_isfinite(h)) bad = true;
-    else if (h != 0.0) {
-      const int x = dexp(h);
-      e = max(e, x);
-      lo = min(lo, x);
-      mx = fmax(mx, fabs(h));
+  const double2 *col = B + j * k;
+  for (int64_t p0 = 0; p0 < k; p0 += 8 * kThreads) {
+    double h[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int64_t p = p0 + threadIdx.x + (int64_t)u * kThreads;
+      h[u] = p < k ? col[p].x : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (!mf_isfinite(h[u])) bad = true;
+      else if (h[u] != 0.0) {
+        const int x = dexp(h[u]);
+        e = max(e, x);
+        lo = min(lo, x);
+        mx = fmax(mx, fabs(h[u]));
+      }
     }
   }
   if (bad) e = kNonFinite;
-  __shared__ int rmax[kThreads], rmin[kThreads];
-  __shared__ double rmx[kThreads];
-  rmax[threadIdx.x] = e;
-  rmin[threadIdx.x] = lo;
-  rmx[threadIdx.x] = mx;
-  __syncthreads();
-  for (int s = kThreads / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      rmax[threadIdx.x] = max(rmax[threadIdx.x], rmax[threadIdx.x + s]);
-      rmin[threadIdx.x] = min(rmin[threadIdx.x], rmin[threadIdx.x + s]);
-      rmx[threadIdx.x] = fmax(rmx[threadIdx.x], rmx[threadIdx.x + s]);
-    }
-    __syncthreads();
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    e = max(e, __shfl_xor(e, off));
+    lo = min(lo, __shfl_xor(lo, off));
+    mx = fmax(mx, __shfl_xor(mx, off));
   }
+  __shared__ int rmax[kThreads / 64], rmin[kThreads / 64];
+  __shared__ double rmx[kThreads / 64];
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    rmax[wv] = e;
+    rmin[wv] = lo;
+    rmx[wv] = mx;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    const int eh = headroom(rmax[0], rmx[0]);
+    for (int w = 1; w < kThreads / 64; ++w) {
+      e = max(e, rmax[w]);
+      lo = min(lo, rmin[w]);
+      mx = fmax(mx, rmx[w]);
+    }
+    const int eh = headroom(e, mx);
     F[j] = eh;
-    if (eh != kEmpty && eh != kNonFinite) atomicMax(range, eh - rmin[0]);
+    if (eh != kEmpty && eh != kNonFinite) atomicMax(range, eh - lo);
   }
 }
 
-__global__ void k_fill_int(int *__restrict__ p, int64_t n, int v) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
+// Initial state of the row scan: E = empty, Emin = non-finite, Mbits = 0,
+// both ranges = 0 (one launch).
+__global__ void k_scan_init(int *__restrict__ E, int *__restrict__ Emin, unsigned long long *__restrict__ Mbits,
+                            int *__restrict__ range, int64_t m) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) {
+    E[i] = kEmpty;
+    Emin[i] = kNonFinite;
+    Mbits[i] = 0ull;
+  }
+  if (i < 2) range[i] = 0;
 }
 
 // Base-256 digits of x (|x| < 127, exact DD), S <= MAXS of them, in
@@ -943,10 +969,7 @@ hipError_t ozaki_scan(const double2 *A, const double2 *B, int64_t m, int64_t k, 
   if (m <= 0 || n <= 0 || k <= 0) return hipSuccess;
   if (!oz_args_ok(k, S) || work_bytes < ozaki_workspace_bytes(m, k, n, S)) return hipErrorInvalidValue;
   OzWork w = carve(work, m, k, n, S);
-  hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, w.E, m, kEmpty);
-  hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, w.Emin, m, kNonFinite);
-  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, s, w.range, (int64_t)2, 0);
-  hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((2 * m + 255) / 256)), dim3(256), 0, s, (int *)w.Mbits, 2 * m, 0);
+  hipLaunchKernelGGL(k_scan_init, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, w.E, w.Emin, w.Mbits, w.range, m);
   const int64_t kchunk = 64;
   hipLaunchKernelGGL(k_row_exp, dim3((unsigned)((m + kThreads - 1) / kThreads), (unsigned)((k + kchunk - 1) / kchunk)),
                      dim3(kThreads), 0, s, A, m, k, kchunk, w.E, w.Emin, w.Mbits);
