@@ -82,6 +82,28 @@ def test_trsm_zero_skips_match_oracle(gpu):
         check(side, uplo, ta, "N", m, n, 1.0, A, B0, "skips %s%s%s" % (side, uplo, ta))
 
 
+
+def test_trsm_signed_zeros_and_zero_leading_terms(gpu):
+    """Updates whose two leading terms are both zero take dd_sub's special
+    case (the sign of the zero, the low parts subtracted as they are): -0.0
+    entries, entries (0, tiny) and zero coefficients, mixed into ordinary
+    ones, must give the shim's bits in every nest."""
+    rng = np.random.default_rng(5)
+    m, n = 96, 70
+    for side, uplo, ta in itertools.product("LR", "UL", "NT"):
+        s = m if side == "L" else n
+        A, B0 = system(rng, side, uplo, "N", m, n)
+        for _ in range(s // 2):
+            i, j = sorted(rng.integers(0, s, size=2))
+            if i != j:
+                if uplo == "L":
+                    i, j = j, i
+                A[i, j] = (0.0, 0.0) if rng.random() < 0.5 else (-0.0, 0.0)
+        idx = rng.integers(0, m, size=40), rng.integers(0, n, size=40)
+        for t, (i, j) in enumerate(zip(*idx)):
+            B0[i, j] = (-0.0, 0.0) if t % 3 == 0 else (0.0, 2.0 ** -1060) if t % 3 == 1 else (-0.0, -2.0 ** -1070)
+        check(side, uplo, ta, "N", m, n, 1.0, A, B0, "zeros %s%s%s" % (side, uplo, ta))
+
 def test_trsm_alpha_zero_and_quick_returns(gpu):
     rng = np.random.default_rng(5)
     A, B0 = system(rng, "L", "U", "N", 20, 10)
