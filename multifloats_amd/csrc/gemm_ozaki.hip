@@ -455,7 +455,8 @@ hipError_t launch_main(const signed char *As, const signed char *Bs, const int *
 }
 
 // ---------------------------------------------------------------------------
-// Ozaki scheme II (CRT form). The S digits of an element are one integer
+// CRT form (the modular "scheme II" of Ozaki, Uchino and Imamura, 2025, in a
+// design of our own). The S digits of an element are one integer
 // X = sum_s d_s 256^(S-1-s) (|X| < 2^(8S-1)), so C_int = A_int B_int is an
 // exact integer product. Instead of S(S+1)/2 digit-pair products it is formed
 // modulo N pairwise-coprime moduli m_t <= 256 (one plain INT8 GEMM each: the
@@ -843,7 +844,7 @@ constexpr int kMinSlices = 7;
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 namespace {
-// Scheme II: moduli needed for S digits and inner dimension k, i.e. the
+// CRT form: moduli needed for S digits and inner dimension k, i.e. the
 // smallest N with M = prod_{t<N} m_t > 2 k 2^(16S-2) (with margin); 0 if the
 // 40 moduli do not reach it.
 int oz2_count(int S, int64_t k) {
@@ -855,8 +856,8 @@ int oz2_count(int S, int64_t k) {
   }
   return 0;
 }
-// Scheme II pays off once the 256 x 256 tiles are mostly full; below that,
-// and wherever N moduli are not enough, the triangular form runs.
+// The CRT form pays off once the 256 x 256 tiles are mostly full; below
+// that, and wherever N moduli are not enough, the triangular form runs.
 bool use_oz2(int64_t m, int64_t n, int64_t k, int S) {
   return m >= 512 && n >= 512 && oz2_count(S, k) > 0;
 }
@@ -944,7 +945,7 @@ OzWork carve(void *work, int64_t m, int64_t k, int64_t n, int S) {
   w.Bs = (signed char *)p;
   return w;
 }
-// scheme II operand and residue planes, in place of the slices
+// CRT-form residue planes of A and B and digit planes, in place of the slices
 struct Oz2Work {
   signed char *As, *Bs;
   unsigned char *R;

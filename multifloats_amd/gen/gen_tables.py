@@ -265,7 +265,7 @@ def main():
     out.append(arr("mf_erfcx_lo", ex_lo))
     out.append(dd_pair("mf_inv_sqrtpi", 1 / mpmath.sqrt(pi)))
 
-    # ---- Ozaki scheme II: moduli for the CRT form of the INT8 GEMM ----
+    # ---- Ozaki CRT form: moduli and CRT constants of the INT8 GEMM ----
     mods = [256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193, 191, 181, 179,
             173, 167, 163, 157, 151, 149, 139, 137, 131, 127, 113, 109, 107, 103, 101, 97, 89, 83, 79, 73, 71]
     for a in range(len(mods)):
@@ -275,7 +275,7 @@ def main():
     for mm in mods:
         big = (1 << 180) // mm
         rl += [float((big >> (36 * (4 - j))) & ((1 << 36) - 1)) for j in range(5)]
-    out.append("\n/* Ozaki scheme II: pairwise-coprime moduli (<= 256, so balanced residues are int8),\n"
+    out.append("\n/* Ozaki CRT form: pairwise-coprime moduli (<= 256, so balanced residues are int8),\n"
                "   and 1/m as five 36-bit limbs */\n")
     out.append("#define MF_OZ2_MODULI %d\n" % len(mods))
     out.append("#ifdef __cplusplus\nstatic constexpr int mf_oz2_m_host[%d] = {%s};\n#endif\n"
