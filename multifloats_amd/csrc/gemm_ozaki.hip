@@ -807,15 +807,35 @@ __global__ __launch_bounds__(kThreads) void k_oz2_crt(const unsigned char *__res
         Lr[l - 1] += c;
       }
       Lr[0] -= __builtin_floor(Lr[0] * iT36) * T36;  // integer part dropped: L0 in [0, 2^36)
-      if (Lr[0] >= 0x1p35) Lr[0] -= T36;             // centre: x / M in [-1/2, 1/2)
-      // x / M = sum_l L_l 2^(-36 (l + 1)), every term exact
-      ddv f = mk(Lr[0] * iT36, 0.0);
-      double sc36 = iT36;
+      // Centre x / M into [-1/2, 1/2): a fraction of 1/2 or more stands for
+      // minus its 180-bit complement (limbwise 2^36 - 1 - L_l, plus one unit
+      // at the bottom, carried), so every limb is non-negative.
+      const bool neg = Lr[0] >= 0x1p35;
+      if (neg) {
 #pragma unroll
-      for (int l = 1; l < kO2Limbs; ++l) {
-        sc36 *= iT36;
-        f = dd_add(f, mk(Lr[l] * sc36, 0.0));
+        for (int l = 0; l < kO2Limbs; ++l) Lr[l] = (T36 - 1.0) - Lr[l];
+        Lr[kO2Limbs - 1] += 1.0;
+#pragma unroll
+        for (int l = kO2Limbs - 1; l > 0; --l) {
+          const double c = __builtin_floor(Lr[l] * iT36);
+          Lr[l] -= c * T36;
+          Lr[l - 1] += c;
+        }
       }
+      // |x / M| = sum_l L_l 2^(-36 (l + 1)), non-overlapping non-negative
+      // terms, summed from the bottom: a nonzero term is at least
+      // 2^(-36 (l + 1)), no less than the rounded sum of those below it, so
+      // every fast_two_sum is exact; the errors, decreasing by 2^36 a level,
+      // form the low part.
+      static_assert(kO2Limbs == 5, "five limbs below");
+      double sm = Lr[4] * 0x1p-180, e3, e2, e1, e0;
+      fast_two_sum(Lr[3] * 0x1p-144, sm, sm, e3);
+      fast_two_sum(Lr[2] * 0x1p-108, sm, sm, e2);
+      fast_two_sum(Lr[1] * 0x1p-72, sm, sm, e1);
+      fast_two_sum(Lr[0] * 0x1p-36, sm, sm, e0);
+      ddv f;
+      fast_two_sum(sm, e0 + (e1 + (e2 + e3)), f.hi, f.lo);
+      if (neg) f = mk(-f.hi, -f.lo);
       const int ei = E[i], fj = F[j];
       double2 out;
       if (ei == kNonFinite || fj == kNonFinite) {
