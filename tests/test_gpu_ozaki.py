@@ -256,3 +256,23 @@ def test_ozaki_products_stage(gpu):
         mf.dev_ozaki_multiply(dC.data_ptr(), m, k, n, S, work.data_ptr(), wb, s)
         torch.cuda.synchronize()
         assert np.array_equal(dC.cpu().numpy().view(np.uint64), full.view(np.uint64))
+
+
+@pytest.mark.parametrize("shape", [(512, 1, 512), (515, 3, 600), (600, 33, 513), (1024, 64, 768)])
+def test_ozaki_crt_form_tiny_k_and_uneven_tile_grids(gpu, shape):
+    """k far below one 2-block stage (zero-padded K blocks), and tile grids
+    that do not split evenly over the XCDs (513 columns: 3 tile columns, so
+    the column-major tile order runs), all outputs checked."""
+    m, k, n = shape
+    rng = np.random.default_rng(m + 3 * k + 5 * n)
+    A, B = mats(rng, m, k, n, 10)
+    for S in (0, 9, 16):
+        C = mf.matmul_ozaki(A, B, S)
+        used = mf.ozaki_slices_used()
+        if used == 0:
+            continue  # the slice rule chose the reference-order GEMM (tested elsewhere)
+        assert mf.ozaki_moduli(m, k, n, used) > 0
+        rr, cc = _sample(rng, m, n, 32)
+        err, mag = nt.gemm_err(A, B, C, rr, cc, threads=4)
+        bound = BOUND if S == 0 else 2.0 ** (-8 * used + 24)
+        assert (err <= bound * mag).all(), (S, used, np.log2((err / mag).max()))
