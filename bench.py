@@ -442,7 +442,8 @@ def bench_elementwise(args, torch, mf):
             "cos": datagen.sin_inputs, "tan": datagen.sin_inputs, "exp2": datagen.exp_inputs,
             "expm1": datagen.config0, "log2": datagen.log_inputs, "log10": datagen.log_inputs,
             "log1p": datagen.config0, "sinh": datagen.erf_inputs, "cosh": datagen.erf_inputs,
-            "tanh": datagen.erf_inputs, "erfc": datagen.erf_inputs, "erfcx": datagen.erf_inputs}
+            "tanh": datagen.erf_inputs, "erfc": datagen.erf_inputs, "erfcx": datagen.erf_inputs,
+            "asinh": datagen.erf_inputs, "acosh": datagen.acosh_inputs, "atanh": datagen.atanh_inputs}
     ops = {nm: getattr(mf, "MFHIP_" + nm.upper()) for nm in gens}
     for nm, gen in gens.items():
         x = torch.from_numpy(gen(rng, n)).cuda()

@@ -448,6 +448,9 @@ MF_UNARY(coshdd_v, mfhip::OP_COSH)
 MF_UNARY(tanhdd_v, mfhip::OP_TANH)
 MF_UNARY(erfcdd_v, mfhip::OP_ERFC)
 MF_UNARY(erfcxdd_v, mfhip::OP_ERFCX)
+MF_UNARY(asinhdd_v, mfhip::OP_ASINH)
+MF_UNARY(acoshdd_v, mfhip::OP_ACOSH)
+MF_UNARY(atanhdd_v, mfhip::OP_ATANH)
 
 float64x2 sumdd(const float64x2 *a, int64_t n) {
   float64x2 r;

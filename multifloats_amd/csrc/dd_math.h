@@ -1,7 +1,7 @@
 // DD transcendentals for the elementwise kernels.
 //
-// exp, exp2, expm1, log, log2, log10, log1p, sin, cos, tan, sinh, cosh and
-// tanh keep the reference's algorithms step for step (the same range
+// exp, exp2, expm1, log, log2, log10, log1p, sin, cos, tan, sinh, cosh,
+// tanh, asinh, acosh and atanh keep the reference's algorithms step for step (the same range
 // reductions, table sizes, polynomial degrees, evaluation trees and special
 // cases), so their results are bit-identical to the reference CPU functions:
 //   dd_exp   src/float64x2/exp_log.inc:237-277 with exp2_from_reduced 26-72
@@ -11,7 +11,8 @@
 //   dd_sin / dd_cos / dd_tan  src/float64x2/trig.inc:241-289,
 //           reduce_pi_half 114-147, sin_eval/cos_eval/sincos_eval 165-215
 //   dd_sinh / dd_cosh / dd_tanh  src/float64x2/hyp.inc:2-87
-//   estrin10/12/17/24  src/float64x2/poly.inc:33-217 (neval's trees)
+//   dd_asinh / dd_acosh / dd_atanh  src/float64x2/inv_trig.inc:226-291
+//   estrin10/12/14/17/24  src/float64x2/poly.inc:33-217 (neval's trees)
 // The constant tables are closed forms produced by gen/gen_tables.py.
 //
 // dd_erf, dd_erfc and dd_erfcx are this project's own design, not the
@@ -56,6 +57,25 @@ MF_HD ddv estrin12(ddv x, const double *hi, const double *lo) {
   ddv p811 = dd_add(p89, dd_mul(p1011, x2));
   ddv top = dd_add(p811, dd_mul(cdd(hi, lo, 12), x4));
   return dd_add(dd_add(p03, dd_mul(p47, x4)), dd_mul(top, x8));
+}
+
+// Degree-14 case of the same tree (asinh, atanh).
+MF_HD ddv estrin14(ddv x, const double *hi, const double *lo) {
+  ddv x2 = dd_mul(x, x);
+  ddv x4 = dd_mul(x2, x2);
+  ddv x8 = dd_mul(x4, x4);
+  ddv p01 = dd_add(cdd(hi, lo, 0), dd_mul(cdd(hi, lo, 1), x));
+  ddv p23 = dd_add(cdd(hi, lo, 2), dd_mul(cdd(hi, lo, 3), x));
+  ddv p45 = dd_add(cdd(hi, lo, 4), dd_mul(cdd(hi, lo, 5), x));
+  ddv p67 = dd_add(cdd(hi, lo, 6), dd_mul(cdd(hi, lo, 7), x));
+  ddv p89 = dd_add(cdd(hi, lo, 8), dd_mul(cdd(hi, lo, 9), x));
+  ddv p1011 = dd_add(cdd(hi, lo, 10), dd_mul(cdd(hi, lo, 11), x));
+  ddv p1213 = dd_add(cdd(hi, lo, 12), dd_mul(cdd(hi, lo, 13), x));
+  ddv p03 = dd_add(p01, dd_mul(p23, x2));
+  ddv p47 = dd_add(p45, dd_mul(p67, x2));
+  ddv p811 = dd_add(p89, dd_mul(p1011, x2));
+  ddv p1214 = dd_add(p1213, dd_mul(cdd(hi, lo, 14), x2));
+  return dd_add(dd_add(p03, dd_mul(p47, x4)), dd_mul(dd_add(p811, dd_mul(p1214, x4)), x8));
 }
 
 // c0 + ... + c15 x^15 by the degree-15 tree (shared by the 17 and 24 cases).
@@ -268,6 +288,48 @@ MF_HD ddv dd_tanh(ddv x) {
   ddv em2 = dd_exp(m2);
   ddv r = dd_div(dd_sub(mk(1.0, 0.0), em2), dd_add(mk(1.0, 0.0), em2));
   return sign ? dd_neg(r) : r;
+}
+
+// ---------------------------------------------------------------- asinh, acosh, atanh
+// src/float64x2/inv_trig.inc:226-254: x P(x^2) below |x| = 0.01; log(2|x|)
+// beyond 1e150; else log1p(|x| + x^2 / (sqrt(x^2 + 1) + 1)) below 1 and
+// log(|x| + sqrt(x^2 + 1)) from 1 on, with the sign restored.
+MF_HD ddv dd_asinh(ddv x) {
+  if (x.hi == 0.0 && x.lo == 0.0) return x;
+  if (!mf_isfinite(x.hi)) return mk(x.hi, 0.0);  // asinh(+-inf) = +-inf, NaN stays
+  if (__builtin_fabs(x.hi) < 0.01) return dd_mul(x, estrin14(dd_mul(x, x), mf_asinhc_hi, mf_asinhc_lo));
+  const bool sign = x.hi < 0.0;
+  const ddv ax = sign ? dd_neg(x) : x;
+  if (ax.hi > 1e150) {
+    const ddv r = dd_add(dd_log(ax), mk(mf_ln2_hi, mf_ln2_lo));
+    return sign ? dd_neg(r) : r;
+  }
+  const ddv axsq = dd_mul(ax, ax);
+  const ddv root = dd_sqrt(dd_add(axsq, mk(1.0, 0.0)));
+  ddv res;
+  if (ax.hi < 1.0) res = dd_log1p(dd_add(ax, dd_div(axsq, dd_add(root, mk(1.0, 0.0)))));
+  else res = dd_log(dd_add(ax, root));
+  return sign ? dd_neg(res) : res;
+}
+
+// inv_trig.inc:256-268: NaN below 1, log(2x) beyond 1e150, else
+// log(x + sqrt(x^2 - 1)).
+MF_HD ddv dd_acosh(ddv x) {
+  if (!mf_isfinite(x.hi)) return mk(x.hi > 0.0 ? x.hi : __builtin_nan(""), 0.0);
+  if (x.hi < 1.0) return mk(__builtin_nan(""), 0.0);
+  if (x.hi > 1e150) return dd_add(dd_log(x), mk(mf_ln2_hi, mf_ln2_lo));
+  return dd_log(dd_add(x, dd_sqrt(dd_sub(dd_mul(x, x), mk(1.0, 0.0)))));
+}
+
+// inv_trig.inc:270-291: x P(x^2) below |x| = 0.01, NaN beyond 1, else
+// log1p(2x / (1 - x)) / 2.
+MF_HD ddv dd_atanh(ddv x) {
+  if (x.hi == 0.0 && x.lo == 0.0) return x;
+  if (!mf_isfinite(x.hi)) return mk(__builtin_nan(""), 0.0);  // atanh(+-inf) and NaN: NaN
+  if (__builtin_fabs(x.hi) > 1.0) return mk(__builtin_nan(""), 0.0);
+  if (__builtin_fabs(x.hi) < 0.01) return dd_mul(x, estrin14(dd_mul(x, x), mf_atanhc_hi, mf_atanhc_lo));
+  const ddv arg = dd_div(dd_add(x, x), dd_sub(mk(1.0, 0.0), x));
+  return dd_mul(mk(0.5, 0.0), dd_log1p(arg));
 }
 
 // ---------------------------------------------------------------- sin

@@ -39,6 +39,9 @@ __device__ __forceinline__ ddv unary_op(ddv x) {
   else if constexpr (OP == mfhip::OP_TANH) return dd_tanh(x);
   else if constexpr (OP == mfhip::OP_ERFC) return dd_erfc(x);
   else if constexpr (OP == mfhip::OP_ERFCX) return dd_erfcx(x);
+  else if constexpr (OP == mfhip::OP_ASINH) return dd_asinh(x);
+  else if constexpr (OP == mfhip::OP_ACOSH) return dd_acosh(x);
+  else if constexpr (OP == mfhip::OP_ATANH) return dd_atanh(x);
   else return x;
 }
 
@@ -180,6 +183,9 @@ hipError_t unary(int op, const double2 *a, double2 *c, int64_t n, hipStream_t s)
     case OP_TANH: return run_unary<OP_TANH>(a, c, n, s);
     case OP_ERFC: return run_unary<OP_ERFC>(a, c, n, s);
     case OP_ERFCX: return run_unary<OP_ERFCX>(a, c, n, s);
+    case OP_ASINH: return run_unary<OP_ASINH>(a, c, n, s);
+    case OP_ACOSH: return run_unary<OP_ACOSH>(a, c, n, s);
+    case OP_ATANH: return run_unary<OP_ATANH>(a, c, n, s);
     default: return hipErrorInvalidValue;
   }
 }

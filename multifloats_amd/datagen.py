@@ -118,6 +118,16 @@ def erf_inputs(rng, n):
     return _with_lo(rng, rng.uniform(-6.0, 6.0, size=n))
 
 
+def acosh_inputs(rng, n):
+    """x in [1, 1e3], log-spaced above 1 (acosh's domain)."""
+    return _with_lo(rng, 1.0 + 10.0 ** rng.uniform(-4.0, 3.0, size=n))
+
+
+def atanh_inputs(rng, n):
+    """x in (-1, 1), within 1e-4 of neither end."""
+    return _with_lo(rng, rng.uniform(-0.9999, 0.9999, size=n))
+
+
 def sin_inputs(rng, n, near_frac=0.01):
     """hi ~ U[-1e4, 1e4]; a fraction of points sits within 1e-12 of k*pi/2.
 

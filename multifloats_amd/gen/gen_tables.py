@@ -16,7 +16,9 @@ fixed by mathematics (so a bit-exact restatement must use the same values):
     pi/4 and 1/sqrt(2) (src/float64x2/trig.inc:114-187);
   - exp2/log10/expm1/log1p/sinh: log10(2), the exp2 clamps, and the Taylor
     coefficients 1/(k+1)!, 2/(2k+1) and 1/(2k+1)! (exp_log.inc:119-128,
-    343-353, 407-469; hyp.inc:2-18).
+    343-353, 407-469; hyp.inc:2-18);
+  - asinh/atanh: the Taylor coefficients (-1)^k (2k)!/(4^k k!^2 (2k+1)) and
+    1/(2k+1) of asinh(x)/x and atanh(x)/x in x^2 (inv_trig.inc:226-280).
 
 The erf, erfc and erfcx tables are this project's own design (the reference fits erf with
 rational approximations; see csrc/dd_math.h for the piecewise Taylor scheme):
@@ -172,6 +174,14 @@ def main():
     out.append("\n/* sinh(x)/x Taylor coefficients in x^2: 1/(2k+1)!, k = 0..12 */\n")
     out.append(arr("mf_sinhc_hi", [limbs(v)[0] for v in sh]))
     out.append(arr("mf_sinhc_lo", [limbs(v)[1] for v in sh]))
+    ash = [(-1) ** k * mpmath.factorial(2 * k) / (4 ** k * mpmath.factorial(k) ** 2 * (2 * k + 1)) for k in range(15)]
+    out.append("\n/* asinh(x)/x Taylor coefficients in x^2: (-1)^k (2k)! / (4^k k!^2 (2k+1)), k = 0..14 */\n")
+    out.append(arr("mf_asinhc_hi", [limbs(v)[0] for v in ash]))
+    out.append(arr("mf_asinhc_lo", [limbs(v)[1] for v in ash]))
+    ath = [mpf(1) / (2 * k + 1) for k in range(15)]
+    out.append("\n/* atanh(x)/x Taylor coefficients in x^2: 1/(2k+1), k = 0..14 */\n")
+    out.append(arr("mf_atanhc_hi", [limbs(v)[0] for v in ath]))
+    out.append(arr("mf_atanhc_lo", [limbs(v)[1] for v in ath]))
 
     # ---- erf ----
     two_rtpi = 2 / mpmath.sqrt(pi)

@@ -503,6 +503,22 @@ static dd o_estrin12(dd x, const double *h, const double *l) {
   dd top = o_add(p811, o_mul(cdd(h, l, 12), x4));
   return o_add(o_add(p03, o_mul(p47, x4)), o_mul(top, x8));
 }
+/* poly.inc:116-129 (neval, degree 14) */
+static dd o_estrin14(dd x, const double *h, const double *l) {
+  dd x2 = o_mul(x, x), x4 = o_mul(x2, x2), x8 = o_mul(x4, x4);
+  dd p01 = o_add(cdd(h, l, 0), o_mul(cdd(h, l, 1), x));
+  dd p23 = o_add(cdd(h, l, 2), o_mul(cdd(h, l, 3), x));
+  dd p45 = o_add(cdd(h, l, 4), o_mul(cdd(h, l, 5), x));
+  dd p67 = o_add(cdd(h, l, 6), o_mul(cdd(h, l, 7), x));
+  dd p89 = o_add(cdd(h, l, 8), o_mul(cdd(h, l, 9), x));
+  dd p1011 = o_add(cdd(h, l, 10), o_mul(cdd(h, l, 11), x));
+  dd p1213 = o_add(cdd(h, l, 12), o_mul(cdd(h, l, 13), x));
+  dd p03 = o_add(p01, o_mul(p23, x2));
+  dd p47 = o_add(p45, o_mul(p67, x2));
+  dd p811 = o_add(p89, o_mul(p1011, x2));
+  dd p1214 = o_add(p1213, o_mul(cdd(h, l, 14), x2));
+  return o_add(o_add(p03, o_mul(p47, x4)), o_mul(o_add(p811, o_mul(p1214, x4)), x8));
+}
 
 /* src/float64x2/exp_log.inc:26-72 (exp2_from_reduced) */
 static dd o_exp2_reduced(double n_d, double rh, double rl) {
@@ -751,6 +767,43 @@ ARR1(or_log1p, o_log1p)
 ARR1(or_sinh, o_sinh)
 ARR1(or_cosh, o_cosh)
 ARR1(or_tanh, o_tanh)
+
+/* src/float64x2/inv_trig.inc:226-254 (asinh) */
+static dd o_asinh(dd x) {
+  if (x.hi == 0.0 && x.lo == 0.0) return x;
+  if (!isfinite(x.hi)) return mkdd(asinh(x.hi), 0.0);
+  if (fabs(x.hi) < 0.01) return o_mul(x, o_estrin14(o_mul(x, x), mf_asinhc_hi, mf_asinhc_lo));
+  int sign = x.hi < 0.0;
+  dd ax = sign ? o_neg(x) : x;
+  if (ax.hi > 1e150) {
+    dd r = o_add(o_log(ax), mkdd(mf_ln2_hi, mf_ln2_lo));
+    return sign ? o_neg(r) : r;
+  }
+  dd axsq = o_mul(ax, ax);
+  dd root = o_sqrt(o_add(axsq, mkdd(1.0, 0.0)));
+  dd res = ax.hi < 1.0 ? o_log1p(o_add(ax, o_div(axsq, o_add(root, mkdd(1.0, 0.0)))))
+                       : o_log(o_add(ax, root));
+  return sign ? o_neg(res) : res;
+}
+/* inv_trig.inc:256-268 (acosh) */
+static dd o_acosh(dd x) {
+  if (!isfinite(x.hi)) return mkdd(acosh(x.hi), 0.0);
+  if (x.hi < 1.0) return mkdd(NAN, 0.0);
+  if (x.hi > 1e150) return o_add(o_log(x), mkdd(mf_ln2_hi, mf_ln2_lo));
+  return o_log(o_add(x, o_sqrt(o_sub(o_mul(x, x), mkdd(1.0, 0.0)))));
+}
+/* inv_trig.inc:270-291 (atanh) */
+static dd o_atanh(dd x) {
+  if (x.hi == 0.0 && x.lo == 0.0) return x;
+  if (!isfinite(x.hi)) return mkdd(atanh(x.hi), 0.0);
+  if (fabs(x.hi) > 1.0) return mkdd(NAN, 0.0);
+  if (fabs(x.hi) < 0.01) return o_mul(x, o_estrin14(o_mul(x, x), mf_atanhc_hi, mf_atanhc_lo));
+  dd arg = o_div(o_add(x, x), o_sub(mkdd(1.0, 0.0), x));
+  return o_mul(mkdd(0.5, 0.0), o_log1p(arg));
+}
+ARR1(or_asinh, o_asinh)
+ARR1(or_acosh, o_acosh)
+ARR1(or_atanh, o_atanh)
 
 /* ------------------------------------------------- exact accumulator ---
  * A fixed-point superaccumulator over 2^-1216 .. 2^1120 in 32-bit digits held

@@ -81,6 +81,9 @@ program bridge_check
     write(u) tanh(x)
     write(u) erfc(x)
     write(u) erfc_scaled(x)
+    write(u) asinh(x)
+    write(u) acosh(x * x + exp(y * y))  ! argument >= 1
+    write(u) atanh(x)
     ! xGEMM: C = x(1) A B + y(1) C with A passed transposed and C padded to m + 2 rows
     at = transpose(a)
     allocate(cc(m + 2, n))

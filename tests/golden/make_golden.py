@@ -7,12 +7,12 @@ known-answer values the reference's own tests use; expected outputs come from
     reference sources by oracle/Makefile), called through its C ABI: adddd,
     subdd, muldd, divdd, sqrtdd, expdd, logdd, sindd, erfdd, matmuldd_mm/mv/vm,
     and cosdd, tandd, exp2dd, expm1dd, log2dd, log10dd, log1pdd, sinhdd,
-    coshdd, tanhdd, erfcdd, erfcxdd;
+    coshdd, tanhdd, erfcdd, erfcxdd, asinhdd, acoshdd, atanhdd;
   - mpmath at 320 bits for the transcendental values (stored as three limbs so
     relative errors down to 1e-40 can be measured).
 Every fixture is data only (inputs and expected outputs).
 
-Usage: python3 tests/golden/make_golden.py
+Usage: python3 tests/golden/make_golden.py [fixture ...]   (default: all)
 """
 import ctypes
 import os
@@ -41,7 +41,8 @@ def load_ref():
     for nm in ("adddd", "subdd", "muldd", "divdd"):
         getattr(L, nm).argtypes = [F2, F2]
         getattr(L, nm).restype = F2
-    for nm in ("sqrtdd", "expdd", "logdd", "sindd", "erfdd", "erfcdd", "erfcxdd") + tuple(f + "dd" for f in TRANSC2):
+    for nm in ("sqrtdd", "expdd", "logdd", "sindd", "erfdd", "erfcdd", "erfcxdd") + tuple(
+            f + "dd" for f in TRANSC2 + TRANSC3):
         getattr(L, nm).argtypes = [F2]
         getattr(L, nm).restype = F2
     P, I64 = ctypes.c_void_p, ctypes.c_int64
@@ -158,6 +159,7 @@ def transc(L, rng):
 
 
 TRANSC2 = ("cos", "tan", "exp2", "expm1", "log2", "log10", "log1p", "sinh", "cosh", "tanh")
+TRANSC3 = ("asinh", "acosh", "atanh")
 
 
 def transc2(L, rng):
@@ -213,6 +215,45 @@ def transc2(L, rng):
                 pass
         out[nm + "_exact"] = exact
     np.savez_compressed(os.path.join(HERE, "transc2.npz"), **out)
+
+
+def transc3(L, rng):
+    """asinh, acosh, atanh: seeded draws over every branch of the reference's
+    algorithms (|x| < 0.01 Taylor, below / above 1, beyond 1e150, the domain
+    edges), special values, and 320-bit values for accuracy."""
+    n = 1500
+    w = datagen._with_lo
+    U = rng.uniform
+    edge = np.array([(0.01, 0.0), (-0.01, 0.0), (np.nextafter(0.01, 0), 0.0), (1.0, 0.0), (-1.0, 0.0),
+                     (1.0, -1e-20), (-1.0, 1e-20), (1.0, 1e-20), (np.nextafter(1.0, 0), 0.0), (1e150, 0.0),
+                     (1.0000000000000002e150, 0.0), (-1e151, 0.0), (2.0, 0.0), (0.5, 0.0)])
+    cases = {
+        "asinh": np.concatenate([w(rng, U(-0.01, 0.01, 300)), w(rng, U(-1, 1, n)), w(rng, U(-40, 40, 400)),
+                                 w(rng, 10.0 ** U(-30, 300, 300) * rng.choice([-1.0, 1.0], 300)), edge]),
+        "acosh": np.concatenate([w(rng, 1.0 + 10.0 ** U(-16, 0, 400)), w(rng, U(1, 40, n)),
+                                 w(rng, 10.0 ** U(1, 308, 300)), w(rng, U(-3, 1, 100)), edge]),
+        "atanh": np.concatenate([w(rng, U(-0.01, 0.01, 300)), w(rng, U(-1, 1, n)),
+                                 w(rng, (1.0 - 10.0 ** U(-16, -1, 300)) * rng.choice([-1.0, 1.0], 300)),
+                                 w(rng, U(-3, 3, 100)), edge]),
+    }
+    mpfn = {"asinh": mpmath.asinh, "acosh": mpmath.acosh, "atanh": mpmath.atanh}
+    out = {}
+    for nm, x in cases.items():
+        x = np.concatenate([x, np.array(SPECIALS)])
+        out[nm + "_in"] = x
+        out[nm + "_ref"] = call1(L, nm + "dd", x)
+        exact = np.full((x.shape[0], 3), np.nan)
+        for i in range(x.shape[0]):
+            if not np.isfinite(x[i, 0]):
+                continue
+            try:
+                v = mpfn[nm](mp_of(x[i]))
+                if mpmath.im(v) == 0 and mpmath.isfinite(v):
+                    exact[i] = limbs3(mpmath.re(v))
+            except (OverflowError, ValueError, ZeroDivisionError):
+                pass
+        out[nm + "_exact"] = exact
+    np.savez_compressed(os.path.join(HERE, "transc3.npz"), **out)
 
 
 def special2(L, rng):
@@ -308,18 +349,25 @@ def cmatmul(L, rng):
     np.savez_compressed(os.path.join(HERE, "cmatmul.npz"), **out)
 
 
-def main():
+def main(which):
     L = load_ref()
     rng = np.random.default_rng(20261203)
-    arith(L, rng)
-    transc(L, rng)
-    matmul(L, rng)
-    cmatmul(L, np.random.default_rng(777))
-    transc2(L, np.random.default_rng(4242))
-    special2(L, np.random.default_rng(777777))
-    for f in ("arith.npz", "transc.npz", "matmul.npz", "cmatmul.npz", "transc2.npz", "special2.npz"):
+    if not which or "arith" in which or "transc" in which or "matmul" in which:
+        # these three share one generator stream, so they are written together
+        arith(L, rng)
+        transc(L, rng)
+        matmul(L, rng)
+    if not which or "cmatmul" in which:
+        cmatmul(L, np.random.default_rng(777))
+    if not which or "transc2" in which:
+        transc2(L, np.random.default_rng(4242))
+    if not which or "special2" in which:
+        special2(L, np.random.default_rng(777777))
+    if not which or "transc3" in which:
+        transc3(L, np.random.default_rng(31337))
+    for f in ("arith.npz", "transc.npz", "matmul.npz", "cmatmul.npz", "transc2.npz", "special2.npz", "transc3.npz"):
         print(f, os.path.getsize(os.path.join(HERE, f)))
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -97,6 +97,9 @@ def test_fortran_bridge_routes_to_the_library(gpu, tmp_path):
         nt.assert_bits(r.take(nv), getattr(mf, nm)(mf.mul(x, x)), nm)
     for nm in ("log1p", "sinh", "cosh", "tanh", "erfc", "erfcx"):
         nt.assert_bits(r.take(nv), getattr(mf, nm)(x), nm)
+    nt.assert_bits(r.take(nv), mf.asinh(x), "asinh")
+    nt.assert_bits(r.take(nv), mf.acosh(mf.add(mf.mul(x, x), mf.exp(mf.mul(y, y)))), "acosh")
+    nt.assert_bits(r.take(nv), mf.atanh(x), "atanh")
     At = nt.cm(A.transpose(1, 0, 2))
     Cw = np.empty((m + 2, n, 2))
     Cw[:m] = A[:, :n]

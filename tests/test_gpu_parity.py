@@ -86,6 +86,42 @@ def test_transcendental2_bits_vs_oracle(gpu, fn):
     nt.assert_bits(getattr(mf, fn)(x), nt.or_unary(fn, x), fn)
 
 
+@pytest.mark.parametrize("fn", nt.TRANSC3)
+def test_inverse_hyperbolic_bits_golden(gpu, fn):
+    g = nt.load_golden("transc3.npz")
+    nt.assert_bits(getattr(mf, fn)(g[fn + "_in"]), g[fn + "_ref"], fn)
+
+
+def _transc3_inputs(rng, fn, n):
+    w, U = datagen._with_lo, rng.uniform
+    q = n // 4
+    if fn == "asinh":
+        return np.concatenate([w(rng, U(-0.01, 0.01, q)), w(rng, U(-1, 1, q)), w(rng, U(-50, 50, q)),
+                               w(rng, 10.0 ** U(-20, 300, n - 3 * q) * rng.choice([-1.0, 1.0], n - 3 * q))])
+    if fn == "acosh":
+        return np.concatenate([w(rng, 1.0 + 10.0 ** U(-16, 0, q)), w(rng, U(1, 50, q)),
+                               w(rng, 10.0 ** U(1, 308, q)), w(rng, U(-2, 1, n - 3 * q))])
+    return np.concatenate([w(rng, U(-0.01, 0.01, q)), w(rng, U(-1, 1, q)),
+                           w(rng, (1.0 - 10.0 ** U(-16, -1, q)) * rng.choice([-1.0, 1.0], q)),
+                           w(rng, U(-2, 2, n - 3 * q))])
+
+
+@pytest.mark.parametrize("fn", nt.TRANSC3)
+def test_inverse_hyperbolic_bits_vs_oracle(gpu, fn):
+    rng = np.random.default_rng(sum(map(ord, fn)) + 3)
+    x = _transc3_inputs(rng, fn, 200000)
+    nt.assert_bits(getattr(mf, fn)(x), nt.or_unary(fn, x), fn)
+    # the device entry point with the op code reaches the same kernel
+    torch = pytest.importorskip("torch")
+    xs = x[:5000]
+    dx = to_dev(torch, xs)
+    dy = torch.empty_like(dx)
+    mf.dev_unary(getattr(mf, "MFHIP_" + fn.upper()), dx.data_ptr(), dy.data_ptr(), xs.shape[0],
+                 torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    nt.assert_bits(dy.cpu().numpy(), nt.or_unary(fn, xs), "device " + fn)
+
+
 def test_transcendental2_device_entry(torch_cuda):
     """mfhip_unary with the new op codes reaches the same kernels."""
     torch = torch_cuda

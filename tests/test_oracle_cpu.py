@@ -30,6 +30,27 @@ def test_oracle_transcendentals2_match_reference_bits(fn):
     nt.assert_bits(nt.or_unary(fn, g[fn + "_in"]), g[fn + "_ref"], fn)
 
 
+@pytest.mark.parametrize("fn", nt.TRANSC3)
+def test_oracle_inverse_hyperbolics_match_reference_bits(fn):
+    """asinh / acosh / atanh restated from inv_trig.inc:226-291 against the
+    compiled reference on every branch (Taylor below 0.01, log1p and log
+    forms, the 1e150 cut, domain edges, specials)."""
+    g = nt.load_golden("transc3.npz")
+    nt.assert_bits(nt.or_unary(fn, g[fn + "_in"]), g[fn + "_ref"], fn)
+
+
+def test_inverse_hyperbolics_are_accurate_where_the_reference_is():
+    """A check of the fixture itself against 320-bit values: asinh holds DD
+    accuracy everywhere, and acosh / atanh wherever their argument is away
+    from the points where the reference's formulas cancel."""
+    g = nt.load_golden("transc3.npz")
+    for fn, cap in (("asinh", 1e-31), ("acosh", 1e-16), ("atanh", 1e-17)):
+        ex = g[fn + "_exact"]
+        fin = np.isfinite(ex[:, 0]) & (np.abs(ex[:, 0]) > 1e-290)
+        assert fin.sum() > 1000
+        assert nt.rel_err_vs_exact(g[fn + "_ref"][fin], ex[fin]).max() < cap, fn
+
+
 def test_oracle_matmul_matches_reference_bits():
     g = nt.load_golden("matmul.npz")
     O = nt.oracle()
