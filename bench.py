@@ -443,7 +443,9 @@ def bench_elementwise(args, torch, mf):
             "expm1": datagen.config0, "log2": datagen.log_inputs, "log10": datagen.log_inputs,
             "log1p": datagen.config0, "sinh": datagen.erf_inputs, "cosh": datagen.erf_inputs,
             "tanh": datagen.erf_inputs, "erfc": datagen.erf_inputs, "erfcx": datagen.erf_inputs,
-            "asinh": datagen.erf_inputs, "acosh": datagen.acosh_inputs, "atanh": datagen.atanh_inputs}
+            "asinh": datagen.erf_inputs, "acosh": datagen.acosh_inputs, "atanh": datagen.atanh_inputs,
+            "exp10": datagen.exp10_inputs, "exp2m1": datagen.exp_inputs, "exp10m1": datagen.exp10_inputs,
+            "log2p1": datagen.config0, "log10p1": datagen.config0}
     ops = {nm: getattr(mf, "MFHIP_" + nm.upper()) for nm in gens}
     for nm, gen in gens.items():
         x = torch.from_numpy(gen(rng, n)).cuda()

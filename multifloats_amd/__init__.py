@@ -33,7 +33,8 @@ DD_FMA_RENORM_INTERVAL = 8
 
 MFHIP_SQRT, MFHIP_EXP, MFHIP_LOG, MFHIP_SIN, MFHIP_ERF, MFHIP_NEG = range(6)
 (MFHIP_COS, MFHIP_TAN, MFHIP_EXP2, MFHIP_EXPM1, MFHIP_LOG2, MFHIP_LOG10, MFHIP_LOG1P, MFHIP_SINH, MFHIP_COSH,
- MFHIP_TANH, MFHIP_ERFC, MFHIP_ERFCX, MFHIP_ASINH, MFHIP_ACOSH, MFHIP_ATANH) = range(6, 21)
+ MFHIP_TANH, MFHIP_ERFC, MFHIP_ERFCX, MFHIP_ASINH, MFHIP_ACOSH, MFHIP_ATANH, MFHIP_EXP10, MFHIP_EXP2M1,
+ MFHIP_EXP10M1, MFHIP_LOG2P1, MFHIP_LOG10P1) = range(6, 26)
 MFHIP_ADD, MFHIP_SUB, MFHIP_MUL, MFHIP_DIV = range(4)
 MFHIP_DOT, MFHIP_SUM, MFHIP_NORM2, MFHIP_PRODUCT, MFHIP_MAXVAL, MFHIP_MINVAL = range(6)
 
@@ -68,7 +69,8 @@ def lib():
             getattr(L, nm).argtypes = [P, P, P, I64]
             getattr(L, nm).restype = None
         for nm in ("sqrtdd_v", "expdd_v", "logdd_v", "sindd_v", "erfdd_v", "erfcdd_v", "erfcxdd_v", "asinhdd_v",
-                   "acoshdd_v", "atanhdd_v") + tuple(f + "dd_v" for f in _TRANSC2):
+                   "acoshdd_v", "atanhdd_v", "exp10dd_v", "exp2m1dd_v", "exp10m1dd_v", "log2p1dd_v",
+                   "log10p1dd_v") + tuple(f + "dd_v" for f in _TRANSC2):
             getattr(L, nm).argtypes = [P, P, I64]
             getattr(L, nm).restype = None
         L.sumdd.argtypes = [P, I64]
@@ -365,6 +367,11 @@ def erfcx(a): return _unary("erfcxdd_v", a)  # Fortran erfc_scaled
 def asinh(a): return _unary("asinhdd_v", a)
 def acosh(a): return _unary("acoshdd_v", a)
 def atanh(a): return _unary("atanhdd_v", a)
+def exp10(a): return _unary("exp10dd_v", a)
+def exp2m1(a): return _unary("exp2m1dd_v", a)
+def exp10m1(a): return _unary("exp10m1dd_v", a)
+def log2p1(a): return _unary("log2p1dd_v", a)
+def log10p1(a): return _unary("log10p1dd_v", a)
 def add(a, b): return _binary("adddd_v", a, b)
 def sub(a, b): return _binary("subdd_v", a, b)
 def mul(a, b): return _binary("muldd_v", a, b)

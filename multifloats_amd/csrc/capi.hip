@@ -451,6 +451,11 @@ MF_UNARY(erfcxdd_v, mfhip::OP_ERFCX)
 MF_UNARY(asinhdd_v, mfhip::OP_ASINH)
 MF_UNARY(acoshdd_v, mfhip::OP_ACOSH)
 MF_UNARY(atanhdd_v, mfhip::OP_ATANH)
+MF_UNARY(exp10dd_v, mfhip::OP_EXP10)
+MF_UNARY(exp2m1dd_v, mfhip::OP_EXP2M1)
+MF_UNARY(exp10m1dd_v, mfhip::OP_EXP10M1)
+MF_UNARY(log2p1dd_v, mfhip::OP_LOG2P1)
+MF_UNARY(log10p1dd_v, mfhip::OP_LOG10P1)
 
 float64x2 sumdd(const float64x2 *a, int64_t n) {
   float64x2 r;

@@ -1,11 +1,13 @@
 // DD transcendentals for the elementwise kernels.
 //
-// exp, exp2, expm1, log, log2, log10, log1p, sin, cos, tan, sinh, cosh,
-// tanh, asinh, acosh and atanh keep the reference's algorithms step for step (the same range
+// exp, exp2, exp10, expm1, exp2m1, exp10m1, log, log2, log10, log1p,
+// log2p1, log10p1, sin, cos, tan, sinh, cosh, tanh, asinh, acosh and atanh keep the reference's algorithms step for step (the same range
 // reductions, table sizes, polynomial degrees, evaluation trees and special
 // cases), so their results are bit-identical to the reference CPU functions:
 //   dd_exp   src/float64x2/exp_log.inc:237-277 with exp2_from_reduced 26-72
 //   dd_exp2  exp_log.inc:109-128;  dd_expm1 407-427
+//   dd_exp10 exp_log.inc:364-397 with exp2_from_triple 86-106
+//   dd_exp2m1 / dd_exp10m1 / dd_log2p1 / dd_log10p1  exp_log.inc:479-519
 //   dd_log   exp_log.inc:329-341, log2 311-327, log2_kernel 280-308
 //   dd_log10 exp_log.inc:343-353;  dd_log1p 429-469
 //   dd_sin / dd_cos / dd_tan  src/float64x2/trig.inc:241-289,
@@ -331,6 +333,80 @@ MF_HD ddv dd_atanh(ddv x) {
   const ddv arg = dd_div(dd_add(x, x), dd_sub(mk(1.0, 0.0), x));
   return dd_mul(mk(0.5, 0.0), dd_log1p(arg));
 }
+
+// ------------------------------------------------- exp10 and the C23 forms
+// Three-limb running sum: T0 + T1 + T2 += x with two error-free additions
+// (src/multifloats_td.hh:30-43, tsum3 and tsum3_finalize).
+MF_HD void tsum3(double &T0, double &T1, double &T2, double x) {
+  double e;
+  two_sum(T0, x, T0, e);
+  two_sum(T1, e, T1, e);
+  T2 += e;
+}
+
+// 2^(h + l + e) for a triple with (h, l) normalised: split off the nearest
+// integer, carry the remainder as three limbs, evaluate 2^(T0 + T1) and
+// correct to first order for T2 (exp_log.inc:86-106).
+MF_HD ddv exp2_triple(double h, double l, double e) {
+  const double n_d = __builtin_rint(h);
+  double rh, rt;
+  two_sum(h, -n_d, rh, rt);
+  double T0 = rh, T1 = 0.0, T2 = 0.0;
+  tsum3(T0, T1, T2, rt);
+  tsum3(T0, T1, T2, l);
+  tsum3(T0, T1, T2, e);
+  double u;
+  two_sum(T1, T2, T1, u);
+  T2 = u;
+  two_sum(T0, T1, T0, u);
+  T1 = u;
+  const ddv res = exp2_reduced(n_d, T0, T1);
+  if (!mf_isfinite(res.hi)) return res;
+  const double corr = res.hi * (mf_ln2_hi * T2);
+  return dd_add(res, mk(corr, 0.0));
+}
+
+// exp_log.inc:364-397: x log2(10) formed as a triple from the four leading
+// exact products of x with the three-limb log2(10), then exp2_triple.
+MF_HD ddv dd_exp10(ddv x) {
+  if (!mf_isfinite(x.hi)) return mk(x.hi > 0.0 ? x.hi : (x.hi < 0.0 ? 0.0 : x.hi), 0.0);
+  constexpr double hi_lim = mf_exp2_max * mf_log10_2_hi, lo_lim = mf_exp2_min * mf_log10_2_hi;
+  if (x.hi > hi_lim) return mk(__builtin_inf(), 0.0);
+  if (x.hi < lo_lim) return mk(0.0, 0.0);
+  double P1, P1e, P2, P2e, P3, P3e;
+  two_prod(x.hi, mf_log2_10_hi, P1, P1e);
+  two_prod(x.hi, mf_log2_10_lo, P2, P2e);
+  two_prod(x.lo, mf_log2_10_hi, P3, P3e);
+  const double PX = x.hi * mf_log2_10_lo2 + x.lo * mf_log2_10_lo;
+  double h = P1, l = 0.0, c = 0.0, t, u;
+  two_sum(h, P3, h, t);
+  two_sum(l, t, l, u);
+  c += u;
+  const double rest[5] = {P1e, P2, P2e, P3e, PX};
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    two_sum(l, rest[i], l, u);
+    c += u;
+  }
+  double e;
+  two_sum(l, c, l, e);
+  fast_two_sum(h, l, h, l);
+  return exp2_triple(h, l, e);
+}
+
+// exp_log.inc:479-506: 2^x - 1 and 10^x - 1 by the direct form for
+// |x| >= 1/2, else expm1 of x ln2 (x ln10); non-finite x through expm1.
+MF_HD ddv dd_exp2m1(ddv x) {
+  if (mf_isfinite(x.hi) && __builtin_fabs(x.hi) >= 0.5) return dd_sub(dd_exp2(x), mk(1.0, 0.0));
+  return dd_expm1(dd_mul(x, mk(mf_ln2_hi, mf_ln2_lo)));
+}
+MF_HD ddv dd_exp10m1(ddv x) {
+  if (mf_isfinite(x.hi) && __builtin_fabs(x.hi) >= 0.5) return dd_sub(dd_exp10(x), mk(1.0, 0.0));
+  return dd_expm1(dd_mul(x, mk(mf_ln10_hi, mf_ln10_lo)));
+}
+// exp_log.inc:508-519: log1p(x) times log2(e) or 1/ln(10).
+MF_HD ddv dd_log2p1(ddv x) { return dd_mul(dd_log1p(x), mk(mf_log2e_hi, mf_log2e_lo)); }
+MF_HD ddv dd_log10p1(ddv x) { return dd_mul(dd_log1p(x), mk(mf_inv_ln10_hi, mf_inv_ln10_lo)); }
 
 // ---------------------------------------------------------------- sin
 MF_HD void reduce_pio2(ddv x, ddv &r, int &q) {

@@ -42,6 +42,11 @@ __device__ __forceinline__ ddv unary_op(ddv x) {
   else if constexpr (OP == mfhip::OP_ASINH) return dd_asinh(x);
   else if constexpr (OP == mfhip::OP_ACOSH) return dd_acosh(x);
   else if constexpr (OP == mfhip::OP_ATANH) return dd_atanh(x);
+  else if constexpr (OP == mfhip::OP_EXP10) return dd_exp10(x);
+  else if constexpr (OP == mfhip::OP_EXP2M1) return dd_exp2m1(x);
+  else if constexpr (OP == mfhip::OP_EXP10M1) return dd_exp10m1(x);
+  else if constexpr (OP == mfhip::OP_LOG2P1) return dd_log2p1(x);
+  else if constexpr (OP == mfhip::OP_LOG10P1) return dd_log10p1(x);
   else return x;
 }
 
@@ -186,6 +191,11 @@ hipError_t unary(int op, const double2 *a, double2 *c, int64_t n, hipStream_t s)
     case OP_ASINH: return run_unary<OP_ASINH>(a, c, n, s);
     case OP_ACOSH: return run_unary<OP_ACOSH>(a, c, n, s);
     case OP_ATANH: return run_unary<OP_ATANH>(a, c, n, s);
+    case OP_EXP10: return run_unary<OP_EXP10>(a, c, n, s);
+    case OP_EXP2M1: return run_unary<OP_EXP2M1>(a, c, n, s);
+    case OP_EXP10M1: return run_unary<OP_EXP10M1>(a, c, n, s);
+    case OP_LOG2P1: return run_unary<OP_LOG2P1>(a, c, n, s);
+    case OP_LOG10P1: return run_unary<OP_LOG10P1>(a, c, n, s);
     default: return hipErrorInvalidValue;
   }
 }

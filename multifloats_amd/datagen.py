@@ -118,6 +118,11 @@ def erf_inputs(rng, n):
     return _with_lo(rng, rng.uniform(-6.0, 6.0, size=n))
 
 
+def exp10_inputs(rng, n):
+    """x in [-300, 300]: 10^x stays a normal double."""
+    return _with_lo(rng, rng.uniform(-300.0, 300.0, size=n))
+
+
 def acosh_inputs(rng, n):
     """x in [1, 1e3], log-spaced above 1 (acosh's domain)."""
     return _with_lo(rng, 1.0 + 10.0 ** rng.uniform(-4.0, 3.0, size=n))

@@ -162,6 +162,11 @@ def main():
                "   -1080 the result is 0, above 1024 - 2^-42 it is +inf */\n")
     out.append(dd_pair("mf_log10_2", mpmath.log10(2)))
     out.append(scalar("mf_exp2_min", -1080.0) + scalar("mf_exp2_max", 1024.0 - 2.0 ** -42))
+    l10 = limbs(mpmath.log(10) / ln2, 3)
+    out.append("\n/* log2(10) in three round-to-nearest limbs; ln(10) and 1/ln(10) */\n")
+    out.append(scalar("mf_log2_10_hi", l10[0]) + scalar("mf_log2_10_lo", l10[1]) + scalar("mf_log2_10_lo2", l10[2]))
+    out.append(dd_pair("mf_ln10", mpmath.log(10)))
+    out.append(dd_pair("mf_inv_ln10", 1 / mpmath.log(10)))
     em = [1 / mpmath.factorial(k + 1) for k in range(25)]
     out.append("\n/* expm1(x)/x Taylor coefficients: 1/(k+1)!, k = 0..24 */\n")
     out.append(arr("mf_expm1c_hi", [limbs(v)[0] for v in em]))

@@ -805,6 +805,73 @@ ARR1(or_asinh, o_asinh)
 ARR1(or_acosh, o_acosh)
 ARR1(or_atanh, o_atanh)
 
+/* src/multifloats_td.hh:30-35 (tsum3) */
+static void o_tsum3(double *T0, double *T1, double *T2, double x) {
+  double e;
+  tsum(*T0, x, T0, &e);
+  tsum(*T1, e, T1, &e);
+  *T2 += e;
+}
+/* src/float64x2/exp_log.inc:86-106 (exp2_from_triple), with tsum3_finalize
+   of multifloats_td.hh:39-43 */
+static dd o_exp2_from_triple(double h, double l, double e) {
+  double n_d = nearbyint(h), rh, rt, u;
+  tsum(h, -n_d, &rh, &rt);
+  double T0 = rh, T1 = 0.0, T2 = 0.0;
+  o_tsum3(&T0, &T1, &T2, rt);
+  o_tsum3(&T0, &T1, &T2, l);
+  o_tsum3(&T0, &T1, &T2, e);
+  tsum(T1, T2, &T1, &u);
+  T2 = u;
+  tsum(T0, T1, &T0, &u);
+  T1 = u;
+  dd res = o_exp2_reduced(n_d, T0, T1);
+  if (!isfinite(res.hi)) return res;
+  return o_add(res, mkdd(res.hi * (mf_ln2_hi * T2), 0.0));
+}
+/* exp_log.inc:364-397 (exp10) */
+static dd o_exp10(dd x) {
+  if (!isfinite(x.hi)) return mkdd(x.hi > 0.0 ? x.hi : (x.hi < 0.0 ? 0.0 : x.hi), 0.0);
+  if (x.hi > mf_exp2_max * mf_log10_2_hi) return mkdd(INFINITY, 0.0);
+  if (x.hi < mf_exp2_min * mf_log10_2_hi) return mkdd(0.0, 0.0);
+  double P1, P1e, P2, P2e, P3, P3e;
+  tprod(x.hi, mf_log2_10_hi, &P1, &P1e);
+  tprod(x.hi, mf_log2_10_lo, &P2, &P2e);
+  tprod(x.lo, mf_log2_10_hi, &P3, &P3e);
+  double PX = x.hi * mf_log2_10_lo2 + x.lo * mf_log2_10_lo;
+  double h = P1, l = 0.0, c = 0.0, t, u, e;
+  tsum(h, P3, &h, &t);
+  tsum(l, t, &l, &u); c += u;
+  tsum(l, P1e, &l, &u); c += u;
+  tsum(l, P2, &l, &u); c += u;
+  tsum(l, P2e, &l, &u); c += u;
+  tsum(l, P3e, &l, &u); c += u;
+  tsum(l, PX, &l, &u); c += u;
+  tsum(l, c, &l, &e);
+  ftsum(h, l, &h, &l);
+  return o_exp2_from_triple(h, l, e);
+}
+/* exp_log.inc:479-493 (exp2m1) */
+static dd o_exp2m1(dd x) {
+  if (!isfinite(x.hi)) return o_expm1(o_mul(x, mkdd(mf_ln2_hi, mf_ln2_lo)));
+  if (fabs(x.hi) >= 0.5) return o_sub(o_exp2(x), mkdd(1.0, 0.0));
+  return o_expm1(o_mul(x, mkdd(mf_ln2_hi, mf_ln2_lo)));
+}
+/* exp_log.inc:495-506 (exp10m1) */
+static dd o_exp10m1(dd x) {
+  if (!isfinite(x.hi)) return o_expm1(o_mul(x, mkdd(mf_ln10_hi, mf_ln10_lo)));
+  if (fabs(x.hi) >= 0.5) return o_sub(o_exp10(x), mkdd(1.0, 0.0));
+  return o_expm1(o_mul(x, mkdd(mf_ln10_hi, mf_ln10_lo)));
+}
+/* exp_log.inc:508-519 (log2p1, log10p1) */
+static dd o_log2p1(dd x) { return o_mul(o_log1p(x), mkdd(mf_log2e_hi, mf_log2e_lo)); }
+static dd o_log10p1(dd x) { return o_mul(o_log1p(x), mkdd(mf_inv_ln10_hi, mf_inv_ln10_lo)); }
+ARR1(or_exp10, o_exp10)
+ARR1(or_exp2m1, o_exp2m1)
+ARR1(or_exp10m1, o_exp10m1)
+ARR1(or_log2p1, o_log2p1)
+ARR1(or_log10p1, o_log10p1)
+
 /* ------------------------------------------------- exact accumulator ---
  * A fixed-point superaccumulator over 2^-1216 .. 2^1120 in 32-bit digits held
  * in int64 (carry-save), enough for every product of two finite doubles and

@@ -42,7 +42,7 @@ def load_ref():
         getattr(L, nm).argtypes = [F2, F2]
         getattr(L, nm).restype = F2
     for nm in ("sqrtdd", "expdd", "logdd", "sindd", "erfdd", "erfcdd", "erfcxdd") + tuple(
-            f + "dd" for f in TRANSC2 + TRANSC3):
+            f + "dd" for f in TRANSC2 + TRANSC3 + TRANSC4):
         getattr(L, nm).argtypes = [F2]
         getattr(L, nm).restype = F2
     P, I64 = ctypes.c_void_p, ctypes.c_int64
@@ -160,6 +160,7 @@ def transc(L, rng):
 
 TRANSC2 = ("cos", "tan", "exp2", "expm1", "log2", "log10", "log1p", "sinh", "cosh", "tanh")
 TRANSC3 = ("asinh", "acosh", "atanh")
+TRANSC4 = ("exp10", "exp2m1", "exp10m1", "log2p1", "log10p1")
 
 
 def transc2(L, rng):
@@ -254,6 +255,62 @@ def transc3(L, rng):
                 pass
         out[nm + "_exact"] = exact
     np.savez_compressed(os.path.join(HERE, "transc3.npz"), **out)
+
+
+def transc4(L, rng):
+    """exp10, exp2m1, exp10m1, log2p1, log10p1: seeded draws over every branch
+    of the reference's algorithms (the |x| = 1/2 switch of the C23 forms, the
+    overflow and underflow clamps of exp10, the log1p regimes), special values,
+    and 320-bit values for accuracy."""
+    n = 1500
+    w = datagen._with_lo
+    U = rng.uniform
+    near_half = np.array([(0.5 * s, 0.0) for s in (1, -1)] + [(np.nextafter(0.5, 0) * s, 0.0) for s in (1, -1)]
+                         + [(0.5 * s, -1e-20 * s) for s in (1, -1)])
+    hi_lim = (1024.0 - 2.0 ** -42) * float.fromhex("0x1.34413509f79ffp-2")  # exp10's clamps
+    lo_lim = -1080.0 * float.fromhex("0x1.34413509f79ffp-2")
+    ten_edges = np.array([(hi_lim, 0.0), (np.nextafter(hi_lim, 0), 0.0), (np.nextafter(hi_lim, 400), 0.0),
+                          (hi_lim, 1e-14), (lo_lim, 0.0), (np.nextafter(lo_lim, 0), 0.0),
+                          (np.nextafter(lo_lim, -400), 0.0), (lo_lim, -1e-14),
+                          (308.0, 0.0), (-323.0, 0.0), (-307.5, 0.0)]
+                         + [(float(k), 0.0) for k in range(-22, 23)])
+    log1p_like = np.concatenate([w(rng, U(-0.25, 0.25, n)), w(rng, 10.0 ** U(-30, -1, 200)),
+                                 w(rng, -(10.0 ** U(-30, -1, 100))), w(rng, U(-0.999, 3, 300)),
+                                 w(rng, 10.0 ** U(0, 300, 100)), w(rng, -1.0 - 10.0 ** U(-10, 2, 30)),
+                                 np.array([(-1.0, 0.0), (-1.0, 1e-20), (-1.0, -1e-20), (0.25, 0.0), (-0.25, 0.0),
+                                           (1.0, 0.0), (9.0, 0.0), (1023.0, 0.0)])])
+    cases = {
+        "exp10": np.concatenate([w(rng, U(-330, 310, n)), datagen.config0(rng, 300), w(rng, U(300, 309, 100)),
+                                 w(rng, U(-326, -320, 100)), ten_edges]),
+        "exp2m1": np.concatenate([w(rng, U(-0.5, 0.5, n)), w(rng, 10.0 ** U(-30, -1, 200) * rng.choice([-1.0, 1.0], 200)),
+                                  w(rng, U(-1100, 1100, 300)), w(rng, U(1020, 1024, 40)), near_half,
+                                  np.array([(1024.0, 0.0), (-1080.0, 0.0), (-1100.0, 0.0), (1.0, 0.0)])]),
+        "exp10m1": np.concatenate([w(rng, U(-0.5, 0.5, n)), w(rng, 10.0 ** U(-30, -1, 200) * rng.choice([-1.0, 1.0], 200)),
+                                   w(rng, U(-330, 310, 300)), near_half, ten_edges]),
+        "log2p1": log1p_like.copy(),
+        "log10p1": log1p_like.copy(),
+    }
+    ln2, ln10 = mpmath.log(2), mpmath.log(10)
+    mpfn = {"exp10": lambda v: mpmath.power(10, v), "exp2m1": lambda v: mpmath.expm1(v * ln2),
+            "exp10m1": lambda v: mpmath.expm1(v * ln10), "log2p1": lambda v: mpmath.log1p(v) / ln2,
+            "log10p1": lambda v: mpmath.log1p(v) / ln10}
+    out = {}
+    for nm, x in cases.items():
+        x = np.concatenate([x, np.array(SPECIALS)])
+        out[nm + "_in"] = x
+        out[nm + "_ref"] = call1(L, nm + "dd", x)
+        exact = np.full((x.shape[0], 3), np.nan)
+        for i in range(x.shape[0]):
+            if not np.isfinite(x[i, 0]):
+                continue
+            try:
+                v = mpfn[nm](mp_of(x[i]))
+                if mpmath.im(v) == 0 and mpmath.isfinite(v):
+                    exact[i] = limbs3(mpmath.re(v))
+            except (OverflowError, ValueError, ZeroDivisionError):
+                pass
+        out[nm + "_exact"] = exact
+    np.savez_compressed(os.path.join(HERE, "transc4.npz"), **out)
 
 
 def special2(L, rng):
@@ -365,7 +422,10 @@ def main(which):
         special2(L, np.random.default_rng(777777))
     if not which or "transc3" in which:
         transc3(L, np.random.default_rng(31337))
-    for f in ("arith.npz", "transc.npz", "matmul.npz", "cmatmul.npz", "transc2.npz", "special2.npz", "transc3.npz"):
+    if not which or "transc4" in which:
+        transc4(L, np.random.default_rng(5151))
+    for f in ("arith.npz", "transc.npz", "matmul.npz", "cmatmul.npz", "transc2.npz", "special2.npz", "transc3.npz",
+              "transc4.npz"):
         print(f, os.path.getsize(os.path.join(HERE, f)))
 
 

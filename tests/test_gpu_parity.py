@@ -122,6 +122,38 @@ def test_inverse_hyperbolic_bits_vs_oracle(gpu, fn):
     nt.assert_bits(dy.cpu().numpy(), nt.or_unary(fn, xs), "device " + fn)
 
 
+@pytest.mark.parametrize("fn", nt.TRANSC4)
+def test_exp10_and_c23_bits_golden(gpu, fn):
+    g = nt.load_golden("transc4.npz")
+    nt.assert_bits(getattr(mf, fn)(g[fn + "_in"]), g[fn + "_ref"], fn)
+
+
+def _transc4_inputs(rng, fn, n):
+    w, U = datagen._with_lo, rng.uniform
+    q = n // 4
+    if fn in ("log2p1", "log10p1"):
+        return np.concatenate([w(rng, U(-0.25, 0.25, q)), w(rng, 10.0 ** U(-30, -1, q) * rng.choice([-1.0, 1.0], q)),
+                               w(rng, U(-0.999, 10, q)), w(rng, 10.0 ** U(0, 300, n - 3 * q))])
+    big = 1100.0 if fn == "exp2m1" else 330.0
+    return np.concatenate([w(rng, U(-0.5, 0.5, q)), w(rng, 10.0 ** U(-30, -1, q) * rng.choice([-1.0, 1.0], q)),
+                           w(rng, U(-big, big, q)), w(rng, U(-3, 3, n - 3 * q))])
+
+
+@pytest.mark.parametrize("fn", nt.TRANSC4)
+def test_exp10_and_c23_bits_vs_oracle(gpu, fn):
+    rng = np.random.default_rng(sum(map(ord, fn)) + 4)
+    x = _transc4_inputs(rng, fn, 200000)
+    nt.assert_bits(getattr(mf, fn)(x), nt.or_unary(fn, x), fn)
+    torch = pytest.importorskip("torch")
+    xs = x[:5000]
+    dx = to_dev(torch, xs)
+    dy = torch.empty_like(dx)
+    mf.dev_unary(getattr(mf, "MFHIP_" + fn.upper()), dx.data_ptr(), dy.data_ptr(), xs.shape[0],
+                 torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    nt.assert_bits(dy.cpu().numpy(), nt.or_unary(fn, xs), "device " + fn)
+
+
 def test_transcendental2_device_entry(torch_cuda):
     """mfhip_unary with the new op codes reaches the same kernels."""
     torch = torch_cuda

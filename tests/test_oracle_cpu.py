@@ -51,6 +51,29 @@ def test_inverse_hyperbolics_are_accurate_where_the_reference_is():
         assert nt.rel_err_vs_exact(g[fn + "_ref"][fin], ex[fin]).max() < cap, fn
 
 
+@pytest.mark.parametrize("fn", nt.TRANSC4)
+def test_oracle_exp10_and_c23_forms_match_reference_bits(fn):
+    """exp10 (exp_log.inc:364-397 with exp2_from_triple 86-106) and the C23
+    forms exp2m1, exp10m1, log2p1, log10p1 (479-519) against the compiled
+    reference: both sides of the |x| = 1/2 switch, exp10's clamps, the log1p
+    regimes, specials."""
+    g = nt.load_golden("transc4.npz")
+    nt.assert_bits(nt.or_unary(fn, g[fn + "_in"]), g[fn + "_ref"], fn)
+
+
+@pytest.mark.parametrize("fn", nt.TRANSC4)
+def test_oracle_exp10_and_c23_forms_are_dd_accurate(fn):
+    """The oracle itself against 320-bit values: every normal result within
+    2e-31 relative. exp10 without its third log2(10) limb, or with the
+    triple's last limb dropped, is off by up to ~1e-30 at |x| ~ 300; a
+    wrong constant or branch is off by far more."""
+    g = nt.load_golden("transc4.npz")
+    x, ex = g[fn + "_in"], g[fn + "_exact"]
+    fin = np.isfinite(ex[:, 0]) & (np.abs(ex[:, 0]) > 1e-290) & (np.abs(ex[:, 0]) < 1e300)
+    assert fin.sum() > 1500
+    assert nt.rel_err_vs_exact(nt.or_unary(fn, x)[fin], ex[fin]).max() < 2e-31, fn
+
+
 def test_oracle_matmul_matches_reference_bits():
     g = nt.load_golden("matmul.npz")
     O = nt.oracle()
