@@ -655,6 +655,11 @@ __global__ __launch_bounds__(kMainThreads) __attribute__((amdgpu_waves_per_eu(1,
   const int r = lane & 31, half = lane >> 5;
   const signed char *ga = As + ((int64_t)t * TBA + 4 * ib) * KB * kChunk;
   const signed char *gb = Bs + ((int64_t)t * TBB + 4 * jb) * KB * kChunk;
+  // The output plane's address is formed (and its scalar argument load
+  // completed) before the main loop: a scalar load still in flight there
+  // would force every LDS wait in the loop to wait for all reads.
+  unsigned char *gr = R + (int64_t)t * m * n;
+  asm volatile("" : "+s"(gr));
   v16i acc[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
@@ -674,42 +679,72 @@ __global__ __launch_bounds__(kMainThreads) __attribute__((amdgpu_waves_per_eu(1,
     if (p < NS)
 #pragma unroll
       for (int j = 0; j < kO2NL; ++j) issue(p, j);
-  v4i fa[2][4], fb[2][4];
-  for (int64_t s = 0; s < NS; ++s) {
-    // stages s+1 .. s+kO2Bufs-2 may stay in flight
-    if (s + 2 < NS) __builtin_amdgcn_s_waitcnt(waitcnt_vm(2 * kO2NL));
-    else if (s + 1 < NS) __builtin_amdgcn_s_waitcnt(waitcnt_vm(kO2NL));
+  // Wait until this wave's copies of stage x have landed; the copies of the
+  // (at most kO2Bufs - 2) stages issued after it may stay in flight.
+  auto wait_stage = [&](int64_t x) {
+    if (x + 2 < NS) __builtin_amdgcn_s_waitcnt(waitcnt_vm(2 * kO2NL));
+    else if (x + 1 < NS) __builtin_amdgcn_s_waitcnt(waitcnt_vm(kO2NL));
     else __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
-    lds_barrier();
+  };
+  // Stage s holds K blocks q = 0, 1. The fragments of block 0 of stage s + 1
+  // are read from LDS while the MFMAs of block 1 of stage s run, so no stage
+  // starts on an LDS read: the wait for stage s + 1's copies and the barrier
+  // sit between the two blocks of stage s. The copies of stage s + 3, issued
+  // during block 0 of stage s, overwrite the buffer of stage s - 1, whose
+  // last reads completed before the barrier in the middle of stage s - 1.
+  static_assert(kO2KS == 2 && 16 % kO2NL == 0, "two K blocks per stage; copies spread over one block");
+  v4i fa[2][4], fb[2][4];
+  wait_stage(0);
+  lds_barrier();
+#pragma unroll
+  for (int a = 0; a < 4; ++a) fa[0][a] = lds_frag(lds + (2 * wi + (a >> 1)) * kChunk, 32 * (a & 1) + r, half);
+#pragma unroll
+  for (int b = 0; b < 4; ++b) fb[0][b] = lds_frag(lds + (4 + 2 * wj + (b >> 1)) * kChunk, 32 * (b & 1) + r, half);
+  for (int64_t s = 0; s < NS; ++s) {
     const bool more = s + kO2Bufs - 1 < NS;
-    const signed char *sa0 = lds + (int)(s % kO2Bufs) * kO2Stage;
+    const signed char *sa1 = lds + (int)(s % kO2Bufs) * kO2Stage + 8 * kChunk;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) fa[1][a] = lds_frag(sa1 + (2 * wi + (a >> 1)) * kChunk, 32 * (a & 1) + r, half);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) fb[1][b] = lds_frag(sa1 + (4 + 2 * wj + (b >> 1)) * kChunk, 32 * (b & 1) + r, half);
     int slot = 0;
 #pragma unroll
-    for (int a = 0; a < 4; ++a) fa[0][a] = lds_frag(sa0 + (2 * wi + (a >> 1)) * kChunk, 32 * (a & 1) + r, half);
+    for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) fb[0][b] = lds_frag(sa0 + (4 + 2 * wj + (b >> 1)) * kChunk, 32 * (b & 1) + r, half);
-#pragma unroll
-    for (int q = 0; q < kO2KS; ++q) {
-      const int cur = q & 1;
-      if (q + 1 < kO2KS) {
-        const signed char *sa = sa0 + (q + 1) * 8 * kChunk;
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-          fa[cur ^ 1][a] = lds_frag(sa + (2 * wi + (a >> 1)) * kChunk, 32 * (a & 1) + r, half);
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-          fb[cur ^ 1][b] = lds_frag(sa + (4 + 2 * wj + (b >> 1)) * kChunk, 32 * (b & 1) + r, half);
+      for (int b = 0; b < 4; ++b) {
+        acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[0][a], fb[0][b], acc[a][b], 0, 0, 0);
+        constexpr int every = 16 / kO2NL;
+        if (more && slot % every == 0) issue(s + kO2Bufs - 1, slot / every);
+        ++slot;
       }
+    // Unconditional, also after the last stage (the reads then fetch a stale
+    // stage buffer inside the allocation and go unused): a branch here would
+    // make the compiler wait for these reads before the MFMAs below.
+    wait_stage(s + 1);
+    lds_barrier();
+    // One MFMA of block 1 goes first: the compiler's wait for block 1's
+    // fragments then lands before the new reads instead of after them.
+    acc[0][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[1][0], fb[1][0], acc[0][0], 0, 0, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    const signed char *sn = lds + (int)((s + 1) % kO2Bufs) * kO2Stage;
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < 4; ++a) fa[0][a] = lds_frag(sn + (2 * wi + (a >> 1)) * kChunk, 32 * (a & 1) + r, half);
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[cur][a], fb[cur][b], acc[a][b], 0, 0, 0);
-          constexpr int every = 16 * kO2KS / kO2NL;
-          if (more && slot % every == 0) issue(s + kO2Bufs - 1, slot / every);
-          ++slot;
-        }
-    }
+    for (int b = 0; b < 4; ++b) fb[0][b] = lds_frag(sn + (4 + 2 * wj + (b >> 1)) * kChunk, 32 * (b & 1) + r, half);
+    __builtin_amdgcn_sched_barrier(0);  // the reads above stay ahead of these MFMAs
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        if (a + b > 0) acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[1][a], fb[1][b], acc[a][b], 0, 0, 0);
+    // The next stage's fragments leave the loop through an empty register
+    // asm: otherwise the compiler merges these reads with the ones before the
+    // loop into one read at the top of the loop, behind the MFMAs again.
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int a = 0; a < 4; ++a) asm("" : "+v"(fa[0][a]));
+#pragma unroll
+    for (int b = 0; b < 4; ++b) asm("" : "+v"(fb[0][b]));
   }
   // epilogue: row 8(e/4) + 4 half + e%4 and column r of each 32 x 32 block.
   // B's residues carry the factor y_t, so v == c_t y_t (mod m_t). q = RN(v / m)
@@ -718,7 +753,6 @@ __global__ __launch_bounds__(kMainThreads) __attribute__((amdgpu_waves_per_eu(1,
   // digit v - q m in [-m/2, m/2] exactly; it is stored as a signed byte
   // (+128 for m = 256 wraps to -128, the same residue).
   const double md = (double)mf_oz2_m[t], inv = 1.0 / md;
-  unsigned char *gr = R + (int64_t)t * m * n;
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
