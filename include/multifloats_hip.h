@@ -152,6 +152,12 @@ MULTIFLOATS_API void exp2m1dd_v(const float64x2 *a, float64x2 *c, int64_t n);
 MULTIFLOATS_API void exp10m1dd_v(const float64x2 *a, float64x2 *c, int64_t n);
 MULTIFLOATS_API void log2p1dd_v(const float64x2 *a, float64x2 *c, int64_t n);
 MULTIFLOATS_API void log10p1dd_v(const float64x2 *a, float64x2 *c, int64_t n);
+/* Array forms of sinpidd, cospidd, tanpidd (float64x2.h:497-501;
+ * src/float64x2/trig.inc:41-104): sin, cos, tan of pi x, bit-identical to
+ * them element by element (NaN for |x| >= 2^52, as the reference). */
+MULTIFLOATS_API void sinpidd_v(const float64x2 *a, float64x2 *c, int64_t n);
+MULTIFLOATS_API void cospidd_v(const float64x2 *a, float64x2 *c, int64_t n);
+MULTIFLOATS_API void tanpidd_v(const float64x2 *a, float64x2 *c, int64_t n);
 /* Array forms of erfcdd and erfcxdd (float64x2.h:531-533; the Fortran erfc
  * and erfc_scaled, fypp:57). Own design, not the reference's rational fits
  * (src/float64x2/special.inc:56-188): piecewise Taylor rows of erfc and of
@@ -263,7 +269,8 @@ enum {
   MFHIP_COS = 6, MFHIP_TAN = 7, MFHIP_EXP2 = 8, MFHIP_EXPM1 = 9, MFHIP_LOG2 = 10,
   MFHIP_LOG10 = 11, MFHIP_LOG1P = 12, MFHIP_SINH = 13, MFHIP_COSH = 14, MFHIP_TANH = 15,
   MFHIP_ERFC = 16, MFHIP_ERFCX = 17, MFHIP_ASINH = 18, MFHIP_ACOSH = 19, MFHIP_ATANH = 20,
-  MFHIP_EXP10 = 21, MFHIP_EXP2M1 = 22, MFHIP_EXP10M1 = 23, MFHIP_LOG2P1 = 24, MFHIP_LOG10P1 = 25
+  MFHIP_EXP10 = 21, MFHIP_EXP2M1 = 22, MFHIP_EXP10M1 = 23, MFHIP_LOG2P1 = 24, MFHIP_LOG10P1 = 25,
+  MFHIP_SINPI = 26, MFHIP_COSPI = 27, MFHIP_TANPI = 28
 };
 enum { MFHIP_ADD = 0, MFHIP_SUB = 1, MFHIP_MUL = 2, MFHIP_DIV = 3 };
 enum { MFHIP_DOT = 0, MFHIP_SUM = 1, MFHIP_NORM2 = 2, MFHIP_PRODUCT = 3, MFHIP_MAXVAL = 4, MFHIP_MINVAL = 5 };

@@ -456,6 +456,9 @@ MF_UNARY(exp2m1dd_v, mfhip::OP_EXP2M1)
 MF_UNARY(exp10m1dd_v, mfhip::OP_EXP10M1)
 MF_UNARY(log2p1dd_v, mfhip::OP_LOG2P1)
 MF_UNARY(log10p1dd_v, mfhip::OP_LOG10P1)
+MF_UNARY(sinpidd_v, mfhip::OP_SINPI)
+MF_UNARY(cospidd_v, mfhip::OP_COSPI)
+MF_UNARY(tanpidd_v, mfhip::OP_TANPI)
 
 float64x2 sumdd(const float64x2 *a, int64_t n) {
   float64x2 r;

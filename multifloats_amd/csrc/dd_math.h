@@ -1,7 +1,8 @@
 // DD transcendentals for the elementwise kernels.
 //
 // exp, exp2, exp10, expm1, exp2m1, exp10m1, log, log2, log10, log1p,
-// log2p1, log10p1, sin, cos, tan, sinh, cosh, tanh, asinh, acosh and atanh keep the reference's algorithms step for step (the same range
+// log2p1, log10p1, sin, cos, tan, sinpi, cospi, tanpi, sinh, cosh, tanh,
+// asinh, acosh and atanh keep the reference's algorithms step for step (the same range
 // reductions, table sizes, polynomial degrees, evaluation trees and special
 // cases), so their results are bit-identical to the reference CPU functions:
 //   dd_exp   src/float64x2/exp_log.inc:237-277 with exp2_from_reduced 26-72
@@ -12,6 +13,7 @@
 //   dd_log10 exp_log.inc:343-353;  dd_log1p 429-469
 //   dd_sin / dd_cos / dd_tan  src/float64x2/trig.inc:241-289,
 //           reduce_pi_half 114-147, sin_eval/cos_eval/sincos_eval 165-215
+//   dd_sinpi / dd_cospi / dd_tanpi  trig.inc:41-104
 //   dd_sinh / dd_cosh / dd_tanh  src/float64x2/hyp.inc:2-87
 //   dd_asinh / dd_acosh / dd_atanh  src/float64x2/inv_trig.inc:226-291
 //   estrin10/12/14/17/24  src/float64x2/poly.inc:33-217 (neval's trees)
@@ -476,13 +478,8 @@ MF_HD ddv dd_cos(ddv x) {
   return (q == 1 || q == 2) ? dd_neg(v) : v;
 }
 
-// trig.inc:268-289 with sincos_eval 192-215: s / c, or -c / s in odd quadrants
-MF_HD ddv dd_tan(ddv x) {
-  if (x.hi == 0.0 && x.lo == 0.0) return x;
-  if (!mf_isfinite(x.hi) || !(__builtin_fabs(x.hi) < 0x1p55)) return mk(__builtin_nan(""), 0.0);
-  ddv r;
-  int q;
-  reduce_pio2(x, r, q);
+// sin(r) and cos(r) together for |r| <= pi/4 (trig.inc:192-215)
+MF_HD void sincos_both(ddv r, ddv &s, ddv &c) {
   const double pi8 = 0.392699081698724;
   const bool small = __builtin_fabs(r.hi) <= pi8;
   const bool pos = r.hi > 0.0;
@@ -490,13 +487,70 @@ MF_HD ddv dd_tan(ddv x) {
   ddv rp = small ? r : (pos ? dd_sub(r, pi4) : dd_add(r, pi4));
   ddv sk = sin_kernel(rp);
   ddv ck = cos_kernel(rp);
-  ddv s = sk, c = ck;
+  s = sk;
+  c = ck;
   if (!small) {
     ddv is2 = mk(mf_inv_sqrt2_hi, mf_inv_sqrt2_lo);
     s = dd_mul(pos ? dd_add(sk, ck) : dd_sub(sk, ck), is2);
     c = dd_mul(pos ? dd_sub(ck, sk) : dd_add(ck, sk), is2);
   }
+}
+
+// trig.inc:268-289: s / c, or -c / s in odd quadrants
+MF_HD ddv dd_tan(ddv x) {
+  if (x.hi == 0.0 && x.lo == 0.0) return x;
+  if (!mf_isfinite(x.hi) || !(__builtin_fabs(x.hi) < 0x1p55)) return mk(__builtin_nan(""), 0.0);
+  ddv r, s, c;
+  int q;
+  reduce_pio2(x, r, q);
+  sincos_both(r, s, c);
   return (q & 1) ? dd_div(dd_neg(c), s) : dd_div(s, c);
+}
+
+// ---------------------------------------------------------- sinpi, cospi, tanpi
+// trig.inc:41-104: n = RN(2|x|), r = |x| - n/2 (|r| <= 1/4), evaluated at
+// pi r by the sin/cos kernels above and placed by n mod 4. NaN for
+// non-finite x and for |x| >= 2^52, where 2|x| has no fractional part left.
+MF_HD void reduce_half(ddv ax, ddv &pr, int &nmod) {
+  const double n = __builtin_rint(2.0 * ax.hi);
+  pr = dd_mul(mk(mf_pi_hi, mf_pi_lo), dd_add(ax, mk(-0.5 * n, 0.0)));
+  nmod = (int)((long long)n & 3LL);
+}
+MF_HD ddv dd_sinpi(ddv x) {
+  if (!mf_isfinite(x.hi)) return mk(__builtin_nan(""), 0.0);
+  if (x.hi == 0.0 && x.lo == 0.0) return x;
+  const bool sign = x.hi < 0.0;
+  const ddv ax = sign ? dd_neg(x) : x;
+  if (!(ax.hi < 0x1p52)) return mk(__builtin_nan(""), 0.0);
+  ddv pr;
+  int q;
+  reduce_half(ax, pr, q);
+  const ddv v = sincos_eval(pr, q & 1);
+  const ddv res = (q & 2) ? dd_neg(v) : v;
+  return sign ? dd_neg(res) : res;
+}
+MF_HD ddv dd_cospi(ddv x) {
+  if (!mf_isfinite(x.hi)) return mk(__builtin_nan(""), 0.0);
+  const ddv ax = x.hi < 0.0 ? dd_neg(x) : x;
+  if (!(ax.hi < 0x1p52)) return mk(__builtin_nan(""), 0.0);
+  ddv pr;
+  int q;
+  reduce_half(ax, pr, q);
+  const ddv v = sincos_eval(pr, !(q & 1));
+  return (q == 1 || q == 2) ? dd_neg(v) : v;
+}
+MF_HD ddv dd_tanpi(ddv x) {
+  if (!mf_isfinite(x.hi)) return mk(__builtin_nan(""), 0.0);
+  if (x.hi == 0.0 && x.lo == 0.0) return x;
+  const bool sign = x.hi < 0.0;
+  const ddv ax = sign ? dd_neg(x) : x;
+  if (!(ax.hi < 0x1p52)) return mk(__builtin_nan(""), 0.0);
+  ddv pr, s, c;
+  int q;
+  reduce_half(ax, pr, q);
+  sincos_both(pr, s, c);
+  const ddv res = (q & 1) ? dd_div(dd_neg(c), s) : dd_div(s, c);
+  return sign ? dd_neg(res) : res;
 }
 
 // ---------------------------------------------------------------- erf

@@ -47,6 +47,9 @@ __device__ __forceinline__ ddv unary_op(ddv x) {
   else if constexpr (OP == mfhip::OP_EXP10M1) return dd_exp10m1(x);
   else if constexpr (OP == mfhip::OP_LOG2P1) return dd_log2p1(x);
   else if constexpr (OP == mfhip::OP_LOG10P1) return dd_log10p1(x);
+  else if constexpr (OP == mfhip::OP_SINPI) return dd_sinpi(x);
+  else if constexpr (OP == mfhip::OP_COSPI) return dd_cospi(x);
+  else if constexpr (OP == mfhip::OP_TANPI) return dd_tanpi(x);
   else return x;
 }
 
@@ -196,6 +199,9 @@ hipError_t unary(int op, const double2 *a, double2 *c, int64_t n, hipStream_t s)
     case OP_EXP10M1: return run_unary<OP_EXP10M1>(a, c, n, s);
     case OP_LOG2P1: return run_unary<OP_LOG2P1>(a, c, n, s);
     case OP_LOG10P1: return run_unary<OP_LOG10P1>(a, c, n, s);
+    case OP_SINPI: return run_unary<OP_SINPI>(a, c, n, s);
+    case OP_COSPI: return run_unary<OP_COSPI>(a, c, n, s);
+    case OP_TANPI: return run_unary<OP_TANPI>(a, c, n, s);
     default: return hipErrorInvalidValue;
   }
 }

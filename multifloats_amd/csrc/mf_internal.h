@@ -17,7 +17,8 @@ enum {
   OP_COS = 6, OP_TAN = 7, OP_EXP2 = 8, OP_EXPM1 = 9, OP_LOG2 = 10,
   OP_LOG10 = 11, OP_LOG1P = 12, OP_SINH = 13, OP_COSH = 14, OP_TANH = 15,
   OP_ERFC = 16, OP_ERFCX = 17, OP_ASINH = 18, OP_ACOSH = 19, OP_ATANH = 20,
-  OP_EXP10 = 21, OP_EXP2M1 = 22, OP_EXP10M1 = 23, OP_LOG2P1 = 24, OP_LOG10P1 = 25
+  OP_EXP10 = 21, OP_EXP2M1 = 22, OP_EXP10M1 = 23, OP_LOG2P1 = 24, OP_LOG10P1 = 25,
+  OP_SINPI = 26, OP_COSPI = 27, OP_TANPI = 28
 };
 enum { OP_ADD = 0, OP_SUB = 1, OP_MUL = 2, OP_DIV = 3 };
 enum { RED_DOT = 0, RED_SUM = 1, RED_NORM2 = 2, RED_PROD = 3, RED_MAXVAL = 4, RED_MINVAL = 5 };

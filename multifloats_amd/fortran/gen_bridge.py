@@ -1,8 +1,8 @@
 #!/usr/bin/env python3
 """Writes multifloats_hip.F90, the Fortran module that routes the reference's
 array generics (matmul, dot_product, sum, norm2, the elemental math functions
-exp log sin cos tan expm1 log2 log10 log1p sinh cosh tanh asinh acosh atanh erf
-erfc erfc_scaled sqrt, and the
+exp log sin cos tan sinpi cospi tanpi expm1 log2 log10 log1p sinh cosh tanh asinh
+acosh atanh erf erfc erfc_scaled sqrt, and the
 array + - * /) to libmultifloats_hip.so through bind(c) interfaces.
 
 The reference generates its module with fypp (fsrc/multifloats.fypp); fypp is
@@ -18,7 +18,8 @@ UNARY = [("exp", "expdd_v"), ("log", "logdd_v"), ("sin", "sindd_v"), ("erf", "er
          ("cos", "cosdd_v"), ("tan", "tandd_v"), ("expm1", "expm1dd_v"), ("log2", "log2dd_v"),
          ("log10", "log10dd_v"), ("log1p", "log1pdd_v"), ("sinh", "sinhdd_v"), ("cosh", "coshdd_v"),
          ("tanh", "tanhdd_v"), ("erfc", "erfcdd_v"), ("erfc_scaled", "erfcxdd_v"), ("asinh", "asinhdd_v"),
-         ("acosh", "acoshdd_v"), ("atanh", "atanhdd_v")]
+         ("acosh", "acoshdd_v"), ("atanh", "atanhdd_v"), ("sinpi", "sinpidd_v"), ("cospi", "cospidd_v"),
+         ("tanpi", "tanpidd_v")]
 # reductions with mask= and dim= (fypp:1783-2090): generic, C entry stem
 MASKED_REDUCE = [("sum", "sumdd"), ("product", "productdd"), ("maxval", "maxvaldd"), ("minval", "minvaldd")]
 BINARY = [("+", "add", "adddd_v"), ("-", "sub", "subdd_v"), ("*", "mul", "muldd_v"), ("/", "div", "divdd_v")]
@@ -55,7 +56,7 @@ def gen():
 !   maxval minval   dd_maxval_Nd / dd_minval_Nd (+ _dim) 1783-1862 (with mask)
 !   norm2           dd_norm2_Nd / dd_norm2_Nd_dim 2338-2377
 !   exp log sin erf sqrt cos tan expm1 log2 log10 log1p sinh cosh tanh
-!   erfc erfc_scaled asinh acosh atanh
+!   erfc erfc_scaled asinh acosh atanh sinpi cospi tanpi
 !                   elemental delegates 709-736, 1407-1412 (names fypp:57)
 !   + - * /         elemental native operators 2746-2837
 !   DD_FMA_RENORM_INTERVAL and dd_set_fma_renorm_interval 664, 2677-2680
@@ -104,7 +105,7 @@ module multifloats_hip
     public :: wgemm
     public :: matmul, dot_product, sum, product, maxval, minval, norm2
     public :: exp, log, sin, erf, sqrt, cos, tan, expm1, log2, log10, log1p, sinh, cosh, tanh
-    public :: erfc, erfc_scaled, asinh, acosh, atanh
+    public :: erfc, erfc_scaled, asinh, acosh, atanh, sinpi, cospi, tanpi
     public :: operator(+), operator(-), operator(*), operator(/)
 
     interface

@@ -13,7 +13,7 @@
 !   maxval minval   dd_maxval_Nd / dd_minval_Nd (+ _dim) 1783-1862 (with mask)
 !   norm2           dd_norm2_Nd / dd_norm2_Nd_dim 2338-2377
 !   exp log sin erf sqrt cos tan expm1 log2 log10 log1p sinh cosh tanh
-!   erfc erfc_scaled asinh acosh atanh
+!   erfc erfc_scaled asinh acosh atanh sinpi cospi tanpi
 !                   elemental delegates 709-736, 1407-1412 (names fypp:57)
 !   + - * /         elemental native operators 2746-2837
 !   DD_FMA_RENORM_INTERVAL and dd_set_fma_renorm_interval 664, 2677-2680
@@ -62,7 +62,7 @@ module multifloats_hip
     public :: wgemm
     public :: matmul, dot_product, sum, product, maxval, minval, norm2
     public :: exp, log, sin, erf, sqrt, cos, tan, expm1, log2, log10, log1p, sinh, cosh, tanh
-    public :: erfc, erfc_scaled, asinh, acosh, atanh
+    public :: erfc, erfc_scaled, asinh, acosh, atanh, sinpi, cospi, tanpi
     public :: operator(+), operator(-), operator(*), operator(/)
 
     interface
@@ -265,6 +265,21 @@ module multifloats_hip
             integer(c_int64_t), value :: n
         end subroutine
         pure subroutine c_atanhdd_v(a, c, n) bind(c, name='atanhdd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_sinpidd_v(a, c, n) bind(c, name='sinpidd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_cospidd_v(a, c, n) bind(c, name='cospidd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_tanpidd_v(a, c, n) bind(c, name='tanpidd_v')
             import :: c_ptr, c_int64_t
             type(c_ptr), value :: a, c
             integer(c_int64_t), value :: n
@@ -566,6 +581,36 @@ module multifloats_hip
         module procedure dd_atanh_5d
         module procedure dd_atanh_6d
         module procedure dd_atanh_7d
+    end interface
+
+    interface sinpi
+        module procedure dd_sinpi_1d
+        module procedure dd_sinpi_2d
+        module procedure dd_sinpi_3d
+        module procedure dd_sinpi_4d
+        module procedure dd_sinpi_5d
+        module procedure dd_sinpi_6d
+        module procedure dd_sinpi_7d
+    end interface
+
+    interface cospi
+        module procedure dd_cospi_1d
+        module procedure dd_cospi_2d
+        module procedure dd_cospi_3d
+        module procedure dd_cospi_4d
+        module procedure dd_cospi_5d
+        module procedure dd_cospi_6d
+        module procedure dd_cospi_7d
+    end interface
+
+    interface tanpi
+        module procedure dd_tanpi_1d
+        module procedure dd_tanpi_2d
+        module procedure dd_tanpi_3d
+        module procedure dd_tanpi_4d
+        module procedure dd_tanpi_5d
+        module procedure dd_tanpi_6d
+        module procedure dd_tanpi_7d
     end interface
 
     interface operator(+)
@@ -2556,6 +2601,132 @@ contains
         type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:,:)
         type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6), size(x,7))
         call c_atanhdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_sinpi_1d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:)
+        type(real64x2), target :: res(size(x,1))
+        call c_sinpidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_sinpi_2d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2))
+        call c_sinpidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_sinpi_3d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3))
+        call c_sinpidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_sinpi_4d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4))
+        call c_sinpidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_sinpi_5d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5))
+        call c_sinpidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_sinpi_6d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6))
+        call c_sinpidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_sinpi_7d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6), size(x,7))
+        call c_sinpidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_cospi_1d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:)
+        type(real64x2), target :: res(size(x,1))
+        call c_cospidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_cospi_2d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2))
+        call c_cospidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_cospi_3d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3))
+        call c_cospidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_cospi_4d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4))
+        call c_cospidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_cospi_5d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5))
+        call c_cospidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_cospi_6d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6))
+        call c_cospidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_cospi_7d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6), size(x,7))
+        call c_cospidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_tanpi_1d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:)
+        type(real64x2), target :: res(size(x,1))
+        call c_tanpidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_tanpi_2d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2))
+        call c_tanpidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_tanpi_3d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3))
+        call c_tanpidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_tanpi_4d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4))
+        call c_tanpidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_tanpi_5d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5))
+        call c_tanpidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_tanpi_6d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6))
+        call c_tanpidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_tanpi_7d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6), size(x,7))
+        call c_tanpidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
     end function
 
     pure function dd_add_1d(a, b) result(res)

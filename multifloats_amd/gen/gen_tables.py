@@ -166,6 +166,8 @@ def main():
     out.append("\n/* log2(10) in three round-to-nearest limbs; ln(10) and 1/ln(10) */\n")
     out.append(scalar("mf_log2_10_hi", l10[0]) + scalar("mf_log2_10_lo", l10[1]) + scalar("mf_log2_10_lo2", l10[2]))
     out.append(dd_pair("mf_ln10", mpmath.log(10)))
+    out.append("/* pi as a double-double (sinpi, cospi, tanpi) */\n")
+    out.append(dd_pair("mf_pi", pi))
     out.append(dd_pair("mf_inv_ln10", 1 / mpmath.log(10)))
     em = [1 / mpmath.factorial(k + 1) for k in range(25)]
     out.append("\n/* expm1(x)/x Taylor coefficients: 1/(k+1)!, k = 0..24 */\n")

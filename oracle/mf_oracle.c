@@ -872,6 +872,70 @@ ARR1(or_exp10m1, o_exp10m1)
 ARR1(or_log2p1, o_log2p1)
 ARR1(or_log10p1, o_log10p1)
 
+/* trig.inc:192-215 (sincos_eval) */
+static void o_sincos_eval(dd r, dd *s, dd *c) {
+  if (fabs(r.hi) <= 0.392699081698724) {
+    *s = o_sink(r);
+    *c = o_cosk(r);
+    return;
+  }
+  dd pi4 = mkdd(mf_pi_quarter_hi, mf_pi_quarter_lo), is2 = mkdd(mf_inv_sqrt2_hi, mf_inv_sqrt2_lo);
+  int pos = r.hi > 0.0;
+  dd rp = pos ? o_sub(r, pi4) : o_add(r, pi4);
+  dd sk = o_sink(rp), ck = o_cosk(rp);
+  *s = o_mul(pos ? o_add(sk, ck) : o_sub(sk, ck), is2);
+  *c = o_mul(pos ? o_sub(ck, sk) : o_add(ck, sk), is2);
+}
+/* trig.inc:41-61 (sinpi) */
+static dd o_sinpi(dd x) {
+  if (!isfinite(x.hi)) return mkdd(NAN, 0.0);
+  if (x.hi == 0.0 && x.lo == 0.0) return x;
+  int sign = x.hi < 0.0;
+  dd ax = sign ? o_neg(x) : x;
+  if (!(ax.hi < 0x1p52)) return mkdd(NAN, 0.0);
+  double n = nearbyint(2.0 * ax.hi);
+  dd pr = o_mul(mkdd(mf_pi_hi, mf_pi_lo), o_add(ax, mkdd(-0.5 * n, 0.0)));
+  dd res;
+  switch ((long long)n & 3LL) {
+    case 0: res = o_eval(pr, 0); break;
+    case 1: res = o_eval(pr, 1); break;
+    case 2: res = o_neg(o_eval(pr, 0)); break;
+    default: res = o_neg(o_eval(pr, 1)); break;
+  }
+  return sign ? o_neg(res) : res;
+}
+/* trig.inc:63-79 (cospi) */
+static dd o_cospi(dd x) {
+  if (!isfinite(x.hi)) return mkdd(NAN, 0.0);
+  dd ax = x.hi < 0.0 ? o_neg(x) : x;
+  if (!(ax.hi < 0x1p52)) return mkdd(NAN, 0.0);
+  double n = nearbyint(2.0 * ax.hi);
+  dd pr = o_mul(mkdd(mf_pi_hi, mf_pi_lo), o_add(ax, mkdd(-0.5 * n, 0.0)));
+  switch ((long long)n & 3LL) {
+    case 0: return o_eval(pr, 1);
+    case 1: return o_neg(o_eval(pr, 0));
+    case 2: return o_neg(o_eval(pr, 1));
+    default: return o_eval(pr, 0);
+  }
+}
+/* trig.inc:82-104 (tanpi) */
+static dd o_tanpi(dd x) {
+  if (!isfinite(x.hi)) return mkdd(NAN, 0.0);
+  if (x.hi == 0.0 && x.lo == 0.0) return x;
+  int sign = x.hi < 0.0;
+  dd ax = sign ? o_neg(x) : x;
+  if (!(ax.hi < 0x1p52)) return mkdd(NAN, 0.0);
+  double n = nearbyint(2.0 * ax.hi);
+  dd pr = o_mul(mkdd(mf_pi_hi, mf_pi_lo), o_add(ax, mkdd(-0.5 * n, 0.0)));
+  dd s, c;
+  o_sincos_eval(pr, &s, &c);
+  dd res = ((long long)n & 1LL) ? o_div(o_neg(c), s) : o_div(s, c);
+  return sign ? o_neg(res) : res;
+}
+ARR1(or_sinpi, o_sinpi)
+ARR1(or_cospi, o_cospi)
+ARR1(or_tanpi, o_tanpi)
+
 /* ------------------------------------------------- exact accumulator ---
  * A fixed-point superaccumulator over 2^-1216 .. 2^1120 in 32-bit digits held
  * in int64 (carry-save), enough for every product of two finite doubles and

@@ -28,6 +28,7 @@ _or = None
 TRANSC2 = ("cos", "tan", "exp2", "expm1", "log2", "log10", "log1p", "sinh", "cosh", "tanh")
 TRANSC3 = ("asinh", "acosh", "atanh")
 TRANSC4 = ("exp10", "exp2m1", "exp10m1", "log2p1", "log10p1")
+TRANSC5 = ("sinpi", "cospi", "tanpi")
 
 
 def oracle():
@@ -36,7 +37,7 @@ def oracle():
         L = ctypes.CDLL(ORACLE)
         for nm in ("or_add", "or_sub", "or_mul", "or_div"):
             getattr(L, nm).argtypes = [P, P, P, I64]
-        for nm in ("or_sqrt", "or_exp", "or_log", "or_sin") + tuple("or_" + f for f in TRANSC2 + TRANSC3 + TRANSC4):
+        for nm in ("or_sqrt", "or_exp", "or_log", "or_sin") + tuple("or_" + f for f in TRANSC2 + TRANSC3 + TRANSC4 + TRANSC5):
             getattr(L, nm).argtypes = [P, P, I64]
         L.or_matmul_mm.argtypes = [P, P, P, I64, I64, I64, I64]
         L.or_cmatmul_mm.argtypes = [P, P, P, I64, I64, I64, I64]

@@ -84,6 +84,9 @@ program bridge_check
     write(u) asinh(x)
     write(u) acosh(x * x + exp(y * y))  ! argument >= 1
     write(u) atanh(x)
+    write(u) sinpi(x + y)
+    write(u) cospi(x - y)
+    write(u) tanpi(x)
     ! xGEMM: C = x(1) A B + y(1) C with A passed transposed and C padded to m + 2 rows
     at = transpose(a)
     allocate(cc(m + 2, n))

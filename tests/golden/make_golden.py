@@ -42,7 +42,7 @@ def load_ref():
         getattr(L, nm).argtypes = [F2, F2]
         getattr(L, nm).restype = F2
     for nm in ("sqrtdd", "expdd", "logdd", "sindd", "erfdd", "erfcdd", "erfcxdd") + tuple(
-            f + "dd" for f in TRANSC2 + TRANSC3 + TRANSC4):
+            f + "dd" for f in TRANSC2 + TRANSC3 + TRANSC4 + TRANSC5):
         getattr(L, nm).argtypes = [F2]
         getattr(L, nm).restype = F2
     P, I64 = ctypes.c_void_p, ctypes.c_int64
@@ -161,6 +161,7 @@ def transc(L, rng):
 TRANSC2 = ("cos", "tan", "exp2", "expm1", "log2", "log10", "log1p", "sinh", "cosh", "tanh")
 TRANSC3 = ("asinh", "acosh", "atanh")
 TRANSC4 = ("exp10", "exp2m1", "exp10m1", "log2p1", "log10p1")
+TRANSC5 = ("sinpi", "cospi", "tanpi")
 
 
 def transc2(L, rng):
@@ -313,6 +314,43 @@ def transc4(L, rng):
     np.savez_compressed(os.path.join(HERE, "transc4.npz"), **out)
 
 
+def transc5(L, rng):
+    """sinpi, cospi, tanpi: seeded draws over every quadrant (n = RN(2|x|)
+    mod 4) and both sin/cos kernel branches (|pi r| above and below pi/8),
+    exact multiples of 1/4 and 1/2 (zeros, poles, +-1), points next to them,
+    large |x| up to and past the 2^52 cap, special values, and 320-bit values
+    for accuracy."""
+    n = 1500
+    w = datagen._with_lo
+    U = rng.uniform
+    quarters = np.array([(k / 4.0, 0.0) for k in range(-16, 17)])
+    near = np.concatenate([w(rng, np.round(U(-40, 40, 200) * 4) / 4 + 10.0 ** U(-17, -3, 200)
+                            * rng.choice([-1.0, 1.0], 200))])
+    big = np.array([(2.0 ** 52, 0.0), (np.nextafter(2.0 ** 52, 0), 0.0), (-(2.0 ** 52), 0.0), (2.0 ** 51 + 0.5, 0.0),
+                    (2.0 ** 50 + 0.25, 0.0), (3.0e15, 0.0), (1e300, 0.0)])
+    common = [w(rng, U(-2, 2, n)), w(rng, U(-1e6, 1e6, 300)), w(rng, 10.0 ** U(-30, -1, 200) * rng.choice([-1.0, 1.0], 200)),
+              w(rng, U(-0.125, 0.125, 200)), quarters, near, big]
+    cases = {nm: np.concatenate(common) for nm in TRANSC5}
+    mpfn = {"sinpi": mpmath.sinpi, "cospi": mpmath.cospi, "tanpi": lambda v: mpmath.sinpi(v) / mpmath.cospi(v)}
+    out = {}
+    for nm, x in cases.items():
+        x = np.concatenate([x, np.array(SPECIALS)])
+        out[nm + "_in"] = x
+        out[nm + "_ref"] = call1(L, nm + "dd", x)
+        exact = np.full((x.shape[0], 3), np.nan)
+        for i in range(x.shape[0]):
+            if not np.isfinite(x[i, 0]) or abs(x[i, 0]) >= 2.0 ** 52:
+                continue
+            try:
+                v = mpfn[nm](mp_of(x[i]))
+                if mpmath.im(v) == 0 and mpmath.isfinite(v):
+                    exact[i] = limbs3(mpmath.re(v))
+            except (OverflowError, ValueError, ZeroDivisionError):
+                pass
+        out[nm + "_exact"] = exact
+    np.savez_compressed(os.path.join(HERE, "transc5.npz"), **out)
+
+
 def special2(L, rng):
     """erfc and erfcx: per-regime draws (the branch points of both the
     reference's fits and this project's rows), the reference's own outputs on
@@ -424,8 +462,10 @@ def main(which):
         transc3(L, np.random.default_rng(31337))
     if not which or "transc4" in which:
         transc4(L, np.random.default_rng(5151))
+    if not which or "transc5" in which:
+        transc5(L, np.random.default_rng(6161))
     for f in ("arith.npz", "transc.npz", "matmul.npz", "cmatmul.npz", "transc2.npz", "special2.npz", "transc3.npz",
-              "transc4.npz"):
+              "transc4.npz", "transc5.npz"):
         print(f, os.path.getsize(os.path.join(HERE, f)))
 
 

@@ -100,6 +100,9 @@ def test_fortran_bridge_routes_to_the_library(gpu, tmp_path):
     nt.assert_bits(r.take(nv), mf.asinh(x), "asinh")
     nt.assert_bits(r.take(nv), mf.acosh(mf.add(mf.mul(x, x), mf.exp(mf.mul(y, y)))), "acosh")
     nt.assert_bits(r.take(nv), mf.atanh(x), "atanh")
+    nt.assert_bits(r.take(nv), mf.sinpi(mf.add(x, y)), "sinpi")
+    nt.assert_bits(r.take(nv), mf.cospi(mf.sub(x, y)), "cospi")
+    nt.assert_bits(r.take(nv), mf.tanpi(x), "tanpi")
     At = nt.cm(A.transpose(1, 0, 2))
     Cw = np.empty((m + 2, n, 2))
     Cw[:m] = A[:, :n]

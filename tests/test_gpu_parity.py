@@ -154,6 +154,30 @@ def test_exp10_and_c23_bits_vs_oracle(gpu, fn):
     nt.assert_bits(dy.cpu().numpy(), nt.or_unary(fn, xs), "device " + fn)
 
 
+@pytest.mark.parametrize("fn", nt.TRANSC5)
+def test_pi_scaled_trig_bits_golden(gpu, fn):
+    g = nt.load_golden("transc5.npz")
+    nt.assert_bits(getattr(mf, fn)(g[fn + "_in"]), g[fn + "_ref"], fn)
+
+
+@pytest.mark.parametrize("fn", nt.TRANSC5)
+def test_pi_scaled_trig_bits_vs_oracle(gpu, fn):
+    rng = np.random.default_rng(sum(map(ord, fn)) + 5)
+    w, U, q = datagen._with_lo, rng.uniform, 50000
+    x = np.concatenate([w(rng, U(-2, 2, q)), w(rng, U(-1e8, 1e8, q)),
+                        w(rng, np.round(U(-100, 100, q) * 4) / 4 + 10.0 ** U(-17, -2, q) * rng.choice([-1.0, 1.0], q)),
+                        w(rng, 10.0 ** U(-30, 16, q) * rng.choice([-1.0, 1.0], q))])
+    nt.assert_bits(getattr(mf, fn)(x), nt.or_unary(fn, x), fn)
+    torch = pytest.importorskip("torch")
+    xs = x[::40]
+    dx = to_dev(torch, xs)
+    dy = torch.empty_like(dx)
+    mf.dev_unary(getattr(mf, "MFHIP_" + fn.upper()), dx.data_ptr(), dy.data_ptr(), xs.shape[0],
+                 torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    nt.assert_bits(dy.cpu().numpy(), nt.or_unary(fn, xs), "device " + fn)
+
+
 def test_transcendental2_device_entry(torch_cuda):
     """mfhip_unary with the new op codes reaches the same kernels."""
     torch = torch_cuda

@@ -74,6 +74,26 @@ def test_oracle_exp10_and_c23_forms_are_dd_accurate(fn):
     assert nt.rel_err_vs_exact(nt.or_unary(fn, x)[fin], ex[fin]).max() < 2e-31, fn
 
 
+@pytest.mark.parametrize("fn", nt.TRANSC5)
+def test_oracle_pi_scaled_trig_matches_reference_bits(fn):
+    """sinpi / cospi / tanpi (trig.inc:41-104) against the compiled reference:
+    every quadrant, exact quarters (zeros, poles, +-1) and their neighbours,
+    the 2^52 cap, specials."""
+    g = nt.load_golden("transc5.npz")
+    nt.assert_bits(nt.or_unary(fn, g[fn + "_in"]), g[fn + "_ref"], fn)
+
+
+@pytest.mark.parametrize("fn", nt.TRANSC5)
+def test_oracle_pi_scaled_trig_is_dd_accurate(fn):
+    """The oracle against 320-bit values: every normal result within 1e-31
+    relative. pi as one double, a quadrant off by one or a sign slip fails."""
+    g = nt.load_golden("transc5.npz")
+    x, ex = g[fn + "_in"], g[fn + "_exact"]
+    fin = np.isfinite(ex[:, 0]) & (np.abs(ex[:, 0]) > 1e-290) & (np.abs(ex[:, 0]) < 1e300)
+    assert fin.sum() > 2000
+    assert nt.rel_err_vs_exact(nt.or_unary(fn, x)[fin], ex[fin]).max() < 1e-31, fn
+
+
 def test_oracle_matmul_matches_reference_bits():
     g = nt.load_golden("matmul.npz")
     O = nt.oracle()
