@@ -438,6 +438,11 @@ def bench_elementwise(args, torch, mf):
     a.abs_()
     ms = timed(torch, lambda: mf.dev_unary(mf.MFHIP_SQRT, a.data_ptr(), c.data_ptr(), n, stream), args.steps, args.warmup)
     res["sqrt"] = dict(ms=round(ms, 4), GBs=round(32.0 * n / ms / 1e6, 1))
+    # two-argument functions on x in (0, 1], y in [-1, 1]: pow takes its 2^(y log2 x) path
+    for nm, op in (("pow", mf.MFHIP_POW), ("hypot", mf.MFHIP_HYPOT), ("fmax", mf.MFHIP_FMAX)):
+        ms = timed(torch, lambda: mf.dev_binary(op, a.data_ptr(), b.data_ptr(), c.data_ptr(), n, stream),
+                   args.steps, args.warmup)
+        res[nm] = dict(ms=round(ms, 4), GBs=round(48.0 * n / ms / 1e6, 1), Gelem_s=round(n / ms / 1e6, 2))
     gens = {"exp": datagen.exp_inputs, "log": datagen.log_inputs, "sin": datagen.sin_inputs, "erf": datagen.erf_inputs,
             "cos": datagen.sin_inputs, "tan": datagen.sin_inputs, "exp2": datagen.exp_inputs,
             "expm1": datagen.config0, "log2": datagen.log_inputs, "log10": datagen.log_inputs,

@@ -114,6 +114,16 @@ MULTIFLOATS_API void adddd_v(const float64x2 *a, const float64x2 *b, float64x2 *
 MULTIFLOATS_API void subdd_v(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n);
 MULTIFLOATS_API void muldd_v(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n);
 MULTIFLOATS_API void divdd_v(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n);
+/* c[i] = f(a[i], b[i]) for the reference's powdd, hypotdd, fdimdd, fmindd,
+ * fmaxdd, copysigndd (float64x2.h:421-446; src/float64x2/exp_log.inc:531-600,
+ * include/multifloats/float64x2.h:1299-1309, 1518-1520, 2077-2102,
+ * 2130-2136, 2270-2303), bit-identical to them element by element. */
+MULTIFLOATS_API void powdd_v(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n);
+MULTIFLOATS_API void hypotdd_v(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n);
+MULTIFLOATS_API void fdimdd_v(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n);
+MULTIFLOATS_API void fmindd_v(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n);
+MULTIFLOATS_API void fmaxdd_v(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n);
+MULTIFLOATS_API void copysigndd_v(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n);
 /* c[i] = sqrt(a[i]), bit-identical to sqrtdd (float64x2.h:2190-2213). */
 MULTIFLOATS_API void sqrtdd_v(const float64x2 *a, float64x2 *c, int64_t n);
 /* Array forms of expdd / logdd / sindd (float64x2.h:453-483; src/float64x2/
@@ -272,7 +282,10 @@ enum {
   MFHIP_EXP10 = 21, MFHIP_EXP2M1 = 22, MFHIP_EXP10M1 = 23, MFHIP_LOG2P1 = 24, MFHIP_LOG10P1 = 25,
   MFHIP_SINPI = 26, MFHIP_COSPI = 27, MFHIP_TANPI = 28
 };
-enum { MFHIP_ADD = 0, MFHIP_SUB = 1, MFHIP_MUL = 2, MFHIP_DIV = 3 };
+enum {
+  MFHIP_ADD = 0, MFHIP_SUB = 1, MFHIP_MUL = 2, MFHIP_DIV = 3, MFHIP_POW = 4, MFHIP_HYPOT = 5, MFHIP_FDIM = 6,
+  MFHIP_FMIN = 7, MFHIP_FMAX = 8, MFHIP_COPYSIGN = 9
+};
 enum { MFHIP_DOT = 0, MFHIP_SUM = 1, MFHIP_NORM2 = 2, MFHIP_PRODUCT = 3, MFHIP_MAXVAL = 4, MFHIP_MINVAL = 5 };
 
 MULTIFLOATS_API int mfhip_matmul_mm(const float64x2 *a, const float64x2 *b, float64x2 *c,

@@ -35,7 +35,8 @@ MFHIP_SQRT, MFHIP_EXP, MFHIP_LOG, MFHIP_SIN, MFHIP_ERF, MFHIP_NEG = range(6)
 (MFHIP_COS, MFHIP_TAN, MFHIP_EXP2, MFHIP_EXPM1, MFHIP_LOG2, MFHIP_LOG10, MFHIP_LOG1P, MFHIP_SINH, MFHIP_COSH,
  MFHIP_TANH, MFHIP_ERFC, MFHIP_ERFCX, MFHIP_ASINH, MFHIP_ACOSH, MFHIP_ATANH, MFHIP_EXP10, MFHIP_EXP2M1,
  MFHIP_EXP10M1, MFHIP_LOG2P1, MFHIP_LOG10P1, MFHIP_SINPI, MFHIP_COSPI, MFHIP_TANPI) = range(6, 29)
-MFHIP_ADD, MFHIP_SUB, MFHIP_MUL, MFHIP_DIV = range(4)
+(MFHIP_ADD, MFHIP_SUB, MFHIP_MUL, MFHIP_DIV, MFHIP_POW, MFHIP_HYPOT, MFHIP_FDIM, MFHIP_FMIN, MFHIP_FMAX,
+ MFHIP_COPYSIGN) = range(10)
 MFHIP_DOT, MFHIP_SUM, MFHIP_NORM2, MFHIP_PRODUCT, MFHIP_MAXVAL, MFHIP_MINVAL = range(6)
 
 
@@ -65,7 +66,8 @@ def lib():
         for nm in ("matmuldd_mv", "matmulf64x2_mv", "matmuldd_vm", "matmulf64x2_vm"):
             getattr(L, nm).argtypes = [P, P, P, I64, I64, I64]
             getattr(L, nm).restype = None
-        for nm in ("adddd_v", "subdd_v", "muldd_v", "divdd_v"):
+        for nm in ("adddd_v", "subdd_v", "muldd_v", "divdd_v", "powdd_v", "hypotdd_v", "fdimdd_v", "fmindd_v",
+                   "fmaxdd_v", "copysigndd_v"):
             getattr(L, nm).argtypes = [P, P, P, I64]
             getattr(L, nm).restype = None
         for nm in ("sqrtdd_v", "expdd_v", "logdd_v", "sindd_v", "erfdd_v", "erfcdd_v", "erfcxdd_v", "asinhdd_v",
@@ -379,6 +381,12 @@ def add(a, b): return _binary("adddd_v", a, b)
 def sub(a, b): return _binary("subdd_v", a, b)
 def mul(a, b): return _binary("muldd_v", a, b)
 def div(a, b): return _binary("divdd_v", a, b)
+def pow(a, b): return _binary("powdd_v", a, b)
+def hypot(a, b): return _binary("hypotdd_v", a, b)
+def fdim(a, b): return _binary("fdimdd_v", a, b)
+def fmin(a, b): return _binary("fmindd_v", a, b)
+def fmax(a, b): return _binary("fmaxdd_v", a, b)
+def copysign(a, b): return _binary("copysigndd_v", a, b)
 
 
 def _res(r):

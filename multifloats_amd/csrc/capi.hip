@@ -431,6 +431,12 @@ MF_BINARY(adddd_v, mfhip::OP_ADD)
 MF_BINARY(subdd_v, mfhip::OP_SUB)
 MF_BINARY(muldd_v, mfhip::OP_MUL)
 MF_BINARY(divdd_v, mfhip::OP_DIV)
+MF_BINARY(powdd_v, mfhip::OP_POW)
+MF_BINARY(hypotdd_v, mfhip::OP_HYPOT)
+MF_BINARY(fdimdd_v, mfhip::OP_FDIM)
+MF_BINARY(fmindd_v, mfhip::OP_FMIN)
+MF_BINARY(fmaxdd_v, mfhip::OP_FMAX)
+MF_BINARY(copysigndd_v, mfhip::OP_COPYSIGN)
 MF_UNARY(sqrtdd_v, mfhip::OP_SQRT)
 MF_UNARY(expdd_v, mfhip::OP_EXP)
 MF_UNARY(logdd_v, mfhip::OP_LOG)

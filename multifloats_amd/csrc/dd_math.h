@@ -14,6 +14,9 @@
 //   dd_sin / dd_cos / dd_tan  src/float64x2/trig.inc:241-289,
 //           reduce_pi_half 114-147, sin_eval/cos_eval/sincos_eval 165-215
 //   dd_sinpi / dd_cospi / dd_tanpi  trig.inc:41-104
+//   dd_pow   exp_log.inc:531-600 with powi (float64x2.h:2077-2102);
+//   dd_hypot, dd_fdim, dd_fmin, dd_fmax, dd_copysign  float64x2.h:1299-1309,
+//           1518-1520, 2130-2136, 2270-2303
 //   dd_sinh / dd_cosh / dd_tanh  src/float64x2/hyp.inc:2-87
 //   dd_asinh / dd_acosh / dd_atanh  src/float64x2/inv_trig.inc:226-291
 //   estrin10/12/14/17/24  src/float64x2/poly.inc:33-217 (neval's trees)
@@ -410,6 +413,119 @@ MF_HD ddv dd_exp10m1(ddv x) {
 MF_HD ddv dd_log2p1(ddv x) { return dd_mul(dd_log1p(x), mk(mf_log2e_hi, mf_log2e_lo)); }
 MF_HD ddv dd_log10p1(ddv x) { return dd_mul(dd_log1p(x), mk(mf_inv_ln10_hi, mf_inv_ln10_lo)); }
 
+// ------------------------------------------------- two-argument functions
+// Lexicographic DD order of the reference (include/multifloats/float64x2.h:
+// 167-190): NaN unordered, +0 equal to -0.
+MF_HD bool dd_lt(ddv a, ddv b) { return a.hi < b.hi || (a.hi == b.hi && a.lo < b.lo); }
+// Sign of the first nonzero limb; of hi when both are zero (float64x2.h:1449-1455).
+MF_HD bool dd_signbit(ddv x) {
+  return __builtin_signbit(x.hi != 0.0 ? x.hi : (x.lo != 0.0 ? x.lo : x.hi));
+}
+// Unbiased exponent of a finite nonzero double.
+MF_HD int mf_ilogb(double v) {
+  int e;
+  (void)__builtin_frexp(v, &e);
+  return e - 1;
+}
+// float64x2.h:1299-1309: the non-NaN operand when exactly one is NaN
+MF_HD ddv dd_fmin(ddv a, ddv b) { return (dd_lt(a, b) || b.hi != b.hi) ? a : b; }
+MF_HD ddv dd_fmax(ddv a, ddv b) { return (dd_lt(a, b) || a.hi != a.hi) ? b : a; }
+// float64x2.h:1518-1520
+MF_HD ddv dd_copysign(ddv x, ddv y) { return dd_signbit(x) == dd_signbit(y) ? x : dd_neg(x); }
+// float64x2.h:2130-2136: max(x - y, 0), NaN if either is NaN
+MF_HD ddv dd_fdim(ddv x, ddv y) {
+  if (x.hi != x.hi || y.hi != y.hi) return mk(__builtin_nan(""), 0.0);
+  return dd_lt(y, x) ? dd_sub(x, y) : mk(0.0, 0.0);
+}
+// float64x2.h:2270-2303: both scaled by 2^-e (e the exponent of the larger),
+// big sqrt(1 + (small/big)^2), scaled back. Non-finite input: +inf if either
+// is infinite, else NaN (C hypot).
+MF_HD ddv dd_hypot(ddv x, ddv y) {
+  if (!mf_isfinite(x.hi) || !mf_isfinite(y.hi)) {
+    const bool inf = __builtin_isinf(x.hi) || __builtin_isinf(y.hi);
+    return mk(inf ? __builtin_inf() : __builtin_nan(""), 0.0);
+  }
+  const ddv ax = dd_signbit(x) ? dd_neg(x) : x;
+  const ddv ay = dd_signbit(y) ? dd_neg(y) : y;
+  const bool gt = dd_lt(ay, ax);
+  ddv big = gt ? ax : ay, small = gt ? ay : ax;
+  if (big.hi == 0.0) return mk(0.0, 0.0);
+  const int e = mf_ilogb(big.hi);
+  const double down = __builtin_ldexp(1.0, -e), up = __builtin_ldexp(1.0, e);
+  big = mk(big.hi * down, big.lo * down);
+  small = mk(small.hi * down, small.lo * down);
+  const ddv ratio = dd_div(small, big);
+  const ddv root = dd_mul(big, dd_sqrt(dd_add(mk(1.0, 0.0), dd_mul(ratio, ratio))));
+  const double rh = root.hi * up;
+  return mk(rh, mf_isfinite(rh) ? root.lo * up : 0.0);
+}
+// float64x2.h:2077-2102 (powi): binary powering; 1 / x^|n| for n < 0
+MF_HD ddv dd_powi(ddv base, int n) {
+  if (n == 0) return mk(1.0, 0.0);
+  if (base.hi == 0.0 && base.lo == 0.0) {
+    const double sg = (n & 1) ? __builtin_copysign(1.0, base.hi) : 1.0;
+    return mk(sg * (n > 0 ? 0.0 : __builtin_inf()), 0.0);
+  }
+  const bool neg = n < 0;
+  unsigned long long u = neg ? (unsigned long long)(-(long long)n) : (unsigned long long)n;
+  ddv result = mk(1.0, 0.0), b = base;
+  for (;;) {
+    if (u & 1ull) result = dd_mul(result, b);
+    u >>= 1;
+    if (u == 0ull) break;
+    b = dd_mul(b, b);
+  }
+  return neg ? dd_div(mk(1.0, 0.0), result) : result;
+}
+
+// exp_log.inc:531-600: integer y (|y| <= 2^31-1) by dd_powi; non-finite or
+// non-positive x by exp(y log x); else 2^(y log2 x) with log2 x = e + log2 m
+// (m in [1, 2)) and y (e + log2 m) accumulated as a triple from exact
+// products, then exp2_triple.
+MF_HD ddv dd_pow(ddv x, ddv y) {
+  if (x.hi == 0.0 && y.hi == 0.0) return mk(1.0, 0.0);
+  if (x.hi == 1.0 && x.lo == 0.0) return mk(1.0, 0.0);
+  if (x.hi == -1.0 && x.lo == 0.0 && __builtin_isinf(y.hi)) return mk(1.0, 0.0);
+  if (y.lo == 0.0 && mf_isfinite(y.hi) && y.hi == __builtin_trunc(y.hi) && __builtin_fabs(y.hi) <= 2147483647.0)
+    return dd_powi(x, (int)y.hi);
+  if (!mf_isfinite(x.hi) || !mf_isfinite(y.hi) || x.hi <= 0.0) return dd_exp(dd_mul(y, dd_log(x)));
+  const int e_x = mf_ilogb(x.hi);
+  const ddv m = mk(__builtin_ldexp(x.hi, -e_x), __builtin_ldexp(x.lo, -e_x));
+  const ddv Lf = log2_kernel(m);
+  double Lx = 0.0;
+  if (!(m.hi > 15.0 / 16.0 && m.hi < 17.0 / 16.0)) {
+    int idx = (int)((m.hi - 1.0) * 32.0);
+    idx = idx < 0 ? 0 : (idx > 31 ? 31 : idx);
+    Lx = mf_log2v_x[idx];
+  }
+  const double ed = (double)e_x;
+  double AH, AM, ALh, ALl, P1, P1e, P2, P2e, P3, P3e;
+  two_prod(y.hi, ed, AH, AM);
+  two_prod(y.lo, ed, ALh, ALl);
+  two_prod(y.hi, Lf.hi, P1, P1e);
+  two_prod(y.hi, Lf.lo, P2, P2e);
+  two_prod(y.lo, Lf.hi, P3, P3e);
+  const double PX = y.hi * Lx + y.lo * Lf.lo;
+  double h = AH, l = 0.0, c = 0.0, t, u;
+  const double lead[4] = {P1, AM, ALh, P3};
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    two_sum(h, lead[i], h, t);
+    two_sum(l, t, l, u);
+    c += u;
+  }
+  const double rest[5] = {P1e, P2, P2e, P3e, PX};
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    two_sum(l, rest[i], l, u);
+    c += u;
+  }
+  c += ALl;
+  double e;
+  two_sum(l, c, l, e);
+  fast_two_sum(h, l, h, l);
+  return exp2_triple(h, l, e);
+}
 // ---------------------------------------------------------------- sin
 MF_HD void reduce_pio2(ddv x, ddv &r, int &q) {
   double n = __builtin_rint(x.hi * 0.6366197723675814);

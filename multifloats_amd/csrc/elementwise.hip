@@ -58,7 +58,13 @@ __device__ __forceinline__ ddv binary_op(ddv x, ddv y) {
   if constexpr (OP == mfhip::OP_ADD) return dd_add(x, y);
   else if constexpr (OP == mfhip::OP_SUB) return dd_sub(x, y);
   else if constexpr (OP == mfhip::OP_MUL) return dd_mul(x, y);
-  else return dd_div(x, y);
+  else if constexpr (OP == mfhip::OP_DIV) return dd_div(x, y);
+  else if constexpr (OP == mfhip::OP_POW) return dd_pow(x, y);
+  else if constexpr (OP == mfhip::OP_HYPOT) return dd_hypot(x, y);
+  else if constexpr (OP == mfhip::OP_FDIM) return dd_fdim(x, y);
+  else if constexpr (OP == mfhip::OP_FMIN) return dd_fmin(x, y);
+  else if constexpr (OP == mfhip::OP_FMAX) return dd_fmax(x, y);
+  else return dd_copysign(x, y);
 }
 
 template <int OP, int U>
@@ -213,6 +219,12 @@ hipError_t binary(int op, const double2 *a, const double2 *b, double2 *c, int64_
     case OP_SUB: return run_binary<OP_SUB>(a, b, c, n, s);
     case OP_MUL: return run_binary<OP_MUL>(a, b, c, n, s);
     case OP_DIV: return run_binary<OP_DIV>(a, b, c, n, s);
+    case OP_POW: return run_binary<OP_POW>(a, b, c, n, s);
+    case OP_HYPOT: return run_binary<OP_HYPOT>(a, b, c, n, s);
+    case OP_FDIM: return run_binary<OP_FDIM>(a, b, c, n, s);
+    case OP_FMIN: return run_binary<OP_FMIN>(a, b, c, n, s);
+    case OP_FMAX: return run_binary<OP_FMAX>(a, b, c, n, s);
+    case OP_COPYSIGN: return run_binary<OP_COPYSIGN>(a, b, c, n, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -20,7 +20,10 @@ enum {
   OP_EXP10 = 21, OP_EXP2M1 = 22, OP_EXP10M1 = 23, OP_LOG2P1 = 24, OP_LOG10P1 = 25,
   OP_SINPI = 26, OP_COSPI = 27, OP_TANPI = 28
 };
-enum { OP_ADD = 0, OP_SUB = 1, OP_MUL = 2, OP_DIV = 3 };
+enum {
+  OP_ADD = 0, OP_SUB = 1, OP_MUL = 2, OP_DIV = 3, OP_POW = 4, OP_HYPOT = 5, OP_FDIM = 6, OP_FMIN = 7,
+  OP_FMAX = 8, OP_COPYSIGN = 9
+};
 enum { RED_DOT = 0, RED_SUM = 1, RED_NORM2 = 2, RED_PROD = 3, RED_MAXVAL = 4, RED_MINVAL = 5 };
 
 // Plain reference-order GEMM; variant < 0 picks by size, otherwise one of

@@ -148,6 +148,8 @@ def main():
     out.append(arr("mf_log2_centre", centres))
     out.append(arr("mf_log2v_hi", [limbs(v)[0] for v in vals]))
     out.append(arr("mf_log2v_lo", [limbs(v)[1] for v in vals]))
+    out.append("/* third limb of log2(centre), used by pow */\n")
+    out.append(arr("mf_log2v_x", [limbs(v, 3)[2] for v in vals]))
 
     st = [(-1) ** k / mpmath.factorial(2 * k + 1) for k in range(13)]
     ct = [(-1) ** k / mpmath.factorial(2 * k) for k in range(13)]

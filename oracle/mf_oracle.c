@@ -936,6 +936,108 @@ ARR1(or_sinpi, o_sinpi)
 ARR1(or_cospi, o_cospi)
 ARR1(or_tanpi, o_tanpi)
 
+/* include/multifloats/float64x2.h:171-175 (operator<) */
+static int o_lt(dd a, dd b) {
+  if (a.hi < b.hi) return 1;
+  if (a.hi == b.hi) return a.lo < b.lo;
+  return 0;
+}
+/* float64x2.h:1449-1455 (signbit) with first_nonzero_limb_index 1276-1280 */
+static int o_signbit(dd x) {
+  int i = x.hi != 0.0 ? 0 : (x.lo != 0.0 ? 1 : 2);
+  return signbit(i == 1 ? x.lo : x.hi) != 0;
+}
+/* float64x2.h:1299-1309 (fmin, fmax) */
+static dd o_fmin(dd a, dd b) { return (o_lt(a, b) || b.hi != b.hi) ? a : b; }
+static dd o_fmax(dd a, dd b) { return (o_lt(a, b) || a.hi != a.hi) ? b : a; }
+/* float64x2.h:1518-1520 (copysign) */
+static dd o_copysign(dd x, dd y) { return o_signbit(x) == o_signbit(y) ? x : o_neg(x); }
+/* float64x2.h:2130-2136 (fdim) */
+static dd o_fdim(dd x, dd y) {
+  if (x.hi != x.hi || y.hi != y.hi) return mkdd(NAN, 0.0);
+  return o_lt(y, x) ? o_sub(x, y) : mkdd(0.0, 0.0);
+}
+/* float64x2.h:2270-2303 (hypot) */
+static dd o_hypot(dd x, dd y) {
+  if (!isfinite(x.hi) || !isfinite(y.hi)) return mkdd(hypot(x.hi, y.hi), 0.0);
+  dd ax = o_signbit(x) ? o_neg(x) : x, ay = o_signbit(y) ? o_neg(y) : y;
+  dd big = o_lt(ay, ax) ? ax : ay, small = o_lt(ay, ax) ? ay : ax;
+  if (big.hi == 0.0) return mkdd(0.0, 0.0);
+  int e = ilogb(big.hi);
+  double down = ldexp(1.0, -e), up = ldexp(1.0, e);
+  big.hi *= down; big.lo *= down;
+  small.hi *= down; small.lo *= down;
+  dd ratio = o_div(small, big);
+  dd root = o_mul(big, o_sqrt(o_add(mkdd(1.0, 0.0), o_mul(ratio, ratio))));
+  dd r = mkdd(root.hi * up, root.lo * up);
+  if (!isfinite(r.hi)) r.lo = 0.0;
+  return r;
+}
+/* float64x2.h:2077-2102 (powi) */
+static dd o_powi(dd base, int n) {
+  if (n == 0) return mkdd(1.0, 0.0);
+  if (base.hi == 0.0 && base.lo == 0.0) {
+    double sg = (n & 1) ? copysign(1.0, base.hi) : 1.0;
+    return mkdd(sg * (n > 0 ? 0.0 : INFINITY), 0.0);
+  }
+  int neg = n < 0;
+  unsigned long long u = neg ? (unsigned long long)(-(long long)n) : (unsigned long long)n;
+  dd result = mkdd(1.0, 0.0), b = base;
+  for (;;) {
+    if (u & 1ull) result = o_mul(result, b);
+    u >>= 1;
+    if (u == 0ull) break;
+    b = o_mul(b, b);
+  }
+  return neg ? o_div(mkdd(1.0, 0.0), result) : result;
+}
+/* exp_log.inc:531-600 (pow) */
+static dd o_pow(dd x, dd y) {
+  if (x.hi == 0.0 && y.hi == 0.0) return mkdd(1.0, 0.0);
+  if (x.hi == 1.0 && x.lo == 0.0) return mkdd(1.0, 0.0);
+  if (x.hi == -1.0 && x.lo == 0.0 && isinf(y.hi)) return mkdd(1.0, 0.0);
+  if (y.lo == 0.0 && isfinite(y.hi) && y.hi == trunc(y.hi) && fabs(y.hi) <= 2147483647.0)
+    return o_powi(x, (int)y.hi);
+  if (!isfinite(x.hi) || !isfinite(y.hi) || x.hi <= 0.0) return o_exp(o_mul(y, o_log(x)));
+  int e_x = ilogb(x.hi);
+  dd m = mkdd(ldexp(x.hi, -e_x), ldexp(x.lo, -e_x));
+  dd Lf = o_log2_kernel(m);
+  double Lx = 0.0, mh = m.hi;
+  if (!(mh > 15.0 / 16.0 && mh < 17.0 / 16.0)) {
+    int idx = (int)((mh - 1.0) * 32.0);
+    if (idx < 0) idx = 0;
+    if (idx > 31) idx = 31;
+    Lx = mf_log2v_x[idx];
+  }
+  double ed = (double)e_x, AH, AM, ALh, ALl, P1, P1e, P2, P2e, P3, P3e;
+  tprod(y.hi, ed, &AH, &AM);
+  tprod(y.lo, ed, &ALh, &ALl);
+  tprod(y.hi, Lf.hi, &P1, &P1e);
+  tprod(y.hi, Lf.lo, &P2, &P2e);
+  tprod(y.lo, Lf.hi, &P3, &P3e);
+  double PX = y.hi * Lx + y.lo * Lf.lo;
+  double h = AH, l = 0.0, c = 0.0, t, u, e;
+  tsum(h, P1, &h, &t); tsum(l, t, &l, &u); c += u;
+  tsum(h, AM, &h, &t); tsum(l, t, &l, &u); c += u;
+  tsum(h, ALh, &h, &t); tsum(l, t, &l, &u); c += u;
+  tsum(h, P3, &h, &t); tsum(l, t, &l, &u); c += u;
+  tsum(l, P1e, &l, &u); c += u;
+  tsum(l, P2, &l, &u); c += u;
+  tsum(l, P2e, &l, &u); c += u;
+  tsum(l, P3e, &l, &u); c += u;
+  tsum(l, PX, &l, &u); c += u;
+  c += ALl;
+  tsum(l, c, &l, &e);
+  ftsum(h, l, &h, &l);
+  return o_exp2_from_triple(h, l, e);
+}
+ARR2(or_pow, o_pow)
+ARR2(or_hypot, o_hypot)
+ARR2(or_fdim, o_fdim)
+ARR2(or_fmin, o_fmin)
+ARR2(or_fmax, o_fmax)
+ARR2(or_copysign, o_copysign)
+
 /* ------------------------------------------------- exact accumulator ---
  * A fixed-point superaccumulator over 2^-1216 .. 2^1120 in 32-bit digits held
  * in int64 (carry-save), enough for every product of two finite doubles and

@@ -29,13 +29,14 @@ TRANSC2 = ("cos", "tan", "exp2", "expm1", "log2", "log10", "log1p", "sinh", "cos
 TRANSC3 = ("asinh", "acosh", "atanh")
 TRANSC4 = ("exp10", "exp2m1", "exp10m1", "log2p1", "log10p1")
 TRANSC5 = ("sinpi", "cospi", "tanpi")
+BINARY2 = ("pow", "hypot", "fdim", "fmin", "fmax", "copysign")
 
 
 def oracle():
     global _or
     if _or is None:
         L = ctypes.CDLL(ORACLE)
-        for nm in ("or_add", "or_sub", "or_mul", "or_div"):
+        for nm in ("or_add", "or_sub", "or_mul", "or_div") + tuple("or_" + f for f in BINARY2):
             getattr(L, nm).argtypes = [P, P, P, I64]
         for nm in ("or_sqrt", "or_exp", "or_log", "or_sin") + tuple("or_" + f for f in TRANSC2 + TRANSC3 + TRANSC4 + TRANSC5):
             getattr(L, nm).argtypes = [P, P, I64]

@@ -45,6 +45,9 @@ def load_ref():
             f + "dd" for f in TRANSC2 + TRANSC3 + TRANSC4 + TRANSC5):
         getattr(L, nm).argtypes = [F2]
         getattr(L, nm).restype = F2
+    for nm in BINARY2:
+        getattr(L, nm + "dd").argtypes = [F2, F2]
+        getattr(L, nm + "dd").restype = F2
     P, I64 = ctypes.c_void_p, ctypes.c_int64
     L.matmuldd_mm.argtypes = [P, P, P, I64, I64, I64, I64]
     L.matmuldd_mv.argtypes = [P, P, P, I64, I64, I64]
@@ -162,6 +165,7 @@ TRANSC2 = ("cos", "tan", "exp2", "expm1", "log2", "log10", "log1p", "sinh", "cos
 TRANSC3 = ("asinh", "acosh", "atanh")
 TRANSC4 = ("exp10", "exp2m1", "exp10m1", "log2p1", "log10p1")
 TRANSC5 = ("sinpi", "cospi", "tanpi")
+BINARY2 = ("pow", "hypot", "fdim", "fmin", "fmax", "copysign")
 
 
 def transc2(L, rng):
@@ -351,6 +355,58 @@ def transc5(L, rng):
     np.savez_compressed(os.path.join(HERE, "transc5.npz"), **out)
 
 
+def binary2(L, rng):
+    """pow, hypot, fdim, fmin, fmax, copysign: every pair of special values,
+    and per-function draws over the reference's branches (pow: integer y
+    through powi, negative and non-finite x through exp(y log x), x near 1 on
+    log2's direct path, results near overflow and underflow; hypot: wide
+    exponent spreads and subnormals; the order-based functions: equal hi
+    limbs with different lo, signed zeros, NaN in either place), with
+    320-bit values of pow and hypot for accuracy."""
+    n = 1500
+    w = datagen._with_lo
+    U = rng.uniform
+    sp = np.array(SPECIALS)
+    ga, gb = np.repeat(sp, len(sp), axis=0), np.tile(sp, (len(sp), 1))
+    sgn = lambda k: rng.choice([-1.0, 1.0], k)
+    cases = {
+        "pow": (np.concatenate([w(rng, 10.0 ** U(-5, 3, n)), w(rng, U(0.9, 1.1, 300)), w(rng, -(10.0 ** U(-2, 2, 200))),
+                                w(rng, 10.0 ** U(-300, 300, 200)), w(rng, U(0.5, 2, 200))]),
+                np.concatenate([w(rng, U(-60, 60, n)), w(rng, U(-1e4, 1e4, 300)), np.round(U(-40, 40, 200))[:, None]
+                                * np.array([1.0, 0.0]), w(rng, U(-1, 1, 200)), np.round(U(-2e9, 2e9, 200))[:, None]
+                                * np.array([1.0, 0.0])])),
+        "hypot": (w(rng, 2.0 ** U(-500, 500, 2 * n) * sgn(2 * n)), w(rng, 2.0 ** U(-500, 500, 2 * n) * sgn(2 * n))),
+    }
+    base = w(rng, U(-4, 4, n))
+    other = base.copy()
+    other[: n // 2, 1] = other[: n // 2, 1] * U(-2, 2, n // 2)  # equal hi, different lo
+    other[n // 2:] = w(rng, U(-4, 4, n - n // 2))
+    zeros = np.array([(0.0, 0.0), (-0.0, 0.0), (0.0, -1e-320), (-0.0, 1e-320), (0.0, 1e-300), (-0.0, -1e-300)])
+    za, zb = np.repeat(zeros, len(zeros), axis=0), np.tile(zeros, (len(zeros), 1))
+    for nm in ("fdim", "fmin", "fmax", "copysign"):
+        cases[nm] = (np.concatenate([base, za]), np.concatenate([other, zb]))
+    mpfn = {"pow": lambda a, b: mpmath.power(a, b) if a > 0 or b == int(b) else None,
+            "hypot": lambda a, b: mpmath.sqrt(a * a + b * b)}
+    out = {}
+    for nm, (a, b) in cases.items():
+        a, b = np.concatenate([a, ga]), np.concatenate([b, gb])
+        out[nm + "_a"], out[nm + "_b"] = a, b
+        out[nm + "_ref"] = call2(L, nm + "dd", a, b)
+        if nm in mpfn:
+            exact = np.full((a.shape[0], 3), np.nan)
+            for i in range(a.shape[0]):
+                if not (np.isfinite(a[i, 0]) and np.isfinite(b[i, 0])):
+                    continue
+                try:
+                    v = mpfn[nm](mp_of(a[i]), mp_of(b[i]))
+                    if v is not None and mpmath.isfinite(v) and abs(v) < 1e308:
+                        exact[i] = limbs3(v)
+                except (OverflowError, ValueError, ZeroDivisionError):
+                    pass
+            out[nm + "_exact"] = exact
+    np.savez_compressed(os.path.join(HERE, "binary2.npz"), **out)
+
+
 def special2(L, rng):
     """erfc and erfcx: per-regime draws (the branch points of both the
     reference's fits and this project's rows), the reference's own outputs on
@@ -464,8 +520,10 @@ def main(which):
         transc4(L, np.random.default_rng(5151))
     if not which or "transc5" in which:
         transc5(L, np.random.default_rng(6161))
+    if not which or "binary2" in which:
+        binary2(L, np.random.default_rng(7171))
     for f in ("arith.npz", "transc.npz", "matmul.npz", "cmatmul.npz", "transc2.npz", "special2.npz", "transc3.npz",
-              "transc4.npz", "transc5.npz"):
+              "transc4.npz", "transc5.npz", "binary2.npz"):
         print(f, os.path.getsize(os.path.join(HERE, f)))
 
 

@@ -178,6 +178,39 @@ def test_pi_scaled_trig_bits_vs_oracle(gpu, fn):
     nt.assert_bits(dy.cpu().numpy(), nt.or_unary(fn, xs), "device " + fn)
 
 
+@pytest.mark.parametrize("fn", nt.BINARY2)
+def test_binary_functions_bits_golden(gpu, fn):
+    g = nt.load_golden("binary2.npz")
+    nt.assert_bits(getattr(mf, fn)(g[fn + "_a"], g[fn + "_b"]), g[fn + "_ref"], fn)
+
+
+@pytest.mark.parametrize("fn", nt.BINARY2)
+def test_binary_functions_bits_vs_oracle(gpu, fn):
+    rng = np.random.default_rng(sum(map(ord, fn)) + 6)
+    w, U, q = datagen._with_lo, rng.uniform, 50000
+    if fn == "pow":
+        a = np.concatenate([w(rng, 10.0 ** U(-4, 4, 2 * q)), w(rng, U(0.5, 2, q)), w(rng, -(10.0 ** U(-1, 1, q)))])
+        b = np.concatenate([w(rng, U(-40, 40, 2 * q)), w(rng, U(-200, 200, q)),
+                            np.round(U(-50, 50, q))[:, None] * np.array([1.0, 0.0])])
+    elif fn == "hypot":
+        a = w(rng, 2.0 ** U(-600, 600, 4 * q) * rng.choice([-1.0, 1.0], 4 * q))
+        b = w(rng, 2.0 ** U(-600, 600, 4 * q) * rng.choice([-1.0, 1.0], 4 * q))
+    else:
+        a = w(rng, U(-4, 4, 4 * q))
+        b = a.copy()
+        b[:q, 1] *= U(-2, 2, q)
+        b[q:] = w(rng, U(-4, 4, 3 * q))
+    nt.assert_bits(getattr(mf, fn)(a, b), nt.or_binary(fn, a, b), fn)
+    torch = pytest.importorskip("torch")
+    xa, xb = a[::40], b[::40]
+    da, db = to_dev(torch, xa), to_dev(torch, xb)
+    dc = torch.empty_like(da)
+    mf.dev_binary(getattr(mf, "MFHIP_" + fn.upper()), da.data_ptr(), db.data_ptr(), dc.data_ptr(), xa.shape[0],
+                  torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    nt.assert_bits(dc.cpu().numpy(), nt.or_binary(fn, xa, xb), "device " + fn)
+
+
 def test_transcendental2_device_entry(torch_cuda):
     """mfhip_unary with the new op codes reaches the same kernels."""
     torch = torch_cuda

@@ -94,6 +94,52 @@ def test_oracle_pi_scaled_trig_is_dd_accurate(fn):
     assert nt.rel_err_vs_exact(nt.or_unary(fn, x)[fin], ex[fin]).max() < 1e-31, fn
 
 
+@pytest.mark.parametrize("fn", nt.BINARY2)
+def test_oracle_binary_functions_match_reference_bits(fn):
+    """pow (exp_log.inc:531-600 with powi), hypot, fdim, fmin, fmax and
+    copysign (float64x2.h) against the compiled reference: every pair of
+    special values and per-branch draws."""
+    g = nt.load_golden("binary2.npz")
+    nt.assert_bits(nt.or_binary(fn, g[fn + "_a"], g[fn + "_b"]), g[fn + "_ref"], fn)
+
+
+def test_oracle_pow_and_hypot_accuracy():
+    """Against 320-bit values: hypot within 1e-31 relative; pow within
+    1e-29 (1 + |y ln x|) relative (its condition number is |y ln x|), on every
+    normal result the reference defines (it returns NaN for x = -1 and an
+    even |y| >= 2^31, outside its integer path)."""
+    g = nt.load_golden("binary2.npz")
+    a, b, ex = g["hypot_a"], g["hypot_b"], g["hypot_exact"]
+    fin = np.isfinite(ex[:, 0]) & (np.abs(ex[:, 0]) > 1e-290) & (np.abs(ex[:, 0]) < 1e300)
+    assert fin.sum() > 3000
+    assert nt.rel_err_vs_exact(nt.or_binary("hypot", a, b)[fin], ex[fin]).max() < 1e-31
+    a, b, ex = g["pow_a"], g["pow_b"], g["pow_exact"]
+    r = nt.or_binary("pow", a, b)
+    fin = (np.isfinite(ex[:, 0]) & (np.abs(ex[:, 0]) > 1e-290) & (np.abs(ex[:, 0]) < 1e300) & np.isfinite(r[:, 0])
+           & (a[:, 0] != 0))
+    assert fin.sum() > 2000
+    z = np.abs(b[fin, 0] * np.log(np.abs(a[fin, 0])))
+    assert (nt.rel_err_vs_exact(r[fin], ex[fin]) <= 1e-29 * (1 + z)).all()
+
+
+def test_oracle_order_functions_properties():
+    """fmin / fmax return one of their operands bit for bit, fmin <= fmax in
+    the DD order, copysign keeps the magnitude of x, fdim is x - y or +0."""
+    g = nt.load_golden("binary2.npz")
+    a, b = g["fmin_a"], g["fmin_b"]
+    ok = ~np.isnan(a[:, 0]) & ~np.isnan(b[:, 0])
+    lo, hi = nt.or_binary("fmin", a, b), nt.or_binary("fmax", a, b)
+    bits = lambda v: np.ascontiguousarray(v).view(np.uint64)
+    for r in (lo, hi):
+        assert (np.all(bits(r) == bits(a), axis=1) | np.all(bits(r) == bits(b), axis=1)).all()
+    assert ((lo[ok, 0] < hi[ok, 0]) | ((lo[ok, 0] == hi[ok, 0]) & (lo[ok, 1] <= hi[ok, 1]))).all()
+    cs = nt.or_binary("copysign", a, b)
+    assert np.all(np.abs(cs[ok]) == np.abs(a[ok]))
+    fd = nt.or_binary("fdim", a[ok], b[ok])
+    diff = nt.or_binary("sub", a[ok], b[ok])
+    assert (np.all(bits(fd) == bits(diff), axis=1) | np.all(fd == 0.0, axis=1)).all()
+
+
 def test_oracle_matmul_matches_reference_bits():
     g = nt.load_golden("matmul.npz")
     O = nt.oracle()
