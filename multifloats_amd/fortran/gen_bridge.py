@@ -23,6 +23,9 @@ UNARY = [("exp", "expdd_v"), ("log", "logdd_v"), ("sin", "sindd_v"), ("erf", "er
 # reductions with mask= and dim= (fypp:1783-2090): generic, C entry stem
 MASKED_REDUCE = [("sum", "sumdd"), ("product", "productdd"), ("maxval", "maxvaldd"), ("minval", "minvaldd")]
 BINARY = [("+", "add", "adddd_v"), ("-", "sub", "subdd_v"), ("*", "mul", "muldd_v"), ("/", "div", "divdd_v")]
+# two-argument elemental generics whose reference bodies are these C functions'
+# rules (fypp:1550-1587: hypot calls hypotdd; sign is copysigndd's rule)
+BINFN = [("hypot", "hypotdd_v"), ("sign", "copysigndd_v")]
 
 
 def colons(r):
@@ -58,6 +61,7 @@ def gen():
 !   exp log sin erf sqrt cos tan expm1 log2 log10 log1p sinh cosh tanh
 !   erfc erfc_scaled asinh acosh atanh sinpi cospi tanpi
 !                   elemental delegates 709-736, 1407-1412 (names fypp:57)
+!   hypot sign      elemental binary functions 1550-1587 (hypotdd, copysign rule)
 !   + - * /         elemental native operators 2746-2837
 !   DD_FMA_RENORM_INTERVAL and dd_set_fma_renorm_interval 664, 2677-2680
 !
@@ -106,6 +110,7 @@ module multifloats_hip
     public :: matmul, dot_product, sum, product, maxval, minval, norm2
     public :: exp, log, sin, erf, sqrt, cos, tan, expm1, log2, log10, log1p, sinh, cosh, tanh
     public :: erfc, erfc_scaled, asinh, acosh, atanh, sinpi, cospi, tanpi
+    public :: hypot, sign
     public :: operator(+), operator(-), operator(*), operator(/)
 
     interface
@@ -198,7 +203,7 @@ module multifloats_hip
             integer(c_int64_t), value :: n
         end subroutine
 """ % (c, c))
-    for _, nm, c in BINARY:
+    for _, nm, c in BINARY + [(None, f, c) for f, c in BINFN]:
         w("""        pure subroutine c_%s(a, b, c, n) bind(c, name='%s')
             import :: c_ptr, c_int64_t
             type(c_ptr), value :: a, b, c
@@ -239,6 +244,11 @@ module multifloats_hip
         w("    interface operator(%s)\n" % op)
         for r in range(1, MAX_RANK + 1):
             w("        module procedure dd_%s_%dd\n" % (nm, r))
+        w("    end interface\n\n")
+    for f, _ in BINFN:
+        w("    interface %s\n" % f)
+        for r in range(1, MAX_RANK + 1):
+            w("        module procedure dd_%s_%dd\n" % (f, r))
         w("    end interface\n\n")
 
     w("contains\n\n")
@@ -422,7 +432,7 @@ module multifloats_hip
     end function
 
 """ % dict(f=f, r=r, C=C, c=c, S=shape_of("x", r)))
-    for op, nm, c in BINARY:
+    for op, nm, c in BINARY + [(None, f, c) for f, c in BINFN]:
         for r in range(1, MAX_RANK + 1):
             C = colons(r)
             w("""    pure function dd_%(nm)s_%(r)dd(a, b) result(res)

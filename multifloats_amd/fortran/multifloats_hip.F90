@@ -15,6 +15,7 @@
 !   exp log sin erf sqrt cos tan expm1 log2 log10 log1p sinh cosh tanh
 !   erfc erfc_scaled asinh acosh atanh sinpi cospi tanpi
 !                   elemental delegates 709-736, 1407-1412 (names fypp:57)
+!   hypot sign      elemental binary functions 1550-1587 (hypotdd, copysign rule)
 !   + - * /         elemental native operators 2746-2837
 !   DD_FMA_RENORM_INTERVAL and dd_set_fma_renorm_interval 664, 2677-2680
 !
@@ -63,6 +64,7 @@ module multifloats_hip
     public :: matmul, dot_product, sum, product, maxval, minval, norm2
     public :: exp, log, sin, erf, sqrt, cos, tan, expm1, log2, log10, log1p, sinh, cosh, tanh
     public :: erfc, erfc_scaled, asinh, acosh, atanh, sinpi, cospi, tanpi
+    public :: hypot, sign
     public :: operator(+), operator(-), operator(*), operator(/)
 
     interface
@@ -300,6 +302,16 @@ module multifloats_hip
             integer(c_int64_t), value :: n
         end subroutine
         pure subroutine c_divdd_v(a, b, c, n) bind(c, name='divdd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, b, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_hypotdd_v(a, b, c, n) bind(c, name='hypotdd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, b, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_copysigndd_v(a, b, c, n) bind(c, name='copysigndd_v')
             import :: c_ptr, c_int64_t
             type(c_ptr), value :: a, b, c
             integer(c_int64_t), value :: n
@@ -651,6 +663,26 @@ module multifloats_hip
         module procedure dd_div_5d
         module procedure dd_div_6d
         module procedure dd_div_7d
+    end interface
+
+    interface hypot
+        module procedure dd_hypot_1d
+        module procedure dd_hypot_2d
+        module procedure dd_hypot_3d
+        module procedure dd_hypot_4d
+        module procedure dd_hypot_5d
+        module procedure dd_hypot_6d
+        module procedure dd_hypot_7d
+    end interface
+
+    interface sign
+        module procedure dd_sign_1d
+        module procedure dd_sign_2d
+        module procedure dd_sign_3d
+        module procedure dd_sign_4d
+        module procedure dd_sign_5d
+        module procedure dd_sign_6d
+        module procedure dd_sign_7d
     end interface
 
 contains
@@ -2895,6 +2927,90 @@ contains
         type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:,:,:), b(:,:,:,:,:,:,:)
         type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5), size(a,6), size(a,7))
         call c_divdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_hypot_1d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:), b(:)
+        type(real64x2), target :: res(size(a,1))
+        call c_hypotdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_hypot_2d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:), b(:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2))
+        call c_hypotdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_hypot_3d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:), b(:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3))
+        call c_hypotdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_hypot_4d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:), b(:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4))
+        call c_hypotdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_hypot_5d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:), b(:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5))
+        call c_hypotdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_hypot_6d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:,:), b(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5), size(a,6))
+        call c_hypotdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_hypot_7d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:,:,:), b(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5), size(a,6), size(a,7))
+        call c_hypotdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_sign_1d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:), b(:)
+        type(real64x2), target :: res(size(a,1))
+        call c_copysigndd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_sign_2d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:), b(:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2))
+        call c_copysigndd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_sign_3d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:), b(:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3))
+        call c_copysigndd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_sign_4d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:), b(:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4))
+        call c_copysigndd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_sign_5d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:), b(:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5))
+        call c_copysigndd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_sign_6d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:,:), b(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5), size(a,6))
+        call c_copysigndd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_sign_7d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:,:,:), b(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5), size(a,6), size(a,7))
+        call c_copysigndd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
     end function
 
 end module multifloats_hip

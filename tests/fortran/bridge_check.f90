@@ -87,6 +87,8 @@ program bridge_check
     write(u) sinpi(x + y)
     write(u) cospi(x - y)
     write(u) tanpi(x)
+    write(u) hypot(x, y)
+    write(u) sign(x, y)
     ! xGEMM: C = x(1) A B + y(1) C with A passed transposed and C padded to m + 2 rows
     at = transpose(a)
     allocate(cc(m + 2, n))

@@ -103,6 +103,8 @@ def test_fortran_bridge_routes_to_the_library(gpu, tmp_path):
     nt.assert_bits(r.take(nv), mf.sinpi(mf.add(x, y)), "sinpi")
     nt.assert_bits(r.take(nv), mf.cospi(mf.sub(x, y)), "cospi")
     nt.assert_bits(r.take(nv), mf.tanpi(x), "tanpi")
+    nt.assert_bits(r.take(nv), mf.hypot(x, y), "hypot")
+    nt.assert_bits(r.take(nv), mf.copysign(x, y), "sign")
     At = nt.cm(A.transpose(1, 0, 2))
     Cw = np.empty((m + 2, n, 2))
     Cw[:m] = A[:, :n]
