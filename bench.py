@@ -439,7 +439,7 @@ def bench_elementwise(args, torch, mf):
     ms = timed(torch, lambda: mf.dev_unary(mf.MFHIP_SQRT, a.data_ptr(), c.data_ptr(), n, stream), args.steps, args.warmup)
     res["sqrt"] = dict(ms=round(ms, 4), GBs=round(32.0 * n / ms / 1e6, 1))
     # two-argument functions on x in (0, 1], y in [-1, 1]: pow takes its 2^(y log2 x) path
-    for nm, op in (("pow", mf.MFHIP_POW), ("hypot", mf.MFHIP_HYPOT), ("fmax", mf.MFHIP_FMAX)):
+    for nm, op in (("pow", mf.MFHIP_POW), ("hypot", mf.MFHIP_HYPOT), ("fmax", mf.MFHIP_FMAX), ("atan2", mf.MFHIP_ATAN2)):
         ms = timed(torch, lambda: mf.dev_binary(op, a.data_ptr(), b.data_ptr(), c.data_ptr(), n, stream),
                    args.steps, args.warmup)
         res[nm] = dict(ms=round(ms, 4), GBs=round(48.0 * n / ms / 1e6, 1), Gelem_s=round(n / ms / 1e6, 2))
@@ -451,7 +451,7 @@ def bench_elementwise(args, torch, mf):
             "asinh": datagen.erf_inputs, "acosh": datagen.acosh_inputs, "atanh": datagen.atanh_inputs,
             "exp10": datagen.exp10_inputs, "exp2m1": datagen.exp_inputs, "exp10m1": datagen.exp10_inputs,
             "log2p1": datagen.config0, "log10p1": datagen.config0, "sinpi": datagen.erf_inputs,
-            "cospi": datagen.erf_inputs, "tanpi": datagen.erf_inputs}
+            "cospi": datagen.erf_inputs, "tanpi": datagen.erf_inputs, "atan": datagen.erf_inputs}
     ops = {nm: getattr(mf, "MFHIP_" + nm.upper()) for nm in gens}
     for nm, gen in gens.items():
         x = torch.from_numpy(gen(rng, n)).cuda()

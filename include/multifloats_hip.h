@@ -124,6 +124,17 @@ MULTIFLOATS_API void fdimdd_v(const float64x2 *a, const float64x2 *b, float64x2 
 MULTIFLOATS_API void fmindd_v(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n);
 MULTIFLOATS_API void fmaxdd_v(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n);
 MULTIFLOATS_API void copysigndd_v(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n);
+/* Array forms of atandd, atanpidd, atan2dd(a = y, b = x), atan2pidd
+ * (float64x2.h:489-509; the Fortran atan, atanpi, atan2, atan2pi). Own design
+ * for atan (the reference evaluates libquadmath's fitted rationals,
+ * src/float64x2/inv_trig.inc:21-55): a table of atan(k/64) in three limbs and
+ * a Maclaurin tail, one final rounding; the reference's special cases and
+ * atan2 / atanpi wrappers (inv_trig.inc:134-223). No less accurate than the
+ * reference on the same inputs. */
+MULTIFLOATS_API void atandd_v(const float64x2 *a, float64x2 *c, int64_t n);
+MULTIFLOATS_API void atanpidd_v(const float64x2 *a, float64x2 *c, int64_t n);
+MULTIFLOATS_API void atan2dd_v(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n);
+MULTIFLOATS_API void atan2pidd_v(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n);
 /* c[i] = sqrt(a[i]), bit-identical to sqrtdd (float64x2.h:2190-2213). */
 MULTIFLOATS_API void sqrtdd_v(const float64x2 *a, float64x2 *c, int64_t n);
 /* Array forms of expdd / logdd / sindd (float64x2.h:453-483; src/float64x2/
@@ -280,11 +291,11 @@ enum {
   MFHIP_LOG10 = 11, MFHIP_LOG1P = 12, MFHIP_SINH = 13, MFHIP_COSH = 14, MFHIP_TANH = 15,
   MFHIP_ERFC = 16, MFHIP_ERFCX = 17, MFHIP_ASINH = 18, MFHIP_ACOSH = 19, MFHIP_ATANH = 20,
   MFHIP_EXP10 = 21, MFHIP_EXP2M1 = 22, MFHIP_EXP10M1 = 23, MFHIP_LOG2P1 = 24, MFHIP_LOG10P1 = 25,
-  MFHIP_SINPI = 26, MFHIP_COSPI = 27, MFHIP_TANPI = 28
+  MFHIP_SINPI = 26, MFHIP_COSPI = 27, MFHIP_TANPI = 28, MFHIP_ATAN = 29, MFHIP_ATANPI = 30
 };
 enum {
   MFHIP_ADD = 0, MFHIP_SUB = 1, MFHIP_MUL = 2, MFHIP_DIV = 3, MFHIP_POW = 4, MFHIP_HYPOT = 5, MFHIP_FDIM = 6,
-  MFHIP_FMIN = 7, MFHIP_FMAX = 8, MFHIP_COPYSIGN = 9
+  MFHIP_FMIN = 7, MFHIP_FMAX = 8, MFHIP_COPYSIGN = 9, MFHIP_ATAN2 = 10, MFHIP_ATAN2PI = 11
 };
 enum { MFHIP_DOT = 0, MFHIP_SUM = 1, MFHIP_NORM2 = 2, MFHIP_PRODUCT = 3, MFHIP_MAXVAL = 4, MFHIP_MINVAL = 5 };
 

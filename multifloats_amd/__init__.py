@@ -34,9 +34,9 @@ DD_FMA_RENORM_INTERVAL = 8
 MFHIP_SQRT, MFHIP_EXP, MFHIP_LOG, MFHIP_SIN, MFHIP_ERF, MFHIP_NEG = range(6)
 (MFHIP_COS, MFHIP_TAN, MFHIP_EXP2, MFHIP_EXPM1, MFHIP_LOG2, MFHIP_LOG10, MFHIP_LOG1P, MFHIP_SINH, MFHIP_COSH,
  MFHIP_TANH, MFHIP_ERFC, MFHIP_ERFCX, MFHIP_ASINH, MFHIP_ACOSH, MFHIP_ATANH, MFHIP_EXP10, MFHIP_EXP2M1,
- MFHIP_EXP10M1, MFHIP_LOG2P1, MFHIP_LOG10P1, MFHIP_SINPI, MFHIP_COSPI, MFHIP_TANPI) = range(6, 29)
+ MFHIP_EXP10M1, MFHIP_LOG2P1, MFHIP_LOG10P1, MFHIP_SINPI, MFHIP_COSPI, MFHIP_TANPI, MFHIP_ATAN, MFHIP_ATANPI) = range(6, 31)
 (MFHIP_ADD, MFHIP_SUB, MFHIP_MUL, MFHIP_DIV, MFHIP_POW, MFHIP_HYPOT, MFHIP_FDIM, MFHIP_FMIN, MFHIP_FMAX,
- MFHIP_COPYSIGN) = range(10)
+ MFHIP_COPYSIGN, MFHIP_ATAN2, MFHIP_ATAN2PI) = range(12)
 MFHIP_DOT, MFHIP_SUM, MFHIP_NORM2, MFHIP_PRODUCT, MFHIP_MAXVAL, MFHIP_MINVAL = range(6)
 
 
@@ -67,12 +67,12 @@ def lib():
             getattr(L, nm).argtypes = [P, P, P, I64, I64, I64]
             getattr(L, nm).restype = None
         for nm in ("adddd_v", "subdd_v", "muldd_v", "divdd_v", "powdd_v", "hypotdd_v", "fdimdd_v", "fmindd_v",
-                   "fmaxdd_v", "copysigndd_v"):
+                   "fmaxdd_v", "copysigndd_v", "atan2dd_v", "atan2pidd_v"):
             getattr(L, nm).argtypes = [P, P, P, I64]
             getattr(L, nm).restype = None
         for nm in ("sqrtdd_v", "expdd_v", "logdd_v", "sindd_v", "erfdd_v", "erfcdd_v", "erfcxdd_v", "asinhdd_v",
                    "acoshdd_v", "atanhdd_v", "exp10dd_v", "exp2m1dd_v", "exp10m1dd_v", "log2p1dd_v",
-                   "log10p1dd_v", "sinpidd_v", "cospidd_v", "tanpidd_v") + tuple(f + "dd_v" for f in _TRANSC2):
+                   "log10p1dd_v", "sinpidd_v", "cospidd_v", "tanpidd_v", "atandd_v", "atanpidd_v") + tuple(f + "dd_v" for f in _TRANSC2):
             getattr(L, nm).argtypes = [P, P, I64]
             getattr(L, nm).restype = None
         L.sumdd.argtypes = [P, I64]
@@ -377,6 +377,8 @@ def log10p1(a): return _unary("log10p1dd_v", a)
 def sinpi(a): return _unary("sinpidd_v", a)
 def cospi(a): return _unary("cospidd_v", a)
 def tanpi(a): return _unary("tanpidd_v", a)
+def atan(a): return _unary("atandd_v", a)
+def atanpi(a): return _unary("atanpidd_v", a)
 def add(a, b): return _binary("adddd_v", a, b)
 def sub(a, b): return _binary("subdd_v", a, b)
 def mul(a, b): return _binary("muldd_v", a, b)
@@ -387,6 +389,8 @@ def fdim(a, b): return _binary("fdimdd_v", a, b)
 def fmin(a, b): return _binary("fmindd_v", a, b)
 def fmax(a, b): return _binary("fmaxdd_v", a, b)
 def copysign(a, b): return _binary("copysigndd_v", a, b)
+def atan2(y, x): return _binary("atan2dd_v", y, x)
+def atan2pi(y, x): return _binary("atan2pidd_v", y, x)
 
 
 def _res(r):

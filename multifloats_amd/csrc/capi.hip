@@ -437,6 +437,8 @@ MF_BINARY(fdimdd_v, mfhip::OP_FDIM)
 MF_BINARY(fmindd_v, mfhip::OP_FMIN)
 MF_BINARY(fmaxdd_v, mfhip::OP_FMAX)
 MF_BINARY(copysigndd_v, mfhip::OP_COPYSIGN)
+MF_BINARY(atan2dd_v, mfhip::OP_ATAN2)
+MF_BINARY(atan2pidd_v, mfhip::OP_ATAN2PI)
 MF_UNARY(sqrtdd_v, mfhip::OP_SQRT)
 MF_UNARY(expdd_v, mfhip::OP_EXP)
 MF_UNARY(logdd_v, mfhip::OP_LOG)
@@ -465,6 +467,8 @@ MF_UNARY(log10p1dd_v, mfhip::OP_LOG10P1)
 MF_UNARY(sinpidd_v, mfhip::OP_SINPI)
 MF_UNARY(cospidd_v, mfhip::OP_COSPI)
 MF_UNARY(tanpidd_v, mfhip::OP_TANPI)
+MF_UNARY(atandd_v, mfhip::OP_ATAN)
+MF_UNARY(atanpidd_v, mfhip::OP_ATANPI)
 
 float64x2 sumdd(const float64x2 *a, int64_t n) {
   float64x2 r;

@@ -22,8 +22,10 @@
 //   estrin10/12/14/17/24  src/float64x2/poly.inc:33-217 (neval's trees)
 // The constant tables are closed forms produced by gen/gen_tables.py.
 //
-// dd_erf, dd_erfc and dd_erfcx are this project's own design, not the
-// reference's rational fits (src/float64x2/special.inc:18-188): piecewise
+// dd_atan (with atan2, atanpi, atan2pi around it), dd_erf, dd_erfc and
+// dd_erfcx are this project's own design, not the
+// reference's rational fits (src/float64x2/special.inc:18-188,
+// inv_trig.inc:21-55): piecewise
 // Taylor evaluations whose coefficient row is picked per lane. Their accuracy
 // target is the reference's own error on the same inputs.
 #pragma once
@@ -864,3 +866,88 @@ MF_HD ddv dd_erfcx(ddv x) {
   if (!mf_isfinite(ex2.hi)) return mk(__builtin_inf(), 0.0);
   return dd_sub(dd_mul(mk(2.0, 0.0), ex2), erfcx_pos(ax));
 }
+
+// ------------------------------------------------------------------- atan
+// Own design (the reference evaluates libquadmath's fitted rationals,
+// src/float64x2/inv_trig.inc:21-55; its special cases and the atan2 / atanpi
+// wrappers, 134-224, are kept). Every path ends in one rounding of a
+// three-limb base plus a double-double correction that is at most 1/15 of
+// the result:
+//   |x| < 1/8:      x + x^3 Q(x^2)
+//   1/8 <= |x| <= 8: atan(k/64) + atan(u), k = RN(64|x|),
+//                   u = (|x| - k/64) / (1 + |x| k/64), |u| <= 1/128
+//   |x| > 8:        pi/2 - atan(1/|x|)
+// with Q the Maclaurin series of (atan(u) - u) / u^3 to 17 terms (|u| <= 1/8).
+MF_HD ddv atan_tail(ddv u) {
+  const ddv s = dd_mul(u, u);
+  ddv q = cdd(mf_atanc_hi, mf_atanc_lo, 16);
+  for (int j = 15; j >= 0; --j) q = dd_add(cdd(mf_atanc_hi, mf_atanc_lo, j), dd_mul(s, q));
+  return dd_mul(dd_mul(s, u), q);
+}
+MF_HD ddv dd_atan(ddv x) {
+  if (!mf_isfinite(x.hi)) return mk(x.hi != x.hi ? x.hi : __builtin_copysign(mf_pio2_cw1, x.hi), 0.0);
+  if (__builtin_fabs(x.hi) < 1.85e-16) return x;  // the cubic term is below half a DD ulp
+  const bool neg = x.hi < 0.0;
+  const ddv t = neg ? dd_neg(x) : x;
+  ddv r;
+  if (t.hi < 0.125) {
+    r = sum3_round(t.hi, t.lo, 0.0, atan_tail(t));
+  } else if (t.hi <= 8.0) {
+    const int k = (int)__builtin_rint(t.hi * 64.0);
+    const ddv c = mk((double)k * 0.015625, 0.0);
+    const ddv u = dd_div(dd_sub(t, c), dd_add(mk(1.0, 0.0), dd_mul(t, c)));
+    const double *b = mf_atant + 3 * (k - 8);
+    r = sum3_round(b[0], b[1], b[2], dd_add(u, atan_tail(u)));
+  } else {
+    const ddv v = dd_div(mk(1.0, 0.0), t);
+    r = sum3_round(mf_pio2_cw1, mf_pio2_cw2, mf_pio2_cw3, dd_neg(dd_add(v, atan_tail(v))));
+  }
+  return neg ? dd_neg(r) : r;
+}
+// atan2 for finite arguments, not both with a zero leading limb
+// (inv_trig.inc:176-193): atan of the smaller ratio, moved by pi or pi/2.
+MF_HD ddv atan2_main(ddv y, ddv x) {
+  const ddv pi = mk(mf_pi_hi, mf_pi_lo), pio2 = mk(mf_pio2_cw1, mf_pio2_cw2);
+  if (__builtin_fabs(x.hi) >= __builtin_fabs(y.hi)) {
+    ddv res = dd_atan(dd_div(y, x));
+    if (x.hi < 0.0) res = __builtin_signbit(y.hi) ? dd_sub(res, pi) : dd_add(res, pi);
+    return res;
+  }
+  const ddv res = dd_atan(dd_div(x, y));
+  return y.hi > 0.0 ? dd_sub(pio2, res) : dd_sub(dd_neg(pio2), res);
+}
+// inv_trig.inc:134-193: non-finite arguments give C atan2's double result
+// (NaN; +-pi/4, +-3pi/4 or +-pi/2 for an infinite y; +-0 or +-pi for an
+// infinite x); both leading limbs zero: the exact quadrant value for signed
+// zeros, else atan2 of the low limbs rounded to a double.
+MF_HD ddv dd_atan2(ddv y, ddv x) {
+  if (!mf_isfinite(x.hi) || !mf_isfinite(y.hi)) {
+    if (x.hi != x.hi || y.hi != y.hi) return mk(__builtin_nan(""), 0.0);
+    const double pi = mf_pi_hi, sy = __builtin_copysign(1.0, y.hi);
+    if (__builtin_isinf(y.hi)) {
+      const double a = __builtin_isinf(x.hi) ? (x.hi > 0.0 ? 0x1.921fb54442d18p-1 : 0x1.2d97c7f3321d2p+1) : mf_pio2_cw1;
+      return mk(sy * a, 0.0);
+    }
+    return mk(x.hi > 0.0 ? sy * 0.0 : sy * pi, 0.0);
+  }
+  if (x.hi == 0.0 && y.hi == 0.0) {
+    if (y.lo != 0.0 || x.lo != 0.0) {
+      const double yl = y.lo != 0.0 ? y.lo : __builtin_copysign(0.0, y.hi);
+      const double xl = x.lo != 0.0 ? x.lo : __builtin_copysign(0.0, x.hi);
+      if (yl == 0.0)  // C atan2(+-0, x): +-0 for x > 0, +-pi for x < 0
+        return mk(__builtin_copysign(xl > 0.0 ? 0.0 : mf_pi_hi, yl), 0.0);
+      if (xl == 0.0) return mk(__builtin_copysign(mf_pio2_cw1, yl), 0.0);
+      return mk(atan2_main(mk(yl, 0.0), mk(xl, 0.0)).hi, 0.0);
+    }
+    if (!__builtin_signbit(x.hi)) return mk(__builtin_copysign(0.0, y.hi), 0.0);
+    return __builtin_signbit(y.hi) ? mk(-mf_pi_hi, -mf_pi_lo) : mk(mf_pi_hi, mf_pi_lo);
+  }
+  return atan2_main(y, x);
+}
+// inv_trig.inc:207-223: atan / pi, with atanpi(+-inf) = +-1/2 exactly
+MF_HD ddv dd_atanpi(ddv x) {
+  if (x.hi == 0.0 && x.lo == 0.0) return x;
+  if (!mf_isfinite(x.hi)) return mk(x.hi != x.hi ? x.hi : (__builtin_signbit(x.hi) ? -0.5 : 0.5), 0.0);
+  return dd_mul(dd_atan(x), mk(mf_inv_pi_hi, mf_inv_pi_lo));
+}
+MF_HD ddv dd_atan2pi(ddv y, ddv x) { return dd_mul(dd_atan2(y, x), mk(mf_inv_pi_hi, mf_inv_pi_lo)); }

@@ -50,6 +50,8 @@ __device__ __forceinline__ ddv unary_op(ddv x) {
   else if constexpr (OP == mfhip::OP_SINPI) return dd_sinpi(x);
   else if constexpr (OP == mfhip::OP_COSPI) return dd_cospi(x);
   else if constexpr (OP == mfhip::OP_TANPI) return dd_tanpi(x);
+  else if constexpr (OP == mfhip::OP_ATAN) return dd_atan(x);
+  else if constexpr (OP == mfhip::OP_ATANPI) return dd_atanpi(x);
   else return x;
 }
 
@@ -64,7 +66,9 @@ __device__ __forceinline__ ddv binary_op(ddv x, ddv y) {
   else if constexpr (OP == mfhip::OP_FDIM) return dd_fdim(x, y);
   else if constexpr (OP == mfhip::OP_FMIN) return dd_fmin(x, y);
   else if constexpr (OP == mfhip::OP_FMAX) return dd_fmax(x, y);
-  else return dd_copysign(x, y);
+  else if constexpr (OP == mfhip::OP_COPYSIGN) return dd_copysign(x, y);
+  else if constexpr (OP == mfhip::OP_ATAN2) return dd_atan2(x, y);
+  else return dd_atan2pi(x, y);
 }
 
 template <int OP, int U>
@@ -208,6 +212,8 @@ hipError_t unary(int op, const double2 *a, double2 *c, int64_t n, hipStream_t s)
     case OP_SINPI: return run_unary<OP_SINPI>(a, c, n, s);
     case OP_COSPI: return run_unary<OP_COSPI>(a, c, n, s);
     case OP_TANPI: return run_unary<OP_TANPI>(a, c, n, s);
+    case OP_ATAN: return run_unary<OP_ATAN>(a, c, n, s);
+    case OP_ATANPI: return run_unary<OP_ATANPI>(a, c, n, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -225,6 +231,8 @@ hipError_t binary(int op, const double2 *a, const double2 *b, double2 *c, int64_
     case OP_FMIN: return run_binary<OP_FMIN>(a, b, c, n, s);
     case OP_FMAX: return run_binary<OP_FMAX>(a, b, c, n, s);
     case OP_COPYSIGN: return run_binary<OP_COPYSIGN>(a, b, c, n, s);
+    case OP_ATAN2: return run_binary<OP_ATAN2>(a, b, c, n, s);
+    case OP_ATAN2PI: return run_binary<OP_ATAN2PI>(a, b, c, n, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -13,9 +13,9 @@
 !   maxval minval   dd_maxval_Nd / dd_minval_Nd (+ _dim) 1783-1862 (with mask)
 !   norm2           dd_norm2_Nd / dd_norm2_Nd_dim 2338-2377
 !   exp log sin erf sqrt cos tan expm1 log2 log10 log1p sinh cosh tanh
-!   erfc erfc_scaled asinh acosh atanh sinpi cospi tanpi
+!   erfc erfc_scaled asinh acosh atanh sinpi cospi tanpi atan atanpi
 !                   elemental delegates 709-736, 1407-1412 (names fypp:57)
-!   hypot sign      elemental binary functions 1550-1587 (hypotdd, copysign rule)
+!   hypot sign atan2 atan2pi  elemental binary functions 1550-1592
 !   + - * /         elemental native operators 2746-2837
 !   DD_FMA_RENORM_INTERVAL and dd_set_fma_renorm_interval 664, 2677-2680
 !
@@ -64,7 +64,7 @@ module multifloats_hip
     public :: matmul, dot_product, sum, product, maxval, minval, norm2
     public :: exp, log, sin, erf, sqrt, cos, tan, expm1, log2, log10, log1p, sinh, cosh, tanh
     public :: erfc, erfc_scaled, asinh, acosh, atanh, sinpi, cospi, tanpi
-    public :: hypot, sign
+    public :: hypot, sign, atan, atanpi, atan2, atan2pi
     public :: operator(+), operator(-), operator(*), operator(/)
 
     interface
@@ -286,6 +286,16 @@ module multifloats_hip
             type(c_ptr), value :: a, c
             integer(c_int64_t), value :: n
         end subroutine
+        pure subroutine c_atandd_v(a, c, n) bind(c, name='atandd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_atanpidd_v(a, c, n) bind(c, name='atanpidd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, c
+            integer(c_int64_t), value :: n
+        end subroutine
         pure subroutine c_adddd_v(a, b, c, n) bind(c, name='adddd_v')
             import :: c_ptr, c_int64_t
             type(c_ptr), value :: a, b, c
@@ -312,6 +322,16 @@ module multifloats_hip
             integer(c_int64_t), value :: n
         end subroutine
         pure subroutine c_copysigndd_v(a, b, c, n) bind(c, name='copysigndd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, b, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_atan2dd_v(a, b, c, n) bind(c, name='atan2dd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, b, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_atan2pidd_v(a, b, c, n) bind(c, name='atan2pidd_v')
             import :: c_ptr, c_int64_t
             type(c_ptr), value :: a, b, c
             integer(c_int64_t), value :: n
@@ -625,6 +645,26 @@ module multifloats_hip
         module procedure dd_tanpi_7d
     end interface
 
+    interface atan
+        module procedure dd_atan_1d
+        module procedure dd_atan_2d
+        module procedure dd_atan_3d
+        module procedure dd_atan_4d
+        module procedure dd_atan_5d
+        module procedure dd_atan_6d
+        module procedure dd_atan_7d
+    end interface
+
+    interface atanpi
+        module procedure dd_atanpi_1d
+        module procedure dd_atanpi_2d
+        module procedure dd_atanpi_3d
+        module procedure dd_atanpi_4d
+        module procedure dd_atanpi_5d
+        module procedure dd_atanpi_6d
+        module procedure dd_atanpi_7d
+    end interface
+
     interface operator(+)
         module procedure dd_add_1d
         module procedure dd_add_2d
@@ -683,6 +723,26 @@ module multifloats_hip
         module procedure dd_sign_5d
         module procedure dd_sign_6d
         module procedure dd_sign_7d
+    end interface
+
+    interface atan2
+        module procedure dd_atan2_1d
+        module procedure dd_atan2_2d
+        module procedure dd_atan2_3d
+        module procedure dd_atan2_4d
+        module procedure dd_atan2_5d
+        module procedure dd_atan2_6d
+        module procedure dd_atan2_7d
+    end interface
+
+    interface atan2pi
+        module procedure dd_atan2pi_1d
+        module procedure dd_atan2pi_2d
+        module procedure dd_atan2pi_3d
+        module procedure dd_atan2pi_4d
+        module procedure dd_atan2pi_5d
+        module procedure dd_atan2pi_6d
+        module procedure dd_atan2pi_7d
     end interface
 
 contains
@@ -2761,6 +2821,90 @@ contains
         call c_tanpidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
     end function
 
+    pure function dd_atan_1d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:)
+        type(real64x2), target :: res(size(x,1))
+        call c_atandd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_atan_2d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2))
+        call c_atandd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_atan_3d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3))
+        call c_atandd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_atan_4d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4))
+        call c_atandd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_atan_5d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5))
+        call c_atandd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_atan_6d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6))
+        call c_atandd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_atan_7d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6), size(x,7))
+        call c_atandd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_atanpi_1d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:)
+        type(real64x2), target :: res(size(x,1))
+        call c_atanpidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_atanpi_2d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2))
+        call c_atanpidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_atanpi_3d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3))
+        call c_atanpidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_atanpi_4d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4))
+        call c_atanpidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_atanpi_5d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5))
+        call c_atanpidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_atanpi_6d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6))
+        call c_atanpidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_atanpi_7d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6), size(x,7))
+        call c_atanpidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
     pure function dd_add_1d(a, b) result(res)
         type(real64x2), intent(in), target, contiguous :: a(:), b(:)
         type(real64x2), target :: res(size(a,1))
@@ -3011,6 +3155,90 @@ contains
         type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:,:,:), b(:,:,:,:,:,:,:)
         type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5), size(a,6), size(a,7))
         call c_copysigndd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_atan2_1d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:), b(:)
+        type(real64x2), target :: res(size(a,1))
+        call c_atan2dd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_atan2_2d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:), b(:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2))
+        call c_atan2dd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_atan2_3d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:), b(:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3))
+        call c_atan2dd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_atan2_4d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:), b(:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4))
+        call c_atan2dd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_atan2_5d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:), b(:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5))
+        call c_atan2dd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_atan2_6d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:,:), b(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5), size(a,6))
+        call c_atan2dd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_atan2_7d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:,:,:), b(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5), size(a,6), size(a,7))
+        call c_atan2dd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_atan2pi_1d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:), b(:)
+        type(real64x2), target :: res(size(a,1))
+        call c_atan2pidd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_atan2pi_2d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:), b(:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2))
+        call c_atan2pidd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_atan2pi_3d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:), b(:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3))
+        call c_atan2pidd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_atan2pi_4d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:), b(:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4))
+        call c_atan2pidd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_atan2pi_5d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:), b(:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5))
+        call c_atan2pidd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_atan2pi_6d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:,:), b(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5), size(a,6))
+        call c_atan2pidd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_atan2pi_7d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:,:,:), b(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5), size(a,6), size(a,7))
+        call c_atan2pidd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
     end function
 
 end module multifloats_hip

@@ -170,6 +170,14 @@ def main():
     out.append(dd_pair("mf_ln10", mpmath.log(10)))
     out.append("/* pi as a double-double (sinpi, cospi, tanpi) */\n")
     out.append(dd_pair("mf_pi", pi))
+    out.append(dd_pair("mf_inv_pi", 1 / pi))
+    ac = [mpf(-1) ** k / (2 * k + 1) for k in range(1, 18)]
+    out.append("\n/* atan(u) = u + u^3 Q(u^2): Q's coefficients (-1)^k / (2k+1), k = 1..17 */\n")
+    out.append(arr("mf_atanc_hi", [limbs(v)[0] for v in ac]))
+    out.append(arr("mf_atanc_lo", [limbs(v)[1] for v in ac]))
+    at = [limbs(mpmath.atan(mpf(k) / 64), 3) for k in range(8, 513)]
+    out.append("\n/* atan(k/64), k = 8..512, in three round-to-nearest limbs */\n")
+    out.append(arr("mf_atant", [v for t in at for v in t]))
     out.append(dd_pair("mf_inv_ln10", 1 / mpmath.log(10)))
     em = [1 / mpmath.factorial(k + 1) for k in range(25)]
     out.append("\n/* expm1(x)/x Taylor coefficients: 1/(k+1)!, k = 0..24 */\n")

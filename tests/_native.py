@@ -30,6 +30,9 @@ TRANSC3 = ("asinh", "acosh", "atanh")
 TRANSC4 = ("exp10", "exp2m1", "exp10m1", "log2p1", "log10p1")
 TRANSC5 = ("sinpi", "cospi", "tanpi")
 BINARY2 = ("pow", "hypot", "fdim", "fmin", "fmax", "copysign")
+INVTRIG = ("atan", "atanpi", "atan2", "atan2pi")
+# worst relative error of this build on the DD-accurate rows of invtrig.npz
+INVTRIG_CAP = {"atan": 1.3e-32, "atanpi": 4.0e-32, "atan2": 5.5e-32, "atan2pi": 6.0e-32}
 
 
 def oracle():
@@ -295,3 +298,9 @@ def erfc_family_errors(got, g, nm):
 
 def load_golden(name):
     return np.load(os.path.join(GOLDEN, name), allow_pickle=False)
+
+
+def invtrig_eval(fn, nm, g):
+    """Apply fn (one- or two-argument, numpy in/out) to the inputs of
+    invtrig.npz for function nm."""
+    return fn(g["x"]) if nm in ("atan", "atanpi") else fn(g["y2"], g["x2"])

@@ -89,6 +89,10 @@ program bridge_check
     write(u) tanpi(x)
     write(u) hypot(x, y)
     write(u) sign(x, y)
+    write(u) atan(x)
+    write(u) atanpi(x)
+    write(u) atan2(x, y)
+    write(u) atan2pi(x, y)
     ! xGEMM: C = x(1) A B + y(1) C with A passed transposed and C padded to m + 2 rows
     at = transpose(a)
     allocate(cc(m + 2, n))

@@ -45,7 +45,10 @@ def load_ref():
             f + "dd" for f in TRANSC2 + TRANSC3 + TRANSC4 + TRANSC5):
         getattr(L, nm).argtypes = [F2]
         getattr(L, nm).restype = F2
-    for nm in BINARY2:
+    for nm in INVTRIG1:
+        getattr(L, nm + "dd").argtypes = [F2]
+        getattr(L, nm + "dd").restype = F2
+    for nm in BINARY2 + INVTRIG2:
         getattr(L, nm + "dd").argtypes = [F2, F2]
         getattr(L, nm + "dd").restype = F2
     P, I64 = ctypes.c_void_p, ctypes.c_int64
@@ -166,6 +169,8 @@ TRANSC3 = ("asinh", "acosh", "atanh")
 TRANSC4 = ("exp10", "exp2m1", "exp10m1", "log2p1", "log10p1")
 TRANSC5 = ("sinpi", "cospi", "tanpi")
 BINARY2 = ("pow", "hypot", "fdim", "fmin", "fmax", "copysign")
+INVTRIG1 = ("atan", "atanpi")
+INVTRIG2 = ("atan2", "atan2pi")
 
 
 def transc2(L, rng):
@@ -407,6 +412,49 @@ def binary2(L, rng):
     np.savez_compressed(os.path.join(HERE, "binary2.npz"), **out)
 
 
+def invtrig(L, rng):
+    """atan, atanpi, atan2, atan2pi (this project's own atan, so the bar is
+    accuracy): the reference's outputs and 320-bit values on draws over every
+    branch of both designs (|x| below and above 1/8 and 8, table cells and
+    their edges, huge and tiny |x|), all four atan2 quadrants with either
+    ratio, every pair of special values, and signed zeros with and without
+    a nonzero low limb."""
+    n = 1500
+    w = datagen._with_lo
+    U = rng.uniform
+    sg = lambda k: rng.choice([-1.0, 1.0], k)
+    cells = np.array([((k + d) / 64.0, 0.0) for k in range(8, 513, 7) for d in (-0.5, 0.5, 0.0)])
+    x1 = np.concatenate([w(rng, U(-0.2, 0.2, n)), w(rng, U(-9, 9, 2 * n)), w(rng, 10.0 ** U(-20, 300, n) * sg(n)),
+                         cells, np.array([(0.125, 0.0), (np.nextafter(0.125, 0), 0.0), (8.0, 0.0), (8.0, 1e-16),
+                                          (1.85e-16, 0.0), (1.8e-16, 1e-33)]), np.array(SPECIALS)])
+    sp = np.array(SPECIALS)
+    zeros = np.array([(0.0, 0.0), (-0.0, 0.0), (0.0, 1e-300), (-0.0, -1e-300), (0.0, -5e-324), (-0.0, 3e-310)])
+    ya = np.concatenate([w(rng, U(-5, 5, 2 * n)), w(rng, 10.0 ** U(-30, 30, n) * sg(n)), np.repeat(sp, len(sp), axis=0),
+                         np.repeat(zeros, len(zeros), axis=0)])
+    xa = np.concatenate([w(rng, U(-5, 5, 2 * n)), w(rng, 10.0 ** U(-30, 30, n) * sg(n)), np.tile(sp, (len(sp), 1)),
+                         np.tile(zeros, (len(zeros), 1))])
+    out = {"x": x1, "y2": ya, "x2": xa}
+    for nm in INVTRIG1:
+        out[nm + "_ref"] = call1(L, nm + "dd", x1)
+    for nm in INVTRIG2:
+        out[nm + "_ref"] = call2(L, nm + "dd", ya, xa)
+    pi = mpmath.pi
+    ex = {nm: np.full((x1.shape[0], 3), np.nan) for nm in INVTRIG1}
+    for i in range(x1.shape[0]):
+        if np.isfinite(x1[i, 0]) and x1[i, 0] != 0.0:
+            v = mpmath.atan(mp_of(x1[i]))
+            ex["atan"][i], ex["atanpi"][i] = limbs3(v), limbs3(v / pi)
+    ex2 = {nm: np.full((xa.shape[0], 3), np.nan) for nm in INVTRIG2}
+    for i in range(xa.shape[0]):
+        yv, xv = ya[i], xa[i]
+        if np.isfinite(yv[0]) and np.isfinite(xv[0]) and yv[0] != 0.0 and xv[0] != 0.0:
+            v = mpmath.atan2(mp_of(yv), mp_of(xv))
+            ex2["atan2"][i], ex2["atan2pi"][i] = limbs3(v), limbs3(v / pi)
+    for nm, e in list(ex.items()) + list(ex2.items()):
+        out[nm + "_exact"] = e
+    np.savez_compressed(os.path.join(HERE, "invtrig.npz"), **out)
+
+
 def special2(L, rng):
     """erfc and erfcx: per-regime draws (the branch points of both the
     reference's fits and this project's rows), the reference's own outputs on
@@ -522,8 +570,10 @@ def main(which):
         transc5(L, np.random.default_rng(6161))
     if not which or "binary2" in which:
         binary2(L, np.random.default_rng(7171))
+    if not which or "invtrig" in which:
+        invtrig(L, np.random.default_rng(8181))
     for f in ("arith.npz", "transc.npz", "matmul.npz", "cmatmul.npz", "transc2.npz", "special2.npz", "transc3.npz",
-              "transc4.npz", "transc5.npz", "binary2.npz"):
+              "transc4.npz", "transc5.npz", "binary2.npz", "invtrig.npz"):
         print(f, os.path.getsize(os.path.join(HERE, f)))
 
 

@@ -30,3 +30,7 @@ H2(h_add, dd_add)
 H2(h_sub, dd_sub)
 H2(h_mul, dd_mul)
 H2(h_div, dd_div)
+H1(h_atan, dd_atan)
+H1(h_atanpi, dd_atanpi)
+H2(h_atan2, dd_atan2)
+H2(h_atan2pi, dd_atan2pi)

@@ -105,6 +105,10 @@ def test_fortran_bridge_routes_to_the_library(gpu, tmp_path):
     nt.assert_bits(r.take(nv), mf.tanpi(x), "tanpi")
     nt.assert_bits(r.take(nv), mf.hypot(x, y), "hypot")
     nt.assert_bits(r.take(nv), mf.copysign(x, y), "sign")
+    nt.assert_bits(r.take(nv), mf.atan(x), "atan")
+    nt.assert_bits(r.take(nv), mf.atanpi(x), "atanpi")
+    nt.assert_bits(r.take(nv), mf.atan2(x, y), "atan2")
+    nt.assert_bits(r.take(nv), mf.atan2pi(x, y), "atan2pi")
     At = nt.cm(A.transpose(1, 0, 2))
     Cw = np.empty((m + 2, n, 2))
     Cw[:m] = A[:, :n]

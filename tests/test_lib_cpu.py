@@ -197,3 +197,44 @@ def test_grouped_tile_choice_minimises_padded_work():
     # a slower variant loses ties
     table = {"variant_rates": [1.0, 2.0, 1.0, 1.0, 1.0, 1.0]}
     assert autotune.plan_variants(np.array([[64, 64, 64]]), table) == [1]
+
+
+def _host(nm):
+    L = ctypes.CDLL(os.path.join(nt.ROOT, "tests", "native", "libhostmath.so"))
+    f = getattr(L, "h_" + nm)
+
+    def run(*args):
+        out = np.empty_like(args[0])
+        f(*[nt.ptr(np.ascontiguousarray(a)) for a in args], nt.ptr(out), ctypes.c_long(args[0].shape[0]))
+        return out
+    return run
+
+
+@pytest.mark.parametrize("nm", nt.INVTRIG)
+def test_hostmath_atan_family_accuracy(nm):
+    """The own atan (and the reference's atan2 / atanpi wrappers around it)
+    against 320-bit values: no less accurate than the reference's atandd ...
+    atan2pidd on the same inputs, below a fixed cap on every row the
+    reference gets to DD accuracy, and never worse than the reference on the
+    rows where the wrapper's y / x leaves the double range."""
+    g = nt.load_golden("invtrig.npz")
+    got = nt.invtrig_eval(_host(nm), nm, g)
+    ex, ref = g[nm + "_exact"], g[nm + "_ref"]
+    fin = np.isfinite(ex[:, 0]) & (np.abs(ex[:, 0]) > 1e-290)
+    eo, er = nt.rel_err_vs_exact(got[fin], ex[fin]), nt.rel_err_vs_exact(ref[fin], ex[fin])
+    dd = er < 1e-30
+    assert dd.sum() > 4000
+    assert eo[dd].max() <= er[dd].max() and eo[dd].max() <= nt.INVTRIG_CAP[nm], (eo[dd].max(), er[dd].max())
+    assert (eo[~dd] <= er[~dd]).all()
+
+
+@pytest.mark.parametrize("nm", nt.INVTRIG)
+def test_hostmath_atan_family_special_values_match_reference_bits(nm):
+    """Non-finite arguments, zeros with and without a low limb, and every
+    pair of the fixture's special values give the reference's bits."""
+    g = nt.load_golden("invtrig.npz")
+    got = nt.invtrig_eval(_host(nm), nm, g)
+    ex = g[nm + "_exact"]
+    sp = ~(np.isfinite(ex[:, 0]) & (np.abs(ex[:, 0]) > 1e-290))
+    assert sp.sum() > 5
+    nt.assert_bits(got[sp], g[nm + "_ref"][sp], nm)
