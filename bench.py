@@ -451,7 +451,8 @@ def bench_elementwise(args, torch, mf):
             "asinh": datagen.erf_inputs, "acosh": datagen.acosh_inputs, "atanh": datagen.atanh_inputs,
             "exp10": datagen.exp10_inputs, "exp2m1": datagen.exp_inputs, "exp10m1": datagen.exp10_inputs,
             "log2p1": datagen.config0, "log10p1": datagen.config0, "sinpi": datagen.erf_inputs,
-            "cospi": datagen.erf_inputs, "tanpi": datagen.erf_inputs, "atan": datagen.erf_inputs}
+            "cospi": datagen.erf_inputs, "tanpi": datagen.erf_inputs, "atan": datagen.erf_inputs,
+            "asin": datagen.atanh_inputs, "acos": datagen.atanh_inputs}
     ops = {nm: getattr(mf, "MFHIP_" + nm.upper()) for nm in gens}
     for nm, gen in gens.items():
         x = torch.from_numpy(gen(rng, n)).cuda()

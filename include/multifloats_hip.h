@@ -135,6 +135,16 @@ MULTIFLOATS_API void atandd_v(const float64x2 *a, float64x2 *c, int64_t n);
 MULTIFLOATS_API void atanpidd_v(const float64x2 *a, float64x2 *c, int64_t n);
 MULTIFLOATS_API void atan2dd_v(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n);
 MULTIFLOATS_API void atan2pidd_v(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n);
+/* Array forms of asindd, acosdd, asinpidd, acospidd (float64x2.h:485-505;
+ * the Fortran asin, acos, asinpi, acospi). Own design (the reference
+ * evaluates libquadmath's fitted rationals, src/float64x2/inv_trig.inc:56-130):
+ * sqrt(1 - x^2) in three limbs, then the atan reduction of x / s or s / x;
+ * the reference's special cases and pi-scaled wrappers (inv_trig.inc:197-205).
+ * No less accurate than the reference on the same inputs. */
+MULTIFLOATS_API void asindd_v(const float64x2 *a, float64x2 *c, int64_t n);
+MULTIFLOATS_API void acosdd_v(const float64x2 *a, float64x2 *c, int64_t n);
+MULTIFLOATS_API void asinpidd_v(const float64x2 *a, float64x2 *c, int64_t n);
+MULTIFLOATS_API void acospidd_v(const float64x2 *a, float64x2 *c, int64_t n);
 /* c[i] = sqrt(a[i]), bit-identical to sqrtdd (float64x2.h:2190-2213). */
 MULTIFLOATS_API void sqrtdd_v(const float64x2 *a, float64x2 *c, int64_t n);
 /* Array forms of expdd / logdd / sindd (float64x2.h:453-483; src/float64x2/
@@ -291,7 +301,8 @@ enum {
   MFHIP_LOG10 = 11, MFHIP_LOG1P = 12, MFHIP_SINH = 13, MFHIP_COSH = 14, MFHIP_TANH = 15,
   MFHIP_ERFC = 16, MFHIP_ERFCX = 17, MFHIP_ASINH = 18, MFHIP_ACOSH = 19, MFHIP_ATANH = 20,
   MFHIP_EXP10 = 21, MFHIP_EXP2M1 = 22, MFHIP_EXP10M1 = 23, MFHIP_LOG2P1 = 24, MFHIP_LOG10P1 = 25,
-  MFHIP_SINPI = 26, MFHIP_COSPI = 27, MFHIP_TANPI = 28, MFHIP_ATAN = 29, MFHIP_ATANPI = 30
+  MFHIP_SINPI = 26, MFHIP_COSPI = 27, MFHIP_TANPI = 28, MFHIP_ATAN = 29, MFHIP_ATANPI = 30,
+  MFHIP_ASIN = 31, MFHIP_ACOS = 32, MFHIP_ASINPI = 33, MFHIP_ACOSPI = 34
 };
 enum {
   MFHIP_ADD = 0, MFHIP_SUB = 1, MFHIP_MUL = 2, MFHIP_DIV = 3, MFHIP_POW = 4, MFHIP_HYPOT = 5, MFHIP_FDIM = 6,

@@ -34,7 +34,8 @@ DD_FMA_RENORM_INTERVAL = 8
 MFHIP_SQRT, MFHIP_EXP, MFHIP_LOG, MFHIP_SIN, MFHIP_ERF, MFHIP_NEG = range(6)
 (MFHIP_COS, MFHIP_TAN, MFHIP_EXP2, MFHIP_EXPM1, MFHIP_LOG2, MFHIP_LOG10, MFHIP_LOG1P, MFHIP_SINH, MFHIP_COSH,
  MFHIP_TANH, MFHIP_ERFC, MFHIP_ERFCX, MFHIP_ASINH, MFHIP_ACOSH, MFHIP_ATANH, MFHIP_EXP10, MFHIP_EXP2M1,
- MFHIP_EXP10M1, MFHIP_LOG2P1, MFHIP_LOG10P1, MFHIP_SINPI, MFHIP_COSPI, MFHIP_TANPI, MFHIP_ATAN, MFHIP_ATANPI) = range(6, 31)
+ MFHIP_EXP10M1, MFHIP_LOG2P1, MFHIP_LOG10P1, MFHIP_SINPI, MFHIP_COSPI, MFHIP_TANPI, MFHIP_ATAN, MFHIP_ATANPI,
+ MFHIP_ASIN, MFHIP_ACOS, MFHIP_ASINPI, MFHIP_ACOSPI) = range(6, 35)
 (MFHIP_ADD, MFHIP_SUB, MFHIP_MUL, MFHIP_DIV, MFHIP_POW, MFHIP_HYPOT, MFHIP_FDIM, MFHIP_FMIN, MFHIP_FMAX,
  MFHIP_COPYSIGN, MFHIP_ATAN2, MFHIP_ATAN2PI) = range(12)
 MFHIP_DOT, MFHIP_SUM, MFHIP_NORM2, MFHIP_PRODUCT, MFHIP_MAXVAL, MFHIP_MINVAL = range(6)
@@ -72,7 +73,8 @@ def lib():
             getattr(L, nm).restype = None
         for nm in ("sqrtdd_v", "expdd_v", "logdd_v", "sindd_v", "erfdd_v", "erfcdd_v", "erfcxdd_v", "asinhdd_v",
                    "acoshdd_v", "atanhdd_v", "exp10dd_v", "exp2m1dd_v", "exp10m1dd_v", "log2p1dd_v",
-                   "log10p1dd_v", "sinpidd_v", "cospidd_v", "tanpidd_v", "atandd_v", "atanpidd_v") + tuple(f + "dd_v" for f in _TRANSC2):
+                   "log10p1dd_v", "sinpidd_v", "cospidd_v", "tanpidd_v", "atandd_v", "atanpidd_v",
+                   "asindd_v", "acosdd_v", "asinpidd_v", "acospidd_v") + tuple(f + "dd_v" for f in _TRANSC2):
             getattr(L, nm).argtypes = [P, P, I64]
             getattr(L, nm).restype = None
         L.sumdd.argtypes = [P, I64]
@@ -379,6 +381,10 @@ def cospi(a): return _unary("cospidd_v", a)
 def tanpi(a): return _unary("tanpidd_v", a)
 def atan(a): return _unary("atandd_v", a)
 def atanpi(a): return _unary("atanpidd_v", a)
+def asin(a): return _unary("asindd_v", a)
+def acos(a): return _unary("acosdd_v", a)
+def asinpi(a): return _unary("asinpidd_v", a)
+def acospi(a): return _unary("acospidd_v", a)
 def add(a, b): return _binary("adddd_v", a, b)
 def sub(a, b): return _binary("subdd_v", a, b)
 def mul(a, b): return _binary("muldd_v", a, b)

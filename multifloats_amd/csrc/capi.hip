@@ -469,6 +469,10 @@ MF_UNARY(cospidd_v, mfhip::OP_COSPI)
 MF_UNARY(tanpidd_v, mfhip::OP_TANPI)
 MF_UNARY(atandd_v, mfhip::OP_ATAN)
 MF_UNARY(atanpidd_v, mfhip::OP_ATANPI)
+MF_UNARY(asindd_v, mfhip::OP_ASIN)
+MF_UNARY(acosdd_v, mfhip::OP_ACOS)
+MF_UNARY(asinpidd_v, mfhip::OP_ASINPI)
+MF_UNARY(acospidd_v, mfhip::OP_ACOSPI)
 
 float64x2 sumdd(const float64x2 *a, int64_t n) {
   float64x2 r;

@@ -951,3 +951,122 @@ MF_HD ddv dd_atanpi(ddv x) {
   return dd_mul(dd_atan(x), mk(mf_inv_pi_hi, mf_inv_pi_lo));
 }
 MF_HD ddv dd_atan2pi(ddv y, ddv x) { return dd_mul(dd_atan2(y, x), mk(mf_inv_pi_hi, mf_inv_pi_lo)); }
+
+// ------------------------------------------------------------- asin, acos
+// Own design on the atan reduction above, with every quantity that can
+// cancel carried in three limbs: s = sqrt(1 - x^2) from the exact expansion of
+// x^2 and one Newton correction of the double-double root; then
+// asin |x| = atan(|x| / s), acos |x| = atan(s / |x|), acos x = pi - acos |x|
+// for x < 0 (the reference's special cases, inv_trig.inc:56-130, are kept).
+struct td3 {
+  double h, m, l;
+};
+// a - c b for three-limb a, b and a short double c, summed from exact products
+MF_HD ddv td_sub_cmul(td3 a, double c, td3 b) {
+  double p1, e1, p2, e2;
+  two_prod(c, b.h, p1, e1);
+  two_prod(c, b.m, p2, e2);
+  double T0 = a.h, T1 = 0.0, T2 = 0.0;
+  tsum3(T0, T1, T2, -p1);
+  tsum3(T0, T1, T2, a.m);
+  tsum3(T0, T1, T2, -p2);
+  tsum3(T0, T1, T2, -e1);
+  tsum3(T0, T1, T2, a.l - c * b.l - e2);
+  double u;
+  two_sum(T1, T2, T1, u);
+  two_sum(T0, T1, T0, u);
+  return mk(T0, u);
+}
+// atan(n / d) for n, d >= 0 given in three limbs, not both zero
+MF_HD ddv atan_ratio(td3 n, td3 d) {
+  if (d.h == 0.0) return mk(mf_pio2_cw1, mf_pio2_cw2);
+  if (n.h == 0.0) return mk(0.0, 0.0);
+  const ddv nd = mk(n.h, n.m), dd = mk(d.h, d.m);
+  const ddv z = dd_div(nd, dd);
+  if (z.hi < 0.125) {
+    // z in three limbs: z + (n - z d) / d, the residual from exact products
+    double p1, e1, p2, e2, p3, e3;
+    two_prod(z.hi, d.h, p1, e1);
+    two_prod(z.hi, d.m, p2, e2);
+    two_prod(z.lo, d.h, p3, e3);
+    // the partial sums are ~2^-53 of n and r only ~2^-106: add them exactly
+    double R0 = n.h - p1, R1 = 0.0, R2 = 0.0;  // exact: p1 is within an ulp of n.h
+    tsum3(R0, R1, R2, -e1);
+    tsum3(R0, R1, R2, n.m);
+    tsum3(R0, R1, R2, -p2);
+    tsum3(R0, R1, R2, -p3);
+    tsum3(R0, R1, R2, n.l - e2 - e3 - z.hi * d.l - z.lo * d.m);
+    const double z2 = (R0 + (R1 + R2)) / d.h;
+    double zh, zl, zc;
+    fast_two_sum(z.hi, z.lo, zh, zl);
+    two_sum(zl, z2, zl, zc);
+    return sum3_round(zh, zl, zc, atan_tail(z));
+  }
+  if (z.hi <= 8.0) {
+    const int k = (int)__builtin_rint(z.hi * 64.0);
+    const double c = (double)k * 0.015625;
+    const ddv num = td_sub_cmul(n, c, d);
+    const ddv den = dd_add(dd, dd_mul(mk(c, 0.0), nd));
+    const ddv u = dd_div(num, den);
+    const double *b = mf_atant + 3 * (k - 8);
+    return sum3_round(b[0], b[1], b[2], dd_add(u, atan_tail(u)));
+  }
+  const ddv w = dd_div(dd, nd);
+  return sum3_round(mf_pio2_cw1, mf_pio2_cw2, mf_pio2_cw3, dd_neg(dd_add(w, atan_tail(w))));
+}
+// sqrt(1 - x^2) in three limbs for |x| <= 1 (NaN beyond, as sqrt of a negative)
+MF_HD td3 sqrt_one_minus_sq(ddv x) {
+  double p1, e1, p2, e2;
+  two_prod(x.hi, x.hi, p1, e1);
+  two_prod(2.0 * x.hi, x.lo, p2, e2);
+  double T0 = 1.0, T1 = 0.0, T2 = 0.0;
+  tsum3(T0, T1, T2, -p1);
+  tsum3(T0, T1, T2, -p2);
+  tsum3(T0, T1, T2, -e1);
+  tsum3(T0, T1, T2, -e2 - x.lo * x.lo);
+  double u;
+  two_sum(T1, T2, T1, u);
+  T2 = u;
+  two_sum(T0, T1, T0, u);
+  T1 = u;
+  if (T0 == 0.0) return td3{0.0, 0.0, 0.0};
+  const ddv sq = dd_sqrt(mk(T0, T1));
+  if (!(sq.hi > 0.0)) return td3{sq.hi, 0.0, 0.0};  // NaN for 1 - x^2 < 0
+  double q1, f1, q2, f2;
+  two_prod(sq.hi, sq.hi, q1, f1);
+  two_prod(2.0 * sq.hi, sq.lo, q2, f2);
+  double P0 = T0 - q1, P1 = 0.0, P2 = 0.0;  // exact: q1 is within an ulp of T0
+  tsum3(P0, P1, P2, T1);
+  tsum3(P0, P1, P2, -q2);
+  tsum3(P0, P1, P2, -f1);
+  tsum3(P0, P1, P2, T2 - f2 - sq.lo * sq.lo);
+  return td3{sq.hi, sq.lo, (P0 + (P1 + P2)) / (2.0 * sq.hi)};
+}
+MF_HD ddv dd_asin(ddv x) {
+  if (!mf_isfinite(x.hi)) return mk(__builtin_nan(""), 0.0);
+  if (__builtin_fabs(x.hi) > 1.0) return mk(__builtin_nan(""), 0.0);
+  if (__builtin_fabs(x.hi) < 1.85e-16) return x;  // the cubic term is below half a DD ulp
+  const bool neg = x.hi < 0.0;
+  const ddv ax = neg ? dd_neg(x) : x;
+  const td3 s = sqrt_one_minus_sq(ax);
+  if (s.h != s.h) return mk(__builtin_nan(""), 0.0);
+  const ddv r = atan_ratio(td3{ax.hi, ax.lo, 0.0}, s);
+  return neg ? dd_neg(r) : r;
+}
+MF_HD ddv dd_acos(ddv x) {
+  if (!mf_isfinite(x.hi)) return mk(__builtin_nan(""), 0.0);
+  if (__builtin_fabs(x.hi) > 1.0) return mk(__builtin_nan(""), 0.0);
+  const bool neg = x.hi < 0.0;
+  const ddv ax = neg ? dd_neg(x) : x;
+  const td3 s = sqrt_one_minus_sq(ax);
+  if (s.h != s.h) return mk(__builtin_nan(""), 0.0);
+  const ddv r = atan_ratio(s, td3{ax.hi, ax.lo, 0.0});
+  if (!neg) return r;
+  return sum3_round(2.0 * mf_pio2_cw1, 2.0 * mf_pio2_cw2, 2.0 * mf_pio2_cw3, dd_neg(r));
+}
+// inv_trig.inc:197-205: asin / pi (odd, zeros kept) and acos / pi
+MF_HD ddv dd_asinpi(ddv x) {
+  if (x.hi == 0.0 && x.lo == 0.0) return x;
+  return dd_mul(dd_asin(x), mk(mf_inv_pi_hi, mf_inv_pi_lo));
+}
+MF_HD ddv dd_acospi(ddv x) { return dd_mul(dd_acos(x), mk(mf_inv_pi_hi, mf_inv_pi_lo)); }

@@ -52,6 +52,10 @@ __device__ __forceinline__ ddv unary_op(ddv x) {
   else if constexpr (OP == mfhip::OP_TANPI) return dd_tanpi(x);
   else if constexpr (OP == mfhip::OP_ATAN) return dd_atan(x);
   else if constexpr (OP == mfhip::OP_ATANPI) return dd_atanpi(x);
+  else if constexpr (OP == mfhip::OP_ASIN) return dd_asin(x);
+  else if constexpr (OP == mfhip::OP_ACOS) return dd_acos(x);
+  else if constexpr (OP == mfhip::OP_ASINPI) return dd_asinpi(x);
+  else if constexpr (OP == mfhip::OP_ACOSPI) return dd_acospi(x);
   else return x;
 }
 
@@ -214,6 +218,10 @@ hipError_t unary(int op, const double2 *a, double2 *c, int64_t n, hipStream_t s)
     case OP_TANPI: return run_unary<OP_TANPI>(a, c, n, s);
     case OP_ATAN: return run_unary<OP_ATAN>(a, c, n, s);
     case OP_ATANPI: return run_unary<OP_ATANPI>(a, c, n, s);
+    case OP_ASIN: return run_unary<OP_ASIN>(a, c, n, s);
+    case OP_ACOS: return run_unary<OP_ACOS>(a, c, n, s);
+    case OP_ASINPI: return run_unary<OP_ASINPI>(a, c, n, s);
+    case OP_ACOSPI: return run_unary<OP_ACOSPI>(a, c, n, s);
     default: return hipErrorInvalidValue;
   }
 }

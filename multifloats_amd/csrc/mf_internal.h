@@ -18,7 +18,8 @@ enum {
   OP_LOG10 = 11, OP_LOG1P = 12, OP_SINH = 13, OP_COSH = 14, OP_TANH = 15,
   OP_ERFC = 16, OP_ERFCX = 17, OP_ASINH = 18, OP_ACOSH = 19, OP_ATANH = 20,
   OP_EXP10 = 21, OP_EXP2M1 = 22, OP_EXP10M1 = 23, OP_LOG2P1 = 24, OP_LOG10P1 = 25,
-  OP_SINPI = 26, OP_COSPI = 27, OP_TANPI = 28, OP_ATAN = 29, OP_ATANPI = 30
+  OP_SINPI = 26, OP_COSPI = 27, OP_TANPI = 28, OP_ATAN = 29, OP_ATANPI = 30,
+  OP_ASIN = 31, OP_ACOS = 32, OP_ASINPI = 33, OP_ACOSPI = 34
 };
 enum {
   OP_ADD = 0, OP_SUB = 1, OP_MUL = 2, OP_DIV = 3, OP_POW = 4, OP_HYPOT = 5, OP_FDIM = 6, OP_FMIN = 7,

@@ -19,7 +19,8 @@ UNARY = [("exp", "expdd_v"), ("log", "logdd_v"), ("sin", "sindd_v"), ("erf", "er
          ("log10", "log10dd_v"), ("log1p", "log1pdd_v"), ("sinh", "sinhdd_v"), ("cosh", "coshdd_v"),
          ("tanh", "tanhdd_v"), ("erfc", "erfcdd_v"), ("erfc_scaled", "erfcxdd_v"), ("asinh", "asinhdd_v"),
          ("acosh", "acoshdd_v"), ("atanh", "atanhdd_v"), ("sinpi", "sinpidd_v"), ("cospi", "cospidd_v"),
-         ("tanpi", "tanpidd_v"), ("atan", "atandd_v"), ("atanpi", "atanpidd_v")]
+         ("tanpi", "tanpidd_v"), ("atan", "atandd_v"), ("atanpi", "atanpidd_v"),
+         ("asin", "asindd_v"), ("acos", "acosdd_v"), ("asinpi", "asinpidd_v"), ("acospi", "acospidd_v")]
 # reductions with mask= and dim= (fypp:1783-2090): generic, C entry stem
 MASKED_REDUCE = [("sum", "sumdd"), ("product", "productdd"), ("maxval", "maxvaldd"), ("minval", "minvaldd")]
 BINARY = [("+", "add", "adddd_v"), ("-", "sub", "subdd_v"), ("*", "mul", "muldd_v"), ("/", "div", "divdd_v")]
@@ -59,7 +60,8 @@ def gen():
 !   maxval minval   dd_maxval_Nd / dd_minval_Nd (+ _dim) 1783-1862 (with mask)
 !   norm2           dd_norm2_Nd / dd_norm2_Nd_dim 2338-2377
 !   exp log sin erf sqrt cos tan expm1 log2 log10 log1p sinh cosh tanh
-!   erfc erfc_scaled asinh acosh atanh sinpi cospi tanpi atan atanpi
+!   erfc erfc_scaled asinh acosh atanh sinpi cospi tanpi atan atanpi asin acos
+!   asinpi acospi
 !                   elemental delegates 709-736, 1407-1412 (names fypp:57)
 !   hypot sign atan2 atan2pi  elemental binary functions 1550-1592
 !   + - * /         elemental native operators 2746-2837
@@ -110,7 +112,7 @@ module multifloats_hip
     public :: matmul, dot_product, sum, product, maxval, minval, norm2
     public :: exp, log, sin, erf, sqrt, cos, tan, expm1, log2, log10, log1p, sinh, cosh, tanh
     public :: erfc, erfc_scaled, asinh, acosh, atanh, sinpi, cospi, tanpi
-    public :: hypot, sign, atan, atanpi, atan2, atan2pi
+    public :: hypot, sign, atan, atanpi, atan2, atan2pi, asin, acos, asinpi, acospi
     public :: operator(+), operator(-), operator(*), operator(/)
 
     interface

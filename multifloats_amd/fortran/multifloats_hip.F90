@@ -13,7 +13,8 @@
 !   maxval minval   dd_maxval_Nd / dd_minval_Nd (+ _dim) 1783-1862 (with mask)
 !   norm2           dd_norm2_Nd / dd_norm2_Nd_dim 2338-2377
 !   exp log sin erf sqrt cos tan expm1 log2 log10 log1p sinh cosh tanh
-!   erfc erfc_scaled asinh acosh atanh sinpi cospi tanpi atan atanpi
+!   erfc erfc_scaled asinh acosh atanh sinpi cospi tanpi atan atanpi asin acos
+!   asinpi acospi
 !                   elemental delegates 709-736, 1407-1412 (names fypp:57)
 !   hypot sign atan2 atan2pi  elemental binary functions 1550-1592
 !   + - * /         elemental native operators 2746-2837
@@ -64,7 +65,7 @@ module multifloats_hip
     public :: matmul, dot_product, sum, product, maxval, minval, norm2
     public :: exp, log, sin, erf, sqrt, cos, tan, expm1, log2, log10, log1p, sinh, cosh, tanh
     public :: erfc, erfc_scaled, asinh, acosh, atanh, sinpi, cospi, tanpi
-    public :: hypot, sign, atan, atanpi, atan2, atan2pi
+    public :: hypot, sign, atan, atanpi, atan2, atan2pi, asin, acos, asinpi, acospi
     public :: operator(+), operator(-), operator(*), operator(/)
 
     interface
@@ -292,6 +293,26 @@ module multifloats_hip
             integer(c_int64_t), value :: n
         end subroutine
         pure subroutine c_atanpidd_v(a, c, n) bind(c, name='atanpidd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_asindd_v(a, c, n) bind(c, name='asindd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_acosdd_v(a, c, n) bind(c, name='acosdd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_asinpidd_v(a, c, n) bind(c, name='asinpidd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_acospidd_v(a, c, n) bind(c, name='acospidd_v')
             import :: c_ptr, c_int64_t
             type(c_ptr), value :: a, c
             integer(c_int64_t), value :: n
@@ -663,6 +684,46 @@ module multifloats_hip
         module procedure dd_atanpi_5d
         module procedure dd_atanpi_6d
         module procedure dd_atanpi_7d
+    end interface
+
+    interface asin
+        module procedure dd_asin_1d
+        module procedure dd_asin_2d
+        module procedure dd_asin_3d
+        module procedure dd_asin_4d
+        module procedure dd_asin_5d
+        module procedure dd_asin_6d
+        module procedure dd_asin_7d
+    end interface
+
+    interface acos
+        module procedure dd_acos_1d
+        module procedure dd_acos_2d
+        module procedure dd_acos_3d
+        module procedure dd_acos_4d
+        module procedure dd_acos_5d
+        module procedure dd_acos_6d
+        module procedure dd_acos_7d
+    end interface
+
+    interface asinpi
+        module procedure dd_asinpi_1d
+        module procedure dd_asinpi_2d
+        module procedure dd_asinpi_3d
+        module procedure dd_asinpi_4d
+        module procedure dd_asinpi_5d
+        module procedure dd_asinpi_6d
+        module procedure dd_asinpi_7d
+    end interface
+
+    interface acospi
+        module procedure dd_acospi_1d
+        module procedure dd_acospi_2d
+        module procedure dd_acospi_3d
+        module procedure dd_acospi_4d
+        module procedure dd_acospi_5d
+        module procedure dd_acospi_6d
+        module procedure dd_acospi_7d
     end interface
 
     interface operator(+)
@@ -2903,6 +2964,174 @@ contains
         type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:,:)
         type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6), size(x,7))
         call c_atanpidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_asin_1d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:)
+        type(real64x2), target :: res(size(x,1))
+        call c_asindd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_asin_2d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2))
+        call c_asindd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_asin_3d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3))
+        call c_asindd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_asin_4d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4))
+        call c_asindd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_asin_5d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5))
+        call c_asindd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_asin_6d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6))
+        call c_asindd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_asin_7d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6), size(x,7))
+        call c_asindd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_acos_1d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:)
+        type(real64x2), target :: res(size(x,1))
+        call c_acosdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_acos_2d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2))
+        call c_acosdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_acos_3d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3))
+        call c_acosdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_acos_4d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4))
+        call c_acosdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_acos_5d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5))
+        call c_acosdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_acos_6d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6))
+        call c_acosdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_acos_7d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6), size(x,7))
+        call c_acosdd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_asinpi_1d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:)
+        type(real64x2), target :: res(size(x,1))
+        call c_asinpidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_asinpi_2d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2))
+        call c_asinpidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_asinpi_3d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3))
+        call c_asinpidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_asinpi_4d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4))
+        call c_asinpidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_asinpi_5d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5))
+        call c_asinpidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_asinpi_6d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6))
+        call c_asinpidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_asinpi_7d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6), size(x,7))
+        call c_asinpidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_acospi_1d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:)
+        type(real64x2), target :: res(size(x,1))
+        call c_acospidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_acospi_2d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2))
+        call c_acospidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_acospi_3d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3))
+        call c_acospidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_acospi_4d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4))
+        call c_acospidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_acospi_5d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5))
+        call c_acospidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_acospi_6d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6))
+        call c_acospidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
+    end function
+
+    pure function dd_acospi_7d(x) result(res)
+        type(real64x2), intent(in), target, contiguous :: x(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(x,1), size(x,2), size(x,3), size(x,4), size(x,5), size(x,6), size(x,7))
+        call c_acospidd_v(c_loc(x), c_loc(res), int(size(x), c_int64_t))
     end function
 
     pure function dd_add_1d(a, b) result(res)

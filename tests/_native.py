@@ -32,7 +32,9 @@ TRANSC5 = ("sinpi", "cospi", "tanpi")
 BINARY2 = ("pow", "hypot", "fdim", "fmin", "fmax", "copysign")
 INVTRIG = ("atan", "atanpi", "atan2", "atan2pi")
 # worst relative error of this build on the DD-accurate rows of invtrig.npz
-INVTRIG_CAP = {"atan": 1.3e-32, "atanpi": 4.0e-32, "atan2": 5.5e-32, "atan2pi": 6.0e-32}
+INVTRIG_CAP = {"atan": 1.3e-32, "atanpi": 4.0e-32, "atan2": 5.5e-32, "atan2pi": 6.0e-32, "asin": 1.3e-32,
+               "acos": 2.1e-32, "asinpi": 3.3e-32, "acospi": 4.0e-32}
+INVTRIG3 = ("asin", "acos", "asinpi", "acospi")
 
 
 def oracle():
@@ -300,7 +302,11 @@ def load_golden(name):
     return np.load(os.path.join(GOLDEN, name), allow_pickle=False)
 
 
+def invtrig_fixture(nm):
+    return load_golden("invtrig3.npz" if nm in INVTRIG3 else "invtrig.npz")
+
+
 def invtrig_eval(fn, nm, g):
     """Apply fn (one- or two-argument, numpy in/out) to the inputs of
-    invtrig.npz for function nm."""
-    return fn(g["x"]) if nm in ("atan", "atanpi") else fn(g["y2"], g["x2"])
+    invtrig.npz / invtrig3.npz for function nm."""
+    return fn(g["x"]) if nm in ("atan", "atanpi") + INVTRIG3 else fn(g["y2"], g["x2"])

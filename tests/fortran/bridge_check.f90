@@ -93,6 +93,8 @@ program bridge_check
     write(u) atanpi(x)
     write(u) atan2(x, y)
     write(u) atan2pi(x, y)
+    write(u) asin(x * y / (x * x + y * y))   ! |argument| <= 1/2
+    write(u) acos(x * y / (x * x + y * y))
     ! xGEMM: C = x(1) A B + y(1) C with A passed transposed and C padded to m + 2 rows
     at = transpose(a)
     allocate(cc(m + 2, n))

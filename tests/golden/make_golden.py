@@ -45,7 +45,7 @@ def load_ref():
             f + "dd" for f in TRANSC2 + TRANSC3 + TRANSC4 + TRANSC5):
         getattr(L, nm).argtypes = [F2]
         getattr(L, nm).restype = F2
-    for nm in INVTRIG1:
+    for nm in INVTRIG1 + INVTRIG3:
         getattr(L, nm + "dd").argtypes = [F2]
         getattr(L, nm + "dd").restype = F2
     for nm in BINARY2 + INVTRIG2:
@@ -171,6 +171,7 @@ TRANSC5 = ("sinpi", "cospi", "tanpi")
 BINARY2 = ("pow", "hypot", "fdim", "fmin", "fmax", "copysign")
 INVTRIG1 = ("atan", "atanpi")
 INVTRIG2 = ("atan2", "atan2pi")
+INVTRIG3 = ("asin", "acos", "asinpi", "acospi")
 
 
 def transc2(L, rng):
@@ -455,6 +456,39 @@ def invtrig(L, rng):
     np.savez_compressed(os.path.join(HERE, "invtrig.npz"), **out)
 
 
+def invtrig3(L, rng):
+    """asin, acos, asinpi, acospi (own designs, so the bar is accuracy): the
+    reference's outputs and 320-bit values on draws over (-1, 1), near +-1
+    (1 - 10^-16 .. 1 - 10^-1), tiny |x|, the small-ratio, table and
+    reflected paths of the atan reduction, the domain edges and special
+    values."""
+    n = 2000
+    w = datagen._with_lo
+    U = rng.uniform
+    sg = lambda k: rng.choice([-1.0, 1.0], k)
+    x = np.concatenate([w(rng, U(-1, 1, n)), w(rng, (1 - 10.0 ** U(-16, -1, 600)) * sg(600)),
+                        w(rng, 10.0 ** U(-18, -1, 600) * sg(600)), w(rng, U(-0.15, 0.15, 600)),
+                        np.array([(1.0, 0.0), (-1.0, 0.0), (1.0, -1e-17), (-1.0, 1e-17), (1.0, 1e-17), (0.5, 0.0),
+                                  (-0.5, 0.0), (np.nextafter(1.0, 0), 0.0), (1.85e-16, 0.0), (1e-300, 0.0)]),
+                        np.array(SPECIALS)])
+    out = {"x": x}
+    pi = mpmath.pi
+    for nm in INVTRIG3:
+        out[nm + "_ref"] = call1(L, nm + "dd", x)
+    fn = {"asin": mpmath.asin, "acos": mpmath.acos}
+    for nm in INVTRIG3:
+        ex = np.full((x.shape[0], 3), np.nan)
+        for i in range(x.shape[0]):
+            v = mp_of(x[i]) if np.isfinite(x[i, 0]) else None
+            if v is None or abs(v) > 1:
+                continue
+            r = fn[nm[:4]](v) / (pi if nm.endswith("pi") else 1)
+            if r != 0:
+                ex[i] = limbs3(r)
+        out[nm + "_exact"] = ex
+    np.savez_compressed(os.path.join(HERE, "invtrig3.npz"), **out)
+
+
 def special2(L, rng):
     """erfc and erfcx: per-regime draws (the branch points of both the
     reference's fits and this project's rows), the reference's own outputs on
@@ -572,8 +606,11 @@ def main(which):
         binary2(L, np.random.default_rng(7171))
     if not which or "invtrig" in which:
         invtrig(L, np.random.default_rng(8181))
+    if not which or "invtrig3" in which:
+        invtrig3(L, np.random.default_rng(9191))
     for f in ("arith.npz", "transc.npz", "matmul.npz", "cmatmul.npz", "transc2.npz", "special2.npz", "transc3.npz",
-              "transc4.npz", "transc5.npz", "binary2.npz", "invtrig.npz"):
+              "transc4.npz", "transc5.npz", "binary2.npz", "invtrig.npz",
+              "invtrig3.npz"):
         print(f, os.path.getsize(os.path.join(HERE, f)))
 
 

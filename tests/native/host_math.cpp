@@ -34,3 +34,7 @@ H1(h_atan, dd_atan)
 H1(h_atanpi, dd_atanpi)
 H2(h_atan2, dd_atan2)
 H2(h_atan2pi, dd_atan2pi)
+H1(h_asin, dd_asin)
+H1(h_acos, dd_acos)
+H1(h_asinpi, dd_asinpi)
+H1(h_acospi, dd_acospi)

@@ -109,6 +109,9 @@ def test_fortran_bridge_routes_to_the_library(gpu, tmp_path):
     nt.assert_bits(r.take(nv), mf.atanpi(x), "atanpi")
     nt.assert_bits(r.take(nv), mf.atan2(x, y), "atan2")
     nt.assert_bits(r.take(nv), mf.atan2pi(x, y), "atan2pi")
+    q = mf.div(mf.mul(x, y), mf.add(mf.mul(x, x), mf.mul(y, y)))
+    nt.assert_bits(r.take(nv), mf.asin(q), "asin")
+    nt.assert_bits(r.take(nv), mf.acos(q), "acos")
     At = nt.cm(A.transpose(1, 0, 2))
     Cw = np.empty((m + 2, n, 2))
     Cw[:m] = A[:, :n]
