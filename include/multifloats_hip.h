@@ -252,8 +252,9 @@ MULTIFLOATS_API void minvaldd_dim(const float64x2 *a, const int32_t *mask, int64
                                   int64_t outer, float64x2 *out);
 
 /* BLAS-style C = alpha op(A) op(B) + beta C on float64x2 (the interface of
- * the reference's xGEMM shim test/integration/blas/wgemm.f90:1-14; SURVEY's
- * "next" GEMM interface). transa/transb: 'N', 'T' or 'C' (= 'T' for real);
+ * the reference's xGEMM shim test/integration/blas/wgemm.f90:1-14, the GEMM
+ * interface its architecture notes defer, doc/dev/architecture.md:307-310).
+ * transa/transb: 'N', 'T' or 'C' (= 'T' for real);
  * column-major with leading dimensions lda, ldb, ldc; argument checks and
  * quick returns as wgemm.f90:16-69 (alpha == 0 or k == 0: C = beta C without
  * reading A, B; beta == 0: C is not read). op(A) op(B) is formed by the
