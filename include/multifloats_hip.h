@@ -135,6 +135,14 @@ MULTIFLOATS_API void atandd_v(const float64x2 *a, float64x2 *c, int64_t n);
 MULTIFLOATS_API void atanpidd_v(const float64x2 *a, float64x2 *c, int64_t n);
 MULTIFLOATS_API void atan2dd_v(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n);
 MULTIFLOATS_API void atan2pidd_v(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n);
+/* Array forms of the Fortran generics min, max and dim on real64x2 (the
+ * reference's module bodies, fsrc/multifloats.fypp:1539-1552, 1573-1583):
+ * min = a < b ? a : b, max = a < b ? b : a (the module's lexicographic `<`),
+ * dim = a - b when either is NaN, else a > b ? a - b : 0. Bit-identical to
+ * them; they differ from fmindd / fmaxdd / fdimdd only for NaN operands. */
+MULTIFLOATS_API void mindd_v(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n);
+MULTIFLOATS_API void maxdd_v(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n);
+MULTIFLOATS_API void dimdd_v(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n);
 /* Array forms of asindd, acosdd, asinpidd, acospidd (float64x2.h:485-505;
  * the Fortran asin, acos, asinpi, acospi). Own design (the reference
  * evaluates libquadmath's fitted rationals, src/float64x2/inv_trig.inc:56-130):
@@ -307,7 +315,8 @@ enum {
 };
 enum {
   MFHIP_ADD = 0, MFHIP_SUB = 1, MFHIP_MUL = 2, MFHIP_DIV = 3, MFHIP_POW = 4, MFHIP_HYPOT = 5, MFHIP_FDIM = 6,
-  MFHIP_FMIN = 7, MFHIP_FMAX = 8, MFHIP_COPYSIGN = 9, MFHIP_ATAN2 = 10, MFHIP_ATAN2PI = 11
+  MFHIP_FMIN = 7, MFHIP_FMAX = 8, MFHIP_COPYSIGN = 9, MFHIP_ATAN2 = 10, MFHIP_ATAN2PI = 11,
+  MFHIP_MINF = 12, MFHIP_MAXF = 13, MFHIP_DIMF = 14
 };
 enum { MFHIP_DOT = 0, MFHIP_SUM = 1, MFHIP_NORM2 = 2, MFHIP_PRODUCT = 3, MFHIP_MAXVAL = 4, MFHIP_MINVAL = 5 };
 

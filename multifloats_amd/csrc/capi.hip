@@ -439,6 +439,9 @@ MF_BINARY(fmaxdd_v, mfhip::OP_FMAX)
 MF_BINARY(copysigndd_v, mfhip::OP_COPYSIGN)
 MF_BINARY(atan2dd_v, mfhip::OP_ATAN2)
 MF_BINARY(atan2pidd_v, mfhip::OP_ATAN2PI)
+MF_BINARY(mindd_v, mfhip::OP_MINF)
+MF_BINARY(maxdd_v, mfhip::OP_MAXF)
+MF_BINARY(dimdd_v, mfhip::OP_DIMF)
 MF_UNARY(sqrtdd_v, mfhip::OP_SQRT)
 MF_UNARY(expdd_v, mfhip::OP_EXP)
 MF_UNARY(logdd_v, mfhip::OP_LOG)

@@ -432,6 +432,16 @@ MF_HD int mf_ilogb(double v) {
 // float64x2.h:1299-1309: the non-NaN operand when exactly one is NaN
 MF_HD ddv dd_fmin(ddv a, ddv b) { return (dd_lt(a, b) || b.hi != b.hi) ? a : b; }
 MF_HD ddv dd_fmax(ddv a, ddv b) { return (dd_lt(a, b) || a.hi != a.hi) ? b : a; }
+// The Fortran generics min, max and dim of the reference's module
+// (fsrc/multifloats.fypp:1539-1552, 1573-1583): the order is its `<` (the same
+// lexicographic rule as dd_lt), so a NaN in y gives y for min and x for max;
+// dim gives x - y when either is NaN (C fdim gives (NaN, 0)).
+MF_HD ddv dd_fortran_min(ddv x, ddv y) { return dd_lt(x, y) ? x : y; }
+MF_HD ddv dd_fortran_max(ddv x, ddv y) { return dd_lt(x, y) ? y : x; }
+MF_HD ddv dd_fortran_dim(ddv x, ddv y) {
+  if (x.hi != x.hi || y.hi != y.hi) return dd_sub(x, y);
+  return dd_lt(y, x) ? dd_sub(x, y) : mk(0.0, 0.0);
+}
 // float64x2.h:1518-1520
 MF_HD ddv dd_copysign(ddv x, ddv y) { return dd_signbit(x) == dd_signbit(y) ? x : dd_neg(x); }
 // float64x2.h:2130-2136: max(x - y, 0), NaN if either is NaN

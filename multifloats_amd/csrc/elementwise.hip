@@ -72,7 +72,10 @@ __device__ __forceinline__ ddv binary_op(ddv x, ddv y) {
   else if constexpr (OP == mfhip::OP_FMAX) return dd_fmax(x, y);
   else if constexpr (OP == mfhip::OP_COPYSIGN) return dd_copysign(x, y);
   else if constexpr (OP == mfhip::OP_ATAN2) return dd_atan2(x, y);
-  else return dd_atan2pi(x, y);
+  else if constexpr (OP == mfhip::OP_ATAN2PI) return dd_atan2pi(x, y);
+  else if constexpr (OP == mfhip::OP_MINF) return dd_fortran_min(x, y);
+  else if constexpr (OP == mfhip::OP_MAXF) return dd_fortran_max(x, y);
+  else return dd_fortran_dim(x, y);
 }
 
 template <int OP, int U>
@@ -241,6 +244,9 @@ hipError_t binary(int op, const double2 *a, const double2 *b, double2 *c, int64_
     case OP_COPYSIGN: return run_binary<OP_COPYSIGN>(a, b, c, n, s);
     case OP_ATAN2: return run_binary<OP_ATAN2>(a, b, c, n, s);
     case OP_ATAN2PI: return run_binary<OP_ATAN2PI>(a, b, c, n, s);
+    case OP_MINF: return run_binary<OP_MINF>(a, b, c, n, s);
+    case OP_MAXF: return run_binary<OP_MAXF>(a, b, c, n, s);
+    case OP_DIMF: return run_binary<OP_DIMF>(a, b, c, n, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -26,7 +26,8 @@ MASKED_REDUCE = [("sum", "sumdd"), ("product", "productdd"), ("maxval", "maxvald
 BINARY = [("+", "add", "adddd_v"), ("-", "sub", "subdd_v"), ("*", "mul", "muldd_v"), ("/", "div", "divdd_v")]
 # two-argument elemental generics whose reference bodies are these C functions'
 # rules (fypp:1550-1587: hypot calls hypotdd; sign is copysigndd's rule)
-BINFN = [("hypot", "hypotdd_v"), ("sign", "copysigndd_v"), ("atan2", "atan2dd_v"), ("atan2pi", "atan2pidd_v")]
+BINFN = [("hypot", "hypotdd_v"), ("sign", "copysigndd_v"), ("atan2", "atan2dd_v"), ("atan2pi", "atan2pidd_v"),
+         ("min", "mindd_v"), ("max", "maxdd_v"), ("dim", "dimdd_v")]
 
 
 def colons(r):
@@ -63,7 +64,7 @@ def gen():
 !   erfc erfc_scaled asinh acosh atanh sinpi cospi tanpi atan atanpi asin acos
 !   asinpi acospi
 !                   elemental delegates 709-736, 1407-1412 (names fypp:57)
-!   hypot sign atan2 atan2pi  elemental binary functions 1550-1592
+!   hypot sign atan2 atan2pi min max dim  elemental binary functions 1536-1592
 !   + - * /         elemental native operators 2746-2837
 !   DD_FMA_RENORM_INTERVAL and dd_set_fma_renorm_interval 664, 2677-2680
 !
@@ -112,7 +113,7 @@ module multifloats_hip
     public :: matmul, dot_product, sum, product, maxval, minval, norm2
     public :: exp, log, sin, erf, sqrt, cos, tan, expm1, log2, log10, log1p, sinh, cosh, tanh
     public :: erfc, erfc_scaled, asinh, acosh, atanh, sinpi, cospi, tanpi
-    public :: hypot, sign, atan, atanpi, atan2, atan2pi, asin, acos, asinpi, acospi
+    public :: hypot, sign, atan, atanpi, atan2, atan2pi, asin, acos, asinpi, acospi, min, max, dim
     public :: operator(+), operator(-), operator(*), operator(/)
 
     interface

@@ -16,7 +16,7 @@
 !   erfc erfc_scaled asinh acosh atanh sinpi cospi tanpi atan atanpi asin acos
 !   asinpi acospi
 !                   elemental delegates 709-736, 1407-1412 (names fypp:57)
-!   hypot sign atan2 atan2pi  elemental binary functions 1550-1592
+!   hypot sign atan2 atan2pi min max dim  elemental binary functions 1536-1592
 !   + - * /         elemental native operators 2746-2837
 !   DD_FMA_RENORM_INTERVAL and dd_set_fma_renorm_interval 664, 2677-2680
 !
@@ -65,7 +65,7 @@ module multifloats_hip
     public :: matmul, dot_product, sum, product, maxval, minval, norm2
     public :: exp, log, sin, erf, sqrt, cos, tan, expm1, log2, log10, log1p, sinh, cosh, tanh
     public :: erfc, erfc_scaled, asinh, acosh, atanh, sinpi, cospi, tanpi
-    public :: hypot, sign, atan, atanpi, atan2, atan2pi, asin, acos, asinpi, acospi
+    public :: hypot, sign, atan, atanpi, atan2, atan2pi, asin, acos, asinpi, acospi, min, max, dim
     public :: operator(+), operator(-), operator(*), operator(/)
 
     interface
@@ -353,6 +353,21 @@ module multifloats_hip
             integer(c_int64_t), value :: n
         end subroutine
         pure subroutine c_atan2pidd_v(a, b, c, n) bind(c, name='atan2pidd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, b, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_mindd_v(a, b, c, n) bind(c, name='mindd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, b, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_maxdd_v(a, b, c, n) bind(c, name='maxdd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, b, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_dimdd_v(a, b, c, n) bind(c, name='dimdd_v')
             import :: c_ptr, c_int64_t
             type(c_ptr), value :: a, b, c
             integer(c_int64_t), value :: n
@@ -804,6 +819,36 @@ module multifloats_hip
         module procedure dd_atan2pi_5d
         module procedure dd_atan2pi_6d
         module procedure dd_atan2pi_7d
+    end interface
+
+    interface min
+        module procedure dd_min_1d
+        module procedure dd_min_2d
+        module procedure dd_min_3d
+        module procedure dd_min_4d
+        module procedure dd_min_5d
+        module procedure dd_min_6d
+        module procedure dd_min_7d
+    end interface
+
+    interface max
+        module procedure dd_max_1d
+        module procedure dd_max_2d
+        module procedure dd_max_3d
+        module procedure dd_max_4d
+        module procedure dd_max_5d
+        module procedure dd_max_6d
+        module procedure dd_max_7d
+    end interface
+
+    interface dim
+        module procedure dd_dim_1d
+        module procedure dd_dim_2d
+        module procedure dd_dim_3d
+        module procedure dd_dim_4d
+        module procedure dd_dim_5d
+        module procedure dd_dim_6d
+        module procedure dd_dim_7d
     end interface
 
 contains
@@ -3468,6 +3513,132 @@ contains
         type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:,:,:), b(:,:,:,:,:,:,:)
         type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5), size(a,6), size(a,7))
         call c_atan2pidd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_min_1d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:), b(:)
+        type(real64x2), target :: res(size(a,1))
+        call c_mindd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_min_2d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:), b(:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2))
+        call c_mindd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_min_3d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:), b(:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3))
+        call c_mindd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_min_4d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:), b(:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4))
+        call c_mindd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_min_5d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:), b(:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5))
+        call c_mindd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_min_6d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:,:), b(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5), size(a,6))
+        call c_mindd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_min_7d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:,:,:), b(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5), size(a,6), size(a,7))
+        call c_mindd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_max_1d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:), b(:)
+        type(real64x2), target :: res(size(a,1))
+        call c_maxdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_max_2d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:), b(:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2))
+        call c_maxdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_max_3d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:), b(:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3))
+        call c_maxdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_max_4d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:), b(:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4))
+        call c_maxdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_max_5d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:), b(:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5))
+        call c_maxdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_max_6d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:,:), b(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5), size(a,6))
+        call c_maxdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_max_7d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:,:,:), b(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5), size(a,6), size(a,7))
+        call c_maxdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_dim_1d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:), b(:)
+        type(real64x2), target :: res(size(a,1))
+        call c_dimdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_dim_2d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:), b(:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2))
+        call c_dimdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_dim_3d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:), b(:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3))
+        call c_dimdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_dim_4d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:), b(:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4))
+        call c_dimdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_dim_5d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:), b(:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5))
+        call c_dimdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_dim_6d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:,:), b(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5), size(a,6))
+        call c_dimdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_dim_7d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:,:,:), b(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5), size(a,6), size(a,7))
+        call c_dimdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
     end function
 
 end module multifloats_hip

@@ -95,6 +95,9 @@ program bridge_check
     write(u) atan2pi(x, y)
     write(u) asin(x * y / (x * x + y * y))   ! |argument| <= 1/2
     write(u) acos(x * y / (x * x + y * y))
+    write(u) min(x, y)
+    write(u) max(x, y)
+    write(u) dim(x, y)
     ! xGEMM: C = x(1) A B + y(1) C with A passed transposed and C padded to m + 2 rows
     at = transpose(a)
     allocate(cc(m + 2, n))

@@ -112,6 +112,8 @@ def test_fortran_bridge_routes_to_the_library(gpu, tmp_path):
     q = mf.div(mf.mul(x, y), mf.add(mf.mul(x, x), mf.mul(y, y)))
     nt.assert_bits(r.take(nv), mf.asin(q), "asin")
     nt.assert_bits(r.take(nv), mf.acos(q), "acos")
+    for nm in ("min", "max", "dim"):
+        nt.assert_bits(r.take(nv), getattr(mf, "fortran_" + nm)(x, y), nm)
     At = nt.cm(A.transpose(1, 0, 2))
     Cw = np.empty((m + 2, n, 2))
     Cw[:m] = A[:, :n]

@@ -237,6 +237,24 @@ def test_atan_family_device_matches_host_build(gpu, nm):
         nt.assert_bits(getattr(mf, nm)(y, x), want, nm)
 
 
+def test_fortran_min_max_dim_follow_the_module_bodies(gpu):
+    """mindd_v / maxdd_v / dimdd_v against the reference module's bodies
+    (fsrc/multifloats.fypp:1539-1552, 1573-1583) written out in numpy: the
+    lexicographic `<`, and x - y (the oracle's DD subtraction) for dim,
+    including NaN in either operand, equal hi limbs and signed zeros."""
+    g = nt.load_golden("binary2.npz")
+    a, b = g["fmin_a"], g["fmin_b"]
+    lt = (a[:, 0] < b[:, 0]) | ((a[:, 0] == b[:, 0]) & (a[:, 1] < b[:, 1]))
+    gt = (b[:, 0] < a[:, 0]) | ((a[:, 0] == b[:, 0]) & (b[:, 1] < a[:, 1]))
+    nt.assert_bits(mf.fortran_min(a, b), np.where(lt[:, None], a, b), "min")
+    nt.assert_bits(mf.fortran_max(a, b), np.where(lt[:, None], b, a), "max")
+    nan = np.isnan(a[:, 0]) | np.isnan(b[:, 0])
+    diff = nt.or_binary("sub", a, b)
+    want = np.where((nan | gt)[:, None], diff, 0.0)
+    nt.assert_bits(mf.fortran_dim(a, b), want, "dim")
+    assert nan.sum() > 10 and (a[:, 0] == b[:, 0]).sum() > 100
+
+
 def test_transcendental2_device_entry(torch_cuda):
     """mfhip_unary with the new op codes reaches the same kernels."""
     torch = torch_cuda
