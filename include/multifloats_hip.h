@@ -347,6 +347,18 @@ MULTIFLOATS_API size_t mfhip_reduce_workspace_bytes(int64_t n);
 MULTIFLOATS_API int mfhip_reduce(int kind, const float64x2 *a, const float64x2 *b, int64_t n,
                                  int64_t renorm_interval, float64x2 *out, void *work,
                                  void *stream);
+/* mfhip_reduce without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc). No counterpart in the reference. The
+ * partials go to a 16 MiB array that the library holds in each GPU's memory
+ * until the process ends; the call itself allocates nothing. Same kernels
+ * and results (bit for bit) as mfhip_reduce. n above MFHIP_NOALLOC_MAX_N is
+ * rejected (hipErrorInvalidValue) before anything is enqueued. All calls on
+ * one GPU share its array: calls that overlap on that GPU (on different
+ * streams, or from different threads without ordering) race. */
+#define MFHIP_NOALLOC_MAX_N ((int64_t)1 << 30)
+MULTIFLOATS_API int mfhip_reduce_noalloc(int kind, const float64x2 *a, const float64x2 *b, int64_t n,
+                                         int64_t renorm_interval, float64x2 *out, void *stream);
 
 /* Device forms of sumdd_mask (work as for mfhip_reduce), cdotdd (work of
  * mfhip_cdot_workspace_bytes(n) bytes; *out is one complex64x2) and the

@@ -39,6 +39,7 @@ MFHIP_SQRT, MFHIP_EXP, MFHIP_LOG, MFHIP_SIN, MFHIP_ERF, MFHIP_NEG = range(6)
 (MFHIP_ADD, MFHIP_SUB, MFHIP_MUL, MFHIP_DIV, MFHIP_POW, MFHIP_HYPOT, MFHIP_FDIM, MFHIP_FMIN, MFHIP_FMAX,
  MFHIP_COPYSIGN, MFHIP_ATAN2, MFHIP_ATAN2PI, MFHIP_MINF, MFHIP_MAXF, MFHIP_DIMF) = range(15)
 MFHIP_DOT, MFHIP_SUM, MFHIP_NORM2, MFHIP_PRODUCT, MFHIP_MAXVAL, MFHIP_MINVAL = range(6)
+NOALLOC_MAX_N = 1 << 30  # MFHIP_NOALLOC_MAX_N
 
 
 class float64x2(ctypes.Structure):
@@ -114,6 +115,7 @@ def lib():
         L.mfhip_reduce_workspace_bytes.argtypes = [I64]
         L.mfhip_reduce_workspace_bytes.restype = ctypes.c_size_t
         L.mfhip_reduce.argtypes = [I, P, P, I64, I64, P, P, VP]
+        L.mfhip_reduce_noalloc.argtypes = [I, P, P, I64, I64, P, VP]
         L.matmuldd_mm_ozaki.argtypes = [P, P, P, I64, I64, I64, I]
         L.matmuldd_mm_ozaki.restype = None
         L.mfhip_ozaki_workspace_bytes.argtypes = [I64, I64, I64, I]
@@ -596,6 +598,12 @@ def dev_binary(op, a, b, c, n, stream=0):
 
 def dev_reduce(kind, a, b, n, out, work, renorm_interval=DD_FMA_RENORM_INTERVAL, stream=0):
     _dev(lib().mfhip_reduce(kind, a, b, n, renorm_interval, out, work, stream))
+
+
+def dev_reduce_noalloc(kind, a, b, n, out, renorm_interval=DD_FMA_RENORM_INTERVAL, stream=0):
+    """dev_reduce without a workspace: the library's static device scratch
+    (n <= NOALLOC_MAX_N; calls that overlap on one GPU race)."""
+    _dev(lib().mfhip_reduce_noalloc(kind, a, b, n, renorm_interval, out, stream))
 
 
 def reduce_workspace_bytes(n):

@@ -10,6 +10,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
+
 
 #include "../../include/multifloats_hip.h"
 #include "dd_core.h"
@@ -18,7 +20,37 @@
 using multifloats::complex64x2;
 using multifloats::float64x2;
 
+// Device scratch of mfhip_reduce_noalloc: the fold partials of one reduction,
+// held by each GPU the library runs on until the process ends. Its name is a
+// host-side stand-in: host code uses only the address that
+// hipGetSymbolAddress returns for the current device.
+constexpr size_t kNoAllocScratchBytes = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 namespace {
+
+// Per-device address of `scratch`, looked up once per device (with that
+// device current) on the first mfhip_reduce_noalloc call there.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+hipError_t noalloc_scratch(void **out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
 
 struct Staging {
   int device = -1;
@@ -648,6 +680,18 @@ int mfhip_reduce(int kind, const float64x2 *a, const float64x2 *b, int64_t n, in
                  float64x2 *out, void *work, void *stream) {
   CHECK_ALIGN(a, b, out, work);
   if (!work || !out) return status(hipErrorInvalidValue);
+  return status(mfhip::reduce(kind, D(a), D(b), n, ri, W(out), W(work), (hipStream_t)stream));
+}
+
+int mfhip_reduce_noalloc(int kind, const float64x2 *a, const float64x2 *b, int64_t n, int64_t ri,
+                         float64x2 *out, void *stream) {
+  CHECK_ALIGN(a, b, out);
+  if (!out) return status(hipErrorInvalidValue);
+  if (n > MFHIP_NOALLOC_MAX_N || mfhip::reduce_workspace_bytes(n) > kNoAllocScratchBytes)
+    return status(hipErrorInvalidValue);
+  void *work = nullptr;
+  hipError_t e = noalloc_scratch(&work);
+  if (e != hipSuccess) return status(e);
   return status(mfhip::reduce(kind, D(a), D(b), n, ri, W(out), W(work), (hipStream_t)stream));
 }
 
