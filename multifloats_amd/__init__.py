@@ -27,7 +27,8 @@ import numpy as np
 from . import datagen
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libmultifloats_hip.so")
+# MULTIFLOATS_HIP_LIB selects another build of the library (e.g. one with host-side sanitizers)
+LIB_PATH = os.environ.get("MULTIFLOATS_HIP_LIB") or os.path.join(_HERE, "libmultifloats_hip.so")
 
 DD_FMA_RENORM_INTERVAL = 8
 
@@ -251,6 +252,17 @@ def gemm(transa, transb, alpha, a, b, beta=0.0, c=None, m=None, n=None, k=None,
         beta = 0.0
     elif not datagen.is_colmajor(c):
         c = _mat(c)
+    # The library reads ld * (cols - 1) + rows elements of each operand; the
+    # leading dimensions come from the arrays (and are checked by the
+    # library), the column counts must be checked here.
+    okt = (b"N", b"n", b"T", b"t", b"C", b"c")
+    if m > 0 and n > 0:  # invalid option letters are left to the library's own check
+        if k > 0 and ta in okt and a.shape[1] < (k if nota else m):
+            raise ValueError("gemm: a has fewer columns than op(A) needs")
+        if k > 0 and tb in okt and b.shape[1] < (n if notb else k):
+            raise ValueError("gemm: b has fewer columns than op(B) needs")
+        if c.shape[1] < n:
+            raise ValueError("gemm: c has fewer than n columns")
     lib().wgemmdd(ta, tb, m, n, k, _f2(alpha), _ptr(a), max(a.shape[0], 1), _ptr(b), max(b.shape[0], 1),
                   _f2(beta), _ptr(c), max(c.shape[0], 1), renorm_interval)
     _check_status()
@@ -272,6 +284,12 @@ def trsm(side, uplo, transa, diag, alpha, a, b, m=None, n=None):
         b = _mat(b)
     m = b.shape[0] if m is None else m
     n = b.shape[1] if n is None else n
+    sd = _trans(side)
+    if m > 0 and n > 0:
+        if b.shape[1] < n:
+            raise ValueError("trsm: b has fewer than n columns")
+        if sd in (b"L", b"l", b"R", b"r") and a.shape[1] < (m if sd in (b"L", b"l") else n):
+            raise ValueError("trsm: a has fewer columns than the triangle needs")
     lib().wtrsmdd(_trans(side), _trans(uplo), _trans(transa), _trans(diag), m, n, _f2(alpha), _ptr(a),
                   max(a.shape[0], 1), _ptr(b), max(b.shape[0], 1))
     _check_status()

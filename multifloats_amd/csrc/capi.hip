@@ -52,11 +52,38 @@ hipError_t noalloc_scratch(void **out) {
   return hipSuccess;
 }
 
+// Per-thread device buffers and stream of the host-pointer entry points. They
+// are released when the thread exits and when the thread moves to another
+// device, with their own device made current for the frees.
 struct Staging {
   int device = -1;
   hipStream_t stream = nullptr;
   void *buf[4] = {nullptr, nullptr, nullptr, nullptr};
   size_t cap[4] = {0, 0, 0, 0};
+
+  Staging() = default;
+  Staging(const Staging &) = delete;
+  Staging &operator=(const Staging &) = delete;
+  ~Staging() { release(); }
+
+  void release() {
+    if (device < 0) return;
+    int cur = -1;
+    const bool switched = hipGetDevice(&cur) == hipSuccess && cur != device && hipSetDevice(device) == hipSuccess;
+    if (stream) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipStreamDestroy(stream);
+    }
+    for (int i = 0; i < 4; ++i)
+      if (buf[i]) (void)hipFree(buf[i]);
+    if (switched) (void)hipSetDevice(cur);
+    device = -1;
+    stream = nullptr;
+    for (int i = 0; i < 4; ++i) {
+      buf[i] = nullptr;
+      cap[i] = 0;
+    }
+  }
 };
 
 thread_local Staging tl_stage;
@@ -67,9 +94,7 @@ hipError_t stage_init() {
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;
   if (tl_stage.stream && tl_stage.device == dev) return hipSuccess;
-  // A device switch: the old buffers and stream belong to the other device and
-  // are left to it (freeing would need that device current again).
-  tl_stage = Staging();
+  tl_stage.release();  // a device switch: free the other device's set first
   tl_stage.device = dev;
   return hipStreamCreateWithFlags(&tl_stage.stream, hipStreamNonBlocking);
 }
@@ -115,6 +140,22 @@ hipError_t h2d(void *dst, const void *src, size_t bytes) {
 hipError_t d2h(void *dst, const void *src, size_t bytes) {
   if (!bytes) return hipSuccess;
   return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, tl_stage.stream);
+}
+
+// Column-major block of `rows` x `cols` elements of `esz` bytes with leading
+// dimension ld: only the first `rows` entries of each column are copied, so
+// the copy stays inside the ld (cols - 1) + rows elements a BLAS caller owns
+// and leaves the rows below `rows` untouched (other threads may be updating
+// them). Device and host blocks use the same leading dimension.
+hipError_t h2d_block(void *dst, const void *src, int64_t rows, int64_t cols, int64_t ld, size_t esz) {
+  if (rows <= 0 || cols <= 0) return hipSuccess;
+  return hipMemcpy2DAsync(dst, esz * (size_t)ld, src, esz * (size_t)ld, esz * (size_t)rows, (size_t)cols,
+                          hipMemcpyHostToDevice, tl_stage.stream);
+}
+hipError_t d2h_block(void *dst, const void *src, int64_t rows, int64_t cols, int64_t ld, size_t esz) {
+  if (rows <= 0 || cols <= 0) return hipSuccess;
+  return hipMemcpy2DAsync(dst, esz * (size_t)ld, src, esz * (size_t)ld, esz * (size_t)rows, (size_t)cols,
+                          hipMemcpyDeviceToHost, tl_stage.stream);
 }
 
 const double2 *D(const void *p) { return static_cast<const double2 *>(p); }
@@ -277,19 +318,19 @@ hipError_t host_gemm_blas(char ta, char tb, int64_t m, int64_t n, int64_t k, flo
   if (m == 0 || n == 0) return hipSuccess;
   TRY(stage_init());
   const bool nota = ta == 'N' || ta == 'n', notb = tb == 'N' || tb == 'n';
-  // elements of A, B, C the call may touch (column-major with leading dims)
-  const size_t na = (size_t)lda * (size_t)(nota ? k : m), nb = (size_t)ldb * (size_t)(notb ? n : k);
-  const size_t nc = (size_t)ldc * (size_t)n;
+  // the blocks of A, B, C the call reads (column-major with leading dims);
+  // device copies keep the caller's leading dimensions
+  const int64_t ar = nota ? m : k, ac = nota ? k : m, br = notb ? k : n, bc = notb ? n : k;
   void *da, *db, *dc;
-  TRY(stage_buf(0, 16 * na, &da));
-  TRY(stage_buf(1, 16 * nb, &db));
-  TRY(stage_buf(2, 16 * nc, &dc));
-  TRY(h2d(da, a, 16 * na));
-  TRY(h2d(db, b, 16 * nb));
-  TRY(h2d(dc, c, 16 * nc));
+  TRY(stage_buf(0, 16 * (size_t)lda * (size_t)(ac > 0 ? ac : 0), &da));
+  TRY(stage_buf(1, 16 * (size_t)ldb * (size_t)(bc > 0 ? bc : 0), &db));
+  TRY(stage_buf(2, 16 * (size_t)ldc * (size_t)n, &dc));
+  TRY(h2d_block(da, a, ar, ac, lda, 16));
+  TRY(h2d_block(db, b, br, bc, ldb, 16));
+  TRY(h2d_block(dc, c, m, n, ldc, 16));
   TRY(mfhip::gemm_blas(ta, tb, m, n, k, ddv{alpha.limbs[0], alpha.limbs[1]}, D(da), lda, D(db), ldb,
                        ddv{beta.limbs[0], beta.limbs[1]}, W(dc), ldc, ri, tl_stage.stream));
-  TRY(d2h(c, dc, 16 * nc));
+  TRY(d2h_block(c, dc, m, n, ldc, 16));
   return hipStreamSynchronize(tl_stage.stream);
 }
 
@@ -300,17 +341,16 @@ hipError_t host_trsm(char side, char uplo, char ta, char diag, int64_t m, int64_
   if (m == 0 || n == 0) return hipSuccess;
   TRY(stage_init());
   const int64_t s = (side == 'L' || side == 'l') ? m : n;
-  const size_t na = (size_t)lda * (size_t)s, nb = (size_t)ldb * (size_t)n;
   const size_t wb = mfhip::trsm_workspace_bytes(side, m, n);
   void *da, *db, *dw;
-  TRY(stage_buf(0, 16 * na, &da));
-  TRY(stage_buf(1, 16 * nb, &db));
+  TRY(stage_buf(0, 16 * (size_t)lda * (size_t)s, &da));
+  TRY(stage_buf(1, 16 * (size_t)ldb * (size_t)n, &db));
   TRY(stage_buf(3, wb, &dw));
-  TRY(h2d(da, a, 16 * na));
-  TRY(h2d(db, b, 16 * nb));
+  TRY(h2d_block(da, a, s, s, lda, 16));  // the s x s triangle's block
+  TRY(h2d_block(db, b, m, n, ldb, 16));
   TRY(mfhip::trsm(side, uplo, ta, diag, m, n, ddv{alpha.limbs[0], alpha.limbs[1]}, D(da), lda, W(db), ldb, dw, wb,
                   tl_stage.stream));
-  TRY(d2h(b, db, 16 * nb));
+  TRY(d2h_block(b, db, m, n, ldb, 16));
   return hipStreamSynchronize(tl_stage.stream);
 }
 

@@ -156,9 +156,9 @@ unsigned grid_for(int64_t n, int U, int max_blocks) {
   return (unsigned)g;
 }
 
-// Workgroups for one lane per element; beyond the grid limit the
-// grid-stride form takes over.
-constexpr int64_t kMaxOneShot = (int64_t)kThreads * 0x7fffffff;
+// Elements for one lane per element: a launch holds fewer than 2^32
+// work-items in total; beyond that the grid-stride form takes over.
+constexpr int64_t kMaxOneShot = (int64_t)(0xffffffffu / kThreads) * kThreads;
 
 template <int OP>
 hipError_t run_unary(const double2 *a, double2 *c, int64_t n, hipStream_t s) {

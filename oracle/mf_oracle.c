@@ -18,6 +18,7 @@
  *       (integer sums exact, Pythagorean triples), and planted-mistake tests.
  *   or_exact_*: brute force against Python integers/fractions on small inputs.
  */
+#include <limits.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -520,11 +521,17 @@ static dd o_estrin14(dd x, const double *h, const double *l) {
   return o_add(o_add(p03, o_mul(p47, x4)), o_mul(o_add(p811, o_mul(p1214, x4)), x8));
 }
 
+/* (int)x as the reference's static_cast<int> behaves on x86-64, without the
+   undefined behaviour of the C cast: NaN and values outside int's range give
+   INT_MIN (cvttsd2si's "integer indefinite"). Only NaN arguments reach that
+   case here, and their results are NaN whatever the index. */
+static int o_toint(double x) { return (x == x && fabs(x) < 2147483648.0) ? (int)x : INT_MIN; }
+
 /* src/float64x2/exp_log.inc:26-72 (exp2_from_reduced) */
 static dd o_exp2_reduced(double n_d, double rh, double rl) {
   if (n_d > 1100.0) return mkdd(INFINITY, 0.0);
   if (n_d < -1100.0) return mkdd(0.0, 0.0);
-  int k = (int)nearbyint(rh * 256.0);
+  int k = o_toint(nearbyint(rh * 256.0));
   if (k > 128) k = 128;
   if (k < -128) k = -128;
   double sh = rh - (double)k / 256.0, sl = rl;
@@ -534,7 +541,7 @@ static dd o_exp2_reduced(double n_d, double rh, double rl) {
   for (int j = 8; j >= 1; --j) q = o_add(o_mul(q, s), cdd(mf_exp2c_hi, mf_exp2c_lo, j));
   dd pow2s = o_add(mkdd(1.0, 0.0), o_mul(s, q));
   dd mant = o_mul(cdd(mf_exp2t_hi, mf_exp2t_lo, k + 128), pow2s);
-  int n = (int)n_d;
+  int n = o_toint(n_d);
   return mkdd(ldexp(mant.hi, n), ldexp(mant.lo, n));
 }
 /* exp_log.inc:237-277 (exp) */

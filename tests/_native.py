@@ -12,7 +12,10 @@ from multifloats_amd import datagen
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLDEN = os.path.join(ROOT, "tests", "golden")
-ORACLE = os.path.join(ROOT, "oracle", "liboracle.so")
+# MF_ORACLE_LIB / MF_HOSTMATH_LIB select other builds of the same sources
+# (tests/sanitize_host.sh points them at sanitizer builds)
+ORACLE = os.environ.get("MF_ORACLE_LIB") or os.path.join(ROOT, "oracle", "liboracle.so")
+HOSTMATH = os.environ.get("MF_HOSTMATH_LIB") or os.path.join(ROOT, "tests", "native", "libhostmath.so")
 REF = os.path.join(ROOT, "oracle", "_ref", "libmfref.so")
 
 P, I64 = ctypes.c_void_p, ctypes.c_int64

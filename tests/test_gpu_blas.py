@@ -139,3 +139,74 @@ def test_device_gemm_entry_point(gpu):
         torch.cuda.synchronize()
         got = dC.cpu().numpy().transpose(1, 0, 2)
         nt.assert_bits(got, want, "device " + ta + tb)
+
+
+
+def _tight(S, rows, cols, canary):
+    """The ld (cols - 1) + rows elements of column-major storage S that a BLAS
+    call on its leading rows x cols block owns, followed by canary elements."""
+    ld = S.shape[0]
+    flat = np.ascontiguousarray(S.transpose(1, 0, 2)).reshape(-1, 2)[: ld * (cols - 1) + rows]
+    return np.concatenate([flat, np.full((canary, 2), 7.0)])
+
+
+def test_gemm_touches_only_the_callers_span(gpu):
+    """A LAPACK-style call on a sub-block C(i0:i0+m, 0:n) of a larger array
+    whose buffer ends right after the block's last element: the library may
+    read and write only the ldc (n - 1) + m elements that start at C(i0, 0).
+    Canary elements follow the buffer, rows outside the block (someone
+    else's) must keep their bits, and A and B are tight buffers too."""
+    import ctypes
+    rng = np.random.default_rng(21)
+    m, n, k, ld, canary = 13, 7, 11, 40, 64
+    i0 = ld - m
+    span = ld * (n - 1) + m
+    big = rng.standard_normal((i0 + span + canary, 2))
+    before = big.copy()
+    A, B = store(rng, m, k, m + 2), store(rng, k, n, k + 3)
+    a_buf, b_buf = _tight(A, m, k, canary), _tight(B, k, n, canary)
+    P = lambda x: x.ctypes.data_as(ctypes.c_void_p)
+    L = mf.lib()
+    L.wgemmdd(b"N", b"N", m, n, k, mf._f2(ALPHA), P(a_buf), m + 2, P(b_buf), k + 3, mf._f2(BETA),
+              P(big[i0:]), ld, 8)
+    assert L.mfhip_last_error() == 0
+    assert np.array_equal(big[i0 + span:], before[i0 + span:]), "canary behind C"
+    assert (a_buf[-canary:] == 7.0).all() and (b_buf[-canary:] == 7.0).all()
+    inside = np.zeros(big.shape[0], bool)
+    for j in range(n):
+        inside[i0 + j * ld: i0 + j * ld + m] = True
+    assert np.array_equal(big[~inside], before[~inside]), "rows outside the block"
+    C0 = np.stack([before[i0 + j * ld: i0 + j * ld + m] for j in range(n)], axis=1)
+    want = nt.cm(C0)
+    assert nt.or_gemm_blas("N", "N", m, n, k, ALPHA, A, B, BETA, want, 8) == 0
+    got = np.stack([big[i0 + j * ld: i0 + j * ld + m] for j in range(n)], axis=1)
+    nt.assert_bits(got, want, "sub-block")
+
+
+def test_trsm_touches_only_the_callers_span(gpu):
+    """wtrsmdd on a tight B buffer (ldb (n - 1) + m elements) and a tight
+    triangle: canaries stay, rows below the block keep their bits."""
+    import ctypes
+    rng = np.random.default_rng(22)
+    m, n, ld, canary = 12, 6, 30, 64
+    A = store(rng, m, m, m + 4)
+    for d in range(m):
+        A[d, d] = [4.0 + d, 0.0]  # a well-conditioned triangle
+    span = ld * (n - 1) + m
+    bbuf = rng.standard_normal((span + canary, 2))
+    before = bbuf.copy()
+    a_buf = _tight(A, m, m, canary)
+    P = lambda x: x.ctypes.data_as(ctypes.c_void_p)
+    L = mf.lib()
+    L.wtrsmdd(b"L", b"U", b"N", b"N", m, n, mf._f2(1.5), P(a_buf), m + 4, P(bbuf), ld)
+    assert L.mfhip_last_error() == 0
+    assert np.array_equal(bbuf[span:], before[span:]), "canary behind B"
+    assert (a_buf[-canary:] == 7.0).all()
+    inside = np.zeros(bbuf.shape[0], bool)
+    for j in range(n):
+        inside[j * ld: j * ld + m] = True
+    assert np.array_equal(bbuf[~inside], before[~inside]), "rows outside the block"
+    want = nt.cm(np.stack([before[j * ld: j * ld + m] for j in range(n)], axis=1))
+    assert nt.or_trsm("L", "U", "N", "N", m, n, 1.5, A, want) == 0
+    got = np.stack([bbuf[j * ld: j * ld + m] for j in range(n)], axis=1)
+    nt.assert_bits(got, want, "trsm sub-block")

@@ -215,7 +215,7 @@ def test_binary_functions_bits_vs_oracle(gpu, fn):
 def test_atan_family_device_matches_host_build(gpu, nm):
     """The device kernels give the host build's bits (that build is checked
     against 320-bit values and the reference on the CPU)."""
-    L = ctypes.CDLL(nt.os.path.join(nt.ROOT, "tests", "native", "libhostmath.so"))
+    L = ctypes.CDLL(nt.HOSTMATH)
     rng = np.random.default_rng(sum(map(ord, nm)))
     w, U, q = datagen._with_lo, rng.uniform, 60000
     g = nt.invtrig_fixture(nm)
@@ -283,7 +283,7 @@ def test_erf_accuracy_golden(gpu):
 
 def test_erf_device_matches_host_build(gpu):
     """The device compile of dd_erf returns the same bits as its host build."""
-    L = ctypes.CDLL(nt.os.path.join(nt.ROOT, "tests", "native", "libhostmath.so"))
+    L = ctypes.CDLL(nt.HOSTMATH)
     rng = np.random.default_rng(5)
     x = np.concatenate([datagen.erf_inputs(rng, 100000), datagen.config0(rng, 10000)])
     want = np.empty_like(x)
@@ -313,7 +313,7 @@ def test_erfc_family_accuracy_golden(gpu, nm):
 
 @pytest.mark.parametrize("nm", ["erfc", "erfcx"])
 def test_erfc_family_device_matches_host_build(gpu, nm):
-    L = ctypes.CDLL(nt.os.path.join(nt.ROOT, "tests", "native", "libhostmath.so"))
+    L = ctypes.CDLL(nt.HOSTMATH)
     rng = np.random.default_rng(6)
     x = np.concatenate([datagen._with_lo(rng, rng.uniform(-30, 30, 100000)), datagen.config0(rng, 10000),
                         datagen._with_lo(rng, 10.0 ** rng.uniform(-5, 5, 10000))])

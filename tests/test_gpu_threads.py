@@ -76,3 +76,25 @@ def test_concurrent_ozaki_keeps_per_thread_slice_counts(gpu):
         for i, ((ca, sa), (cb, sb)) in enumerate(zip(alone, together)):
             assert sa == sb, (rep, i, sa, sb)
             assert np.array_equal(_bits(ca), _bits(cb)), (rep, i)
+
+
+def test_short_lived_threads_release_their_staging(gpu):
+    """Every thread that calls a host-pointer entry point gets its own device
+    buffers and stream; they are freed when the thread exits. 48 short-lived
+    threads that each stage about 20 MB must leave the device's free memory
+    where it was (a leak would hold about 1 GB)."""
+    import torch
+    torch.cuda.init()
+    rng = np.random.default_rng(31)
+    m = k = n = 512
+    A = datagen.matrix(rng, m, k, 10)
+    B = datagen.matrix(rng, k, n, 10)
+    want = _bits(mf.matmul(A, B, 8))
+    torch.cuda.synchronize()
+    free0, _ = torch.cuda.mem_get_info()
+    for wave in range(6):
+        jobs = [lambda: mf.matmul(A, B, 8) for _ in range(8)]
+        for got in _run_concurrently(jobs):
+            assert np.array_equal(_bits(got), want)
+    free1, _ = torch.cuda.mem_get_info()
+    assert free0 - free1 < 256 << 20, (free0 - free1) / 2 ** 20

@@ -90,7 +90,7 @@ def test_sin_generator_plants_points_near_multiples_of_half_pi():
 
 def test_hostmath_matches_oracle_bits():
     """The kernels' DD headers built for the host agree with the oracle."""
-    L = ctypes.CDLL(os.path.join(nt.ROOT, "tests", "native", "libhostmath.so"))
+    L = ctypes.CDLL(nt.HOSTMATH)
     rng = np.random.default_rng(4)
     gens = {"exp": datagen.exp_inputs, "log": datagen.log_inputs, "sin": datagen.sin_inputs}
     for nm, gen in gens.items():
@@ -101,7 +101,7 @@ def test_hostmath_matches_oracle_bits():
 
 
 def test_hostmath_erf_accuracy():
-    L = ctypes.CDLL(os.path.join(nt.ROOT, "tests", "native", "libhostmath.so"))
+    L = ctypes.CDLL(nt.HOSTMATH)
     g = nt.load_golden("transc.npz")
     x, ex = g["erf_in"], g["erf_exact"]
     got = np.empty_like(x)
@@ -116,7 +116,7 @@ def test_hostmath_erfc_family_accuracy(nm):
     reference's erfcdd / erfcxdd on the same inputs (320-bit values), and
     below 4.5e-32 (erfc) / 6.5e-32 (erfcx) relative wherever the result is a
     normal double-double."""
-    L = ctypes.CDLL(os.path.join(nt.ROOT, "tests", "native", "libhostmath.so"))
+    L = ctypes.CDLL(nt.HOSTMATH)
     g = nt.load_golden("special2.npz")
     x = g[nm + "_in"]
     got = np.empty_like(x)
@@ -200,7 +200,7 @@ def test_grouped_tile_choice_minimises_padded_work():
 
 
 def _host(nm):
-    L = ctypes.CDLL(os.path.join(nt.ROOT, "tests", "native", "libhostmath.so"))
+    L = ctypes.CDLL(nt.HOSTMATH)
     f = getattr(L, "h_" + nm)
 
     def run(*args):
