@@ -284,17 +284,26 @@ MULTIFLOATS_API void wtrsmdd(char side, char uplo, char transa, char diag, int64
                              const float64x2 *a, int64_t lda, float64x2 *b, int64_t ldb);
 
 /* Precision form of matmuldd_mm (no counterpart in the reference): the Ozaki
- * split-precision GEMM on the INT8 matrix cores. A and B are cut into
- * `slices` int8 digits (8 bits each) per row / column scale. For m or n below
- * 512, digit pairs whose levels sum to fewer than `slices` are multiplied
- * with exact INT32 accumulation and recombined in DD (triangular form); from
- * m, n >= 512 the digit integers are multiplied exactly modulo pairwise-coprime
- * moduli (one INT8 GEMM each) and C is rebuilt by the Chinese remainder
- * theorem (CRT form). Not bit-identical to matmuldd_mm; the target is
- * |C - AB| <= 2^-100 (|A||B|) elementwise.
- * slices <= 0 (recommended) chooses the count from k and the exponent ranges
- * of A's rows and B's columns, and when more than 16 would be needed runs the
- * reference-order GEMM instead; explicit counts must lie in 7..16.
+ * split-precision GEMM on the INT8 matrix cores. A and B are scaled per row /
+ * column by a power of two and represented by `slices` base-256 digits (8
+ * bits each). The digit integers are multiplied exactly modulo pairwise-
+ * coprime moduli (one INT8 GEMM each) and C is rebuilt by the Chinese
+ * remainder theorem (CRT form); with an explicit count and m or n below 512,
+ * digit pairs whose levels sum to fewer than `slices` are multiplied instead
+ * and recombined in DD (triangular form). Not bit-identical to matmuldd_mm.
+ *
+ * slices <= 0 (recommended): every output satisfies
+ *     |C - AB|_ij <= 2^-100 (|A||B|)_ij
+ * for any finite input whose products and results stay in the normal range
+ * (2^-968 .. 2^1023), whatever the distribution of magnitudes. A certificate
+ * (a bf16 MFMA product of the scaled magnitudes, bounding (|A||B|)_ij from
+ * below, against a rigorous bound of the digit representation's error) gives
+ * each output the digits it needs; the slice count is then chosen on the
+ * device, and the outputs it does not cover (e.g. rows whose largest entries
+ * meet small entries of the column) are recomputed by a compensated dot
+ * accurate to 2^-102.6 (|A||B|)_ij. Nothing is synchronised with the host.
+ * Explicit counts (7..16) skip the certificate: the error then depends on
+ * the data (about 2^(-8 slices + 8) (|A||B|)_ij on well-spread inputs).
  * k <= 32768. Rows of A or columns of B holding Inf/NaN give NaN outputs. */
 MULTIFLOATS_API void matmuldd_mm_ozaki(const float64x2 *a, const float64x2 *b, float64x2 *c,
                                        int64_t m, int64_t k, int64_t n, int slices);
@@ -391,15 +400,25 @@ MULTIFLOATS_API int mfhip_trsm(char side, char uplo, char transa, char diag, int
                                void *work, size_t work_bytes, void *stream);
 
 /* Ozaki GEMM on device pointers; `work` is caller-owned device scratch of at
- * least mfhip_ozaki_workspace_bytes(m, k, n, slices) bytes. With slices <= 0
- * the call synchronises `stream` once to read the exponent ranges. */
+ * least mfhip_ozaki_workspace_bytes(m, k, n, slices) bytes (the size for
+ * slices <= 0 also serves every explicit count). The call only enqueues work:
+ * with slices <= 0 the slice count is chosen on the device, so the call may be
+ * captured in a HIP graph. Calls sharing a workspace must not overlap. */
 MULTIFLOATS_API size_t mfhip_ozaki_workspace_bytes(int64_t m, int64_t k, int64_t n, int slices);
 MULTIFLOATS_API int mfhip_ozaki_matmul_mm(const float64x2 *a, const float64x2 *b, float64x2 *c,
                                           int64_t m, int64_t k, int64_t n, int slices, void *work,
                                           size_t work_bytes, void *stream);
-/* Slice count used by the calling thread's last Ozaki call (0 = it ran the
- * reference-order GEMM instead), and the rule behind slices <= 0. */
+/* What the last Ozaki call on `work` (same m, k, n, slices) did: the slice
+ * count of its products and how many outputs the fix-up recomputed (0 for
+ * explicit counts). Synchronises `stream`. No counterpart in the reference. */
+MULTIFLOATS_API int mfhip_ozaki_info(const void *work, int64_t m, int64_t k, int64_t n, int slices,
+                                     int *slices_used, int64_t *fixups, void *stream);
+/* Slice count and fix-up count of the calling thread's last
+ * matmuldd_mm_ozaki call; after mfhip_ozaki_matmul_mm the count is the
+ * explicit one, or -1 for an automatic call (use mfhip_ozaki_info). The
+ * exponent-range rule with which an automatic call starts its search. */
 MULTIFLOATS_API int mfhip_ozaki_slices_used(void);
+MULTIFLOATS_API int64_t mfhip_ozaki_fixups(void);
 MULTIFLOATS_API int mfhip_ozaki_choose_slices(int64_t k, int range_a, int range_b);
 /* Products form of an Ozaki call with these sizes and slice count (7..16):
  * the number N of pairwise-coprime moduli of the CRT form (N INT8 GEMMs with

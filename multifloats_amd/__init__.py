@@ -127,6 +127,9 @@ def lib():
         L.cmatmuldd_vm.argtypes = [P, P, P, I64, I64, I64]
         L.mfhip_cmatmul_mm.argtypes = [P, P, P, I64, I64, I64, I64, VP]
         L.mfhip_ozaki_slices_used.argtypes = []
+        L.mfhip_ozaki_fixups.argtypes = []
+        L.mfhip_ozaki_fixups.restype = ctypes.c_int64
+        L.mfhip_ozaki_info.argtypes = [P, I64, I64, I64, I, ctypes.POINTER(I), ctypes.POINTER(I64), VP]
         L.mfhip_ozaki_choose_slices.argtypes = [I64, I, I]
         L.mfhip_ozaki_moduli.argtypes = [I64, I64, I64, I]
         L.mfhip_ozaki_prepare.argtypes = [P, P, I64, I64, I64, I, P, ctypes.c_size_t, VP]
@@ -341,8 +344,11 @@ def dev_cmatmul(a, b, c, m, k, n, renorm_interval=DD_FMA_RENORM_INTERVAL, stream
 def matmul_ozaki(a, b, slices=0):
     """Ozaki split-precision DD GEMM on INT8 matrix cores (matmuldd_mm_ozaki).
 
-    slices=0 chooses the slice count from the data (ozaki_slices_used() tells
-    which; 0 there means the reference-order GEMM ran instead)."""
+    slices=0 certifies every output to 2^-100 (|A||B|)_ij: the slice count is
+    chosen on the device from the data (ozaki_slices_used() tells which) and
+    outputs it does not cover are recomputed by a compensated dot
+    (ozaki_fixups() tells how many). Explicit slices (7..16) skip the
+    certificate."""
     a, b = _mat(a), _mat(b)
     m, k = a.shape[:2]
     if b.shape[0] != k:
@@ -573,7 +579,23 @@ def dev_trsm(side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb, work, work_b
 
 
 def ozaki_slices_used():
+    """Slices of the calling thread's last matmul_ozaki (host API); -1 after an
+    automatic device call (see ozaki_info)."""
     return lib().mfhip_ozaki_slices_used()
+
+
+def ozaki_fixups():
+    """Outputs the fix-up recomputed in the calling thread's last matmul_ozaki."""
+    return lib().mfhip_ozaki_fixups()
+
+
+def ozaki_info(work, m, k, n, slices=0, stream=0):
+    """(slices used, outputs recomputed by the fix-up) of the last Ozaki call
+    on the device workspace `work`; synchronises the stream."""
+    S = ctypes.c_int(0)
+    F = ctypes.c_int64(0)
+    _dev(lib().mfhip_ozaki_info(work, m, k, n, slices, ctypes.byref(S), ctypes.byref(F), stream))
+    return S.value, F.value
 
 
 def ozaki_choose_slices(k, range_a, range_b):

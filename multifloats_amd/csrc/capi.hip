@@ -355,6 +355,7 @@ hipError_t host_trsm(char side, char uplo, char ta, char diag, int64_t m, int64_
 }
 
 thread_local int tl_oz_used = 0;
+thread_local int64_t tl_oz_fix = 0;
 
 hipError_t host_cmm(const complex64x2 *a, const complex64x2 *b, complex64x2 *c, int64_t m,
                     int64_t k, int64_t n, int64_t ri) {
@@ -380,9 +381,6 @@ void nan_fill_c(complex64x2 *out, int64_t n) {
     out[i].re.limbs[0] = out[i].re.limbs[1] = out[i].im.limbs[0] = out[i].im.limbs[1] = NAN;
 }
 
-// the Ozaki workspace is always sized for the largest slice count a call may use
-int oz_ws_slices(int s) { return s <= 0 ? mfhip::ozaki_max_slices() : s; }
-
 hipError_t host_mm_ozaki(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t m,
                          int64_t k, int64_t n, int slices) {
   tl_err = 0;
@@ -391,16 +389,17 @@ hipError_t host_mm_ozaki(const float64x2 *a, const float64x2 *b, float64x2 *c, i
   TRY(stage_init());
   const size_t kk = k > 0 ? (size_t)k : 0;
   void *da, *db, *dc, *dw;
-  const size_t wb = mfhip::ozaki_workspace_bytes(m, k, n, oz_ws_slices(slices));
+  const size_t wb = mfhip::ozaki_workspace_bytes(m, k, n, slices);
   TRY(stage_buf(0, 16 * (size_t)m * kk, &da));
   TRY(stage_buf(1, 16 * kk * (size_t)n, &db));
   TRY(stage_buf(2, 16 * (size_t)m * n, &dc));
   TRY(stage_buf(3, wb, &dw));
   TRY(h2d(da, a, 16 * (size_t)m * kk));
   TRY(h2d(db, b, 16 * kk * (size_t)n));
-  TRY(mfhip::ozaki_gemm(D(da), D(db), W(dc), m, k, n, slices, dw, wb, tl_stage.stream, &tl_oz_used));
+  TRY(mfhip::ozaki_gemm(D(da), D(db), W(dc), m, k, n, slices, dw, wb, tl_stage.stream));
   TRY(d2h(c, dc, 16 * (size_t)m * n));
-  return hipStreamSynchronize(tl_stage.stream);
+  // the call is synchronous anyway: read back what the device chose
+  return mfhip::ozaki_info(dw, m, k, n, slices, &tl_oz_used, &tl_oz_fix, tl_stage.stream);
 }
 
 float64x2 nan_dd() {
@@ -782,7 +781,7 @@ int mfhip_trsm(char side, char uplo, char transa, char diag, int64_t m, int64_t 
 }
 
 size_t mfhip_ozaki_workspace_bytes(int64_t m, int64_t k, int64_t n, int slices) {
-  return mfhip::ozaki_workspace_bytes(m, k, n, oz_ws_slices(slices));
+  return mfhip::ozaki_workspace_bytes(m, k, n, slices);
 }
 int mfhip_ozaki_matmul_mm(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t m,
                           int64_t k, int64_t n, int slices, void *work, size_t work_bytes,
@@ -790,10 +789,18 @@ int mfhip_ozaki_matmul_mm(const float64x2 *a, const float64x2 *b, float64x2 *c, 
   CHECK_ALIGN(a, b, c, work);
   if (slices > mfhip::ozaki_max_slices() || (slices > 0 && slices < 7))
     return status(hipErrorInvalidValue);
-  return status(mfhip::ozaki_gemm(D(a), D(b), W(c), m, k, n, slices, work, work_bytes,
-                                  (hipStream_t)stream, &tl_oz_used));
+  tl_oz_used = slices > 0 ? slices : -1;  // automatic: chosen on the device (mfhip_ozaki_info)
+  tl_oz_fix = 0;
+  return status(mfhip::ozaki_gemm(D(a), D(b), W(c), m, k, n, slices, work, work_bytes, (hipStream_t)stream));
+}
+int mfhip_ozaki_info(const void *work, int64_t m, int64_t k, int64_t n, int slices, int *slices_used,
+                     int64_t *fixups, void *stream) {
+  CHECK_ALIGN(work);
+  if (!work) return status(hipErrorInvalidValue);
+  return status(mfhip::ozaki_info(work, m, k, n, slices, slices_used, fixups, (hipStream_t)stream));
 }
 int mfhip_ozaki_slices_used(void) { return tl_oz_used; }
+int64_t mfhip_ozaki_fixups(void) { return tl_oz_fix; }
 int mfhip_ozaki_moduli(int64_t m, int64_t k, int64_t n, int slices) {
   return mfhip::ozaki_moduli(m, k, n, slices);
 }

@@ -165,15 +165,19 @@ __global__ __launch_bounds__(kThreads) void k_col_exp(const double2 *__restrict_
 }
 
 // Initial state of the row scan: E = empty, Emin = non-finite, Mbits = 0,
-// both ranges = 0 (one launch).
+// both ranges = 0, and the certificate's row and column sums = 0 (one launch
+// over max(m, n) threads).
 __global__ void k_scan_init(int *__restrict__ E, int *__restrict__ Emin, unsigned long long *__restrict__ Mbits,
-                            int *__restrict__ range, int64_t m) {
+                            int *__restrict__ range, unsigned long long *__restrict__ rsum,
+                            unsigned long long *__restrict__ csum, int64_t m, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < m) {
     E[i] = kEmpty;
     Emin[i] = kNonFinite;
     Mbits[i] = 0ull;
+    rsum[i] = 0ull;
   }
+  if (i < n) csum[i] = 0ull;
   if (i < 2) range[i] = 0;
 }
 
@@ -482,6 +486,156 @@ struct Oz2Weights {
   float inv[MF_OZ2_MODULI];  // RN(1 / m_t)
 };
 
+// Slice counts the certificate may ask for: kMinSlices..kMaxSlices, and
+// kOverCap for an output that no count up to kMaxSlices certifies.
+constexpr int kMinSlices = 7;
+constexpr int kMaxSlices = 16;
+constexpr int kOverCap = kMaxSlices + 1;
+constexpr int kOzBins = kOverCap + 1;
+
+// Control block of one call of the CRT form, in the workspace. The plan
+// kernels write it and every later kernel reads it, so the slice count S and
+// the moduli count N can be chosen on the device from the data (no host
+// round trip; the call can be captured in a graph).
+struct OzCtl {
+  int S;        // slices of the residues in the workspace (and of the products)
+  int N;        // moduli of the products and the reconstruction
+  int redo;     // 1: the certificate raised S, the residues are formed again
+  int S_first;  // slices of the first residue pass (0: explicit slice count)
+  unsigned long long hist[kOzBins];  // candidate outputs by the slices they need
+  unsigned long long ncand;          // outputs in the candidate list
+  unsigned long long nfix;           // outputs the fix-up recomputed
+  Oz2Consts cs;
+  Oz2Weights wa, wb;  // residue weights of A (f_t = 1) and of B (f_t = y_t)
+};
+
+// Moduli count of the CRT form for each slice count, computed once on the
+// host for the call's k (the counts the workspace is sized for).
+struct OzNtab {
+  int n[kMaxSlices + 1];
+};
+
+// The exponent-range rule for the first slice count of an automatic call
+// (host and device): an empirical model of how far (|A||B|)_ij falls below
+// 2^(E_i + F_j) on iid inputs of the workloads (entries s*2^U(-e,e)*u),
+// fitted on MI355X, in bits of digit precision b = 8S:
+//  * error |err_ij| ~ 2^(E_i + F_j - b - 1) sqrt(k);
+//  * (|A||B|)_ij / 2^(E_i + F_j) below 1 by at most W = (range_a + range_b)
+//    f(k) binades, f the measured envelope (max over e = 10, 20, 30 of the
+//    worst output among 512 x 512), interpolated in log2 k;
+//  * 3 binades of margin.
+// It only sets where the search starts: the certificate of the automatic
+// call (k_oz2_gemm<kOzMag>) decides which outputs a count covers.
+__host__ __device__ inline int oz_guess_slices(int64_t k, int range_a, int range_b) {
+  if (k <= 0) return kMinSlices;
+  const double lk[9] = {4, 5, 6, 7, 8.23, 9.97, 11, 12, 13};
+  const double fr[9] = {0.42, 0.354, 0.28, 0.20, 0.12, 0.05, 0.0275, 0.004, 0.002};
+  const double x = log2((double)k);
+  double f = fr[0];
+  if (x >= lk[8]) {
+    f = fr[8];
+  } else if (x > lk[0]) {
+    for (int i = 0; i < 8; ++i)
+      if (x <= lk[i + 1]) {
+        const double t = (x - lk[i]) / (lk[i + 1] - lk[i]);
+        f = fr[i] + t * (fr[i + 1] - fr[i]);
+        break;
+      }
+  }
+  const double need = 101.0 + 0.5 * x + (double)(range_a + range_b) * f + 3.0;
+  const int S = (int)ceil(need / 8.0);
+  return S < kMinSlices ? kMinSlices : S;
+}
+
+// Balanced int8 residue of p (0 <= p < m) as a byte.
+__device__ __forceinline__ unsigned bal_byte(int p, int m) {
+  const int bal = (m == 256 ? p >= 128 : p > (m - 1) / 2) ? p - m : p;
+  return (unsigned)bal & 255u;
+}
+
+// Residue weights and reconstruction constants of S slices and N moduli into
+// ctl: thread t (< 64) fills modulus t; byte j of w[t] is f_t 256^j mod m_t
+// for j < S, zero above.
+__device__ void oz_set_plan(OzCtl *ctl, int S, int N, int t) {
+  if (t < MF_OZ2_MODULI) {
+    unsigned wa[4] = {0, 0, 0, 0}, wb[4] = {0, 0, 0, 0};
+    float inv = 0.0f;
+    if (t < N) {
+      const int mt = mf_oz2_m[t];
+      int pa = 1 % mt, pb = mf_oz2_y[(N - 1) * MF_OZ2_MODULI + t] % mt;
+      for (int j = 0; j < S; ++j) {
+        wa[j / 4] |= bal_byte(pa, mt) << (8 * (j % 4));
+        wb[j / 4] |= bal_byte(pb, mt) << (8 * (j % 4));
+        pa = pa * 256 % mt;
+        pb = pb * 256 % mt;
+      }
+      inv = 1.0f / (float)mt;
+    }
+    for (int q = 0; q < 4; ++q) {
+      ctl->wa.w[t][q] = (int)wa[q];
+      ctl->wb.w[t][q] = (int)wb[q];
+    }
+    ctl->wa.inv[t] = ctl->wb.inv[t] = inv;
+  }
+  if (t == 0) {
+    // M 2^-(16(S-1)) for the product M of the first N moduli (M rounded to a
+    // double-double, so within 2^-106 M)
+    ctl->cs.mp_hi = __builtin_ldexp(mf_oz2_mp[2 * (N - 1)], -16 * (S - 1));
+    ctl->cs.mp_lo = __builtin_ldexp(mf_oz2_mp[2 * (N - 1) + 1], -16 * (S - 1));
+  }
+}
+
+// First plan of a call (one wave): the explicit slice count, or for an
+// automatic call (S_explicit <= 0) the exponent-range rule's count capped at
+// kMaxSlices; counters cleared.
+__global__ __launch_bounds__(64) void k_oz_plan_begin(OzCtl *__restrict__ ctl, const int *__restrict__ range,
+                                                      OzNtab nt, int S_explicit, int64_t k) {
+  int S = S_explicit;
+  if (S <= 0) {
+    S = oz_guess_slices(k, range[0], range[1]);
+    if (S > kMaxSlices) S = kMaxSlices;
+  }
+  const int t = threadIdx.x;
+  if (t == 0) {
+    ctl->S = S;
+    ctl->N = nt.n[S];
+    ctl->redo = 0;
+    ctl->S_first = S_explicit > 0 ? 0 : S;
+    ctl->ncand = ctl->nfix = 0ull;
+  }
+  if (t < kOzBins) ctl->hist[t] = 0ull;
+  oz_set_plan(ctl, S, nt.n[S], t);
+}
+
+// Final plan of an automatic call (one wave), after the certificate: the
+// smallest S >= the first pass's count whose uncertified outputs (those that
+// need more slices) number at most `budget` -- the fix-up recomputes those --
+// or, when even kMaxSlices leaves more than that, the smallest S that leaves
+// no more than kMaxSlices does. A raised S sets `redo`: the residues are then
+// formed again with the new weights.
+__global__ __launch_bounds__(64) void k_oz_plan_final(OzCtl *__restrict__ ctl, OzNtab nt,
+                                                      unsigned long long budget) {
+  const int S0 = ctl->S;
+  unsigned long long over[kOzBins];  // over[s] = candidates needing more than s slices
+  unsigned long long acc = 0;
+  for (int s = kOverCap; s >= 0; --s) {
+    over[s] = acc;
+    acc += ctl->hist[s];
+  }
+  const unsigned long long target = budget > over[kMaxSlices] ? budget : over[kMaxSlices];
+  int S = S0;
+  while (S < kMaxSlices && over[S] > target) ++S;
+  const int t = threadIdx.x;
+  if (S != S0) {
+    if (t == 0) {
+      ctl->S = S;
+      ctl->N = nt.n[S];
+      ctl->redo = 1;
+    }
+    oz_set_plan(ctl, S, nt.n[S], t);
+  }
+}
+
 // Balanced residue of v (|v| <= 2^18) modulo m <= 256 as a signed byte:
 // v - RN(v / m) m, in [-(m-1)/2, (m-1)/2] for odd m and [-128, 128] for 256
 // (+128 wraps to -128, the same residue). The float quotient is exact: v / m
@@ -520,6 +674,15 @@ __device__ __forceinline__ void int_double_to_i128(double D, uint64_t &lo, uint6
   hi = h;
 }
 
+// Magnitude of a certificate operand: |h| 2^-e (0 <= x < 2 here) as a bf16
+// rounded toward zero, so it never exceeds the element's magnitude: RN to
+// float, stepped down when that rounded up, then the top 16 bits.
+__device__ __forceinline__ unsigned bf16_floor(double x) {
+  float f = (float)x;
+  if ((double)f > x) f = __uint_as_float(__float_as_uint(f) - 1u);
+  return __float_as_uint(f) >> 16;
+}
+
 // Residue planes of A (or of B with TRANS): plane t holds the 64 x 32 chunks
 // of tile row block ib (TB of them, padded to whole 256-row tiles) and K
 // block kb at ((t TB + ib) KB + kb) 2048 in the slices' swizzled layout.
@@ -536,13 +699,27 @@ __device__ __forceinline__ void int_double_to_i128(double D, uint64_t &lo, uint6
 // 256^j: X + K = sum_j (e_j + 128) 256^j is the plain base-256 expansion.
 // These are the digits of the triangular form's slicing (the same integer,
 // up to the rounding of an exact tie).
+// S, N and the weights come from the call's control block. pass 1 is the
+// second residue pass of an automatic call: it runs only when the plan raised
+// S (ctl->redo). With mag set (first pass of an automatic call) the kernel
+// also writes the certificate operand -- |hi| 2^-E as bf16 rounded down, in
+// the same chunk layout with 16 k-elements per 32-byte row (K block 2 kb + c0 /
+// 16) -- and adds an upper bound of sum_p (|hi| + |lo|) 2^-E over its chunk,
+// in units of 2^-40 rounded up, to sums[row] (integer atomics: the total does
+// not depend on their order).
 template <bool TRANS>
 __global__ __launch_bounds__(kThreads) void k_oz2_resid(const double2 *__restrict__ X, int64_t rows,
-                                                        int64_t k, const int *__restrict__ Ex, int S, int N,
-                                                        int64_t KB, int64_t TB, Oz2Weights wt,
-                                                        signed char *__restrict__ out) {
+                                                        int64_t k, const int *__restrict__ Ex, int64_t KB,
+                                                        int64_t TB, const OzCtl *__restrict__ ctl, int pass,
+                                                        signed char *__restrict__ out,
+                                                        unsigned char *__restrict__ mag,
+                                                        unsigned long long *__restrict__ sums) {
+  if (pass == 1 && !ctl->redo) return;  // uniform: one control word for the grid
   constexpr int kPad = kK + 1;
   __shared__ double2 stage[TRANS ? kT * kPad : 1];
+  __shared__ unsigned long long rsum[kThreads / 64][kT];
+  const int S = ctl->S, N = ctl->N;
+  const Oz2Weights *__restrict__ wt = TRANS ? &ctl->wb : &ctl->wa;
   const int ib = blockIdx.x, kb = blockIdx.y;
   const int tid = threadIdx.x, r = tid & 63, c0 = 8 * (tid >> 6);
   const int64_t i = (int64_t)ib * kT + r;
@@ -562,12 +739,20 @@ __global__ __launch_bounds__(kThreads) void k_oz2_resid(const double2 *__restric
   const uint64_t K0 = S >= 8 ? k80 : k80 & ((1ull << (8 * S)) - 1);
   const uint64_t K1 = S <= 8 ? 0ull : S >= 16 ? k80 : k80 & ((1ull << (8 * (S - 8))) - 1);
   int dp[8][4];
+  unsigned mg[4] = {0, 0, 0, 0};
+  unsigned long long part = 0;
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
     const int64_t p = (int64_t)kb * kK + c0 + q;
     double2 v = make_double2(0.0, 0.0);
     if (TRANS) v = stage[r * kPad + c0 + q];
     else if (i < rows && p < k) v = X[i + p * rows];
+    if (mag && live) {
+      const double ah = fabs(v.x);
+      mg[q >> 1] |= bf16_floor(__builtin_ldexp(ah, -Ei)) << (16 * (q & 1));
+      // (|hi| + |lo|) 2^(40-E) < 2^41; + 1 covers the rounding of the sum
+      part += (unsigned long long)__builtin_ceil(__builtin_ldexp(ah + fabs(v.y), 40 - Ei)) + (ah != 0.0);
+    }
     const double H = live ? __builtin_ldexp(v.x, P) : 0.0;
     const double L = live ? __builtin_ldexp(v.y, P) : 0.0;
     const double Hi = __builtin_trunc(H);
@@ -584,12 +769,23 @@ __global__ __launch_bounds__(kThreads) void k_oz2_resid(const double2 *__restric
     dp[q][2] = (int)(uint32_t)hi;
     dp[q][3] = (int)(uint32_t)(hi >> 32);
   }
+  if (mag) {
+    // 8 bf16 = one 16-byte half of a 32-byte chunk row of K block 2 kb + c0 / 16
+    unsigned char *dm = mag + ((int64_t)ib * (2 * KB) + 2 * kb + (c0 >> 4)) * kChunk + tile_off(r, 2 * (c0 & 15));
+    *(v4i *)dm = v4i{(int)mg[0], (int)mg[1], (int)mg[2], (int)mg[3]};
+    rsum[tid >> 6][r] = part;
+    __syncthreads();
+    if (tid < kT) {
+      const unsigned long long tot = rsum[0][tid] + rsum[1][tid] + rsum[2][tid] + rsum[3][tid];
+      if (tot && i < rows) atomicAdd(&sums[i], tot);
+    }
+  }
   signed char *dst0 = out + ((int64_t)ib * KB + kb) * kChunk + tile_off(r, c0);
   const int64_t plane = TB * KB * kChunk;
   for (int t = 0; t < N; ++t) {
     const int mt = mf_oz2_m[t];
-    const float inv = wt.inv[t];
-    const int w0 = wt.w[t][0], w1 = wt.w[t][1], w2 = wt.w[t][2], w3 = wt.w[t][3];
+    const float inv = wt->inv[t];
+    const int w0 = wt->w[t][0], w1 = wt->w[t][1], w2 = wt->w[t][2], w3 = wt->w[t][3];
     unsigned lo = 0, hi = 0;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
@@ -642,29 +838,105 @@ constexpr int waitcnt_vm(int n) { return (n & 15) | ((n >> 4) << 14) | (7 << 4) 
 // the copies of the stages still in flight; the copies a stage needs are
 // waited for explicitly before this barrier, and LDS needs no cache flush.
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+// The two uses of the GEMM kernel below: the N residue products of the CRT
+// form (INT8 MFMA, exact INT32 sums) and the certificate of an automatic call
+// (bf16 MFMA on the magnitude operands, FP32 sums).
+constexpr int kOzRes = 0;
+constexpr int kOzMag = 1;
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+template <int MODE>
+struct O2Mma;
+template <>
+struct O2Mma<kOzRes> {
+  typedef v16i acc_t;
+  static __device__ __forceinline__ acc_t run(v4i a, v4i b, acc_t c) {
+    return __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct O2Mma<kOzMag> {
+  typedef v16f acc_t;
+  static __device__ __forceinline__ acc_t run(v4i a, v4i b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), c, 0,
+                                                   0, 0);
+  }
+};
+
+struct Oz2GemmArgs {
+  const signed char *As, *Bs;  // operand planes (residues, or bf16 magnitudes)
+  unsigned char *R;            // digit planes (kOzRes)
+  int64_t m, n, KB, TBA, TBB;  // KB: 32-byte K blocks per chunk row, padded to kO2KS
+  int tiles_m, tiles_n;
+  OzCtl *ctl;
+  // certificate (kOzMag)
+  const int *E, *F;
+  const unsigned long long *rsum, *csum;  // sum_p (|hi| + |lo|) 2^-E in units of 2^-40, rounded up
+  unsigned long long *cand;                // candidate list: (slices << 56) | (i + j m)
+};
+
+// Lower slice-count bound of one output (i, j) from its certificate sum G:
+// with E = E_i, F = F_j, the CRT form's representation error is at most
+// 2^(E + F + 1 - 8S) (R_A + R_B) (1 + 2^-20), R_A = sum_p |a_ip| 2^-E and
+// R_B = sum_p |b_pj| 2^-F (each element within 2^(E + 1 - 8S) of its S-slice
+// integer; the 2^-20 covers the second-order term and the reconstruction's
+// limb truncation), and its reconstruction adds at most 2^-103 (|A||B|)_ij.
+// (|A||B|)_ij >= 2^(E + F) G (1 - 2^-6): G sums products of bf16 values
+// rounded down, in FP32 with at most k roundings of non-negative terms
+// (k <= 2^15). So 2^-100 (|A||B|)_ij holds once
+// 8S >= 101.2 + log2((R_A + R_B) / (G (1 - 2^-6))); 0.1 more covers the FP32
+// logarithms. Returns the smallest such S in kMinSlices..kMaxSlices, or
+// kOverCap.
+__device__ __forceinline__ int oz_cert_slices(float G, double ra, double rb) {
+  const float L = G * (1.0f - 0x1p-6f);
+  if (!(L > 0.0f)) return kOverCap;
+  const float bits = 101.3f + __builtin_log2f((float)(ra + rb)) - __builtin_log2f(L);
+  const int S = (int)__builtin_ceilf(bits * 0.125f);
+  return S < kMinSlices ? kMinSlices : (S > kMaxSlices ? kOverCap : S);
+}
+
+// Exclusive prefix sum over the 64 lanes of a wave, and the total.
+__device__ __forceinline__ unsigned wave_excl_scan(unsigned v, int lane, unsigned &total) {
+  unsigned x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const unsigned y = __shfl_up(x, off);
+    if (lane >= off) x += y;
+  }
+  total = __shfl(x, 63);
+  return x - v;
+}
+
+template <int MODE>
 __global__ __launch_bounds__(kMainThreads) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_oz2_gemm(
-    const signed char *__restrict__ As, const signed char *__restrict__ Bs, unsigned char *__restrict__ R,
-    int64_t m, int64_t n, int64_t KB, int64_t TBA, int64_t TBB, int tiles_m, int tiles_n) {
+    Oz2GemmArgs g) {
+  typedef typename O2Mma<MODE>::acc_t acc_t;
   extern __shared__ __attribute__((aligned(16))) signed char lds[];
+  const int t = blockIdx.y;
+  if (MODE == kOzRes && t >= g.ctl->N) return;  // planes beyond the call's moduli count
+  __shared__ unsigned hist_l[kOzBins];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int t = blockIdx.y;
+  if (MODE == kOzMag && tid < kOzBins) hist_l[tid] = 0u;  // published by the first barrier below
   int ib, jb;
-  xcd_tile(blockIdx.x, tiles_m, tiles_n, ib, jb);
+  xcd_tile(blockIdx.x, g.tiles_m, g.tiles_n, ib, jb);
   const int wi = wave & 1, wj = wave >> 1;
   const int r = lane & 31, half = lane >> 5;
-  const signed char *ga = As + ((int64_t)t * TBA + 4 * ib) * KB * kChunk;
-  const signed char *gb = Bs + ((int64_t)t * TBB + 4 * jb) * KB * kChunk;
+  const int64_t KB = g.KB;
+  const signed char *ga = g.As + ((int64_t)t * g.TBA + 4 * ib) * KB * kChunk;
+  const signed char *gb = g.Bs + ((int64_t)t * g.TBB + 4 * jb) * KB * kChunk;
   // The output plane's address is formed (and its scalar argument load
   // completed) before the main loop: a scalar load still in flight there
   // would force every LDS wait in the loop to wait for all reads.
-  unsigned char *gr = R + (int64_t)t * m * n;
+  unsigned char *gr = g.R + (int64_t)t * g.m * g.n;
   asm volatile("" : "+s"(gr));
-  v16i acc[4][4];
+  acc_t acc[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[a][b][e] = 0;
   // copy v = wave + 4 j of stage kbs: K block q = v / 16, operand (v / 8) & 1,
   // chunk (v / 2) & 3, half KiB v & 1
   auto issue = [&](int64_t kbs, int j) {
@@ -712,7 +984,7 @@ __global__ __launch_bounds__(kMainThreads) __attribute__((amdgpu_waves_per_eu(1,
     for (int a = 0; a < 4; ++a)
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
-        acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[0][a], fb[0][b], acc[a][b], 0, 0, 0);
+        acc[a][b] = O2Mma<MODE>::run(fa[0][a], fb[0][b], acc[a][b]);
         constexpr int every = 16 / kO2NL;
         if (more && slot % every == 0) issue(s + kO2Bufs - 1, slot / every);
         ++slot;
@@ -724,7 +996,7 @@ __global__ __launch_bounds__(kMainThreads) __attribute__((amdgpu_waves_per_eu(1,
     lds_barrier();
     // One MFMA of block 1 goes first: the compiler's wait for block 1's
     // fragments then lands before the new reads instead of after them.
-    acc[0][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[1][0], fb[1][0], acc[0][0], 0, 0, 0);
+    acc[0][0] = O2Mma<MODE>::run(fa[1][0], fb[1][0], acc[0][0]);
     __builtin_amdgcn_sched_barrier(0);
     const signed char *sn = lds + (int)((s + 1) % kO2Bufs) * kO2Stage;
 #pragma unroll
@@ -736,7 +1008,7 @@ __global__ __launch_bounds__(kMainThreads) __attribute__((amdgpu_waves_per_eu(1,
     for (int a = 0; a < 4; ++a)
 #pragma unroll
       for (int b = 0; b < 4; ++b)
-        if (a + b > 0) acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[1][a], fb[1][b], acc[a][b], 0, 0, 0);
+        if (a + b > 0) acc[a][b] = O2Mma<MODE>::run(fa[1][a], fb[1][b], acc[a][b]);
     // The next stage's fragments leave the loop through an empty register
     // asm: otherwise the compiler merges these reads with the ones before the
     // loop into one read at the top of the loop, behind the MFMAs again.
@@ -746,37 +1018,94 @@ __global__ __launch_bounds__(kMainThreads) __attribute__((amdgpu_waves_per_eu(1,
 #pragma unroll
     for (int b = 0; b < 4; ++b) asm("" : "+v"(fb[0][b]));
   }
-  // epilogue: row 8(e/4) + 4 half + e%4 and column r of each 32 x 32 block.
-  // B's residues carry the factor y_t, so v == c_t y_t (mod m_t). q = RN(v / m)
-  // is exact (v / m is an exact binary fraction for m = 256 and at least
-  // 1/(2m) from a half-integer otherwise), so the fma gives the balanced
-  // digit v - q m in [-m/2, m/2] exactly; it is stored as a signed byte
-  // (+128 for m = 256 wraps to -128, the same residue).
-  const double md = (double)mf_oz2_m[t], inv = 1.0 / md;
+  // Accumulator e of block (a, b) is row 8(e/4) + 4 half + e%4 and column r of
+  // that 32 x 32 block.
+  if constexpr (MODE == kOzRes) {
+    // B's residues carry the factor y_t, so v == c_t y_t (mod m_t). q = RN(v / m)
+    // is exact (v / m is an exact binary fraction for m = 256 and at least
+    // 1/(2m) from a half-integer otherwise), so the fma gives the balanced
+    // digit v - q m in [-m/2, m/2] exactly; it is stored as a signed byte
+    // (+128 for m = 256 wraps to -128, the same residue).
+    const int64_t m = g.m, n = g.n;
+    const double md = (double)mf_oz2_m[t], inv = 1.0 / md;
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int64_t col = (int64_t)kO2Tile * jb + 128 * wj + 32 * b + r;
+        if (col >= n) continue;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int64_t row0 = (int64_t)kO2Tile * ib + 128 * wi + 32 * a + 8 * q + 4 * half;
+          unsigned word = 0;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const double v = (double)acc[a][b][4 * q + e];
+            const double u = __builtin_fma(-__builtin_rint(v * inv), md, v);
+            word |= ((unsigned)(int)u & 255u) << (8 * e);
+          }
+          if (row0 + 3 < m && (m & 3) == 0) {
+            *(unsigned *)(gr + col * m + row0) = word;
+          } else {
+            for (int e = 0; e < 4; ++e)
+              if (row0 + e < m) gr[col * m + row0 + e] = (unsigned char)(word >> (8 * e));
+          }
+        }
+      }
+  } else {
+    // Certificate: the slices each output needs (oz_cert_slices). Outputs
+    // that need more than the first residue pass's count are appended to the
+    // candidate list (one atomic per wave and 32-column block) and counted by
+    // slices; rows and columns that are empty (exact zero outputs) or hold
+    // Inf/NaN (NaN outputs) need none.
+    const int64_t m = g.m, n = g.n;
+    const int Sf = g.ctl->S;
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
       const int64_t col = (int64_t)kO2Tile * jb + 128 * wj + 32 * b + r;
-      if (col >= n) continue;
+      const int Fj = col < n ? g.F[col] : kEmpty;
+      const bool cl = Fj != kEmpty && Fj != kNonFinite;
+      const double rb = cl ? (double)g.csum[col] * 0x1p-40 : 0.0;
 #pragma unroll
-      for (int g = 0; g < 4; ++g) {
-        const int64_t row0 = (int64_t)kO2Tile * ib + 128 * wi + 32 * a + 8 * g + 4 * half;
-        unsigned word = 0;
+      for (int a = 0; a < 4; ++a) {
+        unsigned need = 0;  // 5 bits per output of this 32 x 32 block: 16 outputs per lane
+        unsigned long long packed[2] = {0ull, 0ull};
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const double v = (double)acc[a][b][4 * g + e];
-          const double u = __builtin_fma(-__builtin_rint(v * inv), md, v);
-          word |= ((unsigned)(int)u & 255u) << (8 * e);
+        for (int e = 0; e < 16; ++e) {
+          const int64_t row = (int64_t)kO2Tile * ib + 128 * wi + 32 * a + 8 * (e >> 2) + 4 * half + (e & 3);
+          int sl = 0;
+          if (cl && row < m) {
+            const int Ei = g.E[row];
+            if (Ei != kEmpty && Ei != kNonFinite)
+              sl = oz_cert_slices(acc[a][b][e], (double)g.rsum[row] * 0x1p-40, rb);
+          }
+          if (sl > Sf) {
+            ++need;
+            packed[e >> 3] |= (unsigned long long)sl << (8 * (e & 7));
+          }
         }
-        if (row0 + 3 < m && (m & 3) == 0) {
-          *(unsigned *)(gr + col * m + row0) = word;
-        } else {
-          for (int e = 0; e < 4; ++e)
-            if (row0 + e < m) gr[col * m + row0 + e] = (unsigned char)(word >> (8 * e));
+        if (__ballot(need != 0)) {  // uniform: the candidates of this block
+          unsigned total;
+          const unsigned before = wave_excl_scan(need, lane, total);
+          unsigned long long base = 0;
+          if (lane == 0) base = atomicAdd(&g.ctl->ncand, (unsigned long long)total);
+          base = __shfl(base, 0);
+          unsigned long long slot = base + before;
+#pragma unroll
+          for (int e = 0; e < 16; ++e) {
+            const int sl = (int)((packed[e >> 3] >> (8 * (e & 7))) & 255ull);
+            if (sl) {
+              const int64_t row = (int64_t)kO2Tile * ib + 128 * wi + 32 * a + 8 * (e >> 2) + 4 * half + (e & 3);
+              g.cand[slot++] = ((unsigned long long)sl << 56) | (unsigned long long)(row + col * m);
+              atomicAdd(&hist_l[sl], 1u);
+            }
+          }
         }
       }
     }
+    __syncthreads();
+    if (tid < kOzBins && hist_l[tid]) atomicAdd(&g.ctl->hist[tid], (unsigned long long)hist_l[tid]);
+  }
 }
 
 // CRT: x / M = frac(sum_t u_t / m_t) with the balanced digits
@@ -794,10 +1123,12 @@ __global__ __launch_bounds__(kMainThreads) __attribute__((amdgpu_waves_per_eu(1,
 // VEC-byte load per residue plane, eight planes in flight.
 constexpr int kO2Limbs = 5;
 template <int VEC>
-__global__ __launch_bounds__(kThreads) void k_oz2_crt(const unsigned char *__restrict__ R, int N, int64_t m,
-                                                      int64_t n, const int *__restrict__ E,
-                                                      const int *__restrict__ F, Oz2Consts cs,
+__global__ __launch_bounds__(kThreads) void k_oz2_crt(const unsigned char *__restrict__ R,
+                                                      const OzCtl *__restrict__ ctl, int64_t m, int64_t n,
+                                                      const int *__restrict__ E, const int *__restrict__ F,
                                                       double2 *__restrict__ C) {
+  const int N = ctl->N;
+  const Oz2Consts cs = ctl->cs;
   const int64_t mn = m * n, groups = mn / VEC;  // VEC divides m n
   for (int64_t gi = (int64_t)blockIdx.x * kThreads + threadIdx.x; gi < groups;
        gi += (int64_t)gridDim.x * kThreads) {
@@ -888,19 +1219,106 @@ __global__ __launch_bounds__(kThreads) void k_oz2_crt(const unsigned char *__res
   }
 }
 
+// Adds x to the value s1 + s2 + s3 kept as a cascade: (s1, e) = TwoSum(s1, x),
+// (s2, e2) = TwoSum(s2, e), s3 += e2. Only the last addition rounds; after n
+// additions of terms with sum |x| = T, |s2| <= n u T and the rounding errors
+// of s3 total at most (n u)^3 T (u = 2^-53).
+__device__ __forceinline__ void casc3(double &s1, double &s2, double &s3, double x) {
+  double e, e2;
+  two_sum(s1, x, s1, e);
+  two_sum(s2, e, s2, e2);
+  s3 += e2;
+}
+
+// 64-bit value of lane 0, uniform for the compiler (two readfirstlanes).
+__device__ __forceinline__ unsigned long long rfl64(unsigned long long v) {
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+// Outputs of an automatic call that the slice count in force does not
+// certify (candidates needing more slices than ctl->S), recomputed one per
+// wave: lane l takes p = l, l + 64, ...; each product a b of double-doubles is
+// ph + q with ph = RN(ah bh), q = RN(fma(ah, bh, -ph) + fma(ah, bl, RN(al bh)))
+// (al bl dropped), within 7 2^-106 |a||b| of the product; the ph go through a
+// three-term cascade and the q through a two-term one, the lanes' cascades
+// are merged by a butterfly, and s1 + s2 + s3 is rounded to a double-double.
+// Error: 7 2^-106 (|A||B|)_ij from the products, (8192 u)^3 from the
+// cascades, 2^-104.9 from the final rounding: below 2^-102.6 (|A||B|)_ij,
+// with no dependence on how the row's and column's magnitudes are spread.
+// Wave w of the grid takes list entries w, w + W, w + 2W, ... (W waves): the
+// loop index, the bound and each entry are wave-uniform scalars, so the loop
+// keeps one level of uniform control flow (a shared work counter whose ticket
+// is broadcast from one lane makes the loop test divergent for the compiler,
+// which then splits it into nested loops in which lanes can run on a stale
+// ticket and never leave). Entries cost the same k-long dot, so the static
+// order balances.
+__global__ __launch_bounds__(kThreads) void k_oz_fixup(const double2 *__restrict__ A, const double2 *__restrict__ B,
+                                                       double2 *__restrict__ C, int64_t m, int64_t k,
+                                                       const unsigned long long *__restrict__ cand,
+                                                       OzCtl *__restrict__ ctl) {
+  const unsigned long long nc = rfl64(ctl->ncand);
+  const int S = __builtin_amdgcn_readfirstlane(ctl->S);
+  const int lane = threadIdx.x & 63;
+  const unsigned long long nw = (unsigned long long)gridDim.x * (kThreads / 64);
+  const unsigned long long w0 = __builtin_amdgcn_readfirstlane(blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6));
+  for (unsigned long long idx = w0; idx < nc; idx += nw) {
+    const unsigned long long ent = rfl64(cand[idx]);
+    if ((int)(ent >> 56) <= S) continue;  // certified by the slice count in force
+    const int64_t o = (int64_t)(ent & ((1ull << 56) - 1));
+    const int64_t j = o / m, i = o - j * m;
+    double s1 = 0.0, s2 = 0.0, s3 = 0.0, t1 = 0.0, t2 = 0.0;
+    const double2 *arow = A + i, *bcol = B + j * k;
+    for (int64_t p = lane; p < k; p += 64) {
+      const double2 a = arow[p * m], b = bcol[p];
+      const double ph = a.x * b.x;
+      const double pe = __builtin_fma(a.x, b.x, -ph);
+      const double q = pe + __builtin_fma(a.x, b.y, a.y * b.x);
+      casc3(s1, s2, s3, ph);
+      double e;
+      two_sum(t1, q, t1, e);
+      t2 += e;
+    }
+    casc3(s1, s2, s3, t1);
+    casc3(s1, s2, s3, t2);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const double o1 = __shfl_xor(s1, off), o2 = __shfl_xor(s2, off), o3 = __shfl_xor(s3, off);
+      casc3(s1, s2, s3, o1);
+      casc3(s1, s2, s3, o2);
+      casc3(s1, s2, s3, o3);
+    }
+    if (lane == 0) {
+      double h, l1, l2;
+      two_sum(s1, s2, h, l1);
+      two_sum(h, s3, h, l2);
+      double hi, lo;
+      two_sum(h, l1 + l2, hi, lo);  // h may have cancelled below l1 + l2
+      C[o] = make_double2(hi, lo);
+      atomicAdd(&ctl->nfix, 1ull);
+    }
+  }
+}
+
+// C = 0 (m n elements) by a fill kernel, which graph capture records safely.
+__global__ void k_zero2(double2 *__restrict__ C, int64_t count) {
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < count; x += (int64_t)gridDim.x * blockDim.x)
+    C[x] = make_double2(0.0, 0.0);
+}
+
 }  // namespace
 
 namespace mfhip {
 
-int ozaki_max_slices() { return 16; }
-constexpr int kMinSlices = 7;
+int ozaki_max_slices() { return kMaxSlices; }
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 namespace {
 // CRT form: moduli needed for S digits and inner dimension k, i.e. the
 // smallest N with M = prod_{t<N} m_t > 2 k 2^(16S-2) (with margin); 0 if the
-// 40 moduli do not reach it.
+// 40 moduli do not reach it (they reach it for every S <= 16, k <= 32768).
 int oz2_count(int S, int64_t k) {
   const double need = 16.0 * S - 1.0 + log2((double)(k > 0 ? k : 1)) + 0.05;
   double have = 0.0;
@@ -910,8 +1328,15 @@ int oz2_count(int S, int64_t k) {
   }
   return 0;
 }
-// The CRT form pays off once the 256 x 256 tiles are mostly full; below
-// that, and wherever N moduli are not enough, the triangular form runs.
+OzNtab oz2_ntab(int64_t k) {
+  OzNtab t{};
+  for (int S = kMinSlices; S <= kMaxSlices; ++S) t.n[S] = oz2_count(S, k);
+  return t;
+}
+// With an explicit slice count the CRT form runs once the 256 x 256 tiles
+// are mostly full and the triangular form below that; an automatic call
+// always runs the CRT form (its slice count is chosen on the device, and the
+// triangular kernel is specialised on it).
 bool use_oz2(int64_t m, int64_t n, int64_t k, int S) {
   return m >= 512 && n >= 512 && oz2_count(S, k) > 0;
 }
@@ -928,46 +1353,42 @@ Oz2Dims oz2_dims(int64_t m, int64_t k, int64_t n, int S) {
   d.KBp = (KB + kO2KS - 1) / kO2KS * kO2KS;
   return d;
 }
-size_t oz2_bytes(int64_t m, int64_t k, int64_t n, int S) {
+// residue planes of A and B, digit planes; with `cert` (automatic calls) also
+// the bf16 magnitude operands (16 k per chunk row: 2 KBp K blocks) and the
+// candidate list (one 8-byte entry per output at most)
+size_t oz2_bytes(int64_t m, int64_t k, int64_t n, int S, bool cert) {
   const Oz2Dims d = oz2_dims(m, k, n, S);
-  return align256((size_t)d.N * d.TBA * d.KBp * kChunk) + align256((size_t)d.N * d.TBB * d.KBp * kChunk) +
-         align256((size_t)d.N * m * n);
+  size_t b = align256((size_t)d.N * d.TBA * d.KBp * kChunk) + align256((size_t)d.N * d.TBB * d.KBp * kChunk) +
+             align256((size_t)d.N * m * n);
+  if (cert)
+    b += align256((size_t)d.TBA * 2 * d.KBp * kChunk) + align256((size_t)d.TBB * 2 * d.KBp * kChunk) +
+         align256(sizeof(unsigned long long) * (size_t)m * n);
+  return b;
 }
-// Residue-pass weights for S slices: f_t 256^j mod m_t (j < S), balanced,
-// with f_t = 1 (A) or the CRT inverse y_t (B, with_y).
-Oz2Weights oz2_weights(int N, int S, bool with_y) {
-  Oz2Weights c{};
-  for (int t = 0; t < N; ++t) {
-    const int mt = mf_oz2_m_host[t];
-    int pw = with_y ? mf_oz2_y_host[(N - 1) * MF_OZ2_MODULI + t] % mt : 1 % mt;
-    for (int j = 0; j < S; ++j, pw = pw * 256 % mt) {
-      const int bal = (mt == 256 ? pw >= 128 : pw > (mt - 1) / 2) ? pw - mt : pw;
-      c.w[t][j / 4] = (int)((unsigned)c.w[t][j / 4] | (((unsigned)bal & 255u) << (8 * (j % 4))));
-    }
-    c.inv[t] = 1.0f / (float)mt;
-  }
-  return c;
-}
-// M 2^-(16(S-1)) for the product M of the first N moduli (M rounded to a
-// double-double, so within 2^-106 M), from the generated tables.
-Oz2Consts oz2_consts(int N, int S) {
-  Oz2Consts c{};
-  c.mp_hi = ldexp(mf_oz2_mp_host[2 * (N - 1)], -16 * (S - 1));
-  c.mp_lo = ldexp(mf_oz2_mp_host[2 * (N - 1) + 1], -16 * (S - 1));
-  return c;
+// Bytes of the parts that do not depend on the form: exponent arrays,
+// ranges, control block, certificate sums.
+size_t oz_head_bytes(int64_t m, int64_t n) {
+  return 2 * align256(sizeof(int) * (size_t)m) + align256(sizeof(int) * (size_t)n) + 256 +
+         align256(sizeof(unsigned long long) * (size_t)m) + align256(sizeof(OzCtl)) +
+         align256(sizeof(unsigned long long) * (size_t)m) + align256(sizeof(unsigned long long) * (size_t)n);
 }
 }  // namespace
 
 int ozaki_moduli(int64_t m, int64_t k, int64_t n, int S) {
-  return (S >= kMinSlices && S <= ozaki_max_slices() && use_oz2(m, n, k, S)) ? oz2_count(S, k) : 0;
+  return (S >= kMinSlices && S <= kMaxSlices && use_oz2(m, n, k, S)) ? oz2_count(S, k) : 0;
 }
 
 size_t ozaki_workspace_bytes(int64_t m, int64_t k, int64_t n, int S) {
+  if (m < 0) m = 0;
+  if (n < 0) n = 0;
+  // automatic calls: the CRT form at kMaxSlices with the certificate; also
+  // enough for any explicit count, so one workspace serves both
+  const bool aut = S <= 0;
+  if (aut) S = kMaxSlices;
   const int64_t tm = (m + kT - 1) / kT, tn = (n + kT - 1) / kT, KB = (k + kK - 1) / kK;
   const size_t one = align256((size_t)tm * KB * S * kChunk) + align256((size_t)tn * KB * S * kChunk);
-  const size_t two = oz2_count(S, k) ? oz2_bytes(m, k, n, S) : 0;
-  return (one > two ? one : two) + 2 * align256(sizeof(int) * (size_t)m) + align256(sizeof(int) * (size_t)n) + 256 +
-         align256(sizeof(unsigned long long) * (size_t)m);
+  const size_t two = oz2_count(S, k) ? oz2_bytes(m, k, n, S, aut) : 0;
+  return oz_head_bytes(m, n) + (one > two ? one : two);
 }
 
 namespace {
@@ -975,6 +1396,8 @@ struct OzWork {
   signed char *As, *Bs;
   int *E, *Emin, *F, *range;  // range[0] rows of A, range[1] columns of B
   unsigned long long *Mbits;  // largest |hi| per row of A (bit pattern)
+  OzCtl *ctl;
+  unsigned long long *rsum, *csum;  // certificate: sum |a| 2^-E per row of A / column of B
   int64_t tm, tn, KB;
 };
 OzWork carve(void *work, int64_t m, int64_t k, int64_t n, int S) {
@@ -982,7 +1405,7 @@ OzWork carve(void *work, int64_t m, int64_t k, int64_t n, int S) {
   w.tm = (m + kT - 1) / kT;
   w.tn = (n + kT - 1) / kT;
   w.KB = (k + kK - 1) / kK;
-  // exponent arrays first, at offsets independent of S, then the slices
+  // the parts every form uses first, at offsets independent of S
   char *p = (char *)work;
   w.E = (int *)p;
   p += align256(sizeof(int) * (size_t)m);
@@ -994,86 +1417,143 @@ OzWork carve(void *work, int64_t m, int64_t k, int64_t n, int S) {
   p += 256;
   w.Mbits = (unsigned long long *)p;
   p += align256(sizeof(unsigned long long) * (size_t)m);
+  w.ctl = (OzCtl *)p;
+  p += align256(sizeof(OzCtl));
+  w.rsum = (unsigned long long *)p;
+  p += align256(sizeof(unsigned long long) * (size_t)m);
+  w.csum = (unsigned long long *)p;
+  p += align256(sizeof(unsigned long long) * (size_t)n);
   w.As = (signed char *)p;
-  p += align256((size_t)w.tm * w.KB * S * kChunk);
+  p += align256((size_t)w.tm * w.KB * (S > 0 ? S : kMaxSlices) * kChunk);
   w.Bs = (signed char *)p;
   return w;
 }
-// CRT-form residue planes of A and B and digit planes, in place of the slices
+// CRT-form residue planes of A and B, digit planes and (automatic calls, with
+// planes for kMaxSlices) the certificate's operands and candidate list, in
+// place of the slices
 struct Oz2Work {
   signed char *As, *Bs;
   unsigned char *R;
+  unsigned char *magA, *magB;
+  unsigned long long *cand;
   Oz2Dims d;
 };
 Oz2Work carve2(const OzWork &w, int64_t m, int64_t k, int64_t n, int S) {
   Oz2Work v;
-  v.d = oz2_dims(m, k, n, S);
+  v.d = oz2_dims(m, k, n, S > 0 ? S : kMaxSlices);
   char *p = (char *)w.As;
   v.As = (signed char *)p;
   p += align256((size_t)v.d.N * v.d.TBA * v.d.KBp * kChunk);
   v.Bs = (signed char *)p;
   p += align256((size_t)v.d.N * v.d.TBB * v.d.KBp * kChunk);
   v.R = (unsigned char *)p;
+  p += align256((size_t)v.d.N * m * n);
+  v.magA = (unsigned char *)p;
+  p += align256((size_t)v.d.TBA * 2 * v.d.KBp * kChunk);
+  v.magB = (unsigned char *)p;
+  p += align256((size_t)v.d.TBB * 2 * v.d.KBp * kChunk);
+  v.cand = (unsigned long long *)p;
+  if (S > 0) v.magA = v.magB = nullptr, v.cand = nullptr;  // explicit calls have no certificate
   return v;
 }
-bool oz_args_ok(int64_t k, int S) { return S >= kMinSlices && S <= ozaki_max_slices() && k <= 32768; }
-}  // namespace
+bool oz_args_ok(int64_t k, int S) { return S >= kMinSlices && S <= kMaxSlices && k <= 32768; }
 
-hipError_t ozaki_scan(const double2 *A, const double2 *B, int64_t m, int64_t k, int64_t n,
-                      int S, void *work, size_t work_bytes, hipStream_t s) {
-  if (m <= 0 || n <= 0 || k <= 0) return hipSuccess;
-  if (!oz_args_ok(k, S) || work_bytes < ozaki_workspace_bytes(m, k, n, S)) return hipErrorInvalidValue;
-  OzWork w = carve(work, m, k, n, S);
-  hipLaunchKernelGGL(k_scan_init, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, w.E, w.Emin, w.Mbits, w.range, m);
+// Exponent scan of A's rows and B's columns (every form).
+void oz_scan(const double2 *A, const double2 *B, int64_t m, int64_t k, int64_t n, const OzWork &w, hipStream_t s) {
+  const int64_t mx = m > n ? m : n;
+  hipLaunchKernelGGL(k_scan_init, dim3((unsigned)((mx + 255) / 256)), dim3(256), 0, s, w.E, w.Emin, w.Mbits, w.range,
+                     w.rsum, w.csum, m, n);
   const int64_t kchunk = 64;
   hipLaunchKernelGGL(k_row_exp, dim3((unsigned)((m + kThreads - 1) / kThreads), (unsigned)((k + kchunk - 1) / kchunk)),
                      dim3(kThreads), 0, s, A, m, k, kchunk, w.E, w.Emin, w.Mbits);
   hipLaunchKernelGGL(k_row_finish, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, w.E, w.Emin, w.Mbits, m,
                      w.range);
   hipLaunchKernelGGL(k_col_exp, dim3((unsigned)n), dim3(kThreads), 0, s, B, k, w.F, w.range + 1);
+}
+
+// Residue pass of the CRT form over both operands (pass 0: S from the first
+// plan, with the certificate operands when v.magA is set; pass 1: again if
+// the final plan raised S).
+void oz2_resid(const double2 *A, const double2 *B, int64_t m, int64_t k, int64_t n, const OzWork &w,
+               const Oz2Work &v, int pass, hipStream_t s) {
+  const bool mag = pass == 0 && v.magA != nullptr;
+  hipLaunchKernelGGL(k_oz2_resid<false>, dim3((unsigned)v.d.TBA, (unsigned)v.d.KBp), dim3(kThreads), 0, s, A, m, k,
+                     w.E, v.d.KBp, v.d.TBA, w.ctl, pass, v.As, mag ? v.magA : nullptr, w.rsum);
+  hipLaunchKernelGGL(k_oz2_resid<true>, dim3((unsigned)v.d.TBB, (unsigned)v.d.KBp), dim3(kThreads), 0, s, B, n, k,
+                     w.F, v.d.KBp, v.d.TBB, w.ctl, pass, v.Bs, mag ? v.magB : nullptr, w.csum);
+}
+
+Oz2GemmArgs oz2_args(const OzWork &w, const Oz2Work &v, int64_t m, int64_t n) {
+  Oz2GemmArgs g{};
+  g.m = m;
+  g.n = n;
+  g.TBA = v.d.TBA;
+  g.TBB = v.d.TBB;
+  g.tiles_m = (int)(v.d.TBA / 4);
+  g.tiles_n = (int)(v.d.TBB / 4);
+  g.ctl = w.ctl;
+  g.E = w.E;
+  g.F = w.F;
+  g.rsum = w.rsum;
+  g.csum = w.csum;
+  return g;
+}
+
+template <int MODE>
+hipError_t oz2_launch_gemm(const Oz2GemmArgs &g, int planes, hipStream_t s) {
+  const size_t lds = (size_t)kO2Bufs * kO2Stage;
+  hipError_t e =
+      hipFuncSetAttribute((const void *)k_oz2_gemm<MODE>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_oz2_gemm<MODE>, dim3((unsigned)(g.tiles_m * g.tiles_n), (unsigned)planes),
+                     dim3(kMainThreads), lds, s, g);
   return hipGetLastError();
 }
 
-hipError_t ozaki_read_ranges(void *work, int64_t m, int64_t k, int64_t n, int S, int *ranges,
-                             hipStream_t s) {
-  OzWork w = carve(work, m, k, n, S);
-  hipError_t e = hipMemcpyAsync(ranges, w.range, 2 * sizeof(int), hipMemcpyDeviceToHost, s);
-  if (e != hipSuccess) return e;
-  return hipStreamSynchronize(s);
+// The residue GEMMs of the CRT form (planes beyond the call's moduli count
+// leave at once), from the residue planes into R.
+hipError_t oz2_products(const OzWork &w, const Oz2Work &v, int64_t m, int64_t n, hipStream_t s) {
+  Oz2GemmArgs g = oz2_args(w, v, m, n);
+  g.As = v.As;
+  g.Bs = v.Bs;
+  g.R = v.R;
+  g.KB = v.d.KBp;
+  return oz2_launch_gemm<kOzRes>(g, v.d.N, s);
 }
 
-int ozaki_choose_slices(int64_t k, int range_a, int range_b) {
-  // Empirical model, fitted on MI355X and on the input distributions of the
-  // workloads (entries s*2^U(-e,e)*u), in bits of digit precision b = 8S:
-  //  * truncation error |err_ij| ~ 2^(E_i + F_j - b - 1) sqrt(k);
-  //  * (|A||B|)_ij / 2^(E_i + F_j) falls below 1 by at most W binades, with
-  //    W = (range_a + range_b) * f(k), f the measured envelope below (max
-  //    over e = 10, 20, 30 of the worst output among 512 x 512), interpolated
-  //    in log2 k; W can never exceed range_a + range_b + 2;
-  //  * 3 binades of margin (model mismatch seen at k = 300).
-  // The smallest S with the 2^-100 (|A||B|)_ij target met under that model.
-  // (The model was first fitted to 7-bit digits, S' = need / 7; 8-bit digits
-  // with the headroom rule above reach the same errors at 8S = 7S', checked
-  // with an exact model of the pipeline at k = 4096, spreads 2^+-20/30.)
-  if (k <= 0) return kMinSlices;
-  static const double lk[] = {4, 5, 6, 7, 8.23, 9.97, 11, 12, 13};
-  static const double fr[] = {0.42, 0.354, 0.28, 0.20, 0.12, 0.05, 0.0275, 0.004, 0.002};
-  const double x = log2((double)k);
-  double f = fr[0];
-  if (x >= lk[8]) {
-    f = fr[8];
-  } else if (x > lk[0]) {
-    for (int i = 0; i < 8; ++i)
-      if (x <= lk[i + 1]) {
-        const double t = (x - lk[i]) / (lk[i + 1] - lk[i]);
-        f = fr[i] + t * (fr[i + 1] - fr[i]);
-        break;
-      }
+// The Chinese-remainder reconstruction of C from the digit planes.
+void oz2_reconstruct(const OzWork &w, const Oz2Work &v, double2 *C, int64_t m, int64_t n, hipStream_t s) {
+  if ((m * n) % 4 == 0) {
+    const int64_t g = (m * n / 4 + kThreads - 1) / kThreads;
+    hipLaunchKernelGGL(k_oz2_crt<4>, dim3((unsigned)(g < 65536 ? g : 65536)), dim3(kThreads), 0, s, v.R, w.ctl, m,
+                       n, w.E, w.F, C);
+  } else {
+    const int64_t g = (m * n + kThreads - 1) / kThreads;
+    hipLaunchKernelGGL(k_oz2_crt<1>, dim3((unsigned)(g < 65536 ? g : 65536)), dim3(kThreads), 0, s, v.R, w.ctl, m,
+                       n, w.E, w.F, C);
   }
-  const double need = 101.0 + 0.5 * x + (double)(range_a + range_b) * f + 3.0;
-  int S = (int)ceil(need / 8.0);
-  return S < kMinSlices ? kMinSlices : S;
 }
+
+// Fix-up budget of an automatic call: how many uncertified outputs the fix-up
+// kernel takes before the plan raises S instead (each output costs one
+// wave's k-long compensated dot; a higher S costs a second residue pass and
+// more residue GEMMs).
+unsigned long long oz_fix_budget(int64_t m, int64_t n) {
+  const unsigned long long b = (unsigned long long)m * (unsigned long long)n / 1024ull;
+  return b > 4096ull ? b : 4096ull;
+}
+constexpr int kFixupGroups = 512;
+}  // namespace
+
+hipError_t ozaki_scan(const double2 *A, const double2 *B, int64_t m, int64_t k, int64_t n,
+                      int S, void *work, size_t work_bytes, hipStream_t s) {
+  if (m <= 0 || n <= 0 || k <= 0) return hipSuccess;
+  if (!oz_args_ok(k, S) || work_bytes < ozaki_workspace_bytes(m, k, n, S)) return hipErrorInvalidValue;
+  oz_scan(A, B, m, k, n, carve(work, m, k, n, S), s);
+  return hipGetLastError();
+}
+
+int ozaki_choose_slices(int64_t k, int range_a, int range_b) { return oz_guess_slices(k, range_a, range_b); }
 
 hipError_t ozaki_slice(const double2 *A, const double2 *B, int64_t m, int64_t k, int64_t n, int S,
                        void *work, size_t work_bytes, hipStream_t s) {
@@ -1082,10 +1562,8 @@ hipError_t ozaki_slice(const double2 *A, const double2 *B, int64_t m, int64_t k,
   OzWork w = carve(work, m, k, n, S);
   if (use_oz2(m, n, k, S)) {
     const Oz2Work v = carve2(w, m, k, n, S);
-    hipLaunchKernelGGL(k_oz2_resid<false>, dim3((unsigned)v.d.TBA, (unsigned)v.d.KBp), dim3(kThreads), 0, s,
-                       A, m, k, w.E, S, v.d.N, v.d.KBp, v.d.TBA, oz2_weights(v.d.N, S, false), v.As);
-    hipLaunchKernelGGL(k_oz2_resid<true>, dim3((unsigned)v.d.TBB, (unsigned)v.d.KBp), dim3(kThreads), 0, s,
-                       B, n, k, w.F, S, v.d.N, v.d.KBp, v.d.TBB, oz2_weights(v.d.N, S, true), v.Bs);
+    hipLaunchKernelGGL(k_oz_plan_begin, dim3(1), dim3(64), 0, s, w.ctl, w.range, oz2_ntab(k), S, k);
+    oz2_resid(A, B, m, k, n, w, v, 0, s);
     return hipGetLastError();
   }
   hipLaunchKernelGGL(k_slice_a<16>, dim3((unsigned)w.tm, (unsigned)w.KB), dim3(kThreads), 0, s, A, m, k, w.E, S, w.KB, w.As);
@@ -1100,46 +1578,28 @@ hipError_t ozaki_prepare(const double2 *A, const double2 *B, int64_t m, int64_t 
   return ozaki_slice(A, B, m, k, n, S, work, work_bytes, s);
 }
 
-namespace {
-// The N residue GEMMs of the CRT form, from the residue planes into R.
-hipError_t oz2_products(const Oz2Work &v, int64_t m, int64_t n, hipStream_t s) {
-  const size_t lds = (size_t)kO2Bufs * kO2Stage;
-  hipError_t e = hipFuncSetAttribute((const void *)k_oz2_gemm, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e != hipSuccess) return e;
-  const int64_t tiles_m = v.d.TBA / 4, tiles_n = v.d.TBB / 4;
-  hipLaunchKernelGGL(k_oz2_gemm, dim3((unsigned)(tiles_m * tiles_n), (unsigned)v.d.N), dim3(kMainThreads), lds, s,
-                     v.As, v.Bs, v.R, m, n, v.d.KBp, v.d.TBA, v.d.TBB, (int)tiles_m, (int)tiles_n);
-  return hipGetLastError();
-}
-}  // namespace
-
 hipError_t ozaki_products(int64_t m, int64_t k, int64_t n, int S, void *work, size_t work_bytes, hipStream_t s) {
   if (m <= 0 || n <= 0 || k <= 0) return hipSuccess;
   if (!oz_args_ok(k, S) || work_bytes < ozaki_workspace_bytes(m, k, n, S) || !use_oz2(m, n, k, S))
     return hipErrorInvalidValue;
-  return oz2_products(carve2(carve(work, m, k, n, S), m, k, n, S), m, n, s);
+  const OzWork w = carve(work, m, k, n, S);
+  return oz2_products(w, carve2(w, m, k, n, S), m, n, s);
 }
 
 hipError_t ozaki_multiply(double2 *C, int64_t m, int64_t k, int64_t n, int S, void *work,
                           size_t work_bytes, hipStream_t s) {
   if (m <= 0 || n <= 0) return hipSuccess;
-  if (k <= 0) return hipMemsetAsync(C, 0, sizeof(double2) * m * n, s);
+  if (k <= 0) {
+    hipLaunchKernelGGL(k_zero2, dim3(1024), dim3(256), 0, s, C, m * n);
+    return hipGetLastError();
+  }
   if (!oz_args_ok(k, S) || work_bytes < ozaki_workspace_bytes(m, k, n, S)) return hipErrorInvalidValue;
   OzWork w = carve(work, m, k, n, S);
   if (use_oz2(m, n, k, S)) {
     const Oz2Work v = carve2(w, m, k, n, S);
-    hipError_t e = oz2_products(v, m, n, s);
+    hipError_t e = oz2_products(w, v, m, n, s);
     if (e != hipSuccess) return e;
-    const Oz2Consts cs = oz2_consts(v.d.N, S);
-    if ((m * n) % 4 == 0) {
-      const int64_t g = (m * n / 4 + kThreads - 1) / kThreads;
-      hipLaunchKernelGGL(k_oz2_crt<4>, dim3((unsigned)(g < 65536 ? g : 65536)), dim3(kThreads), 0, s, v.R, v.d.N,
-                         m, n, w.E, w.F, cs, C);
-    } else {
-      const int64_t g = (m * n + kThreads - 1) / kThreads;
-      hipLaunchKernelGGL(k_oz2_crt<1>, dim3((unsigned)(g < 65536 ? g : 65536)), dim3(kThreads), 0, s, v.R, v.d.N,
-                         m, n, w.E, w.F, cs, C);
-    }
+    oz2_reconstruct(w, v, C, m, n, s);
     return hipGetLastError();
   }
   switch (S) {
@@ -1156,30 +1616,73 @@ hipError_t ozaki_multiply(double2 *C, int64_t m, int64_t k, int64_t n, int S, vo
   }
 }
 
-hipError_t ozaki_gemm(const double2 *A, const double2 *B, double2 *C, int64_t m, int64_t k,
-                      int64_t n, int S, void *work, size_t work_bytes, hipStream_t s, int *used) {
-  if (used) *used = S;
-  if (m <= 0 || n <= 0) return hipSuccess;
-  if (k <= 0) return hipMemsetAsync(C, 0, sizeof(double2) * m * n, s);
-  const int Sw = S > 0 ? S : ozaki_max_slices();
-  hipError_t e = ozaki_scan(A, B, m, k, n, Sw, work, work_bytes, s);
+// An automatic call: every stage enqueued, the slice count chosen on the
+// device (no synchronisation, so the call can be captured in a graph).
+//   scan -> first plan (exponent-range rule) -> residues + certificate
+//   operands -> certificate GEMM -> final plan -> residues again only if S
+//   was raised -> residue GEMMs -> reconstruction -> fix-up of the outputs
+//   the final S does not certify.
+hipError_t ozaki_auto(const double2 *A, const double2 *B, double2 *C, int64_t m, int64_t k, int64_t n, void *work,
+                      size_t work_bytes, hipStream_t s) {
+  if (k > 32768 || work_bytes < ozaki_workspace_bytes(m, k, n, 0)) return hipErrorInvalidValue;
+  const OzWork w = carve(work, m, k, n, 0);
+  const Oz2Work v = carve2(w, m, k, n, 0);
+  const OzNtab nt = oz2_ntab(k);
+  oz_scan(A, B, m, k, n, w, s);
+  hipLaunchKernelGGL(k_oz_plan_begin, dim3(1), dim3(64), 0, s, w.ctl, w.range, nt, 0, k);
+  oz2_resid(A, B, m, k, n, w, v, 0, s);
+  Oz2GemmArgs g = oz2_args(w, v, m, n);
+  g.As = (const signed char *)v.magA;
+  g.Bs = (const signed char *)v.magB;
+  g.R = nullptr;
+  g.KB = 2 * v.d.KBp;
+  g.cand = v.cand;
+  hipError_t e = oz2_launch_gemm<kOzMag>(g, 1, s);
   if (e != hipSuccess) return e;
-  if (S <= 0) {
-    int r[2];
-    e = ozaki_read_ranges(work, m, k, n, Sw, r, s);
-    if (e != hipSuccess) return e;
-    S = ozaki_choose_slices(k, r[0], r[1]);
-    if (used) *used = S > ozaki_max_slices() ? 0 : S;
-    // More slices than the fused kernel holds: the reference-order GEMM
-    // (FP64 VALU, bit-identical to the reference) is both exact enough and,
-    // at the small k that needs it, cheap.
-    if (S > ozaki_max_slices()) return gemm_plain(A, B, C, m, k, n, 8, -1, s);
-    e = ozaki_slice(A, B, m, k, n, S, work, work_bytes, s);
-  } else {
-    e = ozaki_slice(A, B, m, k, n, S, work, work_bytes, s);
+  hipLaunchKernelGGL(k_oz_plan_final, dim3(1), dim3(64), 0, s, w.ctl, nt, oz_fix_budget(m, n));
+  oz2_resid(A, B, m, k, n, w, v, 1, s);
+  e = oz2_products(w, v, m, n, s);
+  if (e != hipSuccess) return e;
+  oz2_reconstruct(w, v, C, m, n, s);
+  hipLaunchKernelGGL(k_oz_fixup, dim3(kFixupGroups), dim3(kThreads), 0, s, A, B, C, m, k, v.cand, w.ctl);
+  return hipGetLastError();
+}
+
+hipError_t ozaki_gemm(const double2 *A, const double2 *B, double2 *C, int64_t m, int64_t k,
+                      int64_t n, int S, void *work, size_t work_bytes, hipStream_t s) {
+  if (m <= 0 || n <= 0) return hipSuccess;
+  if (k <= 0) {
+    hipLaunchKernelGGL(k_zero2, dim3(1024), dim3(256), 0, s, C, m * n);
+    return hipGetLastError();
   }
+  if (S <= 0) return ozaki_auto(A, B, C, m, k, n, work, work_bytes, s);
+  hipError_t e = ozaki_prepare(A, B, m, k, n, S, work, work_bytes, s);
   if (e != hipSuccess) return e;
   return ozaki_multiply(C, m, k, n, S, work, work_bytes, s);
+}
+
+hipError_t ozaki_info(const void *work, int64_t m, int64_t k, int64_t n, int S, int *slices, int64_t *fixups,
+                      hipStream_t s) {
+  if (m <= 0 || n <= 0 || k <= 0) {
+    if (slices) *slices = 0;
+    if (fixups) *fixups = 0;
+    return hipSuccess;
+  }
+  const OzWork w = carve(const_cast<void *>(work), m, k, n, S);
+  if (!use_oz2(m, n, k, S) && S > 0) {  // triangular form: no control block
+    if (slices) *slices = S;
+    if (fixups) *fixups = 0;
+    return hipSuccess;
+  }
+  int hS = 0;
+  unsigned long long hf = 0;
+  hipError_t e = hipMemcpyAsync(&hS, &w.ctl->S, sizeof(int), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(&hf, &w.ctl->nfix, sizeof(hf), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return e;
+  if (slices) *slices = hS;
+  if (fixups) *fixups = (int64_t)hf;
+  return hipSuccess;
 }
 
 }  // namespace mfhip

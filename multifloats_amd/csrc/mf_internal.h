@@ -76,15 +76,19 @@ hipError_t cdot(const double4 *a, const double4 *b, int64_t n, int64_t ri, doubl
 hipError_t reduce_dim(int kind, const double2 *a, const int32_t *mask, int64_t inner, int64_t len,
                       int64_t outer, double2 *out, hipStream_t s);
 
-// Ozaki split-precision GEMM with S int8 slices per operand (8 <= S <= 16).
+// Ozaki split-precision GEMM with S int8 slices per operand (7 <= S <= 16),
+// or S <= 0: the CRT form with the slice count chosen on the device and every
+// output certified to 2^-100 (|A||B|)_ij (uncertified outputs recomputed by
+// a compensated dot). Enqueues only; never synchronises.
 int ozaki_max_slices();
 size_t ozaki_workspace_bytes(int64_t m, int64_t k, int64_t n, int S);
-// S <= 0 chooses the slice count from the data (synchronises the stream to
-// read the exponent ranges) and falls back to gemm_plain when the choice
-// exceeds ozaki_max_slices(); *used receives the count (0 = plain fallback).
 hipError_t ozaki_gemm(const double2 *A, const double2 *B, double2 *C, int64_t m, int64_t k,
-                      int64_t n, int S, void *work, size_t work_bytes, hipStream_t s, int *used);
+                      int64_t n, int S, void *work, size_t work_bytes, hipStream_t s);
+// The exponent-range rule that starts an automatic call's search.
 int ozaki_choose_slices(int64_t k, int range_a, int range_b);
+// Slice count and fix-up count of the last call on `work` (synchronises s).
+hipError_t ozaki_info(const void *work, int64_t m, int64_t k, int64_t n, int S, int *slices, int64_t *fixups,
+                      hipStream_t s);
 // Moduli of the CRT form that a call with these sizes and S slices runs
 // (0: the triangular form runs).
 int ozaki_moduli(int64_t m, int64_t k, int64_t n, int S);

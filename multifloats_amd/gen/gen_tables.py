@@ -309,8 +309,9 @@ def main():
                % (len(mods), ", ".join(map(str, mods))))
     out.append("MF_TABLE int mf_oz2_m[%d] = {%s};\n" % (len(mods), ", ".join(map(str, mods))))
     out.append(arr("mf_oz2_rlimb", rl, per_line=4))
-    # CRT constants for the first N moduli, N = 1..40 (host side): y[N-1][t] =
-    # (M_N / m_t)^-1 mod m_t and M_N = prod_{t<N} m_t rounded to a double-double
+    # CRT constants for the first N moduli, N = 1..40 (read by the device-side
+    # plan of a call): y[N-1][t] = (M_N / m_t)^-1 mod m_t and
+    # M_N = prod_{t<N} m_t rounded to a double-double
     ys, mp = [], []
     for N in range(1, len(mods) + 1):
         MN = 1
@@ -320,12 +321,10 @@ def main():
         ys += [0] * (len(mods) - N)
         h = float(MN)
         mp += [h, float(MN - int(h))]
-    out.append("#ifdef __cplusplus\nstatic constexpr int mf_oz2_y_host[%d] = {\n%s\n};\n"
+    out.append("MF_TABLE int mf_oz2_y[%d] = {\n%s\n};\n"
                % (len(ys), ",\n".join("    " + ", ".join(map(str, ys[r:r + len(mods)]))
                                       for r in range(0, len(ys), len(mods)))))
-    out.append("static constexpr double mf_oz2_mp_host[%d] = {\n%s\n};\n#endif\n"
-               % (len(mp), "\n".join("    " + ", ".join(hx(v) for v in mp[r:r + 4]) + ","
-                                     for r in range(0, len(mp), 4))))
+    out.append(arr("mf_oz2_mp", mp, per_line=4))
 
     dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "csrc", "dd_tables.h")
     with open(dst, "w") as f:

@@ -214,8 +214,8 @@ def test_crt_form_tables():
         limbs = [int(v) for v in rl[5 * t:5 * t + 5]]
         assert all(0 <= v < 2 ** 36 for v in limbs)
         assert sum(v << (36 * (4 - j)) for j, v in enumerate(limbs)) == (1 << 180) // q
-    ys = vals(r"static constexpr int mf_oz2_y_host", int)
-    mp = vals(r"static constexpr double mf_oz2_mp_host", fl)
+    ys = vals(r"MF_TABLE int mf_oz2_y", int)
+    mp = vals(r"MF_TABLE double mf_oz2_mp", fl)
     for N in range(1, 41):
         MN = math.prod(mods[:N])
         for t in range(40):
