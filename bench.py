@@ -1,31 +1,42 @@
 #!/usr/bin/env python3
 """Benchmark of the DD hot path on one MI355X.
 
-Default (the headline): float64x2 GEMM C = A*B at m = n = k = 4096, entries
-s*2^U(-20,20)*u with lo per the config-0 rule, inputs and output resident in
-HBM, in two forms on the same inputs:
-  ozaki  the INT8-MFMA split-precision GEMM (slices chosen from the data), whole
-         call per step (exponent scan, residues, products, reconstruction);
+Default: every workload of the measurement plan in one run, printed as ONE
+JSON line. The headline (metric, value, roofline, cpu_baseline) is the
+float64x2 GEMM at m = n = k = 4096, entries s*2^U(-20,20)*u with lo per the
+config-0 rule, inputs and output resident in HBM:
+  ozaki  the INT8-MFMA split-precision GEMM, automatic form (slice count
+         chosen on the device, every output certified to 2^-100 (|A||B|)_ij),
+         whole call per step;
   plain  the reference-order FP64 GEMM (bit-identical to the reference),
          renorm interval 8.
-Prints ONE JSON line: value = Ozaki DD-GFLOP/s (2*m*n*k per step) when its
-measured error on a sampled block is within 2^-100 (|A||B|) elementwise, with
-the plain form, the error figures and the speed-up in the same line.
+value = Ozaki DD-GFLOP/s (2 m n k per step) when its measured error on a
+sampled block is within 2^-100 (|A||B|) elementwise. The key "configs" holds
+the five workloads, each with its own roofline, CPU baseline and accuracy:
+  C0  plain GEMM 256^3, config-0 entries (bit-identical to the reference);
+  C1  elementwise + - * / sqrt exp log sin erf and sum / dot / norm2 at
+      N = 2^26 (the C1 input distributions);
+  C2  the headline pair, plus the complex64x2 GEMM at 2048^3;
+  C3  Ozaki 8192^3 at exponent spreads 2^+-10 and 2^+-30, plain 8192^3 beside;
+  C4  10,000 GEMMs + 50,000 dot products in one persistent grouped launch
+      replayed from a HIP graph, per-call launches as the comparison.
 
-  python bench.py [--steps K] [--warmup W] [--workload gemm|cgemm|blas|trsm|batched|elementwise|reduce]
-                  [--no-cpu] [--variant V]
+  python bench.py [--steps K] [--warmup W] [--workload all|gemm|cgemm|blas|trsm|batched|elementwise|reduce]
+                  [--no-cpu] [--quick]
 
 Timing: torch.cuda events recorded on the stream the kernels are launched on
 (torch's current stream, whose handle is passed to the C ABI), bracketed by
-device synchronisation. The CPU baseline is the reference's own compiled
-matmul (oracle/_ref) when present, else the oracle's C restatement, timed on
-one host core over a bounded slice (all of A times the first columns of B)
-of the same workload; its outputs are also compared bit-for-bit with the GPU's.
+device synchronisation. CPU baselines run on one host core over a bounded
+sample of each workload (stated in each "sample"): the reference's own
+compiled code (oracle/_ref: matmuldd_mm, and array loops over its scalar
+expdd, logdd, ...) where it has an entry point ("reference"), else the
+oracle's C restatement ("port": the Fortran-only reductions).
 """
 import argparse
 import ctypes
 import json
 import os
+import platform
 import sys
 import time
 
@@ -35,20 +46,29 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 vector (and matrix) peak
+FP64_ISSUE_PEAK = 39.3e12    # FP64 lane-instructions/s: 256 CUs x 64 lanes x 2.4 GHz
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
 FP64_FLOPS_PER_DDMAC = 15.75  # 2 MUL + 2 FMA(x2) + 9 ADD + 6/8 renorm at ri = 8
+FP64_INSTR_PER_DDMAC = 13.75  # 2 MUL + 2 FMA + 9 ADD + 6/8 renorm instructions
 INT8_PEAK_TOPS = 5000.0      # MI355X dense INT8 MFMA
 
 # HBM-side bytes per launch from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
-# (KiB per dispatch) on the 4096^3 workload, scaled by what the same counters
-# read for a 1 GiB device copy: FETCH_SIZE x 2.0 (it reads 524,300 KiB for
-# 1 GiB fetched), WRITE_SIZE x 1.0 (1,048,576 KiB). Summary and commands:
-# profiles/pmc_traffic_gemm_4096.md. Keyed by (kernel, size, slices).
+# (KiB per dispatch), scaled by what the same counters read for a 1 GiB
+# device copy: FETCH_SIZE x 2.0 (it reads 524,300 KiB for 1 GiB fetched),
+# WRITE_SIZE x 1.0 (1,048,576 KiB). Keyed by (kernel, size, slices); the
+# summaries and commands are in profiles/pmc_traffic_gemm_4096.md.
 PMC_TRAFFIC = {
     ("k_ozaki", 4096, 14): 2765231.0 * 2.0 * 1024 + 286820.0 * 1024,
     ("k_oz2_gemm", 4096, 14): 1682594.0 * 2.0 * 1024 + 599483.0 * 1024,
     ("k_gemm_plain", 4096, 0): 3144165.19 * 2.0 * 1024 + 286727.06 * 1024,
 }
+
+# FP64 VALU lane-instructions per element of the transcendental array
+# kernels on the C1 input distributions: (SQ_INSTS_VALU_ADD_F64 +
+# SQ_INSTS_VALU_MUL_F64 + SQ_INSTS_VALU_FMA_F64) x 64 / N from rocprofv3
+# --pmc at N = 2^26 (profiles/elementwise_fp64_counts_2p26.md). With these
+# the FP64-issue fraction is instructions/element x elements/s / 39.3e12.
+FP64_INSTR_PER_ELEM = {}
 
 
 def parse():
@@ -56,16 +76,18 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=2)
-    p.add_argument("--workload", default="gemm", choices=["gemm", "cgemm", "blas", "trsm", "batched", "elementwise", "reduce"])
+    p.add_argument("--workload", default="all",
+                   choices=["all", "gemm", "cgemm", "blas", "trsm", "batched", "elementwise", "reduce"])
     p.add_argument("--retune", action="store_true", help="batched: rerun the tile autotuner first")
     p.add_argument("--size", type=int, default=4096)
     p.add_argument("--spread", type=float, default=20.0)
-    p.add_argument("--slices", type=int, default=0, help="0 = chosen from the data")
+    p.add_argument("--slices", type=int, default=0, help="0 = automatic (certified) form")
     p.add_argument("--form", default="both", choices=["both", "plain", "ozaki"])
     p.add_argument("--err-sample", type=int, default=256, help="error on an N x N block of sampled outputs")
     p.add_argument("--variant", type=int, default=-1)
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--cpu-cols", type=int, default=512, help="CPU baseline: all of A times this many columns of B")
+    p.add_argument("--cpu-cols", type=int, default=128, help="CPU GEMM baseline: all of A times this many columns of B")
+    p.add_argument("--quick", action="store_true", help="all: skip C3 and C4 (a shorter smoke of the bench)")
     return p.parse_args()
 
 
@@ -84,29 +106,54 @@ def timed(torch, fn, steps, warmup):
     return e0.elapsed_time(e1) / steps  # ms per step
 
 
-def cpu_gemm_baseline(A, B, C_gpu, cols, ri):
-    """Time the reference matmul on one core over all of A times B[:, :cols]."""
-    from tests import _native as nt  # test infrastructure, timed as the baseline
+def host_info():
+    model = platform.processor() or ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count()}
+
+
+def ref_libs():
+    """(matmul library, array-loop library) of the compiled reference, or None."""
+    from tests import _native as nt  # test infrastructure: paths of the CPU oracle builds
+    if not os.path.exists(nt.REF):
+        return None, None
+    L = ctypes.CDLL(nt.REF)
+    L.matmuldd_mm.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int64] * 4
+    loops = os.path.join(os.path.dirname(nt.REF), "librefloop.so")
+    R = None
+    if os.path.exists(loops):
+        R = ctypes.CDLL(loops)
+        R.refloop_unary.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
+        R.refloop_binary.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
+    return L, R
+
+
+def cpu_matmul(A, B, cols, ri, reps=1):
+    """Time the reference matmul (the oracle's restatement if the reference
+    build is absent) on one core over all of A times B[:, :cols]; returns
+    (seconds, C block, kind)."""
+    from tests import _native as nt
     from multifloats_amd import datagen
     m, k = A.shape[:2]
     Bs = datagen.as_colmajor(B[:, :cols])
     A = datagen.as_colmajor(A)
     C = datagen.as_colmajor(np.zeros((m, cols, 2)))
-    kind = "reference"
-    if os.path.exists(nt.REF):
-        L = ctypes.CDLL(nt.REF)
-        fn = L.matmuldd_mm
-    else:
-        kind = "port"
-        fn = nt.oracle().or_matmul_mm
-    fn.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int64] * 4
-    t0 = time.perf_counter()
-    fn(A.ctypes.data, Bs.ctypes.data, C.ctypes.data, m, k, cols, ri)
-    dt = time.perf_counter() - t0
-    same = bool(nt.bits_equal(C, C_gpu[:, :cols]).all())
-    return dict(value=round(2.0 * m * k * cols / dt / 1e9, 4), unit="DD-GFLOP/s", cores=1, kind=kind,
-                sample="C[:, :%d] = A(%dx%d) * B[:, :%d] on one host core, %.1f s; bit-identical to the GPU: %s"
-                % (cols, m, k, cols, dt, same)), same
+    L, _ = ref_libs()
+    kind, fn = ("reference", L.matmuldd_mm) if L is not None else ("port", nt.oracle().or_matmul_mm)
+    best = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn(A.ctypes.data, Bs.ctypes.data, C.ctypes.data, m, k, cols, ri)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    return best, C, kind
 
 
 def sampled_error(A, B, C, rows, cols):
@@ -117,11 +164,56 @@ def sampled_error(A, B, C, rows, cols):
     return float(rel.max()), float(rel.mean())
 
 
+def gemm_roofline_plain(m, n, k, ms, size_key=None, variant=-1):
+    achieved = FP64_FLOPS_PER_DDMAC * m * n * k / (ms * 1e-3) / 1e12
+    issue = FP64_INSTR_PER_DDMAC * m * n * k / (ms * 1e-3) / FP64_ISSUE_PEAK
+    return {"bound": "fp64", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "fp64_issue_frac": round(issue, 4),
+            "traffic": PMC_TRAFFIC.get(("k_gemm_plain", size_key, 0)) if variant < 0 else None,
+            "note": "k_gemm_plain, FP64 VALU: %.2f FP64 flops (%.2f FP64 instructions) per DD-MAC, FMA = 2 flops; "
+                    "fp64_issue_frac = instructions issued / (256 CUs x 64 lanes x 2.4 GHz)"
+                    % (FP64_FLOPS_PER_DDMAC, FP64_INSTR_PER_DDMAC)}
+
+
+def ozaki_run(torch, mf, dA, dB, dC, m, k, n, steps, warmup, stream, slices=0):
+    """Time the Ozaki call (automatic form unless slices > 0), then its parts:
+    the same call with the slice count it chose (explicit: no certificate)
+    and the residue GEMMs alone (k_oz2_gemm, the dominant kernel)."""
+    wb = mf.ozaki_workspace_bytes(m, k, n, 0)
+    work = torch.empty(wb, dtype=torch.uint8, device="cuda")
+
+    def step():
+        mf.dev_matmul_ozaki(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), m, k, n, slices, work.data_ptr(), wb, stream)
+    ms = timed(torch, step, steps, warmup)
+    S, fix = mf.ozaki_info(work.data_ptr(), m, k, n, slices, stream)
+    C = dC.cpu().numpy().copy()
+
+    def step_explicit():
+        mf.dev_matmul_ozaki(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), m, k, n, S, work.data_ptr(), wb, stream)
+    ms_explicit = timed(torch, step_explicit, max(3, steps // 2), 1)
+    N = mf.ozaki_moduli(m, k, n, S)
+    res = dict(ms=ms, ms_explicit_slices=ms_explicit, slices=S, fixups=fix, moduli=N)
+    if N:
+        mf.dev_ozaki_prepare(dA.data_ptr(), dB.data_ptr(), m, k, n, S, work.data_ptr(), wb, stream)
+        ms_kern = timed(torch, lambda: mf.dev_ozaki_products(m, k, n, S, work.data_ptr(), wb, stream),
+                        max(3, steps // 2), 1)
+        tops = N * 2.0 * m * n * k / (ms_kern * 1e-3) / 1e12
+        res["ms_kernel"] = ms_kern
+        res["roofline"] = {"bound": "mfma", "achieved": round(tops, 2), "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
+                           "frac": round(tops / INT8_PEAK_TOPS, 4),
+                           "traffic": PMC_TRAFFIC.get(("k_oz2_gemm", m, S)) if m == n == k else None,
+                           "note": "k_oz2_gemm: %d residue INT8 GEMMs x 2mnk int8 ops per launch (MFMA), timed alone "
+                                   "on its stream; the rest of the call: exponent scan, residues with the certificate "
+                                   "operands, certificate GEMM (bf16 MFMA), plans, reconstruction, fix-up" % N}
+    del work
+    return res, C
+
+
 def bench_gemm(args, torch, mf):
+    """C2 headline: float64x2 GEMM m = n = k = size, Ozaki (automatic) and plain."""
     from multifloats_amd import datagen
     m = n = k = args.size
     ri = 8
-    S = args.slices
     rng = np.random.default_rng(2026)
     A = datagen.matrix(rng, m, k, args.spread)
     B = datagen.matrix(rng, k, n, args.spread)
@@ -141,74 +233,41 @@ def bench_gemm(args, torch, mf):
         ms = timed(torch, step_plain, args.steps, args.warmup)
         C = datagen.unflat(dC.cpu().numpy(), m, n)
         emax, emean = sampled_error(A, B, C, rows, cols)
-        achieved = FP64_FLOPS_PER_DDMAC * m * n * k / (ms * 1e-3) / 1e12
         res["plain"] = dict(ms=ms, value=flops / (ms * 1e-3) / 1e9, err_max=emax, err_mean=emean,
-                            roofline={"bound": "fp64", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
-                                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                                      "traffic": PMC_TRAFFIC.get(("k_gemm_plain", args.size, 0))
-                                      if args.variant < 0 else None,
-                                      "note": "k_gemm_plain, FP64 VALU: %.2f FP64 flops per DD-MAC (FMA = 2)"
-                                              % FP64_FLOPS_PER_DDMAC})
+                            roofline=gemm_roofline_plain(m, n, k, ms, args.size, args.variant))
         if not args.no_cpu:
-            res["cpu"], _ = cpu_gemm_baseline(A, B, C, args.cpu_cols, ri)
+            dt, Cc, kind = cpu_matmul(A, B, args.cpu_cols, ri)
+            from tests import _native as nt
+            same = bool(nt.bits_equal(Cc, C[:, :args.cpu_cols]).all())
+            res["cpu"] = dict(value=round(2.0 * m * k * args.cpu_cols / dt / 1e9, 4), unit="DD-GFLOP/s", cores=1,
+                              kind=kind, sample="C[:, :%d] = A(%dx%d) * B[:, :%d] on one host core, %.1f s; "
+                                                "bit-identical to the GPU's plain form: %s"
+                                                % (args.cpu_cols, m, k, args.cpu_cols, dt, same), **host_info())
         del C
 
     if args.form in ("both", "ozaki"):
-        wb = mf.ozaki_workspace_bytes(m, k, n, S)
-        work = torch.empty(wb, dtype=torch.uint8, device="cuda")
-
-        def step_oz():
-            mf.dev_matmul_ozaki(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), m, k, n, S, work.data_ptr(), wb, stream)
-
-        def step_mult():
-            mf.dev_ozaki_multiply(dC.data_ptr(), m, k, n, S, work.data_ptr(), wb, stream)
-
-        def step_products():
-            mf.dev_ozaki_products(m, k, n, S, work.data_ptr(), wb, stream)
-        ms = timed(torch, step_oz, args.steps, args.warmup)
-        S = mf.ozaki_slices_used()
-        if S == 0:
-            raise SystemExit("bench: the slice rule chose the reference-order GEMM for this input")
-        mf.dev_ozaki_prepare(dA.data_ptr(), dB.data_ptr(), m, k, n, S, work.data_ptr(), wb, stream)
-        ms_mult = timed(torch, step_mult, args.steps, 1)
-        N = mf.ozaki_moduli(m, k, n, S)
-        if N:
-            # CRT form: the dominant kernel is k_oz2_gemm, one launch of the N
-            # residue INT8 GEMMs, timed alone through the products entry point
-            # (the multiply stage adds the reconstruction, k_oz2_crt)
-            ms_kern = timed(torch, step_products, args.steps, 1)
-            mf.dev_ozaki_multiply(dC.data_ptr(), m, k, n, S, work.data_ptr(), wb, stream)
-            P, kern = N, "k_oz2_gemm"
-            note = ("k_oz2_gemm: %d residue INT8 GEMMs x 2mnk int8 ops per launch (MFMA), timed alone; the "
-                    "reconstruction k_oz2_crt takes the rest of ms_multiply" % N)
-        else:
-            ms_kern = ms_mult
-            P, kern = S * (S + 1) // 2, "k_ozaki"
-            note = "k_ozaki INT8 MFMA: %d slice pairs x 2mnk int8 ops per launch" % P
-        C = datagen.unflat(dC.cpu().numpy(), m, n)
-        emax, emean = sampled_error(A, B, C, rows, cols)
-        tops = P * 2.0 * m * n * k / (ms_kern * 1e-3) / 1e12
-        res["ozaki"] = dict(ms=ms, ms_multiply=ms_mult, ms_kernel=ms_kern, value=flops / (ms * 1e-3) / 1e9,
-                            err_max=emax,
-                            err_mean=emean, slices=S, form="crt" if N else "triangular",
-                            int8_products=P,
-                            roofline={"bound": "mfma", "achieved": round(tops, 2), "peak": INT8_PEAK_TOPS,
-                                      "unit": "TOP/s", "frac": round(tops / INT8_PEAK_TOPS, 4),
-                                      "traffic": PMC_TRAFFIC.get((kern, args.size, S)), "note": note})
+        r, Cf = ozaki_run(torch, mf, dA, dB, dC, m, k, n, args.steps, args.warmup, stream, args.slices)
+        C = datagen.unflat(Cf, m, n)
+        r["err_max"], r["err_mean"] = sampled_error(A, B, C, rows, cols)
+        r["value"] = flops / (r["ms"] * 1e-3) / 1e9
+        r["form"] = "crt, automatic (certified)" if args.slices <= 0 else "explicit slices"
+        res["ozaki"] = r
         del C
 
     head = res.get("ozaki") if args.form != "plain" else res["plain"]
     ok = head["err_max"] <= 2.0 ** -100 if args.form != "plain" else True
+    S = res["ozaki"]["slices"] if "ozaki" in res else 0
+    metric = ("DD-GEMM DD-GFLOP/s at %d^3: Ozaki-INT8 vs plain FP64, err<=2^-100*|A||B|" % m)
     out = {
-        "metric": "DD-GEMM DD-GFLOP/s at 4096^3: Ozaki-INT8 vs plain FP64, err<=2^-100*|A||B|",
+        "metric": metric,
         "value": round(head["value"], 2) if ok else None, "unit": "DD-GFLOP/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(head["ms"], 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "int8-mfma+f64" if args.form != "plain" else "f64", "data": "synthetic",
         "config": {"workload": "float64x2 GEMM m=n=k=%d, entries s*2^U(-%g,%g)*u, lo per config-0; %s"
                                % (m, args.spread, args.spread,
-                                  "Ozaki INT8 (slices chosen from the data: %d) and plain reference-order "
-                                  "(renorm 8) forms" % S
+                                  "Ozaki INT8 automatic form (slice count chosen on the device: %d; every output "
+                                  "certified to 2^-100 |A||B|) and plain reference-order (renorm 8) forms" % S
                                   if args.form == "both" else args.form),
                    "m": m, "n": n, "k": k, "slices": S, "renorm_interval": ri,
                    "error_sample": "%dx%d outputs vs exact accumulator" % (es, es)},
@@ -220,7 +279,8 @@ def bench_gemm(args, torch, mf):
             out[form] = {kk: (round(v, 4) if isinstance(v, float) and kk.startswith("ms") else
                               round(v, 2) if kk == "value" else v)
                          for kk, v in r.items() if kk != "roofline"}
-            out[form]["roofline"] = r["roofline"]
+            if "roofline" in r:
+                out[form]["roofline"] = r["roofline"]
     if "ozaki" in res and "plain" in res:
         out["speedup_ozaki_vs_plain"] = round(res["plain"]["ms"] / res["ozaki"]["ms"], 3)
     if "cpu" in res:
@@ -228,30 +288,329 @@ def bench_gemm(args, torch, mf):
     return out
 
 
-def bench_cgemm(args, torch, mf):
-    """complex64x2 GEMM m = n = k = 2048 (4 real chains per output, fused)."""
+def bench_c0(args, torch, mf):
+    """C0: plain GEMM 256^3, config-0 entries, renorm 8; all outputs compared
+    bit for bit with the reference's own matmuldd_mm timed on one core."""
     from multifloats_amd import datagen
-    m = n = k = 2048 if args.size == 4096 else args.size
-    rng = np.random.default_rng(3)
+    from tests import _native as nt
+    m = n = k = 256
+    rng = np.random.default_rng(0)
+    A, B = datagen.matrix(rng, m, k), datagen.matrix(rng, k, n)
+    dA = torch.from_numpy(datagen.flat(A)).cuda()
+    dB = torch.from_numpy(datagen.flat(B)).cuda()
+    dC = torch.empty((m * n, 2), dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    reps = max(args.steps, 50)
+    ms = timed(torch, lambda: mf.dev_matmul(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), m, k, n, 8, stream),
+               reps, args.warmup)
+    C = datagen.unflat(dC.cpu().numpy(), m, n)
+    out = {"workload": "float64x2 GEMM m=n=k=256, config-0 entries (hi ~ U(-1,1)), renorm 8",
+           "value": round(2.0 * m * n * k / (ms * 1e-3) / 1e9, 2), "unit": "DD-GFLOP/s", "ms": round(ms, 4),
+           "steps": reps, "roofline": gemm_roofline_plain(m, n, k, ms)}
+    out["roofline"]["note"] += "; at 256^3 one launch of 16 workgroups: latency-bound, not issue-bound"
+    if not args.no_cpu:
+        dt, Cc, kind = cpu_matmul(A, B, n, 8, reps=3)
+        out["cpu_baseline"] = dict(value=round(2.0 * m * n * k / dt / 1e9, 4), unit="DD-GFLOP/s", cores=1, kind=kind,
+                                   sample="the whole 256^3 product on one host core, best of 3: %.4f s" % dt,
+                                   **host_info())
+        out["bit_identical_to_reference"] = bool(nt.bits_equal(Cc, C).all())
+    return out
 
-    def cm(r, c):
-        re, im = datagen.matrix(rng, r, c, args.spread), datagen.matrix(rng, r, c, args.spread)
+
+# C1: (name, op code, kind, generator, bytes per element)
+C1_OPS = [("add", "ADD", "binary", "config0", 48), ("sub", "SUB", "binary", "config0", 48),
+          ("mul", "MUL", "binary", "config0", 48), ("div", "DIV", "binary", "config0", 48),
+          ("sqrt", "SQRT", "unary", "abs", 32), ("exp", "EXP", "unary", "exp_inputs", 32),
+          ("log", "LOG", "unary", "log_inputs", 32), ("sin", "SIN", "unary", "sin_inputs", 32),
+          ("erf", "ERF", "unary", "erf_inputs", 32)]
+REF_LOOP_UNARY = {"sqrt": 0, "exp": 1, "log": 2, "sin": 3, "erf": 4}
+REF_LOOP_BINARY = {"add": 0, "sub": 1, "mul": 2, "div": 3}
+
+
+def bench_c1(args, torch, mf):
+    """C1: elementwise and reductions at N = 2^26 on the C1 distributions."""
+    from multifloats_amd import datagen
+    from tests import _native as nt
+    n = 1 << 26
+    rng = np.random.default_rng(7)
+    stream = torch.cuda.current_stream().cuda_stream
+    _, R = ref_libs()
+    res = {}
+    base = datagen.config0(rng, n)
+    other = datagen.config0(rng, n)
+    dbase = torch.from_numpy(base).cuda()
+    dother = torch.from_numpy(other).cuda()
+    c = torch.empty_like(dbase)
+    check = 1 << 16  # outputs compared bit for bit with the oracle
+    for nm, code, kind, gen, bpe in C1_OPS:
+        if gen == "config0":
+            x, dx = base, dbase
+        elif gen == "abs":
+            x = np.where(base[:, :1] < 0, -base, base)  # |x| as a double-double
+            dx = torch.from_numpy(x).cuda()
+        else:
+            x = getattr(datagen, gen)(rng, n)
+            dx = torch.from_numpy(x).cuda()
+        op = getattr(mf, "MFHIP_" + code)
+        if kind == "binary":
+            fn = lambda: mf.dev_binary(op, dx.data_ptr(), dother.data_ptr(), c.data_ptr(), n, stream)
+        else:
+            fn = lambda: mf.dev_unary(op, dx.data_ptr(), c.data_ptr(), n, stream)
+        ms = timed(torch, fn, args.steps, args.warmup)
+        gbs = bpe * n / ms / 1e6
+        rec = {"ms": round(ms, 4), "GBs": round(gbs, 1), "Gelem_s": round(n / ms / 1e6, 2),
+               "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                            "note": "%d algorithmic bytes per element (inputs + output)" % bpe}}
+        if nm in ("exp", "log", "sin", "erf"):
+            rec["roofline"]["bound"] = "fp64"
+            ipe = FP64_INSTR_PER_ELEM.get(nm)
+            if ipe:
+                rec["fp64_instr_per_elem"] = ipe
+                rec["fp64_issue_frac"] = round(ipe * n / (ms * 1e-3) / FP64_ISSUE_PEAK, 4)
+        got = c[:check].cpu().numpy()
+        if nm != "erf":  # bit-identical to the reference (erf: own design, accuracy in the tests)
+            want = nt.or_binary(nm, x[:check], other[:check]) if kind == "binary" else nt.or_unary(nm, x[:check])
+            rec["bit_identical_sample"] = bool(nt.bits_equal(got, want).all())
+        if not args.no_cpu and R is not None:
+            ns = (1 << 22) if kind == "binary" or nm == "sqrt" else (1 << 17)
+            xs = np.ascontiguousarray(x[:ns])
+            ys = np.ascontiguousarray(other[:ns])
+            cs = np.empty_like(xs)
+            t0 = time.perf_counter()
+            if kind == "binary":
+                R.refloop_binary(REF_LOOP_BINARY[nm], xs.ctypes.data, ys.ctypes.data, cs.ctypes.data, ns)
+            else:
+                R.refloop_unary(REF_LOOP_UNARY[nm], xs.ctypes.data, cs.ctypes.data, ns)
+            dt = time.perf_counter() - t0
+            rec["cpu_baseline"] = {"value": round(ns / dt / 1e9, 5), "unit": "G elements/s", "cores": 1,
+                                   "kind": "reference", "ns_per_elem": round(dt / ns * 1e9, 2),
+                                   "sample": "%d elements of the same inputs through the reference's %sdd on one "
+                                             "host core, %.2f s" % (ns, nm, dt)}
+            rec["speedup_vs_cpu_core"] = round((n / ms / 1e6) / (ns / dt / 1e9), 1)
+        res[nm] = rec
+        if gen not in ("config0",):
+            del dx
+    del dbase, c
+    # reductions: dot on 10% near-cancelling pairs, sum on near-cancelling
+    # neighbours, norm2 on config-0 entries
+    a_np, b_np = datagen.cancel_pairs(rng, n)
+    s_np = datagen.sum_inputs(rng, n)
+    work = torch.empty(mf.reduce_workspace_bytes(n) // 8 + 2, dtype=torch.float64, device="cuda")
+    out = torch.empty(2, dtype=torch.float64, device="cuda")
+    O = nt.oracle()
+    for nm, kind, bpe, xs in (("dot", mf.MFHIP_DOT, 32, (a_np, b_np)), ("sum", mf.MFHIP_SUM, 16, (s_np, s_np)),
+                              ("norm2", mf.MFHIP_NORM2, 16, (other, other))):
+        da = torch.from_numpy(xs[0]).cuda()
+        db = torch.from_numpy(xs[1]).cuda() if nm == "dot" else da
+        fn = lambda: mf.dev_reduce(kind, da.data_ptr(), db.data_ptr(), n, out.data_ptr(), work.data_ptr(), 8, stream)
+        ms = timed(torch, fn, args.steps, args.warmup)
+        gbs = bpe * n / ms / 1e6
+        val = out.cpu().numpy()
+        rec = {"ms": round(ms, 4), "GBs": round(gbs, 1),
+               "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                            "note": "%d algorithmic bytes per element" % bpe}}
+        if nm == "dot":
+            e, mg = nt.exact_dot_err(xs[0], xs[1], val)
+            rec["err_over_n_sum_abs_terms"] = e / mg / n
+        elif nm == "sum":
+            e, mg = nt.exact_sum_err(xs[0], val)
+            rec["err_over_n_sum_abs_terms"] = e / mg / n
+        if not args.no_cpu:
+            ns = 1 << 22
+            p0 = np.ascontiguousarray(xs[0][:ns])
+            p1 = np.ascontiguousarray(xs[1][:ns])
+            t0 = time.perf_counter()
+            if nm == "dot":
+                O.or_dot(nt.ptr(p0), nt.ptr(p1), ns, 8)
+            elif nm == "sum":
+                O.or_sum(nt.ptr(p0), ns)
+            else:
+                O.or_norm2(nt.ptr(p0), ns)
+            dt = time.perf_counter() - t0
+            rec["cpu_baseline"] = {"value": round(bpe * ns / dt / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+                                   "sample": "%d elements through the oracle's restatement of the Fortran %s loop "
+                                             "(the reference has no C entry point for it) on one host core, %.3f s"
+                                             % (ns, nm, dt)}
+        res[nm] = rec
+        del da, db
+    hbm_ops = [k_ for k_ in res if res[k_]["roofline"]["bound"] == "hbm"]
+    worst = min(hbm_ops, key=lambda k_: res[k_]["GBs"])
+    out = {"workload": "float64x2 arrays N=2^26 per op: + - * / on config-0 pairs, sqrt on |x|, exp on U[-700,700], "
+                       "log on 10^U(-300,300), sin on U[-1e4,1e4] with 1%% near k pi/2, erf on U[-6,6]; dot on 10%% "
+                       "near-cancelling pairs, sum on near-cancelling neighbours, norm2 on config-0",
+           "value": res[worst]["GBs"], "unit": "GB/s", "value_is": "the slowest HBM-bound op (%s)" % worst,
+           "roofline": res[worst]["roofline"], "ops": res}
+    if "cpu_baseline" in res[worst]:
+        out["cpu_baseline"] = dict(res[worst]["cpu_baseline"], **host_info())
+    return out
+
+
+def bench_c2_complex(args, torch, mf):
+    """complex64x2 GEMM m = n = k = 2048, spread 2^+-20, renorm 8; a sampled
+    block compared bit for bit with the oracle's four-GEMM composition."""
+    from multifloats_amd import datagen
+    from tests import _native as nt
+    m = n = k = 2048
+    rng = np.random.default_rng(3)
+    re_a, im_a = datagen.matrix(rng, m, k, 20), datagen.matrix(rng, m, k, 20)
+    re_b, im_b = datagen.matrix(rng, k, n, 20), datagen.matrix(rng, k, n, 20)
+
+    def pack(re, im):
         return np.ascontiguousarray(np.concatenate([re, im], axis=2).transpose(1, 0, 2)).reshape(-1, 4)
-    dA = torch.from_numpy(cm(m, k)).cuda()
-    dB = torch.from_numpy(cm(k, n)).cuda()
+    dA, dB = torch.from_numpy(pack(re_a, im_a)).cuda(), torch.from_numpy(pack(re_b, im_b)).cuda()
     dC = torch.empty((m * n, 4), dtype=torch.float64, device="cuda")
     stream = torch.cuda.current_stream().cuda_stream
     ms = timed(torch, lambda: mf.dev_cmatmul(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), m, k, n, 8, stream),
-               args.steps, args.warmup)
-    flops = 8.0 * m * n * k
+               max(3, args.steps // 2), args.warmup)
+    C = dC.cpu().numpy().reshape(n, m, 4).transpose(1, 0, 2)
+    rows = rng.choice(m, 8, replace=False)
+    cols = rng.choice(n, 8, replace=False)
+    sa = np.concatenate([re_a[rows], im_a[rows]], axis=2)
+    sb = np.concatenate([re_b[:, cols], im_b[:, cols]], axis=2)
+    O = nt.oracle()
+    Ac = np.ascontiguousarray(sa.transpose(1, 0, 2))
+    Bc = np.ascontiguousarray(sb.transpose(1, 0, 2))
+    Wc = np.zeros((8, 8, 4))
+    O.or_cmatmul_mm(nt.ptr(Ac), nt.ptr(Bc), nt.ptr(Wc), 8, k, 8, 8)
+    want = Wc.transpose(1, 0, 2)
+    got = C[np.ix_(rows, cols)]
     achieved = 4 * FP64_FLOPS_PER_DDMAC * m * n * k / (ms * 1e-3) / 1e12
-    return {"metric": "complex DD-GEMM DD-GFLOP/s", "value": round(flops / (ms * 1e-3) / 1e9, 2),
-            "unit": "DD-GFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-            "ms_per_step": round(ms, 4), "higher_is_better": True, "dtype": "f64", "data": "synthetic",
-            "config": {"workload": "complex64x2 GEMM m=n=k=%d, spread 2^+-%g, renorm 8, bit-identical to the "
-                                   "reference's four-GEMM composition" % (m, args.spread)},
-            "roofline": {"bound": "fp64", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
-                         "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None}}
+    return {"workload": "complex64x2 GEMM m=n=k=2048, spread 2^+-20, renorm 8 (fused: four reference-order chains "
+                        "per output)", "value": round(8.0 * m * n * k / (ms * 1e-3) / 1e9, 2), "unit": "DD-GFLOP/s",
+            "ms": round(ms, 4),
+            "roofline": {"bound": "fp64", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None},
+            "bit_identical_sample_8x8": bool(nt.bits_equal(got, want).all())}
+
+
+def bench_c3(args, torch, mf):
+    """C3: Ozaki 8192^3 (automatic form) at exponent spreads 2^+-10 and 2^+-30,
+    with the plain reference-order GEMM at 8192^3 (spread 2^+-10) beside it;
+    error max / mean on a 64 x 64 block of sampled outputs vs the exact
+    accumulator."""
+    from multifloats_amd import datagen
+    m = n = k = 8192
+    stream = torch.cuda.current_stream().cuda_stream
+    out = {"workload": "float64x2 GEMM m=n=k=8192, entries s*2^U(-e,e)*u (e = 10, 30), lo per config-0; Ozaki INT8 "
+                       "automatic form; plain reference-order form beside it (e = 10)"}
+    flops = 2.0 * m * n * k
+    per = {}
+    for spread in (10, 30):
+        rng = np.random.default_rng(8192 + spread)
+        A = datagen.matrix(rng, m, k, spread)
+        B = datagen.matrix(rng, k, n, spread)
+        dA = torch.from_numpy(datagen.flat(A)).cuda()
+        dB = torch.from_numpy(datagen.flat(B)).cuda()
+        dC = torch.empty((m * n, 2), dtype=torch.float64, device="cuda")
+        r, Cf = ozaki_run(torch, mf, dA, dB, dC, m, k, n, args.steps, args.warmup, stream)
+        rows = rng.choice(m, 64, replace=False)
+        cols = rng.choice(n, 64, replace=False)
+        C = datagen.unflat(Cf, m, n)
+        r["err_max"], r["err_mean"] = sampled_error(A, B, C, rows, cols)
+        r["value"] = round(flops / (r["ms"] * 1e-3) / 1e9, 2)
+        r = {kk: (round(v, 4) if isinstance(v, float) and kk.startswith("ms") else v) for kk, v in r.items()}
+        per["spread_%d" % spread] = r
+        if spread == 10:
+            steps_p = min(args.steps, 3)
+            ms_p = timed(torch, lambda: mf.dev_matmul(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), m, k, n, 8, stream),
+                         steps_p, 1)
+            Cp = datagen.unflat(dC.cpu().numpy(), m, n)
+            emax, emean = sampled_error(A, B, Cp, rows, cols)
+            out["plain"] = {"ms": round(ms_p, 3), "steps": steps_p, "value": round(flops / (ms_p * 1e-3) / 1e9, 2),
+                            "err_max": emax, "err_mean": emean, "roofline": gemm_roofline_plain(m, n, k, ms_p)}
+            if not args.no_cpu:
+                cols_cpu = 4
+                dt, Cc, kind = cpu_matmul(A, B, cols_cpu, 8)
+                from tests import _native as nt
+                same = bool(nt.bits_equal(Cc, Cp[:, :cols_cpu]).all())
+                out["cpu_baseline"] = dict(value=round(2.0 * m * k * cols_cpu / dt / 1e9, 4), unit="DD-GFLOP/s",
+                                           cores=1, kind=kind,
+                                           sample="C[:, :%d] = A(8192x8192) * B[:, :%d] (spread 2^+-10) on one host "
+                                                  "core, %.1f s; the whole product at that rate: %.0f s; "
+                                                  "bit-identical to the GPU's plain form: %s"
+                                                  % (cols_cpu, cols_cpu, dt, dt * n / cols_cpu, same), **host_info())
+            del Cp
+        del A, B, C, dA, dB, dC
+        torch.cuda.empty_cache()
+    out["ozaki"] = per
+    head = per["spread_10"]
+    out["value"] = head["value"]
+    out["unit"] = "DD-GFLOP/s"
+    out["value_is"] = "Ozaki at spread 2^+-10"
+    out["roofline"] = head.get("roofline")
+    out["speedup_ozaki_vs_plain"] = {kk: round(out["plain"]["ms"] / v["ms"], 2) for kk, v in per.items()}
+    out["within_2^-100"] = all(v["err_max"] <= 2.0 ** -100 for v in per.values())
+    return out
+
+
+def bench_c4(args, torch, mf):
+    """C4: the batching form, with a CPU baseline of the same problems."""
+    from multifloats_amd import datagen
+    from tests import _native as nt
+    out, extra = bench_batched(args, torch, mf, keep=True)
+    if not args.no_cpu:
+        sizes, dlen, gb, db, offs, doff = extra
+        rng = np.random.default_rng(44)
+        O = nt.oracle()
+        t_g = t_d = 0.0
+        macs = elems = 0
+        for i in rng.choice(len(sizes), 150, replace=False):
+            m, k, n = (int(v) for v in sizes[i])
+            o = offs[i]
+            A = np.ascontiguousarray(gb[o:o + m * k].cpu().numpy())
+            B = np.ascontiguousarray(gb[o + m * k:o + m * k + k * n].cpu().numpy())
+            C = np.zeros((m * n, 2))
+            t0 = time.perf_counter()
+            O.or_matmul_mm(nt.ptr(A), nt.ptr(B), nt.ptr(C), m, k, n, 8)
+            t_g += time.perf_counter() - t0
+            macs += m * k * n
+        for i in rng.choice(len(dlen), 600, replace=False):
+            L, o = int(dlen[i]), doff[i]
+            a = np.ascontiguousarray(db[o:o + L].cpu().numpy())
+            b = np.ascontiguousarray(db[o + L:o + 2 * L].cpu().numpy())
+            t0 = time.perf_counter()
+            O.or_dot(nt.ptr(a), nt.ptr(b), L, 8)
+            t_d += time.perf_counter() - t0
+            elems += L
+        rate = (2.0 * macs + 2.0 * elems) / (t_g + t_d) / 1e9
+        out["cpu_baseline"] = dict(value=round(rate, 4), unit="DD-GFLOP/s", cores=1, kind="port",
+                                   sample="150 of the GEMMs (%.3g MACs, %.2f s) and 600 of the dots (%d elements, "
+                                          "%.2f s) through the oracle's restatements of matmuldd_mm and the Fortran "
+                                          "dot_product, one problem after another on one host core"
+                                          % (macs, t_g, elems, t_d), **host_info())
+    return out
+
+
+def bench_all(args, torch, mf):
+    t0 = time.perf_counter()
+    head = bench_gemm(args, torch, mf)
+    configs = {}
+    timing = {}
+
+    def run(name, fn):
+        t = time.perf_counter()
+        print("bench: %s ..." % name, file=sys.stderr, flush=True)
+        configs[name] = fn(args, torch, mf)
+        timing[name] = round(time.perf_counter() - t, 1)
+        torch.cuda.empty_cache()
+    c2 = {"workload": head["config"]["workload"], "value": head["value"], "unit": "DD-GFLOP/s",
+          "roofline": head["roofline"], "ozaki": head.get("ozaki"), "plain": head.get("plain"),
+          "speedup_ozaki_vs_plain": head.get("speedup_ozaki_vs_plain")}
+    if "cpu_baseline" in head:
+        c2["cpu_baseline"] = head["cpu_baseline"]
+    run("C0", bench_c0)
+    run("C1", bench_c1)
+    configs["C2"] = c2
+    run("C2_complex", bench_c2_complex)
+    configs["C2"]["complex"] = configs.pop("C2_complex")
+    if not args.quick:
+        run("C3", bench_c3)
+        run("C4", bench_c4)
+    head["configs"] = {k_: configs[k_] for k_ in sorted(configs)}
+    head["bench_seconds"] = dict(timing, total=round(time.perf_counter() - t0, 1))
+    return head
 
 
 def bench_blas(args, torch, mf):
@@ -333,7 +692,7 @@ def bench_trsm(args, torch, mf):
             "vs_baseline": None}
 
 
-def bench_batched(args, torch, mf):
+def bench_batched(args, torch, mf, keep=False):
     """10,000 float64x2 GEMMs (m, n, k log-uniform in [8, 512]) and 50,000
     dot products (length log-uniform in [16, 65536]), config-0 entries, in one
     persistent grouped launch replayed from a HIP graph; per-call launches of
@@ -408,7 +767,7 @@ def bench_batched(args, torch, mf):
         e, mg = nt.exact_dot_err(a, b, dres[i])
         derr = max(derr, e / mg / L)
     plan.close()
-    return {"metric": "batched DD GEMMs + dots, aggregate DD-GFLOP/s", "value": round(dd_flops / (ms * 1e-3) / 1e9, 2),
+    out = {"metric": "batched DD GEMMs + dots, aggregate DD-GFLOP/s", "value": round(dd_flops / (ms * 1e-3) / 1e9, 2),
             "unit": "DD-GFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": round(ms, 4), "higher_is_better": True, "dtype": "f64", "data": "synthetic",
             "config": {"workload": "10000 float64x2 GEMMs m,n,k log-U[8,512] + 50000 dots len log-U[16,65536], "
@@ -420,29 +779,27 @@ def bench_batched(args, torch, mf):
             "gemm_dd_gflops": round(2.0 * macs / (ms * 1e-3) / 1e9, 2),
             "accuracy": {"sampled_gemms_bit_identical_to_reference_order": bits_ok,
                          "gemm_max_err_over_absAB": gerr, "dot_max_err_over_n_sum_abs": derr}}
+    gemm_ops = 2.0 * macs / (ms * 1e-3) / 1e12 * FP64_FLOPS_PER_DDMAC / 2.0
+    out["roofline"] = {"bound": "fp64", "achieved": round(gemm_ops, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                       "frac": round(gemm_ops / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                       "note": "the grouped kernel's GEMM work at %.2f FP64 flops per DD-MAC over the whole graph "
+                               "replay (the dots' %.2f GB of HBM reads share the launch)"
+                               % (FP64_FLOPS_PER_DDMAC, 32.0 * dtot / 1e9)}
+    if keep:
+        return out, (sizes, dlen, gb, db, offs, doff)
+    return out
+
+
 
 
 def bench_elementwise(args, torch, mf):
+    """Every array function at N = 2^26 (C1 covers the hot-path ones)."""
     from multifloats_amd import datagen
     n = 1 << 26
     rng = np.random.default_rng(7)
     stream = torch.cuda.current_stream().cuda_stream
     res = {}
-    a = torch.from_numpy(datagen.config0(rng, n)).cuda()
-    b = torch.from_numpy(datagen.config0(rng, n)).cuda()
-    c = torch.empty_like(a)
-    for nm, op in (("add", mf.MFHIP_ADD), ("mul", mf.MFHIP_MUL), ("div", mf.MFHIP_DIV)):
-        ms = timed(torch, lambda: mf.dev_binary(op, a.data_ptr(), b.data_ptr(), c.data_ptr(), n, stream),
-                   args.steps, args.warmup)
-        res[nm] = dict(ms=round(ms, 4), GBs=round(48.0 * n / ms / 1e6, 1))
-    a.abs_()
-    ms = timed(torch, lambda: mf.dev_unary(mf.MFHIP_SQRT, a.data_ptr(), c.data_ptr(), n, stream), args.steps, args.warmup)
-    res["sqrt"] = dict(ms=round(ms, 4), GBs=round(32.0 * n / ms / 1e6, 1))
-    # two-argument functions on x in (0, 1], y in [-1, 1]: pow takes its 2^(y log2 x) path
-    for nm, op in (("pow", mf.MFHIP_POW), ("hypot", mf.MFHIP_HYPOT), ("fmax", mf.MFHIP_FMAX), ("atan2", mf.MFHIP_ATAN2)):
-        ms = timed(torch, lambda: mf.dev_binary(op, a.data_ptr(), b.data_ptr(), c.data_ptr(), n, stream),
-                   args.steps, args.warmup)
-        res[nm] = dict(ms=round(ms, 4), GBs=round(48.0 * n / ms / 1e6, 1), Gelem_s=round(n / ms / 1e6, 2))
+    c = torch.empty((n, 2), dtype=torch.float64, device="cuda")
     gens = {"exp": datagen.exp_inputs, "log": datagen.log_inputs, "sin": datagen.sin_inputs, "erf": datagen.erf_inputs,
             "cos": datagen.sin_inputs, "tan": datagen.sin_inputs, "exp2": datagen.exp_inputs,
             "expm1": datagen.config0, "log2": datagen.log_inputs, "log10": datagen.log_inputs,
@@ -453,43 +810,18 @@ def bench_elementwise(args, torch, mf):
             "log2p1": datagen.config0, "log10p1": datagen.config0, "sinpi": datagen.erf_inputs,
             "cospi": datagen.erf_inputs, "tanpi": datagen.erf_inputs, "atan": datagen.erf_inputs,
             "asin": datagen.atanh_inputs, "acos": datagen.atanh_inputs}
-    ops = {nm: getattr(mf, "MFHIP_" + nm.upper()) for nm in gens}
     for nm, gen in gens.items():
         x = torch.from_numpy(gen(rng, n)).cuda()
-        ms = timed(torch, lambda: mf.dev_unary(ops[nm], x.data_ptr(), c.data_ptr(), n, stream), args.steps, args.warmup)
+        op = getattr(mf, "MFHIP_" + nm.upper())
+        ms = timed(torch, lambda: mf.dev_unary(op, x.data_ptr(), c.data_ptr(), n, stream), args.steps, args.warmup)
         res[nm] = dict(ms=round(ms, 4), GBs=round(32.0 * n / ms / 1e6, 1), Gelem_s=round(n / ms / 1e6, 2))
         del x
-    best = max(v["GBs"] for k_, v in res.items() if k_ in ("add", "mul", "div", "sqrt"))
-    return {"metric": "elementwise DD arrays, N=2^26", "value": best, "unit": "GB/s", "n_gpus": 1,
-            "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "dtype": "f64",
-            "data": "synthetic", "config": {"workload": "elementwise float64x2 N=2^26"},
-            "roofline": {"bound": "hbm", "achieved": best, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(best / HBM_PEAK_GBS, 4), "traffic": None}, "ops": res}
-
-
-def bench_reduce(args, torch, mf):
-    from multifloats_amd import datagen
-    n = 1 << 26
-    rng = np.random.default_rng(8)
-    a_np, b_np = datagen.cancel_pairs(rng, n)
-    a = torch.from_numpy(a_np).cuda()
-    b = torch.from_numpy(b_np).cuda()
-    out = torch.empty(2, dtype=torch.float64, device="cuda")
-    work = torch.empty(mf.reduce_workspace_bytes(n) // 8 + 2, dtype=torch.float64, device="cuda")
-    stream = torch.cuda.current_stream().cuda_stream
-    res = {}
-    for nm, kind, bytes_per in (("dot", mf.MFHIP_DOT, 32), ("sum", mf.MFHIP_SUM, 16), ("norm2", mf.MFHIP_NORM2, 16),
-                                ("product", mf.MFHIP_PRODUCT, 16), ("maxval", mf.MFHIP_MAXVAL, 16),
-                                ("minval", mf.MFHIP_MINVAL, 16)):
-        ms = timed(torch, lambda: mf.dev_reduce(kind, a.data_ptr(), b.data_ptr(), n, out.data_ptr(),
-                                                work.data_ptr(), 8, stream), args.steps, args.warmup)
-        res[nm] = dict(ms=round(ms, 4), GBs=round(bytes_per * n / ms / 1e6, 1))
-    best = res["dot"]["GBs"]
-    return {"metric": "DD dot_product N=2^26", "value": best, "unit": "GB/s", "n_gpus": 1,
-            "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "dtype": "f64",
-            "data": "synthetic", "config": {"workload": "dot/sum/norm2/product/maxval/minval float64x2 N=2^26"},
-            "roofline": {"bound": "hbm", "achieved": best, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(best / HBM_PEAK_GBS, 4), "traffic": None}, "ops": res}
+    worst = min(res, key=lambda k_: res[k_]["Gelem_s"])
+    return {"metric": "elementwise DD array functions, N=2^26, G elements/s (slowest function)",
+            "value": res[worst]["Gelem_s"], "unit": "G elements/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "every one-argument array function on float64x2, N=2^26", "slowest": worst},
+            "ops": res}
 
 
 def main():
@@ -502,8 +834,9 @@ def main():
     import multifloats_amd as mf
     if not torch.cuda.is_available():
         sys.exit("bench.py: no GPU visible")
-    fn = {"gemm": bench_gemm, "cgemm": bench_cgemm, "blas": bench_blas, "trsm": bench_trsm, "batched": bench_batched,
-          "elementwise": bench_elementwise, "reduce": bench_reduce}[args.workload]
+    fn = {"all": bench_all, "gemm": bench_gemm, "cgemm": lambda a, t, m: bench_c2_complex(a, t, m),
+          "blas": bench_blas, "trsm": bench_trsm, "batched": bench_batched, "elementwise": bench_elementwise,
+          "reduce": lambda a, t, m: bench_c1(a, t, m)}[args.workload]
     print(json.dumps(fn(args, torch, mf)), flush=True)
 
 
