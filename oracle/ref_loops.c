@@ -22,8 +22,14 @@ f2 expdd(f2);
 f2 logdd(f2);
 f2 sindd(f2);
 f2 erfdd(f2);
+f2 erfcdd(f2);
+f2 erfcxdd(f2);
+f2 atandd(f2);
+f2 asindd(f2);
+f2 acosdd(f2);
 
-/* op: 0 sqrt, 1 exp, 2 log, 3 sin, 4 erf (the C ABI's MFHIP_* unary codes) */
+/* op: 0 sqrt, 1 exp, 2 log, 3 sin, 4 erf (the C ABI's MFHIP_* unary codes),
+ * and 16 erfc, 17 erfcx, 29 atan, 31 asin, 32 acos (likewise) */
 int refloop_unary(int op, const f2 *a, f2 *c, int64_t n) {
   f2 (*fn)(f2) = 0;
   switch (op) {
@@ -32,6 +38,11 @@ int refloop_unary(int op, const f2 *a, f2 *c, int64_t n) {
     case 2: fn = logdd; break;
     case 3: fn = sindd; break;
     case 4: fn = erfdd; break;
+    case 16: fn = erfcdd; break;
+    case 17: fn = erfcxdd; break;
+    case 29: fn = atandd; break;
+    case 31: fn = asindd; break;
+    case 32: fn = acosdd; break;
     default: return -1;
   }
   for (int64_t i = 0; i < n; ++i) c[i] = fn(a[i]);

@@ -582,6 +582,27 @@ def cmatmul(L, rng):
     np.savez_compressed(os.path.join(HERE, "cmatmul.npz"), **out)
 
 
+def c0(L):
+    """Config C0: C = A B at m = n = k = 256 with config-0 entries (seed
+    20261204: A then B from datagen.matrix), renorm interval 8, by the
+    compiled reference. Stored compactly: the SHA-256 of all outputs (the
+    column-major float64x2 bytes) and the first column in full."""
+    import hashlib
+    import json
+    rng = np.random.default_rng(20261204)
+    m = n = k = 256
+    A = datagen.as_colmajor(datagen.matrix(rng, m, k))
+    B = datagen.as_colmajor(datagen.matrix(rng, k, n))
+    C = datagen.as_colmajor(np.zeros((m, n, 2)))
+    L.matmuldd_mm(A.ctypes.data, B.ctypes.data, C.ctypes.data, m, k, n, 8)
+    flat = np.ascontiguousarray(datagen.flat(C))
+    out = {"seed": 20261204, "m": m, "k": k, "n": n, "renorm_interval": 8,
+           "sha256": hashlib.sha256(flat.tobytes()).hexdigest(),
+           "first_column": [[float.hex(float(h)), float.hex(float(l))] for h, l in flat[:m]]}
+    with open(os.path.join(HERE, "c0_256.json"), "w") as f:
+        json.dump(out, f, indent=0)
+
+
 def main(which):
     L = load_ref()
     rng = np.random.default_rng(20261203)
@@ -608,9 +629,11 @@ def main(which):
         invtrig(L, np.random.default_rng(8181))
     if not which or "invtrig3" in which:
         invtrig3(L, np.random.default_rng(9191))
+    if not which or "c0" in which:
+        c0(L)
     for f in ("arith.npz", "transc.npz", "matmul.npz", "cmatmul.npz", "transc2.npz", "special2.npz", "transc3.npz",
               "transc4.npz", "transc5.npz", "binary2.npz", "invtrig.npz",
-              "invtrig3.npz"):
+              "invtrig3.npz", "c0_256.json"):
         print(f, os.path.getsize(os.path.join(HERE, f)))
 
 

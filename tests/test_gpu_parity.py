@@ -565,3 +565,16 @@ def test_device_reduction_entry_points(torch_cuda):
     mf.dev_reduce_dim(mf.MFHIP_SUM, dA.data_ptr(), None, 1, m_, k_, dout.data_ptr())
     torch.cuda.synchronize()
     nt.assert_bits(dout.cpu().numpy(), nt.or_reduce_dim("sum", A, None, 1, m_, k_), "device sum dim")
+
+
+@pytest.mark.parametrize("nm", ["erf", "erfc", "erfcx", "atan", "asin", "acos"])
+def test_own_designs_device_against_mpfr(gpu, nm):
+    """The device kernels of the functions of this project's own design
+    against 256-bit MPFR values directly, on the same >= 100k points as the
+    host build's test: within the cap (erf: the reference's published
+    1.55e-32)."""
+    from tests import pin_points as pp
+    x, v, ok = pp.points(nm), pp.exact(nm), pp.checked(nm)
+    got = getattr(mf, nm)(x)
+    rel = nt.rel_err_vs_exact(got[ok], v[ok])
+    assert rel.max() <= pp.CAPS[nm], (nm, rel.max(), x[ok][np.argmax(rel)].tolist())

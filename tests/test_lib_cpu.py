@@ -240,3 +240,23 @@ def test_hostmath_atan_family_special_values_match_reference_bits(nm):
     sp = ~(np.isfinite(ex[:, 0]) & (np.abs(ex[:, 0]) > 1e-290))
     assert sp.sum() > 5
     nt.assert_bits(got[sp], g[nm + "_ref"][sp], nm)
+
+
+@pytest.mark.parametrize("nm", ["erf", "erfc", "erfcx", "atan", "asin", "acos"])
+def test_hostmath_own_designs_against_mpfr(nm):
+    """The host build of the functions of this project's own design against
+    256-bit MPFR values on >= 100k points each (every row and table entry,
+    both sides of each row edge and switch point, tiny and huge arguments):
+    within the cap, and no less accurate than the compiled reference on the
+    same points."""
+    from tests import pin_points as pp
+    L = ctypes.CDLL(nt.HOSTMATH)
+    x, v, ok = pp.points(nm), pp.exact(nm), pp.checked(nm)
+    assert x.shape[0] >= 100000 and ok.sum() >= 95000
+    got = np.empty_like(x)
+    getattr(L, "h_" + nm)(nt.ptr(x), nt.ptr(got), ctypes.c_long(x.shape[0]))
+    ours = nt.rel_err_vs_exact(got[ok], v[ok]).max()
+    assert ours <= pp.CAPS[nm], (nm, ours)
+    ref = pp.reference_outputs(nm)
+    if ref is not None:
+        assert ours <= nt.rel_err_vs_exact(ref[ok], v[ok]).max(), nm
