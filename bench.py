@@ -52,23 +52,33 @@ FP64_FLOPS_PER_DDMAC = 15.75  # 2 MUL + 2 FMA(x2) + 9 ADD + 6/8 renorm at ri = 8
 FP64_INSTR_PER_DDMAC = 13.75  # 2 MUL + 2 FMA + 9 ADD + 6/8 renorm instructions
 INT8_PEAK_TOPS = 5000.0      # MI355X dense INT8 MFMA
 
-# HBM-side bytes per launch from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
-# (KiB per dispatch), scaled by what the same counters read for a 1 GiB
-# device copy: FETCH_SIZE x 2.0 (it reads 524,300 KiB for 1 GiB fetched),
-# WRITE_SIZE x 1.0 (1,048,576 KiB). Keyed by (kernel, size, slices); the
-# summaries and commands are in profiles/pmc_traffic_gemm_4096.md.
+# HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB
+# per dispatch), scaled by what the same counters read for a 1 GiB device
+# copy: FETCH_SIZE x 2.0 (it reads 524,307 KiB for 1 GiB fetched), WRITE_SIZE
+# x 1.0 (1,048,576 KiB). From profiles/hot_kernels_4096_and_2p26_pmc.md
+# (bench.py --workload profile). Keyed by (kernel, size, slices).
 PMC_TRAFFIC = {
-    ("k_ozaki", 4096, 14): 2765231.0 * 2.0 * 1024 + 286820.0 * 1024,
-    ("k_oz2_gemm", 4096, 14): 1682594.0 * 2.0 * 1024 + 599483.0 * 1024,
-    ("k_gemm_plain", 4096, 0): 3144165.19 * 2.0 * 1024 + 286727.06 * 1024,
+    ("k_oz2_gemm", 4096, 14): (2.0 * 1878723 + 608275) * 1024,
+    ("k_gemm_plain", 4096, 0): (2.0 * 3142688 + 262144) * 1024,
+}
+# the C1 element kernels and reduction passes at N = 2^26, same runs
+C1_TRAFFIC = {
+    "add": (2.0 * 1048600 + 1048593) * 1024, "sub": (2.0 * 1048582 + 1048577) * 1024,
+    "mul": (2.0 * 1048586 + 1048576) * 1024, "div": (2.0 * 1048584 + 1048576) * 1024,
+    "sqrt": (2.0 * 524292 + 1048655) * 1024, "exp": (2.0 * 524358 + 1048576) * 1024,
+    "log": (2.0 * 524386 + 1048576) * 1024, "sin": (2.0 * 524378 + 1048576) * 1024,
+    "erf": (2.0 * 524629 + 1048576) * 1024, "dot": (2.0 * 1048626 + 1024) * 1024,
+    "sum": (2.0 * 524334 + 512) * 1024, "norm2": (2.0 * 524372 + 512) * 1024,
 }
 
 # FP64 VALU lane-instructions per element of the transcendental array
 # kernels on the C1 input distributions: (SQ_INSTS_VALU_ADD_F64 +
 # SQ_INSTS_VALU_MUL_F64 + SQ_INSTS_VALU_FMA_F64) x 64 / N from rocprofv3
-# --pmc at N = 2^26 (profiles/elementwise_fp64_counts_2p26.md). With these
-# the FP64-issue fraction is instructions/element x elements/s / 39.3e12.
-FP64_INSTR_PER_ELEM = {}
+# --pmc at N = 2^26 (profiles/hot_kernels_4096_and_2p26_pmc.md). With these
+# the FP64-issue fraction is instructions/element x elements/s / 39.3e12
+# (other FP64 work -- ldexp, rounding, conversions -- is not in these three
+# counters, so the fraction is a lower bound of the FP64 pipe's use).
+FP64_INSTR_PER_ELEM = {"exp": 319.0, "log": 307.3, "sin": 946.0, "erf": 356.0}
 
 
 def parse():
@@ -77,7 +87,7 @@ def parse():
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--workload", default="all",
-                   choices=["all", "gemm", "cgemm", "blas", "trsm", "batched", "elementwise", "reduce"])
+                   choices=["all", "gemm", "cgemm", "blas", "trsm", "batched", "elementwise", "reduce", "profile"])
     p.add_argument("--retune", action="store_true", help="batched: rerun the tile autotuner first")
     p.add_argument("--size", type=int, default=4096)
     p.add_argument("--spread", type=float, default=20.0)
@@ -360,14 +370,20 @@ def bench_c1(args, torch, mf):
         gbs = bpe * n / ms / 1e6
         rec = {"ms": round(ms, 4), "GBs": round(gbs, 1), "Gelem_s": round(n / ms / 1e6, 2),
                "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": C1_TRAFFIC.get(nm),
                             "note": "%d algorithmic bytes per element (inputs + output)" % bpe}}
-        if nm in ("exp", "log", "sin", "erf"):
-            rec["roofline"]["bound"] = "fp64"
-            ipe = FP64_INSTR_PER_ELEM.get(nm)
-            if ipe:
-                rec["fp64_instr_per_elem"] = ipe
-                rec["fp64_issue_frac"] = round(ipe * n / (ms * 1e-3) / FP64_ISSUE_PEAK, 4)
+        ipe = FP64_INSTR_PER_ELEM.get(nm)
+        if ipe:
+            # above the FP64-issue / HBM ridge (~5 instructions per byte): the
+            # FP64 pipe bounds these; the HBM figure stays beside it
+            issue = ipe * n / (ms * 1e-3)
+            rec["hbm_frac"] = rec["roofline"]["frac"]
+            rec["roofline"] = {"bound": "fp64", "achieved": round(issue / 1e12, 3),
+                               "peak": round(FP64_ISSUE_PEAK / 1e12, 1), "unit": "T FP64 instructions/s",
+                               "frac": round(issue / FP64_ISSUE_PEAK, 4), "traffic": C1_TRAFFIC.get(nm),
+                               "note": "%.1f FP64 ADD/MUL/FMA lane-instructions per element (PMC, this input "
+                                       "distribution) x elements/s, against 256 CUs x 64 lanes x 2.4 GHz" % ipe}
+            rec["fp64_issue_frac"] = rec["roofline"]["frac"]
         got = c[:check].cpu().numpy()
         if nm != "erf":  # bit-identical to the reference (erf: own design, accuracy in the tests)
             want = nt.or_binary(nm, x[:check], other[:check]) if kind == "binary" else nt.or_unary(nm, x[:check])
@@ -409,8 +425,9 @@ def bench_c1(args, torch, mf):
         val = out.cpu().numpy()
         rec = {"ms": round(ms, 4), "GBs": round(gbs, 1),
                "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                            "note": "%d algorithmic bytes per element" % bpe}}
+                            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": C1_TRAFFIC.get(nm),
+                            "note": "%d algorithmic bytes per element (first pass; the fold passes read "
+                                    "one partial per 2,048 or 4,096 elements)" % bpe}}
         if nm == "dot":
             e, mg = nt.exact_dot_err(xs[0], xs[1], val)
             rec["err_over_n_sum_abs_terms"] = e / mg / n
@@ -824,6 +841,52 @@ def bench_elementwise(args, torch, mf):
             "ops": res}
 
 
+def bench_profile(args, torch, mf):
+    """One launch of each hot kernel at its full size, for rocprofv3 runs
+    (--kernel-trace --stats, or --pmc counter passes): a 1 GiB device copy
+    (the calibration of the FETCH_SIZE / WRITE_SIZE counters), the Ozaki
+    automatic call and the plain GEMM at 4096^3 (spread 2^+-20), + - * / sqrt
+    exp log sin erf on the C1 inputs, and dot / sum / norm2 at N = 2^26."""
+    from multifloats_amd import datagen
+    s = torch.cuda.current_stream().cuda_stream
+    x = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    x.fill_(1.0)
+    y.copy_(x)  # 1 GiB read + 1 GiB written
+    del x, y
+    m = n = k = 4096
+    rng = np.random.default_rng(2026)
+    dA = torch.from_numpy(datagen.flat(datagen.matrix(rng, m, k, 20))).cuda()
+    dB = torch.from_numpy(datagen.flat(datagen.matrix(rng, k, n, 20))).cuda()
+    dC = torch.empty((m * n, 2), dtype=torch.float64, device="cuda")
+    wb = mf.ozaki_workspace_bytes(m, k, n, 0)
+    work = torch.empty(wb, dtype=torch.uint8, device="cuda")
+    mf.dev_matmul_ozaki(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), m, k, n, 0, work.data_ptr(), wb, s)
+    mf.dev_matmul(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), m, k, n, 8, s)
+    torch.cuda.synchronize()
+    del dA, dB, dC, work
+    n1 = 1 << 26
+    rng = np.random.default_rng(7)
+    a = torch.from_numpy(datagen.config0(rng, n1)).cuda()
+    b = torch.from_numpy(datagen.config0(rng, n1)).cuda()
+    c = torch.empty_like(a)
+    for code in ("ADD", "SUB", "MUL", "DIV"):
+        mf.dev_binary(getattr(mf, "MFHIP_" + code), a.data_ptr(), b.data_ptr(), c.data_ptr(), n1, s)
+    a.abs_()
+    mf.dev_unary(mf.MFHIP_SQRT, a.data_ptr(), c.data_ptr(), n1, s)
+    for nm in ("exp", "log", "sin", "erf"):
+        x = torch.from_numpy(getattr(datagen, nm + "_inputs")(rng, n1)).cuda()
+        mf.dev_unary(getattr(mf, "MFHIP_" + nm.upper()), x.data_ptr(), c.data_ptr(), n1, s)
+        torch.cuda.synchronize()
+        del x
+    work = torch.empty(mf.reduce_workspace_bytes(n1) // 8 + 2, dtype=torch.float64, device="cuda")
+    out = torch.empty(2, dtype=torch.float64, device="cuda")
+    for kind in (mf.MFHIP_DOT, mf.MFHIP_SUM, mf.MFHIP_NORM2):
+        mf.dev_reduce(kind, a.data_ptr(), b.data_ptr(), n1, out.data_ptr(), work.data_ptr(), 8, s)
+    torch.cuda.synchronize()
+    return {"metric": "profile run (one launch of each hot kernel)", "value": None, "unit": None}
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -836,7 +899,7 @@ def main():
         sys.exit("bench.py: no GPU visible")
     fn = {"all": bench_all, "gemm": bench_gemm, "cgemm": lambda a, t, m: bench_c2_complex(a, t, m),
           "blas": bench_blas, "trsm": bench_trsm, "batched": bench_batched, "elementwise": bench_elementwise,
-          "reduce": lambda a, t, m: bench_c1(a, t, m)}[args.workload]
+          "reduce": lambda a, t, m: bench_c1(a, t, m), "profile": bench_profile}[args.workload]
     print(json.dumps(fn(args, torch, mf)), flush=True)
 
 
