@@ -58,6 +58,20 @@ __global__ __launch_bounds__(kThreads) void k_scale_c(double2 *__restrict__ C, i
   }
 }
 
+// p[0..n) = 0 (the k = 0 products): a fill kernel rather than a memset node,
+// so the call stays safe to record into a HIP graph.
+__global__ __launch_bounds__(kThreads) void k_zero(double *__restrict__ p, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (t < n) p[t] = 0.0;
+}
+
+hipError_t zero_doubles(double *p, int64_t n, hipStream_t s) {
+  const int64_t g = (n + kThreads - 1) / kThreads;
+  if (g > 0x7fffffff) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)g), dim3(kThreads), 0, s, p, n);
+  return hipGetLastError();
+}
+
 // y(m) = A(m,k) x(k): one thread per row, A read down its columns (coalesced),
 // x broadcast. Reference matmul.inc:207-211 via gaxpy_mv_panel (47-88).
 __global__ __launch_bounds__(kThreads) void k_gemv(const double2 *__restrict__ A,
@@ -293,13 +307,25 @@ hipError_t launch_gemm(const double2 *A, const double2 *B, double2 *C, int64_t m
 
 namespace mfhip {
 
-int gemm_plain_variants() { return 7; }
+int gemm_plain_variants() { return 8; }
+
+// Tile rule. Every output is one sequential chain, so a call has exactly m*n
+// chains of work to spread: 64x64 tiles (16 chains per lane, the most
+// in-flight work per wave) once they give at least 1024 workgroups (4 per
+// CU), else 32x32 tiles once those give 512, else 16x16 tiles (one chain per
+// lane; 256^2 outputs then fill all 256 CUs instead of 16).
+int plain_variant_for(int64_t m, int64_t n) {
+  const int64_t t64 = ((m + 63) / 64) * ((n + 63) / 64), t32 = ((m + 31) / 32) * ((n + 31) / 32);
+  if (t64 >= 1024) return 1;
+  if (t32 >= 512) return 0;
+  return 7;
+}
 
 hipError_t gemm_plain(const double2 *A, const double2 *B, double2 *C, int64_t m, int64_t k,
                       int64_t n, int64_t ri, int variant, hipStream_t s) {
   if (m <= 0 || n <= 0) return hipSuccess;
-  if (k <= 0) return hipMemsetAsync(C, 0, sizeof(double2) * m * n, s);
-  if (variant < 0) variant = (m * n >= (int64_t)256 * 256) ? 1 : 0;
+  if (k <= 0) return zero_doubles((double *)C, 2 * m * n, s);
+  if (variant < 0) variant = plain_variant_for(m, n);
   switch (variant) {
     case 0: return launch_gemm<2, 2, 16, 4>(A, B, C, m, k, n, ri, s);
     case 1: return launch_gemm<4, 4, 16, 2>(A, B, C, m, k, n, ri, s);
@@ -307,7 +333,8 @@ hipError_t gemm_plain(const double2 *A, const double2 *B, double2 *C, int64_t m,
     case 3: return launch_gemm<4, 4, 16, 1>(A, B, C, m, k, n, ri, s);
     case 4: return launch_gemm<4, 8, 16, 1>(A, B, C, m, k, n, ri, s);
     case 5: return launch_gemm<8, 4, 16, 1>(A, B, C, m, k, n, ri, s);
-    default: return launch_gemm<4, 8, 16, 2>(A, B, C, m, k, n, ri, s);
+    case 6: return launch_gemm<4, 8, 16, 2>(A, B, C, m, k, n, ri, s);
+    default: return launch_gemm<1, 1, 16, 4>(A, B, C, m, k, n, ri, s);
   }
 }
 
@@ -329,9 +356,11 @@ hipError_t launch_blas(const double2 *A, int64_t lda, const double2 *B, int64_t 
 template <bool TA, bool TB>
 hipError_t blas_by_size(const double2 *A, int64_t lda, const double2 *B, int64_t ldb, double2 *C, int64_t ldc,
                         int64_t m, int64_t k, int64_t n, int64_t ri, ddv alpha, ddv beta, hipStream_t s) {
-  if (m * n >= (int64_t)256 * 256)
-    return launch_blas<4, 4, 2, TA, TB>(A, lda, B, ldb, C, ldc, m, k, n, ri, alpha, beta, s);
-  return launch_blas<2, 2, 4, TA, TB>(A, lda, B, ldb, C, ldc, m, k, n, ri, alpha, beta, s);
+  switch (plain_variant_for(m, n)) {
+    case 1: return launch_blas<4, 4, 2, TA, TB>(A, lda, B, ldb, C, ldc, m, k, n, ri, alpha, beta, s);
+    case 0: return launch_blas<2, 2, 4, TA, TB>(A, lda, B, ldb, C, ldc, m, k, n, ri, alpha, beta, s);
+    default: return launch_blas<1, 1, 4, TA, TB>(A, lda, B, ldb, C, ldc, m, k, n, ri, alpha, beta, s);
+  }
 }
 bool trans_ok(char t) { return t == 'N' || t == 'n' || t == 'T' || t == 't' || t == 'C' || t == 'c'; }
 }  // namespace
@@ -373,7 +402,7 @@ hipError_t gemm_blas(char ta, char tb, int64_t m, int64_t n, int64_t k, ddv alph
 hipError_t gemm_cplx(const double4 *A, const double4 *B, double4 *C, int64_t m, int64_t k,
                      int64_t n, int64_t ri, hipStream_t s) {
   if (m <= 0 || n <= 0) return hipSuccess;
-  if (k <= 0) return hipMemsetAsync(C, 0, sizeof(double4) * m * n, s);
+  if (k <= 0) return zero_doubles((double *)C, 4 * m * n, s);
   constexpr int TM = 2, TN = 2, BK = 16;
   const int64_t tm = (m + 16 * TM - 1) / (16 * TM), tn = (n + 16 * TN - 1) / (16 * TN);
   if (tm * tn > 0x7fffffff) return hipErrorInvalidValue;
