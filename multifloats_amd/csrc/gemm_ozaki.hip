@@ -29,23 +29,13 @@
 #include "dd_core.h"
 #include "dd_tables.h"
 #include "mf_internal.h"
+#include "ozaki_shared.h"
 
 #include <math.h>
 
 namespace {
 
-typedef int v4i __attribute__((ext_vector_type(4)));
-typedef int v16i __attribute__((ext_vector_type(16)));
-
-constexpr int kT = 64;        // output tile edge (rows of A / cols of B per block)
-constexpr int kK = 32;        // K block (one MFMA K step)
-constexpr int kChunk = kT * kK;  // bytes of one slice tile (64 x 32)
-constexpr int kLdsSlice = kChunk;  // LDS image of a slice tile = its global image
-constexpr int kThreads = 256;  // pre-pass kernels
-constexpr int kWaves = 4;       // main kernel: 4 waves, 1 per SIMD
-constexpr int kMainThreads = 64 * kWaves;
-constexpr int kEmpty = -2147483647 - 1;  // exponent of an all-zero row/column
-constexpr int kNonFinite = 2147483647;   // a row/column holding Inf or NaN
+using namespace mfhip_oz;
 
 __device__ __forceinline__ int dexp(double x) {
   int e;
@@ -207,15 +197,6 @@ __device__ __forceinline__ void digits(double h, double l, int S, int *d) {
 }
 
 __device__ __forceinline__ int scale_of(int e) { return (e == kEmpty || e == kNonFinite) ? 0 : e; }
-
-// Byte offset of element (r, c) in a 64 x 32 slice tile. Row r is 32 bytes;
-// its two 16-byte halves are swapped when bit 2 xor bit 3 of r is set, so the
-// 16-byte fragment reads of 8 or 16 consecutive lanes (rows r..r+7 or
-// r..r+15, same half) land on distinct LDS banks.
-__device__ __forceinline__ int swz(int r) { return ((r >> 2) ^ (r >> 3)) & 1; }
-__device__ __forceinline__ int tile_off(int r, int c) {
-  return r * kK + ((((c >> 4) ^ swz(r))) << 4) + (c & 15);
-}
 
 // Slice a 64-row x 32-k tile of A. Element (r, c) of the tile is A(i0+r, p0+c);
 // its digit s goes to out[((ib*KB + kb)*S + s)*2048 + tile_off(r, c)].
@@ -469,46 +450,6 @@ hipError_t launch_main(const signed char *As, const signed char *Bs, const int *
 // The truncated levels of the triangular form are not dropped here, so the
 // error is the digit representation's alone (no larger than the triangular
 // form's with the same S).
-constexpr int kO2Tile = 256;  // modular GEMM output tile edge (4 x 4 blocks of 64)
-constexpr int kO2KS = 2;      // K blocks per stage
-
-struct Oz2Consts {
-  double mp_hi, mp_lo;  // M 2^-(16(S-1)) as a double-double
-};
-
-// Weights of the residue pass: byte j of w[t] (16 bytes) is f_t 256^j mod m_t,
-// balanced into int8, and zero for j >= S, so that f_t X mod m_t ==
-// sum_j e_j w_tj for X = sum_j e_j 256^j. f_t = 1 for A;
-// for B, f_t = y_t = (M / m_t)^-1 mod m_t, so that the residue GEMM sums
-// straight to the CRT digit c_t y_t.
-struct Oz2Weights {
-  int w[MF_OZ2_MODULI][4];
-  float inv[MF_OZ2_MODULI];  // RN(1 / m_t)
-};
-
-// Slice counts the certificate may ask for: kMinSlices..kMaxSlices, and
-// kOverCap for an output that no count up to kMaxSlices certifies.
-constexpr int kMinSlices = 7;
-constexpr int kMaxSlices = 16;
-constexpr int kOverCap = kMaxSlices + 1;
-constexpr int kOzBins = kOverCap + 1;
-
-// Control block of one call of the CRT form, in the workspace. The plan
-// kernels write it and every later kernel reads it, so the slice count S and
-// the moduli count N can be chosen on the device from the data (no host
-// round trip; the call can be captured in a graph).
-struct OzCtl {
-  int S;        // slices of the residues in the workspace (and of the products)
-  int N;        // moduli of the products and the reconstruction
-  int redo;     // 1: the certificate raised S, the residues are formed again
-  int S_first;  // slices of the first residue pass (0: explicit slice count)
-  unsigned long long hist[kOzBins];  // candidate outputs by the slices they need
-  unsigned long long ncand;          // outputs in the candidate list
-  unsigned long long nfix;           // outputs the fix-up recomputed
-  Oz2Consts cs;
-  Oz2Weights wa, wb;  // residue weights of A (f_t = 1) and of B (f_t = y_t)
-};
-
 // Moduli count of the CRT form for each slice count, computed once on the
 // host for the call's k (the counts the workspace is sized for).
 struct OzNtab {
@@ -555,7 +496,8 @@ __device__ __forceinline__ unsigned bal_byte(int p, int m) {
 
 // Residue weights and reconstruction constants of S slices and N moduli into
 // ctl: thread t (< 64) fills modulus t; byte j of w[t] is f_t 256^j mod m_t
-// for j < S, zero above.
+// for j < S, zero above, except byte 15 = 64 when S <= 15 (k_oz2_resid's
+// bias slot).
 __device__ void oz_set_plan(OzCtl *ctl, int S, int N, int t) {
   if (t < MF_OZ2_MODULI) {
     unsigned wa[4] = {0, 0, 0, 0}, wb[4] = {0, 0, 0, 0};
@@ -568,6 +510,10 @@ __device__ void oz_set_plan(OzCtl *ctl, int S, int N, int t) {
         wb[j / 4] |= bal_byte(pb, mt) << (8 * (j % 4));
         pa = pa * 256 % mt;
         pb = pb * 256 % mt;
+      }
+      if (S <= 15) {  // slot 15: the residue pass's bias of 128 (digit 2 x weight 64)
+        wa[3] |= 64u << 24;
+        wb[3] |= 64u << 24;
       }
       inv = 1.0f / (float)mt;
     }
@@ -633,171 +579,6 @@ __global__ __launch_bounds__(64) void k_oz_plan_final(OzCtl *__restrict__ ctl, O
       ctl->redo = 1;
     }
     oz_set_plan(ctl, S, nt.n[S], t);
-  }
-}
-
-// Balanced residue of v (|v| <= 2^18) modulo m <= 256 as a signed byte:
-// v - RN(v / m) m, in [-(m-1)/2, (m-1)/2] for odd m and [-128, 128] for 256
-// (+128 wraps to -128, the same residue). The float quotient is exact: v / m
-// is an exact binary fraction for m = 256 and lies at least 1/(2m) from a
-// half-integer otherwise, beyond the product's error (|v| 2^-23 / m); the fma
-// is exact (an integer below 2^24).
-__device__ __forceinline__ unsigned bal_mod_byte(int v, int m, float inv) {
-  const float vf = (float)v;
-  return (unsigned)(int)__builtin_fmaf(-rintf(vf * inv), (float)m, vf) & 255u;
-}
-
-// Two's complement 128-bit value (lo, hi) of an integer-valued double D,
-// |D| < 2^127: |D| = mant 2^sh with -52 <= sh <= 74, and the bits that a right
-// shift drops are zero.
-__device__ __forceinline__ void int_double_to_i128(double D, uint64_t &lo, uint64_t &hi) {
-  const uint64_t bits = (uint64_t)__double_as_longlong(D);
-  const int e = (int)((bits >> 52) & 0x7FF);
-  const uint64_t mant = (bits & 0xFFFFFFFFFFFFFull) | (1ull << 52);
-  const int sh = e - 1075;
-  uint64_t l = 0, h = 0;
-  if (e != 0) {
-    if (sh <= 0) {
-      l = mant >> (-sh);
-    } else if (sh < 64) {
-      l = mant << sh;
-      h = mant >> (64 - sh);
-    } else {
-      h = mant << (sh - 64);
-    }
-  }
-  if (bits >> 63) {
-    l = ~l + 1;
-    h = ~h + (l == 0 ? 1 : 0);
-  }
-  lo = l;
-  hi = h;
-}
-
-// Magnitude of a certificate operand: |h| 2^-e (0 <= x < 2 here) as a bf16
-// rounded toward zero, so it never exceeds the element's magnitude: RN to
-// float, stepped down when that rounded up, then the top 16 bits.
-__device__ __forceinline__ unsigned bf16_floor(double x) {
-  float f = (float)x;
-  if ((double)f > x) f = __uint_as_float(__float_as_uint(f) - 1u);
-  return __float_as_uint(f) >> 16;
-}
-
-// Residue planes of A (or of B with TRANS): plane t holds the 64 x 32 chunks
-// of tile row block ib (TB of them, padded to whole 256-row tiles) and K
-// block kb at ((t TB + ib) KB + kb) 2048 in the slices' swizzled layout.
-// Thread (lane, wave) owns row r = lane (a row of A, a column of B) and the 8
-// consecutive k-elements c = 8 wave .. 8 wave + 7 of the chunk, whose residues
-// are 8 adjacent bytes of the layout: one 8-byte store per modulus. A is read
-// directly (lanes walk down a column: coalesced); B's chunk is staged through
-// LDS (lanes walk down its columns) with a padded row of 33 elements.
-// Each element's integer X = RN(x 256^(S-1)) (x = a 2^(6-E), |x| < 127) is
-// formed exactly in 128 bits from the integer part of H = h 2^P and the rounded
-// rest RN(frac(H) + l 2^P), P = 6 - E + 8(S-1). Its balanced base-256 digits
-// e_j in [-128, 127] (X = sum_j e_j 256^j, j < S; |X| <= 127 256^(S-1) + 1/2
-// is within their range) are the bytes of (X + K) xor K, K = 128 sum_{j<S}
-// 256^j: X + K = sum_j (e_j + 128) 256^j is the plain base-256 expansion.
-// These are the digits of the triangular form's slicing (the same integer,
-// up to the rounding of an exact tie).
-// S, N and the weights come from the call's control block. pass 1 is the
-// second residue pass of an automatic call: it runs only when the plan raised
-// S (ctl->redo). With mag set (first pass of an automatic call) the kernel
-// also writes the certificate operand -- |hi| 2^-E as bf16 rounded down, in
-// the same chunk layout with 16 k-elements per 32-byte row (K block 2 kb + c0 /
-// 16) -- and adds an upper bound of sum_p (|hi| + |lo|) 2^-E over its chunk,
-// in units of 2^-40 rounded up, to sums[row] (integer atomics: the total does
-// not depend on their order).
-template <bool TRANS>
-__global__ __launch_bounds__(kThreads) void k_oz2_resid(const double2 *__restrict__ X, int64_t rows,
-                                                        int64_t k, const int *__restrict__ Ex, int64_t KB,
-                                                        int64_t TB, const OzCtl *__restrict__ ctl, int pass,
-                                                        signed char *__restrict__ out,
-                                                        unsigned char *__restrict__ mag,
-                                                        unsigned long long *__restrict__ sums) {
-  if (pass == 1 && !ctl->redo) return;  // uniform: one control word for the grid
-  constexpr int kPad = kK + 1;
-  __shared__ double2 stage[TRANS ? kT * kPad : 1];
-  __shared__ unsigned long long rsum[kThreads / 64][kT];
-  const int S = ctl->S, N = ctl->N;
-  const Oz2Weights *__restrict__ wt = TRANS ? &ctl->wb : &ctl->wa;
-  const int ib = blockIdx.x, kb = blockIdx.y;
-  const int tid = threadIdx.x, r = tid & 63, c0 = 8 * (tid >> 6);
-  const int64_t i = (int64_t)ib * kT + r;
-  if (TRANS) {
-#pragma unroll
-    for (int q = 0; q < kChunk / kThreads; ++q) {
-      const int e = tid + q * kThreads, pp = e % kK, col = e / kK;
-      const int64_t gi = (int64_t)ib * kT + col, gp = (int64_t)kb * kK + pp;
-      stage[col * kPad + pp] = (gi < rows && gp < k) ? X[gp + gi * k] : make_double2(0.0, 0.0);
-    }
-    __syncthreads();
-  }
-  const int Ei = i < rows ? Ex[i] : kEmpty;
-  const bool live = Ei != kEmpty && Ei != kNonFinite;
-  const int P = live ? 6 - Ei + 8 * (S - 1) : 0;
-  const uint64_t k80 = 0x8080808080808080ull;
-  const uint64_t K0 = S >= 8 ? k80 : k80 & ((1ull << (8 * S)) - 1);
-  const uint64_t K1 = S <= 8 ? 0ull : S >= 16 ? k80 : k80 & ((1ull << (8 * (S - 8))) - 1);
-  int dp[8][4];
-  unsigned mg[4] = {0, 0, 0, 0};
-  unsigned long long part = 0;
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const int64_t p = (int64_t)kb * kK + c0 + q;
-    double2 v = make_double2(0.0, 0.0);
-    if (TRANS) v = stage[r * kPad + c0 + q];
-    else if (i < rows && p < k) v = X[i + p * rows];
-    if (mag && live) {
-      const double ah = fabs(v.x);
-      mg[q >> 1] |= bf16_floor(__builtin_ldexp(ah, -Ei)) << (16 * (q & 1));
-      // (|hi| + |lo|) 2^(40-E) < 2^41; + 1 covers the rounding of the sum
-      part += (unsigned long long)__builtin_ceil(__builtin_ldexp(ah + fabs(v.y), 40 - Ei)) + (ah != 0.0);
-    }
-    const double H = live ? __builtin_ldexp(v.x, P) : 0.0;
-    const double L = live ? __builtin_ldexp(v.y, P) : 0.0;
-    const double Hi = __builtin_trunc(H);
-    const double Lr = __builtin_rint((H - Hi) + L);
-    uint64_t a0, a1, b0, b1;
-    int_double_to_i128(Hi, a0, a1);
-    int_double_to_i128(Lr, b0, b1);
-    uint64_t lo = a0 + b0, hi = a1 + b1 + (lo < a0 ? 1 : 0);
-    const uint64_t lk = lo + K0;
-    hi = (hi + K1 + (lk < lo ? 1 : 0)) ^ K1;
-    lo = lk ^ K0;
-    dp[q][0] = (int)(uint32_t)lo;
-    dp[q][1] = (int)(uint32_t)(lo >> 32);
-    dp[q][2] = (int)(uint32_t)hi;
-    dp[q][3] = (int)(uint32_t)(hi >> 32);
-  }
-  if (mag) {
-    // 8 bf16 = one 16-byte half of a 32-byte chunk row of K block 2 kb + c0 / 16
-    unsigned char *dm = mag + ((int64_t)ib * (2 * KB) + 2 * kb + (c0 >> 4)) * kChunk + tile_off(r, 2 * (c0 & 15));
-    *(v4i *)dm = v4i{(int)mg[0], (int)mg[1], (int)mg[2], (int)mg[3]};
-    rsum[tid >> 6][r] = part;
-    __syncthreads();
-    if (tid < kT) {
-      const unsigned long long tot = rsum[0][tid] + rsum[1][tid] + rsum[2][tid] + rsum[3][tid];
-      if (tot && i < rows) atomicAdd(&sums[i], tot);
-    }
-  }
-  signed char *dst0 = out + ((int64_t)ib * KB + kb) * kChunk + tile_off(r, c0);
-  const int64_t plane = TB * KB * kChunk;
-  for (int t = 0; t < N; ++t) {
-    const int mt = mf_oz2_m[t];
-    const float inv = wt->inv[t];
-    const int w0 = wt->w[t][0], w1 = wt->w[t][1], w2 = wt->w[t][2], w3 = wt->w[t][3];
-    unsigned lo = 0, hi = 0;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      int v = __builtin_amdgcn_sdot4(dp[q][0], w0, 0, false);
-      v = __builtin_amdgcn_sdot4(dp[q][1], w1, v, false);
-      v = __builtin_amdgcn_sdot4(dp[q][2], w2, v, false);
-      v = __builtin_amdgcn_sdot4(dp[q][3], w3, v, false);
-      const unsigned byte = bal_mod_byte(v, mt, inv);
-      if (q < 4) lo |= byte << (8 * q);
-      else hi |= byte << (8 * (q - 4));
-    }
-    *(uint2 *)(dst0 + t * plane) = make_uint2(lo, hi);
   }
 }
 
@@ -1477,10 +1258,8 @@ void oz_scan(const double2 *A, const double2 *B, int64_t m, int64_t k, int64_t n
 void oz2_resid(const double2 *A, const double2 *B, int64_t m, int64_t k, int64_t n, const OzWork &w,
                const Oz2Work &v, int pass, hipStream_t s) {
   const bool mag = pass == 0 && v.magA != nullptr;
-  hipLaunchKernelGGL(k_oz2_resid<false>, dim3((unsigned)v.d.TBA, (unsigned)v.d.KBp), dim3(kThreads), 0, s, A, m, k,
-                     w.E, v.d.KBp, v.d.TBA, w.ctl, pass, v.As, mag ? v.magA : nullptr, w.rsum);
-  hipLaunchKernelGGL(k_oz2_resid<true>, dim3((unsigned)v.d.TBB, (unsigned)v.d.KBp), dim3(kThreads), 0, s, B, n, k,
-                     w.F, v.d.KBp, v.d.TBB, w.ctl, pass, v.Bs, mag ? v.magB : nullptr, w.csum);
+  oz2_resid_launch(false, s, A, m, k, w.E, v.d.KBp, v.d.TBA, w.ctl, pass, v.As, mag ? v.magA : nullptr, w.rsum);
+  oz2_resid_launch(true, s, B, n, k, w.F, v.d.KBp, v.d.TBB, w.ctl, pass, v.Bs, mag ? v.magB : nullptr, w.csum);
 }
 
 Oz2GemmArgs oz2_args(const OzWork &w, const Oz2Work &v, int64_t m, int64_t n) {
