@@ -143,6 +143,21 @@ MULTIFLOATS_API void atan2pidd_v(const float64x2 *a, const float64x2 *b, float64
 MULTIFLOATS_API void mindd_v(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n);
 MULTIFLOATS_API void maxdd_v(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n);
 MULTIFLOATS_API void dimdd_v(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n);
+/* Array forms of fmoddd and modulodd (float64x2.h:434-440, 2010-2072; the
+ * Fortran mod and modulo, fsrc/multifloats.fypp:1562-1570): the reference's
+ * reduction loop step for step, so every call on which that loop returns
+ * gives its bits. The reference's loop never returns for some operands (a
+ * divisor of -0, subnormal or tiny divisors of large dividends, quotients
+ * beyond the double range: its remainder comes back to a value it held);
+ * these give NaN here, found by a cycle search over the loop's states within
+ * at most 512 steps (returning calls take a few). */
+MULTIFLOATS_API void fmoddd_v(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n);
+MULTIFLOATS_API void modulodd_v(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n);
+/* Array form of the Fortran nearest(a, b) on real64x2 (the module's body,
+ * fsrc/multifloats.fypp:1595-1607): the next double after a.hi toward +inf
+ * when b.hi > 0, else toward -inf (gfortran's NEAREST, nextafter toward that
+ * infinity), low limb 0. */
+MULTIFLOATS_API void nearestdd_v(const float64x2 *a, const float64x2 *b, float64x2 *c, int64_t n);
 /* Array forms of asindd, acosdd, asinpidd, acospidd (float64x2.h:485-505;
  * the Fortran asin, acos, asinpi, acospi). Own design (the reference
  * evaluates libquadmath's fitted rationals, src/float64x2/inv_trig.inc:56-130):
@@ -329,7 +344,7 @@ enum {
 enum {
   MFHIP_ADD = 0, MFHIP_SUB = 1, MFHIP_MUL = 2, MFHIP_DIV = 3, MFHIP_POW = 4, MFHIP_HYPOT = 5, MFHIP_FDIM = 6,
   MFHIP_FMIN = 7, MFHIP_FMAX = 8, MFHIP_COPYSIGN = 9, MFHIP_ATAN2 = 10, MFHIP_ATAN2PI = 11,
-  MFHIP_MINF = 12, MFHIP_MAXF = 13, MFHIP_DIMF = 14
+  MFHIP_MINF = 12, MFHIP_MAXF = 13, MFHIP_DIMF = 14, MFHIP_FMOD = 15, MFHIP_MODULO = 16, MFHIP_NEAREST = 17
 };
 enum { MFHIP_DOT = 0, MFHIP_SUM = 1, MFHIP_NORM2 = 2, MFHIP_PRODUCT = 3, MFHIP_MAXVAL = 4, MFHIP_MINVAL = 5 };
 

@@ -38,7 +38,8 @@ MFHIP_SQRT, MFHIP_EXP, MFHIP_LOG, MFHIP_SIN, MFHIP_ERF, MFHIP_NEG = range(6)
  MFHIP_EXP10M1, MFHIP_LOG2P1, MFHIP_LOG10P1, MFHIP_SINPI, MFHIP_COSPI, MFHIP_TANPI, MFHIP_ATAN, MFHIP_ATANPI,
  MFHIP_ASIN, MFHIP_ACOS, MFHIP_ASINPI, MFHIP_ACOSPI) = range(6, 35)
 (MFHIP_ADD, MFHIP_SUB, MFHIP_MUL, MFHIP_DIV, MFHIP_POW, MFHIP_HYPOT, MFHIP_FDIM, MFHIP_FMIN, MFHIP_FMAX,
- MFHIP_COPYSIGN, MFHIP_ATAN2, MFHIP_ATAN2PI, MFHIP_MINF, MFHIP_MAXF, MFHIP_DIMF) = range(15)
+ MFHIP_COPYSIGN, MFHIP_ATAN2, MFHIP_ATAN2PI, MFHIP_MINF, MFHIP_MAXF, MFHIP_DIMF, MFHIP_FMOD, MFHIP_MODULO,
+ MFHIP_NEAREST) = range(18)
 MFHIP_DOT, MFHIP_SUM, MFHIP_NORM2, MFHIP_PRODUCT, MFHIP_MAXVAL, MFHIP_MINVAL = range(6)
 NOALLOC_MAX_N = 1 << 30  # MFHIP_NOALLOC_MAX_N
 
@@ -70,7 +71,8 @@ def lib():
             getattr(L, nm).argtypes = [P, P, P, I64, I64, I64]
             getattr(L, nm).restype = None
         for nm in ("adddd_v", "subdd_v", "muldd_v", "divdd_v", "powdd_v", "hypotdd_v", "fdimdd_v", "fmindd_v",
-                   "fmaxdd_v", "copysigndd_v", "atan2dd_v", "atan2pidd_v", "mindd_v", "maxdd_v", "dimdd_v"):
+                   "fmaxdd_v", "copysigndd_v", "atan2dd_v", "atan2pidd_v", "mindd_v", "maxdd_v", "dimdd_v",
+                   "fmoddd_v", "modulodd_v", "nearestdd_v"):
             getattr(L, nm).argtypes = [P, P, P, I64]
             getattr(L, nm).restype = None
         for nm in ("sqrtdd_v", "expdd_v", "logdd_v", "sindd_v", "erfdd_v", "erfcdd_v", "erfcxdd_v", "asinhdd_v",
@@ -426,6 +428,9 @@ def atan2pi(y, x): return _binary("atan2pidd_v", y, x)
 def fortran_min(a, b): return _binary("mindd_v", a, b)  # Fortran min(a, b) on real64x2
 def fortran_max(a, b): return _binary("maxdd_v", a, b)
 def fortran_dim(a, b): return _binary("dimdd_v", a, b)
+def fmod(a, b): return _binary("fmoddd_v", a, b)  # C fmod; Fortran mod
+def modulo(a, b): return _binary("modulodd_v", a, b)  # Fortran modulo
+def nearest(a, b): return _binary("nearestdd_v", a, b)  # Fortran nearest
 
 
 def _res(r):

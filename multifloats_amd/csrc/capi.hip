@@ -513,6 +513,9 @@ MF_BINARY(atan2pidd_v, mfhip::OP_ATAN2PI)
 MF_BINARY(mindd_v, mfhip::OP_MINF)
 MF_BINARY(maxdd_v, mfhip::OP_MAXF)
 MF_BINARY(dimdd_v, mfhip::OP_DIMF)
+MF_BINARY(fmoddd_v, mfhip::OP_FMOD)
+MF_BINARY(modulodd_v, mfhip::OP_MODULO)
+MF_BINARY(nearestdd_v, mfhip::OP_NEAREST)
 MF_UNARY(sqrtdd_v, mfhip::OP_SQRT)
 MF_UNARY(expdd_v, mfhip::OP_EXP)
 MF_UNARY(logdd_v, mfhip::OP_LOG)

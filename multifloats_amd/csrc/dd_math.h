@@ -17,6 +17,9 @@
 //   dd_pow   exp_log.inc:531-600 with powi (float64x2.h:2077-2102);
 //   dd_hypot, dd_fdim, dd_fmin, dd_fmax, dd_copysign  float64x2.h:1299-1309,
 //           1518-1520, 2130-2136, 2270-2303
+//   dd_fmod, dd_modulo  float64x2.h:2010-2072 (a bounded form of the
+//           reference's reduction loop: NaN where that loop never returns);
+//           dd_nearest  fsrc/multifloats.fypp:1595-1607
 //   dd_sinh / dd_cosh / dd_tanh  src/float64x2/hyp.inc:2-87
 //   dd_asinh / dd_acosh / dd_atanh  src/float64x2/inv_trig.inc:226-291
 //   estrin10/12/14/17/24  src/float64x2/poly.inc:33-217 (neval's trees)
@@ -449,6 +452,115 @@ MF_HD ddv dd_fdim(ddv x, ddv y) {
   if (x.hi != x.hi || y.hi != y.hi) return mk(__builtin_nan(""), 0.0);
   return dd_lt(y, x) ? dd_sub(x, y) : mk(0.0, 0.0);
 }
+// ------------------------------------------------- fmod, modulo, nearest
+// float64x2.h:1540-1555 (floor) and 1574-1576 (trunc); zeros kept
+MF_HD ddv dd_floor(ddv x) {
+  if (x.hi == 0.0 && x.lo == 0.0) return x;
+  const double f = __builtin_floor(x.hi);
+  if (f != x.hi) return mk(f, 0.0);
+  double h, l;
+  fast_two_sum(f, __builtin_floor(x.lo), h, l);
+  return mk(h, l);
+}
+MF_HD ddv dd_trunc(ddv x) { return __builtin_signbit(x.hi) ? dd_neg(dd_floor(dd_neg(x))) : dd_floor(x); }
+// float64x2.h:186-190 (operator>=): an equal hi limb decides on lo; NaN false
+MF_HD bool dd_ge(ddv a, ddv b) { return b.hi < a.hi || (a.hi == b.hi && a.lo >= b.lo); }
+// std::ilogb as the reference's x86-64 build has it (glibc): INT_MIN for
+// zero and NaN, INT_MAX for infinities, the true exponent of a subnormal.
+MF_HD int ref_ilogb(double v) {
+  if (v == 0.0 || v != v) return -2147483647 - 1;
+  if (__builtin_isinf(v)) return 2147483647;
+  return mf_ilogb(v);
+}
+// One step of the reference's fmod loop (float64x2.h:2028-2054) on the
+// remainder r, ay = |y|: false at the loop's exit (r unchanged, its
+// magnitude the result), true after a step that leaves r nonzero (a zero r
+// also ends the loop). The exponent gap is the 32-bit difference with
+// wrap-around, as the compiled loop computes it: with ilogb(0) = INT_MIN a
+// divisor of -0 gives q = -inf and r - ay, the same r again.
+MF_HD bool dd_fmod_step(ddv &r, ddv ay) {
+  if (r.hi < 0.0) {
+    const double r_abs = -r.hi;
+    const int gap = (int)((unsigned)ref_ilogb(r_abs) - (unsigned)ref_ilogb(ay.hi));
+    if (gap <= 0) {
+      r = dd_add(r, ay);
+    } else if (gap <= 53) {
+      const double q = __builtin_trunc(r_abs / ay.hi) + 1.0;
+      r = dd_add(r, dd_mul(ay, mk(q, 0.0)));
+    } else {
+      r = dd_add(r, dd_mul(dd_add(dd_trunc(dd_div(dd_neg(r), ay)), mk(1.0, 0.0)), ay));
+    }
+  } else if (dd_ge(r, ay)) {
+    const int gap = (int)((unsigned)ref_ilogb(r.hi) - (unsigned)ref_ilogb(ay.hi));
+    if (gap <= 53) {
+      const double q = __builtin_trunc(r.hi / ay.hi);
+      r = q <= 1.0 ? dd_sub(r, ay) : dd_sub(r, dd_mul(ay, mk(q, 0.0)));
+    } else {
+      r = dd_sub(r, dd_mul(dd_trunc(dd_div(r, ay)), ay));
+    }
+  } else {
+    return false;
+  }
+  return !(r.hi == 0.0 && r.lo == 0.0);
+}
+// float64x2.h:2010-2059 (fmod) with a bounded loop. The reference's loop is a
+// function of the remainder alone (ay is fixed), so it returns exactly when
+// the orbit of r reaches an exit, and never when r comes back to a value it
+// held (y = -0, subnormal or tiny divisors whose products vanish, quotients
+// beyond the double range). Brent's search (r saved after 1, 2, 4, ... steps
+// since the last save) sees such a return within twice the length of the
+// orbit; the step cap bounds the search. Both give NaN (C fmod's value for a
+// zero divisor). Every other call takes the reference's steps and returns its
+// bits; those orbits end within a few steps (10 on the test sets).
+constexpr int kFmodMaxSteps = 512;
+MF_HD ddv dd_fmod(ddv x, ddv y) {
+  const bool x_neg = x.hi < 0.0;
+  const ddv ax = x_neg ? dd_neg(x) : x;
+  const ddv ay = y.hi < 0.0 ? dd_neg(y) : y;
+  if (dd_lt(ax, ay)) return x;
+  ddv r = ax, saved = ax;
+  int power = 1, since = 0;
+  for (int n = 1;; ++n) {
+    if (!dd_fmod_step(r, ay)) break;
+    if ((r.hi == saved.hi && r.lo == saved.lo && __builtin_signbit(r.hi) == __builtin_signbit(saved.hi) &&
+         __builtin_signbit(r.lo) == __builtin_signbit(saved.lo)) ||
+        n >= kFmodMaxSteps)
+      return mk(__builtin_nan(""), 0.0);
+    if (++since == power) {
+      saved = r;
+      power *= 2;
+      since = 0;
+    }
+  }
+  return x_neg ? dd_neg(r) : r;
+}
+// float64x2.h:2068-2072 (modulo): the floored remainder, sign of y
+MF_HD ddv dd_modulo(ddv x, ddv y) {
+  const ddv r = dd_fmod(x, y);
+  if (r.hi == 0.0 && r.lo == 0.0) return r;
+  return dd_signbit(r) != dd_signbit(y) ? dd_add(r, y) : r;
+}
+// fsrc/multifloats.fypp:1595-1607 (nearest): the next double after x.hi
+// toward +inf when y.hi > 0, else toward -inf (gfortran's NEAREST is
+// nextafter toward the infinity of the direction's sign); low limb 0.
+MF_HD ddv dd_nearest(ddv x, ddv y) {
+  const double h = x.hi;
+  double out;
+  if (h != h) {
+    out = h;
+  } else if (h == 0.0) {
+    out = y.hi > 0.0 ? 0x1p-1074 : -0x1p-1074;
+  } else {
+    const bool up = y.hi > 0.0;
+    const long long b = __builtin_bit_cast(long long, h);
+    // away from zero when the step's direction has h's sign; +-inf stays
+    // put in its own direction (nextafter(inf, inf) = inf)
+    const bool away = (h > 0.0) == up;
+    out = (away && __builtin_isinf(h)) ? h : __builtin_bit_cast(double, away ? b + 1 : b - 1);
+  }
+  return mk(out, 0.0);
+}
+
 // float64x2.h:2270-2303: both scaled by 2^-e (e the exponent of the larger),
 // big sqrt(1 + (small/big)^2), scaled back. Non-finite input: +inf if either
 // is infinite, else NaN (C hypot).

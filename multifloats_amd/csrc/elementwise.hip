@@ -75,6 +75,9 @@ __device__ __forceinline__ ddv binary_op(ddv x, ddv y) {
   else if constexpr (OP == mfhip::OP_ATAN2PI) return dd_atan2pi(x, y);
   else if constexpr (OP == mfhip::OP_MINF) return dd_fortran_min(x, y);
   else if constexpr (OP == mfhip::OP_MAXF) return dd_fortran_max(x, y);
+  else if constexpr (OP == mfhip::OP_FMOD) return dd_fmod(x, y);
+  else if constexpr (OP == mfhip::OP_MODULO) return dd_modulo(x, y);
+  else if constexpr (OP == mfhip::OP_NEAREST) return dd_nearest(x, y);
   else return dd_fortran_dim(x, y);
 }
 
@@ -247,6 +250,9 @@ hipError_t binary(int op, const double2 *a, const double2 *b, double2 *c, int64_
     case OP_MINF: return run_binary<OP_MINF>(a, b, c, n, s);
     case OP_MAXF: return run_binary<OP_MAXF>(a, b, c, n, s);
     case OP_DIMF: return run_binary<OP_DIMF>(a, b, c, n, s);
+    case OP_FMOD: return run_binary<OP_FMOD>(a, b, c, n, s);
+    case OP_MODULO: return run_binary<OP_MODULO>(a, b, c, n, s);
+    case OP_NEAREST: return run_binary<OP_NEAREST>(a, b, c, n, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -24,7 +24,7 @@ enum {
 enum {
   OP_ADD = 0, OP_SUB = 1, OP_MUL = 2, OP_DIV = 3, OP_POW = 4, OP_HYPOT = 5, OP_FDIM = 6, OP_FMIN = 7,
   OP_FMAX = 8, OP_COPYSIGN = 9, OP_ATAN2 = 10, OP_ATAN2PI = 11,
-  OP_MINF = 12, OP_MAXF = 13, OP_DIMF = 14
+  OP_MINF = 12, OP_MAXF = 13, OP_DIMF = 14, OP_FMOD = 15, OP_MODULO = 16, OP_NEAREST = 17
 };
 enum { RED_DOT = 0, RED_SUM = 1, RED_NORM2 = 2, RED_PROD = 3, RED_MAXVAL = 4, RED_MINVAL = 5 };
 

@@ -25,9 +25,12 @@ UNARY = [("exp", "expdd_v"), ("log", "logdd_v"), ("sin", "sindd_v"), ("erf", "er
 MASKED_REDUCE = [("sum", "sumdd"), ("product", "productdd"), ("maxval", "maxvaldd"), ("minval", "minvaldd")]
 BINARY = [("+", "add", "adddd_v"), ("-", "sub", "subdd_v"), ("*", "mul", "muldd_v"), ("/", "div", "divdd_v")]
 # two-argument elemental generics whose reference bodies are these C functions'
-# rules (fypp:1550-1587: hypot calls hypotdd; sign is copysigndd's rule)
+# rules (fypp:1550-1607: hypot calls hypotdd; sign is copysigndd's rule; mod
+# calls fmoddd; modulo is fmoddd plus y on a sign change, modulodd's rule;
+# nearest steps the leading limb)
 BINFN = [("hypot", "hypotdd_v"), ("sign", "copysigndd_v"), ("atan2", "atan2dd_v"), ("atan2pi", "atan2pidd_v"),
-         ("min", "mindd_v"), ("max", "maxdd_v"), ("dim", "dimdd_v")]
+         ("min", "mindd_v"), ("max", "maxdd_v"), ("dim", "dimdd_v"), ("mod", "fmoddd_v"),
+         ("modulo", "modulodd_v"), ("nearest", "nearestdd_v")]
 
 
 def colons(r):
@@ -114,6 +117,7 @@ module multifloats_hip
     public :: exp, log, sin, erf, sqrt, cos, tan, expm1, log2, log10, log1p, sinh, cosh, tanh
     public :: erfc, erfc_scaled, asinh, acosh, atanh, sinpi, cospi, tanpi
     public :: hypot, sign, atan, atanpi, atan2, atan2pi, asin, acos, asinpi, acospi, min, max, dim
+    public :: mod, modulo, nearest
     public :: operator(+), operator(-), operator(*), operator(/)
 
     interface

@@ -66,6 +66,7 @@ module multifloats_hip
     public :: exp, log, sin, erf, sqrt, cos, tan, expm1, log2, log10, log1p, sinh, cosh, tanh
     public :: erfc, erfc_scaled, asinh, acosh, atanh, sinpi, cospi, tanpi
     public :: hypot, sign, atan, atanpi, atan2, atan2pi, asin, acos, asinpi, acospi, min, max, dim
+    public :: mod, modulo, nearest
     public :: operator(+), operator(-), operator(*), operator(/)
 
     interface
@@ -368,6 +369,21 @@ module multifloats_hip
             integer(c_int64_t), value :: n
         end subroutine
         pure subroutine c_dimdd_v(a, b, c, n) bind(c, name='dimdd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, b, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_fmoddd_v(a, b, c, n) bind(c, name='fmoddd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, b, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_modulodd_v(a, b, c, n) bind(c, name='modulodd_v')
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: a, b, c
+            integer(c_int64_t), value :: n
+        end subroutine
+        pure subroutine c_nearestdd_v(a, b, c, n) bind(c, name='nearestdd_v')
             import :: c_ptr, c_int64_t
             type(c_ptr), value :: a, b, c
             integer(c_int64_t), value :: n
@@ -849,6 +865,36 @@ module multifloats_hip
         module procedure dd_dim_5d
         module procedure dd_dim_6d
         module procedure dd_dim_7d
+    end interface
+
+    interface mod
+        module procedure dd_mod_1d
+        module procedure dd_mod_2d
+        module procedure dd_mod_3d
+        module procedure dd_mod_4d
+        module procedure dd_mod_5d
+        module procedure dd_mod_6d
+        module procedure dd_mod_7d
+    end interface
+
+    interface modulo
+        module procedure dd_modulo_1d
+        module procedure dd_modulo_2d
+        module procedure dd_modulo_3d
+        module procedure dd_modulo_4d
+        module procedure dd_modulo_5d
+        module procedure dd_modulo_6d
+        module procedure dd_modulo_7d
+    end interface
+
+    interface nearest
+        module procedure dd_nearest_1d
+        module procedure dd_nearest_2d
+        module procedure dd_nearest_3d
+        module procedure dd_nearest_4d
+        module procedure dd_nearest_5d
+        module procedure dd_nearest_6d
+        module procedure dd_nearest_7d
     end interface
 
 contains
@@ -3639,6 +3685,132 @@ contains
         type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:,:,:), b(:,:,:,:,:,:,:)
         type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5), size(a,6), size(a,7))
         call c_dimdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_mod_1d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:), b(:)
+        type(real64x2), target :: res(size(a,1))
+        call c_fmoddd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_mod_2d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:), b(:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2))
+        call c_fmoddd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_mod_3d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:), b(:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3))
+        call c_fmoddd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_mod_4d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:), b(:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4))
+        call c_fmoddd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_mod_5d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:), b(:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5))
+        call c_fmoddd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_mod_6d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:,:), b(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5), size(a,6))
+        call c_fmoddd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_mod_7d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:,:,:), b(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5), size(a,6), size(a,7))
+        call c_fmoddd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_modulo_1d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:), b(:)
+        type(real64x2), target :: res(size(a,1))
+        call c_modulodd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_modulo_2d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:), b(:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2))
+        call c_modulodd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_modulo_3d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:), b(:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3))
+        call c_modulodd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_modulo_4d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:), b(:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4))
+        call c_modulodd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_modulo_5d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:), b(:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5))
+        call c_modulodd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_modulo_6d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:,:), b(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5), size(a,6))
+        call c_modulodd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_modulo_7d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:,:,:), b(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5), size(a,6), size(a,7))
+        call c_modulodd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_nearest_1d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:), b(:)
+        type(real64x2), target :: res(size(a,1))
+        call c_nearestdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_nearest_2d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:), b(:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2))
+        call c_nearestdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_nearest_3d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:), b(:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3))
+        call c_nearestdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_nearest_4d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:), b(:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4))
+        call c_nearestdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_nearest_5d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:), b(:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5))
+        call c_nearestdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_nearest_6d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:,:), b(:,:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5), size(a,6))
+        call c_nearestdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
+    end function
+
+    pure function dd_nearest_7d(a, b) result(res)
+        type(real64x2), intent(in), target, contiguous :: a(:,:,:,:,:,:,:), b(:,:,:,:,:,:,:)
+        type(real64x2), target :: res(size(a,1), size(a,2), size(a,3), size(a,4), size(a,5), size(a,6), size(a,7))
+        call c_nearestdd_v(c_loc(a), c_loc(b), c_loc(res), int(min(size(a), size(b)), c_int64_t))
     end function
 
 end module multifloats_hip

@@ -1038,6 +1038,119 @@ static dd o_pow(dd x, dd y) {
   ftsum(h, l, &h, &l);
   return o_exp2_from_triple(h, l, e);
 }
+/* float64x2.h:1540-1555 (floor) and 1574-1576 (trunc) */
+static dd o_floor(dd x) {
+  if (x.hi == 0.0 && x.lo == 0.0) return x;
+  double f = floor(x.hi);
+  if (f == x.hi) {
+    double h, l;
+    ftsum(f, floor(x.lo), &h, &l);
+    return mkdd(h, l);
+  }
+  return mkdd(f, 0.0);
+}
+static dd o_trunc(dd x) { return signbit(x.hi) ? o_neg(o_floor(o_neg(x))) : o_floor(x); }
+/* float64x2.h:186-190 (operator>=) */
+static int o_ge(dd a, dd b) {
+  if (b.hi < a.hi) return 1;
+  if (a.hi == b.hi) return a.lo >= b.lo;
+  return 0;
+}
+/* std::ilogb as glibc defines it on x86-64: FP_ILOGB0 = FP_ILOGBNAN =
+ * INT_MIN, INT_MAX for infinities, the true exponent of a subnormal. */
+static int ref_ilogb(double v) {
+  if (v == 0.0 || v != v) return INT_MIN;
+  if (isinf(v)) return INT_MAX;
+  int e;
+  (void)frexp(v, &e);
+  return e - 1;
+}
+/* One step of the reference's fmod loop (float64x2.h:2028-2054) on the
+ * remainder r (ay = |y|): returns 0 at the loop's exit (r then holds the
+ * result's magnitude), 1 after a step. The exponent gap is the wrapping
+ * 32-bit difference, as the compiled loop computes it (ilogb(0) = INT_MIN:
+ * y = -0 gives the gap INT_MIN + ilogb(r)). */
+static int o_fmod_step(dd *rp, dd ay) {
+  dd r = *rp;
+  if (r.hi < 0.0) {
+    double r_abs = -r.hi;
+    int gap = (int)((unsigned)ref_ilogb(r_abs) - (unsigned)ref_ilogb(ay.hi));
+    if (gap <= 0) {
+      r = o_add(r, ay);
+    } else if (gap <= 53) {
+      double q = trunc(r_abs / ay.hi) + 1.0;
+      r = o_add(r, o_mul(ay, mkdd(q, 0.0)));
+    } else {
+      r = o_add(r, o_mul(o_add(o_trunc(o_div(o_neg(r), ay)), mkdd(1.0, 0.0)), ay));
+    }
+  } else if (o_ge(r, ay)) {
+    int gap = (int)((unsigned)ref_ilogb(r.hi) - (unsigned)ref_ilogb(ay.hi));
+    if (gap <= 53) {
+      double q = trunc(r.hi / ay.hi);
+      r = (q <= 1.0) ? o_sub(r, ay) : o_sub(r, o_mul(ay, mkdd(q, 0.0)));
+    } else {
+      r = o_sub(r, o_mul(o_trunc(o_div(r, ay)), ay));
+    }
+  } else {
+    return 0;
+  }
+  *rp = r;
+  return !(r.hi == 0.0 && r.lo == 0.0);
+}
+static int same_bits(dd a, dd b) { return memcmp(&a, &b, sizeof(dd)) == 0; }
+/* float64x2.h:2010-2059 (fmod), bounded. The reference's loop is a function
+ * of r alone (ay is fixed), so it returns iff its orbit reaches an exit; an
+ * orbit that revisits a remainder never returns. Brent's cycle search
+ * (remainder saved at step counts 1, 2, 4, ...) finds such a revisit within
+ * twice the orbit's length, and FMOD_MAX_STEPS bounds the search: both give
+ * NaN. Every returning call restates the reference's bits. *steps (if set)
+ * receives the steps taken (-1: cycle, -2: cap). */
+#define FMOD_MAX_STEPS 512
+static dd o_fmod_n(dd x, dd y, int *steps) {
+  int x_neg = x.hi < 0.0;
+  dd ax = x_neg ? o_neg(x) : x;
+  dd ay = (y.hi < 0.0) ? o_neg(y) : y;
+  if (steps) *steps = 0;
+  if (o_lt(ax, ay)) return x;
+  dd r = ax, saved = ax;
+  int n = 0, power = 1, since = 0;
+  for (;;) {
+    if (!o_fmod_step(&r, ay)) break;
+    ++n;
+    if (same_bits(r, saved)) {
+      if (steps) *steps = -1;
+      return mkdd(NAN, 0.0);
+    }
+    if (n >= FMOD_MAX_STEPS) {
+      if (steps) *steps = -2;
+      return mkdd(NAN, 0.0);
+    }
+    if (++since == power) {
+      saved = r;
+      power *= 2;
+      since = 0;
+    }
+  }
+  if (steps) *steps = n;
+  return x_neg ? o_neg(r) : r;
+}
+static dd o_fmod(dd x, dd y) { return o_fmod_n(x, y, NULL); }
+/* float64x2.h:2068-2072 (modulo) */
+static dd o_modulo(dd x, dd y) {
+  dd r = o_fmod(x, y);
+  if (r.hi == 0.0 && r.lo == 0.0) return r;
+  return (o_signbit(r) != o_signbit(y)) ? o_add(r, y) : r;
+}
+/* fsrc/multifloats.fypp:1595-1607 (nearest): the next double after x.hi in
+ * the direction of +inf when y.hi > 0, else of -inf (gfortran's NEAREST is
+ * nextafter toward the infinity of that sign); low limb 0. */
+static dd o_nearest(dd x, dd y) { return mkdd(nextafter(x.hi, y.hi > 0.0 ? INFINITY : -INFINITY), 0.0); }
+void or_fmod_steps(const dd *a, const dd *b, int *steps, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) (void)o_fmod_n(a[i], b[i], &steps[i]);
+}
+ARR2(or_fmod, o_fmod)
+ARR2(or_modulo, o_modulo)
+ARR2(or_nearest, o_nearest)
 ARR2(or_pow, o_pow)
 ARR2(or_hypot, o_hypot)
 ARR2(or_fdim, o_fdim)

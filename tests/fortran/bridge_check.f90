@@ -98,6 +98,9 @@ program bridge_check
     write(u) min(x, y)
     write(u) max(x, y)
     write(u) dim(x, y)
+    write(u) mod(x, y)
+    write(u) modulo(x, y)
+    write(u) nearest(x, y)
     ! xGEMM: C = x(1) A B + y(1) C with A passed transposed and C padded to m + 2 rows
     at = transpose(a)
     allocate(cc(m + 2, n))

@@ -48,7 +48,7 @@ def load_ref():
     for nm in INVTRIG1 + INVTRIG3:
         getattr(L, nm + "dd").argtypes = [F2]
         getattr(L, nm + "dd").restype = F2
-    for nm in BINARY2 + INVTRIG2:
+    for nm in BINARY2 + INVTRIG2 + ("fmod", "modulo"):
         getattr(L, nm + "dd").argtypes = [F2, F2]
         getattr(L, nm + "dd").restype = F2
     P, I64 = ctypes.c_void_p, ctypes.c_int64
@@ -413,6 +413,100 @@ def binary2(L, rng):
     np.savez_compressed(os.path.join(HERE, "binary2.npz"), **out)
 
 
+def call2_guarded(nm, a, b, stall=1.0):
+    """The reference's nm(a_i, b_i) for every i, in a forked child that marks
+    each finished call in shared memory; a call that makes no progress for
+    `stall` seconds is recorded as not returning (the child is killed and a
+    new one resumes after it). Returns (outputs, returned)."""
+    import mmap
+    import signal
+    import time
+    n = a.shape[0]
+    out = np.full((n, 2), np.nan)
+    done = np.zeros(n, dtype=bool)
+    start = 0
+    while start < n:
+        buf = mmap.mmap(-1, n * 24)
+        view = np.frombuffer(buf, dtype=np.float64).reshape(n, 3)
+        pid = os.fork()
+        if pid == 0:  # child: its own copy of the library
+            L = load_ref()
+            fn = getattr(L, nm)
+            for i in range(start, n):
+                r = fn(f2(a[i]), f2(b[i]))
+                view[i, 0], view[i, 1], view[i, 2] = r.limbs[0], r.limbs[1], 1.0
+            os._exit(0)
+        last, t_last = -1, time.time()
+        while True:
+            fin, _ = os.waitpid(pid, os.WNOHANG)
+            prog = int(np.count_nonzero(view[start:, 2]))
+            if fin:
+                break
+            if prog != last:
+                last, t_last = prog, time.time()
+            elif time.time() - t_last > stall:
+                os.kill(pid, signal.SIGKILL)
+                os.waitpid(pid, 0)
+                break
+            time.sleep(0.01)
+        got = view[:, 2] == 1.0
+        out[got] = view[got, :2]
+        done |= got
+        nxt = start + int(np.count_nonzero(view[start:, 2]))
+        if nxt < n and not got[nxt]:
+            nxt += 1  # the call at nxt did not return
+        start = nxt
+        del view
+        buf.close()
+    return out, done
+
+
+def fmod_golden(L, rng):
+    """fmod and modulo: every pair of special values, and draws over the
+    reference's loop regimes (x/y below 2^53, the DD-quotient steps of wide
+    exponent gaps up to the double range, near-multiples, negative operands
+    and the add-back path, subnormal divisors), outputs of the compiled
+    reference and a flag for the calls that do not return (recorded with a
+    time limit), plus the exact remainders (320-bit) for the mathematical
+    checks. Inputs are canonical double-doubles."""
+    w = datagen._with_lo
+    U = rng.uniform
+    sgn = lambda k: rng.choice([-1.0, 1.0], k)
+    sp = np.array(SPECIALS)
+    ga, gb = np.repeat(sp, len(sp), axis=0), np.tile(sp, (len(sp), 1))
+    n = 600
+    ys = w(rng, 10.0 ** U(-3, 3, n) * sgn(n))
+    near = ys * np.round(U(1, 1e6, n))[:, None]  # near-multiples: q y, perturbed in lo
+    near = w(rng, near[:, 0])
+    parts = [
+        (w(rng, 10.0 ** U(-3, 3, n) * sgn(n)), w(rng, 10.0 ** U(-3, 3, n) * sgn(n))),
+        (w(rng, 2.0 ** U(0, 52, n) * sgn(n)), w(rng, U(0.5, 2, n) * sgn(n))),
+        (w(rng, 2.0 ** U(53, 1000, n) * sgn(n)), w(rng, 2.0 ** U(-20, 20, n) * sgn(n))),
+        (w(rng, 2.0 ** U(-500, 1000, n) * sgn(n)), w(rng, 2.0 ** U(-1000, 500, n) * sgn(n))),
+        (near, ys),
+        (w(rng, 2.0 ** U(-10, 60, 200) * sgn(200)), np.stack([2.0 ** U(-1074, -1022, 200) * sgn(200),
+                                                                np.zeros(200)], 1)),
+    ]
+    a = np.concatenate([p[0] for p in parts] + [ga])
+    b = np.concatenate([p[1] for p in parts] + [gb])
+    out = {"a": a, "b": b}
+    for nm in ("fmod", "modulo"):
+        r, ok = call2_guarded(nm + "dd", a, b)
+        out[nm + "_ref"], out[nm + "_returns"] = r, ok
+        print(nm, "calls that do not return:", int((~ok).sum()), flush=True)
+    exact = np.full((a.shape[0], 3), np.nan)
+    for i in range(a.shape[0]):
+        if not (np.isfinite(a[i, 0]) and np.isfinite(b[i, 0])) or b[i, 0] == 0.0:
+            continue
+        mpmath.mp.prec = 2400
+        x, y = mp_of(a[i]), mp_of(b[i])
+        r = x - mpmath.floor(abs(x / y)) * mpmath.sign(x / y) * y
+        mpmath.mp.prec = 320
+        exact[i] = limbs3(r)
+    out["fmod_exact"] = exact
+    np.savez_compressed(os.path.join(HERE, "fmod.npz"), **out)
+
+
 def invtrig(L, rng):
     """atan, atanpi, atan2, atan2pi (this project's own atan, so the bar is
     accuracy): the reference's outputs and 320-bit values on draws over every
@@ -631,9 +725,11 @@ def main(which):
         invtrig3(L, np.random.default_rng(9191))
     if not which or "c0" in which:
         c0(L)
+    if not which or "fmod" in which:
+        fmod_golden(L, np.random.default_rng(1212))
     for f in ("arith.npz", "transc.npz", "matmul.npz", "cmatmul.npz", "transc2.npz", "special2.npz", "transc3.npz",
               "transc4.npz", "transc5.npz", "binary2.npz", "invtrig.npz",
-              "invtrig3.npz", "c0_256.json"):
+              "invtrig3.npz", "c0_256.json", "fmod.npz"):
         print(f, os.path.getsize(os.path.join(HERE, f)))
 
 

@@ -38,3 +38,6 @@ H1(h_asin, dd_asin)
 H1(h_acos, dd_acos)
 H1(h_asinpi, dd_asinpi)
 H1(h_acospi, dd_acospi)
+H2(h_fmod, dd_fmod)
+H2(h_modulo, dd_modulo)
+H2(h_nearest, dd_nearest)

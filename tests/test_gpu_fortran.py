@@ -114,6 +114,8 @@ def test_fortran_bridge_routes_to_the_library(gpu, tmp_path):
     nt.assert_bits(r.take(nv), mf.acos(q), "acos")
     for nm in ("min", "max", "dim"):
         nt.assert_bits(r.take(nv), getattr(mf, "fortran_" + nm)(x, y), nm)
+    for nm, fn in (("mod", mf.fmod), ("modulo", mf.modulo), ("nearest", mf.nearest)):
+        nt.assert_bits(r.take(nv), fn(x, y), nm)
     At = nt.cm(A.transpose(1, 0, 2))
     Cw = np.empty((m + 2, n, 2))
     Cw[:m] = A[:, :n]
