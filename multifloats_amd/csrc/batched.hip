@@ -11,10 +11,11 @@
 // Plan: the host cuts every GEMM into output tiles of its tile variant (one
 // of six shapes, chosen by the caller's tuned table or by least padded work)
 // and every dot into chunks of
-// kDotChunk (2^20) elements, then sorts the items by estimated cost, largest
-// first (longest processing time). The persistent kernel's workgroups take
-// the sorted items round robin; every workgroup's loop is bounded by the item
-// count, so the grid always drains. A dot of one chunk is a single item that
+// kDotChunk (2^20) elements, then groups the items by kind (tile shape, or
+// dot) and sorts each group by estimated cost, largest first (longest
+// processing time). One persistent launch per kind takes its group's items
+// round robin; every workgroup's loop is bounded by the item count, so each
+// grid drains. A dot of one chunk is a single item that
 // writes its result directly. A longer dot's chunks leave DD partials; the
 // chunk that completes the dot last (per-dot atomic counter) folds the
 // partials in index order, so results do not depend on scheduling. That
@@ -42,17 +43,16 @@ constexpr int64_t kDotChunk = 1 << 20;  // elements per work item of one dot
 
 // GEMM tile variants (kind of a GEMM work item): (TM, TN) outputs per lane,
 // (16 TM) x (16 TN) tiles. 0: 32x32, 1: 64x64, 2: 128x16, 3: 16x128,
-// 4: 64x32, 5: 32x64. The skinny shapes keep the padding of tall or wide
-// problems small.
-constexpr int kNumTileVariants = 6;
-constexpr int kTileTM[kNumTileVariants] = {2, 4, 8, 1, 4, 2};
-constexpr int kTileTN[kNumTileVariants] = {2, 4, 1, 8, 2, 4};
+// 4: 64x32, 5: 32x64, 6: 16x16, 7: 48x48. The skinny shapes keep the padding
+// of tall or wide problems small, 16x16 that of tiny ones, 48x48 that of
+// edges between the powers of two.
+constexpr int kNumTileVariants = 8;
+constexpr int kTileTM[kNumTileVariants] = {2, 4, 8, 1, 4, 2, 1, 3};
+constexpr int kTileTN[kNumTileVariants] = {2, 4, 1, 8, 2, 4, 1, 3};
 // Relative cost per padded MAC of each variant (fewer chains per lane or
 // more LDS reads per MAC cost more); used by the built-in choice.
-constexpr double kTileCost[kNumTileVariants] = {1.3, 1.0, 1.15, 1.15, 1.08, 1.08};
+constexpr double kTileCost[kNumTileVariants] = {1.3, 1.05, 1.25, 1.3, 1.0, 1.06, 2.0, 1.03};
 enum { ITEM_DOT = kNumTileVariants };
-// LDS of the largest tile (2 * 16 * (16 TM + 16 TN) doubles).
-constexpr int kGroupedLds = 2 * 16 * (16 * 8 + 16 * 1);
 
 struct Item {
   int kind;
@@ -182,25 +182,30 @@ __device__ void dot_chunk(const DotD &d, int c, double2 *partials, int *dot_done
   }
 }
 
-// Static longest-processing-time round robin: workgroup g takes items g,
-// g + G, g + 2G, ... of the cost-sorted list. The item index derives from
-// blockIdx only, so every branch and loop around the barriers inside the item
-// bodies is uniform.
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_grouped(
-    const Item *__restrict__ items, int nitems, const GemmD *__restrict__ gemms,
-    const DotD *__restrict__ dots, double2 *__restrict__ partials, int *__restrict__ dot_done) {
-  __shared__ __attribute__((aligned(16))) double lds[kGroupedLds];
+// Static longest-processing-time round robin over one kind's slice of the
+// cost-sorted items: workgroup g takes items g, g + G, g + 2G, ... The item
+// index derives from blockIdx only, so every branch and loop around the
+// barriers inside the item bodies is uniform. Each kind has a kernel of its
+// own: one kernel holding every tile body needs more registers than any body
+// alone and spilled (the 64 x 64 body alone already does at 256 VGPRs); split,
+// each body gets its own register budget and occupancy.
+template <int TM, int TN, int WPE>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WPE))) void k_grouped_gemm(
+    const Item *__restrict__ items, int nitems, const GemmD *__restrict__ gemms) {
+  __shared__ __attribute__((aligned(16))) double lds[2 * kBK * (16 * TM + 16 * TN)];
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const Item w = items[it];
-    switch (w.kind) {
-      case 0: gemm_tile<2, 2>(gemms[w.prob], w.a, w.b, lds); break;
-      case 1: gemm_tile<4, 4>(gemms[w.prob], w.a, w.b, lds); break;
-      case 2: gemm_tile<8, 1>(gemms[w.prob], w.a, w.b, lds); break;
-      case 3: gemm_tile<1, 8>(gemms[w.prob], w.a, w.b, lds); break;
-      case 4: gemm_tile<4, 2>(gemms[w.prob], w.a, w.b, lds); break;
-      case 5: gemm_tile<2, 4>(gemms[w.prob], w.a, w.b, lds); break;
-      default: dot_chunk(dots[w.prob], w.a, partials, dot_done, lds); break;
-    }
+    gemm_tile<TM, TN>(gemms[w.prob], w.a, w.b, lds);
+  }
+}
+
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_grouped_dot(
+    const Item *__restrict__ items, int nitems, const DotD *__restrict__ dots, double2 *__restrict__ partials,
+    int *__restrict__ dot_done) {
+  __shared__ __attribute__((aligned(16))) double lds[2 * kThreads];
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const Item w = items[it];
+    dot_chunk(dots[w.prob], w.a, partials, dot_done, lds);
   }
 }
 
@@ -216,12 +221,13 @@ namespace {
 
 struct Plan {
   int device = -1;
-  Item *items = nullptr;
+  Item *items = nullptr;  // grouped by kind, each kind's slice sorted by cost
   GemmD *gemms = nullptr;
   DotD *dots = nullptr;
   double2 *partials = nullptr;
   int *counters = nullptr;  // per-dot completion counts
-  int nitems = 0, ndot = 0, grid = 0;
+  int nitems = 0, ndot = 0, cus = 1;
+  int first[ITEM_DOT + 2] = {};  // items of kind q: [first[q], first[q + 1])
   int64_t ncounters = 0;
   hipStream_t gstream = nullptr;
   hipGraph_t graph = nullptr;
@@ -229,11 +235,35 @@ struct Plan {
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
 };
 
+// One persistent launch per kind with items: grid = the kind's resident
+// workgroups over all CUs (its occupancy), or fewer when it has fewer items.
+template <int TM, int TN, int WPE>
+void launch_kind(const Plan *p, int q, hipStream_t s) {
+  const int n = p->first[q + 1] - p->first[q];
+  if (n <= 0) return;
+  const int resident = p->cus * WPE;  // 4-wave workgroups: WPE per CU
+  const int grid = n < resident ? n : resident;
+  hipLaunchKernelGGL((k_grouped_gemm<TM, TN, WPE>), dim3((unsigned)grid), dim3(kThreads), 0, s,
+                     p->items + p->first[q], n, p->gemms);
+}
+
 hipError_t run_plan(Plan *p, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_ints, dim3((unsigned)((p->ncounters + 255) / 256)), dim3(256), 0, s,
                      p->counters, p->ncounters);
-  hipLaunchKernelGGL(k_grouped, dim3((unsigned)p->grid), dim3(kThreads), 0, s, p->items, p->nitems,
-                     p->gemms, p->dots, p->partials, p->counters);
+  launch_kind<2, 2, 4>(p, 0, s);
+  launch_kind<4, 4, 1>(p, 1, s);
+  launch_kind<8, 1, 2>(p, 2, s);
+  launch_kind<1, 8, 2>(p, 3, s);
+  launch_kind<4, 2, 3>(p, 4, s);
+  launch_kind<2, 4, 3>(p, 5, s);
+  launch_kind<1, 1, 4>(p, 6, s);
+  launch_kind<3, 3, 2>(p, 7, s);
+  const int nd = p->first[ITEM_DOT + 1] - p->first[ITEM_DOT];
+  if (nd > 0) {
+    const int grid = nd < 2 * p->cus ? nd : 2 * p->cus;
+    hipLaunchKernelGGL(k_grouped_dot, dim3((unsigned)grid), dim3(kThreads), 0, s, p->items + p->first[ITEM_DOT], nd,
+                       p->dots, p->partials, p->counters);
+  }
   return hipGetLastError();
 }
 
@@ -307,18 +337,24 @@ void *mfhip_grouped_create(const mfhip_gemm_problem *gemms, int ngemm, const int
     }
     nparts += chunks;
   }
-  // longest processing time first (stable for determinism of the plan)
+  // grouped by kind; longest processing time first within a kind (stable
+  // for determinism of the plan)
   std::vector<size_t> order(items.size());
   for (size_t i = 0; i < order.size(); ++i) order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
+  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+    if (items[x].kind != items[y].kind) return items[x].kind < items[y].kind;
+    return cost[x] > cost[y];
+  });
   std::vector<Item> sorted(items.size());
   for (size_t i = 0; i < order.size(); ++i) sorted[i] = items[order[i]];
 
   Plan *p = new Plan();
+  for (const Item &it : sorted) ++p->first[it.kind + 1];
+  for (int q = 0; q <= ITEM_DOT; ++q) p->first[q + 1] += p->first[q];
   bool ok = hipGetDevice(&p->device) == hipSuccess;
   int cus = 0;
   ok = ok && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device) == hipSuccess;
-  p->grid = 2 * (cus > 0 ? cus : 1);
+  p->cus = cus > 0 ? cus : 1;
   p->nitems = (int)sorted.size();
   p->ndot = ndot;
   p->ncounters = ndot > 0 ? (int64_t)ndot : 1;

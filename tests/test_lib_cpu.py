@@ -183,20 +183,25 @@ def test_fortran_bridge_fails_loudly_without_gpu(tmp_path):
 
 def test_grouped_tile_choice_minimises_padded_work():
     """The tuned planner gives tall, wide and small problems the tile shapes
-    that waste the fewest lanes (equal rates: pure padding)."""
+    that waste the fewest lanes (equal rates: pure padding), and follows the
+    rates of each problem's k class."""
     from multifloats_amd import autotune
-    table = {"variant_rates": [1.0] * len(autotune.TILES)}
-    sizes = np.array([[400, 64, 10], [10, 64, 400], [64, 64, 64], [20, 64, 20], [120, 30, 60]])
+    nv, nc = len(autotune.TILES), len(autotune.K_EDGES) - 1
+    table = {"class_rates": [[1.0] * nv for _ in range(nc)]}
+    sizes = np.array([[400, 64, 10], [10, 64, 400], [64, 64, 64], [20, 64, 20], [120, 30, 60], [8, 8, 8],
+                      [96, 200, 96]])
     v = autotune.plan_variants(sizes, table)
-    assert autotune.TILES[v[0]] == (128, 16)
-    assert autotune.TILES[v[1]] == (16, 128)
-    assert autotune.TILES[v[2]] == (64, 64) or autotune.padded_macs(64, 64, 64, v[2]) == 64 ** 3
-    assert autotune.TILES[v[3]] == (32, 32)
+    assert autotune.TILES[v[0]][1] == 16 and autotune.TILES[v[1]][0] == 16
+    assert autotune.TILES[v[5]] == (16, 16)
+    assert autotune.TILES[v[6]] == (48, 48) or autotune.padded_macs(96, 200, 96, v[6]) == 96 * 200 * 96
     for (m, k, n), q in zip(sizes, v):
         assert autotune.padded_macs(m, k, n, q) == min(autotune.padded_macs(m, k, n, w) for w in autotune.VARIANTS)
-    # a slower variant loses ties
-    table = {"variant_rates": [1.0, 2.0, 1.0, 1.0, 1.0, 1.0]}
-    assert autotune.plan_variants(np.array([[64, 64, 64]]), table) == [1]
+    # a faster variant wins where padding ties, per k class
+    fast = [[1.0] * nv for _ in range(nc)]
+    fast[autotune.k_class(64)][1] = 2.0
+    assert autotune.plan_variants(np.array([[64, 64, 64], [64, 8, 64]]), {"class_rates": fast})[0] == 1
+    assert autotune.plan_variants(np.array([[64, 8, 64]]), {"class_rates": fast})[0] != 1
+    assert [autotune.k_class(k) for k in (8, 23, 24, 63, 64, 159, 160, 512)] == [0, 0, 1, 1, 2, 2, 3, 3]
 
 
 def _host(nm):
