@@ -55,26 +55,26 @@ INT8_PEAK_TOPS = 5000.0      # MI355X dense INT8 MFMA
 # HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB
 # per dispatch), scaled by what the same counters read for a 1 GiB device
 # copy: FETCH_SIZE x 2.0 (it reads 524,307 KiB for 1 GiB fetched), WRITE_SIZE
-# x 1.0 (1,048,576 KiB). From profiles/hot_kernels_4096_and_2p26_pmc.md
+# x 1.0 (1,048,576 KiB). From profiles/hot_kernels_profile_workload_pmc.md
 # (bench.py --workload profile). Keyed by (kernel, size, slices).
 PMC_TRAFFIC = {
-    ("k_oz2_gemm", 4096, 14): (2.0 * 1878723 + 608275) * 1024,
-    ("k_gemm_plain", 4096, 0): (2.0 * 3142688 + 262144) * 1024,
+    ("k_oz2_gemm", 4096, 14): (2.0 * 1596081 + 604614) * 1024,
+    ("k_gemm_plain", 4096, 0): (2.0 * 3142426 + 262151) * 1024,
 }
 # the C1 element kernels and reduction passes at N = 2^26, same runs
 C1_TRAFFIC = {
-    "add": (2.0 * 1048600 + 1048593) * 1024, "sub": (2.0 * 1048582 + 1048577) * 1024,
-    "mul": (2.0 * 1048586 + 1048576) * 1024, "div": (2.0 * 1048584 + 1048576) * 1024,
-    "sqrt": (2.0 * 524292 + 1048655) * 1024, "exp": (2.0 * 524358 + 1048576) * 1024,
-    "log": (2.0 * 524386 + 1048576) * 1024, "sin": (2.0 * 524378 + 1048576) * 1024,
-    "erf": (2.0 * 524629 + 1048576) * 1024, "dot": (2.0 * 1048626 + 1024) * 1024,
-    "sum": (2.0 * 524334 + 512) * 1024, "norm2": (2.0 * 524372 + 512) * 1024,
+    "add": (2.0 * 1048599 + 1048578) * 1024, "sub": (2.0 * 1048580 + 1048578) * 1024,
+    "mul": (2.0 * 1048586 + 1048576) * 1024, "div": (2.0 * 1048582 + 1048576) * 1024,
+    "sqrt": (2.0 * 524300 + 1048645) * 1024, "exp": (2.0 * 524354 + 1048576) * 1024,
+    "log": (2.0 * 524384 + 1048576) * 1024, "sin": (2.0 * 524387 + 1048576) * 1024,
+    "erf": (2.0 * 524615 + 1048576) * 1024, "dot": (2.0 * 1048625 + 1024) * 1024,
+    "sum": (2.0 * 524334 + 512) * 1024, "norm2": (2.0 * 524371 + 512) * 1024,
 }
 
 # FP64 VALU lane-instructions per element of the transcendental array
 # kernels on the C1 input distributions: (SQ_INSTS_VALU_ADD_F64 +
 # SQ_INSTS_VALU_MUL_F64 + SQ_INSTS_VALU_FMA_F64) x 64 / N from rocprofv3
-# --pmc at N = 2^26 (profiles/hot_kernels_4096_and_2p26_pmc.md). With these
+# --pmc at N = 2^26 (profiles/hot_kernels_profile_workload_pmc.md). With these
 # the FP64-issue fraction is instructions/element x elements/s / 39.3e12
 # (other FP64 work -- ldexp, rounding, conversions -- is not in these three
 # counters, so the fraction is a lower bound of the FP64 pipe's use).
@@ -317,7 +317,8 @@ def bench_c0(args, torch, mf):
     out = {"workload": "float64x2 GEMM m=n=k=256, config-0 entries (hi ~ U(-1,1)), renorm 8",
            "value": round(2.0 * m * n * k / (ms * 1e-3) / 1e9, 2), "unit": "DD-GFLOP/s", "ms": round(ms, 4),
            "steps": reps, "roofline": gemm_roofline_plain(m, n, k, ms)}
-    out["roofline"]["note"] += "; at 256^3 one launch of 16 workgroups: latency-bound, not issue-bound"
+    out["roofline"]["note"] += ("; at 256^3: 16x16 tiles, one chain per lane, 256 workgroups (one per CU): "
+                                "each lane's 256-step MAC chain is latency-bound, not issue-bound")
     if not args.no_cpu:
         dt, Cc, kind = cpu_matmul(A, B, n, 8, reps=3)
         out["cpu_baseline"] = dict(value=round(2.0 * m * n * k / dt / 1e9, 4), unit="DD-GFLOP/s", cores=1, kind=kind,
@@ -846,7 +847,8 @@ def bench_profile(args, torch, mf):
     (--kernel-trace --stats, or --pmc counter passes): a 1 GiB device copy
     (the calibration of the FETCH_SIZE / WRITE_SIZE counters), the Ozaki
     automatic call and the plain GEMM at 4096^3 (spread 2^+-20), + - * / sqrt
-    exp log sin erf on the C1 inputs, and dot / sum / norm2 at N = 2^26."""
+    exp log sin erf on the C1 inputs, dot / sum / norm2 at N = 2^26, and the
+    C4 mix (10,000 GEMMs + 50,000 dots) through the grouped plan."""
     from multifloats_amd import datagen
     s = torch.cuda.current_stream().cuda_stream
     x = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
@@ -884,6 +886,33 @@ def bench_profile(args, torch, mf):
     for kind in (mf.MFHIP_DOT, mf.MFHIP_SUM, mf.MFHIP_NORM2):
         mf.dev_reduce(kind, a.data_ptr(), b.data_ptr(), n1, out.data_ptr(), work.data_ptr(), 8, s)
     torch.cuda.synchronize()
+    del a, b, c, work
+    # the C4 mix: one eager run of the grouped plan (one launch per tile kind, then the dots)
+    from multifloats_amd import autotune
+    sizes, dlen = datagen.batched_sizes(np.random.default_rng(4))
+    tot = int(sum(m * k + k * n + m * n for m, k, n in sizes))
+    gbuf = torch.empty(2 * tot, dtype=torch.float64, device="cuda")
+    datagen.fill_config0_torch(torch, gbuf, 101)
+    dtot = int(dlen.sum())
+    dbuf = torch.empty(4 * dtot, dtype=torch.float64, device="cuda")
+    datagen.fill_config0_torch(torch, dbuf, 202)
+    dout = torch.zeros((len(dlen), 2), dtype=torch.float64, device="cuda")
+    gemms, off = [], 0
+    for m, k, n in sizes:
+        m, k, n = int(m), int(k), int(n)
+        g0 = gbuf.data_ptr() + 16 * off
+        gemms.append((g0, g0 + 16 * m * k, g0 + 16 * (m * k + k * n), m, k, n, 8))
+        off += m * k + k * n + m * n
+    dots, off = [], 0
+    for i, L in enumerate(dlen):
+        L = int(L)
+        d0 = dbuf.data_ptr() + 16 * off
+        dots.append((d0, d0 + 16 * L, dout.data_ptr() + 16 * i, L, 8))
+        off += 2 * L
+    plan = mf.GroupedPlan(gemms, dots, autotune.plan_variants(sizes))
+    plan.run(s)
+    torch.cuda.synchronize()
+    plan.close()
     return {"metric": "profile run (one launch of each hot kernel)", "value": None, "unit": None}
 
 
