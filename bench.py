@@ -87,7 +87,8 @@ def parse():
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--workload", default="all",
-                   choices=["all", "gemm", "cgemm", "blas", "trsm", "batched", "elementwise", "reduce", "profile"])
+                   choices=["all", "gemm", "cgemm", "blas", "trsm", "batched", "elementwise", "reduce", "profile",
+                            "int8ref"])
     p.add_argument("--retune", action="store_true", help="batched: rerun the tile autotuner first")
     p.add_argument("--size", type=int, default=4096)
     p.add_argument("--spread", type=float, default=20.0)
@@ -916,6 +917,32 @@ def bench_profile(args, torch, mf):
     return {"metric": "profile run (one launch of each hot kernel)", "value": None, "unit": None}
 
 
+def bench_int8ref(args, torch, mf):
+    """The vendor's INT8 GEMM (hipBLASLt through torch._int_mm: int8 x int8
+    -> int32, B column-major, its faster layout here) at m = n = k = 4096 and
+    8192 on uniformly random bytes (what the Ozaki residues are), on bytes in
+    [-4, 3] and on constant bytes: the rate the matrix cores reach on each kind
+    of data, the yardstick for k_oz2_gemm."""
+    out = {}
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    for size in (4096, 8192):
+        row = {}
+        for kind in ("random", "small", "constant"):
+            lo, hi = {"random": (-128, 128), "small": (-4, 4), "constant": (3, 4)}[kind]
+            a = torch.randint(lo, hi, (size, size), dtype=torch.int8, device="cuda", generator=g)
+            b = torch.randint(lo, hi, (size, size), dtype=torch.int8, device="cuda", generator=g).t().contiguous().t()
+            ms = timed(torch, lambda: torch._int_mm(a, b), args.steps, args.warmup)
+            row[kind] = {"ms": round(ms, 4), "POPs": round(2.0 * size ** 3 / ms / 1e9, 1)}
+            del a, b
+        out[str(size)] = row
+    return {"metric": "INT8 GEMM rate of the vendor library (hipBLASLt via torch._int_mm) by operand data",
+            "value": out["4096"]["random"]["POPs"], "unit": "TOP/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "higher_is_better": True, "dtype": "int8", "data": "synthetic",
+            "config": {"workload": "int8 x int8 -> int32 GEMM, m = n = k in {4096, 8192}; random, [-4, 3], "
+                                   "constant bytes"}, "rates": out, "peak": INT8_PEAK_TOPS}
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -928,7 +955,7 @@ def main():
         sys.exit("bench.py: no GPU visible")
     fn = {"all": bench_all, "gemm": bench_gemm, "cgemm": lambda a, t, m: bench_c2_complex(a, t, m),
           "blas": bench_blas, "trsm": bench_trsm, "batched": bench_batched, "elementwise": bench_elementwise,
-          "reduce": lambda a, t, m: bench_c1(a, t, m), "profile": bench_profile}[args.workload]
+          "reduce": lambda a, t, m: bench_c1(a, t, m), "profile": bench_profile, "int8ref": bench_int8ref}[args.workload]
     print(json.dumps(fn(args, torch, mf)), flush=True)
 
 

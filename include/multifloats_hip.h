@@ -434,10 +434,12 @@ MULTIFLOATS_API int mfhip_ozaki_info(const void *work, int64_t m, int64_t k, int
                                      int *slices_used, int64_t *fixups, void *stream);
 /* Slice count and fix-up count of the calling thread's last
  * matmuldd_mm_ozaki call; after mfhip_ozaki_matmul_mm the count is the
- * explicit one, or -1 for an automatic call (use mfhip_ozaki_info). The
- * exponent-range rule with which an automatic call starts its search. */
+ * explicit one, or -1 for an automatic call (use mfhip_ozaki_info). */
 MULTIFLOATS_API int mfhip_ozaki_slices_used(void);
 MULTIFLOATS_API int64_t mfhip_ozaki_fixups(void);
+/* The exponent-range rule with which an automatic call starts its search
+ * (range_a, range_b: the spread of the row exponents of A and of the column
+ * exponents of B). */
 MULTIFLOATS_API int mfhip_ozaki_choose_slices(int64_t k, int range_a, int range_b);
 /* Products form of an Ozaki call with these sizes and slice count (7..16):
  * the number N of pairwise-coprime moduli of the CRT form (N INT8 GEMMs with
