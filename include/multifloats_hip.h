@@ -249,8 +249,10 @@ MULTIFLOATS_API complex64x2 cdotdd(const complex64x2 *a, const complex64x2 *b, i
  * outer), with len the extent of dimension `dim`, inner the product of the
  * extents before it and outer the product of those after it. out receives
  * the inner x outer result in column-major order. Each output is the
- * reference's own sequential chain, so the results are bit-identical to it.
- * mask may be NULL. */
+ * reference's sequential chain over dimension dim with the DD operators, in
+ * the loop order of the fypp source; the results are bit-identical to that
+ * restated loop (the oracle's or_sum_dim / or_norm2_dim; the Fortran module
+ * itself is not compiled by the tests). mask may be NULL. */
 MULTIFLOATS_API void sumdd_dim(const float64x2 *a, const int32_t *mask, int64_t inner, int64_t len,
                                int64_t outer, float64x2 *out);
 MULTIFLOATS_API void norm2dd_dim(const float64x2 *a, int64_t inner, int64_t len, int64_t outer,
@@ -261,7 +263,8 @@ MULTIFLOATS_API void norm2dd_dim(const float64x2 *a, int64_t inner, int64_t len,
  *   NULL = all), <f>dd_dim(a, mask, inner, len, outer, out) as sumdd_dim.
  * product: rank 1 is a DD-multiply tree from 1 (not the sequential loop's
  * bits; relative error within 4 n 2^-104 for results clear of overflow and
- * underflow); dim = sequential chains, bit-identical. maxval / minval: the
+ * underflow); dim = sequential chains, bit-identical to the restated loop
+ * (the oracle's or_fibre_reduce_dim). maxval / minval: the
  * reference's scan (first element, replaced by any later x > res / x < res,
  * lexicographic on (hi, lo)), bit-identical in every form including NaN
  * limbs; 0 when no element is selected. */
@@ -294,10 +297,12 @@ MULTIFLOATS_API void wgemmdd(char transa, char transb, int64_t m, int64_t n, int
 
 /* Triangular solve op(A) X = alpha B (side 'L') or X op(A) = alpha B (side
  * 'R'), X overwriting B: the reference's xTRSM shim
- * test/integration/blas/wtrsm.f90:1-10, bit for bit (its eight loop nests,
- * 57-206, in the same operation order, including the skips of zero pivots
- * and zero coefficients). uplo 'U'/'L', transa 'N'/'T'/'C', diag 'U' (unit,
- * diagonal not read) or 'N'. Argument checks and quick returns as wtrsm.f90:
+ * test/integration/blas/wtrsm.f90:1-10, bit-identical to the restated loop
+ * nests (the oracle's or_trsm restates its eight loop nests, 57-206, in the
+ * same operation order, including the skips of zero pivots and zero
+ * coefficients; the Fortran shim itself is not compiled by the tests).
+ * uplo 'U'/'L', transa 'N'/'T'/'C', diag 'U' (unit, diagonal not read) or
+ * 'N'. Argument checks and quick returns as wtrsm.f90:
  * 24-56 (alpha == 0: B = 0). Invalid arguments write NaN to B. */
 MULTIFLOATS_API void wtrsmdd(char side, char uplo, char transa, char diag, int64_t m, int64_t n, float64x2 alpha,
                              const float64x2 *a, int64_t lda, float64x2 *b, int64_t ldb);
