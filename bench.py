@@ -197,13 +197,14 @@ def ozaki_run(torch, mf, dA, dB, dC, m, k, n, steps, warmup, stream, slices=0):
         mf.dev_matmul_ozaki(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), m, k, n, slices, work.data_ptr(), wb, stream)
     ms = timed(torch, step, steps, warmup)
     S, fix = mf.ozaki_info(work.data_ptr(), m, k, n, slices, stream)
+    n_used = mf.ozaki_moduli_used(work.data_ptr(), m, k, n, slices, stream)
     C = dC.cpu().numpy().copy()
 
     def step_explicit():
         mf.dev_matmul_ozaki(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), m, k, n, S, work.data_ptr(), wb, stream)
     ms_explicit = timed(torch, step_explicit, max(3, steps // 2), 1)
     N = mf.ozaki_moduli(m, k, n, S)
-    res = dict(ms=ms, ms_explicit_slices=ms_explicit, slices=S, fixups=fix, moduli=N)
+    res = dict(ms=ms, ms_explicit_slices=ms_explicit, slices=S, fixups=fix, moduli=n_used, moduli_explicit=N)
     if N:
         mf.dev_ozaki_prepare(dA.data_ptr(), dB.data_ptr(), m, k, n, S, work.data_ptr(), wb, stream)
         ms_kern = timed(torch, lambda: mf.dev_ozaki_products(m, k, n, S, work.data_ptr(), wb, stream),
@@ -214,8 +215,10 @@ def ozaki_run(torch, mf, dA, dB, dC, m, k, n, steps, warmup, stream, slices=0):
                            "frac": round(tops / INT8_PEAK_TOPS, 4),
                            "traffic": PMC_TRAFFIC.get(("k_oz2_gemm", m, S)) if m == n == k else None,
                            "note": "k_oz2_gemm: %d residue INT8 GEMMs x 2mnk int8 ops per launch (MFMA), timed alone "
-                                   "on its stream; the rest of the call: exponent scan, residues with the certificate "
-                                   "operands, certificate GEMM (bf16 MFMA), plans, reconstruction, fix-up" % N}
+                                   "on its stream (the explicit slice count's worst-case moduli count; the automatic "
+                                   "call ran %d, bounded by its data); the rest of the call: exponent scan, residues "
+                                   "with the certificate operands, certificate GEMM (FP8 MFMA), plans, reconstruction, "
+                                   "fix-up" % (N, n_used)}
     del work
     return res, C
 

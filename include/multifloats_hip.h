@@ -320,7 +320,7 @@ MULTIFLOATS_API void wtrsmdd(char side, char uplo, char transa, char diag, int64
  *     |C - AB|_ij <= 2^-100 (|A||B|)_ij
  * for any finite input whose products and results stay in the normal range
  * (2^-968 .. 2^1023), whatever the distribution of magnitudes. A certificate
- * (a bf16 MFMA product of the scaled magnitudes, bounding (|A||B|)_ij from
+ * (an FP8 MFMA product of the scaled magnitudes, bounding (|A||B|)_ij from
  * below, against a rigorous bound of the digit representation's error) gives
  * each output the digits it needs; the slice count is then chosen on the
  * device, and the outputs it does not cover (e.g. rows whose largest entries
@@ -437,6 +437,14 @@ MULTIFLOATS_API int mfhip_ozaki_matmul_mm(const float64x2 *a, const float64x2 *b
  * explicit counts). Synchronises `stream`. No counterpart in the reference. */
 MULTIFLOATS_API int mfhip_ozaki_info(const void *work, int64_t m, int64_t k, int64_t n, int slices,
                                      int *slices_used, int64_t *fixups, void *stream);
+/* Moduli count of the residue GEMMs of the last Ozaki call on `work` (0 when
+ * it ran the triangular form). An automatic call uses no more moduli than
+ * mfhip_ozaki_moduli gives for its slice count, and fewer when the data
+ * bound the product (the certificate's row and column sums bound every
+ * |C_int| well below the worst case k 2^(16 slices - 2)). Synchronises
+ * `stream`. No counterpart in the reference. */
+MULTIFLOATS_API int mfhip_ozaki_moduli_used(const void *work, int64_t m, int64_t k, int64_t n, int slices,
+                                            int *moduli, void *stream);
 /* Slice count and fix-up count of the calling thread's last
  * matmuldd_mm_ozaki call; after mfhip_ozaki_matmul_mm the count is the
  * explicit one, or -1 for an automatic call (use mfhip_ozaki_info). */

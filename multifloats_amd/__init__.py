@@ -132,6 +132,7 @@ def lib():
         L.mfhip_ozaki_fixups.argtypes = []
         L.mfhip_ozaki_fixups.restype = ctypes.c_int64
         L.mfhip_ozaki_info.argtypes = [P, I64, I64, I64, I, ctypes.POINTER(I), ctypes.POINTER(I64), VP]
+        L.mfhip_ozaki_moduli_used.argtypes = [P, I64, I64, I64, I, ctypes.POINTER(I), VP]
         L.mfhip_ozaki_choose_slices.argtypes = [I64, I, I]
         L.mfhip_ozaki_moduli.argtypes = [I64, I64, I64, I]
         L.mfhip_ozaki_prepare.argtypes = [P, P, I64, I64, I64, I, P, ctypes.c_size_t, VP]
@@ -601,6 +602,14 @@ def ozaki_info(work, m, k, n, slices=0, stream=0):
     F = ctypes.c_int64(0)
     _dev(lib().mfhip_ozaki_info(work, m, k, n, slices, ctypes.byref(S), ctypes.byref(F), stream))
     return S.value, F.value
+
+
+def ozaki_moduli_used(work, m, k, n, slices=0, stream=0):
+    """Moduli of the residue GEMMs of the last Ozaki call on the device
+    workspace `work` (0 for the triangular form); synchronises the stream."""
+    N = ctypes.c_int(0)
+    _dev(lib().mfhip_ozaki_moduli_used(work, m, k, n, slices, ctypes.byref(N), stream))
+    return N.value
 
 
 def ozaki_choose_slices(k, range_a, range_b):

@@ -399,7 +399,7 @@ hipError_t host_mm_ozaki(const float64x2 *a, const float64x2 *b, float64x2 *c, i
   TRY(mfhip::ozaki_gemm(D(da), D(db), W(dc), m, k, n, slices, dw, wb, tl_stage.stream));
   TRY(d2h(c, dc, 16 * (size_t)m * n));
   // the call is synchronous anyway: read back what the device chose
-  return mfhip::ozaki_info(dw, m, k, n, slices, &tl_oz_used, &tl_oz_fix, tl_stage.stream);
+  return mfhip::ozaki_info(dw, m, k, n, slices, &tl_oz_used, &tl_oz_fix, nullptr, tl_stage.stream);
 }
 
 float64x2 nan_dd() {
@@ -800,7 +800,13 @@ int mfhip_ozaki_info(const void *work, int64_t m, int64_t k, int64_t n, int slic
                      int64_t *fixups, void *stream) {
   CHECK_ALIGN(work);
   if (!work) return status(hipErrorInvalidValue);
-  return status(mfhip::ozaki_info(work, m, k, n, slices, slices_used, fixups, (hipStream_t)stream));
+  return status(mfhip::ozaki_info(work, m, k, n, slices, slices_used, fixups, nullptr, (hipStream_t)stream));
+}
+int mfhip_ozaki_moduli_used(const void *work, int64_t m, int64_t k, int64_t n, int slices, int *moduli,
+                            void *stream) {
+  CHECK_ALIGN(work);
+  if (!work || !moduli) return status(hipErrorInvalidValue);
+  return status(mfhip::ozaki_info(work, m, k, n, slices, nullptr, nullptr, moduli, (hipStream_t)stream));
 }
 int mfhip_ozaki_slices_used(void) { return tl_oz_used; }
 int64_t mfhip_ozaki_fixups(void) { return tl_oz_fix; }

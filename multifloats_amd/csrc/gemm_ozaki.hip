@@ -504,7 +504,7 @@ __device__ void oz_set_plan(OzCtl *ctl, int S, int N, int t) {
     float inv = 0.0f;
     if (t < N) {
       const int mt = mf_oz2_m[t];
-      int pa = 1 % mt, pb = mf_oz2_y[(N - 1) * MF_OZ2_MODULI + t] % mt;
+      int pa = 1 % mt, pb = pa;
       for (int j = 0; j < S; ++j) {
         wa[j / 4] |= bal_byte(pa, mt) << (8 * (j % 4));
         wb[j / 4] |= bal_byte(pb, mt) << (8 * (j % 4));
@@ -548,9 +548,32 @@ __global__ __launch_bounds__(64) void k_oz_plan_begin(OzCtl *__restrict__ ctl, c
     ctl->redo = 0;
     ctl->S_first = S_explicit > 0 ? 0 : S;
     ctl->ncand = ctl->nfix = 0ull;
+    ctl->rmax = ctl->cmax = 0ull;
   }
   if (t < kOzBins) ctl->hist[t] = 0ull;
   oz_set_plan(ctl, S, nt.n[S], t);
+}
+
+// Moduli count from the data (automatic calls): the products need
+// M > 2 max |C_int|, and with X_ip = RN(a_ip 2^(8S-2-E_i)),
+// |X_ip| <= |a_ip| 2^(8S-2-E_i) + 1/2 and every |Y_pj| < 2^(8S-1) (the
+// headroom rule keeps |b| 2^(6-F) < 127),
+//   |C_ij| <= sum_p |X_ip| |Y_pj| < 2^(8S-1) (2^(8S-2) R_A,i + k),
+// and the same with the roles of A and B exchanged; R_A,i = sum_p |a_ip|
+// 2^-E_i is bounded by the certificate's row sum (units of 2^-40, rounded up).
+// rmax / cmax are the largest row / column sums of the live rows and columns,
+// so C_max = 2^(8S-1) (2^(8S-2) min(rmax, cmax) 2^-40 + k) bounds every
+// output (a dead row or column gives zero products). The count is the
+// smallest N whose M (a double within 2^-53 of it) exceeds 2 C_max (1 + 2^-40),
+// never more than n_worst, the count for the worst case k 2^(16S-2).
+__device__ __forceinline__ int oz_data_moduli(int n_worst, int S, int64_t k, unsigned long long rmax,
+                                              unsigned long long cmax) {
+  const unsigned long long r = rmax < cmax ? rmax : cmax;
+  const double cm = __builtin_ldexp(__builtin_ldexp((double)r * (1.0 + 0x1p-50), 8 * S - 42) + (double)k, 8 * S - 1);
+  const double need = 2.0 * cm * (1.0 + 0x1p-40);
+  for (int N = 1; N < n_worst; ++N)
+    if (mf_oz2_mp[2 * (N - 1)] > need) return N;
+  return n_worst;
 }
 
 // Final plan of an automatic call (one wave), after the certificate: the
@@ -560,7 +583,7 @@ __global__ __launch_bounds__(64) void k_oz_plan_begin(OzCtl *__restrict__ ctl, c
 // no more than kMaxSlices does. A raised S sets `redo`: the residues are then
 // formed again with the new weights.
 __global__ __launch_bounds__(64) void k_oz_plan_final(OzCtl *__restrict__ ctl, OzNtab nt,
-                                                      unsigned long long budget) {
+                                                      unsigned long long budget, int64_t k) {
   const int S0 = ctl->S;
   unsigned long long over[kOzBins];  // over[s] = candidates needing more than s slices
   unsigned long long acc = 0;
@@ -571,14 +594,16 @@ __global__ __launch_bounds__(64) void k_oz_plan_final(OzCtl *__restrict__ ctl, O
   const unsigned long long target = budget > over[kMaxSlices] ? budget : over[kMaxSlices];
   int S = S0;
   while (S < kMaxSlices && over[S] > target) ++S;
+  const int N = oz_data_moduli(nt.n[S], S, k, ctl->rmax, ctl->cmax);
   const int t = threadIdx.x;
-  if (S != S0) {
+  if (S != S0 || N != ctl->N) {
+    __syncthreads();  // every lane has read ctl->S and ctl->N above
     if (t == 0) {
       ctl->S = S;
-      ctl->N = nt.n[S];
-      ctl->redo = 1;
+      ctl->N = N;
+      ctl->redo = S != S0;
     }
-    oz_set_plan(ctl, S, nt.n[S], t);
+    oz_set_plan(ctl, S, N, t);
   }
 }
 
@@ -621,10 +646,9 @@ constexpr int waitcnt_vm(int n) { return (n & 15) | ((n >> 4) << 14) | (7 << 4) 
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 // The two uses of the GEMM kernel below: the N residue products of the CRT
 // form (INT8 MFMA, exact INT32 sums) and the certificate of an automatic call
-// (bf16 MFMA on the magnitude operands, FP32 sums).
+// (FP8 MFMA on the magnitude operands, FP32 sums).
 constexpr int kOzRes = 0;
 constexpr int kOzMag = 1;
-typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 template <int MODE>
 struct O2Mma;
@@ -635,17 +659,24 @@ struct O2Mma<kOzRes> {
     return __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c, 0, 0, 0);
   }
 };
+// A fragment holds 16 k of one K block: its two 8-byte halves go to two
+// v_mfma_f32_32x32x16_fp8_fp8 (OCP E4M3 operands). Both operands split the
+// same way, so every k meets its partner once.
 template <>
 struct O2Mma<kOzMag> {
   typedef v16f acc_t;
   static __device__ __forceinline__ acc_t run(v4i a, v4i b, acc_t c) {
-    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), c, 0,
-                                                   0, 0);
+    const long a0 = (long)(((unsigned long)(unsigned)a[1] << 32) | (unsigned)a[0]);
+    const long a1 = (long)(((unsigned long)(unsigned)a[3] << 32) | (unsigned)a[2]);
+    const long b0 = (long)(((unsigned long)(unsigned)b[1] << 32) | (unsigned)b[0]);
+    const long b1 = (long)(((unsigned long)(unsigned)b[3] << 32) | (unsigned)b[2]);
+    c = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(a0, b0, c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(a1, b1, c, 0, 0, 0);
   }
 };
 
 struct Oz2GemmArgs {
-  const signed char *As, *Bs;  // operand planes (residues, or bf16 magnitudes)
+  const signed char *As, *Bs;  // operand planes (residues, or FP8 magnitudes)
   unsigned char *R;            // digit planes (kOzRes)
   int64_t m, n, KB, TBA, TBB;  // KB: 32-byte K blocks per chunk row, padded to kO2KS
   int tiles_m, tiles_n;
@@ -662,9 +693,10 @@ struct Oz2GemmArgs {
 // R_B = sum_p |b_pj| 2^-F (each element within 2^(E + 1 - 8S) of its S-slice
 // integer; the 2^-20 covers the second-order term and the reconstruction's
 // limb truncation), and its reconstruction adds at most 2^-103 (|A||B|)_ij.
-// (|A||B|)_ij >= 2^(E + F) G (1 - 2^-6): G sums products of bf16 values
-// rounded down, in FP32 with at most k roundings of non-negative terms
-// (k <= 2^15). So 2^-100 (|A||B|)_ij holds once
+// (|A||B|)_ij >= 2^(E + F) G (1 - 2^-6): G sums products of FP8 values
+// rounded down (the GEMM's sums of the 2^7-scaled magnitudes, times 2^-14),
+// in FP32 with at most k roundings of non-negative terms (k <= 2^15). So
+// 2^-100 (|A||B|)_ij holds once
 // 8S >= 101.2 + log2((R_A + R_B) / (G (1 - 2^-6))); 0.1 more covers the FP32
 // logarithms. Returns the smallest such S in kMinSlices..kMaxSlices, or
 // kOverCap.
@@ -694,7 +726,8 @@ __global__ __launch_bounds__(kMainThreads) __attribute__((amdgpu_waves_per_eu(1,
   typedef typename O2Mma<MODE>::acc_t acc_t;
   extern __shared__ __attribute__((aligned(16))) signed char lds[];
   const int t = blockIdx.y;
-  if (MODE == kOzRes && t >= g.ctl->N) return;  // planes beyond the call's moduli count
+  const int Nc = MODE == kOzRes ? g.ctl->N : 0;
+  if (MODE == kOzRes && t >= Nc) return;  // planes beyond the call's moduli count
   __shared__ unsigned hist_l[kOzBins];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -802,13 +835,16 @@ __global__ __launch_bounds__(kMainThreads) __attribute__((amdgpu_waves_per_eu(1,
   // Accumulator e of block (a, b) is row 8(e/4) + 4 half + e%4 and column r of
   // that 32 x 32 block.
   if constexpr (MODE == kOzRes) {
-    // B's residues carry the factor y_t, so v == c_t y_t (mod m_t). q = RN(v / m)
-    // is exact (v / m is an exact binary fraction for m = 256 and at least
-    // 1/(2m) from a half-integer otherwise), so the fma gives the balanced
-    // digit v - q m in [-m/2, m/2] exactly; it is stored as a signed byte
-    // (+128 for m = 256 wraps to -128, the same residue).
+    // v == c_t (mod m_t); the CRT digit is u = v y_t mod m_t with the call's
+    // final moduli count N. w = v y_t is exact in a double (|w| < 2^37), and
+    // q = RN(w / m) is exact (w / m is an exact binary fraction for m = 256
+    // and at least 1/(2m) from a half-integer otherwise, far beyond the
+    // product's rounding), so the fma gives the balanced digit w - q m in
+    // [-m/2, m/2] exactly; it is stored as a signed byte (+128 for m = 256
+    // wraps to -128, the same residue).
     const int64_t m = g.m, n = g.n;
     const double md = (double)mf_oz2_m[t], inv = 1.0 / md;
+    const double yt = (double)mf_oz2_y[(Nc - 1) * MF_OZ2_MODULI + t];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -821,7 +857,7 @@ __global__ __launch_bounds__(kMainThreads) __attribute__((amdgpu_waves_per_eu(1,
           unsigned word = 0;
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
-            const double v = (double)acc[a][b][4 * q + e];
+            const double v = (double)acc[a][b][4 * q + e] * yt;
             const double u = __builtin_fma(-__builtin_rint(v * inv), md, v);
             word |= ((unsigned)(int)u & 255u) << (8 * e);
           }
@@ -841,31 +877,48 @@ __global__ __launch_bounds__(kMainThreads) __attribute__((amdgpu_waves_per_eu(1,
     // Inf/NaN (NaN outputs) need none.
     const int64_t m = g.m, n = g.n;
     const int Sf = g.ctl->S;
+    unsigned long long rmx = 0ull, cmx = 0ull;  // largest row / column sums (live ones)
+    // the lane's 4 columns, then per 32-row block its 16 rows: each row's and
+    // column's exponent and sum are loaded once for the tile
+    bool cl[4];
+    double rb[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
       const int64_t col = (int64_t)kO2Tile * jb + 128 * wj + 32 * b + r;
       const int Fj = col < n ? g.F[col] : kEmpty;
-      const bool cl = Fj != kEmpty && Fj != kNonFinite;
-      const double rb = cl ? (double)g.csum[col] * 0x1p-40 : 0.0;
+      cl[b] = Fj != kEmpty && Fj != kNonFinite;
+      const unsigned long long csj = cl[b] ? g.csum[col] : 0ull;
+      cmx = csj > cmx ? csj : cmx;
+      rb[b] = (double)csj * 0x1p-40;
+    }
 #pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        unsigned need = 0;  // 5 bits per output of this 32 x 32 block: 16 outputs per lane
+    for (int a = 0; a < 4; ++a) {
+      double ra[16];
+      unsigned live = 0;  // bit e: row e of this block holds finite, nonzero entries
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int64_t row = (int64_t)kO2Tile * ib + 128 * wi + 32 * a + 8 * (e >> 2) + 4 * half + (e & 3);
+        const int Ei = row < m ? g.E[row] : kEmpty;
+        const bool rl = Ei != kEmpty && Ei != kNonFinite;
+        const unsigned long long rsi = rl ? g.rsum[row] : 0ull;
+        rmx = rsi > rmx ? rsi : rmx;
+        ra[e] = (double)rsi * 0x1p-40;
+        live |= (unsigned)rl << e;
+      }
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        unsigned need = 0;  // outputs of this 32 x 32 block over the first count: 16 per lane
         unsigned long long packed[2] = {0ull, 0ull};
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
-          const int64_t row = (int64_t)kO2Tile * ib + 128 * wi + 32 * a + 8 * (e >> 2) + 4 * half + (e & 3);
-          int sl = 0;
-          if (cl && row < m) {
-            const int Ei = g.E[row];
-            if (Ei != kEmpty && Ei != kNonFinite)
-              sl = oz_cert_slices(acc[a][b][e], (double)g.rsum[row] * 0x1p-40, rb);
-          }
+          const int sl = cl[b] && ((live >> e) & 1u) ? oz_cert_slices(acc[a][b][e] * 0x1p-14f, ra[e], rb[b]) : 0;
           if (sl > Sf) {
             ++need;
             packed[e >> 3] |= (unsigned long long)sl << (8 * (e & 7));
           }
         }
         if (__ballot(need != 0)) {  // uniform: the candidates of this block
+          const int64_t col = (int64_t)kO2Tile * jb + 128 * wj + 32 * b + r;
           unsigned total;
           const unsigned before = wave_excl_scan(need, lane, total);
           unsigned long long base = 0;
@@ -883,6 +936,16 @@ __global__ __launch_bounds__(kMainThreads) __attribute__((amdgpu_waves_per_eu(1,
           }
         }
       }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long ro = __shfl_xor(rmx, off), co = __shfl_xor(cmx, off);
+      rmx = ro > rmx ? ro : rmx;
+      cmx = co > cmx ? co : cmx;
+    }
+    if (lane == 0) {
+      if (rmx) atomicMax(&g.ctl->rmax, rmx);
+      if (cmx) atomicMax(&g.ctl->cmax, cmx);
     }
     __syncthreads();
     if (tid < kOzBins && hist_l[tid]) atomicAdd(&g.ctl->hist[tid], (unsigned long long)hist_l[tid]);
@@ -1135,14 +1198,14 @@ Oz2Dims oz2_dims(int64_t m, int64_t k, int64_t n, int S) {
   return d;
 }
 // residue planes of A and B, digit planes; with `cert` (automatic calls) also
-// the bf16 magnitude operands (16 k per chunk row: 2 KBp K blocks) and the
+// the FP8 magnitude operands (the residues' chunk layout: KBp K blocks) and the
 // candidate list (one 8-byte entry per output at most)
 size_t oz2_bytes(int64_t m, int64_t k, int64_t n, int S, bool cert) {
   const Oz2Dims d = oz2_dims(m, k, n, S);
   size_t b = align256((size_t)d.N * d.TBA * d.KBp * kChunk) + align256((size_t)d.N * d.TBB * d.KBp * kChunk) +
              align256((size_t)d.N * m * n);
   if (cert)
-    b += align256((size_t)d.TBA * 2 * d.KBp * kChunk) + align256((size_t)d.TBB * 2 * d.KBp * kChunk) +
+    b += align256((size_t)d.TBA * d.KBp * kChunk) + align256((size_t)d.TBB * d.KBp * kChunk) +
          align256(sizeof(unsigned long long) * (size_t)m * n);
   return b;
 }
@@ -1230,9 +1293,9 @@ Oz2Work carve2(const OzWork &w, int64_t m, int64_t k, int64_t n, int S) {
   v.R = (unsigned char *)p;
   p += align256((size_t)v.d.N * m * n);
   v.magA = (unsigned char *)p;
-  p += align256((size_t)v.d.TBA * 2 * v.d.KBp * kChunk);
+  p += align256((size_t)v.d.TBA * v.d.KBp * kChunk);
   v.magB = (unsigned char *)p;
-  p += align256((size_t)v.d.TBB * 2 * v.d.KBp * kChunk);
+  p += align256((size_t)v.d.TBB * v.d.KBp * kChunk);
   v.cand = (unsigned long long *)p;
   if (S > 0) v.magA = v.magB = nullptr, v.cand = nullptr;  // explicit calls have no certificate
   return v;
@@ -1414,11 +1477,11 @@ hipError_t ozaki_auto(const double2 *A, const double2 *B, double2 *C, int64_t m,
   g.As = (const signed char *)v.magA;
   g.Bs = (const signed char *)v.magB;
   g.R = nullptr;
-  g.KB = 2 * v.d.KBp;
+  g.KB = v.d.KBp;
   g.cand = v.cand;
   hipError_t e = oz2_launch_gemm<kOzMag>(g, 1, s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_oz_plan_final, dim3(1), dim3(64), 0, s, w.ctl, nt, oz_fix_budget(m, n));
+  hipLaunchKernelGGL(k_oz_plan_final, dim3(1), dim3(64), 0, s, w.ctl, nt, oz_fix_budget(m, n), k);
   oz2_resid(A, B, m, k, n, w, v, 1, s);
   e = oz2_products(w, v, m, n, s);
   if (e != hipSuccess) return e;
@@ -1441,7 +1504,8 @@ hipError_t ozaki_gemm(const double2 *A, const double2 *B, double2 *C, int64_t m,
 }
 
 hipError_t ozaki_info(const void *work, int64_t m, int64_t k, int64_t n, int S, int *slices, int64_t *fixups,
-                      hipStream_t s) {
+                      int *moduli, hipStream_t s) {
+  if (moduli) *moduli = 0;
   if (m <= 0 || n <= 0 || k <= 0) {
     if (slices) *slices = 0;
     if (fixups) *fixups = 0;
@@ -1453,14 +1517,18 @@ hipError_t ozaki_info(const void *work, int64_t m, int64_t k, int64_t n, int S, 
     if (fixups) *fixups = 0;
     return hipSuccess;
   }
-  int hS = 0;
+  int hS = 0, hN = 0;
   unsigned long long hf = 0;
   hipError_t e = hipMemcpyAsync(&hS, &w.ctl->S, sizeof(int), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(&hN, &w.ctl->N, sizeof(int), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipMemcpyAsync(&hf, &w.ctl->nfix, sizeof(hf), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) return e;
   if (slices) *slices = hS;
   if (fixups) *fixups = (int64_t)hf;
+  // the triangular form (automatic calls with m or n below 512 run it when
+  // the plan says so) leaves the CRT form's count unused
+  if (moduli) *moduli = use_oz2(m, n, k, hS) ? hN : 0;
   return hipSuccess;
 }
 

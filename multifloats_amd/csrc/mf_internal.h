@@ -86,9 +86,10 @@ hipError_t ozaki_gemm(const double2 *A, const double2 *B, double2 *C, int64_t m,
                       int64_t n, int S, void *work, size_t work_bytes, hipStream_t s);
 // The exponent-range rule that starts an automatic call's search.
 int ozaki_choose_slices(int64_t k, int range_a, int range_b);
-// Slice count and fix-up count of the last call on `work` (synchronises s).
+// Slice count, fix-up count and moduli count (CRT form; 0 otherwise) of the
+// last call on `work` (synchronises s).
 hipError_t ozaki_info(const void *work, int64_t m, int64_t k, int64_t n, int S, int *slices, int64_t *fixups,
-                      hipStream_t s);
+                      int *moduli, hipStream_t s);
 // Moduli of the CRT form that a call with these sizes and S slices runs
 // (0: the triangular form runs).
 int ozaki_moduli(int64_t m, int64_t k, int64_t n, int S);

@@ -38,13 +38,16 @@ __device__ __forceinline__ void int_double_to_i128(double D, uint64_t &lo, uint6
   hi = h;
 }
 
-// Magnitude of a certificate operand: |h| 2^-e (0 <= x < 2 here) as a bf16
-// rounded toward zero, so it never exceeds the element's magnitude: RN to
-// float, stepped down when that rounded up, then the top 16 bits.
-__device__ __forceinline__ unsigned bf16_floor(double x) {
-  float f = (float)x;
-  if ((double)f > x) f = __uint_as_float(__float_as_uint(f) - 1u);
-  return __float_as_uint(f) >> 16;
+// Magnitude of a certificate operand: x = |h| 2^(7-e) (0 <= x < 256 here) as
+// an OCP FP8 E4M3 byte rounded toward zero, so it never exceeds the
+// element's scaled magnitude: normal values keep the top three mantissa bits
+// of the double (truncation), x < 2^-6 becomes the subnormal m 2^-9 with
+// m = floor(x 2^9) (0 below 2^-9).
+__device__ __forceinline__ unsigned fp8_floor(double x) {
+  const uint64_t bits = (uint64_t)__double_as_longlong(x);
+  const int ex = (int)((bits >> 52) & 0x7FF) - 1023;
+  if (ex >= -6) return (unsigned)(((ex + 7) << 3) | (int)((bits >> 49) & 7));
+  return (unsigned)(int)__builtin_ldexp(x, 9);
 }
 
 // Residue planes of A (or of B with TRANS): plane t holds the 64 x 32 chunks
@@ -83,9 +86,9 @@ __device__ __forceinline__ unsigned bf16_floor(double x) {
 // S, N and the weights come from the call's control block. pass 1 is the
 // second residue pass of an automatic call: it runs only when the plan raised
 // S (ctl->redo). With mag set (first pass of an automatic call) the kernel
-// also writes the certificate operand -- |hi| 2^-E as bf16 rounded down, in
-// the same chunk layout with 16 k-elements per 32-byte row (K block 2 kb + c0 /
-// 16) -- and adds an upper bound of sum_p (|hi| + |lo|) 2^-E over its chunk,
+// also writes the certificate operand -- |hi| 2^(7-E) as FP8 E4M3 rounded
+// down, in the residues' chunk layout (32 k-elements per 32-byte row) -- and
+// adds an upper bound of sum_p (|hi| + |lo|) 2^-E over its chunk,
 // in units of 2^-40 rounded up, to sums[row] (integer atomics: the total does
 // not depend on their order).
 template <bool TRANS>
@@ -136,21 +139,21 @@ __global__ __launch_bounds__(kThreads) void k_oz2_resid(const double2 *__restric
     const int r = lane, c0 = 8 * wave;
     const int Ei = Es[r];
     const bool live = Ei != kEmpty && Ei != kNonFinite;
-    unsigned mg[4] = {0, 0, 0, 0};
+    unsigned mg[2] = {0, 0};
     unsigned long long part = 0;
     if (live) {
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         const double2 v = stage[r * kPad + c0 + q];
         const double ah = fabs(v.x);
-        mg[q >> 1] |= bf16_floor(__builtin_ldexp(ah, -Ei)) << (16 * (q & 1));
+        mg[q >> 2] |= fp8_floor(__builtin_ldexp(ah, 7 - Ei)) << (8 * (q & 3));
         // (|hi| + |lo|) 2^(40-E) < 2^41; + 1 covers the rounding of the sum
         part += (unsigned long long)__builtin_ceil(__builtin_ldexp(ah + fabs(v.y), 40 - Ei)) + (ah != 0.0);
       }
     }
-    // 8 bf16 = one 16-byte half of a 32-byte chunk row of K block 2 kb + c0 / 16
-    unsigned char *dm = mag + ((int64_t)ib * (2 * KB) + 2 * kb + (c0 >> 4)) * kChunk + tile_off(r, 2 * (c0 & 15));
-    *(v4i *)dm = v4i{(int)mg[0], (int)mg[1], (int)mg[2], (int)mg[3]};
+    // 8 FP8 bytes: k = c0 .. c0 + 7 of row r in the chunk of K block kb
+    unsigned char *dm = mag + ((int64_t)ib * KB + kb) * kChunk + tile_off(r, c0);
+    *(uint2 *)dm = make_uint2(mg[0], mg[1]);
     rsum[wave][r] = part;
     __syncthreads();
     if (tid < kT) {

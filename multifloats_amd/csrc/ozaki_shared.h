@@ -38,12 +38,12 @@ struct Oz2Consts {
   double mp_hi, mp_lo;  // M 2^-(16(S-1)) as a double-double
 };
 
-// Weights of the residue pass: byte j of w[t] (16 bytes) is f_t 256^j mod m_t,
-// balanced into int8, and zero for S <= j < 15, so that f_t X mod m_t ==
+// Weights of the residue pass: byte j of w[t] (16 bytes) is 256^j mod m_t,
+// balanced into int8, and zero for S <= j < 15, so that X mod m_t ==
 // sum_j e_j w_tj for X = sum_j e_j 256^j; byte 15 is 64 when S <= 15 (the
-// residue pass's bias slot). f_t = 1 for A;
-// for B, f_t = y_t = (M / m_t)^-1 mod m_t, so that the residue GEMM sums
-// straight to the CRT digit c_t y_t.
+// residue pass's bias slot). The residues do not depend on how many moduli
+// the call ends up using: the CRT factor y_t = (M / m_t)^-1 mod m_t enters in
+// the residue GEMM's epilogue, once the moduli count is final.
 struct Oz2Weights {
   int w[MF_OZ2_MODULI][4];
   float inv[MF_OZ2_MODULI];  // RN(1 / m_t)
@@ -67,6 +67,7 @@ struct OzCtl {
   int S_first;  // slices of the first residue pass (0: explicit slice count)
   unsigned long long hist[kOzBins];  // candidate outputs by the slices they need
   unsigned long long ncand;          // outputs in the candidate list
+  unsigned long long rmax, cmax;     // largest certificate sum over live rows of A / columns of B
   unsigned long long nfix;           // outputs the fix-up recomputed
   Oz2Consts cs;
   Oz2Weights wa, wb;  // residue weights of A (f_t = 1) and of B (f_t = y_t)

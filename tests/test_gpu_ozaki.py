@@ -138,6 +138,10 @@ def test_ozaki_4096_sampled(gpu):
     assert mf.ozaki_slices_used() == -1  # chosen on the device
     used, fix = mf.ozaki_info(work.data_ptr(), m, k, n, 0, s)
     assert used in (14, 15) and fix <= 16384, (used, fix)
+    # the certificate's row and column sums bound |C_int| below the worst
+    # case k 2^(16S - 2): at least one modulus fewer on these inputs
+    n_used = mf.ozaki_moduli_used(work.data_ptr(), m, k, n, 0, s)
+    assert 0 < n_used < mf.ozaki_moduli(m, k, n, used), (n_used, used)
     C = datagen.unflat(dC.cpu().numpy(), m, n)
     rows = rng.choice(m, 24, replace=False)
     cols = rng.choice(n, 24, replace=False)
