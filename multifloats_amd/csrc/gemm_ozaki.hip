@@ -43,17 +43,17 @@ __device__ __forceinline__ int dexp(double x) {
   return e - 1;
 }
 
-// Row exponent maxima and minima of A (column-major m x k): thread = row,
-// block.y = k chunk. Zeros are skipped; a non-finite entry marks the row.
+// Row exponent maxima and minima of A (column-major m x k): thread = row of
+// the 256-row block rb, over k chunk kc. Zeros are skipped; a non-finite
+// entry marks the row.
 // Mbits collects the largest |hi| of the row (the bit patterns of
 // non-negative doubles order like the values).
-__global__ __launch_bounds__(kThreads) void k_row_exp(const double2 *__restrict__ A, int64_t m,
-                                                      int64_t k, int64_t kchunk,
-                                                      int *__restrict__ E, int *__restrict__ Emin,
-                                                      unsigned long long *__restrict__ Mbits) {
-  const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+__device__ __forceinline__ void row_exp(const double2 *__restrict__ A, int64_t m, int64_t k, int64_t kchunk,
+                                        int64_t rb, int64_t kc, int *__restrict__ E, int *__restrict__ Emin,
+                                        unsigned long long *__restrict__ Mbits) {
+  const int64_t i = rb * kThreads + threadIdx.x;
   if (i >= m) return;
-  const int64_t p0 = (int64_t)blockIdx.y * kchunk;
+  const int64_t p0 = kc * kchunk;
   const int64_t p1 = p0 + kchunk < k ? p0 + kchunk : k;
   int e = kEmpty, lo = kNonFinite;
   double mx = 0.0;
@@ -97,13 +97,12 @@ __global__ void k_row_finish(int *__restrict__ E, const int *__restrict__ Emin,
 }
 
 // Column exponent maxima of B (column-major k x n) with the same headroom
-// rule: block = column, threads walk p (all of a thread's loads issued before
+// rule: one workgroup for column j, threads walk p (all of a thread's loads issued before
 // its max/min chain); the block reduces within each wave by shuffles, then
 // across the four waves through LDS. Also folds the column's exponent range
 // into *range.
-__global__ __launch_bounds__(kThreads) void k_col_exp(const double2 *__restrict__ B, int64_t k,
-                                                      int *__restrict__ F, int *__restrict__ range) {
-  const int64_t j = blockIdx.x;
+__device__ __forceinline__ void col_exp(const double2 *__restrict__ B, int64_t k, int64_t j,
+                                        int *__restrict__ F, int *__restrict__ range) {
   int e = kEmpty, lo = kNonFinite;
   double mx = 0.0;
   bool bad = false;
@@ -152,6 +151,22 @@ __global__ __launch_bounds__(kThreads) void k_col_exp(const double2 *__restrict_
     F[j] = eh;
     if (eh != kEmpty && eh != kNonFinite) atomicMax(range, eh - lo);
   }
+}
+
+// The exponent scans of A's rows and B's columns in one launch, so that the
+// two read streams overlap: workgroups below rblocks * kchunks take a
+// (256-row block, k chunk) of A, the rest one column of B each. Both parts
+// are uniform per workgroup (the column part synchronises its workgroup).
+__global__ __launch_bounds__(kThreads) void k_scan_exp(const double2 *__restrict__ A, const double2 *__restrict__ B,
+                                                       int64_t m, int64_t k, int64_t kchunk, int64_t kchunks,
+                                                       int64_t rblocks, int *__restrict__ E,
+                                                       int *__restrict__ Emin, unsigned long long *__restrict__ Mbits,
+                                                       int *__restrict__ F, int *__restrict__ range) {
+  const int64_t b = blockIdx.x, nrow = rblocks * kchunks;
+  if (b < nrow)
+    row_exp(A, m, k, kchunk, b / kchunks, b % kchunks, E, Emin, Mbits);
+  else
+    col_exp(B, k, b - nrow, F, range + 1);
 }
 
 // Initial state of the row scan: E = empty, Emin = non-finite, Mbits = 0,
@@ -1307,12 +1322,11 @@ void oz_scan(const double2 *A, const double2 *B, int64_t m, int64_t k, int64_t n
   const int64_t mx = m > n ? m : n;
   hipLaunchKernelGGL(k_scan_init, dim3((unsigned)((mx + 255) / 256)), dim3(256), 0, s, w.E, w.Emin, w.Mbits, w.range,
                      w.rsum, w.csum, m, n);
-  const int64_t kchunk = 64;
-  hipLaunchKernelGGL(k_row_exp, dim3((unsigned)((m + kThreads - 1) / kThreads), (unsigned)((k + kchunk - 1) / kchunk)),
-                     dim3(kThreads), 0, s, A, m, k, kchunk, w.E, w.Emin, w.Mbits);
+  const int64_t kchunk = 64, kchunks = (k + kchunk - 1) / kchunk, rblocks = (m + kThreads - 1) / kThreads;
+  hipLaunchKernelGGL(k_scan_exp, dim3((unsigned)(rblocks * kchunks + n)), dim3(kThreads), 0, s, A, B, m, k, kchunk,
+                     kchunks, rblocks, w.E, w.Emin, w.Mbits, w.F, w.range);
   hipLaunchKernelGGL(k_row_finish, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, w.E, w.Emin, w.Mbits, m,
                      w.range);
-  hipLaunchKernelGGL(k_col_exp, dim3((unsigned)n), dim3(kThreads), 0, s, B, k, w.F, w.range + 1);
 }
 
 // Residue pass of the CRT form over both operands (pass 0: S from the first
