@@ -17,9 +17,11 @@
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define MF_HD __device__ __forceinline__
+#define MF_HDM __device__ __forceinline__  // member functions
 #else
 #include <math.h>
 #define MF_HD static inline
+#define MF_HDM inline
 #endif
 
 #pragma clang fp contract(off)

@@ -811,17 +811,27 @@ MF_HD ddv sum3_round(double b0, double b1, double b2, ddv corr) {
   return r;
 }
 
+// Where erf reads its row tables: the constant tables here, or a copy that
+// an array kernel stages in LDS (elementwise.hip), with the same values, so
+// both give the same bits.
+struct ErfTabConst {
+  MF_HDM double centre(int row) const { return mf_erf_centre[row]; }
+  MF_HDM double base(int row, int i) const { return mf_erf_base[3 * row + i]; }
+  MF_HDM double coef_hi(int row, int j) const { return mf_erf_ddhi[row * MF_ERF_ND + j]; }
+  MF_HDM double coef_lo(int row, int j) const { return mf_erf_ddlo[row * MF_ERF_ND + j]; }
+  MF_HDM double tail(int row, int j) const { return mf_erf_d[row * (MF_ERF_NT - MF_ERF_ND) + j]; }
+};
+
 // P(u) of erf's row (ND double-double coefficients, the tail in double)
-MF_HD ddv erf_poly(int row, ddv u) {
-  const double *dtail = mf_erf_d + row * (MF_ERF_NT - MF_ERF_ND);
-  double d = dtail[MF_ERF_NT - MF_ERF_ND - 1];
-  for (int j = MF_ERF_NT - MF_ERF_ND - 2; j >= 0; --j) d = __builtin_fma(d, u.hi, dtail[j]);
-  const double *ph = mf_erf_ddhi + row * MF_ERF_ND;
-  const double *pl = mf_erf_ddlo + row * MF_ERF_ND;
-  ddv p = dd_add(mk(ph[MF_ERF_ND - 1], pl[MF_ERF_ND - 1]), dd_mul(u, mk(d, 0.0)));
-  for (int j = MF_ERF_ND - 2; j >= 0; --j) p = dd_add(mk(ph[j], pl[j]), dd_mul(u, p));
+template <class Tab>
+MF_HD ddv erf_poly_t(const Tab &tab, int row, ddv u) {
+  double d = tab.tail(row, MF_ERF_NT - MF_ERF_ND - 1);
+  for (int j = MF_ERF_NT - MF_ERF_ND - 2; j >= 0; --j) d = __builtin_fma(d, u.hi, tab.tail(row, j));
+  ddv p = dd_add(mk(tab.coef_hi(row, MF_ERF_ND - 1), tab.coef_lo(row, MF_ERF_ND - 1)), dd_mul(u, mk(d, 0.0)));
+  for (int j = MF_ERF_ND - 2; j >= 0; --j) p = dd_add(mk(tab.coef_hi(row, j), tab.coef_lo(row, j)), dd_mul(u, p));
   return p;
 }
+MF_HD ddv erf_poly(int row, ddv u) { return erf_poly_t(ErfTabConst{}, row, u); }
 
 // Row 0 (|x| < 1/4):   erf(x) = x + x * Q(x^2)
 // Row i (1/4 <= |x| < 9): erf(c_i + t) = a0 + t * P(t), |t| <= 1/32
@@ -830,7 +840,8 @@ MF_HD ddv erf_poly(int row, ddv u) {
 // double Horner in u.hi. a0 is a three-limb constant and the final sum
 // base + mult*P is rounded once, so the result error stays near one half-ulp
 // of a double-double.
-MF_HD ddv dd_erf(ddv x) {
+template <class Tab>
+MF_HD ddv dd_erf_t(const Tab &tab, ddv x) {
   if (x.hi == 0.0 && x.lo == 0.0) return x;
   if (!mf_isfinite(x.hi)) {
     if (x.hi != x.hi) return mk(x.hi, 0.0);
@@ -853,18 +864,19 @@ MF_HD ddv dd_erf(ddv x) {
     b1 = ax.lo;
     b2 = 0.0;
   } else {
-    double th = ax.hi - mf_erf_centre[row];  // exact (Sterbenz)
+    double th = ax.hi - tab.centre(row);  // exact (Sterbenz)
     double tl = ax.lo;
     fast_two_sum(th, tl, th, tl);
     u = mk(th, tl);
     mult = u;
-    b0 = mf_erf_base[3 * row];
-    b1 = mf_erf_base[3 * row + 1];
-    b2 = mf_erf_base[3 * row + 2];
+    b0 = tab.base(row, 0);
+    b1 = tab.base(row, 1);
+    b2 = tab.base(row, 2);
   }
-  ddv r = sum3_round(b0, b1, b2, dd_mul(mult, erf_poly(row, u)));
+  ddv r = sum3_round(b0, b1, b2, dd_mul(mult, erf_poly_t(tab, row, u)));
   return neg ? dd_neg(r) : r;
 }
+MF_HD ddv dd_erf(ddv x) { return dd_erf_t(ErfTabConst{}, x); }
 
 // ---------------------------------------------------------------- erfc, erfcx
 // This project's own design (the reference uses libquadmath's rational fits,

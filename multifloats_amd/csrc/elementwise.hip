@@ -149,6 +149,48 @@ __global__ __launch_bounds__(kThreads) void k_binary(const double2 *__restrict__
   }
 }
 
+// erf over an array with its row tables in LDS. The evaluation picks one of
+// MF_ERF_ROWS Taylor rows per element, so the lanes of a wave read 28
+// coefficients from as many different rows: from the constant tables every
+// such load is a gather over up to 64 cache lines. Each workgroup copies the
+// rows once into LDS as records of kErfRec doubles -- centre, three base
+// limbs, the ND (hi, lo) coefficient pairs, the tail -- whose odd stride
+// starts consecutive rows on different banks, then runs a grid-stride loop
+// over the elements. The arithmetic is dd_erf's, so the bits are the same.
+constexpr int kErfRec = 33;
+static_assert(4 + 2 * MF_ERF_ND + (MF_ERF_NT - MF_ERF_ND) < kErfRec, "row record");
+struct ErfTabLds {
+  const double *t;
+  __device__ __forceinline__ double centre(int row) const { return t[row * kErfRec]; }
+  __device__ __forceinline__ double base(int row, int i) const { return t[row * kErfRec + 1 + i]; }
+  __device__ __forceinline__ double coef_hi(int row, int j) const { return t[row * kErfRec + 4 + 2 * j]; }
+  __device__ __forceinline__ double coef_lo(int row, int j) const { return t[row * kErfRec + 5 + 2 * j]; }
+  __device__ __forceinline__ double tail(int row, int j) const { return t[row * kErfRec + 4 + 2 * MF_ERF_ND + j]; }
+};
+constexpr int kErfThreads = 512;  // 4 workgroups (37 KB of LDS each) and 8 waves per SIMD
+__global__ __launch_bounds__(kErfThreads) void k_erf_lds(const double2 *__restrict__ a, double2 *__restrict__ c,
+                                                         int64_t n) {
+  __shared__ double tab[MF_ERF_ROWS * kErfRec];
+  for (int i = threadIdx.x; i < MF_ERF_ROWS * kErfRec; i += kErfThreads) {
+    const int row = i / kErfRec, f = i - row * kErfRec;
+    double v = 0.0;
+    if (f == 0) v = mf_erf_centre[row];
+    else if (f < 4) v = mf_erf_base[3 * row + f - 1];
+    else if (f < 4 + 2 * MF_ERF_ND) v = ((f - 4) & 1 ? mf_erf_ddlo : mf_erf_ddhi)[row * MF_ERF_ND + (f - 4) / 2];
+    else if (f < 4 + 2 * MF_ERF_ND + (MF_ERF_NT - MF_ERF_ND))
+      v = mf_erf_d[row * (MF_ERF_NT - MF_ERF_ND) + f - 4 - 2 * MF_ERF_ND];
+    tab[i] = v;
+  }
+  __syncthreads();
+  const ErfTabLds lt{tab};
+  const int64_t stride = (int64_t)gridDim.x * kErfThreads;
+  for (int64_t i = (int64_t)blockIdx.x * kErfThreads + threadIdx.x; i < n; i += stride) {
+    const v2d v = __builtin_nontemporal_load((const v2d *)a + i);
+    const ddv r = dd_erf_t(lt, mk(v.x, v.y));
+    __builtin_nontemporal_store(v2d{r.hi, r.lo}, (v2d *)c + i);
+  }
+}
+
 // Grid: enough workgroups to cover n once with U elements per lane, capped so
 // that every CU holds several resident groups (256 CUs on MI355X).
 unsigned grid_for(int64_t n, int U, int max_blocks) {
@@ -170,6 +212,9 @@ hipError_t run_unary(const double2 *a, double2 *c, int64_t n, hipStream_t s) {
                        a, c, n);
   } else if (OP == mfhip::OP_SQRT || OP == mfhip::OP_NEG) {
     hipLaunchKernelGGL((k_unary<OP, 4>), dim3(grid_for(n, 4, 8192)), dim3(kThreads), 0, s, a, c, n);
+  } else if (OP == mfhip::OP_ERF) {
+    // a few resident workgroups per CU, each staging the tables once
+    hipLaunchKernelGGL(k_erf_lds, dim3(grid_for(n, 32, 1024)), dim3(kErfThreads), 0, s, a, c, n);
   } else {
     hipLaunchKernelGGL((k_unary<OP, 1>), dim3(grid_for(n, 1, 1 << 20)), dim3(kThreads), 0, s, a, c, n);
   }
