@@ -58,7 +58,7 @@ INT8_PEAK_TOPS = 5000.0      # MI355X dense INT8 MFMA
 # x 1.0 (1,048,576 KiB). From profiles/hot_kernels_profile_workload_pmc.md
 # (bench.py --workload profile). Keyed by (kernel, size, slices).
 PMC_TRAFFIC = {
-    ("k_oz2_gemm", 4096, 14): (2.0 * 1596081 + 604614) * 1024,
+    ("k_oz2_gemm", 4096, 14): (2.0 * 1618408 + 571875) * 1024,
     ("k_gemm_plain", 4096, 0): (2.0 * 3142426 + 262151) * 1024,
 }
 # the C1 element kernels and reduction passes at N = 2^26, same runs
@@ -67,7 +67,7 @@ C1_TRAFFIC = {
     "mul": (2.0 * 1048586 + 1048576) * 1024, "div": (2.0 * 1048582 + 1048576) * 1024,
     "sqrt": (2.0 * 524300 + 1048645) * 1024, "exp": (2.0 * 524354 + 1048576) * 1024,
     "log": (2.0 * 524384 + 1048576) * 1024, "sin": (2.0 * 524387 + 1048576) * 1024,
-    "erf": (2.0 * 524615 + 1048576) * 1024, "dot": (2.0 * 1048625 + 1024) * 1024,
+    "erf": (2.0 * 524477 + 1048576) * 1024, "dot": (2.0 * 1048625 + 1024) * 1024,
     "sum": (2.0 * 524334 + 512) * 1024, "norm2": (2.0 * 524371 + 512) * 1024,
 }
 
@@ -205,20 +205,22 @@ def ozaki_run(torch, mf, dA, dB, dC, m, k, n, steps, warmup, stream, slices=0):
     ms_explicit = timed(torch, step_explicit, max(3, steps // 2), 1)
     N = mf.ozaki_moduli(m, k, n, S)
     res = dict(ms=ms, ms_explicit_slices=ms_explicit, slices=S, fixups=fix, moduli=n_used, moduli_explicit=N)
-    if N:
-        mf.dev_ozaki_prepare(dA.data_ptr(), dB.data_ptr(), m, k, n, S, work.data_ptr(), wb, stream)
-        ms_kern = timed(torch, lambda: mf.dev_ozaki_products(m, k, n, S, work.data_ptr(), wb, stream),
+    if n_used:
+        # the residue GEMMs of the automatic call itself (its slice and
+        # moduli counts, from the control block the call left in `work`)
+        step()
+        ms_kern = timed(torch, lambda: mf.dev_ozaki_products(m, k, n, slices, work.data_ptr(), wb, stream),
                         max(3, steps // 2), 1)
+        N = n_used
         tops = N * 2.0 * m * n * k / (ms_kern * 1e-3) / 1e12
         res["ms_kernel"] = ms_kern
         res["roofline"] = {"bound": "mfma", "achieved": round(tops, 2), "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
                            "frac": round(tops / INT8_PEAK_TOPS, 4),
                            "traffic": PMC_TRAFFIC.get(("k_oz2_gemm", m, S)) if m == n == k else None,
-                           "note": "k_oz2_gemm: %d residue INT8 GEMMs x 2mnk int8 ops per launch (MFMA), timed alone "
-                                   "on its stream (the explicit slice count's worst-case moduli count; the automatic "
-                                   "call ran %d, bounded by its data); the rest of the call: exponent scan, residues "
-                                   "with the certificate operands, certificate GEMM (FP8 MFMA), plans, reconstruction, "
-                                   "fix-up" % (N, n_used)}
+                           "note": "k_oz2_gemm: %d residue INT8 GEMMs x 2mnk int8 ops per launch (MFMA): the "
+                                   "timed call's own products launch, timed alone on its stream; the rest of the call: "
+                                   "exponent scan, residues with the certificate operands, certificate GEMM (FP8 "
+                                   "MFMA), plans, reconstruction, fix-up" % N}
     del work
     return res, C
 

@@ -438,7 +438,8 @@ MULTIFLOATS_API int mfhip_ozaki_matmul_mm(const float64x2 *a, const float64x2 *b
 MULTIFLOATS_API int mfhip_ozaki_info(const void *work, int64_t m, int64_t k, int64_t n, int slices,
                                      int *slices_used, int64_t *fixups, void *stream);
 /* Moduli count of the residue GEMMs of the last Ozaki call on `work` (0 when
- * it ran the triangular form). An automatic call uses no more moduli than
+ * an explicit count ran the triangular form; automatic calls always run the
+ * CRT form). An automatic call uses no more moduli than
  * mfhip_ozaki_moduli gives for its slice count, and fewer when the data
  * bound the product (the certificate's row and column sums bound every
  * |C_int| well below the worst case k 2^(16 slices - 2)). Synchronises
@@ -474,8 +475,10 @@ MULTIFLOATS_API int mfhip_ozaki_multiply(float64x2 *c, int64_t m, int64_t k, int
  * (mfhip_ozaki_moduli(m, k, n, slices) > 0) alone: the residue INT8 GEMMs
  * from a prepared `work` into `work`, without the reconstruction of C (e.g.
  * to time or schedule the products separately). Returns an error for the
- * triangular form, whose products and recombination are one kernel. No
- * counterpart in the reference. */
+ * triangular form, whose products and recombination are one kernel. With
+ * slices <= 0 it runs the residue GEMMs of the last automatic call on `work`
+ * again, with the slice and moduli counts that call chose. No counterpart in
+ * the reference. */
 MULTIFLOATS_API int mfhip_ozaki_products(int64_t m, int64_t k, int64_t n, int slices, void *work,
                                          size_t work_bytes, void *stream);
 

@@ -1436,8 +1436,14 @@ hipError_t ozaki_prepare(const double2 *A, const double2 *B, int64_t m, int64_t 
 
 hipError_t ozaki_products(int64_t m, int64_t k, int64_t n, int S, void *work, size_t work_bytes, hipStream_t s) {
   if (m <= 0 || n <= 0 || k <= 0) return hipSuccess;
-  if (!oz_args_ok(k, S) || work_bytes < ozaki_workspace_bytes(m, k, n, S) || !use_oz2(m, n, k, S))
+  if (S <= 0) {
+    // the residue GEMMs of the last automatic call on `work`, with the slice
+    // and moduli counts its plan chose (read from the control block)
+    if (k > 32768 || work_bytes < ozaki_workspace_bytes(m, k, n, 0) || oz2_count(kMaxSlices, k) <= 0)
+      return hipErrorInvalidValue;
+  } else if (!oz_args_ok(k, S) || work_bytes < ozaki_workspace_bytes(m, k, n, S) || !use_oz2(m, n, k, S)) {
     return hipErrorInvalidValue;
+  }
   const OzWork w = carve(work, m, k, n, S);
   return oz2_products(w, carve2(w, m, k, n, S), m, n, s);
 }
@@ -1540,9 +1546,9 @@ hipError_t ozaki_info(const void *work, int64_t m, int64_t k, int64_t n, int S, 
   if (e != hipSuccess) return e;
   if (slices) *slices = hS;
   if (fixups) *fixups = (int64_t)hf;
-  // the triangular form (automatic calls with m or n below 512 run it when
-  // the plan says so) leaves the CRT form's count unused
-  if (moduli) *moduli = use_oz2(m, n, k, hS) ? hN : 0;
+  // automatic calls always run the CRT form; an explicit count runs it when
+  // use_oz2 says so (the triangular form has no moduli)
+  if (moduli) *moduli = (S <= 0 || use_oz2(m, n, k, hS)) ? hN : 0;
   return hipSuccess;
 }
 

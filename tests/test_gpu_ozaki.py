@@ -266,6 +266,35 @@ def test_ozaki_products_stage(gpu):
         assert np.array_equal(dC.cpu().numpy().view(np.uint64), full.view(np.uint64))
 
 
+def test_ozaki_products_stage_of_an_automatic_call(gpu):
+    """mfhip_ozaki_products with slices <= 0 reruns the residue GEMMs of the
+    last automatic call on the workspace (its slice and moduli counts): the
+    digit planes it writes are the call's own, so a second automatic call
+    still gives the first one's bits; an undersized workspace is refused."""
+    import torch
+    rng = np.random.default_rng(17)
+    s = torch.cuda.current_stream().cuda_stream
+    for m, k, n in ((520, 70, 515), (256, 40, 300)):
+        A, B = mats(rng, m, k, n, 20)
+        dA = torch.from_numpy(datagen.flat(A)).cuda()
+        dB = torch.from_numpy(datagen.flat(B)).cuda()
+        dC = torch.empty((m * n, 2), dtype=torch.float64, device="cuda")
+        wb = mf.ozaki_workspace_bytes(m, k, n, 0)
+        work = torch.empty(wb, dtype=torch.uint8, device="cuda")
+        mf.dev_matmul_ozaki(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), m, k, n, 0, work.data_ptr(), wb, s)
+        first = dC.cpu().numpy().copy()
+        n_used = mf.ozaki_moduli_used(work.data_ptr(), m, k, n, 0, s)
+        assert n_used > 0
+        mf.dev_ozaki_products(m, k, n, 0, work.data_ptr(), wb, s)
+        assert mf.ozaki_moduli_used(work.data_ptr(), m, k, n, 0, s) == n_used
+        with pytest.raises(RuntimeError):
+            mf.dev_ozaki_products(m, k, n, 0, work.data_ptr(), wb - 256, s)
+        dC.zero_()
+        mf.dev_matmul_ozaki(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), m, k, n, 0, work.data_ptr(), wb, s)
+        torch.cuda.synchronize()
+        assert np.array_equal(dC.cpu().numpy().view(np.uint64), first.view(np.uint64))
+
+
 @pytest.mark.parametrize("shape", [(512, 1, 512), (515, 3, 600), (600, 33, 513), (1024, 64, 768)])
 def test_ozaki_crt_form_tiny_k_and_uneven_tile_grids(gpu, shape):
     """k far below one 2-block stage (zero-padded K blocks), and tile grids
