@@ -339,7 +339,7 @@ hipError_t gemm_plain(const double2 *A, const double2 *B, double2 *C, int64_t m,
     case 4: return launch_gemm<4, 8, 16, 1>(A, B, C, m, k, n, ri, s);
     case 5: return launch_gemm<8, 4, 16, 1>(A, B, C, m, k, n, ri, s);
     case 6: return launch_gemm<2, 4, 16, 3>(A, B, C, m, k, n, ri, s);
-    default: return launch_gemm<1, 1, 16, 4>(A, B, C, m, k, n, ri, s);
+    default: return launch_gemm<1, 1, 32, 4>(A, B, C, m, k, n, ri, s);  // K blocks of 32: fewer exposed loads at one wave per SIMD
   }
 }
 
