@@ -88,7 +88,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--workload", default="all",
                    choices=["all", "gemm", "cgemm", "blas", "trsm", "batched", "elementwise", "reduce", "profile",
-                            "int8ref"])
+                            "int8ref", "products"])
     p.add_argument("--retune", action="store_true", help="batched: rerun the tile autotuner first")
     p.add_argument("--size", type=int, default=4096)
     p.add_argument("--spread", type=float, default=20.0)
@@ -938,14 +938,104 @@ def bench_int8ref(args, torch, mf):
             a = torch.randint(lo, hi, (size, size), dtype=torch.int8, device="cuda", generator=g)
             b = torch.randint(lo, hi, (size, size), dtype=torch.int8, device="cuda", generator=g).t().contiguous().t()
             ms = timed(torch, lambda: torch._int_mm(a, b), args.steps, args.warmup)
-            row[kind] = {"ms": round(ms, 4), "POPs": round(2.0 * size ** 3 / ms / 1e9, 1)}
+            row[kind] = {"ms": round(ms, 4), "TOPs": round(2.0 * size ** 3 / ms / 1e9, 1)}
             del a, b
         out[str(size)] = row
     return {"metric": "INT8 GEMM rate of the vendor library (hipBLASLt via torch._int_mm) by operand data",
-            "value": out["4096"]["random"]["POPs"], "unit": "TOP/s", "n_gpus": 1, "steps": args.steps,
+            "value": out["4096"]["random"]["TOPs"], "unit": "TOP/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "higher_is_better": True, "dtype": "int8", "data": "synthetic",
             "config": {"workload": "int8 x int8 -> int32 GEMM, m = n = k in {4096, 8192}; random, [-4, 3], "
                                    "constant bytes"}, "rates": out, "peak": INT8_PEAK_TOPS}
+
+
+def _power_sampler(samples, stop):
+    """Socket power (W) and GFX clock (MHz) from amd-smi, every ~0.3 s until
+    `stop` is set; nothing is recorded where amd-smi is absent."""
+    import re
+    import subprocess
+    while not stop.is_set():
+        try:
+            out = subprocess.run(["amd-smi", "metric", "-p", "-c"], capture_output=True, text=True,
+                                 timeout=5).stdout
+        except (OSError, subprocess.SubprocessError):
+            return
+        p = re.search(r"SOCKET_POWER:\s*(\d+)", out)
+        c = re.search(r"GFX_0:\s*\n\s*CLK:\s*(\d+)\s*MHz", out)
+        if p and c:
+            samples.append((int(p.group(1)), int(c.group(1))))
+        stop.wait(0.3)
+
+
+def bench_products(args, torch, mf):
+    """The products stage of an automatic Ozaki call (k_oz2_gemm, the residue
+    INT8 GEMMs, timed alone) by operand data, with the part's socket power and
+    GFX clock sampled while it runs: the bench's iid operands (spread 2^+-20:
+    uniformly random residues) against A = B = all ones (every residue plane
+    constant). Same launch, same cycle count; the clock the part holds under
+    each load sets the rate."""
+    import threading
+    from multifloats_amd import datagen
+    m = n = k = args.size
+    rng = np.random.default_rng(2026)
+    stream = torch.cuda.current_stream().cuda_stream
+    wb = mf.ozaki_workspace_bytes(m, k, n, 0)
+    work = torch.empty(wb, dtype=torch.uint8, device="cuda")
+    dC = torch.empty((m * n, 2), dtype=torch.float64, device="cuda")
+    out = {}
+    for kind in ("random", "constant"):
+        if kind == "random":
+            dA = torch.from_numpy(datagen.flat(datagen.matrix(rng, m, k, args.spread))).cuda()
+            dB = torch.from_numpy(datagen.flat(datagen.matrix(rng, k, n, args.spread))).cuda()
+        else:
+            dA = torch.zeros((m * k, 2), dtype=torch.float64, device="cuda")
+            dA[:, 0] = 1.0
+            dB = torch.zeros((k * n, 2), dtype=torch.float64, device="cuda")
+            dB[:, 0] = 1.0
+        mf.dev_matmul_ozaki(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), m, k, n, 0, work.data_ptr(), wb, stream)
+        N = mf.ozaki_moduli_used(work.data_ptr(), m, k, n, 0, stream)
+
+        def prod():
+            mf.dev_ozaki_products(m, k, n, 0, work.data_ptr(), wb, stream)
+        for _ in range(args.warmup):
+            prod()
+        # about 3 s of back-to-back launches in batches of 100 under the
+        # sampler; the rate is the median batch past the first quarter
+        samples, stop = [], threading.Event()
+        th = threading.Thread(target=_power_sampler, args=(samples, stop), daemon=True)
+        th.start()
+        batches = []
+        t_end = time.time() + 3.0
+        while time.time() < t_end:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(100):
+                prod()
+            e1.record()
+            e1.synchronize()
+            batches.append(e0.elapsed_time(e1) / 100)
+        stop.set()
+        th.join()
+        tail = batches[len(batches) // 4:] or batches
+        ms = sorted(tail)[len(tail) // 2]
+        row = {"moduli": N, "ms": round(ms, 4), "TOPs": round(N * 2.0 * m * n * k / ms / 1e9, 1),
+               "launches": 100 * len(batches)}
+        if samples:
+            body = samples[len(samples) // 4:] or samples  # past the ramp-up
+            pw = sorted(s[0] for s in body)[len(body) // 2]
+            clk = sorted(s[1] for s in body)[len(body) // 2]
+            row.update(socket_power_w=pw, gfx_clock_mhz=clk, mcycles_per_launch=round(ms * 1e-3 * clk * 1e6 / 1e6, 3),
+                       samples=len(samples))
+        out[kind] = row
+        del dA, dB
+    del work
+    return {"metric": "Ozaki products stage (k_oz2_gemm) rate by operand data, with socket power and GFX clock",
+            "value": out["random"]["TOPs"], "unit": "TOP/s", "n_gpus": 1, "steps": out["random"]["launches"],
+            "warmup": args.warmup, "higher_is_better": True, "dtype": "int8", "data": "synthetic",
+            "config": {"workload": "residue INT8 GEMMs of an automatic %d^3 call: iid spread 2^+-%g operands "
+                                   "(random residues) and all-ones operands (constant residue planes)"
+                                   % (m, args.spread)},
+            "cases": out, "peak": INT8_PEAK_TOPS}
 
 
 def main():
@@ -960,7 +1050,8 @@ def main():
         sys.exit("bench.py: no GPU visible")
     fn = {"all": bench_all, "gemm": bench_gemm, "cgemm": lambda a, t, m: bench_c2_complex(a, t, m),
           "blas": bench_blas, "trsm": bench_trsm, "batched": bench_batched, "elementwise": bench_elementwise,
-          "reduce": lambda a, t, m: bench_c1(a, t, m), "profile": bench_profile, "int8ref": bench_int8ref}[args.workload]
+          "reduce": lambda a, t, m: bench_c1(a, t, m), "profile": bench_profile, "int8ref": bench_int8ref,
+          "products": bench_products}[args.workload]
     print(json.dumps(fn(args, torch, mf)), flush=True)
 
 
