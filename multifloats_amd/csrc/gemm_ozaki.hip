@@ -674,19 +674,18 @@ struct O2Mma<kOzRes> {
     return __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c, 0, 0, 0);
   }
 };
-// A fragment holds 16 k of one K block: its two 8-byte halves go to two
-// v_mfma_f32_32x32x16_fp8_fp8 (OCP E4M3 operands). Both operands split the
-// same way, so every k meets its partner once.
+// The certificate's MFMA takes both K blocks of a stage at once: a lane's two
+// 16-byte fragments (block 0, then block 1) form the 32 operand bytes of one
+// v_mfma_scale_f32_32x32x64_f8f6f4 (format 0: OCP E4M3; unit E8M0 scales),
+// the full-rate FP8 form. Both operands concatenate the same way, so every k
+// meets its partner once (the sum does not depend on which k the instruction
+// pairs in which step).
+typedef int v8i __attribute__((ext_vector_type(8)));
 template <>
 struct O2Mma<kOzMag> {
   typedef v16f acc_t;
-  static __device__ __forceinline__ acc_t run(v4i a, v4i b, acc_t c) {
-    const long a0 = (long)(((unsigned long)(unsigned)a[1] << 32) | (unsigned)a[0]);
-    const long a1 = (long)(((unsigned long)(unsigned)a[3] << 32) | (unsigned)a[2]);
-    const long b0 = (long)(((unsigned long)(unsigned)b[1] << 32) | (unsigned)b[0]);
-    const long b1 = (long)(((unsigned long)(unsigned)b[3] << 32) | (unsigned)b[2]);
-    c = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(a0, b0, c, 0, 0, 0);
-    return __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(a1, b1, c, 0, 0, 0);
+  static __device__ __forceinline__ acc_t run(v8i a, v8i b, acc_t c) {
+    return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 0, 0, 0, 127, 0, 127);
   }
 };
 
@@ -794,58 +793,115 @@ __global__ __launch_bounds__(kMainThreads) __attribute__((amdgpu_waves_per_eu(1,
   // during block 0 of stage s, overwrite the buffer of stage s - 1, whose
   // last reads completed before the barrier in the middle of stage s - 1.
   static_assert(kO2KS == 2 && 16 % kO2NL == 0, "two K blocks per stage; copies spread over one block");
-  v4i fa[2][4], fb[2][4];
-  wait_stage(0);
-  lds_barrier();
-#pragma unroll
-  for (int a = 0; a < 4; ++a) fa[0][a] = lds_frag(lds + (2 * wi + (a >> 1)) * kChunk, 32 * (a & 1) + r, half);
-#pragma unroll
-  for (int b = 0; b < 4; ++b) fb[0][b] = lds_frag(lds + (4 + 2 * wj + (b >> 1)) * kChunk, 32 * (b & 1) + r, half);
-  for (int64_t s = 0; s < NS; ++s) {
-    const bool more = s + kO2Bufs - 1 < NS;
-    const signed char *sa1 = lds + (int)(s % kO2Bufs) * kO2Stage + 8 * kChunk;
-#pragma unroll
-    for (int a = 0; a < 4; ++a) fa[1][a] = lds_frag(sa1 + (2 * wi + (a >> 1)) * kChunk, 32 * (a & 1) + r, half);
-#pragma unroll
-    for (int b = 0; b < 4; ++b) fb[1][b] = lds_frag(sa1 + (4 + 2 * wj + (b >> 1)) * kChunk, 32 * (b & 1) + r, half);
-    int slot = 0;
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        acc[a][b] = O2Mma<MODE>::run(fa[0][a], fb[0][b], acc[a][b]);
-        constexpr int every = 16 / kO2NL;
-        if (more && slot % every == 0) issue(s + kO2Bufs - 1, slot / every);
-        ++slot;
-      }
-    // Unconditional, also after the last stage (the reads then fetch a stale
-    // stage buffer inside the allocation and go unused): a branch here would
-    // make the compiler wait for these reads before the MFMAs below.
-    wait_stage(s + 1);
+  if constexpr (MODE == kOzRes) {
+    v4i fa[2][4], fb[2][4];
+    wait_stage(0);
     lds_barrier();
-    // One MFMA of block 1 goes first: the compiler's wait for block 1's
-    // fragments then lands before the new reads instead of after them.
-    acc[0][0] = O2Mma<MODE>::run(fa[1][0], fb[1][0], acc[0][0]);
-    __builtin_amdgcn_sched_barrier(0);
-    const signed char *sn = lds + (int)((s + 1) % kO2Bufs) * kO2Stage;
 #pragma unroll
-    for (int a = 0; a < 4; ++a) fa[0][a] = lds_frag(sn + (2 * wi + (a >> 1)) * kChunk, 32 * (a & 1) + r, half);
+    for (int a = 0; a < 4; ++a) fa[0][a] = lds_frag(lds + (2 * wi + (a >> 1)) * kChunk, 32 * (a & 1) + r, half);
 #pragma unroll
-    for (int b = 0; b < 4; ++b) fb[0][b] = lds_frag(sn + (4 + 2 * wj + (b >> 1)) * kChunk, 32 * (b & 1) + r, half);
-    __builtin_amdgcn_sched_barrier(0);  // the reads above stay ahead of these MFMAs
+    for (int b = 0; b < 4; ++b) fb[0][b] = lds_frag(lds + (4 + 2 * wj + (b >> 1)) * kChunk, 32 * (b & 1) + r, half);
+    for (int64_t s = 0; s < NS; ++s) {
+      const bool more = s + kO2Bufs - 1 < NS;
+      const signed char *sa1 = lds + (int)(s % kO2Bufs) * kO2Stage + 8 * kChunk;
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+      for (int a = 0; a < 4; ++a) fa[1][a] = lds_frag(sa1 + (2 * wi + (a >> 1)) * kChunk, 32 * (a & 1) + r, half);
 #pragma unroll
-      for (int b = 0; b < 4; ++b)
-        if (a + b > 0) acc[a][b] = O2Mma<MODE>::run(fa[1][a], fb[1][b], acc[a][b]);
-    // The next stage's fragments leave the loop through an empty register
-    // asm: otherwise the compiler merges these reads with the ones before the
-    // loop into one read at the top of the loop, behind the MFMAs again.
-    __builtin_amdgcn_sched_barrier(0);
+      for (int b = 0; b < 4; ++b) fb[1][b] = lds_frag(sa1 + (4 + 2 * wj + (b >> 1)) * kChunk, 32 * (b & 1) + r, half);
+      int slot = 0;
 #pragma unroll
-    for (int a = 0; a < 4; ++a) asm("" : "+v"(fa[0][a]));
+      for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) asm("" : "+v"(fb[0][b]));
+        for (int b = 0; b < 4; ++b) {
+          acc[a][b] = O2Mma<MODE>::run(fa[0][a], fb[0][b], acc[a][b]);
+          constexpr int every = 16 / kO2NL;
+          if (more && slot % every == 0) issue(s + kO2Bufs - 1, slot / every);
+          ++slot;
+        }
+      // Unconditional, also after the last stage (the reads then fetch a stale
+      // stage buffer inside the allocation and go unused): a branch here would
+      // make the compiler wait for these reads before the MFMAs below.
+      wait_stage(s + 1);
+      lds_barrier();
+      // One MFMA of block 1 goes first: the compiler's wait for block 1's
+      // fragments then lands before the new reads instead of after them.
+      acc[0][0] = O2Mma<MODE>::run(fa[1][0], fb[1][0], acc[0][0]);
+      __builtin_amdgcn_sched_barrier(0);
+      const signed char *sn = lds + (int)((s + 1) % kO2Bufs) * kO2Stage;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) fa[0][a] = lds_frag(sn + (2 * wi + (a >> 1)) * kChunk, 32 * (a & 1) + r, half);
+#pragma unroll
+      for (int b = 0; b < 4; ++b) fb[0][b] = lds_frag(sn + (4 + 2 * wj + (b >> 1)) * kChunk, 32 * (b & 1) + r, half);
+      __builtin_amdgcn_sched_barrier(0);  // the reads above stay ahead of these MFMAs
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          if (a + b > 0) acc[a][b] = O2Mma<MODE>::run(fa[1][a], fb[1][b], acc[a][b]);
+      // The next stage's fragments leave the loop through an empty register
+      // asm: otherwise the compiler merges these reads with the ones before the
+      // loop into one read at the top of the loop, behind the MFMAs again.
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int a = 0; a < 4; ++a) asm("" : "+v"(fa[0][a]));
+#pragma unroll
+      for (int b = 0; b < 4; ++b) asm("" : "+v"(fb[0][b]));
+    }
+  } else {
+    // Certificate: one MFMA per block pair and stage (both K blocks), rows
+    // of blocks in order. From the middle of stage s (after the wait for
+    // stage s + 1's copies and the barrier) each fragment is reloaded with
+    // stage s + 1's as soon as its last MFMA of stage s has issued: A
+    // fragments 0 and 1 at once, A fragment 2 after its row, the B
+    // fragments during the last row, A fragment 3 at the end. The stage's
+    // copies go out during the first half.
+    static_assert(kO2NL == 8, "one copy per MFMA of the first half");
+    v8i xa[4], xb[4];
+    auto rda = [&](const signed char *st, int a) {
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const v4i f = lds_frag(st + (q * 8 + 2 * wi + (a >> 1)) * kChunk, 32 * (a & 1) + r, half);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) xa[a][4 * q + e] = f[e];
+      }
+    };
+    auto rdb = [&](const signed char *st, int b) {
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const v4i f = lds_frag(st + (q * 8 + 4 + 2 * wj + (b >> 1)) * kChunk, 32 * (b & 1) + r, half);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) xb[b][4 * q + e] = f[e];
+      }
+    };
+    wait_stage(0);
+    lds_barrier();
+#pragma unroll
+    for (int a = 0; a < 4; ++a) rda(lds, a);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) rdb(lds, b);
+    for (int64_t s = 0; s < NS; ++s) {
+      const bool more = s + kO2Bufs - 1 < NS;
+      // unconditional reloads, also after the last stage (a stale buffer
+      // inside the allocation, unused)
+      const signed char *sn = lds + (int)((s + 1) % kO2Bufs) * kO2Stage;
+#pragma unroll
+      for (int x = 0; x < 16; ++x) {
+        const int a = x >> 2, b = x & 3;
+        if (x == 8) {
+          wait_stage(s + 1);
+          lds_barrier();
+          rda(sn, 0);
+          rda(sn, 1);
+          __builtin_amdgcn_sched_barrier(0);
+        }
+        acc[a][b] = O2Mma<MODE>::run(xa[a], xb[b], acc[a][b]);
+        if (x < 8 && more) issue(s + kO2Bufs - 1, x);
+        if (x == 11) rda(sn, 2);
+        if (a == 3) rdb(sn, b);
+        if (x == 15) rda(sn, 3);
+        if (x >= 8) __builtin_amdgcn_sched_barrier(0);
+      }
+    }
   }
   // Accumulator e of block (a, b) is row 8(e/4) + 4 half + e%4 and column r of
   // that 32 x 32 block.
