@@ -24,8 +24,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WPE)))
     int64_t m, int64_t k, int64_t n, int64_t ri, int tiles_m) {
   __shared__ double lds[2 * BK * (16 * TM + 16 * TN)];
   const int tile = blockIdx.x;
-  mfhip_tile::plain_tile<TM, TN, BK, R>(A, B, C, m, k, n, ri, (int64_t)(tile % tiles_m) * (16 * TM),
-                                         (int64_t)(tile / tiles_m) * (16 * TN), lds);
+  // 64 x 64 tiles at two waves per SIMD: operands of the next step loaded
+  // during the current one (the prefetch order of the tile)
+  constexpr bool PF = TM == 4 && TN == 4 && WPE == 2;
+  mfhip_tile::plain_tile<TM, TN, BK, R, PF>(A, B, C, m, k, n, ri, (int64_t)(tile % tiles_m) * (16 * TM),
+                                             (int64_t)(tile / tiles_m) * (16 * TN), lds);
 }
 
 // BLAS-style C = alpha op(A) op(B) + beta C with leading dimensions

@@ -23,6 +23,15 @@
 // static steps (kk+1) % R == 0 and the block is unrolled without branches
 // (the per-step branch otherwise costs a register copy per accumulator at
 // each join). R == 0 (or ri != R): ri is checked per step.
+//
+// PF (4 x 4 chains per lane, static renormalisation path): a step's 16 MACs
+// run as four 2 x 2 blocks (rows 0-1 x cols 0-1, rows 0-1 x cols 2-3, rows
+// 2-3 x cols 0-1, rows 2-3 x cols 2-3), and each operand pair is reloaded
+// with the next step's values as soon as its last MAC of the step has
+// issued: A rows 0-1 after the second block, B cols 0-1 after the third,
+// the rest after the fourth. The next step's first blocks then find their
+// operands loaded 4 or 8 MACs earlier instead of waiting for the LDS at the
+// start of every step. Each chain still takes its MACs in k order.
 #pragma once
 #include "dd_core.h"
 
@@ -37,7 +46,7 @@ constexpr int kThreads = 256;
 // alpha*T + beta*C (DD operators; C is not read when beta == 0).
 // Threads walk the contiguous dimension of each operand, so every layout
 // loads coalesced.
-template <int TM, int TN, int BK, int R, bool TA = false, bool TB = false, bool AB = false>
+template <int TM, int TN, int BK, int R, bool TA = false, bool TB = false, bool AB = false, bool PF = false>
 __device__ __forceinline__ void gemm_tile(const double2 *__restrict__ A, int64_t lda,
                                           const double2 *__restrict__ B, int64_t ldb,
                                           double2 *__restrict__ C, int64_t ldc, int64_t m, int64_t k,
@@ -145,11 +154,63 @@ __device__ __forceinline__ void gemm_tile(const double2 *__restrict__ A, int64_t
     if (R > 0 && rint == R && k0 + BK < k) {
       // static renormalisation points; `until` stays in phase since R | BK
       constexpr int RR = R > 0 ? R : 1;
-#pragma unroll 1
-      for (int c = 0; c < BK; c += RR) {
+      if constexpr (PF) {
+        static_assert(TM == 4 && TN == 4 && (BK & (BK - 1)) == 0, "prefetch order: 4 x 4 chains, BK a power of 2");
+        double ah[4], al[4], bh[4], bl[4];
+        auto ld_a = [&](int kk, int i0) {
 #pragma unroll
-        for (int kk = 0; kk < RR; ++kk) macs(c + kk);
-        renorm_all();
+          for (int i = i0; i < i0 + 2; ++i) {
+            ah[i] = As_h[kk][tx * TM + i];
+            al[i] = As_l[kk][tx * TM + i];
+          }
+        };
+        auto ld_b = [&](int kk, int j0) {
+#pragma unroll
+          for (int j = j0; j < j0 + 2; ++j) {
+            bh[j] = Bs_h[kk][ty * TN + j];
+            bl[j] = Bs_l[kk][ty * TN + j];
+          }
+        };
+        auto blk = [&](int i0, int j0) {
+#pragma unroll
+          for (int i = i0; i < i0 + 2; ++i)
+#pragma unroll
+            for (int j = j0; j < j0 + 2; ++j) mac(ah[i], al[i], bh[j], bl[j], sh[i][j], sl[i][j]);
+        };
+        ld_a(0, 0);
+        ld_a(0, 2);
+        ld_b(0, 0);
+        ld_b(0, 2);
+#pragma unroll 1
+        for (int c = 0; c < BK; c += RR) {
+#pragma unroll
+          for (int kk = 0; kk < RR; ++kk) {
+            // the block's last step reloads step 0 of this block (in LDS,
+            // unused): no branch
+            const int nk = (c + kk + 1) & (BK - 1);
+            // scheduling barriers keep each reload ahead of the next block
+            // (left alone, the scheduler sinks the reads to their uses)
+            blk(0, 0);
+            blk(0, 2);
+            ld_a(nk, 0);
+            __builtin_amdgcn_sched_barrier(0);
+            blk(2, 0);
+            ld_b(nk, 0);
+            __builtin_amdgcn_sched_barrier(0);
+            blk(2, 2);
+            ld_a(nk, 2);
+            ld_b(nk, 2);
+            __builtin_amdgcn_sched_barrier(0);
+          }
+          renorm_all();
+        }
+      } else {
+#pragma unroll 1
+        for (int c = 0; c < BK; c += RR) {
+#pragma unroll
+          for (int kk = 0; kk < RR; ++kk) macs(c + kk);
+          renorm_all();
+        }
       }
       done += BK;
     } else if (kmax == BK) {
@@ -185,11 +246,11 @@ __device__ __forceinline__ void gemm_tile(const double2 *__restrict__ A, int64_t
 }
 
 // The standalone matmuldd_mm tile: contiguous column-major operands.
-template <int TM, int TN, int BK, int R>
+template <int TM, int TN, int BK, int R, bool PF = false>
 __device__ __forceinline__ void plain_tile(const double2 *__restrict__ A, const double2 *__restrict__ B,
                                            double2 *__restrict__ C, int64_t m, int64_t k, int64_t n,
                                            int64_t ri, int64_t m0, int64_t n0, double *lds) {
-  gemm_tile<TM, TN, BK, R>(A, m, B, k, C, m, m, k, n, ri, m0, n0, lds);
+  gemm_tile<TM, TN, BK, R, false, false, false, PF>(A, m, B, k, C, m, m, k, n, ri, m0, n0, lds);
 }
 
 }  // namespace mfhip_tile
