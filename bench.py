@@ -186,10 +186,12 @@ def gemm_roofline_plain(m, n, k, ms, size_key=None, variant=-1):
                     % (FP64_FLOPS_PER_DDMAC, FP64_INSTR_PER_DDMAC)}
 
 
-def ozaki_run(torch, mf, dA, dB, dC, m, k, n, steps, warmup, stream, slices=0):
+def ozaki_run(torch, mf, dA, dB, dC, m, k, n, steps, warmup, stream, slices=0, power=False):
     """Time the Ozaki call (automatic form unless slices > 0), then its parts:
     the same call with the slice count it chose (explicit: no certificate)
-    and the residue GEMMs alone (k_oz2_gemm, the dominant kernel)."""
+    and the residue GEMMs alone (k_oz2_gemm, the dominant kernel). With power,
+    the residue GEMMs also run back to back for about 2.5 s with the board's
+    socket power and GFX clock sampled (amd-smi), beside the roofline."""
     wb = mf.ozaki_workspace_bytes(m, k, n, 0)
     work = torch.empty(wb, dtype=torch.uint8, device="cuda")
 
@@ -221,6 +223,16 @@ def ozaki_run(torch, mf, dA, dB, dC, m, k, n, steps, warmup, stream, slices=0):
                                    "timed call's own products launch, timed alone on its stream; the rest of the call: "
                                    "exponent scan, residues with the certificate operands, certificate GEMM (FP8 "
                                    "MFMA), plans, reconstruction, fix-up" % N}
+        if power:
+            ms_p, pw, clk, ns = power_during(
+                torch, lambda: mf.dev_ozaki_products(m, k, n, slices, work.data_ptr(), wb, stream), 2.5, 50)
+            if pw is not None:
+                res["roofline"]["power"] = {
+                    "socket_power_w": pw, "gfx_clock_mhz": clk, "ms_per_launch": round(ms_p, 4), "samples": ns,
+                    "note": "the products launch back to back for 2.5 s, amd-smi sampled alongside (medians past "
+                            "the first quarter); on these random residues the board runs at its power limit and "
+                            "holds the clock below 2.4 GHz (bench.py --workload products: constant residues run "
+                            "the same launch at the full clock)"}
     del work
     return res, C
 
@@ -262,7 +274,8 @@ def bench_gemm(args, torch, mf):
         del C
 
     if args.form in ("both", "ozaki"):
-        r, Cf = ozaki_run(torch, mf, dA, dB, dC, m, k, n, args.steps, args.warmup, stream, args.slices)
+        r, Cf = ozaki_run(torch, mf, dA, dB, dC, m, k, n, args.steps, args.warmup, stream, args.slices,
+                          power=True)
         C = datagen.unflat(Cf, m, n)
         r["err_max"], r["err_mean"] = sampled_error(A, B, C, rows, cols)
         r["value"] = flops / (r["ms"] * 1e-3) / 1e9
@@ -966,6 +979,37 @@ def _power_sampler(samples, stop):
         stop.wait(0.3)
 
 
+def power_during(torch, fn, seconds=3.0, batch=100):
+    """Run fn back to back for about `seconds` in batches under the amd-smi
+    sampler; returns (median ms per call past the first quarter of the
+    batches, median socket power W, median GFX clock MHz, samples), the last
+    three None where amd-smi is absent."""
+    import threading
+    samples, stop = [], threading.Event()
+    th = threading.Thread(target=_power_sampler, args=(samples, stop), daemon=True)
+    th.start()
+    batches = []
+    t_end = time.time() + seconds
+    while time.time() < t_end:
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(batch):
+            fn()
+        e1.record()
+        e1.synchronize()
+        batches.append(e0.elapsed_time(e1) / batch)
+    stop.set()
+    th.join()
+    tail = batches[len(batches) // 4:] or batches
+    ms = sorted(tail)[len(tail) // 2]
+    if not samples:
+        return ms, None, None, 0
+    body = samples[len(samples) // 4:] or samples  # past the ramp-up
+    return (ms, sorted(s[0] for s in body)[len(body) // 2], sorted(s[1] for s in body)[len(body) // 2],
+            len(samples))
+
+
 def bench_products(args, torch, mf):
     """The products stage of an automatic Ozaki call (k_oz2_gemm, the residue
     INT8 GEMMs, timed alone) by operand data, with the part's socket power and
@@ -973,7 +1017,6 @@ def bench_products(args, torch, mf):
     uniformly random residues) against A = B = all ones (every residue plane
     constant). Same launch, same cycle count; the clock the part holds under
     each load sets the rate."""
-    import threading
     from multifloats_amd import datagen
     m = n = k = args.size
     rng = np.random.default_rng(2026)
@@ -1000,32 +1043,13 @@ def bench_products(args, torch, mf):
             prod()
         # about 3 s of back-to-back launches in batches of 100 under the
         # sampler; the rate is the median batch past the first quarter
-        samples, stop = [], threading.Event()
-        th = threading.Thread(target=_power_sampler, args=(samples, stop), daemon=True)
-        th.start()
-        batches = []
-        t_end = time.time() + 3.0
-        while time.time() < t_end:
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record()
-            for _ in range(100):
-                prod()
-            e1.record()
-            e1.synchronize()
-            batches.append(e0.elapsed_time(e1) / 100)
-        stop.set()
-        th.join()
-        tail = batches[len(batches) // 4:] or batches
-        ms = sorted(tail)[len(tail) // 2]
+        t0 = time.time()
+        ms, pw, clk, ns = power_during(torch, prod, 3.0, 100)
         row = {"moduli": N, "ms": round(ms, 4), "TOPs": round(N * 2.0 * m * n * k / ms / 1e9, 1),
-               "launches": 100 * len(batches)}
-        if samples:
-            body = samples[len(samples) // 4:] or samples  # past the ramp-up
-            pw = sorted(s[0] for s in body)[len(body) // 2]
-            clk = sorted(s[1] for s in body)[len(body) // 2]
+               "launches": int(round((time.time() - t0) * 1e3 / ms))}
+        if pw is not None:
             row.update(socket_power_w=pw, gfx_clock_mhz=clk, mcycles_per_launch=round(ms * 1e-3 * clk * 1e6 / 1e6, 3),
-                       samples=len(samples))
+                       samples=ns)
         out[kind] = row
         del dA, dB
     del work
