@@ -9,13 +9,14 @@
 // additions.
 //
 // Plan: the host cuts every GEMM into output tiles of its tile variant (one
-// of six shapes, chosen by the caller's tuned table or by least padded work)
-// and every dot into chunks of
-// kDotChunk (2^20) elements, then groups the items by kind (tile shape, or
-// dot) and sorts each group by estimated cost, largest first (longest
-// processing time). One persistent launch per kind takes its group's items
-// round robin; every workgroup's loop is bounded by the item count, so each
-// grid drains. A dot of one chunk is a single item that
+// of eight shapes, chosen by the caller's tuned table or by least padded
+// work) and every dot into chunks of kDotChunk (2^20) elements, then groups
+// the items by kind (tile shape, or dot). Within a kind, the items are dealt
+// to the launch's workgroups by longest processing time first: largest
+// estimated cost first, each to the workgroup with the least cost so far.
+// One persistent launch per kind; workgroup g takes its own list, in the
+// order dealt (a static schedule: no queue, every loop bounded by its list),
+// so each grid drains. A dot of one chunk is a single item that
 // writes its result directly. A longer dot's chunks leave DD partials; the
 // chunk that completes the dot last (per-dot atomic counter) folds the
 // partials in index order, so results do not depend on scheduling. That
@@ -28,6 +29,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <queue>
+#include <utility>
 #include <vector>
 
 #include "../../include/multifloats_hip.h"
@@ -182,28 +185,30 @@ __device__ void dot_chunk(const DotD &d, int c, double2 *partials, int *dot_done
   }
 }
 
-// Static longest-processing-time round robin over one kind's slice of the
-// cost-sorted items: workgroup g takes items g, g + G, g + 2G, ... The item
-// index derives from blockIdx only, so every branch and loop around the
-// barriers inside the item bodies is uniform. Each kind has a kernel of its
+// Static longest-processing-time schedule: workgroup g takes items
+// [off[g], off[g + 1]) of its kind's slice. The item index derives from
+// blockIdx only, so every branch and loop around the barriers inside the
+// item bodies is uniform. Each kind has a kernel of its
 // own: one kernel holding every tile body needs more registers than any body
 // alone and spilled (the 64 x 64 body alone already does at 256 VGPRs); split,
 // each body gets its own register budget and occupancy.
 template <int TM, int TN, int WPE>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WPE))) void k_grouped_gemm(
-    const Item *__restrict__ items, int nitems, const GemmD *__restrict__ gemms) {
+    const Item *__restrict__ items, const int *__restrict__ off, const GemmD *__restrict__ gemms) {
   __shared__ __attribute__((aligned(16))) double lds[2 * kBK * (16 * TM + 16 * TN)];
-  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+  const int end = off[blockIdx.x + 1];
+  for (int it = off[blockIdx.x]; it < end; ++it) {
     const Item w = items[it];
     gemm_tile<TM, TN>(gemms[w.prob], w.a, w.b, lds);
   }
 }
 
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_grouped_dot(
-    const Item *__restrict__ items, int nitems, const DotD *__restrict__ dots, double2 *__restrict__ partials,
-    int *__restrict__ dot_done) {
+    const Item *__restrict__ items, const int *__restrict__ off, const DotD *__restrict__ dots,
+    double2 *__restrict__ partials, int *__restrict__ dot_done) {
   __shared__ __attribute__((aligned(16))) double lds[2 * kThreads];
-  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+  const int end = off[blockIdx.x + 1];
+  for (int it = off[blockIdx.x]; it < end; ++it) {
     const Item w = items[it];
     dot_chunk(dots[w.prob], w.a, partials, dot_done, lds);
   }
@@ -228,6 +233,9 @@ struct Plan {
   int *counters = nullptr;  // per-dot completion counts
   int nitems = 0, ndot = 0, cus = 1;
   int first[ITEM_DOT + 2] = {};  // items of kind q: [first[q], first[q + 1])
+  int *offs = nullptr;           // per kind: its workgroups' item ranges, relative to first[q]
+  int ofirst[ITEM_DOT + 1] = {};  // kind q's ranges: offs[ofirst[q] .. ofirst[q] + groups[q]]
+  int groups[ITEM_DOT + 1] = {};  // kind q's launch size
   int64_t ncounters = 0;
   hipStream_t gstream = nullptr;
   hipGraph_t graph = nullptr;
@@ -235,35 +243,36 @@ struct Plan {
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
 };
 
+// Workgroups per CU of each kind's launch (its waves per SIMD: 4-wave
+// workgroups), in kind order; the dots run two.
+constexpr int kKindWPE[ITEM_DOT + 1] = {4, 1, 2, 2, 3, 3, 4, 2, 2};
+
 // One persistent launch per kind with items: grid = the kind's resident
 // workgroups over all CUs (its occupancy), or fewer when it has fewer items.
-template <int TM, int TN, int WPE>
-void launch_kind(const Plan *p, int q, hipStream_t s) {
-  const int n = p->first[q + 1] - p->first[q];
-  if (n <= 0) return;
-  const int resident = p->cus * WPE;  // 4-wave workgroups: WPE per CU
-  const int grid = n < resident ? n : resident;
-  hipLaunchKernelGGL((k_grouped_gemm<TM, TN, WPE>), dim3((unsigned)grid), dim3(kThreads), 0, s,
-                     p->items + p->first[q], n, p->gemms);
+template <int TM, int TN, int WPE, int Q>
+void launch_kind(const Plan *p, hipStream_t s) {
+  static_assert(kKindWPE[Q] == WPE, "the plan deals kind Q's items to kKindWPE[Q] workgroups per CU");
+  constexpr int q = Q;
+  if (p->groups[q] <= 0) return;
+  hipLaunchKernelGGL((k_grouped_gemm<TM, TN, WPE>), dim3((unsigned)p->groups[q]), dim3(kThreads), 0, s,
+                     p->items + p->first[q], p->offs + p->ofirst[q], p->gemms);
 }
 
 hipError_t run_plan(Plan *p, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_ints, dim3((unsigned)((p->ncounters + 255) / 256)), dim3(256), 0, s,
                      p->counters, p->ncounters);
-  launch_kind<2, 2, 4>(p, 0, s);
-  launch_kind<4, 4, 1>(p, 1, s);
-  launch_kind<8, 1, 2>(p, 2, s);
-  launch_kind<1, 8, 2>(p, 3, s);
-  launch_kind<4, 2, 3>(p, 4, s);
-  launch_kind<2, 4, 3>(p, 5, s);
-  launch_kind<1, 1, 4>(p, 6, s);
-  launch_kind<3, 3, 2>(p, 7, s);
-  const int nd = p->first[ITEM_DOT + 1] - p->first[ITEM_DOT];
-  if (nd > 0) {
-    const int grid = nd < 2 * p->cus ? nd : 2 * p->cus;
-    hipLaunchKernelGGL(k_grouped_dot, dim3((unsigned)grid), dim3(kThreads), 0, s, p->items + p->first[ITEM_DOT], nd,
-                       p->dots, p->partials, p->counters);
-  }
+  launch_kind<2, 2, 4, 0>(p, s);
+  launch_kind<4, 4, 1, 1>(p, s);
+  launch_kind<8, 1, 2, 2>(p, s);
+  launch_kind<1, 8, 2, 3>(p, s);
+  launch_kind<4, 2, 3, 4>(p, s);
+  launch_kind<2, 4, 3, 5>(p, s);
+  launch_kind<1, 1, 4, 6>(p, s);
+  launch_kind<3, 3, 2, 7>(p, s);
+  if (p->groups[ITEM_DOT] > 0)
+    hipLaunchKernelGGL(k_grouped_dot, dim3((unsigned)p->groups[ITEM_DOT]), dim3(kThreads), 0, s,
+                       p->items + p->first[ITEM_DOT], p->offs + p->ofirst[ITEM_DOT], p->dots, p->partials,
+                       p->counters);
   return hipGetLastError();
 }
 
@@ -275,6 +284,7 @@ void free_plan(Plan *p) {
   if (p->ev_out) (void)hipEventDestroy(p->ev_out);
   if (p->gstream) (void)hipStreamDestroy(p->gstream);
   (void)hipFree(p->items);
+  (void)hipFree(p->offs);
   (void)hipFree(p->gemms);
   (void)hipFree(p->dots);
   (void)hipFree(p->partials);
@@ -346,7 +356,11 @@ void *mfhip_grouped_create(const mfhip_gemm_problem *gemms, int ngemm, const int
     return cost[x] > cost[y];
   });
   std::vector<Item> sorted(items.size());
-  for (size_t i = 0; i < order.size(); ++i) sorted[i] = items[order[i]];
+  std::vector<double> scost(items.size());
+  for (size_t i = 0; i < order.size(); ++i) {
+    sorted[i] = items[order[i]];
+    scost[i] = cost[order[i]];
+  }
 
   Plan *p = new Plan();
   for (const Item &it : sorted) ++p->first[it.kind + 1];
@@ -355,6 +369,36 @@ void *mfhip_grouped_create(const mfhip_gemm_problem *gemms, int ngemm, const int
   int cus = 0;
   ok = ok && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device) == hipSuccess;
   p->cus = cus > 0 ? cus : 1;
+  // Deal each kind's items (largest first) to the workgroup with the least
+  // estimated cost so far; ties go to the lower index (the plan is
+  // deterministic). Workgroup g's list is offs[g] .. offs[g + 1] - 1 of the
+  // kind's slice, in the order dealt.
+  std::vector<Item> dealt(sorted.size());
+  std::vector<int> offs;
+  for (int q = 0; q <= ITEM_DOT; ++q) {
+    const int b = p->first[q], n = p->first[q + 1] - b;
+    const int resident = p->cus * kKindWPE[q];
+    const int G = n < resident ? n : resident;
+    p->groups[q] = G;
+    p->ofirst[q] = (int)offs.size();
+    if (G <= 0) continue;
+    std::vector<std::vector<int>> lists((size_t)G);
+    typedef std::pair<double, int> Load;
+    std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
+    for (int g = 0; g < G; ++g) heap.push(Load(0.0, g));
+    for (int i = 0; i < n; ++i) {
+      const Load l = heap.top();
+      heap.pop();
+      lists[(size_t)l.second].push_back(b + i);
+      heap.push(Load(l.first + scost[(size_t)(b + i)], l.second));
+    }
+    int pos = b;
+    for (int g = 0; g < G; ++g) {
+      offs.push_back(pos - b);
+      for (int i : lists[(size_t)g]) dealt[(size_t)pos++] = sorted[(size_t)i];
+    }
+    offs.push_back(pos - b);
+  }
   p->nitems = (int)sorted.size();
   p->ndot = ndot;
   p->ncounters = ndot > 0 ? (int64_t)ndot : 1;
@@ -363,7 +407,8 @@ void *mfhip_grouped_create(const mfhip_gemm_problem *gemms, int ngemm, const int
     ok = hipMalloc(dst, bytes ? bytes : 16) == hipSuccess;
     if (ok && bytes) ok = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
   };
-  upload((void **)&p->items, sorted.data(), sizeof(Item) * sorted.size());
+  upload((void **)&p->items, dealt.data(), sizeof(Item) * dealt.size());
+  upload((void **)&p->offs, offs.data(), sizeof(int) * offs.size());
   upload((void **)&p->gemms, hg.data(), sizeof(GemmD) * hg.size());
   upload((void **)&p->dots, hd.data(), sizeof(DotD) * hd.size());
   ok = ok && hipMalloc((void **)&p->partials, sizeof(double2) * (size_t)(nparts > 0 ? nparts : 1)) == hipSuccess;
