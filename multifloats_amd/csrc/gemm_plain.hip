@@ -22,13 +22,18 @@ template <int TM, int TN, int BK, int WPE, int R>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WPE))) void k_gemm_plain(
     const double2 *__restrict__ A, const double2 *__restrict__ B, double2 *__restrict__ C,
     int64_t m, int64_t k, int64_t n, int64_t ri, int tiles_m) {
-  __shared__ double lds[2 * BK * (16 * TM + 16 * TN)];
   const int tile = blockIdx.x;
-  // 64 x 64 tiles at two waves per SIMD: operands of the next step loaded
-  // during the current one (the prefetch order of the tile)
-  constexpr bool PF = TM == 4 && TN == 4 && WPE == 2;
-  mfhip_tile::plain_tile<TM, TN, BK, R, PF>(A, B, C, m, k, n, ri, (int64_t)(tile % tiles_m) * (16 * TM),
-                                             (int64_t)(tile / tiles_m) * (16 * TN), lds);
+  const int64_t m0 = (int64_t)(tile % tiles_m) * (16 * TM), n0 = (int64_t)(tile / tiles_m) * (16 * TN);
+  if constexpr (TM == 4 && TN == 4 && WPE == 2) {
+    // 64 x 64 tiles at two waves per SIMD: operands copied straight into
+    // two LDS stages (dynamic LDS, kDirectLds bytes)
+    static_assert(BK == mfhip_tile::kDirectBK, "K block of the direct tile");
+    extern __shared__ __attribute__((aligned(16))) char dyn_lds[];
+    mfhip_tile::plain_tile_direct<R>(A, B, C, m, k, n, ri, m0, n0, dyn_lds);
+  } else {
+    __shared__ double lds[2 * BK * (16 * TM + 16 * TN)];
+    mfhip_tile::plain_tile<TM, TN, BK, R>(A, B, C, m, k, n, ri, m0, n0, lds);
+  }
 }
 
 // BLAS-style C = alpha op(A) op(B) + beta C with leading dimensions
@@ -297,11 +302,21 @@ hipError_t launch_gemm(const double2 *A, const double2 *B, double2 *C, int64_t m
   const int64_t tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
   if (tm * tn > 0x7fffffff) return hipErrorInvalidValue;
   const bool r8 = ri == 8 && k > 8;
+  constexpr bool direct = TM == 4 && TN == 4 && WPE == 2;
+  const int dyn = direct ? mfhip_tile::kDirectLds : 0;
+  if (direct) {
+    // above the 64 KiB default: allowed once per process and kernel
+    static const bool ok8 = hipFuncSetAttribute((const void *)k_gemm_plain<TM, TN, BK, WPE, 8>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, dyn) == hipSuccess;
+    static const bool ok0 = hipFuncSetAttribute((const void *)k_gemm_plain<TM, TN, BK, WPE, 0>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, dyn) == hipSuccess;
+    if (!ok8 || !ok0) return hipErrorInvalidConfiguration;
+  }
   if (r8)
-    hipLaunchKernelGGL((k_gemm_plain<TM, TN, BK, WPE, 8>), dim3((unsigned)(tm * tn)), dim3(kThreads), 0,
+    hipLaunchKernelGGL((k_gemm_plain<TM, TN, BK, WPE, 8>), dim3((unsigned)(tm * tn)), dim3(kThreads), dyn,
                        s, A, B, C, m, k, n, ri, (int)tm);
   else
-    hipLaunchKernelGGL((k_gemm_plain<TM, TN, BK, WPE, 0>), dim3((unsigned)(tm * tn)), dim3(kThreads), 0,
+    hipLaunchKernelGGL((k_gemm_plain<TM, TN, BK, WPE, 0>), dim3((unsigned)(tm * tn)), dim3(kThreads), dyn,
                        s, A, B, C, m, k, n, ri, (int)tm);
   return hipGetLastError();
 }

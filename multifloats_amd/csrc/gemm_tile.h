@@ -23,15 +23,7 @@
 // static steps (kk+1) % R == 0 and the block is unrolled without branches
 // (the per-step branch otherwise costs a register copy per accumulator at
 // each join). R == 0 (or ri != R): ri is checked per step.
-//
-// PF (4 x 4 chains per lane, static renormalisation path): a step's 16 MACs
-// run as four 2 x 2 blocks (rows 0-1 x cols 0-1, rows 0-1 x cols 2-3, rows
-// 2-3 x cols 0-1, rows 2-3 x cols 2-3), and each operand pair is reloaded
-// with the next step's values as soon as its last MAC of the step has
-// issued: A rows 0-1 after the second block, B cols 0-1 after the third,
-// the rest after the fourth. The next step's first blocks then find their
-// operands loaded 4 or 8 MACs earlier instead of waiting for the LDS at the
-// start of every step. Each chain still takes its MACs in k order.
+
 #pragma once
 #include "dd_core.h"
 
@@ -46,7 +38,7 @@ constexpr int kThreads = 256;
 // alpha*T + beta*C (DD operators; C is not read when beta == 0).
 // Threads walk the contiguous dimension of each operand, so every layout
 // loads coalesced.
-template <int TM, int TN, int BK, int R, bool TA = false, bool TB = false, bool AB = false, bool PF = false>
+template <int TM, int TN, int BK, int R, bool TA = false, bool TB = false, bool AB = false>
 __device__ __forceinline__ void gemm_tile(const double2 *__restrict__ A, int64_t lda,
                                           const double2 *__restrict__ B, int64_t ldb,
                                           double2 *__restrict__ C, int64_t ldc, int64_t m, int64_t k,
@@ -154,63 +146,11 @@ __device__ __forceinline__ void gemm_tile(const double2 *__restrict__ A, int64_t
     if (R > 0 && rint == R && k0 + BK < k) {
       // static renormalisation points; `until` stays in phase since R | BK
       constexpr int RR = R > 0 ? R : 1;
-      if constexpr (PF) {
-        static_assert(TM == 4 && TN == 4 && (BK & (BK - 1)) == 0, "prefetch order: 4 x 4 chains, BK a power of 2");
-        double ah[4], al[4], bh[4], bl[4];
-        auto ld_a = [&](int kk, int i0) {
-#pragma unroll
-          for (int i = i0; i < i0 + 2; ++i) {
-            ah[i] = As_h[kk][tx * TM + i];
-            al[i] = As_l[kk][tx * TM + i];
-          }
-        };
-        auto ld_b = [&](int kk, int j0) {
-#pragma unroll
-          for (int j = j0; j < j0 + 2; ++j) {
-            bh[j] = Bs_h[kk][ty * TN + j];
-            bl[j] = Bs_l[kk][ty * TN + j];
-          }
-        };
-        auto blk = [&](int i0, int j0) {
-#pragma unroll
-          for (int i = i0; i < i0 + 2; ++i)
-#pragma unroll
-            for (int j = j0; j < j0 + 2; ++j) mac(ah[i], al[i], bh[j], bl[j], sh[i][j], sl[i][j]);
-        };
-        ld_a(0, 0);
-        ld_a(0, 2);
-        ld_b(0, 0);
-        ld_b(0, 2);
 #pragma unroll 1
-        for (int c = 0; c < BK; c += RR) {
+      for (int c = 0; c < BK; c += RR) {
 #pragma unroll
-          for (int kk = 0; kk < RR; ++kk) {
-            // the block's last step reloads step 0 of this block (in LDS,
-            // unused): no branch
-            const int nk = (c + kk + 1) & (BK - 1);
-            // scheduling barriers keep each reload ahead of the next block
-            // (left alone, the scheduler sinks the reads to their uses)
-            blk(0, 0);
-            blk(0, 2);
-            ld_a(nk, 0);
-            __builtin_amdgcn_sched_barrier(0);
-            blk(2, 0);
-            ld_b(nk, 0);
-            __builtin_amdgcn_sched_barrier(0);
-            blk(2, 2);
-            ld_a(nk, 2);
-            ld_b(nk, 2);
-            __builtin_amdgcn_sched_barrier(0);
-          }
-          renorm_all();
-        }
-      } else {
-#pragma unroll 1
-        for (int c = 0; c < BK; c += RR) {
-#pragma unroll
-          for (int kk = 0; kk < RR; ++kk) macs(c + kk);
-          renorm_all();
-        }
+        for (int kk = 0; kk < RR; ++kk) macs(c + kk);
+        renorm_all();
       }
       done += BK;
     } else if (kmax == BK) {
@@ -245,12 +185,193 @@ __device__ __forceinline__ void gemm_tile(const double2 *__restrict__ A, int64_t
   }
 }
 
+// 64 x 64 tile of the standalone matmuldd_mm (contiguous column-major
+// operands, 4 x 4 chains per lane) with operands copied straight from global
+// memory into LDS: no staging registers, no de-interleave pass, one barrier
+// per K block. Two LDS stages of BK k: the copies of block b + 1 run during
+// the MACs of block b.
+//   A stage: BK columns of 64 rows, {hi, lo} pairs, column kk at kk * 1 KiB
+//     (one 1 KiB copy per column: 64 lanes x 16 bytes);
+//   B stage: 64 columns of BK k, 1 KiB copy q holding columns 4q .. 4q + 3
+//     (BK = 16) at q * 1 KiB + 16 (q % 4) + 64 (q / 4): the four copies read
+//     by one wave's four lane groups sit on different LDS banks.
+// Lane (tx, ty) owns rows tx + 16 i and columns 4 ty + j (i, j < 4): the A
+// reads of a step are 16 consecutive pairs (all banks once), broadcast over
+// the four lane groups. Copies clamp their rows, columns and k into the
+// operand, so an edge tile never reads outside it: the rows and columns past
+// m and n are computed from repeated data and not stored, and the k past the
+// operand's end are never taken (the last block runs kmax steps).
+// In the static renormalisation path a step's 16 MACs run as four 2 x 2
+// blocks (rows 0-1 x cols 0-1, rows 0-1 x cols 2-3, rows 2-3 x cols 0-1,
+// rows 2-3 x cols 2-3), and each operand pair is reloaded with the next
+// step's values as soon as its last MAC of the step has issued (A rows 0-1
+// after the second block, B cols 0-1 after the third, the rest after the
+// fourth; scheduling barriers keep each reload ahead of the next block, which
+// the scheduler would otherwise sink to its use). Each output's MACs,
+// renormalisation points and final TwoSum are those of gemm_tile, in the same
+// order, so the bits are the same.
+constexpr int kDirectBK = 16;
+constexpr int kDirectStageA = kDirectBK * 1024;
+constexpr int kDirectStageB = 16 * 1024 + 3 * 16 + 3 * 64 + 64;  // copy 15 ends at 15 KiB + 48 + 192 + 1 KiB
+constexpr int kDirectStage = kDirectStageA + kDirectStageB;
+constexpr int kDirectLds = 2 * kDirectStage;
+constexpr int direct_b_base(int q) { return q * 1024 + 16 * (q & 3) + 64 * (q >> 2); }
+
+template <int R>
+__device__ __forceinline__ void plain_tile_direct(const double2 *__restrict__ A, const double2 *__restrict__ B,
+                                                  double2 *__restrict__ C, int64_t m, int64_t k, int64_t n,
+                                                  int64_t ri, int64_t m0, int64_t n0, char *lds) {
+  constexpr int BK = kDirectBK;
+  static_assert(BK == 16, "B copies of four 16-k columns");
+  static_assert(direct_b_base(15) + 1024 <= kDirectStageB, "B stage size");
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int tx = tid & 15, ty = tid >> 4;
+  const int64_t nb = (k + BK - 1) / BK;
+  // copies of K block kb into stage buffer buf: this wave's A columns
+  // wave, wave + 4, ... and B copies wave, wave + 4, ...
+  const int64_t ra = m0 + lane < m ? m0 + lane : m - 1;   // A row of this lane
+  const int bc = lane >> 4, bk = lane & 15;                 // B copy: column in the copy, k
+  auto issue = [&](int64_t kb, int buf) {
+    char *st = lds + buf * kDirectStage;
+#pragma unroll
+    for (int c = wave; c < BK; c += 4) {
+      const int64_t p = kb * BK + c < k ? kb * BK + c : k - 1;
+      __builtin_amdgcn_global_load_lds((const void *)(A + ra + p * m),
+                                       (__attribute__((address_space(3))) void *)(st + c * 1024), 16, 0, 0);
+    }
+#pragma unroll
+    for (int q = wave; q < 16; q += 4) {
+      const int64_t col = n0 + 4 * q + bc < n ? n0 + 4 * q + bc : n - 1;
+      const int64_t p = kb * BK + bk < k ? kb * BK + bk : k - 1;
+      __builtin_amdgcn_global_load_lds((const void *)(B + p + col * k),
+                                       (__attribute__((address_space(3))) void *)(st + kDirectStageA + direct_b_base(q)),
+                                       16, 0, 0);
+    }
+  };
+
+  double sh[4][4], sl[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) sh[i][j] = sl[i][j] = 0.0;
+  const int64_t rint = (ri > 0 && k > ri) ? ri : 0;
+  int64_t until = rint;
+  int64_t done = 0;
+  double ah[4], al[4], bh[4], bl[4];
+  auto ld_a = [&](const char *st, int kk, int i0) {
+#pragma unroll
+    for (int i = i0; i < i0 + 2; ++i) {
+      const double2 v = *(const double2 *)(st + kk * 1024 + (tx + 16 * i) * 16);
+      ah[i] = v.x;
+      al[i] = v.y;
+    }
+  };
+  auto ld_b = [&](const char *st, int kk, int j0) {
+#pragma unroll
+    for (int j = j0; j < j0 + 2; ++j) {
+      const double2 v = *(const double2 *)(st + kDirectStageA + direct_b_base(ty) + j * BK * 16 + kk * 16);
+      bh[j] = v.x;
+      bl[j] = v.y;
+    }
+  };
+  auto blk = [&](int i0, int j0) {
+#pragma unroll
+    for (int i = i0; i < i0 + 2; ++i)
+#pragma unroll
+      for (int j = j0; j < j0 + 2; ++j) mac(ah[i], al[i], bh[j], bl[j], sh[i][j], sl[i][j]);
+  };
+  auto renorm_all = [&]() {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) renorm(sh[i][j], sl[i][j]);
+  };
+
+  issue(0, 0);
+  for (int64_t kb = 0; kb < nb; ++kb) {
+    // this wave's copies of block kb have landed (the only ones in flight),
+    // every wave's too after the barrier, and every wave is done with the
+    // other buffer (block kb - 1)
+    __syncthreads();
+    if (kb + 1 < nb) issue(kb + 1, (int)((kb + 1) & 1));
+    const char *st = lds + (int)(kb & 1) * kDirectStage;
+    const int64_t k0 = kb * BK;
+    const int kmax = (int)((k - k0) < BK ? (k - k0) : BK);
+    if (R > 0 && rint == R && k0 + BK < k) {
+      constexpr int RR = R > 0 ? R : 1;
+      ld_a(st, 0, 0);
+      ld_a(st, 0, 2);
+      ld_b(st, 0, 0);
+      ld_b(st, 0, 2);
+#pragma unroll 1
+      for (int c = 0; c < BK; c += RR) {
+#pragma unroll
+        for (int kk = 0; kk < RR; ++kk) {
+          // the block's last step reloads step 0 of this block (unused): no branch
+          const int nk = (c + kk + 1) & (BK - 1);
+          static_assert((BK & (BK - 1)) == 0, "BK a power of 2");
+          blk(0, 0);
+          blk(0, 2);
+          ld_a(st, nk, 0);
+          __builtin_amdgcn_sched_barrier(0);
+          blk(2, 0);
+          ld_b(st, nk, 0);
+          __builtin_amdgcn_sched_barrier(0);
+          blk(2, 2);
+          ld_a(st, nk, 2);
+          ld_b(st, nk, 2);
+          __builtin_amdgcn_sched_barrier(0);
+        }
+        renorm_all();
+      }
+      done += BK;
+    } else {
+      auto step = [&](int kk) {
+        ld_a(st, kk, 0);
+        ld_a(st, kk, 2);
+        ld_b(st, kk, 0);
+        ld_b(st, kk, 2);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) mac(ah[i], al[i], bh[j], bl[j], sh[i][j], sl[i][j]);
+        ++done;
+        if (rint && --until == 0) {
+          until = rint;
+          if (done < k) renorm_all();
+        }
+      };
+      if (kmax == BK) {
+#pragma unroll
+        for (int kk = 0; kk < BK; ++kk) step(kk);
+      } else {
+        for (int kk = 0; kk < kmax; ++kk) step(kk);
+      }
+    }
+  }
+
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int64_t gj = n0 + 4 * ty + j;
+    if (gj >= n) continue;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int64_t gi = m0 + tx + 16 * i;
+      if (gi >= m) continue;
+      double h = sh[i][j], l = sl[i][j];
+      renorm(h, l);
+      C[gi + gj * m] = make_double2(h, l);
+    }
+  }
+}
+
 // The standalone matmuldd_mm tile: contiguous column-major operands.
-template <int TM, int TN, int BK, int R, bool PF = false>
+template <int TM, int TN, int BK, int R>
 __device__ __forceinline__ void plain_tile(const double2 *__restrict__ A, const double2 *__restrict__ B,
                                            double2 *__restrict__ C, int64_t m, int64_t k, int64_t n,
                                            int64_t ri, int64_t m0, int64_t n0, double *lds) {
-  gemm_tile<TM, TN, BK, R, false, false, false, PF>(A, m, B, k, C, m, m, k, n, ri, m0, n0, lds);
+  gemm_tile<TM, TN, BK, R>(A, m, B, k, C, m, m, k, n, ri, m0, n0, lds);
 }
 
 }  // namespace mfhip_tile
