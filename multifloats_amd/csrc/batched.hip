@@ -78,16 +78,18 @@ struct DotD {
   int id;  // index of the dot's completion counter
 };
 
-// One output tile of a reference-order GEMM: the tile body of the standalone
-// GEMM (gemm_tile.h), with the static renormalisation schedule when the
-// problem's interval is 8 (the Fortran default).
+// One output tile of a reference-order GEMM: the direct-copy tile body
+// (gemm_tile.h: operands copied straight into two LDS stages, the same
+// chains as the standalone GEMM), with the static renormalisation schedule
+// when the problem's interval is 8 (the Fortran default).
 template <int TM, int TN>
-__device__ __forceinline__ void gemm_tile(const GemmD &g, int tr, int tc, double *lds) {
+__device__ __forceinline__ void gemm_tile(const GemmD &g, int tr, int tc, char *lds) {
+  static_assert(mfhip_tile::kDirectBK == kBK, "K block");
   const int64_t m0 = (int64_t)tr * (16 * TM), n0 = (int64_t)tc * (16 * TN);
   if (g.ri == 8)
-    mfhip_tile::plain_tile<TM, TN, kBK, 8>(g.A, g.B, g.C, g.m, g.k, g.n, g.ri, m0, n0, lds);
+    mfhip_tile::grouped_tile_direct<TM, TN, 8>(g.A, g.B, g.C, g.m, g.k, g.n, g.ri, m0, n0, lds);
   else
-    mfhip_tile::plain_tile<TM, TN, kBK, 0>(g.A, g.B, g.C, g.m, g.k, g.n, g.ri, m0, n0, lds);
+    mfhip_tile::grouped_tile_direct<TM, TN, 0>(g.A, g.B, g.C, g.m, g.k, g.n, g.ri, m0, n0, lds);
 }
 
 // Block tree over 256 normalised pairs, fixed shape (the same as reduce.hip).
@@ -195,7 +197,7 @@ __device__ void dot_chunk(const DotD &d, int c, double2 *partials, int *dot_done
 template <int TM, int TN, int WPE>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WPE))) void k_grouped_gemm(
     const Item *__restrict__ items, const int *__restrict__ off, const GemmD *__restrict__ gemms) {
-  __shared__ __attribute__((aligned(16))) double lds[2 * kBK * (16 * TM + 16 * TN)];
+  extern __shared__ __attribute__((aligned(16))) char lds[];  // DirectShape<TM, TN>::kLds bytes
   const int end = off[blockIdx.x + 1];
   for (int it = off[blockIdx.x]; it < end; ++it) {
     const Item w = items[it];
@@ -254,7 +256,14 @@ void launch_kind(const Plan *p, hipStream_t s) {
   static_assert(kKindWPE[Q] == WPE, "the plan deals kind Q's items to kKindWPE[Q] workgroups per CU");
   constexpr int q = Q;
   if (p->groups[q] <= 0) return;
-  hipLaunchKernelGGL((k_grouped_gemm<TM, TN, WPE>), dim3((unsigned)p->groups[q]), dim3(kThreads), 0, s,
+  constexpr int lds = mfhip_tile::DirectShape<TM, TN>::kLds;
+  static_assert(lds * WPE <= 160 * 1024, "the kind's workgroups per CU fit in LDS");
+  if (lds > 65536) {  // above the default dynamic LDS limit: raised once per kind
+    static const bool raised = hipFuncSetAttribute((const void *)k_grouped_gemm<TM, TN, WPE>,
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, lds) == hipSuccess;
+    (void)raised;
+  }
+  hipLaunchKernelGGL((k_grouped_gemm<TM, TN, WPE>), dim3((unsigned)p->groups[q]), dim3(kThreads), lds, s,
                      p->items + p->first[q], p->offs + p->ofirst[q], p->gemms);
 }
 

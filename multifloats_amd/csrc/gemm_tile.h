@@ -288,7 +288,7 @@ __device__ __forceinline__ void plain_tile_direct(const double2 *__restrict__ A,
       for (int j = 0; j < 4; ++j) renorm(sh[i][j], sl[i][j]);
   };
 
-  issue(0, 0);
+  if (nb > 0) issue(0, 0);  // (the launcher never passes k = 0)
   for (int64_t kb = 0; kb < nb; ++kb) {
     // this wave's copies of block kb have landed (the only ones in flight),
     // every wave's too after the barrier, and every wave is done with the
@@ -357,6 +357,160 @@ __device__ __forceinline__ void plain_tile_direct(const double2 *__restrict__ A,
     if (gj >= n) continue;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
+      const int64_t gi = m0 + tx + 16 * i;
+      if (gi >= m) continue;
+      double h = sh[i][j], l = sl[i][j];
+      renorm(h, l);
+      C[gi + gj * m] = make_double2(h, l);
+    }
+  }
+}
+
+// The same direct copies for a (16 TM) x (16 TN) tile of any of the grouped
+// shapes (TM, TN in 1..8; one MAC per chain and step, in the order of
+// gemm_tile's steps). Layout of a stage:
+//   A: column kk at kk * BM * 16 bytes, its BM rows contiguous (ceil(BM / 64)
+//     copies of up to 64 rows; lanes past the tile's rows copy nothing);
+//   B: 4 TN copies of four 16-k columns, copy q at direct_b_base(q).
+// Lane group ty = 4 w + u (w = ty / 4, u = ty % 4) owns, for j < TN, the
+// column 4 q + c of copy q = u + 4 t, with t = (TN w + j) % TN and
+// c = (TN w + j) / TN: a bijection onto the tile's columns in which the four
+// lane groups of a wave always read four copies with different bank offsets.
+template <int TM, int TN>
+struct DirectShape {
+  static constexpr int BK = kDirectBK;
+  static constexpr int BM = 16 * TM, BN = 16 * TN;
+  static constexpr int kACopies = (BM + 63) / 64;  // copies per A column
+  static constexpr int kBCopies = 4 * TN;
+  static constexpr int kStageA = BK * BM * 16;
+  static constexpr int kStageB = ((direct_b_base(kBCopies - 1) + 1024) + 15) / 16 * 16;
+  static constexpr int kStage = kStageA + kStageB;
+  static constexpr int kLds = 2 * kStage;
+};
+
+template <int TM, int TN, int R>
+__device__ __forceinline__ void grouped_tile_direct(const double2 *__restrict__ A, const double2 *__restrict__ B,
+                                                    double2 *__restrict__ C, int64_t m, int64_t k, int64_t n,
+                                                    int64_t ri, int64_t m0, int64_t n0, char *lds) {
+  typedef DirectShape<TM, TN> S;
+  constexpr int BK = S::BK, BM = S::BM;
+  static_assert(BK == 16 && (BK * S::kACopies) % 4 == 0 && S::kBCopies % 4 == 0, "copies split over 4 waves");
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int tx = tid & 15, ty = tid >> 4;
+  const int tw = ty >> 2, tu = ty & 3;
+  const int64_t nb = (k + BK - 1) / BK;
+  const int bc = lane >> 4, bk = lane & 15;  // B copy: column in the copy, k
+  auto issue = [&](int64_t kb, int buf) {
+    char *st = lds + buf * S::kStage;
+#pragma unroll
+    for (int a = wave; a < BK * S::kACopies; a += 4) {
+      const int kk = a / S::kACopies, cc = a % S::kACopies;
+      const int rows = BM - 64 * cc < 64 ? BM - 64 * cc : 64;  // rows of this copy
+      if (lane < rows) {
+        const int64_t r = m0 + 64 * cc + lane < m ? m0 + 64 * cc + lane : m - 1;
+        const int64_t p = kb * BK + kk < k ? kb * BK + kk : k - 1;
+        __builtin_amdgcn_global_load_lds(
+            (const void *)(A + r + p * m),
+            (__attribute__((address_space(3))) void *)(st + kk * BM * 16 + 64 * cc * 16), 16, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int q = wave; q < S::kBCopies; q += 4) {
+      const int64_t col = n0 + 4 * q + bc < n ? n0 + 4 * q + bc : n - 1;
+      const int64_t p = kb * BK + bk < k ? kb * BK + bk : k - 1;
+      __builtin_amdgcn_global_load_lds((const void *)(B + p + col * k),
+                                       (__attribute__((address_space(3))) void *)(st + S::kStageA + direct_b_base(q)),
+                                       16, 0, 0);
+    }
+  };
+  // this lane's B columns: copy and column in the copy
+  int bq[TN], bcol[TN];
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    const int idx = TN * tw + j;
+    bq[j] = tu + 4 * (idx % TN);
+    bcol[j] = idx / TN;
+  }
+
+  double sh[TM][TN], sl[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) sh[i][j] = sl[i][j] = 0.0;
+  const int64_t rint = (ri > 0 && k > ri) ? ri : 0;
+  int64_t until = rint;
+  int64_t done = 0;
+  auto macs = [&](const char *st, int kk) {
+    double ah[TM], al[TM], bh[TN], bl[TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      const double2 v = *(const double2 *)(st + kk * BM * 16 + (tx + 16 * i) * 16);
+      ah[i] = v.x;
+      al[i] = v.y;
+    }
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const double2 v = *(const double2 *)(st + S::kStageA + direct_b_base(bq[j]) + bcol[j] * BK * 16 + kk * 16);
+      bh[j] = v.x;
+      bl[j] = v.y;
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) mac(ah[i], al[i], bh[j], bl[j], sh[i][j], sl[i][j]);
+  };
+  auto renorm_all = [&]() {
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) renorm(sh[i][j], sl[i][j]);
+  };
+
+  if (nb > 0) issue(0, 0);  // k = 0: no copies (the outputs are zeros)
+  for (int64_t kb = 0; kb < nb; ++kb) {
+    // this wave's copies of block kb have landed, every wave's after the
+    // barrier, and every wave is done with the other buffer
+    __syncthreads();
+    if (kb + 1 < nb) issue(kb + 1, (int)((kb + 1) & 1));
+    const char *st = lds + (int)(kb & 1) * S::kStage;
+    const int64_t k0 = kb * BK;
+    const int kmax = (int)((k - k0) < BK ? (k - k0) : BK);
+    if (R > 0 && rint == R && k0 + BK < k) {
+      constexpr int RR = R > 0 ? R : 1;
+#pragma unroll 1
+      for (int c = 0; c < BK; c += RR) {
+#pragma unroll
+        for (int kk = 0; kk < RR; ++kk) macs(st, c + kk);
+        renorm_all();
+      }
+      done += BK;
+    } else {
+      auto step = [&](int kk) {
+        macs(st, kk);
+        ++done;
+        if (rint && --until == 0) {
+          until = rint;
+          if (done < k) renorm_all();
+        }
+      };
+      if (kmax == BK) {
+#pragma unroll
+        for (int kk = 0; kk < BK; ++kk) step(kk);
+      } else {
+        for (int kk = 0; kk < kmax; ++kk) step(kk);
+      }
+    }
+  }
+  // the next item's copies reuse both buffers: every wave done reading them
+  __syncthreads();
+
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    const int64_t gj = n0 + 4 * bq[j] + bcol[j];
+    if (gj >= n) continue;
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
       const int64_t gi = m0 + tx + 16 * i;
       if (gi >= m) continue;
       double h = sh[i][j], l = sl[i][j];
