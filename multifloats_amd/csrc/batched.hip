@@ -247,7 +247,7 @@ struct Plan {
 
 // Workgroups per CU of each kind's launch (its waves per SIMD: 4-wave
 // workgroups), in kind order; the dots run two.
-constexpr int kKindWPE[ITEM_DOT + 1] = {4, 1, 2, 2, 3, 3, 4, 2, 2};
+constexpr int kKindWPE[ITEM_DOT + 1] = {4, 2, 2, 2, 3, 3, 5, 3, 2};
 
 // One persistent launch per kind with items: grid = the kind's resident
 // workgroups over all CUs (its occupancy), or fewer when it has fewer items.
@@ -271,13 +271,13 @@ hipError_t run_plan(Plan *p, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_ints, dim3((unsigned)((p->ncounters + 255) / 256)), dim3(256), 0, s,
                      p->counters, p->ncounters);
   launch_kind<2, 2, 4, 0>(p, s);
-  launch_kind<4, 4, 1, 1>(p, s);
+  launch_kind<4, 4, 2, 1>(p, s);
   launch_kind<8, 1, 2, 2>(p, s);
   launch_kind<1, 8, 2, 3>(p, s);
   launch_kind<4, 2, 3, 4>(p, s);
   launch_kind<2, 4, 3, 5>(p, s);
-  launch_kind<1, 1, 4, 6>(p, s);
-  launch_kind<3, 3, 2, 7>(p, s);
+  launch_kind<1, 1, 5, 6>(p, s);
+  launch_kind<3, 3, 3, 7>(p, s);
   if (p->groups[ITEM_DOT] > 0)
     hipLaunchKernelGGL(k_grouped_dot, dim3((unsigned)p->groups[ITEM_DOT]), dim3(kThreads), 0, s,
                        p->items + p->first[ITEM_DOT], p->offs + p->ofirst[ITEM_DOT], p->dots, p->partials,
