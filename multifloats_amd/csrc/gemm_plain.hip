@@ -156,58 +156,56 @@ __global__ __launch_bounds__(kThreads) void k_gevm(const double2 *__restrict__ x
 // (same MAC operand order, same renormalisation schedule), then applies the
 // same subtraction and addition: identical bits, inputs read once, no
 // temporaries. mv and vm are the n = 1 and m = 1 cases (fypp:2623-2649).
-template <int TM, int TN, int BK>
+//
+// A 32 x 32 tile per 256-lane workgroup, four chains per output and two
+// outputs per lane in each direction (16 chains). Operands are copied
+// straight from global memory into two LDS stages of 16 k (the plain 64 x 64
+// kernel's scheme): an A column of 32 complex elements is one 1 KiB copy,
+// two 16-k B columns are one, the next block's copies run during the current
+// block's MACs, one barrier per block. Lane (tx, ty) owns rows tx + 16 i and
+// columns 2 ty + j; its A reads take the real pair of rows 0-7 and the
+// imaginary pair of rows 8-15 first (and the other halves second), so the
+// 16 rows of a read fall on distinct banks. Copy addresses are clamped into
+// the operands (edge rows and columns are not stored; k past the end is never
+// taken). With ri = 8 (R) the renormalisation points are static within a
+// block, as in the real kernels.
+constexpr int kCplxBK = 16;
+constexpr int kCplxStageA = kCplxBK * 1024;
+constexpr int kCplxStageB = mfhip_tile::direct_b_base(15) + 1024;
+constexpr int kCplxStage = kCplxStageA + kCplxStageB;
+constexpr int kCplxLds = 2 * kCplxStage;
+template <int R>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_gemm_cplx(
     const double4 *__restrict__ A, const double4 *__restrict__ B, double4 *__restrict__ C,
     int64_t m, int64_t k, int64_t n, int64_t ri, int tiles_m) {
-  constexpr int BM = 16 * TM, BN = 16 * TN;
-  constexpr int A_PER = BM * BK / kThreads;
-  constexpr int B_PER = BN * BK / kThreads;
-  static_assert(A_PER * kThreads == BM * BK && B_PER * kThreads == BN * BK, "tile");
-  __shared__ double As[4][BK][BM];  // re.hi, re.lo, im.hi, im.lo
-  __shared__ double Bs[4][BK][BN];
-
-  const int tid = threadIdx.x;
+  constexpr int BK = kCplxBK, TM = 2, TN = 2;
+  extern __shared__ __attribute__((aligned(16))) char lds[];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int tx = tid & 15, ty = tid >> 4;
   const int tile = blockIdx.x;
-  const int64_t m0 = (int64_t)(tile % tiles_m) * BM;
-  const int64_t n0 = (int64_t)(tile / tiles_m) * BN;
-
-  double4 ra[A_PER], rb[B_PER];
-  auto load = [&](int64_t k0) {
+  const int64_t m0 = (int64_t)(tile % tiles_m) * 32;
+  const int64_t n0 = (int64_t)(tile / tiles_m) * 32;
+  const int64_t nb = (k + BK - 1) / BK;
+  // A copy: lane = (row lane / 2, half lane % 2); B copy q: (column 2q + lane / 32, k (lane / 2) % 16, half)
+  const int64_t ar = m0 + (lane >> 1) < m ? m0 + (lane >> 1) : m - 1;
+  const int bcol = lane >> 5, bkk = (lane >> 1) & 15, half = lane & 1;
+  auto issue = [&](int64_t kb, int buf) {
+    char *st = lds + buf * kCplxStage;
 #pragma unroll
-    for (int q = 0; q < A_PER; ++q) {
-      int e = tid + q * kThreads;
-      int r = e % BM, c = e / BM;
-      int64_t gi = m0 + r, gp = k0 + c;
-      ra[q] = (gi < m && gp < k) ? A[gi + gp * m] : make_double4(0.0, 0.0, 0.0, 0.0);
+    for (int c = wave; c < BK; c += 4) {
+      const int64_t p = kb * BK + c < k ? kb * BK + c : k - 1;
+      __builtin_amdgcn_global_load_lds((const void *)((const char *)(A + ar + p * m) + 16 * (lane & 1)),
+                                       (__attribute__((address_space(3))) void *)(st + c * 1024), 16, 0, 0);
     }
 #pragma unroll
-    for (int q = 0; q < B_PER; ++q) {
-      int e = tid + q * kThreads;
-      int r = e % BK, c = e / BK;
-      int64_t gp = k0 + r, gj = n0 + c;
-      rb[q] = (gp < k && gj < n) ? B[gp + gj * k] : make_double4(0.0, 0.0, 0.0, 0.0);
-    }
-  };
-  auto stash = [&]() {
-#pragma unroll
-    for (int q = 0; q < A_PER; ++q) {
-      int e = tid + q * kThreads;
-      int r = e % BM, c = e / BM;
-      As[0][c][r] = ra[q].x;
-      As[1][c][r] = ra[q].y;
-      As[2][c][r] = ra[q].z;
-      As[3][c][r] = ra[q].w;
-    }
-#pragma unroll
-    for (int q = 0; q < B_PER; ++q) {
-      int e = tid + q * kThreads;
-      int r = e % BK, c = e / BK;
-      Bs[0][r][c] = rb[q].x;
-      Bs[1][r][c] = rb[q].y;
-      Bs[2][r][c] = rb[q].z;
-      Bs[3][r][c] = rb[q].w;
+    for (int q = wave; q < 16; q += 4) {
+      const int64_t col = n0 + 2 * q + bcol < n ? n0 + 2 * q + bcol : n - 1;
+      const int64_t p = kb * BK + bkk < k ? kb * BK + bkk : k - 1;
+      __builtin_amdgcn_global_load_lds((const void *)((const char *)(B + p + col * k) + 16 * half),
+                                       (__attribute__((address_space(3))) void *)(st + kCplxStageA +
+                                                                                   mfhip_tile::direct_b_base(q)),
+                                       16, 0, 0);
     }
   };
 
@@ -219,25 +217,31 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     for (int i = 0; i < TM; ++i)
 #pragma unroll
       for (int j = 0; j < TN; ++j) sh[c][i][j] = sl[c][i][j] = 0.0;
-
   const int64_t rint = (ri > 0 && k > ri) ? ri : 0;
   int64_t until = rint, done = 0;
-
-  auto step = [&](int kk) {
+  const int hf = (tx >> 3) & 1;  // which half of a row this lane reads first
+  auto macs = [&](const char *st, int kk) {
     double arh[TM], arl[TM], aih[TM], ail[TM], brh[TN], brl[TN], bih[TN], bil[TN];
 #pragma unroll
     for (int i = 0; i < TM; ++i) {
-      arh[i] = As[0][kk][tx * TM + i];
-      arl[i] = As[1][kk][tx * TM + i];
-      aih[i] = As[2][kk][tx * TM + i];
-      ail[i] = As[3][kk][tx * TM + i];
+      const char *e = st + kk * 1024 + (tx + 16 * i) * 32;
+      const double2 v0 = *(const double2 *)(e + 16 * hf);
+      const double2 v1 = *(const double2 *)(e + 16 * (1 - hf));
+      const double2 re = hf ? v1 : v0, im = hf ? v0 : v1;
+      arh[i] = re.x;
+      arl[i] = re.y;
+      aih[i] = im.x;
+      ail[i] = im.y;
     }
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
-      brh[j] = Bs[0][kk][ty * TN + j];
-      brl[j] = Bs[1][kk][ty * TN + j];
-      bih[j] = Bs[2][kk][ty * TN + j];
-      bil[j] = Bs[3][kk][ty * TN + j];
+      // column 2 ty + j: copy ty, column j of the copy
+      const char *e = st + kCplxStageA + mfhip_tile::direct_b_base(ty) + j * 512 + kk * 32;
+      const double2 re = *(const double2 *)e, im = *(const double2 *)(e + 16);
+      brh[j] = re.x;
+      brl[j] = re.y;
+      bih[j] = im.x;
+      bil[j] = im.y;
     }
 #pragma unroll
     for (int i = 0; i < TM; ++i)
@@ -248,38 +252,55 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         mac(arh[i], arl[i], bih[j], bil[j], sh[2][i][j], sl[2][i][j]);
         mac(aih[i], ail[i], brh[j], brl[j], sh[3][i][j], sl[3][i][j]);
       }
-    ++done;
-    if (rint && --until == 0) {
-      until = rint;
-      if (done < k) {
+  };
+  auto renorm_all = [&]() {
 #pragma unroll
-        for (int c = 0; c < 4; ++c)
+    for (int c = 0; c < 4; ++c)
 #pragma unroll
-          for (int i = 0; i < TM; ++i)
+      for (int i = 0; i < TM; ++i)
 #pragma unroll
-            for (int j = 0; j < TN; ++j) renorm(sh[c][i][j], sl[c][i][j]);
-      }
-    }
+        for (int j = 0; j < TN; ++j) renorm(sh[c][i][j], sl[c][i][j]);
   };
 
-  load(0);
-  for (int64_t k0 = 0; k0 < k; k0 += BK) {
+  if (nb > 0) issue(0, 0);
+  for (int64_t kb = 0; kb < nb; ++kb) {
+    // this wave's copies of block kb have landed, every wave's after the
+    // barrier, and every wave is done with the other buffer
     __syncthreads();
-    stash();
-    __syncthreads();
-    if (k0 + BK < k) load(k0 + BK);
-    const int kmax = (k - k0) < BK ? (int)(k - k0) : BK;
+    if (kb + 1 < nb) issue(kb + 1, (int)((kb + 1) & 1));
+    const char *st = lds + (int)(kb & 1) * kCplxStage;
+    const int64_t k0 = kb * BK;
+    const int kmax = (int)((k - k0) < BK ? (k - k0) : BK);
+    if (R > 0 && rint == R && k0 + BK < k) {
+      constexpr int RR = R > 0 ? R : 1;
+      static_assert(R == 0 || BK % RR == 0, "static renorm interval divides the K block");
 #pragma unroll 1
-    for (int kk = 0; kk < kmax; ++kk) step(kk);
+      for (int c = 0; c < BK; c += RR) {
+#pragma unroll
+        for (int kk = 0; kk < RR; ++kk) macs(st, c + kk);
+        renorm_all();
+      }
+      done += BK;
+    } else {
+#pragma unroll 1
+      for (int kk = 0; kk < kmax; ++kk) {
+        macs(st, kk);
+        ++done;
+        if (rint && --until == 0) {
+          until = rint;
+          if (done < k) renorm_all();
+        }
+      }
+    }
   }
 
 #pragma unroll
   for (int j = 0; j < TN; ++j) {
-    const int64_t gj = n0 + ty * TN + j;
+    const int64_t gj = n0 + 2 * ty + j;
     if (gj >= n) continue;
 #pragma unroll
     for (int i = 0; i < TM; ++i) {
-      const int64_t gi = m0 + tx * TM + i;
+      const int64_t gi = m0 + tx + 16 * i;
       if (gi >= m) continue;
       ddv p[4];
 #pragma unroll
@@ -427,11 +448,20 @@ hipError_t gemm_cplx(const double4 *A, const double4 *B, double4 *C, int64_t m, 
                      int64_t n, int64_t ri, hipStream_t s) {
   if (m <= 0 || n <= 0) return hipSuccess;
   if (k <= 0) return zero_doubles((double *)C, 4 * m * n, s);
-  constexpr int TM = 2, TN = 2, BK = 16;
-  const int64_t tm = (m + 16 * TM - 1) / (16 * TM), tn = (n + 16 * TN - 1) / (16 * TN);
+  const int64_t tm = (m + 31) / 32, tn = (n + 31) / 32;
   if (tm * tn > 0x7fffffff) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((k_gemm_cplx<TM, TN, BK>), dim3((unsigned)(tm * tn)), dim3(kThreads), 0, s, A, B,
-                     C, m, k, n, ri, (int)tm);
+  // above the 64 KiB default: raised once per process and kernel
+  static const bool ok8 = hipFuncSetAttribute((const void *)k_gemm_cplx<8>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              kCplxLds) == hipSuccess;
+  static const bool ok0 = hipFuncSetAttribute((const void *)k_gemm_cplx<0>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              kCplxLds) == hipSuccess;
+  if (!ok8 || !ok0) return hipErrorInvalidConfiguration;
+  if (ri == 8 && k > 8)
+    hipLaunchKernelGGL(k_gemm_cplx<8>, dim3((unsigned)(tm * tn)), dim3(kThreads), kCplxLds, s, A, B, C, m, k, n, ri,
+                       (int)tm);
+  else
+    hipLaunchKernelGGL(k_gemm_cplx<0>, dim3((unsigned)(tm * tn)), dim3(kThreads), kCplxLds, s, A, B, C, m, k, n, ri,
+                       (int)tm);
   return hipGetLastError();
 }
 
