@@ -30,6 +30,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WPE)))
     static_assert(BK == mfhip_tile::kDirectBK, "K block of the direct tile");
     extern __shared__ __attribute__((aligned(16))) char dyn_lds[];
     mfhip_tile::plain_tile_direct<R>(A, B, C, m, k, n, ri, m0, n0, dyn_lds);
+  } else if constexpr (BK == mfhip_tile::kDirectBK) {
+    // the other shapes with 16-k blocks: the general direct-copy tile
+    extern __shared__ __attribute__((aligned(16))) char dyn_lds[];
+    mfhip_tile::grouped_tile_direct<TM, TN, R>(A, B, C, m, k, n, ri, m0, n0, dyn_lds);
   } else {
     __shared__ double lds[2 * BK * (16 * TM + 16 * TN)];
     mfhip_tile::plain_tile<TM, TN, BK, R>(A, B, C, m, k, n, ri, m0, n0, lds);
@@ -323,9 +327,12 @@ hipError_t launch_gemm(const double2 *A, const double2 *B, double2 *C, int64_t m
   const int64_t tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
   if (tm * tn > 0x7fffffff) return hipErrorInvalidValue;
   const bool r8 = ri == 8 && k > 8;
-  constexpr bool direct = TM == 4 && TN == 4 && WPE == 2;
-  const int dyn = direct ? mfhip_tile::kDirectLds : 0;
-  if (direct) {
+  constexpr bool direct = BK == mfhip_tile::kDirectBK;
+  constexpr int dyn = !direct ? 0
+                      : (TM == 4 && TN == 4 && WPE == 2) ? mfhip_tile::kDirectLds
+                                                          : mfhip_tile::DirectShape<TM, TN>::kLds;
+  static_assert(dyn * WPE <= 160 * 1024, "the workgroups per CU fit in LDS");
+  if (direct && dyn > 65536) {
     // above the 64 KiB default: allowed once per process and kernel
     static const bool ok8 = hipFuncSetAttribute((const void *)k_gemm_plain<TM, TN, BK, WPE, 8>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, dyn) == hipSuccess;
