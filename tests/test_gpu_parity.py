@@ -334,7 +334,8 @@ def test_matmul_bits_golden(gpu):
         nt.assert_bits(mf.matmul(g[key + "_x"], B, ri), g[key + "_vm"], "vm " + key)
 
 
-@pytest.mark.parametrize("shape", [(1, 1, 1), (63, 65, 67), (128, 300, 96), (200, 17, 129)])
+# (400, 37, 330): 525 tiles of 16 x 16, where that variant takes its direct-copy form
+@pytest.mark.parametrize("shape", [(1, 1, 1), (63, 65, 67), (128, 300, 96), (200, 17, 129), (400, 37, 330)])
 @pytest.mark.parametrize("ri", [0, 8, 5])
 def test_matmul_every_variant_bits_vs_oracle(torch_cuda, shape, ri):
     torch = torch_cuda
