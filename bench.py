@@ -263,6 +263,16 @@ def bench_gemm(args, torch, mf):
         emax, emean = sampled_error(A, B, C, rows, cols)
         res["plain"] = dict(ms=ms, value=flops / (ms * 1e-3) / 1e9, err_max=emax, err_mean=emean,
                             roofline=gemm_roofline_plain(m, n, k, ms, args.size, args.variant))
+        if m == n == k == 4096:
+            # the same call back to back for about 2.5 s, amd-smi sampled alongside
+            ms_p, pw, clk, ns = power_during(torch, step_plain, 2.5, 4)
+            if pw is not None:
+                res["plain"]["roofline"]["power"] = {
+                    "socket_power_w": pw, "gfx_clock_mhz": clk, "ms_per_call": round(ms_p, 3), "samples": ns,
+                    "fp64_issue_frac_at_clock": round(res["plain"]["roofline"]["fp64_issue_frac"] * 2400.0 / clk, 4),
+                    "note": "the call back to back for 2.5 s with amd-smi sampled alongside (medians past the first "
+                            "quarter); fp64_issue_frac_at_clock rescales the issue fraction from 2.4 GHz to the "
+                            "sampled clock"}
         if not args.no_cpu:
             dt, Cc, kind = cpu_matmul(A, B, args.cpu_cols, ri)
             from tests import _native as nt
