@@ -33,7 +33,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WPE)))
   } else if constexpr (BK == mfhip_tile::kDirectBK) {
     // the other shapes with 16-k blocks: the general direct-copy tile
     extern __shared__ __attribute__((aligned(16))) char dyn_lds[];
-    mfhip_tile::grouped_tile_direct<TM, TN, R>(A, B, C, m, k, n, ri, m0, n0, dyn_lds);
+    mfhip_tile::grouped_tile_direct<TM, TN, R>(A, m, B, k, C, m, m, k, n, ri, m0, n0, dyn_lds);
   } else {
     __shared__ double lds[2 * BK * (16 * TM + 16 * TN)];
     mfhip_tile::plain_tile<TM, TN, BK, R>(A, B, C, m, k, n, ri, m0, n0, lds);
@@ -48,11 +48,19 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WPE)))
     const double2 *__restrict__ A, int64_t lda, const double2 *__restrict__ B, int64_t ldb,
     double2 *__restrict__ C, int64_t ldc, int64_t m, int64_t k, int64_t n, int64_t ri, ddv alpha, ddv beta,
     int tiles_m) {
-  __shared__ double lds[2 * BK * (16 * TM + 16 * TN)];
   const int tile = blockIdx.x;
-  mfhip_tile::gemm_tile<TM, TN, BK, R, TA, TB, true>(A, lda, B, ldb, C, ldc, m, k, n, ri,
-                                                      (int64_t)(tile % tiles_m) * (16 * TM),
-                                                      (int64_t)(tile / tiles_m) * (16 * TN), lds, alpha, beta);
+  const int64_t m0 = (int64_t)(tile % tiles_m) * (16 * TM), n0 = (int64_t)(tile / tiles_m) * (16 * TN);
+  if constexpr (BK == mfhip_tile::kDirectBK) {
+    // the direct-copy tile for every transpose pair (dynamic LDS,
+    // DirectShape<TM, TN, TA, TB>::kLds bytes)
+    extern __shared__ __attribute__((aligned(16))) char dyn_lds[];
+    mfhip_tile::grouped_tile_direct<TM, TN, R, true, TA, TB>(A, lda, B, ldb, C, ldc, m, k, n, ri, m0, n0, dyn_lds,
+                                                             alpha, beta);
+  } else {
+    __shared__ double lds[2 * BK * (16 * TM + 16 * TN)];
+    mfhip_tile::gemm_tile<TM, TN, BK, R, TA, TB, true>(A, lda, B, ldb, C, ldc, m, k, n, ri, m0, n0, lds, alpha,
+                                                        beta);
+  }
 }
 
 // C = beta C (beta == 0: zeros, C not read) for the BLAS quick paths.
@@ -399,17 +407,27 @@ hipError_t gemm_plain(const double2 *A, const double2 *B, double2 *C, int64_t m,
 }
 
 namespace {
-template <int TM, int TN, int WPE, bool TA, bool TB>
+template <int TM, int TN, int WPE, bool TA, bool TB, int BKT = 16>
 hipError_t launch_blas(const double2 *A, int64_t lda, const double2 *B, int64_t ldb, double2 *C, int64_t ldc,
                        int64_t m, int64_t k, int64_t n, int64_t ri, ddv alpha, ddv beta, hipStream_t s) {
+  constexpr int BK = BKT;
   constexpr int BM = 16 * TM, BN = 16 * TN;
   const int64_t tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
   if (tm * tn > 0x7fffffff) return hipErrorInvalidValue;
+  constexpr int dyn = BK == mfhip_tile::kDirectBK ? mfhip_tile::DirectShape<TM, TN, TA, TB>::kLds : 0;
+  static_assert(dyn * WPE <= 160 * 1024, "the workgroups per CU fit in LDS");
+  if (dyn > 65536) {  // above the 64 KiB default: raised once per process and kernel
+    static const bool ok8 = hipFuncSetAttribute((const void *)k_gemm_blas<TM, TN, BK, WPE, 8, TA, TB>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, dyn) == hipSuccess;
+    static const bool ok0 = hipFuncSetAttribute((const void *)k_gemm_blas<TM, TN, BK, WPE, 0, TA, TB>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, dyn) == hipSuccess;
+    if (!ok8 || !ok0) return hipErrorInvalidConfiguration;
+  }
   if (ri == 8 && k > 8)
-    hipLaunchKernelGGL((k_gemm_blas<TM, TN, 16, WPE, 8, TA, TB>), dim3((unsigned)(tm * tn)), dim3(kThreads), 0,
+    hipLaunchKernelGGL((k_gemm_blas<TM, TN, BK, WPE, 8, TA, TB>), dim3((unsigned)(tm * tn)), dim3(kThreads), dyn,
                        s, A, lda, B, ldb, C, ldc, m, k, n, ri, alpha, beta, (int)tm);
   else
-    hipLaunchKernelGGL((k_gemm_blas<TM, TN, 16, WPE, 0, TA, TB>), dim3((unsigned)(tm * tn)), dim3(kThreads), 0,
+    hipLaunchKernelGGL((k_gemm_blas<TM, TN, BK, WPE, 0, TA, TB>), dim3((unsigned)(tm * tn)), dim3(kThreads), dyn,
                        s, A, lda, B, ldb, C, ldc, m, k, n, ri, alpha, beta, (int)tm);
   return hipGetLastError();
 }
@@ -420,7 +438,12 @@ hipError_t blas_by_size(const double2 *A, int64_t lda, const double2 *B, int64_t
     case 1: return launch_blas<4, 4, 2, TA, TB>(A, lda, B, ldb, C, ldc, m, k, n, ri, alpha, beta, s);
     case 2: return launch_blas<4, 2, 3, TA, TB>(A, lda, B, ldb, C, ldc, m, k, n, ri, alpha, beta, s);
     case 0: return launch_blas<2, 2, 4, TA, TB>(A, lda, B, ldb, C, ldc, m, k, n, ri, alpha, beta, s);
-    default: return launch_blas<1, 1, 4, TA, TB>(A, lda, B, ldb, C, ldc, m, k, n, ri, alpha, beta, s);
+    default:
+      // 16 x 16 tiles below 512 of them: register staging with 32-k blocks
+      // (one workgroup per CU hides the direct copies' latency poorly)
+      if (((m + 15) / 16) * ((n + 15) / 16) < 512)
+        return launch_blas<1, 1, 4, TA, TB, 32>(A, lda, B, ldb, C, ldc, m, k, n, ri, alpha, beta, s);
+      return launch_blas<1, 1, 4, TA, TB>(A, lda, B, ldb, C, ldc, m, k, n, ri, alpha, beta, s);
   }
 }
 bool trans_ok(char t) { return t == 'N' || t == 'n' || t == 'T' || t == 't' || t == 'C' || t == 'c'; }

@@ -368,70 +368,124 @@ __device__ __forceinline__ void plain_tile_direct(const double2 *__restrict__ A,
 
 // The same direct copies for a (16 TM) x (16 TN) tile of any of the grouped
 // shapes (TM, TN in 1..8; one MAC per chain and step, in the order of
-// gemm_tile's steps). Layout of a stage:
-//   A: column kk at kk * BM * 16 bytes, its BM rows contiguous (ceil(BM / 64)
-//     copies of up to 64 rows; lanes past the tile's rows copy nothing);
-//   B: 4 TN copies of four 16-k columns, copy q at direct_b_base(q).
-// Lane group ty = 4 w + u (w = ty / 4, u = ty % 4) owns, for j < TN, the
-// column 4 q + c of copy q = u + 4 t, with t = (TN w + j) % TN and
-// c = (TN w + j) / TN: a bijection onto the tile's columns in which the four
-// lane groups of a wave always read four copies with different bank offsets.
-template <int TM, int TN>
+// gemm_tile's steps), for op(A) = A or A^T and op(B) = B or B^T. Layout of
+// a stage:
+//   A (op(A) = A): column kk at kk * BM * 16 bytes, its BM rows contiguous
+//     (ceil(BM / 64) copies of up to 64 rows; lanes past the tile's rows copy
+//     nothing); lane tx owns rows tx + 16 i, and a read takes 16 consecutive
+//     rows;
+//   A (TA: rows contiguous along k): copy q holds rows 4q .. 4q + 3 by 16 k
+//     at direct_b_base(q); lane tx owns rows TM tx + i, so for TM = 4 the 16
+//     rows of a read sit in 16 copies at distinct bank offsets (two-way
+//     conflicts for TM = 2, none for TM = 1);
+//   B (op(B) = B): 4 TN copies of four 16-k columns, copy q at
+//     direct_b_base(q). Lane group ty = 4 w + u (w = ty / 4, u = ty % 4)
+//     owns, for j < TN, the column 4 q + c of copy q = u + 4 t, with
+//     t = (TN w + j) % TN and c = (TN w + j) / TN: a bijection onto the
+//     tile's columns in which the four lane groups of a wave always read four
+//     copies with different bank offsets;
+//   B (TB: columns contiguous for a given k): row kk at kk * BN * 16 bytes
+//     (ceil(BN / 64) copies); lane group ty owns columns ty + 16 j, so the
+//     four lane groups of a wave read four consecutive columns.
+template <int TM, int TN, bool TA = false, bool TB = false>
 struct DirectShape {
   static constexpr int BK = kDirectBK;
   static constexpr int BM = 16 * TM, BN = 16 * TN;
-  static constexpr int kACopies = (BM + 63) / 64;  // copies per A column
-  static constexpr int kBCopies = 4 * TN;
-  static constexpr int kStageA = BK * BM * 16;
-  static constexpr int kStageB = ((direct_b_base(kBCopies - 1) + 1024) + 15) / 16 * 16;
+  static constexpr int kACopies = TA ? BM / 4 : (BM + 63) / 64;  // TA: in total; else per A column
+  static constexpr int kBCopies = TB ? (BN + 63) / 64 : 4 * TN;   // TB: per B row; else in total
+  static constexpr int kStageA = TA ? ((direct_b_base(kACopies - 1) + 1024) + 15) / 16 * 16 : BK * BM * 16;
+  static constexpr int kStageB = TB ? BK * BN * 16 : ((direct_b_base(kBCopies - 1) + 1024) + 15) / 16 * 16;
   static constexpr int kStage = kStageA + kStageB;
   static constexpr int kLds = 2 * kStage;
 };
 
-template <int TM, int TN, int R>
-__device__ __forceinline__ void grouped_tile_direct(const double2 *__restrict__ A, const double2 *__restrict__ B,
-                                                    double2 *__restrict__ C, int64_t m, int64_t k, int64_t n,
-                                                    int64_t ri, int64_t m0, int64_t n0, char *lds) {
-  typedef DirectShape<TM, TN> S;
-  constexpr int BK = S::BK, BM = S::BM;
-  static_assert(BK == 16 && (BK * S::kACopies) % 4 == 0 && S::kBCopies % 4 == 0, "copies split over 4 waves");
+// Leading dimensions lda, ldb, ldc as in gemm_tile (op(A)(i, p) at
+// A[i + p lda], or A[p + i lda] with TA; op(B)(p, j) at B[p + j ldb], or
+// B[j + p ldb] with TB; C(i, j) at C[i + j ldc]); AB: the result T of the
+// chains is written as alpha T + beta C.
+template <int TM, int TN, int R, bool AB = false, bool TA = false, bool TB = false>
+__device__ __forceinline__ void grouped_tile_direct(const double2 *__restrict__ A, int64_t lda,
+                                                    const double2 *__restrict__ B, int64_t ldb,
+                                                    double2 *__restrict__ C, int64_t ldc, int64_t m, int64_t k,
+                                                    int64_t n, int64_t ri, int64_t m0, int64_t n0, char *lds,
+                                                    ddv alpha = ddv{1.0, 0.0}, ddv beta = ddv{0.0, 0.0}) {
+  typedef DirectShape<TM, TN, TA, TB> S;
+  constexpr int BK = S::BK, BM = S::BM, BN = S::BN;
+  static_assert(BK == 16, "copies of 16-k rows and columns");
+  static_assert((TA ? S::kACopies : BK * S::kACopies) % 4 == 0 && (TB ? BK * S::kBCopies : S::kBCopies) % 4 == 0,
+                "copies split over 4 waves");
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int tx = tid & 15, ty = tid >> 4;
   const int tw = ty >> 2, tu = ty & 3;
   const int64_t nb = (k + BK - 1) / BK;
-  const int bc = lane >> 4, bk = lane & 15;  // B copy: column in the copy, k
+  const int bc = lane >> 4, bk = lane & 15;  // a 4 x 16 copy: row or column in the copy, k
   auto issue = [&](int64_t kb, int buf) {
     char *st = lds + buf * S::kStage;
+    if constexpr (TA) {
 #pragma unroll
-    for (int a = wave; a < BK * S::kACopies; a += 4) {
-      const int kk = a / S::kACopies, cc = a % S::kACopies;
-      const int rows = BM - 64 * cc < 64 ? BM - 64 * cc : 64;  // rows of this copy
-      if (lane < rows) {
-        const int64_t r = m0 + 64 * cc + lane < m ? m0 + 64 * cc + lane : m - 1;
-        const int64_t p = kb * BK + kk < k ? kb * BK + kk : k - 1;
-        __builtin_amdgcn_global_load_lds(
-            (const void *)(A + r + p * m),
-            (__attribute__((address_space(3))) void *)(st + kk * BM * 16 + 64 * cc * 16), 16, 0, 0);
+      for (int q = wave; q < S::kACopies; q += 4) {
+        const int64_t r = m0 + 4 * q + bc < m ? m0 + 4 * q + bc : m - 1;
+        const int64_t p = kb * BK + bk < k ? kb * BK + bk : k - 1;
+        __builtin_amdgcn_global_load_lds((const void *)(A + p + r * lda),
+                                         (__attribute__((address_space(3))) void *)(st + direct_b_base(q)), 16, 0, 0);
+      }
+    } else {
+#pragma unroll
+      for (int a = wave; a < BK * S::kACopies; a += 4) {
+        const int kk = a / S::kACopies, cc = a % S::kACopies;
+        const int rows = BM - 64 * cc < 64 ? BM - 64 * cc : 64;  // rows of this copy
+        if (lane < rows) {
+          const int64_t r = m0 + 64 * cc + lane < m ? m0 + 64 * cc + lane : m - 1;
+          const int64_t p = kb * BK + kk < k ? kb * BK + kk : k - 1;
+          __builtin_amdgcn_global_load_lds(
+              (const void *)(A + r + p * lda),
+              (__attribute__((address_space(3))) void *)(st + kk * BM * 16 + 64 * cc * 16), 16, 0, 0);
+        }
       }
     }
+    if constexpr (TB) {
 #pragma unroll
-    for (int q = wave; q < S::kBCopies; q += 4) {
-      const int64_t col = n0 + 4 * q + bc < n ? n0 + 4 * q + bc : n - 1;
-      const int64_t p = kb * BK + bk < k ? kb * BK + bk : k - 1;
-      __builtin_amdgcn_global_load_lds((const void *)(B + p + col * k),
-                                       (__attribute__((address_space(3))) void *)(st + S::kStageA + direct_b_base(q)),
-                                       16, 0, 0);
+      for (int a = wave; a < BK * S::kBCopies; a += 4) {
+        const int kk = a / S::kBCopies, cc = a % S::kBCopies;
+        const int cols = BN - 64 * cc < 64 ? BN - 64 * cc : 64;  // columns of this copy
+        if (lane < cols) {
+          const int64_t col = n0 + 64 * cc + lane < n ? n0 + 64 * cc + lane : n - 1;
+          const int64_t p = kb * BK + kk < k ? kb * BK + kk : k - 1;
+          __builtin_amdgcn_global_load_lds(
+              (const void *)(B + col + p * ldb),
+              (__attribute__((address_space(3))) void *)(st + S::kStageA + kk * BN * 16 + 64 * cc * 16), 16, 0, 0);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int q = wave; q < S::kBCopies; q += 4) {
+        const int64_t col = n0 + 4 * q + bc < n ? n0 + 4 * q + bc : n - 1;
+        const int64_t p = kb * BK + bk < k ? kb * BK + bk : k - 1;
+        __builtin_amdgcn_global_load_lds(
+            (const void *)(B + p + col * ldb),
+            (__attribute__((address_space(3))) void *)(st + S::kStageA + direct_b_base(q)), 16, 0, 0);
+      }
     }
   };
-  // this lane's B columns: copy and column in the copy
-  int bq[TN], bcol[TN];
+  // this lane's rows and columns, and where a step's operands sit in a stage
+  auto arow = [&](int i) { return TA ? TM * tx + i : tx + 16 * i; };
+  int bq[TN], bcol[TN];  // op(B) = B: copy and column in the copy
 #pragma unroll
   for (int j = 0; j < TN; ++j) {
     const int idx = TN * tw + j;
     bq[j] = tu + 4 * (idx % TN);
     bcol[j] = idx / TN;
   }
+  auto bcolumn = [&](int j) { return TB ? ty + 16 * j : 4 * bq[j] + bcol[j]; };
+  auto a_at = [&](const char *st, int kk, int i) -> const double2 * {
+    const int r = arow(i);
+    return (const double2 *)(TA ? st + direct_b_base(r >> 2) + (r & 3) * 256 + kk * 16 : st + kk * BM * 16 + r * 16);
+  };
+  auto b_at = [&](const char *st, int kk, int j) -> const double2 * {
+    return (const double2 *)(TB ? st + S::kStageA + kk * BN * 16 + bcolumn(j) * 16
+                                : st + S::kStageA + direct_b_base(bq[j]) + bcol[j] * BK * 16 + kk * 16);
+  };
 
   double sh[TM][TN], sl[TM][TN];
 #pragma unroll
@@ -445,13 +499,13 @@ __device__ __forceinline__ void grouped_tile_direct(const double2 *__restrict__ 
     double ah[TM], al[TM], bh[TN], bl[TN];
 #pragma unroll
     for (int i = 0; i < TM; ++i) {
-      const double2 v = *(const double2 *)(st + kk * BM * 16 + (tx + 16 * i) * 16);
+      const double2 v = *a_at(st, kk, i);
       ah[i] = v.x;
       al[i] = v.y;
     }
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
-      const double2 v = *(const double2 *)(st + S::kStageA + direct_b_base(bq[j]) + bcol[j] * BK * 16 + kk * 16);
+      const double2 v = *b_at(st, kk, j);
       bh[j] = v.x;
       bl[j] = v.y;
     }
@@ -507,15 +561,24 @@ __device__ __forceinline__ void grouped_tile_direct(const double2 *__restrict__ 
 
 #pragma unroll
   for (int j = 0; j < TN; ++j) {
-    const int64_t gj = n0 + 4 * bq[j] + bcol[j];
+    const int64_t gj = n0 + bcolumn(j);
     if (gj >= n) continue;
 #pragma unroll
     for (int i = 0; i < TM; ++i) {
-      const int64_t gi = m0 + tx + 16 * i;
+      const int64_t gi = m0 + arow(i);
       if (gi >= m) continue;
       double h = sh[i][j], l = sl[i][j];
       renorm(h, l);
-      C[gi + gj * m] = make_double2(h, l);
+      if (AB) {
+        ddv o = dd_mul(alpha, mk(h, l));
+        if (beta.hi != 0.0 || beta.lo != 0.0) {
+          const double2 c0 = C[gi + gj * ldc];
+          o = dd_add(o, dd_mul(beta, mk(c0.x, c0.y)));
+        }
+        h = o.hi;
+        l = o.lo;
+      }
+      C[gi + gj * ldc] = make_double2(h, l);
     }
   }
 }
