@@ -87,9 +87,9 @@ __device__ __forceinline__ void gemm_tile(const GemmD &g, int tr, int tc, char *
   static_assert(mfhip_tile::kDirectBK == kBK, "K block");
   const int64_t m0 = (int64_t)tr * (16 * TM), n0 = (int64_t)tc * (16 * TN);
   if (g.ri == 8)
-    mfhip_tile::grouped_tile_direct<TM, TN, 8>(g.A, g.B, g.C, g.m, g.k, g.n, g.ri, m0, n0, lds);
+    mfhip_tile::grouped_tile_direct<TM, TN, 8>(g.A, g.m, g.B, g.k, g.C, g.m, g.m, g.k, g.n, g.ri, m0, n0, lds);
   else
-    mfhip_tile::grouped_tile_direct<TM, TN, 0>(g.A, g.B, g.C, g.m, g.k, g.n, g.ri, m0, n0, lds);
+    mfhip_tile::grouped_tile_direct<TM, TN, 0>(g.A, g.m, g.B, g.k, g.C, g.m, g.m, g.k, g.n, g.ri, m0, n0, lds);
 }
 
 // Block tree over 256 normalised pairs, fixed shape (the same as reduce.hip).
@@ -258,13 +258,22 @@ void launch_kind(const Plan *p, hipStream_t s) {
   if (p->groups[q] <= 0) return;
   constexpr int lds = mfhip_tile::DirectShape<TM, TN>::kLds;
   static_assert(lds * WPE <= 160 * 1024, "the kind's workgroups per CU fit in LDS");
-  if (lds > 65536) {  // above the default dynamic LDS limit: raised once per kind
-    static const bool raised = hipFuncSetAttribute((const void *)k_grouped_gemm<TM, TN, WPE>,
-                                                   hipFuncAttributeMaxDynamicSharedMemorySize, lds) == hipSuccess;
-    (void)raised;
-  }
   hipLaunchKernelGGL((k_grouped_gemm<TM, TN, WPE>), dim3((unsigned)p->groups[q]), dim3(kThreads), lds, s,
                      p->items + p->first[q], p->offs + p->ofirst[q], p->gemms);
+}
+
+// Dynamic LDS above the 64 KiB default, allowed once per plan on its device
+// (at creation: nothing but launches happens while a run is captured).
+template <int TM, int TN, int WPE>
+bool allow_kind_lds() {
+  constexpr int lds = mfhip_tile::DirectShape<TM, TN>::kLds;
+  return lds <= 65536 || hipFuncSetAttribute((const void *)k_grouped_gemm<TM, TN, WPE>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, lds) == hipSuccess;
+}
+bool allow_lds() {
+  return allow_kind_lds<2, 2, 4>() && allow_kind_lds<4, 4, 2>() && allow_kind_lds<8, 1, 2>() &&
+         allow_kind_lds<1, 8, 2>() && allow_kind_lds<4, 2, 3>() && allow_kind_lds<2, 4, 3>() &&
+         allow_kind_lds<1, 1, 5>() && allow_kind_lds<3, 3, 3>();
 }
 
 hipError_t run_plan(Plan *p, hipStream_t s) {
@@ -377,6 +386,7 @@ void *mfhip_grouped_create(const mfhip_gemm_problem *gemms, int ngemm, const int
   bool ok = hipGetDevice(&p->device) == hipSuccess;
   int cus = 0;
   ok = ok && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device) == hipSuccess;
+  ok = ok && allow_lds();
   p->cus = cus > 0 ? cus : 1;
   // Deal each kind's items (largest first) to the workgroup with the least
   // estimated cost so far; ties go to the lower index (the plan is

@@ -341,12 +341,10 @@ hipError_t launch_gemm(const double2 *A, const double2 *B, double2 *C, int64_t m
                                                           : mfhip_tile::DirectShape<TM, TN>::kLds;
   static_assert(dyn * WPE <= 160 * 1024, "the workgroups per CU fit in LDS");
   if (direct && dyn > 65536) {
-    // above the 64 KiB default: allowed once per process and kernel
-    static const bool ok8 = hipFuncSetAttribute((const void *)k_gemm_plain<TM, TN, BK, WPE, 8>,
-                                                hipFuncAttributeMaxDynamicSharedMemorySize, dyn) == hipSuccess;
-    static const bool ok0 = hipFuncSetAttribute((const void *)k_gemm_plain<TM, TN, BK, WPE, 0>,
-                                                hipFuncAttributeMaxDynamicSharedMemorySize, dyn) == hipSuccess;
-    if (!ok8 || !ok0) return hipErrorInvalidConfiguration;
+    // above the 64 KiB default: allowed on every call (any device current)
+    const void *kern = r8 ? (const void *)k_gemm_plain<TM, TN, BK, WPE, 8> : (const void *)k_gemm_plain<TM, TN, BK, WPE, 0>;
+    const hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+    if (e != hipSuccess) return e;
   }
   if (r8)
     hipLaunchKernelGGL((k_gemm_plain<TM, TN, BK, WPE, 8>), dim3((unsigned)(tm * tn)), dim3(kThreads), dyn,
@@ -416,12 +414,11 @@ hipError_t launch_blas(const double2 *A, int64_t lda, const double2 *B, int64_t 
   if (tm * tn > 0x7fffffff) return hipErrorInvalidValue;
   constexpr int dyn = BK == mfhip_tile::kDirectBK ? mfhip_tile::DirectShape<TM, TN, TA, TB>::kLds : 0;
   static_assert(dyn * WPE <= 160 * 1024, "the workgroups per CU fit in LDS");
-  if (dyn > 65536) {  // above the 64 KiB default: raised once per process and kernel
-    static const bool ok8 = hipFuncSetAttribute((const void *)k_gemm_blas<TM, TN, BK, WPE, 8, TA, TB>,
-                                                hipFuncAttributeMaxDynamicSharedMemorySize, dyn) == hipSuccess;
-    static const bool ok0 = hipFuncSetAttribute((const void *)k_gemm_blas<TM, TN, BK, WPE, 0, TA, TB>,
-                                                hipFuncAttributeMaxDynamicSharedMemorySize, dyn) == hipSuccess;
-    if (!ok8 || !ok0) return hipErrorInvalidConfiguration;
+  if (dyn > 65536) {  // above the 64 KiB default: allowed on every call (any device current)
+    const void *kern = (ri == 8 && k > 8) ? (const void *)k_gemm_blas<TM, TN, BK, WPE, 8, TA, TB>
+                                          : (const void *)k_gemm_blas<TM, TN, BK, WPE, 0, TA, TB>;
+    const hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+    if (e != hipSuccess) return e;
   }
   if (ri == 8 && k > 8)
     hipLaunchKernelGGL((k_gemm_blas<TM, TN, BK, WPE, 8, TA, TB>), dim3((unsigned)(tm * tn)), dim3(kThreads), dyn,
@@ -489,12 +486,10 @@ hipError_t gemm_cplx(const double4 *A, const double4 *B, double4 *C, int64_t m, 
   if (k <= 0) return zero_doubles((double *)C, 4 * m * n, s);
   const int64_t tm = (m + 31) / 32, tn = (n + 31) / 32;
   if (tm * tn > 0x7fffffff) return hipErrorInvalidValue;
-  // above the 64 KiB default: raised once per process and kernel
-  static const bool ok8 = hipFuncSetAttribute((const void *)k_gemm_cplx<8>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                              kCplxLds) == hipSuccess;
-  static const bool ok0 = hipFuncSetAttribute((const void *)k_gemm_cplx<0>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                              kCplxLds) == hipSuccess;
-  if (!ok8 || !ok0) return hipErrorInvalidConfiguration;
+  // above the 64 KiB default: allowed on every call (any device current)
+  const hipError_t e = hipFuncSetAttribute((ri == 8 && k > 8) ? (const void *)k_gemm_cplx<8> : (const void *)k_gemm_cplx<0>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, kCplxLds);
+  if (e != hipSuccess) return e;
   if (ri == 8 && k > 8)
     hipLaunchKernelGGL(k_gemm_cplx<8>, dim3((unsigned)(tm * tn)), dim3(kThreads), kCplxLds, s, A, B, C, m, k, n, ri,
                        (int)tm);
