@@ -489,8 +489,12 @@ MULTIFLOATS_API int mfhip_ozaki_products(int64_t m, int64_t k, int64_t n, int sl
  * the same buffers; run_graph captures the launch in a HIP graph on first use
  * and replays it afterwards. tile_variant[i] picks the GEMM tile for problem
  * i: 0 = 32x32, 1 = 64x64, 2 = 128x16, 3 = 16x128, 4 = 64x32, 5 = 32x64,
- * -1 = built-in rule: least padded work weighted by a per-shape cost (NULL:
- * rule for all). */
+ * 6 = 16x16, 7 = 48x48, -1 = built-in rule: least padded work weighted by a
+ * per-shape cost (NULL: rule for all). The problems must be independent: no
+ * output may overlap another problem's input or output (the GEMM tile kinds
+ * run in an order of the plan's choosing, and the dots of a plan with GEMMs
+ * run on a second stream at the same time as the GEMMs). Runs of one plan
+ * must not overlap one another. */
 typedef struct {
   const float64x2 *a;
   const float64x2 *b;

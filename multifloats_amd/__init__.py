@@ -705,7 +705,10 @@ class GroupedPlan:
     """Many DD GEMMs and dot products in one persistent grouped launch
     (mfhip_grouped_*). gemms: list of (a, b, c, m, k, n, ri) with device
     addresses; dots: list of (a, b, out, n, ri); variants: per-GEMM tile
-    variant (0 = 32x32, 1 = 64x64, -1 = built-in rule)."""
+    variant (0 = 32x32, 1 = 64x64, 2 = 128x16, 3 = 16x128, 4 = 64x32,
+    5 = 32x64, 6 = 16x16, 7 = 48x48, -1 = built-in rule). The problems must
+    be independent: no output may overlap another problem's input or output
+    (the dots of a plan with GEMMs run at the same time as the GEMMs)."""
 
     def __init__(self, gemms, dots, variants=None):
         L = lib()

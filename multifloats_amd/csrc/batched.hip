@@ -111,17 +111,19 @@ __device__ ddv tree256(ddv v, double *th, double *tl) {
   return mk(th[0], tl[0]);
 }
 
+template <int KE>
 __device__ void dot_chunk(const DotD &d, int c, double2 *partials, int *dot_done, double *lds) {
   const int tid = threadIdx.x;
   const int64_t p0 = (int64_t)c * kDotChunk;
   const int64_t p1 = p0 + kDotChunk < d.n ? p0 + kDotChunk : d.n;
   double sh = 0.0, sl = 0.0;
   int64_t until = d.ri;
-  // Lane tid takes elements p0 + tid + 256 j in order. They are loaded eight
-  // at a time with the next eight already in flight (nontemporal 16-byte
-  // loads, two register buffers), so the latency-bound accumulation chain
-  // does not starve the memory system.
-  constexpr int kE = 8;
+  // Lane tid takes elements p0 + tid + 256 j in order. They are loaded KE
+  // at a time with the next KE already in flight (nontemporal 16-byte loads,
+  // two register buffers), so the latency-bound accumulation chain does not
+  // starve the memory system. KE changes the load batching, not the order of
+  // the chain: the result's bits do not depend on it.
+  constexpr int kE = KE;
   constexpr int64_t kRound = (int64_t)kThreads * kE;
   v2d xa[kE], xb[kE], ya[kE], yb[kE];  // two register buffers, indexed statically
   auto fetch = [&](v2d (&va)[kE], v2d (&vb)[kE], int64_t i0) {
@@ -205,14 +207,21 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WPE)))
   }
 }
 
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_grouped_dot(
+// The dots. Alone (a plan without GEMMs): eight elements per register buffer,
+// two waves per SIMD. Beside the GEMM kinds (the
+// dots' launch on a second stream, running while the GEMM kinds run): two
+// elements per buffer and at most 64 VGPRs (the eight-per-SIMD budget), so
+// that two dot workgroups per CU fit in the VGPRs a GEMM kind leaves free
+// (64 x 32 tiles: three workgroups of 128 VGPRs per lane, 128 left of 512).
+template <int KE, int WPE>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WPE))) void k_grouped_dot(
     const Item *__restrict__ items, const int *__restrict__ off, const DotD *__restrict__ dots,
     double2 *__restrict__ partials, int *__restrict__ dot_done) {
   __shared__ __attribute__((aligned(16))) double lds[2 * kThreads];
   const int end = off[blockIdx.x + 1];
   for (int it = off[blockIdx.x]; it < end; ++it) {
     const Item w = items[it];
-    dot_chunk(dots[w.prob], w.a, partials, dot_done, lds);
+    dot_chunk<KE>(dots[w.prob], w.a, partials, dot_done, lds);
   }
 }
 
@@ -243,10 +252,14 @@ struct Plan {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  // dots beside the GEMM kinds: their stream and the fork / join events
+  hipStream_t dstream = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  int order[ITEM_DOT] = {};  // GEMM kinds in launch order: largest estimated cost first
 };
 
 // Workgroups per CU of each kind's launch (its waves per SIMD: 4-wave
-// workgroups), in kind order; the dots run two.
+// workgroups), in kind order; the dots run two, alone or beside a GEMM kind.
 constexpr int kKindWPE[ITEM_DOT + 1] = {4, 2, 2, 2, 3, 3, 5, 3, 2};
 
 // One persistent launch per kind with items: grid = the kind's resident
@@ -276,21 +289,58 @@ bool allow_lds() {
          allow_kind_lds<1, 1, 5>() && allow_kind_lds<3, 3, 3>();
 }
 
+void launch_gemm_kind(const Plan *p, int q, hipStream_t s) {
+  switch (q) {
+    case 0: launch_kind<2, 2, 4, 0>(p, s); break;
+    case 1: launch_kind<4, 4, 2, 1>(p, s); break;
+    case 2: launch_kind<8, 1, 2, 2>(p, s); break;
+    case 3: launch_kind<1, 8, 2, 3>(p, s); break;
+    case 4: launch_kind<4, 2, 3, 4>(p, s); break;
+    case 5: launch_kind<2, 4, 3, 5>(p, s); break;
+    case 6: launch_kind<1, 1, 5, 6>(p, s); break;
+    default: launch_kind<3, 3, 3, 7>(p, s); break;
+  }
+}
+
+void launch_dots(const Plan *p, bool beside, hipStream_t s) {
+  const int q = ITEM_DOT;
+  if (beside)
+    hipLaunchKernelGGL((k_grouped_dot<2, 8>), dim3((unsigned)p->groups[q]), dim3(kThreads), 0, s, p->items + p->first[q],
+                       p->offs + p->ofirst[q], p->dots, p->partials, p->counters);
+  else
+    hipLaunchKernelGGL((k_grouped_dot<8, 2>), dim3((unsigned)p->groups[q]), dim3(kThreads), 0, s, p->items + p->first[q],
+                       p->offs + p->ofirst[q], p->dots, p->partials, p->counters);
+}
+
+// The counters are cleared, then the GEMM kinds run one after another on s,
+// largest estimated cost first. With GEMMs in the plan, the dots run on the
+// plan's second stream from the start of the first GEMM kind (forked after the
+// clear, joined back into s at the end): the dots are HBM-bound, the GEMM
+// kinds VALU-bound, and the dot workgroups fit in the registers and LDS the
+// first kind leaves free. Without GEMMs the dots run alone on s.
 hipError_t run_plan(Plan *p, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_ints, dim3((unsigned)((p->ncounters + 255) / 256)), dim3(256), 0, s,
                      p->counters, p->ncounters);
-  launch_kind<2, 2, 4, 0>(p, s);
-  launch_kind<4, 4, 2, 1>(p, s);
-  launch_kind<8, 1, 2, 2>(p, s);
-  launch_kind<1, 8, 2, 3>(p, s);
-  launch_kind<4, 2, 3, 4>(p, s);
-  launch_kind<2, 4, 3, 5>(p, s);
-  launch_kind<1, 1, 5, 6>(p, s);
-  launch_kind<3, 3, 3, 7>(p, s);
-  if (p->groups[ITEM_DOT] > 0)
-    hipLaunchKernelGGL(k_grouped_dot, dim3((unsigned)p->groups[ITEM_DOT]), dim3(kThreads), 0, s,
-                       p->items + p->first[ITEM_DOT], p->offs + p->ofirst[ITEM_DOT], p->dots, p->partials,
-                       p->counters);
+  const bool dots = p->groups[ITEM_DOT] > 0, beside = dots && p->dstream;
+  hipError_t e = hipSuccess;
+  if (beside) {
+    e = hipEventRecord(p->ev_fork, s);
+    if (e == hipSuccess) e = hipStreamWaitEvent(p->dstream, p->ev_fork, 0);
+    if (e != hipSuccess) return e;
+  }
+  for (int i = 0; i < ITEM_DOT; ++i) {
+    launch_gemm_kind(p, p->order[i], s);
+    if (i == 0 && beside) {
+      launch_dots(p, true, p->dstream);
+      e = hipEventRecord(p->ev_join, p->dstream);
+      if (e != hipSuccess) return e;
+    }
+  }
+  if (beside)
+    e = hipStreamWaitEvent(s, p->ev_join, 0);
+  else if (dots)
+    launch_dots(p, false, s);
+  if (e != hipSuccess) return e;
   return hipGetLastError();
 }
 
@@ -301,6 +351,9 @@ void free_plan(Plan *p) {
   if (p->ev_in) (void)hipEventDestroy(p->ev_in);
   if (p->ev_out) (void)hipEventDestroy(p->ev_out);
   if (p->gstream) (void)hipStreamDestroy(p->gstream);
+  if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+  if (p->ev_join) (void)hipEventDestroy(p->ev_join);
+  if (p->dstream) (void)hipStreamDestroy(p->dstream);
   (void)hipFree(p->items);
   (void)hipFree(p->offs);
   (void)hipFree(p->gemms);
@@ -394,8 +447,10 @@ void *mfhip_grouped_create(const mfhip_gemm_problem *gemms, int ngemm, const int
   // kind's slice, in the order dealt.
   std::vector<Item> dealt(sorted.size());
   std::vector<int> offs;
+  double kcost[ITEM_DOT + 1] = {};
   for (int q = 0; q <= ITEM_DOT; ++q) {
     const int b = p->first[q], n = p->first[q + 1] - b;
+    for (int i = 0; i < n; ++i) kcost[q] += scost[(size_t)(b + i)];
     const int resident = p->cus * kKindWPE[q];
     const int G = n < resident ? n : resident;
     p->groups[q] = G;
@@ -417,6 +472,13 @@ void *mfhip_grouped_create(const mfhip_gemm_problem *gemms, int ngemm, const int
       for (int i : lists[(size_t)g]) dealt[(size_t)pos++] = sorted[(size_t)i];
     }
     offs.push_back(pos - b);
+  }
+  for (int q = 0; q < ITEM_DOT; ++q) p->order[q] = q;
+  std::stable_sort(p->order, p->order + ITEM_DOT, [&](int x, int y) { return kcost[x] > kcost[y]; });
+  if (p->groups[ITEM_DOT] > 0 && p->first[ITEM_DOT] > 0) {  // dots and GEMMs: the dots run beside
+    ok = ok && hipStreamCreateWithFlags(&p->dstream, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming) == hipSuccess;
   }
   p->nitems = (int)sorted.size();
   p->ndot = ndot;

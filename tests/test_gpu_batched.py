@@ -88,3 +88,33 @@ def test_grouped_graph_replays_match_eager(gpu):
         assert torch.equal(gbuf, eager_g), rep
         nt.assert_bits(dout.cpu().numpy(), eager_d.cpu().numpy(), "dots replay %d" % rep)
     plan.close()
+
+
+def test_grouped_dots_beside_and_alone_agree(gpu):
+    """A plan with GEMMs runs its dots on a second stream beside the GEMM
+    kinds (two elements per register buffer); a plan of dots only runs them
+    alone on the caller's stream (eight per buffer). Each lane's chain is the
+    same, so the bits are equal; a plan of the GEMMs only writes the mixed
+    plan's GEMM bits."""
+    import torch
+    rng = np.random.default_rng(11)
+    sizes, dlen, gbuf, dbuf, dout, gemms, dots, offs, doff = build(torch, rng, 150, 150, long_dot=2_100_000)
+    s = torch.cuda.current_stream().cuda_stream
+    mixed = mf.GroupedPlan(gemms, dots)
+    mixed.run(s)
+    torch.cuda.synchronize()
+    g_mixed, d_mixed = gbuf.clone(), dout.clone()
+    dots_only = mf.GroupedPlan([], dots)
+    dout.fill_(float("nan"))
+    dots_only.run(s)
+    torch.cuda.synchronize()
+    nt.assert_bits(dout.cpu().numpy(), d_mixed.cpu().numpy(), "dots alone vs beside")
+    gemms_only = mf.GroupedPlan(gemms, [])
+    for (m, k, n), o in zip(sizes, offs):
+        c0 = 2 * (o + int(m) * int(k) + int(k) * int(n))
+        gbuf[c0:c0 + 2 * int(m) * int(n)].fill_(float("nan"))
+    gemms_only.run_graph(s)
+    torch.cuda.synchronize()
+    assert torch.equal(gbuf, g_mixed)
+    for p in (mixed, dots_only, gemms_only):
+        p.close()
