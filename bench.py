@@ -346,8 +346,9 @@ def bench_c0(args, torch, mf):
     out = {"workload": "float64x2 GEMM m=n=k=256, config-0 entries (hi ~ U(-1,1)), renorm 8",
            "value": round(2.0 * m * n * k / (ms * 1e-3) / 1e9, 2), "unit": "DD-GFLOP/s", "ms": round(ms, 4),
            "steps": reps, "roofline": gemm_roofline_plain(m, n, k, ms)}
-    out["roofline"]["note"] += ("; at 256^3: 16x16 tiles, one chain per lane, 256 workgroups (one per CU): "
-                                "each lane's 256-step MAC chain is latency-bound, not issue-bound")
+    out["roofline"]["note"] += ("; at 256^3: 16x16 tiles, one chain per lane (65,536 chains: one wave per SIMD), "
+                                "256 workgroups; each lane's 256-step MAC chain is software-pipelined three steps "
+                                "deep, and a lone wave per SIMD still waits on its operations' latency")
     if not args.no_cpu:
         dt, Cc, kind = cpu_matmul(A, B, n, 8, reps=3)
         out["cpu_baseline"] = dict(value=round(2.0 * m * n * k / dt / 1e9, 4), unit="DD-GFLOP/s", cores=1, kind=kind,

@@ -392,14 +392,15 @@ hipError_t gemm_plain(const double2 *A, const double2 *B, double2 *C, int64_t m,
     case 5: return launch_gemm<8, 4, 16, 1>(A, B, C, m, k, n, ri, s);
     case 6: return launch_gemm<2, 4, 16, 3>(A, B, C, m, k, n, ri, s);
     default: {
-      // 16 x 16 tiles: with two or more workgroups per CU the direct copies
-      // (16-k blocks) hide their latency behind the other workgroup; with
-      // one, register staging with K blocks of 32 exposes fewer loads
-      // (256^3: 22.5-23 us against 32-33 us direct; 384^3, 512^3: direct
-      // 6-8 % faster). Same chains, same bits.
+      // 16 x 16 tiles: with four or more workgroups per CU the direct copies
+      // (16-k blocks) hide their latency behind the other workgroups; with
+      // fewer, register staging with K blocks of 64 and the hand-pipelined
+      // chain (one chain per lane) is faster (256^3: 17.0-17.4 us against
+      // 32-33 us direct; 384^3: 55.0 against 57.1 us; 512^3: direct 2 %
+      // faster). Same chains, same bits.
       const int64_t t16 = ((m + 15) / 16) * ((n + 15) / 16);
-      if (t16 >= 512) return launch_gemm<1, 1, 16, 4>(A, B, C, m, k, n, ri, s);
-      return launch_gemm<1, 1, 32, 4>(A, B, C, m, k, n, ri, s);
+      if (t16 >= 1024) return launch_gemm<1, 1, 16, 4>(A, B, C, m, k, n, ri, s);
+      return launch_gemm<1, 1, 64, 4>(A, B, C, m, k, n, ri, s);
     }
   }
 }

@@ -38,6 +38,90 @@ constexpr int kThreads = 256;
 // alpha*T + beta*C (DD operators; C is not read when beta == 0).
 // Threads walk the contiguous dimension of each operand, so every layout
 // loads coalesced.
+// One K block of a single reference-order chain per lane (1 x 1 outputs per
+// lane, renormalisation every 8 MACs at fixed points), software-pipelined by
+// hand. One such chain per lane is all the parallelism a small product has
+// (m n chains, one wave per SIMD), and a MAC's 13 FP64 operations are 8
+// deep, so the compiler's order (each MAC's operations one after another)
+// leaves most operations waiting on the one just before. Here the 13
+// operations of step j are
+//   1 p = ah bh, 2 e = fma(ah, bh, -p), 3 q = al bh, 4 cr = fma(ah, bl, q),
+//   5 t = s + p, 6 bp = t - s, 7 ap = t - bp, 8 ae = s - ap, 9 be = p - bp,
+//   10 x1 = e + cr, 11 x2 = ae + be, 12 x3 = x1 + x2, 13 sl = sl + x3
+// (s = the running hi part before the step, t after it), and each issue slot
+// of the steady state takes the tail (10-13) of step j, the middle (5-9) of
+// step j + 1 and the head (1-4) of step j + 2, in the order
+//   5(j+1) 10(j) 6(j+1) 1(j+2) 11(j) 7(j+1) 3(j+2) 12(j) 9(j+1) 8(j+1) 13(j) 2(j+2) 4(j+2)
+// so that every operand was produced at least two operations earlier. A group
+// of 8 steps starts with the head of its first two steps and the middle of
+// the first, and ends with the tail of its last step and the TwoSum
+// renormalisation (skipped after the chain's final MAC, as in gemm_tile).
+// Every operation, its operands and each chain's order are those of mac() and
+// renorm(): the bits are the same. Scheduling barriers pin the order; the LDS
+// reads of step j + 4 are issued in slot j, and the first four steps of the
+// block are read at its start.
+//   Ah, Al: this lane's A entries of the block's first k (hi, lo), step sa
+//   apart; Bh, Bl likewise with step sb.
+template <int BK>
+__device__ __forceinline__ void chain_block8(const double *Ah, const double *Al, const double *Bh, const double *Bl,
+                                             int sa, int sb, double &sh, double &sl, bool renorm_end) {
+  static_assert(BK % 8 == 0, "groups of 8 steps");
+  constexpr int W = 4;  // steps in flight: ring slots
+  constexpr int D = 4;  // LDS read distance in steps
+  double oh[2 * D], ol[2 * D], ph[2 * D], pl[2 * D];
+  double p[W], e[W], q[W], cr[W], t[W], s0[W], bp[W], ap[W], ae[W], be[W], x1[W], x2[W];
+  auto pin = [] { __builtin_amdgcn_sched_barrier(0); };
+  auto rd = [&](int j) {
+    if (j >= BK) return;
+    const int r = j % (2 * D);
+    oh[r] = Ah[j * sa];
+    ol[r] = Al[j * sa];
+    ph[r] = Bh[j * sb];
+    pl[r] = Bl[j * sb];
+  };
+  auto op1 = [&](int j) { p[j % W] = oh[j % (2 * D)] * ph[j % (2 * D)]; pin(); };
+  auto op2 = [&](int j) { e[j % W] = __builtin_fma(oh[j % (2 * D)], ph[j % (2 * D)], -p[j % W]); pin(); };
+  auto op3 = [&](int j) { q[j % W] = ol[j % (2 * D)] * ph[j % (2 * D)]; pin(); };
+  auto op4 = [&](int j) { cr[j % W] = __builtin_fma(oh[j % (2 * D)], pl[j % (2 * D)], q[j % W]); pin(); };
+  auto op5 = [&](int j) { s0[j % W] = sh; t[j % W] = sh + p[j % W]; sh = t[j % W]; pin(); };
+  auto op6 = [&](int j) { bp[j % W] = t[j % W] - s0[j % W]; pin(); };
+  auto op7 = [&](int j) { ap[j % W] = t[j % W] - bp[j % W]; pin(); };
+  auto op8 = [&](int j) { ae[j % W] = s0[j % W] - ap[j % W]; pin(); };
+  auto op9 = [&](int j) { be[j % W] = p[j % W] - bp[j % W]; pin(); };
+  auto op10 = [&](int j) { x1[j % W] = e[j % W] + cr[j % W]; pin(); };
+  auto op11 = [&](int j) { x2[j % W] = ae[j % W] + be[j % W]; pin(); };
+  auto op13 = [&](int j) {  // ops 12 and 13
+    const double x3 = x1[j % W] + x2[j % W];
+    pin();
+    sl = sl + x3;
+    pin();
+  };
+#pragma unroll
+  for (int j = 0; j < D; ++j) rd(j);
+#pragma unroll
+  for (int g = 0; g < BK; g += 8) {
+    op1(g); op3(g); op1(g + 1); op3(g + 1); op2(g); op4(g);
+    op5(g); op2(g + 1); op4(g + 1); op6(g); op7(g); op9(g); op8(g);
+#pragma unroll
+    for (int j = g; j < g + 7; ++j) {
+      rd(j + D);
+      const bool head = j + 2 < g + 8;
+      op5(j + 1); op10(j); op6(j + 1);
+      if (head) op1(j + 2);
+      op11(j); op7(j + 1);
+      if (head) op3(j + 2);
+      op13(j); op9(j + 1); op8(j + 1);
+      if (head) { op2(j + 2); op4(j + 2); }
+    }
+    rd(g + 7 + D);
+    op10(g + 7); op11(g + 7); op13(g + 7);
+    if (g + 8 < BK || renorm_end) {
+      renorm(sh, sl);
+      pin();
+    }
+  }
+}
+
 template <int TM, int TN, int BK, int R, bool TA = false, bool TB = false, bool AB = false>
 __device__ __forceinline__ void gemm_tile(const double2 *__restrict__ A, int64_t lda,
                                           const double2 *__restrict__ B, int64_t ldb,
@@ -143,14 +227,23 @@ __device__ __forceinline__ void gemm_tile(const double2 *__restrict__ A, int64_t
     __syncthreads();
     if (k0 + BK < k) load(k0 + BK);
     const int64_t kmax = (k - k0) < BK ? (k - k0) : BK;
-    if (R > 0 && rint == R && k0 + BK < k) {
-      // static renormalisation points; `until` stays in phase since R | BK
+    if (R > 0 && rint == R && kmax == BK) {
+      // static renormalisation points; `until` stays in phase since R | BK.
+      // A full last block (k0 + BK == k) skips the point after its final
+      // MAC, as the counted path does (done == k there). One chain per lane
+      // runs the hand-pipelined block above; with more chains per lane one
+      // group of R steps already holds enough independent work.
       constexpr int RR = R > 0 ? R : 1;
+      const bool last = k0 + BK >= k;
+      if constexpr (TM == 1 && TN == 1 && R == 8) {
+        chain_block8<BK>(&As_h[0][tx], &As_l[0][tx], &Bs_h[0][ty], &Bs_l[0][ty], BM, BN, sh[0][0], sl[0][0], !last);
+      } else {
 #pragma unroll 1
-      for (int c = 0; c < BK; c += RR) {
+        for (int c = 0; c < BK; c += RR) {
 #pragma unroll
-        for (int kk = 0; kk < RR; ++kk) macs(c + kk);
-        renorm_all();
+          for (int kk = 0; kk < RR; ++kk) macs(c + kk);
+          if (!last || c + RR < BK) renorm_all();
+        }
       }
       done += BK;
     } else if (kmax == BK) {
