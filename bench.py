@@ -58,8 +58,8 @@ INT8_PEAK_TOPS = 5000.0      # MI355X dense INT8 MFMA
 # x 1.0 (1,048,576 KiB). From profiles/hot_kernels_profile_workload_pmc.md
 # (bench.py --workload profile). Keyed by (kernel, size, slices).
 PMC_TRAFFIC = {
-    ("k_oz2_gemm", 4096, 14): (2.0 * 1606738 + 568463) * 1024,
-    ("k_gemm_plain", 4096, 0): (2.0 * 3142431 + 262144) * 1024,
+    ("k_oz2_gemm", 4096, 14): (2.0 * 1539677 + 562024) * 1024,
+    ("k_gemm_plain", 4096, 0): (2.0 * 3142425 + 262144) * 1024,
 }
 # the C1 element kernels and reduction passes at N = 2^26, same runs
 C1_TRAFFIC = {
