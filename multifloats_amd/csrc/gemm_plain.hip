@@ -437,10 +437,13 @@ hipError_t blas_by_size(const double2 *A, int64_t lda, const double2 *B, int64_t
     case 2: return launch_blas<4, 2, 3, TA, TB>(A, lda, B, ldb, C, ldc, m, k, n, ri, alpha, beta, s);
     case 0: return launch_blas<2, 2, 4, TA, TB>(A, lda, B, ldb, C, ldc, m, k, n, ri, alpha, beta, s);
     default:
-      // 16 x 16 tiles below 512 of them: register staging with 32-k blocks
-      // (one workgroup per CU hides the direct copies' latency poorly)
+      // 16 x 16 tiles below 512 of them: register staging with 64-k blocks
+      // and the hand-pipelined chain (one workgroup per CU hides the direct
+      // copies' latency poorly; 256^3: 13.7-19.8 us per transpose pair
+      // against 14.8-20.9 with 32-k blocks; at 384^3 the direct copies stay
+      // ahead for op(A) = A^T, op(B) = B)
       if (((m + 15) / 16) * ((n + 15) / 16) < 512)
-        return launch_blas<1, 1, 4, TA, TB, 32>(A, lda, B, ldb, C, ldc, m, k, n, ri, alpha, beta, s);
+        return launch_blas<1, 1, 4, TA, TB, 64>(A, lda, B, ldb, C, ldc, m, k, n, ri, alpha, beta, s);
       return launch_blas<1, 1, 4, TA, TB>(A, lda, B, ldb, C, ldc, m, k, n, ri, alpha, beta, s);
   }
 }
